@@ -1,0 +1,851 @@
+/* mpf oracle: CPU restatement of the reference photon-propagation path.
+ *
+ * TEST INFRASTRUCTURE ONLY: only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py load this, as the checker or as the timed CPU
+ * baseline.  The product library never links or calls it.
+ *
+ * Compiled with -ffp-contract=off so that every floating-point expression is
+ * evaluated operation by operation, in the order the reference writes it.
+ *
+ * Pinning (what each function is checked against, tests/test_oracle.py):
+ *   key_compute / cipher / level / centre / parent / containing
+ *       - the reference's own index class compiled in oracle/_ref
+ *         (bit-exact on random and edge points, incl. out-of-box points);
+ *       - known-answer keys printed by the reference headers
+ *         (cipher(0,0,0) = 0xf888..., compute(9,(0.3,-1.1,1.7)) = 0xdcebec9be0...);
+ *       - closed forms: centre = -2 + 4(2k+1)/2^(l+1), extent 4/2^l,
+ *         ancestor == key prefix, key order == depth-first pre-order.
+ *   locate / ngp / cic / tsc (cached and uncached)
+ *       - oracle/_ref on AMR trees with holes and level jumps (bit-exact);
+ *       - golden fixtures tests/golden/*.npz written from oracle/_ref;
+ *       - closed forms: a field linear in x is reproduced exactly by CIC and
+ *         TSC away from level jumps; weights sum to one.
+ *   integrate / trajectory (all stop criteria)
+ *       - oracle/_ref and golden fixtures (bit-exact per-ray records);
+ *       - invariants: phi == 0 gives straight rays and z = a0/a - 1;
+ *         null condition error() ~ 0.
+ *   jacobian_infinitesimal
+ *       - oracle/_ref and golden fixtures (bit-exact: same libm);
+ *       - closed form: phi linear in x gives A = identity.
+ *   interpolate
+ *       - oracle/_ref (bit-exact), linear tables reproduce exactly.
+ */
+#define _GNU_SOURCE
+#include "mpf_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef unsigned __int128 u128;
+
+#define EXTENT 4.0
+/* marker bit of every nibble: the index holds 4 bits per level from the MSB,
+ * nibble = 8 | z<<2 | y<<1 | x  (simplehyperoctreeindex.h:1167-1189) */
+static const u128 MARK = (((u128)0x8888888888888888ULL) << 64) | (u128)0x8888888888888888ULL;
+/* invalid index = block(0, Bits-1) (simplehyperoctreeindex.h:1358-1361) */
+static const u128 INVALID = (((u128)0x7fffffffffffffffULL) << 64) | (u128)0xffffffffffffffffULL;
+
+static const double C_LIGHT = 299792458.0;          /* constants.h:441-444 */
+static const double C2 = 299792458.0 * 299792458.0; /* constants.h:481-484 */
+
+typedef struct {
+    double v[7]; /* rho, phi, dphidx, dphidy, dphidz, a, dphidt (gravity.h:45-87 tuple order) */
+} fdata;
+enum { F_RHO, F_PHI, F_DX, F_DY, F_DZ, F_A, F_DT };
+
+struct mpfo_octree {
+    uint64_t n;
+    u128* key;
+    float* f32;  /* n x 7 */
+    double* f64; /* n x 7, optional */
+};
+
+/* ------------------------------------------------------------------ keys */
+
+/* Per-axis integer coordinate at level l: Type((1/E)*2^l*(x - (-E/2))) then
+ * glue keeps the low l bits (simplehyperoctreeindex.h:1167-1170).  The
+ * conversion behaves as a signed truncation (checked against oracle/_ref). */
+static inline uint64_t axis_coord(double x, unsigned l) {
+    const double scale = (1.0 / EXTENT) * (double)((u128)1 << l);
+    const double v = scale * (x - (-EXTENT / 2.0));
+    const int64_t t = (int64_t)v;
+    const uint64_t mask = (l >= 64) ? ~0ULL : ((1ULL << l) - 1ULL);
+    return (uint64_t)t & mask;
+}
+
+static inline u128 key_from_coords(unsigned l, uint64_t kx, uint64_t ky, uint64_t kz) {
+    u128 k = 0;
+    for (unsigned i = 0; i < l; ++i) {
+        const unsigned b = l - 1 - i;
+        const unsigned nib = 8u | (unsigned)(((kz >> b) & 1u) << 2) | (unsigned)(((ky >> b) & 1u) << 1) |
+                             (unsigned)((kx >> b) & 1u);
+        k |= ((u128)nib) << (124 - 4 * i);
+    }
+    return k;
+}
+
+/* compute(l, x, y, z) (simplehyperoctreeindex.h:1167-1170); l = 32 is cipher (:1186-1189) */
+static inline u128 key_compute(unsigned l, double x, double y, double z) {
+    if (l == 0) return 0;
+    return key_from_coords(l, axis_coord(x, l), axis_coord(y, l), axis_coord(z, l));
+}
+
+/* level() (simplehyperoctreeindex.h:572-575): trailing zeros over the marker bits */
+static inline unsigned key_level(u128 k) {
+    const u128 m = k & MARK;
+    unsigned tz = 128;
+    if (m) {
+        const uint64_t lo = (uint64_t)m, hi = (uint64_t)(m >> 64);
+        tz = lo ? (unsigned)__builtin_ctzll(lo) : 64u + (unsigned)__builtin_ctzll(hi);
+    }
+    return (128u - tz) / 4u + (k != 0);
+}
+
+/* axis integer coordinate of a key at its level (pext over the axis bits) */
+static inline uint64_t key_axis(u128 k, unsigned l, unsigned d) {
+    uint64_t r = 0;
+    for (unsigned i = 0; i < l; ++i) r = (r << 1) | (uint64_t)((k >> (124 - 4 * i + d)) & 1u);
+    return r;
+}
+
+/* center(d) (simplehyperoctreeindex.h:890-893): -E/2 + E*((2k+1)/2^(l+1)) */
+static inline double key_center(u128 k, unsigned d) {
+    const unsigned l = key_level(k);
+    const uint64_t a = key_axis(k, l, d);
+    return (-EXTENT / 2.0) + (EXTENT * ((double)(2 * (u128)a + 1) / (double)((u128)1 << (l + 1))));
+}
+
+/* extent() with the default unit extent (simplehyperoctreeindex.h:952-955) */
+static inline double key_extent1(u128 k) { return 1.0 / (1.0 * (double)((u128)1 << key_level(k))); }
+
+/* parent() (simplehyperoctreeindex.h:679-682): clear the last nibble */
+static inline u128 key_parent(u128 k) {
+    const unsigned l = key_level(k);
+    if (l == 0) return k;
+    const unsigned sh = 128 - 4 * (l - 1);
+    const u128 keep = (sh >= 128) ? 0 : ~((((u128)1) << sh) - 1);
+    return k & keep;
+}
+
+/* child(isite) (simplehyperoctreeindex.h:690-694) */
+static inline u128 key_child(u128 k, unsigned isite) {
+    const unsigned l = key_level(k);
+    if (l >= 32) return k;
+    return k | (((u128)((isite & 7u) | 8u)) << (128 - 4 * (l + 1)));
+}
+
+/* containing() (simplehyperoctreeindex.h:998-1001) */
+static inline int key_containing(u128 a, u128 b) {
+    const u128 x = a ^ b;
+    return ((~MARK & x) <= (MARK & x)) && (a <= b);
+}
+
+/* ---------------------------------------------------------------- octree */
+
+/* std::upper_bound on the key-sorted container (simplehyperoctree.h:894-899) */
+static inline uint64_t upper_bound(const mpfo_octree* o, u128 q) {
+    uint64_t lo = 0, n = o->n;
+    while (n > 0) {
+        const uint64_t half = n / 2;
+        if (!(q < o->key[lo + half])) { lo = lo + half + 1; n = n - half - 1; }
+        else n = half;
+    }
+    return lo;
+}
+
+static inline void load_data(const mpfo_octree* o, int prec, uint64_t i, fdata* d) {
+    if (prec == 1 && o->f64) {
+        for (int k = 0; k < 7; ++k) d->v[k] = o->f64[7 * i + k];
+    } else {
+        for (int k = 0; k < 7; ++k) d->v[k] = (double)o->f32[7 * i + k];
+    }
+}
+
+static inline void zero_data(fdata* d) { memset(d, 0, sizeof(*d)); }
+static inline int data_empty(const fdata* d) {
+    for (int k = 0; k < 7; ++k) if (d->v[k] != 0.0) return 0;
+    return 1;
+}
+
+/* mac() (simplehyperoctree.h:1870-1910): accumulator += factor*coefficient.
+ * precision 0: accumulator is float, so (float)((double)acc + (double)f*w). */
+static inline void mac(fdata* acc, const fdata* f, double w, int prec) {
+    if (prec == 0) {
+        for (int k = 0; k < 7; ++k) acc->v[k] = (double)(float)(acc->v[k] + f->v[k] * w);
+    } else {
+        for (int k = 0; k < 7; ++k) acc->v[k] = acc->v[k] + f->v[k] * w;
+    }
+}
+
+/* locate() (simplehyperoctree.h:894-899): returns -1 for end() */
+static int64_t locate(const mpfo_octree* o, double x, double y, double z) {
+    const u128 q = key_compute(32, x, y, z);
+    const uint64_t m = upper_bound(o, q);
+    const uint64_t c = m - (m > 0);
+    return key_containing(o->key[c], q) ? (int64_t)c : -1;
+}
+
+/* ngp() (simplehyperoctree.h:1396-1401) */
+static void ngp(const mpfo_octree* o, int prec, double x, double y, double z, fdata* r) {
+    const u128 q = key_compute(32, x, y, z);
+    const uint64_t m = upper_bound(o, q);
+    if (key_containing(o->key[m - (m > 0)], q)) load_data(o, prec, m - 1, r);
+    else zero_data(r);
+}
+
+/* cic() (simplehyperoctree.h:1412-1469) */
+static void cic(const mpfo_octree* o, int prec, double x, double y, double z, fdata* result) {
+    const double one = 1.0, two = 2.0;
+    const double point[3] = {x, y, z};
+    u128 ekey = key_compute(32, x, y, z);
+    uint64_t marker = upper_bound(o, ekey);
+    u128 idx = key_containing(o->key[marker - (marker > 0)], ekey) ? o->key[marker - 1] : INVALID;
+    unsigned ilvl = key_level(idx);
+    unsigned lvl = ilvl;
+    double half = EXTENT / 1.0 * key_extent1(idx) / two;
+    double vertex[3], dist[3];
+    unsigned site = 0;
+    fdata cell;
+    zero_data(result);
+    do {
+        ilvl = lvl;
+        if (idx != INVALID) {
+            for (unsigned d = 0; d < 3; ++d) dist[d] = fabs((key_center(idx, d) - point[d]) / (half + half));
+            for (unsigned isite = 0; isite < 8; ++isite) {
+                if (key_containing(key_child(idx, isite), ekey)) { site = isite; break; }
+            }
+            for (unsigned iv = 0; iv < 8; ++iv) {
+                double weight = one;
+                const unsigned bits = iv, diff = site ^ bits;
+                for (unsigned d = 0; d < 3; ++d) {
+                    const int b = (bits >> d) & 1u, df = (diff >> d) & 1u;
+                    vertex[d] = point[d] + (b * half) - ((!b) * half);
+                    weight *= ((df) * (one - dist[d])) + ((!df) * dist[d]);
+                }
+                ekey = key_compute(ilvl, vertex[0], vertex[1], vertex[2]);
+                marker = upper_bound(o, ekey);
+                if (key_containing(o->key[marker - (marker > 0)], ekey)) {
+                    const unsigned fl = key_level(o->key[marker - 1]);
+                    lvl = lvl < fl ? lvl : fl;
+                    load_data(o, prec, marker - 1, &cell);
+                    mac(result, &cell, weight, prec);
+                } else {
+                    zero_data(result);
+                    break;
+                }
+            }
+        }
+        if (lvl < ilvl) {
+            ekey = key_compute(lvl + 1, x, y, z);
+            idx = key_parent(ekey);
+            zero_data(result);
+            half = EXTENT / 1.0 * key_extent1(idx) / two;
+        }
+    } while (lvl < ilvl);
+}
+
+/* the 27-cell cache of tsc(elemsTsc, ...) (simplehyperoctree.h:1554-1644) */
+typedef struct {
+    u128 key[27];
+    fdata data[27];
+} tsc_cache;
+
+static void tsc_cache_init(tsc_cache* c) { memset(c, 0, sizeof(*c)); }
+
+static inline double tsc_weight(int aix, int aiy, int aiz, const double* dist) {
+    return (aix * 0.5 * (1.5 - dist[0]) * (1.5 - dist[0]) + (1 - aix) * (0.75 - dist[0] * dist[0])) *
+           (aiy * 0.5 * (1.5 - dist[1]) * (1.5 - dist[1]) + (1 - aiy) * (0.75 - dist[1] * dist[1])) *
+           (aiz * 0.5 * (1.5 - dist[2]) * (1.5 - dist[2]) + (1 - aiz) * (0.75 - dist[2] * dist[2]));
+}
+
+/* tsc() with (cache != NULL, simplehyperoctree.h:1554-1644) or without
+ * (cache == NULL, simplehyperoctree.h:1481-1541) the 27-cell cache. */
+static void tsc(const mpfo_octree* o, int prec, tsc_cache* cache, double x, double y, double z, fdata* result) {
+    const double two = 2.0;
+    const double point[3] = {x, y, z};
+    if (point[0] == 0.0 && point[1] == 0.0 && point[2] == 0.0) {
+        cic(o, prec, 0., 0., 0., result);
+        return;
+    }
+    u128 ekey = key_compute(32, x, y, z);
+    uint64_t marker = upper_bound(o, ekey);
+    u128 idx = key_containing(o->key[marker - (marker > 0)], ekey) ? o->key[marker - 1] : INVALID;
+    unsigned ilvl = key_level(idx);
+    unsigned lvl = ilvl;
+    double half = EXTENT / 1.0 * key_extent1(idx) / two;
+    double dist[3];
+    tsc_cache tmp;
+    fdata cell;
+    zero_data(result);
+    do {
+        ilvl = lvl;
+        const double twohalves = two * half;
+        if (cache && cache->key[13] == idx) {
+            unsigned ic = 0;
+            for (int ix = -1; ix <= 1; ++ix)
+                for (int iy = -1; iy <= 1; ++iy)
+                    for (int iz = -1; iz <= 1; ++iz) {
+                        for (unsigned d = 0; d < 3; ++d)
+                            dist[d] = fabs((key_center(cache->key[ic], d) - point[d]) / (twohalves));
+                        const double w = tsc_weight(abs(ix), abs(iy), abs(iz), dist);
+                        mac(result, &cache->data[ic], w, prec);
+                        ic++;
+                    }
+            return;
+        }
+        if (idx != INVALID) {
+            unsigned ic = 0;
+            for (int ix = -1; ix <= 1; ++ix) {
+                for (int iy = -1; iy <= 1; ++iy) {
+                    for (int iz = -1; iz <= 1; ++iz) {
+                        ekey = key_compute(ilvl, point[0] + ix * twohalves, point[1] + iy * twohalves,
+                                           point[2] + iz * twohalves);
+                        marker = upper_bound(o, ekey);
+                        if (key_containing(o->key[marker - (marker > 0)], ekey)) {
+                            const uint64_t found = marker - (marker > 0);
+                            for (unsigned d = 0; d < 3; ++d)
+                                dist[d] = fabs((key_center(ekey, d) - point[d]) / (twohalves));
+                            const double w = tsc_weight(abs(ix), abs(iy), abs(iz), dist);
+                            const unsigned fl = key_level(o->key[found]);
+                            lvl = lvl < fl ? lvl : fl;
+                            if (lvl < ilvl) {
+                                zero_data(result);
+                                goto done;
+                            }
+                            load_data(o, prec, found, &cell);
+                            if (cache) { tmp.key[ic] = o->key[found]; tmp.data[ic] = cell; }
+                            mac(result, &cell, w, prec);
+                            ic++;
+                        } else {
+                            zero_data(result);
+                            goto done;
+                        }
+                    }
+                }
+            }
+        done:
+            if (cache && ic == 27) *cache = tmp;
+        }
+        if (lvl < ilvl) {
+            ekey = key_compute(lvl + 1, x, y, z);
+            idx = key_parent(ekey);
+            zero_data(result);
+            half = EXTENT / 1.0 * key_extent1(idx) / two;
+        }
+    } while (lvl < ilvl);
+}
+
+/* ------------------------------------------------------------ cosmology */
+
+/* Utility::interpolate (utility.h:442-449): upper_bound, clamp to [1, n-1],
+ * linear interpolation/extrapolation. */
+static inline double interpolate(double x0, const double* x, const double* y, int64_t n) {
+    if (n <= 0) return 0.0;
+    if (n == 1) return y[0];
+    int64_t lo = 0, cnt = n;
+    while (cnt > 0) {
+        const int64_t h = cnt / 2;
+        if (!(x0 < x[lo + h])) { lo = lo + h + 1; cnt = cnt - h - 1; }
+        else cnt = h;
+    }
+    int64_t i = lo;
+    i += (i <= 0) - (i >= n);
+    return y[i - 1] + ((y[i] - y[i - 1]) * (x0 - x[i - 1]) / (x[i] - x[i - 1]));
+}
+
+/* ----------------------------------------------------------- integration */
+
+typedef struct {
+    double core[9]; /* a, t, x, y, z, dtdl, dxdl, dydl, dzdl (photon.h:44-47) */
+    double level, ah, rho, phi, dphidx, dphidy, dphidz, dphidl, laplacian, redshift, dsdl2, error, distance;
+    double isw, iswold, chi, lambda, dphidt, s;
+} photon;
+enum { CA, CT, CX, CY, CZ, CKT, CKX, CKY, CKZ };
+
+typedef struct {
+    const mpfo_octree* o;
+    int order, prec;
+    const double *ct, *ca, *cdadt;
+    int64_t ncos;
+    double length;
+    unsigned nsteps;
+    tsc_cache cache;
+} integ_ctx;
+
+static void interp_order(integ_ctx* g, int order, double x, double y, double z, fdata* d) {
+    switch (order) {
+        case 0: ngp(g->o, g->prec, x, y, z, d); break;
+        case 1: cic(g->o, g->prec, x, y, z, d); break;
+        case 2: tsc(g->o, g->prec, &g->cache, x, y, z, d); break;
+        default: zero_data(d); break;
+    }
+}
+
+/* Integrator::dphotondl (integrator.h:409-445) */
+static void dphotondl(integ_ctx* g, double* out, const double* in) {
+    const double two = 2.0;
+    fdata data;
+    interp_order(g, g->order, in[CX], in[CY], in[CZ], &data);
+    const double dphidl = data.v[F_DT] * in[CKT] + (data.v[F_DX] * in[CKX] + data.v[F_DY] * in[CKY] + data.v[F_DZ] * in[CKZ]);
+    const double dadt = interpolate(in[CT], g->ct, g->cdadt, g->ncos);
+    const double scale = g->length;
+    out[CA] = in[CKT] * dadt;
+    out[CT] = in[CKT];
+    out[CX] = in[CKX] / scale;
+    out[CY] = in[CKY] / scale;
+    out[CZ] = in[CKZ] / scale;
+    out[CKT] = -(two * dadt / in[CA] * in[CKT] * in[CKT]) - (two / C2 * in[CKT]) * (dphidl - data.v[F_DT] * in[CKT]);
+    out[CKX] = -(two * dadt / in[CA] * in[CKT] * in[CKX]) + (two / C2 * dphidl * in[CKX]) - (two * data.v[F_DX] * in[CKT] * in[CKT]);
+    out[CKY] = -(two * dadt / in[CA] * in[CKT] * in[CKY]) + (two / C2 * dphidl * in[CKY]) - (two * data.v[F_DY] * in[CKT] * in[CKT]);
+    out[CKZ] = -(two * dadt / in[CA] * in[CKT] * in[CKZ]) + (two / C2 * dphidl * in[CKZ]) - (two * data.v[F_DZ] * in[CKT] * in[CKT]);
+}
+
+static void photon_to_rec(const photon* p, double* o, double nsteps) {
+    for (int k = 0; k < 9; ++k) o[k] = p->core[k];
+    o[MPFO_REDSHIFT] = p->redshift; o[MPFO_LAMBDA] = p->lambda; o[MPFO_S] = p->s; o[MPFO_CHI] = p->chi;
+    o[MPFO_ISW] = p->isw; o[MPFO_ISWOLD] = p->iswold; o[MPFO_PHI] = p->phi; o[MPFO_DPHIDX] = p->dphidx;
+    o[MPFO_DPHIDY] = p->dphidy; o[MPFO_DPHIDZ] = p->dphidz; o[MPFO_DPHIDT] = p->dphidt; o[MPFO_RHO] = p->rho;
+    o[MPFO_AH] = p->ah; o[MPFO_DPHIDL] = p->dphidl; o[MPFO_LAPLACIAN] = p->laplacian; o[MPFO_DSDL2] = p->dsdl2;
+    o[MPFO_ERROR] = p->error; o[MPFO_LEVEL] = p->level; o[MPFO_NSTEPS] = nsteps;
+}
+
+/* growable trajectory (Evolution<Photon>, evolution.h:51, append :908, pop :894) */
+typedef struct {
+    photon* p;
+    uint64_t n, cap;
+    int keep; /* keep all elements (1) or only the last two (0) */
+} traj_t;
+
+static void traj_append(traj_t* t, const photon* p) {
+    if (!t->keep) {
+        if (t->n >= 2) { t->p[0] = t->p[1]; t->p[1] = *p; }
+        else t->p[t->n] = *p;
+        t->n++;
+        return;
+    }
+    if (t->n == t->cap) {
+        t->cap = t->cap ? 2 * t->cap : 1024;
+        t->p = (photon*)realloc(t->p, t->cap * sizeof(photon));
+    }
+    t->p[t->n++] = *p;
+}
+static inline photon* traj_back(traj_t* t) { return t->keep ? &t->p[t->n - 1] : &t->p[t->n >= 2 ? 1 : 0]; }
+static inline photon* traj_prev(traj_t* t) { return t->keep ? &t->p[t->n - 2] : &t->p[0]; }
+
+static double locate_level(const mpfo_octree* o, double x, double y, double z) {
+    const int64_t e = locate(o, x, y, z);
+    return e < 0 ? 0.0 : (double)key_level(o->key[e]);
+}
+static double locate_extent(const mpfo_octree* o, double x, double y, double z) {
+    const int64_t e = locate(o, x, y, z);
+    const unsigned l = e < 0 ? 0u : key_level(o->key[e]);
+    return EXTENT / (1.0 * (double)((u128)1 << l));
+}
+
+/* Integrator::integrate, no stop criterion (integrator.h:477-618) when
+ * stop_kind < 0, else with the criterion of integrator.h:648-824:
+ * 0 radius, 1 lambda, 2 redshift, 3 t, 4 a, 5 plane, 6 s. */
+static void integrate(integ_ctx* g, traj_t* traj, int stop_kind, double ref, const double* kit) {
+    const double zero = 0.0, one = 1.0, two = 2.0, six = 6.0;
+    const double c = C_LIGHT, c2 = C2;
+    const double min = 0.0 - (EXTENT / two), max = 0.0 + (EXTENT / two);
+    const double scale = g->length;
+    fdata data, empty;
+    photon ph;
+    double dcoredl[4][9];
+    zero_data(&empty);
+    memset(&ph, 0, sizeof(ph));
+    if (traj->n == 0) return;
+    tsc_cache_init(&g->cache);
+    {
+        photon* b = traj_back(traj);
+        if (g->order >= 0) interp_order(g, g->order, b->core[CX], b->core[CY], b->core[CZ], &data);
+        else { zero_data(&data); data.v[F_A] = one; }
+        const double norm = sqrt((c2 * (one + two * (data.v[F_PHI] / c2)) * b->core[CKT] * b->core[CKT]) /
+                                 ((one - two * (data.v[F_PHI] / c2)) *
+                                  (b->core[CKX] * b->core[CKX] + b->core[CKY] * b->core[CKY] + b->core[CKZ] * b->core[CKZ])));
+        b->core[CKX] *= norm;
+        b->core[CKY] *= norm;
+        b->core[CKZ] *= norm;
+        b->core[CA] = interpolate(b->core[CT], g->ct, g->ca, g->ncos);
+        b->level = locate_level(g->o, b->core[CX], b->core[CY], b->core[CZ]);
+        b->ah = zero;
+        b->rho = data.v[F_RHO];
+        b->phi = data.v[F_PHI];
+        b->dphidx = data.v[F_DX];
+        b->dphidy = data.v[F_DY];
+        b->dphidz = data.v[F_DZ];
+        b->dphidt = data.v[F_DT];
+        b->dphidl = zero;
+        b->laplacian = zero;
+        b->redshift = zero;
+        b->dsdl2 = (b->core[CA] * b->core[CA]) *
+                   (-(c2 * (one + two * (b->phi / c2)) * b->core[CKT] * b->core[CKT]) +
+                    ((one - two * (b->phi / c2)) *
+                     (b->core[CKX] * b->core[CKX] + b->core[CKY] * b->core[CKY] + b->core[CKZ] * b->core[CKZ])));
+        b->error = one - ((one - two * (b->phi / c2)) *
+                          (b->core[CKX] * b->core[CKX] + b->core[CKY] * b->core[CKY] + b->core[CKZ] * b->core[CKZ])) /
+                             (c2 * (one + two * (b->phi / c2)) * b->core[CKT] * b->core[CKT]);
+        b->distance = zero;
+        b->isw = zero;
+        b->chi = zero;
+        b->iswold = zero;
+        b->lambda = zero;
+        b->s = zero;
+    }
+    photon* b0 = traj_back(traj);
+    double ratio = b0->core[CA] * b0->core[CA] * (scale / c) / g->nsteps;
+    double dl = locate_extent(g->o, b0->core[CX], b0->core[CY], b0->core[CZ]) * ratio;
+    const double a0 = b0->core[CA];
+    const double phi0c2 = b0->phi / c2;
+    const int order_step = g->order;
+    while (!data_empty(&data)) {
+        photon* back = traj_back(traj);
+        dphotondl(g, dcoredl[0], back->core);
+        for (int i = 0; i < 9; ++i) ph.core[i] = back->core[i] + dl / two * dcoredl[0][i];
+        dphotondl(g, dcoredl[1], ph.core);
+        for (int i = 0; i < 9; ++i) ph.core[i] = back->core[i] + dl / two * dcoredl[1][i];
+        dphotondl(g, dcoredl[2], ph.core);
+        for (int i = 0; i < 9; ++i) ph.core[i] = back->core[i] + dl * dcoredl[2][i];
+        dphotondl(g, dcoredl[3], ph.core);
+        for (int i = 0; i < 9; ++i)
+            ph.core[i] = back->core[i] + (dl / six) * (dcoredl[0][i] + two * dcoredl[1][i] + two * dcoredl[2][i] + dcoredl[3][i]);
+        if (order_step >= 0) interp_order(g, order_step, ph.core[CX], ph.core[CY], ph.core[CZ], &data);
+        else { zero_data(&data); data.v[F_A] = one; }
+        if (!(!(ph.core[CA] < zero) && ((ph.core[CX] > min) && (ph.core[CX] < max) && (ph.core[CY] > min) &&
+                                        (ph.core[CY] < max) && (ph.core[CZ] > min) && (ph.core[CZ] < max))))
+            data = empty;
+        ph.level = locate_level(g->o, ph.core[CX], ph.core[CY], ph.core[CZ]);
+        ph.ah = data.v[F_A];
+        ph.rho = data.v[F_RHO];
+        ph.phi = data.v[F_PHI];
+        ph.dphidx = data.v[F_DX];
+        ph.dphidy = data.v[F_DY];
+        ph.dphidz = data.v[F_DZ];
+        ph.dphidt = -data.v[F_DT];
+        ph.dphidl = data.v[F_DT] * ph.core[CKT] + (ph.dphidx * ph.core[CKX] + ph.dphidy * ph.core[CKY] + ph.dphidz * ph.core[CKZ]);
+        ph.laplacian = (data.v[F_PHI] - back->phi) / dl;
+        ph.dsdl2 = (ph.core[CA] * ph.core[CA]) *
+                   ((-c2 * (one + two * (ph.phi / c2)) * ph.core[CKT] * ph.core[CKT]) +
+                    ((one - two * (ph.phi / c2)) *
+                     (ph.core[CKX] * ph.core[CKX] + ph.core[CKY] * ph.core[CKY] + ph.core[CKZ] * ph.core[CKZ])));
+        ph.error = one - ((one - two * (ph.phi / c2)) *
+                          (ph.core[CKX] * ph.core[CKX] + ph.core[CKY] * ph.core[CKY] + ph.core[CKZ] * ph.core[CKZ])) /
+                             (c2 * (one + two * (ph.phi / c2)) * ph.core[CKT] * ph.core[CKT]);
+        ph.distance = zero;
+        ph.isw += (back->dphidt + ph.dphidt) * (ph.core[CT] - back->core[CT]) / c2;
+        ph.iswold -= 2 * (ph.phi - back->phi) / c2 -
+                     ((back->dphidx + ph.dphidx) * (ph.core[CX] - back->core[CX]) * scale +
+                      (back->dphidy + ph.dphidy) * (ph.core[CY] - back->core[CY]) * scale +
+                      (back->dphidz + ph.dphidz) * (ph.core[CZ] - back->core[CZ]) * scale) /
+                         c2;
+        ph.chi = sqrt(ph.core[CX] * ph.core[CX] + ph.core[CY] * ph.core[CY] + ph.core[CZ] * ph.core[CZ]);
+        ph.lambda += dl;
+        ph.s += sqrt((ph.core[CX] - back->core[CX]) * (ph.core[CX] - back->core[CX]) +
+                     (ph.core[CY] - back->core[CY]) * (ph.core[CY] - back->core[CY]) +
+                     (ph.core[CZ] - back->core[CZ]) * (ph.core[CZ] - back->core[CZ]));
+        ph.redshift = a0 / ph.core[CA] * (one + phi0c2 - ph.phi / c2) - one;
+        if (!data_empty(&data)) {
+            ratio = ph.core[CA] * ph.core[CA] * (scale / c) / g->nsteps;
+            dl = locate_extent(g->o, ph.core[CX], ph.core[CY], ph.core[CZ]) * ratio;
+            traj_append(traj, &ph);
+        }
+        if (stop_kind >= 0) {
+            const photon* bk = traj_back(traj);
+            int brk = 0;
+            switch (stop_kind) {
+                case 0: brk = bk->chi > ref; break;
+                case 1: brk = bk->lambda > ref; break;
+                case 2: brk = bk->redshift > ref; break;
+                case 3: brk = bk->core[CT] > ref; break;
+                case 4: brk = bk->core[CA] < ref; break;
+                case 5: brk = (ph.core[CX] * kit[0] + ph.core[CY] * kit[1] + ph.core[CZ] * kit[2]) > ref; break;
+                case 6: brk = bk->s > ref; break;
+                default: brk = 1; break;
+            }
+            if (brk) break;
+        }
+    }
+    if (traj->n > 0) {
+        const photon* bk = traj_back(traj);
+        if (signbit(bk->redshift) || signbit(bk->core[CA])) {
+            if (traj->keep) traj->n--;
+            else {
+                /* only the last two are kept: the popped state is the previous one */
+                if (traj->n >= 2) traj->p[1] = traj->p[0];
+                traj->n--;
+            }
+        }
+    }
+}
+
+static void launch(traj_t* t, const double* in9) {
+    photon p;
+    memset(&p, 0, sizeof(p));
+    for (int k = 0; k < 9; ++k) p.core[k] = in9[k];
+    traj_append(t, &p);
+}
+
+static void ctx_init(integ_ctx* g, const mpfo_octree* o, int order, int prec, const double* tab, uint64_t ncos,
+                     double length, unsigned nsteps) {
+    g->o = o;
+    g->order = order;
+    g->prec = prec;
+    g->ct = tab;
+    g->ca = tab + ncos;
+    g->cdadt = tab + 2 * ncos;
+    g->ncos = (int64_t)ncos;
+    g->length = length;
+    g->nsteps = nsteps;
+    tsc_cache_init(&g->cache);
+}
+
+/* --------------------------------------------------------------- public */
+
+mpfo_octree* mpfo_octree_new(const void* rec48, uint64_t n) {
+    mpfo_octree* o = (mpfo_octree*)calloc(1, sizeof(*o));
+    o->n = n;
+    o->key = (u128*)malloc((n ? n : 1) * sizeof(u128));
+    o->f32 = (float*)malloc((n ? n : 1) * 7 * sizeof(float));
+    const unsigned char* r = (const unsigned char*)rec48;
+    for (uint64_t i = 0; i < n; ++i) {
+        const unsigned char* e = r + 48 * i;
+        uint64_t lo, hi;
+        memcpy(&lo, e, 8);
+        memcpy(&hi, e + 8, 8);
+        o->key[i] = (((u128)hi) << 64) | lo;
+        /* record layout (fact: libstdc++ stores the tuple in reverse):
+         * dphidt@16 a@20 dphidx@24 dphidy@28 dphidz@32 phi@36 rho@40 */
+        float f[7];
+        memcpy(f, e + 16, 28);
+        float* d = o->f32 + 7 * i;
+        d[F_DT] = f[0]; d[F_A] = f[1]; d[F_DX] = f[2]; d[F_DY] = f[3]; d[F_DZ] = f[4]; d[F_PHI] = f[5]; d[F_RHO] = f[6];
+    }
+    return o;
+}
+
+int mpfo_octree_set_fields64(mpfo_octree* o, const double* f7, uint64_t n) {
+    if (n != o->n) return -1;
+    free(o->f64);
+    o->f64 = (double*)malloc((n ? n : 1) * 7 * sizeof(double));
+    memcpy(o->f64, f7, n * 7 * sizeof(double));
+    return 0;
+}
+
+void mpfo_octree_free(mpfo_octree* o) {
+    if (!o) return;
+    free(o->key);
+    free(o->f32);
+    free(o->f64);
+    free(o);
+}
+
+void mpfo_compute_key(unsigned level, const double* xyz, uint64_t m, uint64_t* lohi) {
+    for (uint64_t i = 0; i < m; ++i) {
+        const u128 k = key_compute(level > 32 ? 32 : level, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+        lohi[2 * i] = (uint64_t)k;
+        lohi[2 * i + 1] = (uint64_t)(k >> 64);
+    }
+}
+
+void mpfo_key_info(const uint64_t* lohi, uint64_t m, uint32_t* level, double* center3, double* ext, uint64_t* parent) {
+    for (uint64_t i = 0; i < m; ++i) {
+        const u128 k = (((u128)lohi[2 * i + 1]) << 64) | lohi[2 * i];
+        level[i] = key_level(k);
+        for (unsigned d = 0; d < 3; ++d) center3[3 * i + d] = key_center(k, d);
+        ext[i] = EXTENT / 1.0 * key_extent1(k);
+        const u128 p = key_parent(k);
+        parent[2 * i] = (uint64_t)p;
+        parent[2 * i + 1] = (uint64_t)(p >> 64);
+    }
+}
+
+int mpfo_key_containing(const uint64_t* a, const uint64_t* b) {
+    const u128 ka = (((u128)a[1]) << 64) | a[0];
+    const u128 kb = (((u128)b[1]) << 64) | b[0];
+    return key_containing(ka, kb);
+}
+
+void mpfo_locate(const mpfo_octree* o, const double* xyz, uint64_t m, int64_t* elem, uint32_t* level) {
+    for (uint64_t i = 0; i < m; ++i) {
+        const int64_t e = locate(o, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+        elem[i] = e;
+        level[i] = e < 0 ? 0u : key_level(o->key[e]);
+    }
+}
+
+void mpfo_interp(const mpfo_octree* o, int order, int precision, const double* xyz, uint64_t m, double* out7) {
+    tsc_cache cache;
+    tsc_cache_init(&cache);
+    for (uint64_t i = 0; i < m; ++i) {
+        fdata d;
+        const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+        if (order == 0) ngp(o, precision, x, y, z, &d);
+        else if (order == 1) cic(o, precision, x, y, z, &d);
+        else if (order == 2) tsc(o, precision, NULL, x, y, z, &d);
+        else tsc(o, precision, &cache, x, y, z, &d);
+        for (int k = 0; k < 7; ++k) out7[7 * i + k] = d.v[k];
+    }
+}
+
+typedef struct {
+    const mpfo_octree* o;
+    int order, prec, stop_kind;
+    const double* tab;
+    uint64_t ncos;
+    double length, stop_value;
+    unsigned nsteps;
+    const double* kit;
+    const double* in9;
+    double* out;
+    uint64_t begin, end, steps;
+} work_t;
+
+static void* worker(void* arg) {
+    work_t* w = (work_t*)arg;
+    integ_ctx g;
+    photon buf[2];
+    ctx_init(&g, w->o, w->order, w->prec, w->tab, w->ncos, w->length, w->nsteps);
+    w->steps = 0;
+    for (uint64_t i = w->begin; i < w->end; ++i) {
+        traj_t t = {buf, 0, 2, 0};
+        launch(&t, w->in9 + 9 * i);
+        integrate(&g, &t, w->stop_kind, w->stop_value, w->kit);
+        double* o = w->out + MPFO_NREC * i;
+        if (t.n == 0) {
+            memset(o, 0, MPFO_NREC * sizeof(double));
+            o[MPFO_NSTEPS] = -1.0;
+        } else {
+            photon_to_rec(traj_back(&t), o, (double)(t.n - 1));
+            w->steps += t.n - 1;
+        }
+    }
+    return NULL;
+}
+
+uint64_t mpfo_integrate(const mpfo_octree* o, int order, int precision, const double* cosmo_tab, uint64_t ncos,
+                        double length, unsigned nsteps, int stop_kind, double stop_value, const double* kitarget,
+                        const double* in9, uint64_t n, double* out, int nthreads) {
+    static const double zk[3] = {0, 0, 0};
+    if (nthreads < 1) nthreads = 1;
+    if ((uint64_t)nthreads > n) nthreads = n ? (int)n : 1;
+    work_t* w = (work_t*)calloc((size_t)nthreads, sizeof(work_t));
+    pthread_t* th = (pthread_t*)calloc((size_t)nthreads, sizeof(pthread_t));
+    const uint64_t chunk = (n + nthreads - 1) / nthreads;
+    for (int k = 0; k < nthreads; ++k) {
+        w[k] = (work_t){o, order, precision, stop_kind, cosmo_tab, ncos, length, stop_value, nsteps,
+                        kitarget ? kitarget : zk, in9, out, 0, 0, 0};
+        w[k].begin = chunk * k < n ? chunk * k : n;
+        w[k].end = chunk * (k + 1) < n ? chunk * (k + 1) : n;
+    }
+    for (int k = 1; k < nthreads; ++k) pthread_create(&th[k], NULL, worker, &w[k]);
+    worker(&w[0]);
+    uint64_t total = w[0].steps;
+    for (int k = 1; k < nthreads; ++k) {
+        pthread_join(th[k], NULL);
+        total += w[k].steps;
+    }
+    free(w);
+    free(th);
+    return total;
+}
+
+static void run_full(const mpfo_octree* o, int order, int precision, const double* tab, uint64_t ncos, double length,
+                     unsigned nsteps, int stop_kind, double stop_value, const double* kit, const double* in9,
+                     traj_t* t) {
+    static const double zk[3] = {0, 0, 0};
+    integ_ctx g;
+    ctx_init(&g, o, order, precision, tab, ncos, length, nsteps);
+    t->keep = 1;
+    launch(t, in9);
+    integrate(&g, t, stop_kind, stop_value, kit ? kit : zk);
+}
+
+uint64_t mpfo_trajectory(const mpfo_octree* o, int order, int precision, const double* cosmo_tab, uint64_t ncos,
+                         double length, unsigned nsteps, int stop_kind, double stop_value, const double* kitarget,
+                         const double* in9, double* rows, uint64_t maxlen) {
+    traj_t t = {NULL, 0, 0, 1};
+    run_full(o, order, precision, cosmo_tab, ncos, length, nsteps, stop_kind, stop_value, kitarget, in9, &t);
+    for (uint64_t i = 0; i < t.n && i < maxlen; ++i) photon_to_rec(&t.p[i], rows + MPFO_NREC * i, (double)i);
+    const uint64_t n = t.n;
+    free(t.p);
+    return n;
+}
+
+/* Lensing::dbetadtheta_infinitesimal (lensing.h:399-480) over the stored
+ * trajectory, with its own 27-cell cache. */
+uint64_t mpfo_jacobian_infinitesimal(const mpfo_octree* o, int order, int precision, const double* cosmo_tab,
+                                     uint64_t ncos, double length, unsigned nsteps, int stop_kind, double stop_value,
+                                     const double* kitarget, const double* in9, const double* dist, uint64_t nd,
+                                     double* out4) {
+    traj_t t = {NULL, 0, 0, 1};
+    run_full(o, order, precision, cosmo_tab, ncos, length, nsteps, stop_kind, stop_value, kitarget, in9, &t);
+    const double c2 = C2;
+    const unsigned levelshift = 2; /* log2(EXTENT) */
+    tsc_cache cache;
+    tsc_cache_init(&cache);
+    double* a11 = (double*)calloc(nd * (t.n ? t.n : 1), sizeof(double));
+    double* a12 = (double*)calloc(nd * (t.n ? t.n : 1), sizeof(double));
+    double* a22 = (double*)calloc(nd * (t.n ? t.n : 1), sizeof(double));
+    for (uint64_t i = 1; i < t.n; ++i) {
+        const photon* p = &t.p[i];
+        const double x = p->core[CX], y = p->core[CY], z = p->core[CZ], r = p->chi;
+        const double phi = atan2(y, x);
+        const double theta = acos(z / r);
+        const double cp = cos(phi), sp = sin(phi), ct = cos(theta), st = sin(theta);
+        const double s2p = sin(2 * phi), s2t = sin(2 * theta);
+        const double dl = pow(2, -(p->level - 1 - levelshift));
+        fdata xm, xp, ym, yp, zm, zp;
+        tsc(o, precision, &cache, x - 0.5 * dl, y, z, &xm);
+        tsc(o, precision, &cache, x + 0.5 * dl, y, z, &xp);
+        tsc(o, precision, &cache, x, y - 0.5 * dl, z, &ym);
+        tsc(o, precision, &cache, x, y + 0.5 * dl, z, &yp);
+        tsc(o, precision, &cache, x, y, z - 0.5 * dl, &zm);
+        tsc(o, precision, &cache, x, y, z + 0.5 * dl, &zp);
+        const double dxxp = (xp.v[F_DX] - xm.v[F_DX]) / dl;
+        const double dxyp = (yp.v[F_DX] - ym.v[F_DX]) / dl;
+        const double dxzp = (zp.v[F_DX] - zm.v[F_DX]) / dl;
+        const double dyyp = (yp.v[F_DY] - ym.v[F_DY]) / dl;
+        const double dyzp = (zp.v[F_DY] - zm.v[F_DY]) / dl;
+        const double dzzp = (zp.v[F_DZ] - zm.v[F_DZ]) / dl;
+        const double d1d1p = sp * sp * dxxp + cp * cp * dyyp - s2p * dxyp;
+        const double d2d2p = cp * cp * ct * ct * dxxp + sp * sp * ct * ct * dyyp + st * st * dzzp + s2p * ct * ct * dxyp -
+                             sp * s2t * dyzp - cp * s2t * dxzp;
+        const double d1d2p = ct * cp * sp * (dyyp - dxxp) + (cp * cp - sp * sp) * ct * dxyp + sp * st * dxzp - cp * st * dyzp;
+        const double dr = t.p[i].chi - t.p[i - 1].chi;
+        for (uint64_t j = 0; j < nd; ++j) {
+            const double gxsx = (dist[j] - p->chi) * p->chi / dist[j];
+            a11[j * t.n + i] = a11[j * t.n + i - 1] + gxsx * d1d1p * dr;
+            a12[j * t.n + i] = a12[j * t.n + i - 1] + gxsx * d1d2p * dr;
+            a22[j * t.n + i] = a22[j * t.n + i - 1] + gxsx * d2d2p * dr;
+        }
+    }
+    for (uint64_t j = 0; j < nd; ++j) {
+        /* upper_bound on chi, firstid = marked - (marked > 0) */
+        uint64_t lo = 0, cnt = t.n;
+        while (cnt > 0) {
+            const uint64_t h = cnt / 2;
+            if (!(dist[j] < t.p[lo + h].chi)) { lo = lo + h + 1; cnt = cnt - h - 1; }
+            else cnt = h;
+        }
+        const uint64_t first = lo - (lo > 0);
+        const double v11 = t.n ? a11[j * t.n + first] : 0.0;
+        const double v12 = t.n ? a12[j * t.n + first] : 0.0;
+        const double v22 = t.n ? a22[j * t.n + first] : 0.0;
+        out4[4 * j + 0] = 1 - 2 * v11 / c2 * length;
+        out4[4 * j + 1] = out4[4 * j + 2] = -2 * v12 / c2 * length;
+        out4[4 * j + 3] = 1 - 2 * v22 / c2 * length;
+    }
+    free(a11);
+    free(a12);
+    free(a22);
+    const uint64_t n = t.n;
+    free(t.p);
+    return n;
+}
+
+void mpfo_interpolate(const double* x, const double* y, uint64_t n, const double* x0, uint64_t m, double* out) {
+    for (uint64_t i = 0; i < m; ++i) out[i] = interpolate(x0[i], x, y, (int64_t)n);
+}

@@ -1,0 +1,55 @@
+/* mpf oracle: CPU restatement of the reference photon-propagation path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the checker
+ * or the timed CPU baseline.  The product (libmpf.so) never links or calls it.
+ *
+ * Every function cites the reference file:line it restates.  Pinning (what
+ * each function is checked against) is listed per function in mpf_oracle.c.
+ */
+#ifndef MPF_ORACLE_H
+#define MPF_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mpfo_octree mpfo_octree;
+
+/* Per-ray output record, doubles (same layout as libmpf's mpf_ray_out). */
+enum {
+    MPFO_A, MPFO_T, MPFO_X, MPFO_Y, MPFO_Z, MPFO_KT, MPFO_KX, MPFO_KY, MPFO_KZ,
+    MPFO_REDSHIFT, MPFO_LAMBDA, MPFO_S, MPFO_CHI, MPFO_ISW, MPFO_ISWOLD, MPFO_PHI,
+    MPFO_DPHIDX, MPFO_DPHIDY, MPFO_DPHIDZ, MPFO_DPHIDT, MPFO_RHO, MPFO_AH, MPFO_DPHIDL,
+    MPFO_LAPLACIAN, MPFO_DSDL2, MPFO_ERROR, MPFO_LEVEL, MPFO_NSTEPS, MPFO_NREC
+};
+
+/* precision: 0 = reference (FP32 fields, FP32 accumulation),
+ *            1 = FP64 (fields64 if loaded, else FP32 fields promoted; FP64 accumulation) */
+mpfo_octree* mpfo_octree_new(const void* rec48, uint64_t n);
+int mpfo_octree_set_fields64(mpfo_octree* o, const double* f7, uint64_t n);
+void mpfo_octree_free(mpfo_octree* o);
+
+void mpfo_compute_key(unsigned level, const double* xyz, uint64_t m, uint64_t* lohi);
+void mpfo_key_info(const uint64_t* lohi, uint64_t m, uint32_t* level, double* center3, double* ext, uint64_t* parent);
+int mpfo_key_containing(const uint64_t* a, const uint64_t* b);
+void mpfo_locate(const mpfo_octree* o, const double* xyz, uint64_t m, int64_t* elem, uint32_t* level);
+/* order 0 ngp, 1 cic, 2 uncached tsc, 3 cached tsc over the points in sequence */
+void mpfo_interp(const mpfo_octree* o, int order, int precision, const double* xyz, uint64_t m, double* out7);
+uint64_t mpfo_integrate(const mpfo_octree* o, int order, int precision, const double* cosmo_tab, uint64_t ncos,
+                        double length, unsigned nsteps, int stop_kind, double stop_value, const double* kitarget,
+                        const double* in9, uint64_t n, double* out, int nthreads);
+uint64_t mpfo_trajectory(const mpfo_octree* o, int order, int precision, const double* cosmo_tab, uint64_t ncos,
+                         double length, unsigned nsteps, int stop_kind, double stop_value, const double* kitarget,
+                         const double* in9, double* rows, uint64_t maxlen);
+uint64_t mpfo_jacobian_infinitesimal(const mpfo_octree* o, int order, int precision, const double* cosmo_tab,
+                                     uint64_t ncos, double length, unsigned nsteps, int stop_kind, double stop_value,
+                                     const double* kitarget, const double* in9, const double* dist, uint64_t nd,
+                                     double* out4);
+void mpfo_interpolate(const double* x, const double* y, uint64_t n, const double* x0, uint64_t m, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
