@@ -1,0 +1,131 @@
+/* mpf — MI355X-native photon propagation for Magrathea-Pathfinder inputs.
+ *
+ * C ABI of libmpf.so.  Plain pointers and sizes; no HIP or torch types.
+ * Every function returns an int status: 0 = OK, negative = error; the message
+ * of the last error is returned by mpf_last_error(ctx).  No function
+ * terminates the process (the reference calls std::terminate on bad
+ * parameters, integrator.h:807-810).  A context is single-threaded: it owns
+ * one HIP stream and its device allocations, and calls on one context must
+ * not overlap.  Buffers are host memory unless the name says dev_.
+ *
+ * Each entry point names the reference interface it replaces (file:line in
+ * the Magrathea-Pathfinder v2.0 sources).
+ */
+#ifndef MPF_H
+#define MPF_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mpf_ctx mpf_ctx;
+
+/* interpolation order: the reference ORDER macro (Makefile:35-37) */
+enum { MPF_NGP = 0, MPF_CIC = 1, MPF_TSC = 2 };
+
+/* precision tiers */
+enum {
+    /* FP32 fields, FP64 weights, FP32 accumulation in the reference's term
+     * order, FP64 ray state: bit-identical to the reference
+     * (simplehyperoctree.h:1870-1910 accumulates into Gravity<float>). */
+    MPF_PREC_REF = 0,
+    /* FP64 fields (mpf_load_octree_fields64, else the FP32 record fields
+     * promoted), FP64 accumulation, FP64 state: the plain FP64 kernel. */
+    MPF_PREC_FP64 = 1,
+    /* FP32 perturbation (interpolated phi, grad phi, dphi/dt and the
+     * perturbation terms of the geodesic equation), FP64 background (a, t,
+     * position, affine parameter, redshift and distance accumulations). */
+    MPF_PREC_SPLIT = 2
+};
+
+/* stop criteria of Integrator::integrate (integrator.h:779-811) */
+enum {
+    MPF_STOP_NONE = -1, /* integrator.h:477-618: run until the data ends */
+    MPF_STOP_RADIUS = 0, MPF_STOP_LAMBDA = 1, MPF_STOP_REDSHIFT = 2, MPF_STOP_T = 3,
+    MPF_STOP_A = 4, MPF_STOP_PLANE = 5, MPF_STOP_S = 6
+};
+
+typedef struct {
+    int device;       /* HIP device ordinal */
+    int order;        /* MPF_NGP / MPF_CIC / MPF_TSC */
+    int precision;    /* MPF_PREC_* */
+    unsigned nsteps;  /* integration steps per cell (parameter file `nsteps`) */
+    double length;    /* box length in metres: lboxmpch * mpc / h (rays.cpp:208) */
+} mpf_config;
+
+typedef struct {
+    int kind;          /* MPF_STOP_* */
+    double value;      /* interpRef */
+    double kitarget[3];/* plane normal for MPF_STOP_PLANE */
+} mpf_stop;
+
+/* Per-ray output: the last element of the trajectory after the final
+ * pop (integrator.h:816-821), as 28 doubles. */
+enum {
+    MPF_OUT_A, MPF_OUT_T, MPF_OUT_X, MPF_OUT_Y, MPF_OUT_Z, MPF_OUT_KT, MPF_OUT_KX, MPF_OUT_KY, MPF_OUT_KZ,
+    MPF_OUT_REDSHIFT, MPF_OUT_LAMBDA, MPF_OUT_S, MPF_OUT_CHI, MPF_OUT_ISW, MPF_OUT_ISWOLD, MPF_OUT_PHI,
+    MPF_OUT_DPHIDX, MPF_OUT_DPHIDY, MPF_OUT_DPHIDZ, MPF_OUT_DPHIDT, MPF_OUT_RHO, MPF_OUT_AH, MPF_OUT_DPHIDL,
+    MPF_OUT_LAPLACIAN, MPF_OUT_DSDL2, MPF_OUT_ERROR, MPF_OUT_LEVEL, MPF_OUT_NSTEPS, MPF_NOUT
+};
+
+/* Context lifecycle.  Replaces the per-process setup of rays.cpp:127-219. */
+int mpf_create(mpf_ctx** ctx, const mpf_config* cfg);
+int mpf_destroy(mpf_ctx* ctx);
+const char* mpf_last_error(const mpf_ctx* ctx);
+int mpf_set_precision(mpf_ctx* ctx, int precision);
+
+/* Load n cone-file records of 48 bytes, sorted by key, as read by
+ * Input::load (input.h:1298-1310, datahandler.h:940).  Replaces
+ * Miscellaneous::loadOctree (miscellaneous.h:423-455).  The tree must be
+ * parent-closed (every stored cell below the coarsest stored level has its
+ * parent stored), which the reference's cone files are; else -EINVAL. */
+int mpf_load_octree_records(mpf_ctx* ctx, const void* rec48, uint64_t n);
+/* Optional FP64 copy of the fields, n x 7 in record order
+ * (rho, phi, dphidx, dphidy, dphidz, a, dphidt), used by MPF_PREC_FP64 and as
+ * the source of the FP32 perturbation fields of MPF_PREC_SPLIT. */
+int mpf_load_octree_fields64(mpf_ctx* ctx, const double* f7, uint64_t n);
+int mpf_octree_info(const mpf_ctx* ctx, uint64_t* ncells, uint32_t* min_level, uint32_t* max_level);
+/* Host-only structural check of cone records, no device needed: keys well
+ * formed, strictly sorted (the precondition of the reference's upper_bound
+ * lookups, simplehyperoctree.h:894-899) and parent-closed.  Writes the
+ * coarsest and finest stored levels; on failure a message into msg. */
+int mpf_validate_octree(const void* rec48, uint64_t n, uint32_t* min_level, uint32_t* max_level, char* msg,
+                        uint64_t msg_size);
+
+/* Cosmology table (t, a, da/dt), n rows, t increasing.  The table the
+ * reference builds with Input::acquire + Input::correct (input.h:1091-1233,
+ * 1330-1378) and reads with Utility::interpolate (utility.h:442-449). */
+int mpf_set_cosmology(mpf_ctx* ctx, const double* t, const double* a, const double* dadt, uint64_t n);
+
+/* SimpleHyperOctree::locate (simplehyperoctree.h:894-899): index of the
+ * finest stored cell containing each point, in key (record) order, or -1
+ * for end(); level of that cell (0 for end()). */
+int mpf_locate(mpf_ctx* ctx, const double* xyz, uint64_t n, int64_t* elem_out, uint32_t* level_out);
+
+/* SimpleHyperOctree::ngp/cic/tsc (simplehyperoctree.h:1396, 1412, 1481):
+ * interpolated fields (n x 7, FIELDS order as above) at n points with the
+ * context precision.  order 3 = tsc with one 27-cell cache carried over the
+ * points in sequence (simplehyperoctree.h:1554), as the integrator uses it. */
+int mpf_interpolate(mpf_ctx* ctx, int order, const double* xyz, uint64_t n, double* out7);
+
+/* Integrator::integrate (integrator.h:477-618 without a stop criterion,
+ * 648-824 with one) of n rays.  in9: launch photon per ray (a, t, x, y, z,
+ * dt/dl, dx/dl, dy/dl, dz/dl) as built by Integrator::launch
+ * (integrator.h:264-329).  out: n x MPF_NOUT doubles.  total_steps (may be
+ * NULL): sum over rays of accepted RK4 steps (trajectory length - 1). */
+int mpf_integrate(mpf_ctx* ctx, const double* in9, uint64_t n, const mpf_stop* stop, double* out,
+                  uint64_t* total_steps);
+/* Same with device-resident buffers; returns once the work is queued on the
+ * context stream unless sync != 0. */
+int mpf_integrate_dev(mpf_ctx* ctx, const double* dev_in9, uint64_t n, const mpf_stop* stop, double* dev_out,
+                      int sync);
+/* Accepted steps and duration (HIP events on the context stream) of the
+ * last integration kernel. */
+int mpf_last_run(mpf_ctx* ctx, uint64_t* total_steps, double* kernel_ms);
+int mpf_synchronize(mpf_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

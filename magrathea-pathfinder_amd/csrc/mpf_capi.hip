@@ -1,0 +1,370 @@
+// C ABI of libmpf.so (declared in include/mpf.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mpf.h"
+#include "mpf_kernels.h"
+#include "mpf_split.h"
+#include "mpf_tree.h"
+
+namespace {
+
+constexpr int ERR_INVAL = -22;
+constexpr int ERR_NOMEM = -12;
+constexpr int ERR_HIP = -5;
+constexpr int ERR_STATE = -71;
+
+__global__ void fill_u64_kernel(unsigned long long* p, uint64_t n, unsigned long long v) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t ensure(size_t count) {
+        if (count <= n && p) return hipSuccess;
+        release();
+        hipError_t e = hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T));
+        if (e == hipSuccess) n = count;
+        else p = nullptr;
+        return e;
+    }
+};
+
+}  // namespace
+
+namespace mpf {
+hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_u64_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+}  // namespace mpf
+
+struct mpf_ctx {
+    mpf_config cfg{};
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int cus = 256;
+    std::string err;
+    // octree
+    bool have_tree = false;
+    uint64_t ncells = 0;
+    uint32_t min_level = 0, max_level = 0;
+    std::vector<uint32_t> where;  // record -> level-sorted index
+    DevBuf<uint32_t> base, child_first, elem;
+    DevBuf<uint8_t> child_mask;
+    DevBuf<float> f32;
+    DevBuf<double> f64;
+    mpf::DevTree dt;
+    mpf::SplitTree split;
+    bool split_ready = false;
+    // cosmology
+    DevBuf<double> cosmo;
+    int ncos = 0;
+    // run state
+    DevBuf<unsigned long long> counters;  // [0] queue, [1] steps
+    DevBuf<double> io_in, io_out;
+    DevBuf<int64_t> io_i64;
+    DevBuf<uint32_t> io_u32;
+    uint64_t last_steps = 0;
+    float last_ms = 0.f;
+    bool pending = false;
+};
+
+static int fail(mpf_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+static int hipfail(mpf_ctx* c, hipError_t e, const char* where) {
+    return fail(c, ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(c, expr)                                \
+    do {                                               \
+        hipError_t e_ = (expr);                        \
+        if (e_ != hipSuccess) return hipfail(c, e_, #expr); \
+    } while (0)
+
+extern "C" {
+
+int mpf_create(mpf_ctx** out, const mpf_config* cfg) {
+    if (!out || !cfg) return ERR_INVAL;
+    *out = nullptr;
+    if (cfg->order < 0 || cfg->order > 2 || cfg->precision < 0 || cfg->precision > 2 || cfg->nsteps == 0 ||
+        !(cfg->length > 0))
+        return ERR_INVAL;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return ERR_HIP;
+    if (cfg->device < 0 || cfg->device >= ndev) return ERR_INVAL;
+    mpf_ctx* c = new mpf_ctx;
+    c->cfg = *cfg;
+    if (hipSetDevice(cfg->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return ERR_HIP;
+    }
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, cfg->device) == hipSuccess && cus > 0)
+        c->cus = cus;
+    if (c->counters.ensure(2) != hipSuccess) {
+        delete c;
+        return ERR_NOMEM;
+    }
+    *out = c;
+    return 0;
+}
+
+int mpf_destroy(mpf_ctx* c) {
+    if (!c) return 0;
+    (void)hipSetDevice(c->cfg.device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->base.release(); c->child_first.release(); c->elem.release(); c->child_mask.release();
+    c->f32.release(); c->f64.release(); c->cosmo.release(); c->counters.release();
+    c->io_in.release(); c->io_out.release(); c->io_i64.release(); c->io_u32.release();
+    mpf::split_release(c->split);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+const char* mpf_last_error(const mpf_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int mpf_set_precision(mpf_ctx* c, int precision) {
+    if (!c) return ERR_INVAL;
+    if (precision < 0 || precision > 2) return fail(c, ERR_INVAL, "precision must be 0, 1 or 2");
+    c->cfg.precision = precision;
+    return 0;
+}
+
+int mpf_load_octree_records(mpf_ctx* c, const void* rec48, uint64_t n) {
+    if (!c || (!rec48 && n)) return ERR_INVAL;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    mpf::HostTree t;
+    std::string err;
+    if (!mpf::build_tree(rec48, n, t, err)) return fail(c, ERR_INVAL, err);
+    std::vector<float> soa;
+    mpf::unpack_fields32(rec48, t, soa);
+    c->have_tree = false;
+    c->split_ready = false;
+    c->f64.release();
+    if (c->base.ensure(t.base.size()) != hipSuccess || c->child_first.ensure(n) != hipSuccess ||
+        c->elem.ensure(n) != hipSuccess || c->child_mask.ensure(n) != hipSuccess || c->f32.ensure(7 * n) != hipSuccess)
+        return fail(c, ERR_NOMEM, "device allocation of the octree failed");
+    HIPCHK(c, hipMemcpyAsync(c->base.p, t.base.data(), t.base.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->child_first.p, t.child_first.data(), n * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->elem.p, t.elem.data(), n * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->child_mask.p, t.child_mask.data(), n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->f32.p, soa.data(), soa.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    mpf::DevTree& d = c->dt;
+    d = mpf::DevTree();
+    d.B = t.min_level;
+    d.maxL = t.max_level;
+    d.bx0 = t.bx0; d.by0 = t.by0; d.bz0 = t.bz0;
+    d.bnx = t.bnx; d.bny = t.bny; d.bnz = t.bnz;
+    d.base = c->base.p;
+    d.child_first = c->child_first.p;
+    d.child_mask = c->child_mask.p;
+    d.elem = c->elem.p;
+    d.f32 = c->f32.p;
+    d.f64 = nullptr;
+    d.n = n;
+    c->ncells = n;
+    c->min_level = t.min_level;
+    c->max_level = t.max_level;
+    c->where.swap(t.where);
+    c->have_tree = true;
+    return 0;
+}
+
+int mpf_load_octree_fields64(mpf_ctx* c, const double* f7, uint64_t n) {
+    if (!c || !f7) return ERR_INVAL;
+    if (!c->have_tree) return fail(c, ERR_STATE, "load the octree records first");
+    if (n != c->ncells) return fail(c, ERR_INVAL, "fields64 count differs from the octree cell count");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    std::vector<double> soa(7 * n);
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint64_t g = c->where[i];
+        for (int k = 0; k < 7; ++k) soa[k * n + g] = f7[7 * i + k];
+    }
+    if (c->f64.ensure(7 * n) != hipSuccess) return fail(c, ERR_NOMEM, "device allocation of fields64 failed");
+    HIPCHK(c, hipMemcpyAsync(c->f64.p, soa.data(), soa.size() * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->dt.f64 = c->f64.p;
+    c->split_ready = false;
+    return 0;
+}
+
+int mpf_octree_info(const mpf_ctx* c, uint64_t* ncells, uint32_t* min_level, uint32_t* max_level) {
+    if (!c) return ERR_INVAL;
+    if (ncells) *ncells = c->have_tree ? c->ncells : 0;
+    if (min_level) *min_level = c->min_level;
+    if (max_level) *max_level = c->max_level;
+    return 0;
+}
+
+int mpf_validate_octree(const void* rec48, uint64_t n, uint32_t* min_level, uint32_t* max_level, char* msg,
+                        uint64_t msg_size) {
+    if (!rec48 && n) return ERR_INVAL;
+    mpf::HostTree t;
+    std::string err;
+    const bool ok = mpf::build_tree(rec48, n, t, err);
+    if (msg && msg_size) {
+        const std::string m = ok ? std::string("ok") : err;
+        const size_t k = std::min<size_t>(m.size(), msg_size - 1);
+        std::memcpy(msg, m.data(), k);
+        msg[k] = 0;
+    }
+    if (!ok) return ERR_INVAL;
+    if (min_level) *min_level = t.min_level;
+    if (max_level) *max_level = t.max_level;
+    return 0;
+}
+
+int mpf_set_cosmology(mpf_ctx* c, const double* t, const double* a, const double* dadt, uint64_t n) {
+    if (!c || !t || !a || !dadt) return ERR_INVAL;
+    if (n == 0 || n > (1u << 30)) return fail(c, ERR_INVAL, "cosmology table size");
+    for (uint64_t i = 1; i < n; ++i)
+        if (!(t[i - 1] <= t[i])) return fail(c, ERR_INVAL, "cosmology t column is not sorted");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (c->cosmo.ensure(3 * n) != hipSuccess) return fail(c, ERR_NOMEM, "cosmology allocation");
+    HIPCHK(c, hipMemcpyAsync(c->cosmo.p, t, n * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->cosmo.p + n, a, n * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->cosmo.p + 2 * n, dadt, n * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->ncos = (int)n;
+    return 0;
+}
+
+int mpf_locate(mpf_ctx* c, const double* xyz, uint64_t n, int64_t* elem_out, uint32_t* level_out) {
+    if (!c || (n && (!xyz || !elem_out || !level_out))) return ERR_INVAL;
+    if (!c->have_tree) return fail(c, ERR_STATE, "no octree loaded");
+    if (n == 0) return 0;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (c->io_in.ensure(3 * n) != hipSuccess || c->io_i64.ensure(n) != hipSuccess || c->io_u32.ensure(n) != hipSuccess)
+        return fail(c, ERR_NOMEM, "locate buffers");
+    HIPCHK(c, hipMemcpyAsync(c->io_in.p, xyz, 3 * n * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, mpf::launch_locate(c->dt, c->io_in.p, n, c->io_i64.p, c->io_u32.p, c->stream));
+    HIPCHK(c, hipMemcpyAsync(elem_out, c->io_i64.p, n * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(level_out, c->io_u32.p, n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int mpf_interpolate(mpf_ctx* c, int order, const double* xyz, uint64_t n, double* out7) {
+    if (!c || (n && (!xyz || !out7))) return ERR_INVAL;
+    if (order < 0 || order > 3) return fail(c, ERR_INVAL, "order must be 0..3");
+    if (!c->have_tree) return fail(c, ERR_STATE, "no octree loaded");
+    if (c->cfg.precision == MPF_PREC_SPLIT) return fail(c, ERR_INVAL, "interpolation is offered for the exact tiers");
+    if (n == 0) return 0;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (c->io_in.ensure(3 * n) != hipSuccess || c->io_out.ensure(7 * n) != hipSuccess)
+        return fail(c, ERR_NOMEM, "interpolation buffers");
+    HIPCHK(c, hipMemcpyAsync(c->io_in.p, xyz, 3 * n * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, mpf::launch_interp(c->cfg.precision, c->dt, order, c->io_in.p, n, c->io_out.p, c->stream));
+    HIPCHK(c, hipMemcpyAsync(out7, c->io_out.p, 7 * n * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int mpf_integrate_dev(mpf_ctx* c, const double* dev_in9, uint64_t n, const mpf_stop* stop, double* dev_out, int sync) {
+    if (!c || !stop || (n && (!dev_in9 || !dev_out))) return ERR_INVAL;
+    if (!c->have_tree) return fail(c, ERR_STATE, "no octree loaded");
+    if (c->ncos <= 0) return fail(c, ERR_STATE, "no cosmology table set");
+    if (stop->kind < -1 || stop->kind > 6) return fail(c, ERR_INVAL, "unknown stop criterion");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    mpf::IntegrateArgs A;
+    A.in9 = dev_in9;
+    A.out = dev_out;
+    A.n = n;
+    A.queue = c->counters.p;
+    A.steps = c->counters.p + 1;
+    A.ct = c->cosmo.p;
+    A.ca = c->cosmo.p + c->ncos;
+    A.cdadt = c->cosmo.p + 2 * c->ncos;
+    A.ncos = c->ncos;
+    A.order = c->cfg.order;
+    A.nsteps = c->cfg.nsteps;
+    A.length = c->cfg.length;
+    A.stop_kind = stop->kind;
+    A.stop_value = stop->value;
+    for (int k = 0; k < 3; ++k) A.kit[k] = stop->kitarget[k];
+    HIPCHK(c, mpf::launch_fill_u64(c->counters.p, 2, 0ull, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    if (n) {
+        if (c->cfg.precision == MPF_PREC_SPLIT) {
+            if (!c->split_ready) {
+                std::string err;
+                if (!mpf::split_build(c->split, c->dt, c->cfg.order, c->stream, err)) return fail(c, ERR_NOMEM, err);
+                c->split_ready = true;
+                HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+            }
+            HIPCHK(c, mpf::launch_integrate_split(A, c->dt, c->split, c->cus, c->stream));
+        } else {
+            const uint64_t blocks = (n + 255) / 256;
+            const int grid = (int)std::min<uint64_t>(blocks, (uint64_t)c->cus * 4);
+            HIPCHK(c, mpf::launch_integrate_exact(c->cfg.precision, A, c->dt, grid, c->stream));
+        }
+    }
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    c->pending = true;
+    if (sync) return mpf_synchronize(c);
+    return 0;
+}
+
+int mpf_synchronize(mpf_ctx* c) {
+    if (!c) return ERR_INVAL;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->pending) {
+        unsigned long long h[2] = {0, 0};
+        HIPCHK(c, hipMemcpy(h, c->counters.p, sizeof(h), hipMemcpyDeviceToHost));
+        c->last_steps = h[1];
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        c->last_ms = ms;
+        c->pending = false;
+    }
+    return 0;
+}
+
+int mpf_last_run(mpf_ctx* c, uint64_t* total_steps, double* kernel_ms) {
+    if (!c) return ERR_INVAL;
+    int rc = mpf_synchronize(c);
+    if (rc) return rc;
+    if (total_steps) *total_steps = c->last_steps;
+    if (kernel_ms) *kernel_ms = c->last_ms;
+    return 0;
+}
+
+int mpf_integrate(mpf_ctx* c, const double* in9, uint64_t n, const mpf_stop* stop, double* out,
+                  uint64_t* total_steps) {
+    if (!c || !stop || (n && (!in9 || !out))) return ERR_INVAL;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (c->io_in.ensure(9 * std::max<uint64_t>(n, 1)) != hipSuccess ||
+        c->io_out.ensure((uint64_t)MPF_NOUT * std::max<uint64_t>(n, 1)) != hipSuccess)
+        return fail(c, ERR_NOMEM, "integration buffers");
+    if (n) HIPCHK(c, hipMemcpyAsync(c->io_in.p, in9, 9 * n * 8, hipMemcpyHostToDevice, c->stream));
+    int rc = mpf_integrate_dev(c, c->io_in.p, n, stop, c->io_out.p, 1);
+    if (rc) return rc;
+    if (n) HIPCHK(c, hipMemcpy(out, c->io_out.p, (uint64_t)MPF_NOUT * n * 8, hipMemcpyDeviceToHost));
+    if (total_steps) *total_steps = c->last_steps;
+    return 0;
+}
+
+}  // extern "C"

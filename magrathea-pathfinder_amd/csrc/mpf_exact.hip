@@ -1,0 +1,613 @@
+// Exact-tier kernels: locate, NGP/CIC/TSC interpolation and the RK4 null
+// geodesic integrator, reproducing the reference's arithmetic operation by
+// operation (this file is compiled with -ffp-contract=off).
+//
+//   P = 0 (MPF_PREC_REF):  FP32 fields, FP32 accumulation -> bit-identical
+//                          to the reference (Gravity<float> accumulator).
+//   P = 1 (MPF_PREC_FP64): FP64 fields and accumulation, same operations.
+//
+// One photon per lane; lanes refill from a device work queue (one atomic per
+// wave) when their ray ends, so uneven path lengths do not idle a wave.
+#include "mpf_device.h"
+#include "mpf_kernels.h"
+
+namespace mpf {
+
+namespace {
+
+template <int P>
+__device__ __forceinline__ void load7(const DevTree& t, uint32_t g, double* d) {
+    if (P == 1 && t.f64) {
+#pragma unroll
+        for (int k = 0; k < 7; ++k) d[k] = t.f64[(uint64_t)k * t.n + g];
+    } else {
+#pragma unroll
+        for (int k = 0; k < 7; ++k) d[k] = (double)t.f32[(uint64_t)k * t.n + g];
+    }
+}
+
+// mac() (simplehyperoctree.h:1870-1910): accumulator += factor * coefficient
+template <int P>
+__device__ __forceinline__ void mac7(double* acc, const double* f, double w) {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        if (P == 0) acc[k] = (double)(float)(acc[k] + f[k] * w);
+        else acc[k] = acc[k] + f[k] * w;
+    }
+}
+
+__device__ __forceinline__ void zero7(double* d) {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) d[k] = 0.0;
+}
+
+__device__ __forceinline__ bool empty7(const double* d) {
+    bool e = true;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) e = e && (d[k] == 0.0);
+    return e;
+}
+
+// ngp() (simplehyperoctree.h:1396-1401); *loc = level of the located cell or -1
+template <int P>
+__device__ void ngp(const DevTree& t, double x, double y, double z, double* r, int* loc) {
+    unsigned fl = 0;
+    const uint32_t g = locate_point(t, x, y, z, &fl);
+    *loc = (g == NIL) ? -1 : (int)fl;
+    if (g == NIL) zero7(r);
+    else load7<P>(t, g, r);
+}
+
+// cic() (simplehyperoctree.h:1412-1469)
+template <int P>
+__device__ void cic(const DevTree& t, double x, double y, double z, double* result, int* loc) {
+    const double one = 1.0, two = 2.0;
+    const double point[3] = {x, y, z};
+    // elem key: cipher(point) at first, compute(lvl+1, point) after a level drop
+    unsigned ekl = 32;
+    uint32_t ek[3] = {axis_coord(x, 32), axis_coord(y, 32), axis_coord(z, 32)};
+    unsigned fl = 0;
+    uint32_t g0 = find_cell(t, 32, ek[0], ek[1], ek[2], &fl);
+    *loc = (g0 == NIL) ? -1 : (int)fl;
+    bool invalid = (g0 == NIL);
+    unsigned idl = invalid ? 32u : fl;  // level of the invalid index is 32
+    uint32_t idk[3] = {ek[0] >> (32 - idl), ek[1] >> (32 - idl), ek[2] >> (32 - idl)};
+    unsigned ilvl = idl, lvl = idl;
+    double half = EXTENT / 1.0 * (1.0 / (1.0 * (double)(1ull << idl))) / two;
+    unsigned site = 0;
+    double cell[7];
+    zero7(result);
+    do {
+        ilvl = lvl;
+        if (!invalid) {
+            double dist[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) dist[d] = fabs((cell_center(idl, idk[d]) - point[d]) / (half + half));
+            // child of idx that contains the elem key
+            {
+                const unsigned b = ekl - idl - 1;
+                site = ((ek[0] >> b) & 1u) | (((ek[1] >> b) & 1u) << 1) | (((ek[2] >> b) & 1u) << 2);
+            }
+            for (unsigned iv = 0; iv < 8; ++iv) {
+                double weight = one;
+                const unsigned diff = site ^ iv;
+                double vertex[3];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    const int bb = (iv >> d) & 1u, df = (diff >> d) & 1u;
+                    vertex[d] = point[d] + (bb * half) - ((!bb) * half);
+                    weight *= ((df) * (one - dist[d])) + ((!df) * dist[d]);
+                }
+                unsigned vl = 0;
+                const uint32_t gv = find_cell(t, ilvl, axis_coord(vertex[0], ilvl), axis_coord(vertex[1], ilvl),
+                                              axis_coord(vertex[2], ilvl), &vl);
+                if (gv != NIL) {
+                    lvl = lvl < vl ? lvl : vl;
+                    load7<P>(t, gv, cell);
+                    mac7<P>(result, cell, weight);
+                } else {
+                    zero7(result);
+                    break;
+                }
+            }
+        }
+        if (lvl < ilvl) {
+            // elem = compute(lvl+1, point); idx = parent(elem)
+            ekl = lvl + 1;
+            ek[0] = axis_coord(x, ekl); ek[1] = axis_coord(y, ekl); ek[2] = axis_coord(z, ekl);
+            idl = lvl;
+            idk[0] = ek[0] >> 1; idk[1] = ek[1] >> 1; idk[2] = ek[2] >> 1;
+            invalid = false;
+            zero7(result);
+            half = EXTENT / 1.0 * (1.0 / (1.0 * (double)(1ull << idl))) / two;
+        }
+    } while (lvl < ilvl);
+}
+
+// The 27-element cache of tsc(elemsTsc, ...) (simplehyperoctree.h:1554-1644):
+// the central key, the per-axis coordinates of the cached cells and their
+// cell indices.  `dflt` marks the default-constructed cache (27 root keys).
+struct TscCache {
+    unsigned lvl;
+    uint32_t cx[3], cy[3], cz[3];
+    uint32_t g[27];
+    bool dflt;
+};
+
+__device__ __forceinline__ void cache_init(TscCache& c) {
+    c.dflt = true;
+    c.lvl = 0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) { c.cx[i] = 0; c.cy[i] = 0; c.cz[i] = 0; }
+}
+
+__device__ __forceinline__ double tsc_axis_w(int ai, double d) {
+    return ai * 0.5 * (1.5 - d) * (1.5 - d) + (1 - ai) * (0.75 - d * d);
+}
+
+// tsc() with the cache (simplehyperoctree.h:1554-1644), or without it
+// (UseCache = false, simplehyperoctree.h:1481-1541).
+template <int P, bool UseCache>
+__device__ void tsc(const DevTree& t, TscCache& cache, double x, double y, double z, double* result, int* loc) {
+    const double two = 2.0;
+    const double point[3] = {x, y, z};
+    if (point[0] == 0.0 && point[1] == 0.0 && point[2] == 0.0) {
+        cic<P>(t, 0., 0., 0., result, loc);
+        return;
+    }
+    uint32_t ek[3] = {axis_coord(x, 32), axis_coord(y, 32), axis_coord(z, 32)};
+    unsigned fl = 0;
+    const uint32_t g0 = find_cell(t, 32, ek[0], ek[1], ek[2], &fl);
+    *loc = (g0 == NIL) ? -1 : (int)fl;
+    bool invalid = (g0 == NIL);
+    unsigned idl = invalid ? 32u : fl;
+    uint32_t idk[3] = {ek[0] >> (32 - idl), ek[1] >> (32 - idl), ek[2] >> (32 - idl)};
+    unsigned ilvl = idl, lvl = idl;
+    double half = EXTENT / 1.0 * (1.0 / (1.0 * (double)(1ull << idl))) / two;
+    double cell[7];
+    zero7(result);
+    do {
+        ilvl = lvl;
+        const double twohalves = two * half;
+        bool hit = false;
+        if (UseCache) {
+            if (cache.dflt) hit = !invalid && idl == 0;  // 27 default elements hold the root key
+            else hit = !invalid && cache.lvl == idl && cache.cx[1] == idk[0] && cache.cy[1] == idk[1] &&
+                       cache.cz[1] == idk[2];
+        }
+        if (hit) {
+            if (cache.dflt) {  // default elements carry Data(): the sum stays zero
+                zero7(result);
+                return;
+            }
+            double wx[3], wy[3], wz[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const int a = (i == 1) ? 0 : 1;
+                wx[i] = tsc_axis_w(a, fabs((cell_center(idl, cache.cx[i]) - point[0]) / (twohalves)));
+                wy[i] = tsc_axis_w(a, fabs((cell_center(idl, cache.cy[i]) - point[1]) / (twohalves)));
+                wz[i] = tsc_axis_w(a, fabs((cell_center(idl, cache.cz[i]) - point[2]) / (twohalves)));
+            }
+            for (int ic = 0; ic < 27; ++ic) {
+                const int ix = ic / 9, iy = (ic / 3) % 3, iz = ic % 3;
+                const double w = wx[ix] * wy[iy] * wz[iz];
+                load7<P>(t, cache.g[ic], cell);
+                mac7<P>(result, cell, w);
+            }
+            return;
+        }
+        if (!invalid) {
+            uint32_t nx[3], ny[3], nz[3];
+            double wx[3], wy[3], wz[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const int a = (i == 1) ? 0 : 1;
+                nx[i] = axis_coord(point[0] + (i - 1) * twohalves, ilvl);
+                ny[i] = axis_coord(point[1] + (i - 1) * twohalves, ilvl);
+                nz[i] = axis_coord(point[2] + (i - 1) * twohalves, ilvl);
+                wx[i] = tsc_axis_w(a, fabs((cell_center(ilvl, nx[i]) - point[0]) / (twohalves)));
+                wy[i] = tsc_axis_w(a, fabs((cell_center(ilvl, ny[i]) - point[1]) / (twohalves)));
+                wz[i] = tsc_axis_w(a, fabs((cell_center(ilvl, nz[i]) - point[2]) / (twohalves)));
+            }
+            uint32_t gl[27];
+            int ic = 0;
+            for (; ic < 27; ++ic) {
+                const int ix = ic / 9, iy = (ic / 3) % 3, iz = ic % 3;
+                unsigned vl = 0;
+                const uint32_t gv = find_cell(t, ilvl, nx[ix], ny[iy], nz[iz], &vl);
+                if (gv == NIL) {
+                    zero7(result);
+                    break;
+                }
+                lvl = lvl < vl ? lvl : vl;
+                if (lvl < ilvl) {
+                    zero7(result);
+                    break;
+                }
+                const double w = wx[ix] * wy[iy] * wz[iz];
+                load7<P>(t, gv, cell);
+                mac7<P>(result, cell, w);
+                if (UseCache) gl[ic] = gv;
+            }
+            if (UseCache && ic == 27) {
+                cache.dflt = false;
+                cache.lvl = ilvl;
+#pragma unroll
+                for (int i = 0; i < 3; ++i) { cache.cx[i] = nx[i]; cache.cy[i] = ny[i]; cache.cz[i] = nz[i]; }
+#pragma unroll
+                for (int i = 0; i < 27; ++i) cache.g[i] = gl[i];
+            }
+        }
+        if (lvl < ilvl) {
+            const unsigned el = lvl + 1;
+            idl = lvl;
+            idk[0] = axis_coord(x, el) >> 1;
+            idk[1] = axis_coord(y, el) >> 1;
+            idk[2] = axis_coord(z, el) >> 1;
+            invalid = false;
+            zero7(result);
+            half = EXTENT / 1.0 * (1.0 / (1.0 * (double)(1ull << idl))) / two;
+        }
+    } while (lvl < ilvl);
+}
+
+template <int P>
+__device__ __forceinline__ void interp(int order, const DevTree& t, TscCache& cache, double x, double y, double z,
+                                       double* d, int* loc) {
+    if (order == 0) ngp<P>(t, x, y, z, d, loc);
+    else if (order == 1) cic<P>(t, x, y, z, d, loc);
+    else tsc<P, true>(t, cache, x, y, z, d, loc);
+}
+
+// Utility::interpolate (utility.h:442-449) with a per-lane search hint
+__device__ __forceinline__ double cosmo_interp(double x0, const double* __restrict__ x, const double* __restrict__ y,
+                                               int n, int& hint) {
+    if (n <= 0) return 0.0;
+    if (n == 1) return y[0];
+    int i = hint;
+    while (i < n && !(x0 < x[i])) ++i;
+    while (i > 0 && x0 < x[i - 1]) --i;
+    hint = i;
+    i += (i <= 0) - (i >= n);
+    return y[i - 1] + ((y[i] - y[i - 1]) * (x0 - x[i - 1]) / (x[i] - x[i - 1]));
+}
+
+enum { CA = 0, CT, CX, CY, CZ, CKT, CKX, CKY, CKZ };
+
+// Integrator::dphotondl (integrator.h:409-445) given the interpolated fields
+__device__ __forceinline__ void rhs(const double* in, const double* data, double dadt, double scale, double* out) {
+    const double two = 2.0;
+    const double dphidl = data[F_DT] * in[CKT] + (data[F_DX] * in[CKX] + data[F_DY] * in[CKY] + data[F_DZ] * in[CKZ]);
+    out[CA] = in[CKT] * dadt;
+    out[CT] = in[CKT];
+    out[CX] = in[CKX] / scale;
+    out[CY] = in[CKY] / scale;
+    out[CZ] = in[CKZ] / scale;
+    out[CKT] = -(two * dadt / in[CA] * in[CKT] * in[CKT]) - (two / C2 * in[CKT]) * (dphidl - data[F_DT] * in[CKT]);
+    out[CKX] = -(two * dadt / in[CA] * in[CKT] * in[CKX]) + (two / C2 * dphidl * in[CKX]) - (two * data[F_DX] * in[CKT] * in[CKT]);
+    out[CKY] = -(two * dadt / in[CA] * in[CKT] * in[CKY]) + (two / C2 * dphidl * in[CKY]) - (two * data[F_DY] * in[CKT] * in[CKT]);
+    out[CKZ] = -(two * dadt / in[CA] * in[CKT] * in[CKZ]) + (two / C2 * dphidl * in[CKZ]) - (two * data[F_DZ] * in[CKT] * in[CKT]);
+}
+
+// one trajectory element (photon.h:44-47 core + the extras the output keeps)
+struct Rec {
+    double v[MPF_NOUT];
+};
+
+__device__ __forceinline__ double level_extent(int lvl) {
+    // extent<Type, Position, Extent>() of the located cell: E / (1 * 2^l)
+    const unsigned l = lvl < 0 ? 0u : (unsigned)lvl;
+    return EXTENT / (1.0 * (double)(1ull << l));
+}
+
+struct RayState {
+    Rec back, prev;
+    double data[7];   // fields at back's position
+    double dl, a0, phi0c2;
+    int hint;
+    uint32_t appended;
+    bool has_prev;
+};
+
+template <int P>
+__device__ void ray_launch(const IntegrateArgs& A, const DevTree& t, TscCache& cache, const double* in9, RayState& s) {
+    const double zero = 0.0, one = 1.0, two = 2.0;
+    const double c = C_LIGHT, c2 = C2;
+    const double scale = A.length;
+    double* b = s.back.v;
+#pragma unroll
+    for (int k = 0; k < MPF_NOUT; ++k) b[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) b[k] = in9[k];
+    cache_init(cache);
+    s.hint = 0;
+    int loc = -1;
+    if (A.order >= 0) interp<P>(A.order, t, cache, b[CX], b[CY], b[CZ], s.data, &loc);
+    else { zero7(s.data); s.data[F_A] = one; }
+    double* d = s.data;
+    const double norm = sqrt((c2 * (one + two * (d[F_PHI] / c2)) * b[CKT] * b[CKT]) /
+                             ((one - two * (d[F_PHI] / c2)) * (b[CKX] * b[CKX] + b[CKY] * b[CKY] + b[CKZ] * b[CKZ])));
+    b[CKX] *= norm;
+    b[CKY] *= norm;
+    b[CKZ] *= norm;
+    b[CA] = cosmo_interp(b[CT], A.ct, A.ca, A.ncos, s.hint);
+    unsigned fl = 0;
+    const uint32_t gl = locate_point(t, b[CX], b[CY], b[CZ], &fl);
+    const int lvl0 = (gl == NIL) ? -1 : (int)fl;
+    b[MPF_OUT_LEVEL] = (double)(lvl0 < 0 ? 0 : lvl0);
+    b[MPF_OUT_AH] = zero;
+    b[MPF_OUT_RHO] = d[F_RHO];
+    b[MPF_OUT_PHI] = d[F_PHI];
+    b[MPF_OUT_DPHIDX] = d[F_DX];
+    b[MPF_OUT_DPHIDY] = d[F_DY];
+    b[MPF_OUT_DPHIDZ] = d[F_DZ];
+    b[MPF_OUT_DPHIDT] = d[F_DT];
+    b[MPF_OUT_DPHIDL] = zero;
+    b[MPF_OUT_LAPLACIAN] = zero;
+    b[MPF_OUT_REDSHIFT] = zero;
+    const double phi = b[MPF_OUT_PHI];
+    b[MPF_OUT_DSDL2] = (b[CA] * b[CA]) * (-(c2 * (one + two * (phi / c2)) * b[CKT] * b[CKT]) +
+                                          ((one - two * (phi / c2)) * (b[CKX] * b[CKX] + b[CKY] * b[CKY] + b[CKZ] * b[CKZ])));
+    b[MPF_OUT_ERROR] = one - ((one - two * (phi / c2)) * (b[CKX] * b[CKX] + b[CKY] * b[CKY] + b[CKZ] * b[CKZ])) /
+                                 (c2 * (one + two * (phi / c2)) * b[CKT] * b[CKT]);
+    const double ratio = b[CA] * b[CA] * (scale / c) / A.nsteps;
+    s.dl = level_extent(lvl0) * ratio;
+    s.a0 = b[CA];
+    s.phi0c2 = phi / c2;
+    s.appended = 0;
+    s.has_prev = false;
+}
+
+// One iteration of the integration loop (integrator.h:683-812).  Returns
+// true while the ray continues.
+template <int P>
+__device__ bool ray_step(const IntegrateArgs& A, const DevTree& t, TscCache& cache, RayState& s) {
+    const double zero = 0.0, one = 1.0, two = 2.0, six = 6.0;
+    const double c = C_LIGHT, c2 = C2;
+    const double min = 0.0 - (EXTENT / two), max = 0.0 + (EXTENT / two);
+    const double scale = A.length;
+    const double* bk = s.back.v;
+    const double dl = s.dl;
+    double kin[9], ksum[9], kout[9], stage[7];
+    int loc = -1;
+    // k1 at back: the fields there are the ones interpolated after the
+    // previous step (the reference recomputes them through its cache, with
+    // the same result)
+    if (A.order >= 0) {
+#pragma unroll
+        for (int k = 0; k < 7; ++k) stage[k] = s.data[k];
+    } else zero7(stage);
+    double dadt = cosmo_interp(bk[CT], A.ct, A.cdadt, A.ncos, s.hint);
+    rhs(bk, stage, dadt, scale, kout);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) { ksum[i] = kout[i]; kin[i] = bk[i] + dl / two * kout[i]; }
+    // k2
+    if (A.order >= 0) interp<P>(A.order, t, cache, kin[CX], kin[CY], kin[CZ], stage, &loc); else zero7(stage);
+    dadt = cosmo_interp(kin[CT], A.ct, A.cdadt, A.ncos, s.hint);
+    rhs(kin, stage, dadt, scale, kout);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) { ksum[i] = ksum[i] + two * kout[i]; kin[i] = bk[i] + dl / two * kout[i]; }
+    // k3
+    if (A.order >= 0) interp<P>(A.order, t, cache, kin[CX], kin[CY], kin[CZ], stage, &loc); else zero7(stage);
+    dadt = cosmo_interp(kin[CT], A.ct, A.cdadt, A.ncos, s.hint);
+    rhs(kin, stage, dadt, scale, kout);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) { ksum[i] = ksum[i] + two * kout[i]; kin[i] = bk[i] + dl * kout[i]; }
+    // k4
+    if (A.order >= 0) interp<P>(A.order, t, cache, kin[CX], kin[CY], kin[CZ], stage, &loc); else zero7(stage);
+    dadt = cosmo_interp(kin[CT], A.ct, A.cdadt, A.ncos, s.hint);
+    rhs(kin, stage, dadt, scale, kout);
+    Rec ph;
+    double* p = ph.v;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) p[i] = bk[i] + (dl / six) * (ksum[i] + kout[i]);
+    // fields at the new position
+    double* data = s.data;
+    if (A.order >= 0) interp<P>(A.order, t, cache, p[CX], p[CY], p[CZ], data, &loc);
+    else {
+        zero7(data);
+        data[F_A] = one;
+        unsigned fl = 0;
+        const uint32_t g = locate_point(t, p[CX], p[CY], p[CZ], &fl);
+        loc = (g == NIL) ? -1 : (int)fl;
+    }
+    if (A.order == 0 || A.order == 1 || A.order == 2) {
+        // ngp/cic/tsc located the finest cell at the same point: that is
+        // locate()'s answer (the TSC call at the box centre goes through CIC,
+        // which locates the same point)
+    }
+    if (!(!(p[CA] < zero) && ((p[CX] > min) && (p[CX] < max) && (p[CY] > min) && (p[CY] < max) && (p[CZ] > min) &&
+                              (p[CZ] < max))))
+        zero7(data);
+    p[MPF_OUT_LEVEL] = (double)(loc < 0 ? 0 : loc);
+    p[MPF_OUT_AH] = data[F_A];
+    p[MPF_OUT_RHO] = data[F_RHO];
+    p[MPF_OUT_PHI] = data[F_PHI];
+    p[MPF_OUT_DPHIDX] = data[F_DX];
+    p[MPF_OUT_DPHIDY] = data[F_DY];
+    p[MPF_OUT_DPHIDZ] = data[F_DZ];
+    p[MPF_OUT_DPHIDT] = -data[F_DT];
+    p[MPF_OUT_DPHIDL] = data[F_DT] * p[CKT] + (p[MPF_OUT_DPHIDX] * p[CKX] + p[MPF_OUT_DPHIDY] * p[CKY] + p[MPF_OUT_DPHIDZ] * p[CKZ]);
+    p[MPF_OUT_LAPLACIAN] = (data[F_PHI] - bk[MPF_OUT_PHI]) / dl;
+    const double phi = p[MPF_OUT_PHI];
+    p[MPF_OUT_DSDL2] = (p[CA] * p[CA]) * ((-c2 * (one + two * (phi / c2)) * p[CKT] * p[CKT]) +
+                                          ((one - two * (phi / c2)) * (p[CKX] * p[CKX] + p[CKY] * p[CKY] + p[CKZ] * p[CKZ])));
+    p[MPF_OUT_ERROR] = one - ((one - two * (phi / c2)) * (p[CKX] * p[CKX] + p[CKY] * p[CKY] + p[CKZ] * p[CKZ])) /
+                                 (c2 * (one + two * (phi / c2)) * p[CKT] * p[CKT]);
+    p[MPF_OUT_ISW] = bk[MPF_OUT_ISW] + (bk[MPF_OUT_DPHIDT] + p[MPF_OUT_DPHIDT]) * (p[CT] - bk[CT]) / c2;
+    p[MPF_OUT_ISWOLD] = bk[MPF_OUT_ISWOLD] -
+                        (2 * (p[MPF_OUT_PHI] - bk[MPF_OUT_PHI]) / c2 -
+                         ((bk[MPF_OUT_DPHIDX] + p[MPF_OUT_DPHIDX]) * (p[CX] - bk[CX]) * scale +
+                          (bk[MPF_OUT_DPHIDY] + p[MPF_OUT_DPHIDY]) * (p[CY] - bk[CY]) * scale +
+                          (bk[MPF_OUT_DPHIDZ] + p[MPF_OUT_DPHIDZ]) * (p[CZ] - bk[CZ]) * scale) /
+                             c2);
+    p[MPF_OUT_CHI] = sqrt(p[CX] * p[CX] + p[CY] * p[CY] + p[CZ] * p[CZ]);
+    p[MPF_OUT_LAMBDA] = bk[MPF_OUT_LAMBDA] + dl;
+    p[MPF_OUT_S] = bk[MPF_OUT_S] + sqrt((p[CX] - bk[CX]) * (p[CX] - bk[CX]) + (p[CY] - bk[CY]) * (p[CY] - bk[CY]) +
+                                        (p[CZ] - bk[CZ]) * (p[CZ] - bk[CZ]));
+    p[MPF_OUT_REDSHIFT] = s.a0 / p[CA] * (one + s.phi0c2 - p[MPF_OUT_PHI] / c2) - one;
+    const bool cont = !empty7(data);
+    if (cont) {
+        const double ratio = p[CA] * p[CA] * (scale / c) / A.nsteps;
+        s.dl = level_extent(loc) * ratio;
+        p[MPF_OUT_NSTEPS] = (double)(s.appended + 1);
+        s.prev = s.back;
+        s.back = ph;
+        s.has_prev = true;
+        s.appended++;
+    }
+    if (A.stop_kind >= 0) {
+        const double* b = s.back.v;
+        bool brk;
+        switch (A.stop_kind) {
+            case 0: brk = b[MPF_OUT_CHI] > A.stop_value; break;
+            case 1: brk = b[MPF_OUT_LAMBDA] > A.stop_value; break;
+            case 2: brk = b[MPF_OUT_REDSHIFT] > A.stop_value; break;
+            case 3: brk = b[CT] > A.stop_value; break;
+            case 4: brk = b[CA] < A.stop_value; break;
+            case 5: brk = (p[CX] * A.kit[0] + p[CY] * A.kit[1] + p[CZ] * A.kit[2]) > A.stop_value; break;
+            case 6: brk = b[MPF_OUT_S] > A.stop_value; break;
+            default: brk = true; break;
+        }
+        if (brk) return false;
+    }
+    return cont;
+}
+
+__device__ __forceinline__ void ray_finish(const IntegrateArgs& A, RayState& s, uint64_t ray) {
+    // erase the last element if its redshift or a is negative (integrator.h:816-821)
+    const Rec* r = &s.back;
+    uint32_t steps = s.appended;
+    if (signbit(s.back.v[MPF_OUT_REDSHIFT]) || signbit(s.back.v[CA])) {
+        if (s.has_prev) {
+            r = &s.prev;
+            steps -= 1;
+        } else {
+            // the trajectory becomes empty
+            double* o = A.out + (uint64_t)MPF_NOUT * ray;
+            for (int k = 0; k < MPF_NOUT; ++k) o[k] = 0.0;
+            o[MPF_OUT_NSTEPS] = -1.0;
+            return;
+        }
+    }
+    double* o = A.out + (uint64_t)MPF_NOUT * ray;
+#pragma unroll
+    for (int k = 0; k < MPF_NOUT; ++k) o[k] = r->v[k];
+    o[MPF_OUT_NSTEPS] = (double)steps;
+    atomicAdd(A.steps, (unsigned long long)steps);
+}
+
+template <int P>
+__global__ void __launch_bounds__(256) integrate_exact_kernel(IntegrateArgs A, DevTree t) {
+    RayState s;
+    TscCache cache;
+    cache_init(cache);
+    bool active = false, drained = false;
+    uint64_t ray = 0;
+    const unsigned lane = __lane_id();
+    while (true) {
+        // refill idle lanes from the queue, one atomic per wave
+        const bool need = !active && !drained;
+        const unsigned long long m = __ballot(need);
+        if (m) {
+            const int leader = __ffsll((long long)m) - 1;
+            unsigned long long base = 0;
+            if ((int)lane == leader) base = atomicAdd(A.queue, (unsigned long long)__popcll(m));
+            base = __shfl(base, leader);
+            if (need) {
+                const uint64_t my = base + (uint64_t)__popcll(m & ((1ull << lane) - 1ull));
+                if (my < A.n) {
+                    ray = my;
+                    ray_launch<P>(A, t, cache, A.in9 + 9 * my, s);
+                    active = !empty7(s.data);
+                    if (!active) ray_finish(A, s, ray);
+                } else {
+                    drained = true;
+                }
+            }
+        }
+        if (!__any(active)) {
+            if (__all(drained)) break;
+            continue;
+        }
+        if (active) {
+            if (!ray_step<P>(A, t, cache, s)) {
+                active = false;
+                ray_finish(A, s, ray);
+            }
+        }
+    }
+}
+
+template <int P>
+__global__ void locate_kernel(DevTree t, const double* xyz, uint64_t n, int64_t* elem, uint32_t* level) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned fl = 0;
+    const uint32_t g = locate_point(t, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], &fl);
+    elem[i] = (g == NIL) ? -1 : (int64_t)t.elem[g];
+    level[i] = (g == NIL) ? 0u : fl;
+}
+
+template <int P>
+__global__ void interp_kernel(DevTree t, int order, const double* xyz, uint64_t n, double* out7) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    TscCache cache;
+    cache_init(cache);
+    double d[7];
+    int loc;
+    const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+    if (order == 0) ngp<P>(t, x, y, z, d, &loc);
+    else if (order == 1) cic<P>(t, x, y, z, d, &loc);
+    else tsc<P, false>(t, cache, x, y, z, d, &loc);
+#pragma unroll
+    for (int k = 0; k < 7; ++k) out7[7 * i + k] = d[k];
+}
+
+// cached TSC over the points in sequence: one lane walks all points
+template <int P>
+__global__ void interp_seq_kernel(DevTree t, const double* xyz, uint64_t n, double* out7) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    TscCache cache;
+    cache_init(cache);
+    double d[7];
+    int loc;
+    for (uint64_t i = 0; i < n; ++i) {
+        tsc<P, true>(t, cache, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], d, &loc);
+        for (int k = 0; k < 7; ++k) out7[7 * i + k] = d[k];
+    }
+}
+
+}  // namespace
+
+hipError_t launch_integrate_exact(int precision, const IntegrateArgs& A, const DevTree& t, int grid, hipStream_t st) {
+    if (precision == 0) hipLaunchKernelGGL(integrate_exact_kernel<0>, dim3(grid), dim3(256), 0, st, A, t);
+    else hipLaunchKernelGGL(integrate_exact_kernel<1>, dim3(grid), dim3(256), 0, st, A, t);
+    return hipGetLastError();
+}
+
+hipError_t launch_locate(const DevTree& t, const double* xyz, uint64_t n, int64_t* elem, uint32_t* level,
+                         hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const unsigned grid = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(locate_kernel<0>, dim3(grid), dim3(256), 0, st, t, xyz, n, elem, level);
+    return hipGetLastError();
+}
+
+hipError_t launch_interp(int precision, const DevTree& t, int order, const double* xyz, uint64_t n, double* out7,
+                         hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const unsigned grid = (unsigned)((n + 255) / 256);
+    if (order == 3) {
+        if (precision == 0) hipLaunchKernelGGL(interp_seq_kernel<0>, dim3(1), dim3(64), 0, st, t, xyz, n, out7);
+        else hipLaunchKernelGGL(interp_seq_kernel<1>, dim3(1), dim3(64), 0, st, t, xyz, n, out7);
+    } else if (precision == 0) {
+        hipLaunchKernelGGL(interp_kernel<0>, dim3(grid), dim3(256), 0, st, t, order, xyz, n, out7);
+    } else {
+        hipLaunchKernelGGL(interp_kernel<1>, dim3(grid), dim3(256), 0, st, t, order, xyz, n, out7);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace mpf

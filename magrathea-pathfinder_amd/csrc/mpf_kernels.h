@@ -1,0 +1,36 @@
+// Kernel launch interface shared by the C ABI and the kernel translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "../../include/mpf.h"
+#include "mpf_device.h"
+
+namespace mpf {
+
+struct IntegrateArgs {
+    const double* in9 = nullptr;     // n x 9 launch photons
+    double* out = nullptr;           // n x MPF_NOUT
+    uint64_t n = 0;
+    unsigned long long* queue = nullptr;  // work-queue counter (zeroed before launch)
+    unsigned long long* steps = nullptr;  // accepted-step counter
+    const double* ct = nullptr;      // cosmology table columns
+    const double* ca = nullptr;
+    const double* cdadt = nullptr;
+    int ncos = 0;
+    int order = 2;
+    unsigned nsteps = 10;
+    double length = 1.0;
+    int stop_kind = -1;
+    double stop_value = 0.0;
+    double kit[3] = {0.0, 0.0, 0.0};
+};
+
+hipError_t launch_integrate_exact(int precision, const IntegrateArgs& A, const DevTree& t, int grid, hipStream_t st);
+hipError_t launch_locate(const DevTree& t, const double* xyz, uint64_t n, int64_t* elem, uint32_t* level,
+                         hipStream_t st);
+hipError_t launch_interp(int precision, const DevTree& t, int order, const double* xyz, uint64_t n, double* out7,
+                         hipStream_t st);
+hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v, hipStream_t st);
+
+}  // namespace mpf

@@ -1,0 +1,204 @@
+"""ctypes binding of libmpf.so (include/mpf.h).
+
+The HIP library is the product path: this module fails loudly when it is
+missing, and there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "libmpf.so")
+
+NGP, CIC, TSC = 0, 1, 2
+PREC_REF, PREC_FP64, PREC_SPLIT = 0, 1, 2
+STOP_NONE, STOP_RADIUS, STOP_LAMBDA, STOP_REDSHIFT, STOP_T, STOP_A, STOP_PLANE, STOP_S = -1, 0, 1, 2, 3, 4, 5, 6
+STOP_NAMES = {"none": -1, "r": 0, "radius": 0, "lambda": 1, "redshift": 2, "t": 3, "eta": 3, "a": 4,
+              "plane": 5, "s": 6}
+
+OUT_FIELDS = ("a", "t", "x", "y", "z", "kt", "kx", "ky", "kz", "redshift", "lambda", "s", "chi", "isw",
+              "iswold", "phi", "dphidx", "dphidy", "dphidz", "dphidt", "rho", "ah", "dphidl", "laplacian",
+              "dsdl2", "error", "level", "nsteps")
+NOUT = len(OUT_FIELDS)
+OUT = {name: i for i, name in enumerate(OUT_FIELDS)}
+
+# every symbol declared in include/mpf.h
+EXPORTS = ("mpf_create", "mpf_destroy", "mpf_last_error", "mpf_set_precision", "mpf_load_octree_records",
+           "mpf_load_octree_fields64", "mpf_octree_info", "mpf_set_cosmology", "mpf_locate", "mpf_interpolate",
+           "mpf_integrate", "mpf_integrate_dev", "mpf_last_run", "mpf_synchronize", "mpf_validate_octree")
+
+
+class MpfError(RuntimeError):
+    pass
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("order", ctypes.c_int), ("precision", ctypes.c_int),
+                ("nsteps", ctypes.c_uint), ("length", ctypes.c_double)]
+
+
+class Stop(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("value", ctypes.c_double), ("kitarget", ctypes.c_double * 3)]
+
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libmpf.so; raises if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise MpfError(f"libmpf.so not found at {path}: build it first (__graft_entry__.build())")
+    lib = ctypes.CDLL(path)
+    vp, u64, i32, dbl = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_double
+    lib.mpf_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(Config)]
+    lib.mpf_destroy.argtypes = [vp]
+    lib.mpf_last_error.argtypes = [vp]
+    lib.mpf_last_error.restype = ctypes.c_char_p
+    lib.mpf_set_precision.argtypes = [vp, i32]
+    lib.mpf_load_octree_records.argtypes = [vp, vp, u64]
+    lib.mpf_load_octree_fields64.argtypes = [vp, vp, u64]
+    lib.mpf_octree_info.argtypes = [vp, vp, vp, vp]
+    lib.mpf_set_cosmology.argtypes = [vp, vp, vp, vp, u64]
+    lib.mpf_locate.argtypes = [vp, vp, u64, vp, vp]
+    lib.mpf_interpolate.argtypes = [vp, i32, vp, u64, vp]
+    lib.mpf_integrate.argtypes = [vp, vp, u64, ctypes.POINTER(Stop), vp, vp]
+    lib.mpf_integrate_dev.argtypes = [vp, vp, u64, ctypes.POINTER(Stop), vp, i32]
+    lib.mpf_last_run.argtypes = [vp, vp, vp]
+    lib.mpf_synchronize.argtypes = [vp]
+    lib.mpf_validate_octree.argtypes = [vp, u64, vp, vp, ctypes.c_char_p, u64]
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def validate_octree(rec):
+    """Host-side structural check of cone records (no GPU): (ok, min_level, max_level, message)."""
+    lib = load_library()
+    rec = np.ascontiguousarray(rec)
+    lo = ctypes.c_uint32(0)
+    hi = ctypes.c_uint32(0)
+    buf = ctypes.create_string_buffer(512)
+    rc = lib.mpf_validate_octree(_p(rec), ctypes.c_uint64(rec.size), ctypes.byref(lo), ctypes.byref(hi), buf, 512)
+    return rc == 0, lo.value, hi.value, buf.value.decode()
+
+
+def make_stop(kind="none", value=0.0, kitarget=(0.0, 0.0, 0.0)):
+    if isinstance(kind, str):
+        kind = STOP_NAMES[kind]
+    s = Stop()
+    s.kind = int(kind)
+    s.value = float(value)
+    for i in range(3):
+        s.kitarget[i] = float(kitarget[i])
+    return s
+
+
+class Context:
+    """One HIP device, one octree, one cosmology table (mpf_ctx)."""
+
+    def __init__(self, length, order=TSC, precision=PREC_REF, nsteps=10, device=0):
+        self.lib = load_library()
+        cfg = Config(device, order, precision, nsteps, float(length))
+        h = ctypes.c_void_p()
+        rc = self.lib.mpf_create(ctypes.byref(h), ctypes.byref(cfg))
+        if rc:
+            raise MpfError(f"mpf_create failed ({rc})")
+        self.h = h
+        self.order = order
+        self.precision = precision
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mpf_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc:
+            msg = self.lib.mpf_last_error(self.h).decode()
+            raise MpfError(f"{what} failed ({rc}): {msg}")
+
+    def set_precision(self, precision):
+        self._check(self.lib.mpf_set_precision(self.h, int(precision)), "mpf_set_precision")
+        self.precision = precision
+
+    def load_records(self, rec):
+        rec = np.ascontiguousarray(rec)
+        self._check(self.lib.mpf_load_octree_records(self.h, _p(rec), ctypes.c_uint64(rec.size)),
+                    "mpf_load_octree_records")
+
+    def load_fields64(self, f7):
+        f7 = np.ascontiguousarray(f7, dtype=np.float64)
+        self._check(self.lib.mpf_load_octree_fields64(self.h, _p(f7), ctypes.c_uint64(f7.shape[0])),
+                    "mpf_load_octree_fields64")
+
+    def octree_info(self):
+        n = ctypes.c_uint64(0)
+        lo = ctypes.c_uint32(0)
+        hi = ctypes.c_uint32(0)
+        self._check(self.lib.mpf_octree_info(self.h, ctypes.byref(n), ctypes.byref(lo), ctypes.byref(hi)),
+                    "mpf_octree_info")
+        return n.value, lo.value, hi.value
+
+    def set_cosmology(self, table):
+        t = np.ascontiguousarray(table[0], dtype=np.float64)
+        a = np.ascontiguousarray(table[1], dtype=np.float64)
+        d = np.ascontiguousarray(table[2], dtype=np.float64)
+        self._check(self.lib.mpf_set_cosmology(self.h, _p(t), _p(a), _p(d), ctypes.c_uint64(t.size)),
+                    "mpf_set_cosmology")
+
+    def locate(self, xyz):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        n = xyz.shape[0]
+        elem = np.zeros(n, dtype=np.int64)
+        lvl = np.zeros(n, dtype=np.uint32)
+        self._check(self.lib.mpf_locate(self.h, _p(xyz), ctypes.c_uint64(n), _p(elem), _p(lvl)), "mpf_locate")
+        return elem, lvl
+
+    def interpolate(self, order, xyz):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        n = xyz.shape[0]
+        out = np.zeros((n, 7))
+        self._check(self.lib.mpf_interpolate(self.h, int(order), _p(xyz), ctypes.c_uint64(n), _p(out)),
+                    "mpf_interpolate")
+        return out
+
+    def integrate(self, in9, stop="none", value=0.0, kitarget=(0.0, 0.0, 0.0)):
+        in9 = np.ascontiguousarray(in9, dtype=np.float64).reshape(-1, 9)
+        n = in9.shape[0]
+        out = np.zeros((n, NOUT))
+        steps = ctypes.c_uint64(0)
+        st = make_stop(stop, value, kitarget)
+        self._check(self.lib.mpf_integrate(self.h, _p(in9), ctypes.c_uint64(n), ctypes.byref(st), _p(out),
+                                           ctypes.byref(steps)), "mpf_integrate")
+        return out, steps.value
+
+    def integrate_dev(self, in_ptr, n, out_ptr, stop="none", value=0.0, kitarget=(0.0, 0.0, 0.0), sync=False):
+        """Device-resident integration: in_ptr/out_ptr are device addresses (ints)."""
+        st = make_stop(stop, value, kitarget)
+        self._check(self.lib.mpf_integrate_dev(self.h, ctypes.c_void_p(in_ptr), ctypes.c_uint64(n),
+                                               ctypes.byref(st), ctypes.c_void_p(out_ptr), int(bool(sync))),
+                    "mpf_integrate_dev")
+
+    def last_run(self):
+        steps = ctypes.c_uint64(0)
+        ms = ctypes.c_double(0.0)
+        self._check(self.lib.mpf_last_run(self.h, ctypes.byref(steps), ctypes.byref(ms)), "mpf_last_run")
+        return steps.value, ms.value
+
+    def synchronize(self):
+        self._check(self.lib.mpf_synchronize(self.h), "mpf_synchronize")

@@ -804,12 +804,15 @@ uint64_t mpfo_jacobian_infinitesimal(const mpfo_octree* o, int order, int precis
         tsc(o, precision, &cache, x, y + 0.5 * dl, z, &yp);
         tsc(o, precision, &cache, x, y, z - 0.5 * dl, &zm);
         tsc(o, precision, &cache, x, y, z + 0.5 * dl, &zp);
-        const double dxxp = (xp.v[F_DX] - xm.v[F_DX]) / dl;
-        const double dxyp = (yp.v[F_DX] - ym.v[F_DX]) / dl;
-        const double dxzp = (zp.v[F_DX] - zm.v[F_DX]) / dl;
-        const double dyyp = (yp.v[F_DY] - ym.v[F_DY]) / dl;
-        const double dyzp = (zp.v[F_DY] - zm.v[F_DY]) / dl;
-        const double dzzp = (zp.v[F_DZ] - zm.v[F_DZ]) / dl;
+        /* Gravity<float> members: the differences are float subtractions */
+#define FDIFF(a, b) (precision == 0 ? (double)((float)(a) - (float)(b)) : ((a) - (b)))
+        const double dxxp = FDIFF(xp.v[F_DX], xm.v[F_DX]) / dl;
+        const double dxyp = FDIFF(yp.v[F_DX], ym.v[F_DX]) / dl;
+        const double dxzp = FDIFF(zp.v[F_DX], zm.v[F_DX]) / dl;
+        const double dyyp = FDIFF(yp.v[F_DY], ym.v[F_DY]) / dl;
+        const double dyzp = FDIFF(zp.v[F_DY], zm.v[F_DY]) / dl;
+        const double dzzp = FDIFF(zp.v[F_DZ], zm.v[F_DZ]) / dl;
+#undef FDIFF
         const double d1d1p = sp * sp * dxxp + cp * cp * dyyp - s2p * dxyp;
         const double d2d2p = cp * cp * ct * ct * dxxp + sp * sp * ct * ct * dyyp + st * st * dzzp + s2p * ct * ct * dxyp -
                              sp * s2t * dyzp - cp * s2t * dxzp;

@@ -1,0 +1,200 @@
+"""ctypes bindings of the oracle libraries.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker or the timed CPU baseline.
+
+  Oracle      -- oracle/libmpforacle.so, the C restatement (always built)
+  RefHarness  -- oracle/_ref/libmpfref.so, the reference's own headers
+                 instantiated by oracle/ref_harness.cpp (built only where the
+                 reference sources exist; travels as a built library)
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_LIB = os.path.join(HERE, "libmpforacle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libmpfref.so")
+NREC = 28
+
+_vp = ctypes.c_void_p
+_u64 = ctypes.c_uint64
+
+
+def _p(a):
+    return _vp(a.ctypes.data) if a is not None else None
+
+
+class _Base:
+    prefix = ""
+
+    def __init__(self, path):
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        self.lib = ctypes.CDLL(path)
+        pf = self.prefix
+        L = self.lib
+        getattr(L, pf + "octree_new").restype = _vp
+        getattr(L, pf + "octree_new").argtypes = [_vp, _u64]
+        getattr(L, pf + "octree_free").argtypes = [_vp]
+        getattr(L, pf + "integrate").restype = _u64
+        getattr(L, pf + "trajectory").restype = _u64
+        getattr(L, pf + "jacobian_infinitesimal").restype = _u64
+
+    def tree(self, rec):
+        rec = np.ascontiguousarray(rec)
+        h = getattr(self.lib, self.prefix + "octree_new")(_p(rec), _u64(rec.size))
+        return h
+
+    def free(self, h):
+        getattr(self.lib, self.prefix + "octree_free")(_vp(h))
+
+    def compute_key(self, level, xyz):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros((xyz.shape[0], 2), dtype=np.uint64)
+        getattr(self.lib, self.prefix + "compute_key")(ctypes.c_uint(level), _p(xyz), _u64(xyz.shape[0]), _p(out))
+        return out
+
+    def key_info(self, lohi):
+        lohi = np.ascontiguousarray(lohi, dtype=np.uint64).reshape(-1, 2)
+        m = lohi.shape[0]
+        lvl = np.zeros(m, dtype=np.uint32)
+        cen = np.zeros((m, 3))
+        ext = np.zeros(m)
+        par = np.zeros((m, 2), dtype=np.uint64)
+        getattr(self.lib, self.prefix + "key_info")(_p(lohi), _u64(m), _p(lvl), _p(cen), _p(ext), _p(par))
+        return lvl, cen, ext, par
+
+    def locate(self, h, xyz):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        n = xyz.shape[0]
+        e = np.zeros(n, dtype=np.int64)
+        lv = np.zeros(n, dtype=np.uint32)
+        getattr(self.lib, self.prefix + "locate")(_vp(h), _p(xyz), _u64(n), _p(e), _p(lv))
+        return e, lv
+
+
+class Oracle(_Base):
+    """The C restatement (oracle/mpf_oracle.c)."""
+    prefix = "mpfo_"
+
+    def __init__(self, path=ORACLE_LIB):
+        super().__init__(path)
+        self.lib.mpfo_octree_set_fields64.argtypes = [_vp, _vp, _u64]
+
+    def tree(self, rec, fields64=None):
+        h = super().tree(rec)
+        if fields64 is not None:
+            f = np.ascontiguousarray(fields64, dtype=np.float64)
+            self.lib.mpfo_octree_set_fields64(_vp(h), _p(f), _u64(f.shape[0]))
+        return h
+
+    def interp(self, h, order, xyz, precision=0):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros((xyz.shape[0], 7))
+        self.lib.mpfo_interp(_vp(h), int(order), int(precision), _p(xyz), _u64(xyz.shape[0]), _p(out))
+        return out
+
+    def integrate(self, h, order, table, length, nsteps, stop_kind, stop_value, in9, precision=0, nthreads=1,
+                  kitarget=None):
+        tab = np.ascontiguousarray(table, dtype=np.float64)
+        in9 = np.ascontiguousarray(in9, dtype=np.float64).reshape(-1, 9)
+        out = np.zeros((in9.shape[0], NREC))
+        kit = None if kitarget is None else np.ascontiguousarray(kitarget, dtype=np.float64)
+        steps = self.lib.mpfo_integrate(_vp(h), int(order), int(precision), _p(tab), _u64(tab.shape[1]),
+                                        ctypes.c_double(length), ctypes.c_uint(nsteps), int(stop_kind),
+                                        ctypes.c_double(stop_value), _p(kit), _p(in9), _u64(in9.shape[0]), _p(out),
+                                        int(nthreads))
+        return out, steps
+
+    def trajectory(self, h, order, table, length, nsteps, stop_kind, stop_value, in9, precision=0, maxlen=100000,
+                   kitarget=None):
+        tab = np.ascontiguousarray(table, dtype=np.float64)
+        in9 = np.ascontiguousarray(in9, dtype=np.float64).reshape(9)
+        rows = np.zeros((maxlen, NREC))
+        kit = None if kitarget is None else np.ascontiguousarray(kitarget, dtype=np.float64)
+        n = self.lib.mpfo_trajectory(_vp(h), int(order), int(precision), _p(tab), _u64(tab.shape[1]),
+                                     ctypes.c_double(length), ctypes.c_uint(nsteps), int(stop_kind),
+                                     ctypes.c_double(stop_value), _p(kit), _p(in9), _p(rows), _u64(maxlen))
+        return rows[:min(n, maxlen)]
+
+    def jacobian(self, h, order, table, length, nsteps, stop_kind, stop_value, in9, dist, precision=0,
+                 kitarget=None):
+        tab = np.ascontiguousarray(table, dtype=np.float64)
+        in9 = np.ascontiguousarray(in9, dtype=np.float64).reshape(9)
+        dist = np.ascontiguousarray(dist, dtype=np.float64).ravel()
+        out = np.zeros((dist.size, 4))
+        kit = None if kitarget is None else np.ascontiguousarray(kitarget, dtype=np.float64)
+        self.lib.mpfo_jacobian_infinitesimal(_vp(h), int(order), int(precision), _p(tab), _u64(tab.shape[1]),
+                                             ctypes.c_double(length), ctypes.c_uint(nsteps), int(stop_kind),
+                                             ctypes.c_double(stop_value), _p(kit), _p(in9), _p(dist),
+                                             _u64(dist.size), _p(out))
+        return out
+
+    def interpolate(self, x, y, x0):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        x0 = np.ascontiguousarray(x0, dtype=np.float64)
+        out = np.zeros(x0.size)
+        self.lib.mpfo_interpolate(_p(x), _p(y), _u64(x.size), _p(x0), _u64(x0.size), _p(out))
+        return out
+
+
+class RefHarness(_Base):
+    """The reference's own headers, compiled (oracle/_ref)."""
+    prefix = "ref_"
+
+    def __init__(self, path=REF_LIB):
+        super().__init__(path)
+
+    def interp(self, h, order, xyz):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros((xyz.shape[0], 7), dtype=np.float32)
+        self.lib.ref_interp(_vp(h), int(order), _p(xyz), _u64(xyz.shape[0]), _p(out))
+        return out.astype(np.float64)
+
+    def integrate(self, h, order, table, length, nsteps, stop_kind, stop_value, in9, nthreads=1, kitarget=None):
+        tab = np.ascontiguousarray(table, dtype=np.float64)
+        in9 = np.ascontiguousarray(in9, dtype=np.float64).reshape(-1, 9)
+        out = np.zeros((in9.shape[0], NREC))
+        kit = None if kitarget is None else np.ascontiguousarray(kitarget, dtype=np.float64)
+        steps = self.lib.ref_integrate(_vp(h), int(order), _p(tab), _u64(tab.shape[1]), ctypes.c_double(length),
+                                       ctypes.c_uint(nsteps), int(stop_kind), ctypes.c_double(stop_value), _p(kit),
+                                       _p(in9), _u64(in9.shape[0]), _p(out), int(nthreads))
+        return out, steps
+
+    def trajectory(self, h, order, table, length, nsteps, stop_kind, stop_value, in9, maxlen=100000, kitarget=None):
+        tab = np.ascontiguousarray(table, dtype=np.float64)
+        in9 = np.ascontiguousarray(in9, dtype=np.float64).reshape(9)
+        rows = np.zeros((maxlen, NREC))
+        kit = None if kitarget is None else np.ascontiguousarray(kitarget, dtype=np.float64)
+        n = self.lib.ref_trajectory(_vp(h), int(order), _p(tab), _u64(tab.shape[1]), ctypes.c_double(length),
+                                    ctypes.c_uint(nsteps), int(stop_kind), ctypes.c_double(stop_value), _p(kit),
+                                    _p(in9), _p(rows), _u64(maxlen))
+        return rows[:min(n, maxlen)]
+
+    def jacobian(self, h, order, table, length, nsteps, stop_kind, stop_value, in9, dist, kitarget=None):
+        tab = np.ascontiguousarray(table, dtype=np.float64)
+        in9 = np.ascontiguousarray(in9, dtype=np.float64).reshape(9)
+        dist = np.ascontiguousarray(dist, dtype=np.float64).ravel()
+        out = np.zeros((dist.size, 4))
+        kit = None if kitarget is None else np.ascontiguousarray(kitarget, dtype=np.float64)
+        self.lib.ref_jacobian_infinitesimal(_vp(h), int(order), _p(tab), _u64(tab.shape[1]), ctypes.c_double(length),
+                                            ctypes.c_uint(nsteps), int(stop_kind), ctypes.c_double(stop_value),
+                                            _p(kit), _p(in9), _p(dist), _u64(dist.size), _p(out))
+        return out
+
+    def interpolate(self, x, y, x0):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        x0 = np.ascontiguousarray(x0, dtype=np.float64)
+        out = np.zeros(x0.size)
+        self.lib.ref_interpolate(_p(x), _p(y), _u64(x.size), _p(x0), _u64(x0.size), _p(out))
+        return out
+
+
+def ref_available():
+    return os.path.exists(REF_LIB)

@@ -1,0 +1,96 @@
+"""Deterministic test inputs shared by the golden-fixture writer and the tests.
+
+Trees are small AMR octrees with analytic fields; rays leave the observer at
+the box centre into a cone around (phi, theta) = (20 deg, 20 deg), crossing a
+refined patch (level jumps, TSC level drops), a hole (missing neighbours) and
+the edge of the stored region.
+"""
+from __future__ import annotations
+
+import hashlib
+import math
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "magrathea-pathfinder_amd"))
+
+from mpf import formats, synth  # noqa: E402
+
+LENGTH = 3000.0 * synth.MPC_SI / 0.7
+NSTEPS = 10
+DIR = np.array([math.cos(math.radians(20)) * math.sin(math.radians(20)),
+                math.sin(math.radians(20)) * math.sin(math.radians(20)),
+                math.cos(math.radians(20))])
+
+
+def cosmology():
+    return synth.flat_lcdm_table(h=0.7, omega_m=0.3, a_min=0.2, n=512)
+
+
+def _patch(center, half):
+    c = np.asarray(center)
+
+    def f(level, x, y, z):
+        return (np.abs(x - c[0]) < half) & (np.abs(y - c[1]) < half) & (np.abs(z - c[2]) < half)
+    return f
+
+
+def tree(name):
+    """(records, fields64) of a named test tree."""
+    if name == "uniform":
+        return synth.analytic_tree(base_level=6, half=0.5, length=LENGTH, seed=1)
+    if name == "amr":
+        inner = _patch(0.22 * DIR, 0.09)
+        deep = _patch(0.2 * DIR + np.array([0.02, 0.0, 0.0]), 0.04)
+
+        def refine(level, x, y, z):
+            m = inner(level, x, y, z)
+            if level >= 7:
+                m &= deep(level, x, y, z)
+            return m
+        other = np.array([0.0, 1.0, 0.0]) - DIR[1] * DIR
+        other /= np.linalg.norm(other)
+        hole_c = 0.38 * DIR + 0.05 * other
+        hole = _patch(hole_c, 0.05)
+        return synth.analytic_tree(base_level=6, half=0.5, max_level=9, refine=refine, length=LENGTH, seed=2,
+                                   holes=hole)
+    if name == "coarse":
+        return synth.analytic_tree(base_level=5, half=0.5, max_level=7, refine=_patch(0.15 * DIR, 0.12),
+                                   length=LENGTH, seed=3)
+    raise KeyError(name)
+
+
+TREES = ("uniform", "amr", "coarse")
+
+
+def records_digest(rec):
+    return hashlib.sha256(np.ascontiguousarray(rec).tobytes()).hexdigest()
+
+
+RAYSETS = {"narrow": dict(n_side=8, aperture=6.0), "wide": dict(n_side=8, aperture=40.0)}
+
+
+def rays(n_side=8, aperture=6.0, phi0=20.0, theta0=20.0):
+    ph, th = synth.cone_grid(phi0, theta0, aperture, n_side)
+    return synth.launch_photons(ph, th)
+
+
+def points(rec, n=3000, seed=0):
+    """Query points: uniform in and around the stored region, on cell faces,
+    near the refined patch and exactly at the box centre."""
+    rng = np.random.default_rng(seed)
+    p = rng.uniform(-0.56, 0.56, (n, 3))
+    k = n // 5
+    p[:k] = np.round(p[:k] * 64) / 64               # on level-8 faces/corners
+    p[k:2 * k] = 0.22 * DIR + rng.uniform(-0.1, 0.1, (k, 3))
+    p[2 * k:2 * k + 10] = 0.0
+    p[2 * k + 10:2 * k + 20] = rng.uniform(-1e-12, 1e-12, (10, 3))
+    return p
+
+
+STOPS = (("redshift", 0.3), ("radius", 0.42), ("none", 0.0), ("lambda", 5e16), ("t", 1.2e17),
+         ("a", 0.8), ("plane", 0.3), ("s", 0.33))
+STOP_KIND = {"none": -1, "radius": 0, "lambda": 1, "redshift": 2, "t": 3, "a": 4, "plane": 5, "s": 6}
