@@ -1,0 +1,180 @@
+"""Benchmark: ray-RK4-steps/s of the photon-propagation path on one MI355X.
+
+Workload (config 1 of the benchmark set): a level-8 base grid (256^3) refined
+to level 11 wherever the lognormal overdensity exceeds 8, FP64 fields; 1024 x
+1024 rays on a regular grid over a 10 x 10 degree cone, TSC interpolation,
+integrated to redshift 2.  One bench step = one pass of every ray to its stop
+(a launch of the split-precision integrator over the whole batch), inputs
+resident in HBM.  The plain FP64 tier runs the same rays for the accuracy
+check (relative end-redshift error <= 1e-7) and its own throughput; the
+reference's CPU path (or the oracle when the reference is absent) is timed on
+a bounded subset of the rays.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload cfg1]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "magrathea-pathfinder_amd"))
+
+HBM_PEAK = 8.0e12       # MI355X HBM3E, bytes/s (AMD public spec)
+B_STEP_SPLIT_TSC = 540  # 27 cells x 5 fields x 4 bytes per RK4 step
+B_STEP_FP64_TSC = 1080  # 27 cells x 5 fields x 8 bytes
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def build_workload(name):
+    from mpf import synth
+    w = dict(synth.WORKLOADS[name])
+    t0 = time.time()
+    table = synth.flat_lcdm_table(h=0.7, omega_m=0.3, a_min=0.05, n=4096)
+    rec, f64, length = synth.lightcone_tree(w["base_input_level"], w["box_mpch"], w["sigma"], w["threshold"],
+                                            w["max_input_level"], w["seed"], noise=w.get("noise", 0.0),
+                                            decay=w.get("decay", 0.0), table=table, device="cpu")
+    ph, th = synth.cone_grid(w["phi0"], w["theta0"], w["aperture"], w["nside"])
+    in9 = synth.launch_photons(ph, th)
+    log(f"workload {name}: {rec.size} cells, {in9.shape[0]} rays, built in {time.time() - t0:.1f}s")
+    return w, rec, f64, length, table, in9
+
+
+def cpu_baseline(rec, table, length, in9, w, nthreads, target_s=15.0):
+    """Reference CPU path on a bounded subset of the rays (the reference's own
+    headers in oracle/_ref when present, else the oracle restatement)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    kind = "reference" if orc.ref_available() else "port"
+    lib = orc.RefHarness() if kind == "reference" else orc.Oracle()
+    h = lib.tree(rec)
+    order = w["order"]
+    # calibrate on a small sample, then run a sample sized to ~target_s
+    stride = max(1, in9.shape[0] // 256)
+    sample = in9[::stride][:256]
+    t0 = time.time()
+    kw = dict(nthreads=nthreads)
+    _, steps = lib.integrate(h, order, table, length, 10, 2, w["z_stop"], sample, **kw)
+    dt = time.time() - t0
+    rate = steps / max(dt, 1e-9)
+    per_ray = steps / sample.shape[0]
+    nray = int(min(in9.shape[0], max(256, target_s * rate / max(per_ray, 1.0))))
+    stride = max(1, in9.shape[0] // nray)
+    sample = in9[::stride][:nray]
+    t0 = time.time()
+    _, steps = lib.integrate(h, order, table, length, 10, 2, w["z_stop"], sample, **kw)
+    dt = time.time() - t0
+    lib.free(h)
+    return {"value": steps / dt, "unit": "ray-RK4-steps/s", "cores": nthreads, "kind": kind,
+            "sample": f"{sample.shape[0]} of the {in9.shape[0]} rays (every {stride}th), {steps} RK4 steps, "
+                      f"{dt:.1f}s, {nthreads} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="cfg1")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-tier comparison")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards by rays or "
+              f"cones with no exchange until the map reduction; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    from mpf import capi
+    # libmpf's HIP runtime (ROCm) is loaded before torch, which the workload
+    # generator uses on the CPU only
+    capi.load_library()
+
+    w, rec, f64, length, table, in9 = build_workload(args.workload)
+    order = w["order"]
+    ctx = capi.Context(length, order=order, precision=capi.PREC_SPLIT if order == 2 else capi.PREC_FP64,
+                       nsteps=10, device=0)
+    t0 = time.time()
+    ctx.load_records(rec)
+    ctx.load_fields64(f64)
+    ctx.set_cosmology(table)
+    ncells, lmin, lmax = ctx.octree_info()
+    log(f"octree loaded in {time.time() - t0:.1f}s: {ncells} cells, levels {lmin}..{lmax}")
+    # inputs and outputs resident in HBM (libmpf device buffers)
+    n = in9.shape[0]
+    d_in = ctx.dev_alloc(in9.nbytes)
+    ctx.upload(d_in, in9)
+    d_out = ctx.dev_alloc(n * capi.NOUT * 8)
+    stop = ("redshift", w["z_stop"])
+    for i in range(args.warmup):
+        t0 = time.time()
+        ctx.integrate_dev(d_in, n, d_out, *stop, sync=True)
+        steps, ms = ctx.last_run()
+        log(f"warmup {i}: {steps} steps, kernel {ms:.1f} ms, wall {time.time() - t0:.2f}s")
+    ctx.synchronize()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        ctx.integrate_dev(d_in, n, d_out, *stop, sync=False)
+        steps, ms = ctx.last_run()   # synchronizes the context stream
+        kernel_ms.append(ms)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    total_steps = steps * args.steps
+    value = total_steps / wall
+    kms = float(np.mean(kernel_ms))
+    kernel_rate = steps / (kms / 1e3)
+    log(f"split: {steps} steps per pass, {value / 1e9:.3f} G steps/s wall, {kernel_rate / 1e9:.3f} G steps/s kernel")
+    out_split = ctx.download(d_out, (n, capi.NOUT))
+    result = {
+        "metric": "ray-RK4-steps/s (split FP32-perturbation/FP64-background, TSC)",
+        "value": value, "unit": "ray-RK4-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32-perturbation/fp64-background", "data": "synthetic (seeded GRF lightcone octree)",
+        "config": {"workload": f"{args.workload}: level-8 base (256^3) refined to level 11 where overdensity > 8, "
+                               f"1024x1024 rays over 10x10 deg, TSC, to z=2",
+                   "cells": int(ncells), "levels": [int(lmin), int(lmax)], "rays": int(n),
+                   "rk4_steps_per_pass": int(steps)},
+    }
+    bstep = B_STEP_SPLIT_TSC if order == 2 else 320 // 2
+    achieved = kernel_rate * bstep
+    result["roofline"] = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                          "frac": achieved / HBM_PEAK, "traffic": None,
+                          "bytes_per_step": bstep, "kernel_ms": kms}
+    if not args.no_fp64:
+        ctx.set_precision(capi.PREC_FP64)
+        t0 = time.time()
+        ctx.integrate_dev(d_in, n, d_out, *stop, sync=True)
+        s64, ms64 = ctx.last_run()
+        out64 = ctx.download(d_out, (n, capi.NOUT))
+        log(f"fp64 tier: {s64} steps, kernel {ms64:.1f} ms ({s64 / (ms64 / 1e3) / 1e9:.3f} G steps/s)")
+        ok = (out64[:, capi.OUT["nsteps"]] >= 0) & (out_split[:, capi.OUT["nsteps"]] >= 0)
+        z64 = out64[ok, capi.OUT["redshift"]]
+        zs = out_split[ok, capi.OUT["redshift"]]
+        rel_z = np.abs(zs - z64) / np.abs(z64)
+        dpos = np.linalg.norm(out_split[ok, 2:5] - out64[ok, 2:5], axis=1) / np.linalg.norm(out64[ok, 2:5], axis=1)
+        result["fp64_tier"] = {"value": s64 / (ms64 / 1e3), "unit": "ray-RK4-steps/s", "kernel_ms": ms64,
+                               "steps": int(s64), "speedup_split_over_fp64": kernel_rate / (s64 / (ms64 / 1e3)),
+                               "rel_end_redshift_err_max": float(rel_z.max()),
+                               "rel_end_redshift_err_median": float(np.median(rel_z)),
+                               "rel_end_position_err_max": float(dpos.max()),
+                               "step_count_mismatch_rays": int(np.sum(out64[:, capi.OUT["nsteps"]] !=
+                                                                      out_split[:, capi.OUT["nsteps"]]))}
+        ctx.set_precision(capi.PREC_SPLIT)
+    if not args.no_cpu:
+        nth = min(16, os.cpu_count() or 1)
+        t0 = time.time()
+        result["cpu_baseline"] = cpu_baseline(rec, table, length, in9, w, nth)
+        log(f"cpu baseline: {result['cpu_baseline']} ({time.time() - t0:.1f}s)")
+    ctx.dev_free(d_in)
+    ctx.dev_free(d_out)
+    ctx.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

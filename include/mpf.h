@@ -120,6 +120,13 @@ int mpf_integrate(mpf_ctx* ctx, const double* in9, uint64_t n, const mpf_stop* s
  * context stream unless sync != 0. */
 int mpf_integrate_dev(mpf_ctx* ctx, const double* dev_in9, uint64_t n, const mpf_stop* stop, double* dev_out,
                       int sync);
+/* Device buffers on the context's device, for callers that keep inputs and
+ * outputs resident between calls (mpf_integrate_dev).  Copies are
+ * synchronous. */
+int mpf_dev_alloc(mpf_ctx* ctx, uint64_t bytes, void** dev_ptr);
+int mpf_dev_free(mpf_ctx* ctx, void* dev_ptr);
+int mpf_dev_upload(mpf_ctx* ctx, void* dev_dst, const void* host_src, uint64_t bytes);
+int mpf_dev_download(mpf_ctx* ctx, void* host_dst, const void* dev_src, uint64_t bytes);
 /* Accepted steps and duration (HIP events on the context stream) of the
  * last integration kernel. */
 int mpf_last_run(mpf_ctx* ctx, uint64_t* total_steps, double* kernel_ms);

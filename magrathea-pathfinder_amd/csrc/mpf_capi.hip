@@ -63,6 +63,8 @@ struct mpf_ctx {
     uint64_t ncells = 0;
     uint32_t min_level = 0, max_level = 0;
     std::vector<uint32_t> where;  // record -> level-sorted index
+    std::vector<uint8_t> lev;     // level-sorted cell levels and coordinates
+    std::vector<uint32_t> kx, ky, kz;
     DevBuf<uint32_t> base, child_first, elem;
     DevBuf<uint8_t> child_mask;
     DevBuf<float> f32;
@@ -75,7 +77,7 @@ struct mpf_ctx {
     int ncos = 0;
     // run state
     DevBuf<unsigned long long> counters;  // [0] queue, [1] steps
-    DevBuf<double> io_in, io_out;
+    DevBuf<double> io_in, io_out, scratch;
     DevBuf<int64_t> io_i64;
     DevBuf<uint32_t> io_u32;
     uint64_t last_steps = 0;
@@ -133,7 +135,7 @@ int mpf_destroy(mpf_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->base.release(); c->child_first.release(); c->elem.release(); c->child_mask.release();
     c->f32.release(); c->f64.release(); c->cosmo.release(); c->counters.release();
-    c->io_in.release(); c->io_out.release(); c->io_i64.release(); c->io_u32.release();
+    c->io_in.release(); c->io_out.release(); c->scratch.release(); c->io_i64.release(); c->io_u32.release();
     mpf::split_release(c->split);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -188,6 +190,10 @@ int mpf_load_octree_records(mpf_ctx* c, const void* rec48, uint64_t n) {
     c->min_level = t.min_level;
     c->max_level = t.max_level;
     c->where.swap(t.where);
+    c->lev.swap(t.level);
+    c->kx.swap(t.kx);
+    c->ky.swap(t.ky);
+    c->kz.swap(t.kz);
     c->have_tree = true;
     return 0;
 }
@@ -310,10 +316,14 @@ int mpf_integrate_dev(mpf_ctx* c, const double* dev_in9, uint64_t n, const mpf_s
         if (c->cfg.precision == MPF_PREC_SPLIT) {
             if (!c->split_ready) {
                 std::string err;
-                if (!mpf::split_build(c->split, c->dt, c->cfg.order, c->stream, err)) return fail(c, ERR_NOMEM, err);
+                if (!mpf::split_build(c->split, c->dt, c->lev.data(), c->kx.data(), c->ky.data(), c->kz.data(),
+                                      c->cfg.order, c->stream, err))
+                    return fail(c, ERR_NOMEM, err);
                 c->split_ready = true;
                 HIPCHK(c, hipEventRecord(c->ev0, c->stream));
             }
+            if (c->scratch.ensure(16 * n) != hipSuccess) return fail(c, ERR_NOMEM, "launch-record buffer");
+            A.scratch = c->scratch.p;
             HIPCHK(c, mpf::launch_integrate_split(A, c->dt, c->split, c->cus, c->stream));
         } else {
             const uint64_t blocks = (n + 255) / 256;
@@ -324,6 +334,44 @@ int mpf_integrate_dev(mpf_ctx* c, const double* dev_in9, uint64_t n, const mpf_s
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     c->pending = true;
     if (sync) return mpf_synchronize(c);
+    return 0;
+}
+
+int mpf_dev_alloc(mpf_ctx* c, uint64_t bytes, void** dev_ptr) {
+    if (!c || !dev_ptr) return ERR_INVAL;
+    *dev_ptr = nullptr;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (hipMalloc(dev_ptr, bytes ? bytes : 1) != hipSuccess) {
+        *dev_ptr = nullptr;
+        return fail(c, ERR_NOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+    }
+    return 0;
+}
+
+int mpf_dev_free(mpf_ctx* c, void* dev_ptr) {
+    if (!c) return ERR_INVAL;
+    if (!dev_ptr) return 0;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipFree(dev_ptr));
+    return 0;
+}
+
+int mpf_dev_upload(mpf_ctx* c, void* dev_dst, const void* host_src, uint64_t bytes) {
+    if (!c || (bytes && (!dev_dst || !host_src))) return ERR_INVAL;
+    if (!bytes) return 0;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    HIPCHK(c, hipMemcpyAsync(dev_dst, host_src, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int mpf_dev_download(mpf_ctx* c, void* host_dst, const void* dev_src, uint64_t bytes) {
+    if (!c || (bytes && (!dev_src || !host_dst))) return ERR_INVAL;
+    if (!bytes) return 0;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    HIPCHK(c, hipMemcpyAsync(host_dst, dev_src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }
 

@@ -24,6 +24,7 @@ struct IntegrateArgs {
     int stop_kind = -1;
     double stop_value = 0.0;
     double kit[3] = {0.0, 0.0, 0.0};
+    double* scratch = nullptr;       // per-ray launch records (split tier)
 };
 
 hipError_t launch_integrate_exact(int precision, const IntegrateArgs& A, const DevTree& t, int grid, hipStream_t st);

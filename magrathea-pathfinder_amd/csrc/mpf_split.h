@@ -10,9 +10,14 @@ namespace mpf {
 struct SplitTree {
     void* mem = nullptr;
     size_t bytes = 0;
+    uint64_t nslots = 0;
+    uint32_t* tslot = nullptr;        // cell -> stencil block slot, NIL when the stencil is missing
+    unsigned char* blocks = nullptr;  // per slot: FP64 centre, 1/size, 5 x 27 FP32 coefficients
 };
 
-bool split_build(SplitTree& s, const DevTree& t, int order, hipStream_t st, std::string& err);
+// level/kx/ky/kz: host arrays of the level-sorted cells
+bool split_build(SplitTree& s, const DevTree& t, const uint8_t* level, const uint32_t* kx, const uint32_t* ky,
+                 const uint32_t* kz, int order, hipStream_t st, std::string& err);
 void split_release(SplitTree& s);
 hipError_t launch_integrate_split(const IntegrateArgs& A, const DevTree& t, const SplitTree& s, int cus, hipStream_t st);
 

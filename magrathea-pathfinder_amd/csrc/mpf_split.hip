@@ -1,21 +1,656 @@
+// Split-precision tier (MPF_PREC_SPLIT): FP32 perturbation, FP64 background.
+//
+// Load-time preparation (split_build):
+//   1. For every cell, the cell whose 27-cell TSC stencil serves points whose
+//      finest cell it is: the reference evaluates the stencil at the level of
+//      the finest cell and, when the first bad neighbour in its ix/iy/iz loop
+//      order is coarser, redoes it at that neighbour's level; a missing
+//      neighbour yields Data() (simplehyperoctree.h:1596-1640).  That chain
+//      depends on the cell only, so it is resolved once per cell here.
+//   2. For every stencil-centre ("target") cell, a block with its FP64 centre,
+//      1/size and the TSC interpolant of (dphidt, dphidx, dphidy, dphidz, phi)
+//      as tri-quadratic polynomials in the offset (u, v, w) in [-1/2, 1/2]^3
+//      from the centre, 27 FP32 coefficients per field.  With
+//      W-1(u) = (1/2 - u)^2 / 2, W0(u) = 3/4 - u^2, W+1(u) = (1/2 + u)^2 / 2
+//      (the weights of simplehyperoctree.h:1587-1589 for d = |centre - x|/size),
+//      sum_ijk f_ijk Wi(u) Wj(v) Wk(w) = sum_pqr C_pqr u^p v^q w^r.
+//
+// Kernel: one photon per lane, FP64 state (a, t, position, wave vector) and
+// FP64 accumulations (affine parameter, distance, redshift, ISW); the
+// interpolated fields, dphi/dlambda and the perturbation terms of the
+// geodesic equation in FP32.  A lane keeps its finest cell and the current
+// coefficient block in registers and only searches the tree when a point
+// leaves that cell.  The fields after a step are reused as the first stage
+// of the next step (the reference interpolates the same point twice).
 #include "mpf_split.h"
+
+#include "mpf_exact_dev.h"
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
 
 namespace mpf {
 
-bool split_build(SplitTree& s, const DevTree& t, int order, hipStream_t st, std::string& err) {
-    (void)s; (void)t; (void)order; (void)st;
-    err = "split precision tier is not available in this build";
-    return false;
+namespace {
+
+constexpr int NCOEF = 27;
+constexpr int NFIELD = 5;  // dphidt, dphidx, dphidy, dphidz, phi
+constexpr int FSTRIDE = 28;  // floats per field (27 coefficients + pad): 7 x 16-byte loads
+constexpr int BLOCK_DOUBLES = 4;
+constexpr int BLOCK_BYTES = BLOCK_DOUBLES * 8 + NFIELD * FSTRIDE * 4;  // 592
+
+struct SplitView {
+    const uint32_t* tslot;        // cell -> block slot or NIL
+    const unsigned char* blocks;  // slot -> BLOCK_BYTES
+};
+
+// -------------------------------------------------------------- build
+
+__global__ void target_kernel(DevTree t, const uint8_t* __restrict__ lev, const uint32_t* __restrict__ kx,
+                              const uint32_t* __restrict__ ky, const uint32_t* __restrict__ kz, uint32_t* target) {
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= t.n) return;
+    unsigned l = lev[g];
+    uint32_t cx = kx[g], cy = ky[g], cz = kz[g];
+    uint32_t cur = (uint32_t)g;
+    for (int guard = 0; guard < 40; ++guard) {
+        const uint32_t lim = (l >= 32) ? 0xFFFFFFFFu : ((1u << l) - 1u);
+        unsigned bad_level = l;
+        bool missing = false;
+        for (int ic = 0; ic < 27 && !missing && bad_level == l; ++ic) {
+            const int ox = ic / 9 - 1, oy = (ic / 3) % 3 - 1, oz = ic % 3 - 1;
+            const long long nx = (long long)cx + ox, ny = (long long)cy + oy, nz = (long long)cz + oz;
+            // stencils that wrap round the octree domain are treated as missing
+            if (nx < 0 || ny < 0 || nz < 0 || nx > lim || ny > lim || nz > lim) { missing = true; break; }
+            unsigned fl = 0;
+            const uint32_t gv = find_cell(t, l, (uint32_t)nx, (uint32_t)ny, (uint32_t)nz, &fl);
+            if (gv == NIL) missing = true;
+            else if (fl < l) bad_level = fl;
+        }
+        if (missing) { target[g] = NIL; return; }
+        if (bad_level == l) { target[g] = cur; return; }
+        // drop to the ancestor at the coarser level (stored: the tree is parent-closed)
+        const unsigned sh = l - bad_level;
+        cx >>= sh; cy >>= sh; cz >>= sh;
+        l = bad_level;
+        unsigned fl = 0;
+        cur = find_cell(t, l, cx, cy, cz, &fl);
+        if (cur == NIL || fl != l) { target[g] = NIL; return; }
+    }
+    target[g] = NIL;
+}
+
+__device__ __forceinline__ double field_value(const DevTree& t, uint32_t g, int f) {
+    // f: 0 dphidt, 1 dphidx, 2 dphidy, 3 dphidz, 4 phi
+    const int slot = (f == 0) ? F_DT : (f == 4) ? F_PHI : (F_DX + f - 1);
+    return t.f64 ? t.f64[(uint64_t)slot * t.n + g] : (double)t.f32[(uint64_t)slot * t.n + g];
+}
+
+__global__ void block_kernel(DevTree t, const uint8_t* __restrict__ lev, const uint32_t* __restrict__ kx,
+                             const uint32_t* __restrict__ ky, const uint32_t* __restrict__ kz,
+                             const uint32_t* __restrict__ slot_cell, uint64_t nslots, unsigned char* blocks) {
+    const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    const uint32_t g = slot_cell[s];
+    const unsigned l = lev[g];
+    const uint32_t cx = kx[g], cy = ky[g], cz = kz[g];
+    uint32_t nb[27];
+    for (int ic = 0; ic < 27; ++ic) {
+        const int ox = ic / 9 - 1, oy = (ic / 3) % 3 - 1, oz = ic % 3 - 1;
+        unsigned fl = 0;
+        nb[ic] = find_cell(t, l, cx + ox, cy + oy, cz + oz, &fl);
+    }
+    unsigned char* blk = blocks + s * (uint64_t)BLOCK_BYTES;
+    double* hd = reinterpret_cast<double*>(blk);
+    hd[0] = cell_center(l, cx);
+    hd[1] = cell_center(l, cy);
+    hd[2] = cell_center(l, cz);
+    hd[3] = (double)(1ull << l) / EXTENT;  // 1 / cell size
+    float* co = reinterpret_cast<float*>(blk + BLOCK_DOUBLES * 8);
+    // M[p][i]: coefficient of u^p in W_i, i = -1, 0, +1
+    const double M[3][3] = {{0.125, 0.75, 0.125}, {-0.5, 0.0, 0.5}, {0.5, -1.0, 0.5}};
+    for (int f = 0; f < NFIELD; ++f) {
+        double v[27];
+        for (int ic = 0; ic < 27; ++ic) v[ic] = field_value(t, nb[ic], f);
+        // transform along z, then y, then x
+        double a1[27], a2[27];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                for (int r = 0; r < 3; ++r) {
+                    double acc = 0.0;
+                    for (int k = 0; k < 3; ++k) acc += M[r][k] * v[i * 9 + j * 3 + k];
+                    a1[i * 9 + j * 3 + r] = acc;
+                }
+        for (int i = 0; i < 3; ++i)
+            for (int q = 0; q < 3; ++q)
+                for (int r = 0; r < 3; ++r) {
+                    double acc = 0.0;
+                    for (int j = 0; j < 3; ++j) acc += M[q][j] * a1[i * 9 + j * 3 + r];
+                    a2[i * 9 + q * 3 + r] = acc;
+                }
+        for (int p = 0; p < 3; ++p)
+            for (int q = 0; q < 3; ++q)
+                for (int r = 0; r < 3; ++r) {
+                    double acc = 0.0;
+                    for (int i = 0; i < 3; ++i) acc += M[p][i] * a2[i * 9 + q * 3 + r];
+                    co[f * FSTRIDE + p * 9 + q * 3 + r] = (float)acc;
+                }
+        co[f * FSTRIDE + 27] = 0.0f;
+    }
+}
+
+// ------------------------------------------------------------- kernel
+
+// sum_pqr C_pqr u^p v^q w^r by Horner: 26 FMA
+__device__ __forceinline__ float horner27(const float* c, float u, float v, float w) {
+    float g[3];
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        float h[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const float* cc = c + p * 9 + q * 3;
+            h[q] = __builtin_fmaf(__builtin_fmaf(cc[2], w, cc[1]), w, cc[0]);
+        }
+        g[p] = __builtin_fmaf(__builtin_fmaf(h[2], v, h[1]), v, h[0]);
+    }
+    return __builtin_fmaf(__builtin_fmaf(g[2], u, g[1]), u, g[0]);
+}
+
+// RC = true keeps the current coefficient block in registers; RC = false
+// keeps only its address and reads the coefficients at every evaluation
+// (from L1/L2: a lane reads the same block for ~10 consecutive steps).
+template <bool RC>
+struct Interp {
+    // finest cell of the last point: level and coordinates
+    unsigned lvl;
+    uint32_t kx, ky, kz;
+    bool valid;          // the last point had data
+    uint32_t slot;
+    double cx, cy, cz, inv;
+    float c[RC ? NFIELD * NCOEF : 1];
+};
+
+template <bool RC>
+__device__ __forceinline__ void load_block(const SplitView& sv, uint32_t slot, Interp<RC>& I) {
+    const unsigned char* blk = sv.blocks + (uint64_t)slot * BLOCK_BYTES;
+    const double4 h = *reinterpret_cast<const double4*>(blk);
+    I.cx = h.x; I.cy = h.y; I.cz = h.z; I.inv = h.w;
+    if (RC) {
+        const float4* cf = reinterpret_cast<const float4*>(blk + BLOCK_DOUBLES * 8);
+#pragma unroll
+        for (int f = 0; f < NFIELD; ++f) {
+#pragma unroll
+            for (int i = 0; i < FSTRIDE / 4; ++i) {
+                const float4 q = cf[f * (FSTRIDE / 4) + i];
+                const int b = f * NCOEF + 4 * i;
+                if (4 * i + 0 < NCOEF) I.c[b + 0] = q.x;
+                if (4 * i + 1 < NCOEF) I.c[b + 1] = q.y;
+                if (4 * i + 2 < NCOEF) I.c[b + 2] = q.z;
+                if (4 * i + 3 < NCOEF) I.c[b + 3] = q.w;
+            }
+        }
+    }
+    I.slot = slot;
+}
+
+// Fields at (x, y, z): out[0..3] = dphidt, dphidx, dphidy, dphidz; out[4] = phi
+// when NF == 5.  Returns false when the reference would return Data().
+template <int NF, bool RC>
+__device__ __forceinline__ bool split_eval(const DevTree& t, const SplitView& sv, Interp<RC>& I, double x, double y,
+                                          double z, float* out) {
+    bool same = false;
+    if (I.valid) {
+        same = axis_coord(x, I.lvl) == I.kx && axis_coord(y, I.lvl) == I.ky && axis_coord(z, I.lvl) == I.kz;
+    }
+    if (!same) {
+        unsigned fl = 0;
+        const uint32_t kx32 = axis_coord(x, 32), ky32 = axis_coord(y, 32), kz32 = axis_coord(z, 32);
+        const uint32_t g = find_cell(t, 32, kx32, ky32, kz32, &fl);
+        const uint32_t slot = (g == NIL) ? NIL : sv.tslot[g];
+        if (slot == NIL) {
+            I.valid = false;
+            return false;
+        }
+        I.lvl = fl;
+        I.kx = kx32 >> (32 - fl);
+        I.ky = ky32 >> (32 - fl);
+        I.kz = kz32 >> (32 - fl);
+        I.valid = true;
+        if (slot != I.slot) load_block<RC>(sv, slot, I);
+    }
+    const float u = (float)((x - I.cx) * I.inv);
+    const float v = (float)((y - I.cy) * I.inv);
+    const float w = (float)((z - I.cz) * I.inv);
+    if (RC) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) out[f] = horner27(I.c + f * NCOEF, u, v, w);
+    } else {
+        const float4* cf = reinterpret_cast<const float4*>(sv.blocks + (uint64_t)I.slot * BLOCK_BYTES + BLOCK_DOUBLES * 8);
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            float c[FSTRIDE];
+#pragma unroll
+            for (int i = 0; i < FSTRIDE / 4; ++i) {
+                const float4 q = cf[f * (FSTRIDE / 4) + i];
+                c[4 * i] = q.x; c[4 * i + 1] = q.y; c[4 * i + 2] = q.z; c[4 * i + 3] = q.w;
+            }
+            out[f] = horner27(c, u, v, w);
+        }
+    }
+    return true;
+}
+
+__device__ __forceinline__ double cosmo_lin(double x0, const double* __restrict__ x, const double* __restrict__ y,
+                                            int n, int& hint) {
+    int i = hint;
+    while (i < n && !(x0 < x[i])) ++i;
+    while (i > 0 && x0 < x[i - 1]) --i;
+    hint = i;
+    i += (i <= 0) - (i >= n);
+    return y[i - 1] + ((y[i] - y[i - 1]) * (x0 - x[i - 1]) / (x[i] - x[i - 1]));
+}
+
+enum { CA = 0, CT, CX, CY, CZ, CKT, CKX, CKY, CKZ };
+
+// geodesic right-hand side: FP64 background, FP32 perturbation
+__device__ __forceinline__ void rhs_split(const double* in, const float* F, double dadt, double inv_scale,
+                                          double* out) {
+    const float ktf = (float)in[CKT], kxf = (float)in[CKX], kyf = (float)in[CKY], kzf = (float)in[CKZ];
+    const float c2inv2 = (float)(2.0 / C2);
+    const float grad = F[1] * kxf + F[2] * kyf + F[3] * kzf;  // grad phi . k
+    const float dphidl = F[0] * ktf + grad;
+    const float kt2 = ktf * ktf;
+    const float pert_t = -(c2inv2 * ktf) * grad;                 // -(2/c^2) kt (dphidl - dphidt kt)
+    const float s = c2inv2 * dphidl;
+    const double bg = -(2.0 * dadt / in[CA]) * in[CKT];
+    out[CA] = in[CKT] * dadt;
+    out[CT] = in[CKT];
+    out[CX] = in[CKX] * inv_scale;
+    out[CY] = in[CKY] * inv_scale;
+    out[CZ] = in[CKZ] * inv_scale;
+    out[CKT] = bg * in[CKT] + (double)pert_t;
+    out[CKX] = bg * in[CKX] + (double)(s * kxf - 2.0f * F[1] * kt2);
+    out[CKY] = bg * in[CKY] + (double)(s * kyf - 2.0f * F[2] * kt2);
+    out[CKZ] = bg * in[CKZ] + (double)(s * kzf - 2.0f * F[3] * kt2);
+}
+
+struct SplitRay {
+    double core[9];
+    double lambda, s, isw, iswold, redshift, chi;
+    double a0, phi0c2, dl;
+    float phi, dx, dy, dz, dt;   // fields at the current point (dt: +dphidt at launch, -dphidt after)
+    float F[4];                  // interpolated dphidt, dphidx, dphidy, dphidz at the current point
+    uint32_t appended;
+    int hint;
+};
+
+__device__ __forceinline__ double level_extent(unsigned l) { return EXTENT / (double)(1ull << l); }
+
+// Launch records (integrator.h:669-705) with exact FP64 fields at the start
+// point: normalised core (9), empty flag, phi, dphidx, dphidy, dphidz,
+// dphidt, level of the located cell.
+constexpr int LAUNCH_DOUBLES = 16;
+
+__global__ void __launch_bounds__(256) launch_kernel(IntegrateArgs A, DevTree t) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n) return;
+    const double c2 = C2;
+    double core[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) core[k] = A.in9[9 * i + k];
+    exact::TscCache cache;
+    exact::cache_init(cache);
+    double d[7];
+    int loc = -1;
+    exact::tsc<1, false>(t, cache, core[CX], core[CY], core[CZ], d, &loc);
+    const double phi = d[F_PHI];
+    const double norm = sqrt((c2 * (1.0 + 2.0 * (phi / c2)) * core[CKT] * core[CKT]) /
+                             ((1.0 - 2.0 * (phi / c2)) *
+                              (core[CKX] * core[CKX] + core[CKY] * core[CKY] + core[CKZ] * core[CKZ])));
+    core[CKX] *= norm;
+    core[CKY] *= norm;
+    core[CKZ] *= norm;
+    int hint = 0;
+    core[CA] = cosmo_lin(core[CT], A.ct, A.ca, A.ncos, hint);
+    double* L = A.scratch + (uint64_t)LAUNCH_DOUBLES * i;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) L[k] = core[k];
+    L[9] = exact::empty7(d) ? 1.0 : 0.0;
+    L[10] = phi;
+    L[11] = d[F_DX]; L[12] = d[F_DY]; L[13] = d[F_DZ]; L[14] = d[F_DT];
+    L[15] = (double)(loc < 0 ? 0 : loc);
+}
+
+// rho and a (the `ah` output) of each final point, exact FP64 TSC
+__global__ void __launch_bounds__(256) final_kernel(IntegrateArgs A, DevTree t) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n) return;
+    double* o = A.out + (uint64_t)MPF_NOUT * i;
+    if (o[MPF_OUT_NSTEPS] < 0.0) return;
+    exact::TscCache cache;
+    exact::cache_init(cache);
+    double d[7];
+    int loc = -1;
+    exact::tsc<1, false>(t, cache, o[MPF_OUT_X], o[MPF_OUT_Y], o[MPF_OUT_Z], d, &loc);
+    o[MPF_OUT_RHO] = d[F_RHO];
+    o[MPF_OUT_AH] = o[MPF_OUT_NSTEPS] > 0.0 ? d[F_A] : 0.0;
+}
+
+template <bool RC, int W>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8)))
+integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
+#pragma clang fp contract(fast)
+    const double c = C_LIGHT, c2 = C2;
+    const double inv_scale = 1.0 / A.length;
+    const double ratio_k = (A.length / c) / (double)A.nsteps;
+    Interp<RC> I;
+    I.valid = false;
+    I.slot = NIL;
+    SplitRay R;
+    bool active = false, drained = false;
+    uint64_t ray = 0;
+    const unsigned lane = __lane_id();
+    while (true) {
+        const bool need = !active && !drained;
+        const unsigned long long m = __ballot(need);
+        if (m) {
+            const int leader = __ffsll((long long)m) - 1;
+            unsigned long long base = 0;
+            if ((int)lane == leader) base = atomicAdd(A.queue, (unsigned long long)__popcll(m));
+            base = __shfl(base, leader);
+            if (need) {
+                const uint64_t my = base + (uint64_t)__popcll(m & ((1ull << lane) - 1ull));
+                if (my < A.n) {
+                    ray = my;
+                    // launch record prepared by launch_kernel (integrator.h:669-705)
+                    const double* L = A.scratch + (uint64_t)LAUNCH_DOUBLES * my;
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) R.core[k] = L[k];
+                    const bool empty = L[9] != 0.0;
+                    const double phi = L[10];
+                    R.hint = 0;
+                    R.lambda = R.s = R.isw = R.iswold = R.redshift = R.chi = 0.0;
+                    R.a0 = R.core[CA];
+                    R.phi0c2 = phi / c2;
+                    R.phi = (float)phi;
+                    R.dx = (float)L[11]; R.dy = (float)L[12]; R.dz = (float)L[13];
+                    R.dt = (float)L[14];
+                    R.F[0] = R.dt; R.F[1] = R.dx; R.F[2] = R.dy; R.F[3] = R.dz;
+                    R.appended = 0;
+                    const unsigned l0 = (unsigned)L[15];
+                    R.dl = level_extent(l0) * (R.core[CA] * R.core[CA] * ratio_k);
+                    I.valid = false;
+                    active = !empty;
+                    if (!active) {
+                        double* o = A.out + (uint64_t)MPF_NOUT * ray;
+                        for (int k = 0; k < MPF_NOUT; ++k) o[k] = 0.0;
+#pragma unroll
+                        for (int k = 0; k < 9; ++k) o[k] = R.core[k];
+                        o[MPF_OUT_LEVEL] = (double)l0;
+                    }
+                } else {
+                    drained = true;
+                }
+            }
+        }
+        if (!__any(active)) {
+            if (__all(drained)) break;
+            continue;
+        }
+        if (!active) continue;
+        // ---------------------------------------------------------- step
+        const double dl = R.dl;
+        double ksum[9], kin[9], kout[9];
+        double dadt = cosmo_lin(R.core[CT], A.ct, A.cdadt, A.ncos, R.hint);
+        rhs_split(R.core, R.F, dadt, inv_scale, kout);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) { ksum[i] = kout[i]; kin[i] = R.core[i] + (0.5 * dl) * kout[i]; }
+        float F[5];
+        bool ok = split_eval<4, RC>(t, sv, I, kin[CX], kin[CY], kin[CZ], F);
+        if (!ok) { F[0] = F[1] = F[2] = F[3] = 0.f; }
+        dadt = cosmo_lin(kin[CT], A.ct, A.cdadt, A.ncos, R.hint);
+        rhs_split(kin, F, dadt, inv_scale, kout);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) { ksum[i] += 2.0 * kout[i]; kin[i] = R.core[i] + (0.5 * dl) * kout[i]; }
+        ok = split_eval<4, RC>(t, sv, I, kin[CX], kin[CY], kin[CZ], F);
+        if (!ok) { F[0] = F[1] = F[2] = F[3] = 0.f; }
+        dadt = cosmo_lin(kin[CT], A.ct, A.cdadt, A.ncos, R.hint);
+        rhs_split(kin, F, dadt, inv_scale, kout);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) { ksum[i] += 2.0 * kout[i]; kin[i] = R.core[i] + dl * kout[i]; }
+        ok = split_eval<4, RC>(t, sv, I, kin[CX], kin[CY], kin[CZ], F);
+        if (!ok) { F[0] = F[1] = F[2] = F[3] = 0.f; }
+        dadt = cosmo_lin(kin[CT], A.ct, A.cdadt, A.ncos, R.hint);
+        rhs_split(kin, F, dadt, inv_scale, kout);
+        double p[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) p[i] = R.core[i] + (dl / 6.0) * (ksum[i] + kout[i]);
+        // fields at the new point
+        const bool inside = !(p[CA] < 0.0) && p[CX] > -2.0 && p[CX] < 2.0 && p[CY] > -2.0 && p[CY] < 2.0 &&
+                            p[CZ] > -2.0 && p[CZ] < 2.0;
+        bool cont = inside && split_eval<5, RC>(t, sv, I, p[CX], p[CY], p[CZ], F);
+        if (!cont) { F[0] = F[1] = F[2] = F[3] = F[4] = 0.f; }
+        // extras (integrator.h:756-776) with FP64 accumulation
+        const double nphi = (double)F[4];
+        const double ndt = -(double)F[0];
+        const double pdx = (double)F[1], pdy = (double)F[2], pdz = (double)F[3];
+        const double isw = R.isw + ((double)R.dt + ndt) * (p[CT] - R.core[CT]) / c2;
+        const double ddx = p[CX] - R.core[CX], ddy = p[CY] - R.core[CY], ddz = p[CZ] - R.core[CZ];
+        const double iswold = R.iswold - (2.0 * (nphi - (double)R.phi) / c2 -
+                                          ((((double)R.dx + pdx) * ddx + ((double)R.dy + pdy) * ddy +
+                                            ((double)R.dz + pdz) * ddz) * A.length) / c2);
+        const double chi = sqrt(p[CX] * p[CX] + p[CY] * p[CY] + p[CZ] * p[CZ]);
+        const double lambda = R.lambda + dl;
+        const double s = R.s + sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+        const double redshift = R.a0 / p[CA] * (1.0 + R.phi0c2 - nphi / c2) - 1.0;
+        bool provisional = false;
+        if (cont) {
+            // a photon that the final pop could remove: keep its predecessor as the provisional output
+            if (signbit(redshift) || signbit(p[CA])) {
+                provisional = true;
+                double* o = A.out + (uint64_t)MPF_NOUT * ray;
+#pragma unroll
+                for (int k = 0; k < 9; ++k) o[k] = R.core[k];
+                o[MPF_OUT_REDSHIFT] = R.redshift; o[MPF_OUT_LAMBDA] = R.lambda; o[MPF_OUT_S] = R.s;
+                o[MPF_OUT_CHI] = R.chi; o[MPF_OUT_ISW] = R.isw; o[MPF_OUT_ISWOLD] = R.iswold;
+                o[MPF_OUT_PHI] = R.phi; o[MPF_OUT_DPHIDX] = R.dx; o[MPF_OUT_DPHIDY] = R.dy; o[MPF_OUT_DPHIDZ] = R.dz;
+                o[MPF_OUT_DPHIDT] = R.dt; o[MPF_OUT_NSTEPS] = (double)R.appended;
+                o[MPF_OUT_LEVEL] = (double)(I.valid ? I.lvl : 0u);
+            }
+            (void)provisional;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) R.core[i] = p[i];
+            R.lambda = lambda; R.s = s; R.isw = isw; R.iswold = iswold; R.redshift = redshift; R.chi = chi;
+            R.phi = F[4]; R.dx = F[1]; R.dy = F[2]; R.dz = F[3]; R.dt = -F[0];
+            R.F[0] = F[0]; R.F[1] = F[1]; R.F[2] = F[2]; R.F[3] = F[3];
+            R.dl = level_extent(I.lvl) * (p[CA] * p[CA] * ratio_k);
+            R.appended++;
+        }
+        bool brk = false;
+        switch (A.stop_kind) {
+            case 0: brk = R.chi > A.stop_value; break;
+            case 1: brk = R.lambda > A.stop_value; break;
+            case 2: brk = R.redshift > A.stop_value; break;
+            case 3: brk = R.core[CT] > A.stop_value; break;
+            case 4: brk = R.core[CA] < A.stop_value; break;
+            case 5: brk = (p[CX] * A.kit[0] + p[CY] * A.kit[1] + p[CZ] * A.kit[2]) > A.stop_value; break;
+            case 6: brk = R.s > A.stop_value; break;
+            default: brk = false; break;
+        }
+        if (!cont || brk) {
+            active = false;
+            double* o = A.out + (uint64_t)MPF_NOUT * ray;
+            uint32_t steps = R.appended;
+            if (signbit(R.redshift) || signbit(R.core[CA])) {
+                // pop: the provisional output already holds the previous element
+                steps = steps ? steps - 1 : 0;
+                if (R.appended == 0) o[MPF_OUT_NSTEPS] = -1.0;
+            } else {
+                // final element with the remaining outputs of integrator.h:756-776
+                // (rho and a of the final point are added by final_kernel)
+#pragma unroll
+                for (int k = 0; k < 9; ++k) o[k] = R.core[k];
+                const double phi = R.phi;
+                const double kk = R.core[CKX] * R.core[CKX] + R.core[CKY] * R.core[CKY] + R.core[CKZ] * R.core[CKZ];
+                o[MPF_OUT_REDSHIFT] = R.redshift; o[MPF_OUT_LAMBDA] = R.lambda; o[MPF_OUT_S] = R.s;
+                o[MPF_OUT_CHI] = R.chi; o[MPF_OUT_ISW] = R.isw; o[MPF_OUT_ISWOLD] = R.iswold;
+                o[MPF_OUT_PHI] = phi; o[MPF_OUT_DPHIDX] = R.dx; o[MPF_OUT_DPHIDY] = R.dy; o[MPF_OUT_DPHIDZ] = R.dz;
+                o[MPF_OUT_DPHIDT] = R.dt; 
+                o[MPF_OUT_DPHIDL] = (double)R.F[0] * R.core[CKT] + ((double)R.dx * R.core[CKX] +
+                                                                    (double)R.dy * R.core[CKY] +
+                                                                    (double)R.dz * R.core[CKZ]);
+                o[MPF_OUT_LAPLACIAN] = 0.0;
+                o[MPF_OUT_DSDL2] = (R.core[CA] * R.core[CA]) *
+                                   ((-c2 * (1.0 + 2.0 * (phi / c2)) * R.core[CKT] * R.core[CKT]) +
+                                    ((1.0 - 2.0 * (phi / c2)) * kk));
+                o[MPF_OUT_ERROR] = 1.0 - ((1.0 - 2.0 * (phi / c2)) * kk) /
+                                             (c2 * (1.0 + 2.0 * (phi / c2)) * R.core[CKT] * R.core[CKT]);
+                o[MPF_OUT_LEVEL] = (double)I.lvl;
+                o[MPF_OUT_NSTEPS] = (double)steps;
+            }
+            atomicAdd(A.steps, (unsigned long long)steps);
+        }
+    }
+}
+
+}  // namespace
+
+bool split_build(SplitTree& s, const DevTree& t, const uint8_t* level, const uint32_t* kx, const uint32_t* ky,
+                 const uint32_t* kz, int order, hipStream_t st, std::string& err) {
+    split_release(s);
+    if (order != 2) {
+        err = "the split-precision tier implements TSC (order 2)";
+        return false;
+    }
+    const uint64_t n = t.n;
+    uint8_t* d_lev = nullptr;
+    uint32_t *d_kx = nullptr, *d_ky = nullptr, *d_kz = nullptr, *d_target = nullptr, *d_slot_cell = nullptr;
+    auto cleanup = [&]() {
+        if (d_lev) (void)hipFree(d_lev);
+        if (d_kx) (void)hipFree(d_kx);
+        if (d_ky) (void)hipFree(d_ky);
+        if (d_kz) (void)hipFree(d_kz);
+        if (d_target) (void)hipFree(d_target);
+        if (d_slot_cell) (void)hipFree(d_slot_cell);
+    };
+    auto chk = [&](hipError_t e, const char* what) {
+        if (e != hipSuccess) {
+            err = std::string("split tier preparation: ") + what + ": " + hipGetErrorString(e);
+            return false;
+        }
+        return true;
+    };
+    if (!chk(hipMalloc((void**)&d_lev, n), "alloc") || !chk(hipMalloc((void**)&d_kx, 4 * n), "alloc") ||
+        !chk(hipMalloc((void**)&d_ky, 4 * n), "alloc") || !chk(hipMalloc((void**)&d_kz, 4 * n), "alloc") ||
+        !chk(hipMalloc((void**)&d_target, 4 * n), "alloc")) {
+        cleanup();
+        return false;
+    }
+    bool ok = chk(hipMemcpyAsync(d_lev, level, n, hipMemcpyHostToDevice, st), "copy") &&
+              chk(hipMemcpyAsync(d_kx, kx, 4 * n, hipMemcpyHostToDevice, st), "copy") &&
+              chk(hipMemcpyAsync(d_ky, ky, 4 * n, hipMemcpyHostToDevice, st), "copy") &&
+              chk(hipMemcpyAsync(d_kz, kz, 4 * n, hipMemcpyHostToDevice, st), "copy");
+    if (ok) {
+        hipLaunchKernelGGL(target_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, t, d_lev, d_kx, d_ky,
+                           d_kz, d_target);
+        ok = chk(hipGetLastError(), "target kernel");
+    }
+    std::vector<uint32_t> target(n);
+    if (ok) ok = chk(hipMemcpyAsync(target.data(), d_target, 4 * n, hipMemcpyDeviceToHost, st), "copy") &&
+                 chk(hipStreamSynchronize(st), "sync");
+    if (!ok) {
+        cleanup();
+        return false;
+    }
+    // slots for the stencil-centre cells, in cell order
+    std::vector<uint32_t> slot_of(n, NIL);
+    for (uint64_t g = 0; g < n; ++g)
+        if (target[g] != NIL) slot_of[target[g]] = 0;
+    std::vector<uint32_t> slot_cell;
+    slot_cell.reserve(n);
+    for (uint64_t g = 0; g < n; ++g)
+        if (slot_of[g] != NIL) {
+            slot_of[g] = (uint32_t)slot_cell.size();
+            slot_cell.push_back((uint32_t)g);
+        }
+    std::vector<uint32_t> tslot(n);
+    for (uint64_t g = 0; g < n; ++g) tslot[g] = (target[g] == NIL) ? NIL : slot_of[target[g]];
+    const uint64_t ns = slot_cell.size();
+    s.nslots = ns;
+    s.bytes = n * 4 + std::max<uint64_t>(ns, 1) * BLOCK_BYTES;
+    if (!chk(hipMalloc(&s.mem, s.bytes), "alloc blocks") ||
+        !chk(hipMalloc((void**)&d_slot_cell, 4 * std::max<uint64_t>(ns, 1)), "alloc")) {
+        cleanup();
+        split_release(s);
+        return false;
+    }
+    s.tslot = reinterpret_cast<uint32_t*>(s.mem);
+    s.blocks = reinterpret_cast<unsigned char*>(s.mem) + ((n * 4 + 255) / 256) * 256;
+    s.bytes = ((n * 4 + 255) / 256) * 256 + std::max<uint64_t>(ns, 1) * BLOCK_BYTES;
+    (void)hipFree(s.mem);
+    s.mem = nullptr;
+    if (!chk(hipMalloc(&s.mem, s.bytes), "alloc blocks")) {
+        cleanup();
+        return false;
+    }
+    s.tslot = reinterpret_cast<uint32_t*>(s.mem);
+    s.blocks = reinterpret_cast<unsigned char*>(s.mem) + ((n * 4 + 255) / 256) * 256;
+    ok = chk(hipMemcpyAsync(s.tslot, tslot.data(), 4 * n, hipMemcpyHostToDevice, st), "copy") &&
+         chk(hipMemcpyAsync(d_slot_cell, slot_cell.data(), 4 * std::max<uint64_t>(ns, 1), hipMemcpyHostToDevice, st),
+             "copy");
+    if (ok && ns) {
+        hipLaunchKernelGGL(block_kernel, dim3((unsigned)((ns + 127) / 128)), dim3(128), 0, st, t, d_lev, d_kx, d_ky,
+                           d_kz, d_slot_cell, ns, s.blocks);
+        ok = chk(hipGetLastError(), "block kernel");
+    }
+    if (ok) ok = chk(hipStreamSynchronize(st), "sync");
+    cleanup();
+    if (!ok) split_release(s);
+    return ok;
 }
 
 void split_release(SplitTree& s) {
     if (s.mem) (void)hipFree(s.mem);
-    s.mem = nullptr;
-    s.bytes = 0;
+    s = SplitTree();
 }
 
-hipError_t launch_integrate_split(const IntegrateArgs&, const DevTree&, const SplitTree&, int, hipStream_t) {
-    return hipErrorNotSupported;
+// Kernel variant (MPF_SPLIT_VARIANT): 1 = coefficient block resident in
+// registers (1 wave/SIMD); 2, 3, 4 = coefficients read per evaluation, with
+// at least that many waves per SIMD.
+static int split_variant() {
+    const char* e = getenv("MPF_SPLIT_VARIANT");
+    const int v = e ? atoi(e) : 0;
+    return (v >= 1 && v <= 4) ? v : 2;
+}
+
+static int split_waves_per_cu() {
+    const char* e = getenv("MPF_SPLIT_BLOCKS_PER_CU");
+    const int v = e ? atoi(e) : 0;
+    return v > 0 ? v : 8;
+}
+
+hipError_t launch_integrate_split(const IntegrateArgs& A, const DevTree& t, const SplitTree& s, int cus,
+                                  hipStream_t st) {
+    SplitView sv{s.tslot, s.blocks};
+    const uint64_t blocks = (A.n + 255) / 256;
+    const int grid = (int)std::min<uint64_t>(blocks, (uint64_t)cus * split_waves_per_cu());
+    const unsigned pgrid = (unsigned)((A.n + 255) / 256);
+    hipLaunchKernelGGL(launch_kernel, dim3(pgrid), dim3(256), 0, st, A, t);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    switch (split_variant()) {
+        case 1: hipLaunchKernelGGL((integrate_split_kernel<true, 1>), dim3(grid), dim3(256), 0, st, A, t, sv); break;
+        case 2: hipLaunchKernelGGL((integrate_split_kernel<false, 2>), dim3(grid), dim3(256), 0, st, A, t, sv); break;
+        case 4: hipLaunchKernelGGL((integrate_split_kernel<false, 4>), dim3(grid), dim3(256), 0, st, A, t, sv); break;
+        default: hipLaunchKernelGGL((integrate_split_kernel<false, 3>), dim3(grid), dim3(256), 0, st, A, t, sv); break;
+    }
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(final_kernel, dim3(pgrid), dim3(256), 0, st, A, t);
+    return hipGetLastError();
 }
 
 }  // namespace mpf

@@ -28,7 +28,8 @@ OUT = {name: i for i, name in enumerate(OUT_FIELDS)}
 # every symbol declared in include/mpf.h
 EXPORTS = ("mpf_create", "mpf_destroy", "mpf_last_error", "mpf_set_precision", "mpf_load_octree_records",
            "mpf_load_octree_fields64", "mpf_octree_info", "mpf_set_cosmology", "mpf_locate", "mpf_interpolate",
-           "mpf_integrate", "mpf_integrate_dev", "mpf_last_run", "mpf_synchronize", "mpf_validate_octree")
+           "mpf_integrate", "mpf_integrate_dev", "mpf_last_run", "mpf_synchronize", "mpf_validate_octree",
+           "mpf_dev_alloc", "mpf_dev_free", "mpf_dev_upload", "mpf_dev_download")
 
 
 class MpfError(RuntimeError):
@@ -72,6 +73,10 @@ def load_library(path=LIB_PATH):
     lib.mpf_last_run.argtypes = [vp, vp, vp]
     lib.mpf_synchronize.argtypes = [vp]
     lib.mpf_validate_octree.argtypes = [vp, u64, vp, vp, ctypes.c_char_p, u64]
+    lib.mpf_dev_alloc.argtypes = [vp, u64, ctypes.POINTER(vp)]
+    lib.mpf_dev_free.argtypes = [vp, vp]
+    lib.mpf_dev_upload.argtypes = [vp, vp, vp, u64]
+    lib.mpf_dev_download.argtypes = [vp, vp, vp, u64]
     _lib = lib
     return lib
 
@@ -199,6 +204,26 @@ class Context:
         ms = ctypes.c_double(0.0)
         self._check(self.lib.mpf_last_run(self.h, ctypes.byref(steps), ctypes.byref(ms)), "mpf_last_run")
         return steps.value, ms.value
+
+    def dev_alloc(self, nbytes):
+        """Device buffer on the context's device; returns its address (int)."""
+        ptr = ctypes.c_void_p()
+        self._check(self.lib.mpf_dev_alloc(self.h, ctypes.c_uint64(int(nbytes)), ctypes.byref(ptr)), "mpf_dev_alloc")
+        return ptr.value
+
+    def dev_free(self, ptr):
+        self._check(self.lib.mpf_dev_free(self.h, ctypes.c_void_p(ptr)), "mpf_dev_free")
+
+    def upload(self, ptr, array):
+        a = np.ascontiguousarray(array)
+        self._check(self.lib.mpf_dev_upload(self.h, ctypes.c_void_p(ptr), _p(a), ctypes.c_uint64(a.nbytes)),
+                    "mpf_dev_upload")
+
+    def download(self, ptr, shape, dtype=np.float64):
+        a = np.empty(shape, dtype=dtype)
+        self._check(self.lib.mpf_dev_download(self.h, _p(a), ctypes.c_void_p(ptr), ctypes.c_uint64(a.nbytes)),
+                    "mpf_dev_download")
+        return a
 
     def synchronize(self):
         self._check(self.lib.mpf_synchronize(self.h), "mpf_synchronize")

@@ -222,3 +222,217 @@ def analytic_tree(base_level=6, half=0.5, max_level=None, refine=None, length=1.
     f = analytic_fields(cx, cy, cz, length, seed=seed)
     rec, order = formats.make_records(level, kx, ky, kz, f)
     return rec, f[order]
+
+
+# ----------------------------------------------------- benchmark workloads
+
+def _part1by3(v):
+    """Spread the low 16 bits of v to every 4th bit (bit b -> bit 4b)."""
+    v = v.astype(np.uint64) & np.uint64(0xFFFF)
+    v = (v | (v << np.uint64(24))) & np.uint64(0x000000FF000000FF)
+    v = (v | (v << np.uint64(12))) & np.uint64(0x000F000F000F000F)
+    v = (v | (v << np.uint64(6))) & np.uint64(0x0303030303030303)
+    v = (v | (v << np.uint64(3))) & np.uint64(0x1111111111111111)
+    return v
+
+
+def keys_hi_fast(level, kx, ky, kz):
+    """High key word for levels <= 16 (the low word is then zero)."""
+    level = np.asarray(level, dtype=np.uint64)
+    code = _part1by3(kx) | (_part1by3(ky) << np.uint64(1)) | (_part1by3(kz) << np.uint64(2))
+    marks = np.uint64(0x8888888888888888) & ((np.uint64(1) << (np.uint64(4) * level)) - np.uint64(1))
+    code = code | marks
+    return code << (np.uint64(64) - np.uint64(4) * level)
+
+
+def _trilinear(grid, fx, fy, fz):
+    """Periodic trilinear interpolation of grid (n^3) at fractional grid
+    coordinates (cell centres at integer + 0.5)."""
+    n = grid.shape[0]
+    x = fx - 0.5
+    y = fy - 0.5
+    z = fz - 0.5
+    x0 = np.floor(x).astype(np.int64)
+    y0 = np.floor(y).astype(np.int64)
+    z0 = np.floor(z).astype(np.int64)
+    tx = x - x0
+    ty = y - y0
+    tz = z - z0
+    out = np.zeros(fx.shape)
+    for dx in (0, 1):
+        wx = tx if dx else 1 - tx
+        ix = (x0 + dx) % n
+        for dy in (0, 1):
+            wy = ty if dy else 1 - ty
+            iy = (y0 + dy) % n
+            for dz in (0, 1):
+                wz = tz if dz else 1 - tz
+                iz = (z0 + dz) % n
+                out += wx * wy * wz * grid[ix, iy, iz]
+    return out
+
+
+def _hash_uniform(kx, ky, kz, lvl, seed):
+    """Deterministic uniform noise in [-sqrt(3), sqrt(3)) (unit variance) from
+    cell coordinates: the same on every device."""
+    import torch
+    h = (kx * 73856093) ^ (ky * 19349663) ^ (kz * 83492791) ^ (lvl * 2654435761 + seed * 97)
+    h = h & 0xFFFFFFFF
+    h = (h ^ (h >> 16)) * 0x45d9f3b & 0xFFFFFFFF
+    h = (h ^ (h >> 16)) * 0x45d9f3b & 0xFFFFFFFF
+    h = h ^ (h >> 16)
+    u = h.to(torch.float64) / 4294967296.0
+    return (2.0 * u - 1.0) * math.sqrt(3.0)
+
+
+def _trilinear_t(grid, fx, fy, fz):
+    """Periodic trilinear interpolation (torch) at fractional grid coordinates
+    (cell centres at integer + 0.5)."""
+    import torch
+    n = grid.shape[0]
+    x = fx - 0.5
+    y = fy - 0.5
+    z = fz - 0.5
+    x0 = torch.floor(x)
+    y0 = torch.floor(y)
+    z0 = torch.floor(z)
+    tx, ty, tz = x - x0, y - y0, z - z0
+    x0 = x0.long(); y0 = y0.long(); z0 = z0.long()
+    flat = grid.reshape(-1)
+    out = torch.zeros_like(fx)
+    for dx in (0, 1):
+        wx = tx if dx else 1 - tx
+        ix = (x0 + dx) % n
+        for dy in (0, 1):
+            wxy = wx * (ty if dy else 1 - ty)
+            iy = (y0 + dy) % n
+            for dz in (0, 1):
+                w = wxy * (tz if dz else 1 - tz)
+                iz = (z0 + dz) % n
+                out += w * flat[(ix * n + iy) * n + iz]
+    return out
+
+
+def _part1by3_t(v):
+    v = v & 0xFFFF
+    v = (v | (v << 24)) & 0x000000FF000000FF
+    v = (v | (v << 12)) & 0x000F000F000F000F
+    v = (v | (v << 6)) & 0x0303030303030303
+    v = (v | (v << 3)) & 0x1111111111111111
+    return v
+
+
+def lightcone_tree(base_input_level, box_mpch, sigma, threshold, max_input_level, seed, h=0.7, omega_m=0.3,
+                   rms=1.0e-5, noise=0.0, decay=0.0, table=None, device=None, verbose=False):
+    """Synthetic lightcone octree of the benchmark configurations.
+
+    phi: Gaussian random field with P(k) ~ k^-3 on the base grid, rms phi/c^2 =
+    rms; grad phi by 2nd-order periodic central differences; density lognormal
+    from the same field, rho = exp(sigma g - sigma^2/2) with g the unit-variance
+    field; a(cell) = a(chi = |x| L) along the background null ray.  Base cells
+    with rho - 1 > threshold are split, and each child again while its own
+    overdensity (trilinear log-density plus `noise` x unit-variance noise, less
+    `decay` per level below the base) exceeds the threshold, down to max_input_level.  Parents are kept.
+    Refined cells take phi and grad phi trilinearly interpolated from the base
+    grid.  Runs on `device` (torch; default: the GPU when present).
+    Returns (records, fields64 in record order, length in metres)."""
+    import time
+
+    import torch
+    t0 = time.time()
+    if device is None:
+        device = "cuda" if torch.cuda.is_available() else "cpu"
+    c2 = C_LIGHT ** 2
+    mpc = MPC_SI
+    length = box_mpch * mpc / h
+    n = 1 << base_input_level
+    g = torch.from_numpy(grf_potential(n, seed, rms=1.0)).to(device)
+    phi_grid = g * (rms * c2)
+    dxm = length / n
+    gx, gy, gz = ((torch.roll(phi_grid, -1, ax) - torch.roll(phi_grid, 1, ax)) / (2 * dxm) for ax in range(3))
+    lg = sigma * g - 0.5 * sigma * sigma   # log density
+    if table is None:
+        table = flat_lcdm_table(h=h, omega_m=omega_m, a_min=0.05, n=4096)
+    chi_t = torch.from_numpy(np.ascontiguousarray(C_LIGHT * table[0])).to(device)
+    a_t = torch.from_numpy(np.ascontiguousarray(table[1])).to(device)
+    H0 = h * 1.0e5 / mpc
+    lbase = base_input_level + 2
+    off = (1 << lbase) * 3 // 8   # the base grid spans [-0.5, 0.5) of the octree's [-2, 2)
+    levels, kxs, kys, kzs, fields = [], [], [], [], []
+
+    def emit(lvl, kx, ky, kz, phi, fx, fy, fz, logrho):
+        den = float(1 << (lvl + 1))
+        cx = -2.0 + 4.0 * ((2 * kx.double() + 1) / den)
+        cy = -2.0 + 4.0 * ((2 * ky.double() + 1) / den)
+        cz = -2.0 + 4.0 * ((2 * kz.double() + 1) / den)
+        r = torch.sqrt(cx * cx + cy * cy + cz * cz) * length
+        idx = torch.searchsorted(chi_t, r).clamp(1, chi_t.numel() - 1)
+        w = (r - chi_t[idx - 1]) / (chi_t[idx] - chi_t[idx - 1])
+        a = a_t[idx - 1] + w * (a_t[idx] - a_t[idx - 1])
+        f = torch.stack([torch.exp(logrho), phi, fx, fy, fz, a, -(1.0 - a) * H0 * phi], dim=1)
+        levels.append(torch.full_like(kx, lvl))
+        kxs.append(kx); kys.append(ky); kzs.append(kz); fields.append(f)
+
+    ax = torch.arange(n, device=device, dtype=torch.int64) + off
+    bx, by, bz = (v.reshape(-1) for v in torch.meshgrid(ax, ax, ax, indexing="ij"))
+    emit(lbase, bx, by, bz, phi_grid.reshape(-1), gx.reshape(-1), gy.reshape(-1), gz.reshape(-1), lg.reshape(-1))
+    sel = torch.nonzero(torch.exp(lg.reshape(-1)) - 1.0 > threshold).squeeze(1)
+    px, py, pz = bx[sel], by[sel], bz[sel]
+    for il in range(base_input_level + 1, max_input_level + 1):
+        lvl = il + 2
+        s = torch.arange(8, device=device)
+        kx = (2 * px[:, None] + (s & 1)[None, :]).reshape(-1)
+        ky = (2 * py[:, None] + ((s >> 1) & 1)[None, :]).reshape(-1)
+        kz = (2 * pz[:, None] + ((s >> 2) & 1)[None, :]).reshape(-1)
+        scale = float(1 << (lvl - lbase))
+        fx = (kx.double() + 0.5) / scale - off
+        fy = (ky.double() + 0.5) / scale - off
+        fz = (kz.double() + 0.5) / scale - off
+        phi = _trilinear_t(phi_grid, fx, fy, fz)
+        gxi = _trilinear_t(gx, fx, fy, fz)
+        gyi = _trilinear_t(gy, fx, fy, fz)
+        gzi = _trilinear_t(gz, fx, fy, fz)
+        lr = _trilinear_t(lg, fx, fy, fz) + noise * _hash_uniform(kx, ky, kz, lvl, seed)
+        lsel = lr - decay * (il - base_input_level)
+        del fx, fy, fz
+        emit(lvl, kx, ky, kz, phi, gxi, gyi, gzi, lr)
+        sel = torch.nonzero(torch.exp(lsel) - 1.0 > threshold).squeeze(1)
+        px, py, pz = kx[sel], ky[sel], kz[sel]
+        if verbose:
+            print(f"  level {il}: {kx.numel()} cells, {sel.numel()} refined ({time.time() - t0:.1f}s)", flush=True)
+    level = torch.cat(levels)
+    kx = torch.cat(kxs); ky = torch.cat(kys); kz = torch.cat(kzs)
+    f = torch.cat(fields)
+    del levels, kxs, kys, kzs, fields
+    if int(level.max()) > 16:
+        raise ValueError("lightcone_tree builds trees up to octree level 16")
+    code = _part1by3_t(kx) | (_part1by3_t(ky) << 1) | (_part1by3_t(kz) << 2)
+    # key = (interleaved bits | marker bit of every nibble) << (64 - 4 level)
+    allm = torch.tensor(np.uint64(0x8888888888888888).view(np.int64), device=device)
+    nib = 4 * level
+    mask = torch.where(level >= 16, torch.full_like(level, -1), (torch.ones_like(level) << nib.clamp(max=63)) - 1)
+    key = torch.bitwise_left_shift(code | (allm & mask), (64 - nib).clamp(min=0, max=63))
+    # unsigned order of keys whose top bit is set: flip it and sort as signed
+    key_signed = key ^ torch.tensor(-0x8000000000000000, device=device, dtype=torch.int64)
+    order = torch.argsort(key_signed, stable=True)
+    key = key[order]
+    f = f[order]
+    rec = np.zeros(key.numel(), dtype=formats.RECORD_DTYPE)
+    rec["hi"] = key.cpu().numpy().view(np.uint64)
+    f64 = f.cpu().numpy()
+    for j, name in enumerate(formats.FIELDS7):
+        rec[name] = f64[:, j].astype(np.float32)
+    if verbose:
+        print(f"  tree: {rec.size} cells in {time.time() - t0:.1f}s", flush=True)
+    return rec, f64, length
+
+
+WORKLOADS = {
+    # uniform level-7 grid, CIC, 64x64 rays over 5x5 deg to z = 0.5
+    "cfg0": dict(base_input_level=7, box_mpch=3000.0, sigma=1.0, threshold=1e30, max_input_level=7, seed=42,
+                 order=1, nside=64, aperture=5.0, z_stop=0.5, phi0=20.0, theta0=20.0),
+    # level-8 base refined to level 11 where the overdensity exceeds 8, TSC,
+    # 1024x1024 rays over 10x10 deg to z = 2
+    "cfg1": dict(base_input_level=8, box_mpch=8000.0, sigma=1.5, threshold=8.0, max_input_level=11, seed=42,
+                 noise=1.0, decay=0.75, order=2, nside=1024, aperture=10.0, z_stop=2.0, phi0=20.0, theta0=20.0),
+}
