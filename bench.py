@@ -28,6 +28,19 @@ B_STEP_SPLIT_TSC = 540  # 27 cells x 5 fields x 4 bytes per RK4 step
 B_STEP_FP64_TSC = 1080  # 27 cells x 5 fields x 8 bytes
 
 
+def profiled_traffic(name="integrate_split_tsc_cfg1.txt"):
+    """HBM bytes per launch of the split kernel from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (calibrated with a copy kernel), or None."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
@@ -84,6 +97,8 @@ def main():
     ap.add_argument("--workload", default="cfg1")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-tier comparison")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
+                    help="only run the plain copy kernel over MB megabytes (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards by rays or "
@@ -93,6 +108,16 @@ def main():
     # libmpf's HIP runtime (ROCm) is loaded before torch, which the workload
     # generator uses on the CPU only
     capi.load_library()
+    if args.calibrate_copy:
+        ctx = capi.Context(1.0, device=0)
+        nb = args.calibrate_copy << 20
+        src, dst = ctx.dev_alloc(nb), ctx.dev_alloc(nb)
+        ms = [ctx.dev_copy(dst, src, nb) for _ in range(3)]
+        ctx.dev_free(src)
+        ctx.dev_free(dst)
+        ctx.close()
+        print(json.dumps({"copy_bytes": nb, "kernel_ms": ms, "GB_per_s_read_plus_write": 2 * nb / (min(ms) / 1e3) / 1e9}))
+        return
 
     w, rec, f64, length, table, in9 = build_workload(args.workload)
     order = w["order"]
@@ -143,7 +168,8 @@ def main():
     bstep = B_STEP_SPLIT_TSC if order == 2 else 320 // 2
     achieved = kernel_rate * bstep
     result["roofline"] = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                          "frac": achieved / HBM_PEAK, "traffic": None,
+                          "frac": achieved / HBM_PEAK,
+                          "traffic": profiled_traffic() if order == 2 else None,
                           "bytes_per_step": bstep, "kernel_ms": kms}
     if not args.no_fp64:
         ctx.set_precision(capi.PREC_FP64)

@@ -127,6 +127,11 @@ int mpf_dev_alloc(mpf_ctx* ctx, uint64_t bytes, void** dev_ptr);
 int mpf_dev_free(mpf_ctx* ctx, void* dev_ptr);
 int mpf_dev_upload(mpf_ctx* ctx, void* dev_dst, const void* host_src, uint64_t bytes);
 int mpf_dev_download(mpf_ctx* ctx, void* host_dst, const void* dev_src, uint64_t bytes);
+/* Device-to-device copy with a plain streaming kernel; its duration lands
+ * in mpf_last_run.  A kernel of known traffic, used to calibrate HBM byte
+ * counters (FETCH_SIZE / WRITE_SIZE) against the integration kernels.  No
+ * counterpart in the reference. */
+int mpf_dev_copy(mpf_ctx* ctx, void* dev_dst, const void* dev_src, uint64_t bytes);
 /* Accepted steps and duration (HIP events on the context stream) of the
  * last integration kernel. */
 int mpf_last_run(mpf_ctx* ctx, uint64_t* total_steps, double* kernel_ms);

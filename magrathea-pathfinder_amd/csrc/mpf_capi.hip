@@ -23,6 +23,11 @@ __global__ void fill_u64_kernel(unsigned long long* p, uint64_t n, unsigned long
     if (i < n) p[i] = v;
 }
 
+__global__ void __launch_bounds__(256) copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t n) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
 template <typename T>
 struct DevBuf {
     T* p = nullptr;
@@ -373,6 +378,24 @@ int mpf_dev_download(mpf_ctx* c, void* host_dst, const void* dev_src, uint64_t b
     HIPCHK(c, hipMemcpyAsync(host_dst, dev_src, bytes, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
+}
+
+int mpf_dev_copy(mpf_ctx* c, void* dev_dst, const void* dev_src, uint64_t bytes) {
+    if (!c || (bytes && (!dev_dst || !dev_src))) return ERR_INVAL;
+    if (bytes % 16) return fail(c, ERR_INVAL, "copy size must be a multiple of 16 bytes");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    const uint64_t n = bytes / 16;
+    const unsigned grid = (unsigned)std::min<uint64_t>((n + 255) / 256, (uint64_t)c->cus * 16);
+    HIPCHK(c, mpf::launch_fill_u64(c->counters.p, 2, 0ull, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    if (n) {
+        hipLaunchKernelGGL(copy_kernel, dim3(grid), dim3(256), 0, c->stream, static_cast<const uint4*>(dev_src),
+                           static_cast<uint4*>(dev_dst), n);
+        HIPCHK(c, hipGetLastError());
+    }
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    c->pending = true;
+    return mpf_synchronize(c);
 }
 
 int mpf_synchronize(mpf_ctx* c) {

@@ -29,7 +29,7 @@ OUT = {name: i for i, name in enumerate(OUT_FIELDS)}
 EXPORTS = ("mpf_create", "mpf_destroy", "mpf_last_error", "mpf_set_precision", "mpf_load_octree_records",
            "mpf_load_octree_fields64", "mpf_octree_info", "mpf_set_cosmology", "mpf_locate", "mpf_interpolate",
            "mpf_integrate", "mpf_integrate_dev", "mpf_last_run", "mpf_synchronize", "mpf_validate_octree",
-           "mpf_dev_alloc", "mpf_dev_free", "mpf_dev_upload", "mpf_dev_download")
+           "mpf_dev_alloc", "mpf_dev_free", "mpf_dev_upload", "mpf_dev_download", "mpf_dev_copy")
 
 
 class MpfError(RuntimeError):
@@ -77,6 +77,7 @@ def load_library(path=LIB_PATH):
     lib.mpf_dev_free.argtypes = [vp, vp]
     lib.mpf_dev_upload.argtypes = [vp, vp, vp, u64]
     lib.mpf_dev_download.argtypes = [vp, vp, vp, u64]
+    lib.mpf_dev_copy.argtypes = [vp, vp, vp, u64]
     _lib = lib
     return lib
 
@@ -224,6 +225,12 @@ class Context:
         self._check(self.lib.mpf_dev_download(self.h, _p(a), ctypes.c_void_p(ptr), ctypes.c_uint64(a.nbytes)),
                     "mpf_dev_download")
         return a
+
+    def dev_copy(self, dst, src, nbytes):
+        """Plain streaming copy kernel (counter calibration); returns its duration in ms."""
+        self._check(self.lib.mpf_dev_copy(self.h, ctypes.c_void_p(dst), ctypes.c_void_p(src),
+                                          ctypes.c_uint64(int(nbytes))), "mpf_dev_copy")
+        return self.last_run()[1]
 
     def synchronize(self):
         self._check(self.lib.mpf_synchronize(self.h), "mpf_synchronize")
