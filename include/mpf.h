@@ -120,6 +120,35 @@ int mpf_integrate(mpf_ctx* ctx, const double* in9, uint64_t n, const mpf_stop* s
  * context stream unless sync != 0. */
 int mpf_integrate_dev(mpf_ctx* ctx, const double* dev_in9, uint64_t n, const mpf_stop* stop, double* dev_out,
                       int sync);
+/* Constant-parameter surfaces: the stop criterion of hmaps (`stop_ray`,
+ * surfaces `interpRefvec`, hmaps.cpp:182, hmaps.h:610-660) and of the
+ * catalogue root-finder (radius = source distance, catalogues.h:431-456).
+ * The ray is integrated with stop criterion `kind` at value[n-1]; at each
+ * surface the quantities are interpolated linearly between the last element
+ * before it and the next one, and, when `jacobian` is set, the infinitesimal
+ * lensing Jacobian (lensing.h:399-480) is evaluated at the surface distance. */
+#define MPF_MAX_SURFACES 16
+typedef struct {
+    int kind;          /* MPF_STOP_RADIUS, _LAMBDA, _REDSHIFT, _T, _A or _S */
+    uint32_t n;        /* number of surfaces, 1..MPF_MAX_SURFACES */
+    double value[MPF_MAX_SURFACES]; /* increasing (decreasing for _A) */
+    int jacobian;      /* also compute a11, a12, a22 */
+} mpf_surfaces;
+/* Per ray and surface, MPF_NSURF doubles: */
+enum {
+    MPF_SURF_VALID, MPF_SURF_FIRSTID, MPF_SURF_F, MPF_SURF_X, MPF_SURF_Y, MPF_SURF_Z, MPF_SURF_DIST,
+    MPF_SURF_CHI, MPF_SURF_LAMBDA, MPF_SURF_T, MPF_SURF_A, MPF_SURF_REDSHIFT, MPF_SURF_S, MPF_SURF_ISW,
+    MPF_SURF_RHO, MPF_SURF_PHI, MPF_SURF_RHOMAX, MPF_SURF_KT, MPF_SURF_KX, MPF_SURF_KY, MPF_SURF_KZ,
+    MPF_SURF_A11, MPF_SURF_A12, MPF_SURF_A22, MPF_NSURF = 32
+};
+/* out (n x MPF_NOUT, may be NULL): as mpf_integrate; out_surf: n x surf->n x MPF_NSURF.
+ * VALID is 0 where the reference rejects the pixel or source (firstid 0 or the
+ * last element, hmaps.h:716). */
+int mpf_integrate_surfaces(mpf_ctx* ctx, const double* in9, uint64_t n, const mpf_surfaces* surf, double* out,
+                           double* out_surf);
+int mpf_integrate_surfaces_dev(mpf_ctx* ctx, const double* dev_in9, uint64_t n, const mpf_surfaces* surf,
+                               double* dev_out, double* dev_out_surf, int sync);
+
 /* Device buffers on the context's device, for callers that keep inputs and
  * outputs resident between calls (mpf_integrate_dev).  Copies are
  * synchronous. */

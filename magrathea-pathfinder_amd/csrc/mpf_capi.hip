@@ -82,7 +82,7 @@ struct mpf_ctx {
     int ncos = 0;
     // run state
     DevBuf<unsigned long long> counters;  // [0] queue, [1] steps
-    DevBuf<double> io_in, io_out, scratch;
+    DevBuf<double> io_in, io_out, io_surf, scratch;
     DevBuf<int64_t> io_i64;
     DevBuf<uint32_t> io_u32;
     uint64_t last_steps = 0;
@@ -140,7 +140,7 @@ int mpf_destroy(mpf_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->base.release(); c->child_first.release(); c->elem.release(); c->child_mask.release();
     c->f32.release(); c->f64.release(); c->cosmo.release(); c->counters.release();
-    c->io_in.release(); c->io_out.release(); c->scratch.release(); c->io_i64.release(); c->io_u32.release();
+    c->io_in.release(); c->io_out.release(); c->io_surf.release(); c->scratch.release(); c->io_i64.release(); c->io_u32.release();
     mpf::split_release(c->split);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -293,8 +293,35 @@ int mpf_interpolate(mpf_ctx* c, int order, const double* xyz, uint64_t n, double
     return 0;
 }
 
+static int integrate_impl(mpf_ctx* c, const double* dev_in9, uint64_t n, const mpf_stop* stop, double* dev_out,
+                          const mpf_surfaces* surf, double* dev_surf, int sync);
+
 int mpf_integrate_dev(mpf_ctx* c, const double* dev_in9, uint64_t n, const mpf_stop* stop, double* dev_out, int sync) {
-    if (!c || !stop || (n && (!dev_in9 || !dev_out))) return ERR_INVAL;
+    return integrate_impl(c, dev_in9, n, stop, dev_out, nullptr, nullptr, sync);
+}
+
+int mpf_integrate_surfaces_dev(mpf_ctx* c, const double* dev_in9, uint64_t n, const mpf_surfaces* surf,
+                               double* dev_out, double* dev_out_surf, int sync) {
+    if (!c || !surf) return ERR_INVAL;
+    if (surf->n < 1 || surf->n > MPF_MAX_SURFACES) return fail(c, ERR_INVAL, "surface count must be 1..16");
+    if (surf->kind < 0 || surf->kind > 6 || surf->kind == MPF_STOP_PLANE)
+        return fail(c, ERR_INVAL, "surface kind must be radius, lambda, redshift, t, a or s");
+    for (uint32_t j = 1; j < surf->n; ++j) {
+        const bool ok = (surf->kind == MPF_STOP_A) ? (surf->value[j] < surf->value[j - 1]) : (surf->value[j] > surf->value[j - 1]);
+        if (!ok) return fail(c, ERR_INVAL, "surface values must be strictly monotonic (decreasing for a)");
+    }
+    if (n && !dev_out_surf) return ERR_INVAL;
+    mpf_stop stop{surf->kind, surf->value[surf->n - 1], {0.0, 0.0, 0.0}};
+    return integrate_impl(c, dev_in9, n, &stop, dev_out, surf, dev_out_surf, sync);
+}
+
+static int integrate_impl(mpf_ctx* c, const double* dev_in9, uint64_t n, const mpf_stop* stop, double* dev_out,
+                          const mpf_surfaces* surf, double* dev_surf, int sync) {
+    if (!c || !stop || (n && !dev_in9)) return ERR_INVAL;
+    if (n && !dev_out) {
+        if (c->io_out.ensure((uint64_t)MPF_NOUT * n) != hipSuccess) return fail(c, ERR_NOMEM, "output buffer");
+        dev_out = c->io_out.p;
+    }
     if (!c->have_tree) return fail(c, ERR_STATE, "no octree loaded");
     if (c->ncos <= 0) return fail(c, ERR_STATE, "no cosmology table set");
     if (stop->kind < -1 || stop->kind > 6) return fail(c, ERR_INVAL, "unknown stop criterion");
@@ -315,10 +342,19 @@ int mpf_integrate_dev(mpf_ctx* c, const double* dev_in9, uint64_t n, const mpf_s
     A.stop_kind = stop->kind;
     A.stop_value = stop->value;
     for (int k = 0; k < 3; ++k) A.kit[k] = stop->kitarget[k];
+    if (surf) {
+        A.surf_kind = surf->kind;
+        A.nsurf = (int)surf->n;
+        for (uint32_t j = 0; j < surf->n; ++j) A.surf_val[j] = surf->value[j];
+        A.surf_out = dev_surf;
+        A.jac = surf->jacobian ? 1 : 0;
+    }
     HIPCHK(c, mpf::launch_fill_u64(c->counters.p, 2, 0ull, c->stream));
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     if (n) {
         if (c->cfg.precision == MPF_PREC_SPLIT) {
+            if (A.nsurf && !mpf::split_supports_surfaces())
+                return fail(c, ERR_INVAL, "surfaces are not available in the split tier");
             if (!c->split_ready) {
                 std::string err;
                 if (!mpf::split_build(c->split, c->dt, c->lev.data(), c->kx.data(), c->ky.data(), c->kz.data(),
@@ -335,6 +371,7 @@ int mpf_integrate_dev(mpf_ctx* c, const double* dev_in9, uint64_t n, const mpf_s
             const int grid = (int)std::min<uint64_t>(blocks, (uint64_t)c->cus * 4);
             HIPCHK(c, mpf::launch_integrate_exact(c->cfg.precision, A, c->dt, grid, c->stream));
         }
+        if (A.nsurf) HIPCHK(c, mpf::launch_surface_final(A, c->stream));
     }
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     c->pending = true;
@@ -420,6 +457,24 @@ int mpf_last_run(mpf_ctx* c, uint64_t* total_steps, double* kernel_ms) {
     if (rc) return rc;
     if (total_steps) *total_steps = c->last_steps;
     if (kernel_ms) *kernel_ms = c->last_ms;
+    return 0;
+}
+
+int mpf_integrate_surfaces(mpf_ctx* c, const double* in9, uint64_t n, const mpf_surfaces* surf, double* out,
+                           double* out_surf) {
+    if (!c || !surf || (n && (!in9 || !out_surf))) return ERR_INVAL;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    const uint64_t ns = surf->n;
+    if (ns < 1 || ns > MPF_MAX_SURFACES) return fail(c, ERR_INVAL, "surface count must be 1..16");
+    if (c->io_in.ensure(9 * std::max<uint64_t>(n, 1)) != hipSuccess ||
+        c->io_out.ensure((uint64_t)MPF_NOUT * std::max<uint64_t>(n, 1)) != hipSuccess ||
+        c->io_surf.ensure((uint64_t)MPF_NSURF * ns * std::max<uint64_t>(n, 1)) != hipSuccess)
+        return fail(c, ERR_NOMEM, "integration buffers");
+    if (n) HIPCHK(c, hipMemcpyAsync(c->io_in.p, in9, 9 * n * 8, hipMemcpyHostToDevice, c->stream));
+    int rc = mpf_integrate_surfaces_dev(c, c->io_in.p, n, surf, c->io_out.p, c->io_surf.p, 1);
+    if (rc) return rc;
+    if (n && out) HIPCHK(c, hipMemcpy(out, c->io_out.p, (uint64_t)MPF_NOUT * n * 8, hipMemcpyDeviceToHost));
+    if (n) HIPCHK(c, hipMemcpy(out_surf, c->io_surf.p, (uint64_t)MPF_NSURF * ns * n * 8, hipMemcpyDeviceToHost));
     return 0;
 }
 

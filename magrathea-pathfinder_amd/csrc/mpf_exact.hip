@@ -68,6 +68,7 @@ __device__ __forceinline__ double level_extent(int lvl) {
 }
 
 struct RayState {
+    SurfAcc surf;
     Rec back, prev;
     double data[7];   // fields at back's position
     double dl, a0, phi0c2;
@@ -123,12 +124,48 @@ __device__ void ray_launch(const IntegrateArgs& A, const DevTree& t, TscCache& c
     s.phi0c2 = phi / c2;
     s.appended = 0;
     s.has_prev = false;
+    if (A.nsurf) surf_init(s.surf, d[F_RHO]);
+}
+
+__device__ __forceinline__ SurfElem surf_elem(const double* r) {
+    SurfElem e;
+    e.x = r[CX]; e.y = r[CY]; e.z = r[CZ]; e.chi = r[MPF_OUT_CHI]; e.lambda = r[MPF_OUT_LAMBDA]; e.t = r[CT];
+    e.a = r[CA]; e.redshift = r[MPF_OUT_REDSHIFT]; e.s = r[MPF_OUT_S]; e.isw = r[MPF_OUT_ISW];
+    e.rho = r[MPF_OUT_RHO]; e.phi = r[MPF_OUT_PHI]; e.kt = r[CKT]; e.kx = r[CKX]; e.ky = r[CKY]; e.kz = r[CKZ];
+    return e;
+}
+
+// tidal components at an element from 6 TSC samples of grad phi (lensing.h:421-448)
+template <int P>
+__device__ void element_tidal(const DevTree& t, const double* r, double* T) {
+    const double x = r[CX], y = r[CY], z = r[CZ];
+    const double dl = jacobian_step((unsigned)r[MPF_OUT_LEVEL]);
+    const double pts[6][3] = {{x - 0.5 * dl, y, z}, {x + 0.5 * dl, y, z}, {x, y - 0.5 * dl, z},
+                              {x, y + 0.5 * dl, z}, {x, y, z - 0.5 * dl}, {x, y, z + 0.5 * dl}};
+    double g[6][3];
+    TscCache cache;
+    cache_init(cache);
+    for (int k = 0; k < 6; ++k) {
+        double d[7];
+        int loc;
+        tsc<P, false>(t, cache, pts[k][0], pts[k][1], pts[k][2], d, &loc);
+        g[k][0] = d[F_DX]; g[k][1] = d[F_DY]; g[k][2] = d[F_DZ];
+    }
+    if (P == 0) {
+        // Gravity<float> members: the differences are FP32 subtractions
+        for (int k = 0; k < 6; k += 2)
+            for (int c = 0; c < 3; ++c) {
+                g[k + 1][c] = (double)((float)g[k + 1][c] - (float)g[k][c]);
+                g[k][c] = 0.0;
+            }
+    }
+    tidal_from_samples(x, y, z, r[MPF_OUT_CHI], dl, g, T);
 }
 
 // One iteration of the integration loop (integrator.h:683-812).  Returns
 // true while the ray continues.
 template <int P>
-__device__ bool ray_step(const IntegrateArgs& A, const DevTree& t, TscCache& cache, RayState& s) {
+__device__ bool ray_step(const IntegrateArgs& A, const DevTree& t, TscCache& cache, RayState& s, uint64_t ray) {
     const double zero = 0.0, one = 1.0, two = 2.0, six = 6.0;
     const double c = C_LIGHT, c2 = C2;
     const double min = 0.0 - (EXTENT / two), max = 0.0 + (EXTENT / two);
@@ -222,6 +259,12 @@ __device__ bool ray_step(const IntegrateArgs& A, const DevTree& t, TscCache& cac
         s.back = ph;
         s.has_prev = true;
         s.appended++;
+        if (A.nsurf) {
+            double T[3] = {0.0, 0.0, 0.0};
+            if (A.jac) element_tidal<P>(t, s.back.v, T);
+            surf_append(s.surf, A.surf_kind, A.nsurf, A.surf_val, A.surf_out + (uint64_t)A.nsurf * SF_NREC * ray,
+                        s.appended, surf_elem(s.prev.v), surf_elem(s.back.v), T, A.jac != 0);
+        }
     }
     if (A.stop_kind >= 0) {
         const double* b = s.back.v;
@@ -241,10 +284,31 @@ __device__ bool ray_step(const IntegrateArgs& A, const DevTree& t, TscCache& cac
     return cont;
 }
 
+__device__ __forceinline__ void surf_finish(const IntegrateArgs& A, const RayState& s, uint64_t ray,
+                                            uint32_t size_minus_one) {
+    // surfaces never passed point past the end (firstid = size - 1), and a
+    // surface whose next element was popped ends on the last element: both
+    // are rejected (hmaps.h:716)
+    double* rec = A.surf_out + (uint64_t)A.nsurf * SF_NREC * ray;
+    for (int j = 0; j < A.nsurf; ++j) {
+        double* o = rec + (uint64_t)SF_NREC * j;
+        if (!(s.surf.crossed & (1u << j))) {
+            for (int k = 0; k < SF_NREC; ++k) o[k] = 0.0;
+            o[SF_FIRSTID] = (double)size_minus_one;
+        } else if (o[SF_FIRSTID] >= (double)size_minus_one) {
+            o[SF_VALID] = 0.0;
+        }
+    }
+}
+
 __device__ __forceinline__ void ray_finish(const IntegrateArgs& A, RayState& s, uint64_t ray) {
     // erase the last element if its redshift or a is negative (integrator.h:816-821)
     const Rec* r = &s.back;
     uint32_t steps = s.appended;
+    if (A.nsurf) {
+        const bool pop = signbit(s.back.v[MPF_OUT_REDSHIFT]) || signbit(s.back.v[CA]);
+        surf_finish(A, s, ray, (pop && s.appended) ? s.appended - 1 : s.appended);
+    }
     if (signbit(s.back.v[MPF_OUT_REDSHIFT]) || signbit(s.back.v[CA])) {
         if (s.has_prev) {
             r = &s.prev;
@@ -298,7 +362,7 @@ __global__ void __launch_bounds__(256) integrate_exact_kernel(IntegrateArgs A, D
             continue;
         }
         if (active) {
-            if (!ray_step<P>(A, t, cache, s)) {
+            if (!ray_step<P>(A, t, cache, s, ray)) {
                 active = false;
                 ray_finish(A, s, ray);
             }
@@ -346,7 +410,31 @@ __global__ void interp_seq_kernel(DevTree t, const double* xyz, uint64_t n, doub
     }
 }
 
+// a_c = S1 - S2 / D at each valid surface, then A = 1 - 2 a L / c^2 (lensing.h:470-477)
+__global__ void surface_final_kernel(IntegrateArgs A) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n * (uint64_t)A.nsurf) return;
+    double* o = A.surf_out + (uint64_t)SF_NREC * i;
+    if (!A.jac || o[SF_VALID] == 0.0) {
+        o[SF_A11] = 1.0; o[SF_A12] = 0.0; o[SF_A22] = 1.0;
+        return;
+    }
+    const double D = o[SF_DIST];
+    double a[3];
+    for (int c = 0; c < 3; ++c) a[c] = o[SF_S1 + c] - o[SF_S1 + 3 + c] / D;
+    o[SF_A11] = 1 - 2 * a[0] / C2 * A.length;
+    o[SF_A12] = -2 * a[1] / C2 * A.length;
+    o[SF_A22] = 1 - 2 * a[2] / C2 * A.length;
+}
+
 }  // namespace
+
+hipError_t launch_surface_final(const IntegrateArgs& A, hipStream_t st) {
+    const uint64_t m = A.n * (uint64_t)A.nsurf;
+    if (m == 0) return hipSuccess;
+    hipLaunchKernelGGL(surface_final_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, A);
+    return hipGetLastError();
+}
 
 hipError_t launch_integrate_exact(int precision, const IntegrateArgs& A, const DevTree& t, int grid, hipStream_t st) {
     if (precision == 0) hipLaunchKernelGGL(integrate_exact_kernel<0>, dim3(grid), dim3(256), 0, st, A, t);

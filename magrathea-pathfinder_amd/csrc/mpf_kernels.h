@@ -5,6 +5,7 @@
 
 #include "../../include/mpf.h"
 #include "mpf_device.h"
+#include "mpf_surfaces.h"
 
 namespace mpf {
 
@@ -25,6 +26,12 @@ struct IntegrateArgs {
     double stop_value = 0.0;
     double kit[3] = {0.0, 0.0, 0.0};
     double* scratch = nullptr;       // per-ray launch records (split tier)
+    // constant-parameter surfaces (hmaps.h:615-660) and the infinitesimal Jacobian
+    int surf_kind = 0;               // 0 r, 1 lambda, 2 redshift, 3 t, 4 a, 6 s
+    int nsurf = 0;
+    double surf_val[16] = {0};
+    double* surf_out = nullptr;      // n x nsurf x SF_NREC
+    int jac = 0;
 };
 
 hipError_t launch_integrate_exact(int precision, const IntegrateArgs& A, const DevTree& t, int grid, hipStream_t st);
@@ -32,6 +39,7 @@ hipError_t launch_locate(const DevTree& t, const double* xyz, uint64_t n, int64_
                          hipStream_t st);
 hipError_t launch_interp(int precision, const DevTree& t, int order, const double* xyz, uint64_t n, double* out7,
                          hipStream_t st);
+hipError_t launch_surface_final(const IntegrateArgs& A, hipStream_t st);
 hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v, hipStream_t st);
 
 }  // namespace mpf

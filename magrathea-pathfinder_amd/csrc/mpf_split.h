@@ -19,6 +19,7 @@ struct SplitTree {
 bool split_build(SplitTree& s, const DevTree& t, const uint8_t* level, const uint32_t* kx, const uint32_t* ky,
                  const uint32_t* kz, int order, hipStream_t st, std::string& err);
 void split_release(SplitTree& s);
+bool split_supports_surfaces();
 hipError_t launch_integrate_split(const IntegrateArgs& A, const DevTree& t, const SplitTree& s, int cus, hipStream_t st);
 
 }  // namespace mpf

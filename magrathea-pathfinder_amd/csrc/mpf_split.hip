@@ -25,6 +25,7 @@
 #include "mpf_split.h"
 
 #include "mpf_exact_dev.h"
+#include "mpf_surfaces.h"
 
 #include <algorithm>
 #include <cstdlib>
@@ -339,7 +340,7 @@ __global__ void __launch_bounds__(256) final_kernel(IntegrateArgs A, DevTree t) 
     o[MPF_OUT_AH] = o[MPF_OUT_NSTEPS] > 0.0 ? d[F_A] : 0.0;
 }
 
-template <bool RC, int W>
+template <bool RC, int W, bool SURF>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8)))
 integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
 #pragma clang fp contract(fast)
@@ -350,6 +351,10 @@ integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
     I.valid = false;
     I.slot = NIL;
     SplitRay R;
+    SurfAcc SA;
+    Interp<RC> J;   // second cell cache for the Jacobian samples
+    J.valid = false;
+    J.slot = NIL;
     bool active = false, drained = false;
     uint64_t ray = 0;
     const unsigned lane = __lane_id();
@@ -380,6 +385,7 @@ integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
                     R.dt = (float)L[14];
                     R.F[0] = R.dt; R.F[1] = R.dx; R.F[2] = R.dy; R.F[3] = R.dz;
                     R.appended = 0;
+                    if (SURF) surf_init(SA, 0.0);
                     const unsigned l0 = (unsigned)L[15];
                     R.dl = level_extent(l0) * (R.core[CA] * R.core[CA] * ratio_k);
                     I.valid = false;
@@ -461,6 +467,37 @@ integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
                 o[MPF_OUT_LEVEL] = (double)(I.valid ? I.lvl : 0u);
             }
             (void)provisional;
+            if (SURF) {
+                SurfElem pv, nw;
+                pv.x = R.core[CX]; pv.y = R.core[CY]; pv.z = R.core[CZ]; pv.chi = R.chi; pv.lambda = R.lambda;
+                pv.t = R.core[CT]; pv.a = R.core[CA]; pv.redshift = R.redshift; pv.s = R.s; pv.isw = R.isw;
+                pv.rho = 0.0; pv.phi = R.phi; pv.kt = R.core[CKT]; pv.kx = R.core[CKX]; pv.ky = R.core[CKY];
+                pv.kz = R.core[CKZ];
+                nw.x = p[CX]; nw.y = p[CY]; nw.z = p[CZ]; nw.chi = chi; nw.lambda = lambda; nw.t = p[CT];
+                nw.a = p[CA]; nw.redshift = redshift; nw.s = s; nw.isw = isw; nw.rho = 0.0; nw.phi = nphi;
+                nw.kt = p[CKT]; nw.kx = p[CKX]; nw.ky = p[CKY]; nw.kz = p[CKZ];
+                double T[3] = {0.0, 0.0, 0.0};
+                if (A.jac) {
+                    // 6 samples of grad phi at +-dl/2 (lensing.h:429-436), FP32 interpolation
+                    const double jdl = jacobian_step(I.lvl);
+                    double g[6][3];
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) {
+                        const double off = (k & 1) ? 0.5 * jdl : -0.5 * jdl;
+                        const double sx = p[CX] + ((k >> 1) == 0 ? off : 0.0);
+                        const double sy = p[CY] + ((k >> 1) == 1 ? off : 0.0);
+                        const double sz = p[CZ] + ((k >> 1) == 2 ? off : 0.0);
+                        float G[5];
+                        const bool okj = split_eval<4, RC>(t, sv, J, sx, sy, sz, G);
+                        g[k][0] = okj ? (double)G[1] : 0.0;
+                        g[k][1] = okj ? (double)G[2] : 0.0;
+                        g[k][2] = okj ? (double)G[3] : 0.0;
+                    }
+                    tidal_from_samples(p[CX], p[CY], p[CZ], chi, jdl, g, T);
+                }
+                surf_append(SA, A.surf_kind, A.nsurf, A.surf_val, A.surf_out + (uint64_t)A.nsurf * SF_NREC * ray,
+                            R.appended + 1, pv, nw, T, A.jac != 0);
+            }
 #pragma unroll
             for (int i = 0; i < 9; ++i) R.core[i] = p[i];
             R.lambda = lambda; R.s = s; R.isw = isw; R.iswold = iswold; R.redshift = redshift; R.chi = chi;
@@ -510,6 +547,24 @@ integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
                                              (c2 * (1.0 + 2.0 * (phi / c2)) * R.core[CKT] * R.core[CKT]);
                 o[MPF_OUT_LEVEL] = (double)I.lvl;
                 o[MPF_OUT_NSTEPS] = (double)steps;
+            }
+            if (SURF) {
+                // surfaces never reached, or whose next element was popped, are rejected (hmaps.h:716);
+                // rho is not carried by the split tier
+                const bool pop = signbit(R.redshift) || signbit(R.core[CA]);
+                const uint32_t last = (pop && R.appended) ? R.appended - 1 : R.appended;
+                double* rec = A.surf_out + (uint64_t)A.nsurf * SF_NREC * ray;
+                for (int j = 0; j < A.nsurf; ++j) {
+                    double* so = rec + (uint64_t)SF_NREC * j;
+                    if (!(SA.crossed & (1u << j))) {
+                        for (int k = 0; k < SF_NREC; ++k) so[k] = 0.0;
+                        so[SF_FIRSTID] = (double)last;
+                    } else {
+                        if (so[SF_FIRSTID] >= (double)last) so[SF_VALID] = 0.0;
+                        so[SF_RHO] = __builtin_nan("");
+                        so[SF_RHOMAX] = __builtin_nan("");
+                    }
+                }
             }
             atomicAdd(A.steps, (unsigned long long)steps);
         }
@@ -612,6 +667,8 @@ bool split_build(SplitTree& s, const DevTree& t, const uint8_t* level, const uin
     return ok;
 }
 
+bool split_supports_surfaces() { return true; }
+
 void split_release(SplitTree& s) {
     if (s.mem) (void)hipFree(s.mem);
     s = SplitTree();
@@ -641,11 +698,15 @@ hipError_t launch_integrate_split(const IntegrateArgs& A, const DevTree& t, cons
     hipLaunchKernelGGL(launch_kernel, dim3(pgrid), dim3(256), 0, st, A, t);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    switch (split_variant()) {
-        case 1: hipLaunchKernelGGL((integrate_split_kernel<true, 1>), dim3(grid), dim3(256), 0, st, A, t, sv); break;
-        case 2: hipLaunchKernelGGL((integrate_split_kernel<false, 2>), dim3(grid), dim3(256), 0, st, A, t, sv); break;
-        case 4: hipLaunchKernelGGL((integrate_split_kernel<false, 4>), dim3(grid), dim3(256), 0, st, A, t, sv); break;
-        default: hipLaunchKernelGGL((integrate_split_kernel<false, 3>), dim3(grid), dim3(256), 0, st, A, t, sv); break;
+    if (A.nsurf) {
+        hipLaunchKernelGGL((integrate_split_kernel<false, 2, true>), dim3(grid), dim3(256), 0, st, A, t, sv);
+    } else {
+        switch (split_variant()) {
+            case 1: hipLaunchKernelGGL((integrate_split_kernel<true, 1, false>), dim3(grid), dim3(256), 0, st, A, t, sv); break;
+            case 3: hipLaunchKernelGGL((integrate_split_kernel<false, 3, false>), dim3(grid), dim3(256), 0, st, A, t, sv); break;
+            case 4: hipLaunchKernelGGL((integrate_split_kernel<false, 4, false>), dim3(grid), dim3(256), 0, st, A, t, sv); break;
+            default: hipLaunchKernelGGL((integrate_split_kernel<false, 2, false>), dim3(grid), dim3(256), 0, st, A, t, sv); break;
+        }
     }
     e = hipGetLastError();
     if (e != hipSuccess) return e;

@@ -1,0 +1,145 @@
+// Constant-parameter surfaces and the infinitesimal lensing Jacobian,
+// accumulated while a ray is integrated.
+//
+// Hmaps::FillMap (hmaps.h:615-660) and Catalogues::newtonMethod2d
+// (catalogues.h:431-456) locate, on the stored trajectory, the last element
+// before a surface (first element whose parameter passes the surface value,
+// minus one) and interpolate linearly between it and the next element with
+// f = (v[i+1] - ref) / (v[i+1] - v[i]).  Here the crossing is detected when the
+// next element is appended and the interpolated values are written at once.
+//
+// Lensing::dbetadtheta_infinitesimal (lensing.h:399-480) sums, over elements
+// i >= 1 up to the last one with chi <= D,
+//     a_c(D) = sum (D - chi_i) chi_i / D * T_c,i * (chi_i - chi_i-1)
+// with T the angular tidal components (d1d1, d1d2, d2d2) from 6 TSC samples
+// of grad phi at +-1/2 dl around the element.  Writing
+// (D - chi) chi / D = chi - chi^2 / D, two running sums per component,
+// S1 = sum chi T dr and S2 = sum chi^2 T dr, give a_c(D) = S1 - S2 / D for
+// any D known only when the surface is reached.
+#pragma once
+#include "mpf_device.h"
+
+namespace mpf {
+
+constexpr int MAX_SURF = 16;
+
+// per ray and surface output record (doubles)
+enum {
+    SF_VALID = 0, SF_FIRSTID, SF_F, SF_X, SF_Y, SF_Z, SF_DIST, SF_CHI, SF_LAMBDA, SF_T, SF_A, SF_REDSHIFT, SF_S,
+    SF_ISW, SF_RHO, SF_PHI, SF_RHOMAX, SF_KT, SF_KX, SF_KY, SF_KZ, SF_A11, SF_A12, SF_A22,
+    SF_S1 = 24,  // 24..26: S1 snapshot (11, 12, 22); 27..29: S2 snapshot
+    SF_NREC = 32
+};
+
+// the quantities of one trajectory element the surfaces need
+struct SurfElem {
+    double x, y, z, chi, lambda, t, a, redshift, s, isw, rho, phi, kt, kx, ky, kz;
+};
+
+struct SurfAcc {
+    double S1[3], S2[3];
+    double rmax_prev;    // max rho over elements 0 .. i-2
+    double rmax_cur;     // max rho over elements 0 .. i-1
+    uint32_t crossed;    // bit j: surface j passed
+};
+
+__device__ __forceinline__ void surf_init(SurfAcc& s, double rho0) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) { s.S1[c] = 0.0; s.S2[c] = 0.0; }
+    s.rmax_prev = -1.0e300;
+    s.rmax_cur = rho0;
+    s.crossed = 0;
+}
+
+__device__ __forceinline__ double surf_value(int kind, const SurfElem& e) {
+    switch (kind) {
+        case 0: return e.chi;
+        case 1: return e.lambda;
+        case 2: return e.redshift;
+        case 3: return e.t;
+        case 4: return e.a;
+        default: return e.s;
+    }
+}
+
+// Called after element i (>= 1) is appended, with the previous element.
+// `tidal` (d1d1, d1d2, d2d2) of element i is added after the crossings are
+// recorded, so a snapshot holds the sums up to element i-1 = firstid.
+__device__ __forceinline__ void surf_append(SurfAcc& s, int kind, int nsurf, const double* vals, double* rec,
+                                            uint32_t i, const SurfElem& pv, const SurfElem& nw, const double* tidal,
+                                            bool jac) {
+    const double vn = surf_value(kind, nw);
+    const double vp = surf_value(kind, pv);
+    for (int j = 0; j < nsurf; ++j) {
+        if (s.crossed & (1u << j)) continue;
+        const double ref = vals[j];
+        const bool pass = (kind == 4) ? (vn < ref) : (vn > ref);
+        if (!pass) continue;
+        s.crossed |= (1u << j);
+        double* o = rec + (uint64_t)SF_NREC * j;
+        const double f = (vn - ref) / (vn - vp);
+        o[SF_VALID] = (i >= 2) ? 1.0 : 0.0;   // firstid == 0 is rejected (hmaps.h:716)
+        o[SF_FIRSTID] = (double)(i - 1);
+        o[SF_F] = f;
+        o[SF_X] = pv.x * f + nw.x * (1 - f);
+        o[SF_Y] = pv.y * f + nw.y * (1 - f);
+        o[SF_Z] = pv.z * f + nw.z * (1 - f);
+        o[SF_DIST] = sqrt(o[SF_X] * o[SF_X] + o[SF_Y] * o[SF_Y] + o[SF_Z] * o[SF_Z]);
+        o[SF_CHI] = pv.chi * f + nw.chi * (1 - f);
+        o[SF_LAMBDA] = pv.lambda * f + nw.lambda * (1 - f);
+        o[SF_T] = pv.t * f + nw.t * (1 - f);
+        o[SF_A] = pv.a * f + nw.a * (1 - f);
+        o[SF_REDSHIFT] = pv.redshift * f + nw.redshift * (1 - f);
+        o[SF_S] = pv.s * f + nw.s * (1 - f);
+        o[SF_ISW] = pv.isw * f + nw.isw * (1 - f);
+        o[SF_RHO] = pv.rho * f + nw.rho * (1 - f);
+        o[SF_PHI] = pv.phi * f + nw.phi * (1 - f);
+        o[SF_RHOMAX] = s.rmax_prev;
+        o[SF_KT] = pv.kt * f + nw.kt * (1 - f);
+        o[SF_KX] = pv.kx * f + nw.kx * (1 - f);
+        o[SF_KY] = pv.ky * f + nw.ky * (1 - f);
+        o[SF_KZ] = pv.kz * f + nw.kz * (1 - f);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            o[SF_S1 + c] = s.S1[c];
+            o[SF_S1 + 3 + c] = s.S2[c];
+        }
+    }
+    if (jac) {
+        const double dr = nw.chi - pv.chi;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double td = tidal[c] * dr;
+            s.S1[c] += nw.chi * td;
+            s.S2[c] += nw.chi * nw.chi * td;
+        }
+    }
+    s.rmax_prev = s.rmax_cur;
+    s.rmax_cur = (nw.rho > s.rmax_cur) ? nw.rho : s.rmax_cur;
+}
+
+// Angular tidal components from the 6 samples of grad phi (lensing.h:421-448).
+// g[k][d]: sample k in (x-, x+, y-, y+, z-, z+), component d of grad phi.
+// The angles' sines and cosines come from the position itself.
+__device__ __forceinline__ void tidal_from_samples(double x, double y, double z, double r, double dl,
+                                                   const double (*g)[3], double* T) {
+    const double rho = sqrt(x * x + y * y);
+    const double cp = (rho > 0.0) ? x / rho : 1.0, sp = (rho > 0.0) ? y / rho : 0.0;
+    const double ct = z / r, st = rho / r;
+    const double s2p = 2.0 * sp * cp, s2t = 2.0 * st * ct;
+    const double dxxp = (g[1][0] - g[0][0]) / dl;
+    const double dxyp = (g[3][0] - g[2][0]) / dl;
+    const double dxzp = (g[5][0] - g[4][0]) / dl;
+    const double dyyp = (g[3][1] - g[2][1]) / dl;
+    const double dyzp = (g[5][1] - g[4][1]) / dl;
+    const double dzzp = (g[5][2] - g[4][2]) / dl;
+    T[0] = sp * sp * dxxp + cp * cp * dyyp - s2p * dxyp;
+    T[2] = cp * cp * ct * ct * dxxp + sp * sp * ct * ct * dyyp + st * st * dzzp + s2p * ct * ct * dxyp -
+           sp * s2t * dyzp - cp * s2t * dxzp;
+    T[1] = ct * cp * sp * (dyyp - dxxp) + (cp * cp - sp * sp) * ct * dxyp + sp * st * dxzp - cp * st * dyzp;
+}
+
+// dl = 2^-(level - 1 - log2(EXTENT)) (lensing.h:429)
+__device__ __forceinline__ double jacobian_step(unsigned level) { return ldexp(1.0, 3 - (int)level); }
+
+}  // namespace mpf

@@ -29,7 +29,14 @@ OUT = {name: i for i, name in enumerate(OUT_FIELDS)}
 EXPORTS = ("mpf_create", "mpf_destroy", "mpf_last_error", "mpf_set_precision", "mpf_load_octree_records",
            "mpf_load_octree_fields64", "mpf_octree_info", "mpf_set_cosmology", "mpf_locate", "mpf_interpolate",
            "mpf_integrate", "mpf_integrate_dev", "mpf_last_run", "mpf_synchronize", "mpf_validate_octree",
-           "mpf_dev_alloc", "mpf_dev_free", "mpf_dev_upload", "mpf_dev_download", "mpf_dev_copy")
+           "mpf_dev_alloc", "mpf_dev_free", "mpf_dev_upload", "mpf_dev_download", "mpf_dev_copy",
+           "mpf_integrate_surfaces", "mpf_integrate_surfaces_dev")
+
+SURF_FIELDS = ("valid", "firstid", "f", "x", "y", "z", "dist", "chi", "lambda", "t", "a", "redshift", "s", "isw",
+               "rho", "phi", "rhomax", "kt", "kx", "ky", "kz", "a11", "a12", "a22")
+SURF = {name: i for i, name in enumerate(SURF_FIELDS)}
+NSURF = 32
+MAX_SURFACES = 16
 
 
 class MpfError(RuntimeError):
@@ -39,6 +46,24 @@ class MpfError(RuntimeError):
 class Config(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("order", ctypes.c_int), ("precision", ctypes.c_int),
                 ("nsteps", ctypes.c_uint), ("length", ctypes.c_double)]
+
+
+class Surfaces(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("n", ctypes.c_uint32), ("value", ctypes.c_double * 16),
+                ("jacobian", ctypes.c_int)]
+
+
+def make_surfaces(kind, values, jacobian=True):
+    if isinstance(kind, str):
+        kind = STOP_NAMES[kind]
+    values = list(values)
+    s = Surfaces()
+    s.kind = int(kind)
+    s.n = len(values)
+    for i, v in enumerate(values):
+        s.value[i] = float(v)
+    s.jacobian = int(bool(jacobian))
+    return s
 
 
 class Stop(ctypes.Structure):
@@ -78,6 +103,8 @@ def load_library(path=LIB_PATH):
     lib.mpf_dev_upload.argtypes = [vp, vp, vp, u64]
     lib.mpf_dev_download.argtypes = [vp, vp, vp, u64]
     lib.mpf_dev_copy.argtypes = [vp, vp, vp, u64]
+    lib.mpf_integrate_surfaces.argtypes = [vp, vp, u64, ctypes.POINTER(Surfaces), vp, vp]
+    lib.mpf_integrate_surfaces_dev.argtypes = [vp, vp, u64, ctypes.POINTER(Surfaces), vp, vp, i32]
     _lib = lib
     return lib
 
@@ -192,6 +219,24 @@ class Context:
         self._check(self.lib.mpf_integrate(self.h, _p(in9), ctypes.c_uint64(n), ctypes.byref(st), _p(out),
                                            ctypes.byref(steps)), "mpf_integrate")
         return out, steps.value
+
+    def integrate_surfaces(self, in9, kind, values, jacobian=True):
+        """Rays to the last surface; returns (per-ray records, per-ray per-surface records)."""
+        in9 = np.ascontiguousarray(in9, dtype=np.float64).reshape(-1, 9)
+        n = in9.shape[0]
+        sf = make_surfaces(kind, values, jacobian)
+        out = np.zeros((n, NOUT))
+        surf = np.zeros((n, sf.n, NSURF))
+        self._check(self.lib.mpf_integrate_surfaces(self.h, _p(in9), ctypes.c_uint64(n), ctypes.byref(sf), _p(out),
+                                                    _p(surf)), "mpf_integrate_surfaces")
+        return out, surf
+
+    def integrate_surfaces_dev(self, in_ptr, n, out_ptr, surf_ptr, kind, values, jacobian=True, sync=False):
+        sf = make_surfaces(kind, values, jacobian)
+        self._check(self.lib.mpf_integrate_surfaces_dev(self.h, ctypes.c_void_p(in_ptr), ctypes.c_uint64(n),
+                                                        ctypes.byref(sf), ctypes.c_void_p(out_ptr),
+                                                        ctypes.c_void_p(surf_ptr), int(bool(sync))),
+                    "mpf_integrate_surfaces_dev")
 
     def integrate_dev(self, in_ptr, n, out_ptr, stop="none", value=0.0, kitarget=(0.0, 0.0, 0.0), sync=False):
         """Device-resident integration: in_ptr/out_ptr are device addresses (ints)."""

@@ -849,6 +849,122 @@ uint64_t mpfo_jacobian_infinitesimal(const mpfo_octree* o, int order, int precis
     return n;
 }
 
+static double elem_value(const photon* p, int kind) {
+    switch (kind) {
+        case 0: return p->chi;
+        case 1: return p->lambda;
+        case 2: return p->redshift;
+        case 3: return p->core[CT];
+        case 4: return p->core[CA];
+        default: return p->s;
+    }
+}
+
+/* jacobian of lensing.h:399-480 for one distance on a stored trajectory */
+static void jacobian_at(const mpfo_octree* o, int precision, const traj_t* t, double dist, double length, double* a3) {
+    const double c2 = C2;
+    const unsigned levelshift = 2;
+    tsc_cache cache;
+    tsc_cache_init(&cache);
+    double s11 = 0, s12 = 0, s22 = 0;
+    /* firstid = upper_bound(chi, dist) - 1 */
+    uint64_t lo = 0, cnt = t->n;
+    while (cnt > 0) {
+        const uint64_t h = cnt / 2;
+        if (!(dist < t->p[lo + h].chi)) { lo = lo + h + 1; cnt = cnt - h - 1; }
+        else cnt = h;
+    }
+    const uint64_t first = lo - (lo > 0);
+    for (uint64_t i = 1; i <= first && i < t->n; ++i) {
+        const photon* p = &t->p[i];
+        const double x = p->core[CX], y = p->core[CY], z = p->core[CZ], r = p->chi;
+        const double phi = atan2(y, x);
+        const double theta = acos(z / r);
+        const double cp = cos(phi), sp = sin(phi), ct = cos(theta), st = sin(theta);
+        const double s2p = sin(2 * phi), s2t = sin(2 * theta);
+        const double dl = pow(2, -(p->level - 1 - levelshift));
+        fdata xm, xp, ym, yp, zm, zp;
+        tsc(o, precision, &cache, x - 0.5 * dl, y, z, &xm);
+        tsc(o, precision, &cache, x + 0.5 * dl, y, z, &xp);
+        tsc(o, precision, &cache, x, y - 0.5 * dl, z, &ym);
+        tsc(o, precision, &cache, x, y + 0.5 * dl, z, &yp);
+        tsc(o, precision, &cache, x, y, z - 0.5 * dl, &zm);
+        tsc(o, precision, &cache, x, y, z + 0.5 * dl, &zp);
+#define FDIFF(a, b) (precision == 0 ? (double)((float)(a) - (float)(b)) : ((a) - (b)))
+        const double dxxp = FDIFF(xp.v[F_DX], xm.v[F_DX]) / dl;
+        const double dxyp = FDIFF(yp.v[F_DX], ym.v[F_DX]) / dl;
+        const double dxzp = FDIFF(zp.v[F_DX], zm.v[F_DX]) / dl;
+        const double dyyp = FDIFF(yp.v[F_DY], ym.v[F_DY]) / dl;
+        const double dyzp = FDIFF(zp.v[F_DY], zm.v[F_DY]) / dl;
+        const double dzzp = FDIFF(zp.v[F_DZ], zm.v[F_DZ]) / dl;
+#undef FDIFF
+        const double d1d1p = sp * sp * dxxp + cp * cp * dyyp - s2p * dxyp;
+        const double d2d2p = cp * cp * ct * ct * dxxp + sp * sp * ct * ct * dyyp + st * st * dzzp + s2p * ct * ct * dxyp -
+                             sp * s2t * dyzp - cp * s2t * dxzp;
+        const double d1d2p = ct * cp * sp * (dyyp - dxxp) + (cp * cp - sp * sp) * ct * dxyp + sp * st * dxzp - cp * st * dyzp;
+        const double dr = t->p[i].chi - t->p[i - 1].chi;
+        const double gxsx = (dist - p->chi) * p->chi / dist;
+        s11 = s11 + gxsx * d1d1p * dr;
+        s12 = s12 + gxsx * d1d2p * dr;
+        s22 = s22 + gxsx * d2d2p * dr;
+    }
+    a3[0] = 1 - 2 * s11 / c2 * length;
+    a3[1] = -2 * s12 / c2 * length;
+    a3[2] = 1 - 2 * s22 / c2 * length;
+}
+
+uint64_t mpfo_surfaces(const mpfo_octree* o, int order, int precision, const double* cosmo_tab, uint64_t ncos,
+                       double length, unsigned nsteps, int kind, const double* vals, uint32_t nsurf, int jacobian,
+                       const double* in9, double* out) {
+    traj_t t = {NULL, 0, 0, 1};
+    run_full(o, order, precision, cosmo_tab, ncos, length, nsteps, kind, vals[nsurf - 1], NULL, in9, &t);
+    for (uint32_t j = 0; j < nsurf; ++j) {
+        double* r = out + 32 * j;
+        memset(r, 0, 32 * sizeof(double));
+        const double ref = vals[j];
+        /* hmaps.h:615-660: first element past the surface (find_if for the
+         * redshift, upper_bound otherwise: the same on a monotonic sequence) */
+        uint64_t marked = t.n;
+        for (uint64_t i = 0; i < t.n; ++i) {
+            const double v = elem_value(&t.p[i], kind);
+            if ((kind == 4) ? (v < ref) : (v > ref)) { marked = i; break; }
+        }
+        const uint64_t first = marked - (marked > 0);
+        r[1] = (double)first;
+        if (t.n == 0 || first == t.n - 1 || first == 0) continue; /* rejected (hmaps.h:716) */
+        const photon* a = &t.p[first];
+        const photon* b = &t.p[first + 1];
+        const double f = (elem_value(b, kind) - ref) / (elem_value(b, kind) - elem_value(a, kind));
+        r[0] = 1.0;
+        r[2] = f;
+        r[3] = a->core[CX] * f + b->core[CX] * (1 - f);
+        r[4] = a->core[CY] * f + b->core[CY] * (1 - f);
+        r[5] = a->core[CZ] * f + b->core[CZ] * (1 - f);
+        r[6] = sqrt(r[3] * r[3] + r[4] * r[4] + r[5] * r[5]);
+        r[7] = a->chi * f + b->chi * (1 - f);
+        r[8] = a->lambda * f + b->lambda * (1 - f);
+        r[9] = a->core[CT] * f + b->core[CT] * (1 - f);
+        r[10] = a->core[CA] * f + b->core[CA] * (1 - f);
+        r[11] = a->redshift * f + b->redshift * (1 - f);
+        r[12] = a->s * f + b->s * (1 - f);
+        r[13] = a->isw * f + b->isw * (1 - f);
+        r[14] = a->rho * f + b->rho * (1 - f);
+        r[15] = a->phi * f + b->phi * (1 - f);
+        double rmax = t.p[0].rho;   /* max_element over [0, firstid) (hmaps.h:819) */
+        for (uint64_t i = 1; i < first; ++i) if (t.p[i].rho > rmax) rmax = t.p[i].rho;
+        r[16] = rmax;
+        r[17] = a->core[CKT] * f + b->core[CKT] * (1 - f);
+        r[18] = a->core[CKX] * f + b->core[CKX] * (1 - f);
+        r[19] = a->core[CKY] * f + b->core[CKY] * (1 - f);
+        r[20] = a->core[CKZ] * f + b->core[CKZ] * (1 - f);
+        r[21] = 1.0; r[22] = 0.0; r[23] = 1.0;
+        if (jacobian) jacobian_at(o, precision, &t, r[6], length, r + 21);
+    }
+    const uint64_t n = t.n;
+    free(t.p);
+    return n;
+}
+
 void mpfo_interpolate(const double* x, const double* y, uint64_t n, const double* x0, uint64_t m, double* out) {
     for (uint64_t i = 0; i < m; ++i) out[i] = interpolate(x0[i], x, y, (int64_t)n);
 }

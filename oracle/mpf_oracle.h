@@ -47,6 +47,14 @@ uint64_t mpfo_jacobian_infinitesimal(const mpfo_octree* o, int order, int precis
                                      uint64_t ncos, double length, unsigned nsteps, int stop_kind, double stop_value,
                                      const double* kitarget, const double* in9, const double* dist, uint64_t nd,
                                      double* out4);
+/* Surfaces of Hmaps::FillMap (hmaps.h:615-660, 724-840) on the stored
+ * trajectory, and the infinitesimal Jacobian at each surface distance
+ * (lensing.h:399-480).  kind: 0 radius, 1 lambda, 2 redshift, 3 t, 4 a, 6 s;
+ * the ray stops at vals[nsurf-1].  out: nsurf x 32 doubles (layout of
+ * libmpf's MPF_SURF_*).  Returns the trajectory length. */
+uint64_t mpfo_surfaces(const mpfo_octree* o, int order, int precision, const double* cosmo_tab, uint64_t ncos,
+                       double length, unsigned nsteps, int kind, const double* vals, uint32_t nsurf, int jacobian,
+                       const double* in9, double* out);
 void mpfo_interpolate(const double* x, const double* y, uint64_t n, const double* x0, uint64_t m, double* out);
 
 #ifdef __cplusplus

@@ -43,6 +43,8 @@ class _Base:
         getattr(L, pf + "integrate").restype = _u64
         getattr(L, pf + "trajectory").restype = _u64
         getattr(L, pf + "jacobian_infinitesimal").restype = _u64
+        if pf == "mpfo_":
+            L.mpfo_surfaces.restype = _u64
 
     def tree(self, rec):
         rec = np.ascontiguousarray(rec)
@@ -132,6 +134,17 @@ class Oracle(_Base):
                                              ctypes.c_double(length), ctypes.c_uint(nsteps), int(stop_kind),
                                              ctypes.c_double(stop_value), _p(kit), _p(in9), _p(dist),
                                              _u64(dist.size), _p(out))
+        return out
+
+    def surfaces(self, h, order, table, length, nsteps, kind, vals, in9, jacobian=True, precision=0):
+        """Per-surface records (nsurf x 32) of one ray, hmaps.h:615-660 + lensing.h:399-480."""
+        tab = np.ascontiguousarray(table, dtype=np.float64)
+        in9 = np.ascontiguousarray(in9, dtype=np.float64).reshape(9)
+        vals = np.ascontiguousarray(vals, dtype=np.float64).ravel()
+        out = np.zeros((vals.size, 32))
+        self.lib.mpfo_surfaces(_vp(h), int(order), int(precision), _p(tab), _u64(tab.shape[1]),
+                               ctypes.c_double(length), ctypes.c_uint(nsteps), int(kind), _p(vals),
+                               ctypes.c_uint32(vals.size), int(bool(jacobian)), _p(in9), _p(out))
         return out
 
     def interpolate(self, x, y, x0):
