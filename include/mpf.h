@@ -133,6 +133,11 @@ typedef struct {
     uint32_t n;        /* number of surfaces, 1..MPF_MAX_SURFACES */
     double value[MPF_MAX_SURFACES]; /* increasing (decreasing for _A) */
     int jacobian;      /* also compute a11, a12, a22 */
+    /* optional per-ray value (n doubles; host memory for mpf_integrate_surfaces,
+     * device memory for the _dev call): one surface per ray, at which the ray
+     * also stops (a source distance or redshift); then n must be 1 and
+     * value[] is ignored */
+    const double* ray_value;
 } mpf_surfaces;
 /* Per ray and surface, MPF_NSURF doubles: */
 enum {
@@ -148,6 +153,27 @@ int mpf_integrate_surfaces(mpf_ctx* ctx, const double* in9, uint64_t n, const mp
                            double* out_surf);
 int mpf_integrate_surfaces_dev(mpf_ctx* ctx, const double* dev_in9, uint64_t n, const mpf_surfaces* surf,
                                double* dev_out, double* dev_out_surf, int sync);
+
+/* HEALPix RING scheme (host, no device needed), restated from the published
+ * algorithm (Gorski et al. 2005): the reference links HEALPix C++ 3.70 /
+ * chealpix (pix2vec_ring at hmaps.h:599, nside2npix at hmaps.cpp:185). */
+int mpf_healpix_pix2vec(int64_t nside, const int64_t* pix, uint64_t n, double* vec3);
+int mpf_healpix_vec2pix(int64_t nside, const double* vec3, uint64_t n, int64_t* pix);
+/* Launch photons at pixel centres (Hmaps::FillMap, hmaps.h:599-607), device
+ * buffers: dev_in9 receives n x 9 doubles. */
+int mpf_hmaps_launch_dev(mpf_ctx* ctx, int64_t nside, const int64_t* dev_pix, uint64_t n, double* dev_in9);
+/* Per-pixel lensing maps of rays launched at pixel centres (hmaps.h:724-762,
+ * infinitesimal Jacobian): dev_maps[(j*4 + k)*npix + pix] for surface j and
+ * k = kappa, gamma1, gamma2, 1/magnification; rejected pixels are 0.  Pixels
+ * not in dev_pix are left untouched. */
+int mpf_hmaps_fill_dev(mpf_ctx* ctx, int64_t nside, const int64_t* dev_pix, const double* dev_surf, uint64_t n,
+                       uint32_t nsurf, double* dev_maps);
+/* Bin rays by the pixel of their observed direction (dev_in9 dx/dl, dy/dl,
+ * dz/dl): dev_maps[k*npix + pix], k = mean kappa, mean gamma1, mean gamma2,
+ * ray count, from surface `isurf` of dev_surf (n x nsurf x MPF_NSURF).
+ * Deterministic: 64-bit fixed-point sums (2^-46) reduced per wavefront. */
+int mpf_healpix_bin_dev(mpf_ctx* ctx, int64_t nside, const double* dev_in9, const double* dev_surf, uint64_t n,
+                        uint32_t nsurf, uint32_t isurf, double* dev_maps);
 
 /* Device buffers on the context's device, for callers that keep inputs and
  * outputs resident between calls (mpf_integrate_dev).  Copies are

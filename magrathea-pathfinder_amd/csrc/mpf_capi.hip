@@ -10,6 +10,7 @@
 #include "mpf_kernels.h"
 #include "mpf_split.h"
 #include "mpf_tree.h"
+#include "mpf_healpix.h"
 
 namespace {
 
@@ -82,7 +83,8 @@ struct mpf_ctx {
     int ncos = 0;
     // run state
     DevBuf<unsigned long long> counters;  // [0] queue, [1] steps
-    DevBuf<double> io_in, io_out, io_surf, scratch;
+    DevBuf<double> io_in, io_out, io_surf, io_rv, scratch;
+    DevBuf<unsigned long long> map_acc;
     DevBuf<int64_t> io_i64;
     DevBuf<uint32_t> io_u32;
     uint64_t last_steps = 0;
@@ -140,7 +142,8 @@ int mpf_destroy(mpf_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->base.release(); c->child_first.release(); c->elem.release(); c->child_mask.release();
     c->f32.release(); c->f64.release(); c->cosmo.release(); c->counters.release();
-    c->io_in.release(); c->io_out.release(); c->io_surf.release(); c->scratch.release(); c->io_i64.release(); c->io_u32.release();
+    c->io_in.release(); c->io_out.release(); c->io_surf.release(); c->io_rv.release(); c->scratch.release();
+    c->map_acc.release(); c->io_i64.release(); c->io_u32.release();
     mpf::split_release(c->split);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -304,6 +307,7 @@ int mpf_integrate_surfaces_dev(mpf_ctx* c, const double* dev_in9, uint64_t n, co
                                double* dev_out, double* dev_out_surf, int sync) {
     if (!c || !surf) return ERR_INVAL;
     if (surf->n < 1 || surf->n > MPF_MAX_SURFACES) return fail(c, ERR_INVAL, "surface count must be 1..16");
+    if (surf->ray_value && surf->n != 1) return fail(c, ERR_INVAL, "per-ray surface values need exactly one surface");
     if (surf->kind < 0 || surf->kind > 6 || surf->kind == MPF_STOP_PLANE)
         return fail(c, ERR_INVAL, "surface kind must be radius, lambda, redshift, t, a or s");
     for (uint32_t j = 1; j < surf->n; ++j) {
@@ -311,7 +315,7 @@ int mpf_integrate_surfaces_dev(mpf_ctx* c, const double* dev_in9, uint64_t n, co
         if (!ok) return fail(c, ERR_INVAL, "surface values must be strictly monotonic (decreasing for a)");
     }
     if (n && !dev_out_surf) return ERR_INVAL;
-    mpf_stop stop{surf->kind, surf->value[surf->n - 1], {0.0, 0.0, 0.0}};
+    mpf_stop stop{surf->kind, surf->ray_value ? 0.0 : surf->value[surf->n - 1], {0.0, 0.0, 0.0}};
     return integrate_impl(c, dev_in9, n, &stop, dev_out, surf, dev_out_surf, sync);
 }
 
@@ -348,6 +352,7 @@ static int integrate_impl(mpf_ctx* c, const double* dev_in9, uint64_t n, const m
         for (uint32_t j = 0; j < surf->n; ++j) A.surf_val[j] = surf->value[j];
         A.surf_out = dev_surf;
         A.jac = surf->jacobian ? 1 : 0;
+        A.ray_value = surf->ray_value;
     }
     HIPCHK(c, mpf::launch_fill_u64(c->counters.p, 2, 0ull, c->stream));
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
@@ -435,6 +440,60 @@ int mpf_dev_copy(mpf_ctx* c, void* dev_dst, const void* dev_src, uint64_t bytes)
     return mpf_synchronize(c);
 }
 
+int mpf_healpix_pix2vec(int64_t nside, const int64_t* pix, uint64_t n, double* vec3) {
+    if (nside < 1 || nside > (1 << 29) || (n && (!pix || !vec3))) return ERR_INVAL;
+    const int64_t npix = 12 * nside * nside;
+    for (uint64_t i = 0; i < n; ++i) {
+        if (pix[i] < 0 || pix[i] >= npix) return ERR_INVAL;
+        mpf::healpix::pix2vec_ring(nside, pix[i], vec3 + 3 * i);
+    }
+    return 0;
+}
+
+int mpf_healpix_vec2pix(int64_t nside, const double* vec3, uint64_t n, int64_t* pix) {
+    if (nside < 1 || nside > (1 << 29) || (n && (!pix || !vec3))) return ERR_INVAL;
+    for (uint64_t i = 0; i < n; ++i)
+        pix[i] = mpf::healpix::vec2pix_ring(nside, vec3[3 * i], vec3[3 * i + 1], vec3[3 * i + 2]);
+    return 0;
+}
+
+int mpf_hmaps_launch_dev(mpf_ctx* c, int64_t nside, const int64_t* dev_pix, uint64_t n, double* dev_in9) {
+    if (!c || nside < 1 || nside > (1 << 29) || (n && (!dev_pix || !dev_in9))) return ERR_INVAL;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    HIPCHK(c, mpf::launch_pixel_launch(nside, dev_pix, n, dev_in9, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int mpf_hmaps_fill_dev(mpf_ctx* c, int64_t nside, const int64_t* dev_pix, const double* dev_surf, uint64_t n,
+                       uint32_t nsurf, double* dev_maps) {
+    if (!c || nside < 1 || nside > (1 << 29) || nsurf < 1 || nsurf > MPF_MAX_SURFACES ||
+        (n && (!dev_pix || !dev_surf || !dev_maps)))
+        return ERR_INVAL;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    HIPCHK(c, mpf::launch_pixel_fill(nside, dev_pix, dev_surf, (int)nsurf, n, dev_maps, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int mpf_healpix_bin_dev(mpf_ctx* c, int64_t nside, const double* dev_in9, const double* dev_surf, uint64_t n,
+                        uint32_t nsurf, uint32_t isurf, double* dev_maps) {
+    if (!c || nside < 1 || nside > (1 << 16) || nsurf < 1 || nsurf > MPF_MAX_SURFACES || isurf >= nsurf ||
+        !dev_maps || (n && (!dev_in9 || !dev_surf)))
+        return ERR_INVAL;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    const uint64_t npix = 12ull * (uint64_t)nside * (uint64_t)nside;
+    if (c->map_acc.ensure(4 * npix) != hipSuccess) return fail(c, ERR_NOMEM, "map accumulator");
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    HIPCHK(c, mpf::launch_bin(nside, dev_in9, dev_surf, (int)nsurf, (int)isurf, n, c->map_acc.p, dev_maps, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    return 0;
+}
+
 int mpf_synchronize(mpf_ctx* c) {
     if (!c) return ERR_INVAL;
     HIPCHK(c, hipSetDevice(c->cfg.device));
@@ -471,7 +530,13 @@ int mpf_integrate_surfaces(mpf_ctx* c, const double* in9, uint64_t n, const mpf_
         c->io_surf.ensure((uint64_t)MPF_NSURF * ns * std::max<uint64_t>(n, 1)) != hipSuccess)
         return fail(c, ERR_NOMEM, "integration buffers");
     if (n) HIPCHK(c, hipMemcpyAsync(c->io_in.p, in9, 9 * n * 8, hipMemcpyHostToDevice, c->stream));
-    int rc = mpf_integrate_surfaces_dev(c, c->io_in.p, n, surf, c->io_out.p, c->io_surf.p, 1);
+    mpf_surfaces sd = *surf;
+    if (surf->ray_value) {
+        if (c->io_rv.ensure(std::max<uint64_t>(n, 1)) != hipSuccess) return fail(c, ERR_NOMEM, "per-ray values");
+        if (n) HIPCHK(c, hipMemcpyAsync(c->io_rv.p, surf->ray_value, n * 8, hipMemcpyHostToDevice, c->stream));
+        sd.ray_value = c->io_rv.p;
+    }
+    int rc = mpf_integrate_surfaces_dev(c, c->io_in.p, n, &sd, c->io_out.p, c->io_surf.p, 1);
     if (rc) return rc;
     if (n && out) HIPCHK(c, hipMemcpy(out, c->io_out.p, (uint64_t)MPF_NOUT * n * 8, hipMemcpyDeviceToHost));
     if (n) HIPCHK(c, hipMemcpy(out_surf, c->io_surf.p, (uint64_t)MPF_NSURF * ns * n * 8, hipMemcpyDeviceToHost));

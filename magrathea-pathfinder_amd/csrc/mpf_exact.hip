@@ -69,6 +69,7 @@ __device__ __forceinline__ double level_extent(int lvl) {
 
 struct RayState {
     SurfAcc surf;
+    double stopv;     // this ray's stop value (and its surface when per-ray)
     Rec back, prev;
     double data[7];   // fields at back's position
     double dl, a0, phi0c2;
@@ -125,6 +126,7 @@ __device__ void ray_launch(const IntegrateArgs& A, const DevTree& t, TscCache& c
     s.appended = 0;
     s.has_prev = false;
     if (A.nsurf) surf_init(s.surf, d[F_RHO]);
+    s.stopv = A.stop_value;
 }
 
 __device__ __forceinline__ SurfElem surf_elem(const double* r) {
@@ -262,7 +264,7 @@ __device__ bool ray_step(const IntegrateArgs& A, const DevTree& t, TscCache& cac
         if (A.nsurf) {
             double T[3] = {0.0, 0.0, 0.0};
             if (A.jac) element_tidal<P>(t, s.back.v, T);
-            surf_append(s.surf, A.surf_kind, A.nsurf, A.surf_val, A.surf_out + (uint64_t)A.nsurf * SF_NREC * ray,
+            surf_append(s.surf, A.surf_kind, A.nsurf, A.surf_val, A.ray_value != nullptr, s.stopv, A.surf_out + (uint64_t)A.nsurf * SF_NREC * ray,
                         s.appended, surf_elem(s.prev.v), surf_elem(s.back.v), T, A.jac != 0);
         }
     }
@@ -270,13 +272,13 @@ __device__ bool ray_step(const IntegrateArgs& A, const DevTree& t, TscCache& cac
         const double* b = s.back.v;
         bool brk;
         switch (A.stop_kind) {
-            case 0: brk = b[MPF_OUT_CHI] > A.stop_value; break;
-            case 1: brk = b[MPF_OUT_LAMBDA] > A.stop_value; break;
-            case 2: brk = b[MPF_OUT_REDSHIFT] > A.stop_value; break;
-            case 3: brk = b[CT] > A.stop_value; break;
-            case 4: brk = b[CA] < A.stop_value; break;
-            case 5: brk = (p[CX] * A.kit[0] + p[CY] * A.kit[1] + p[CZ] * A.kit[2]) > A.stop_value; break;
-            case 6: brk = b[MPF_OUT_S] > A.stop_value; break;
+            case 0: brk = b[MPF_OUT_CHI] > s.stopv; break;
+            case 1: brk = b[MPF_OUT_LAMBDA] > s.stopv; break;
+            case 2: brk = b[MPF_OUT_REDSHIFT] > s.stopv; break;
+            case 3: brk = b[CT] > s.stopv; break;
+            case 4: brk = b[CA] < s.stopv; break;
+            case 5: brk = (p[CX] * A.kit[0] + p[CY] * A.kit[1] + p[CZ] * A.kit[2]) > s.stopv; break;
+            case 6: brk = b[MPF_OUT_S] > s.stopv; break;
             default: brk = true; break;
         }
         if (brk) return false;
@@ -350,6 +352,7 @@ __global__ void __launch_bounds__(256) integrate_exact_kernel(IntegrateArgs A, D
                 if (my < A.n) {
                     ray = my;
                     ray_launch<P>(A, t, cache, A.in9 + 9 * my, s);
+                    if (A.ray_value) s.stopv = A.ray_value[my];
                     active = !empty7(s.data);
                     if (!active) ray_finish(A, s, ray);
                 } else {

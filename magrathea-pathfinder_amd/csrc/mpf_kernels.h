@@ -31,6 +31,7 @@ struct IntegrateArgs {
     int nsurf = 0;
     double surf_val[16] = {0};
     double* surf_out = nullptr;      // n x nsurf x SF_NREC
+    const double* ray_value = nullptr; // per-ray stop/surface value (one surface), or null
     int jac = 0;
 };
 
@@ -40,6 +41,13 @@ hipError_t launch_locate(const DevTree& t, const double* xyz, uint64_t n, int64_
 hipError_t launch_interp(int precision, const DevTree& t, int order, const double* xyz, uint64_t n, double* out7,
                          hipStream_t st);
 hipError_t launch_surface_final(const IntegrateArgs& A, hipStream_t st);
+hipError_t launch_bin(int64_t nside, const double* in9, const double* surf, int nsurf, int isurf, uint64_t n,
+                      unsigned long long* acc, double* maps, hipStream_t st);
+hipError_t launch_pixel_fill(int64_t nside, const int64_t* pix, const double* surf, int nsurf, uint64_t n,
+                             double* maps, hipStream_t st);
+hipError_t launch_pixel_launch(int64_t nside, const int64_t* pix, uint64_t n, double* in9, hipStream_t st);
+hipError_t launch_vec2pix(int64_t nside, const double* vec, uint64_t n, int64_t* pix, hipStream_t st);
+hipError_t launch_pix2vec(int64_t nside, const int64_t* pix, uint64_t n, double* vec, hipStream_t st);
 hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v, hipStream_t st);
 
 }  // namespace mpf

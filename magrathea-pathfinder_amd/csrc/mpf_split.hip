@@ -286,6 +286,7 @@ struct SplitRay {
     float F[4];                  // interpolated dphidt, dphidx, dphidy, dphidz at the current point
     uint32_t appended;
     int hint;
+    double stopv;     // this ray's stop value (and its surface when per-ray)
 };
 
 __device__ __forceinline__ double level_extent(unsigned l) { return EXTENT / (double)(1ull << l); }
@@ -385,6 +386,7 @@ integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
                     R.dt = (float)L[14];
                     R.F[0] = R.dt; R.F[1] = R.dx; R.F[2] = R.dy; R.F[3] = R.dz;
                     R.appended = 0;
+                    R.stopv = A.ray_value ? A.ray_value[my] : A.stop_value;
                     if (SURF) surf_init(SA, 0.0);
                     const unsigned l0 = (unsigned)L[15];
                     R.dl = level_extent(l0) * (R.core[CA] * R.core[CA] * ratio_k);
@@ -495,7 +497,7 @@ integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
                     }
                     tidal_from_samples(p[CX], p[CY], p[CZ], chi, jdl, g, T);
                 }
-                surf_append(SA, A.surf_kind, A.nsurf, A.surf_val, A.surf_out + (uint64_t)A.nsurf * SF_NREC * ray,
+                surf_append(SA, A.surf_kind, A.nsurf, A.surf_val, A.ray_value != nullptr, R.stopv, A.surf_out + (uint64_t)A.nsurf * SF_NREC * ray,
                             R.appended + 1, pv, nw, T, A.jac != 0);
             }
 #pragma unroll
@@ -508,13 +510,13 @@ integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
         }
         bool brk = false;
         switch (A.stop_kind) {
-            case 0: brk = R.chi > A.stop_value; break;
-            case 1: brk = R.lambda > A.stop_value; break;
-            case 2: brk = R.redshift > A.stop_value; break;
-            case 3: brk = R.core[CT] > A.stop_value; break;
-            case 4: brk = R.core[CA] < A.stop_value; break;
-            case 5: brk = (p[CX] * A.kit[0] + p[CY] * A.kit[1] + p[CZ] * A.kit[2]) > A.stop_value; break;
-            case 6: brk = R.s > A.stop_value; break;
+            case 0: brk = R.chi > R.stopv; break;
+            case 1: brk = R.lambda > R.stopv; break;
+            case 2: brk = R.redshift > R.stopv; break;
+            case 3: brk = R.core[CT] > R.stopv; break;
+            case 4: brk = R.core[CA] < R.stopv; break;
+            case 5: brk = (p[CX] * A.kit[0] + p[CY] * A.kit[1] + p[CZ] * A.kit[2]) > R.stopv; break;
+            case 6: brk = R.s > R.stopv; break;
             default: brk = false; break;
         }
         if (!cont || brk) {

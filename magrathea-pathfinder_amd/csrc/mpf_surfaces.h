@@ -65,14 +65,15 @@ __device__ __forceinline__ double surf_value(int kind, const SurfElem& e) {
 // Called after element i (>= 1) is appended, with the previous element.
 // `tidal` (d1d1, d1d2, d2d2) of element i is added after the crossings are
 // recorded, so a snapshot holds the sums up to element i-1 = firstid.
-__device__ __forceinline__ void surf_append(SurfAcc& s, int kind, int nsurf, const double* vals, double* rec,
-                                            uint32_t i, const SurfElem& pv, const SurfElem& nw, const double* tidal,
-                                            bool jac) {
+// vals: the surface values, or, when per_ray, the single value rv
+__device__ __forceinline__ void surf_append(SurfAcc& s, int kind, int nsurf, const double* vals, bool per_ray,
+                                            double rv, double* rec, uint32_t i, const SurfElem& pv,
+                                            const SurfElem& nw, const double* tidal, bool jac) {
     const double vn = surf_value(kind, nw);
     const double vp = surf_value(kind, pv);
     for (int j = 0; j < nsurf; ++j) {
         if (s.crossed & (1u << j)) continue;
-        const double ref = vals[j];
+        const double ref = per_ray ? rv : vals[j];
         const bool pass = (kind == 4) ? (vn < ref) : (vn > ref);
         if (!pass) continue;
         s.crossed |= (1u << j);

@@ -30,7 +30,8 @@ EXPORTS = ("mpf_create", "mpf_destroy", "mpf_last_error", "mpf_set_precision", "
            "mpf_load_octree_fields64", "mpf_octree_info", "mpf_set_cosmology", "mpf_locate", "mpf_interpolate",
            "mpf_integrate", "mpf_integrate_dev", "mpf_last_run", "mpf_synchronize", "mpf_validate_octree",
            "mpf_dev_alloc", "mpf_dev_free", "mpf_dev_upload", "mpf_dev_download", "mpf_dev_copy",
-           "mpf_integrate_surfaces", "mpf_integrate_surfaces_dev")
+           "mpf_integrate_surfaces", "mpf_integrate_surfaces_dev", "mpf_healpix_pix2vec", "mpf_healpix_vec2pix",
+           "mpf_hmaps_launch_dev", "mpf_hmaps_fill_dev", "mpf_healpix_bin_dev")
 
 SURF_FIELDS = ("valid", "firstid", "f", "x", "y", "z", "dist", "chi", "lambda", "t", "a", "redshift", "s", "isw",
                "rho", "phi", "rhomax", "kt", "kx", "ky", "kz", "a11", "a12", "a22")
@@ -50,10 +51,10 @@ class Config(ctypes.Structure):
 
 class Surfaces(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int), ("n", ctypes.c_uint32), ("value", ctypes.c_double * 16),
-                ("jacobian", ctypes.c_int)]
+                ("jacobian", ctypes.c_int), ("ray_value", ctypes.c_void_p)]
 
 
-def make_surfaces(kind, values, jacobian=True):
+def make_surfaces(kind, values, jacobian=True, ray_value=None):
     if isinstance(kind, str):
         kind = STOP_NAMES[kind]
     values = list(values)
@@ -63,6 +64,7 @@ def make_surfaces(kind, values, jacobian=True):
     for i, v in enumerate(values):
         s.value[i] = float(v)
     s.jacobian = int(bool(jacobian))
+    s.ray_value = ray_value
     return s
 
 
@@ -105,6 +107,12 @@ def load_library(path=LIB_PATH):
     lib.mpf_dev_copy.argtypes = [vp, vp, vp, u64]
     lib.mpf_integrate_surfaces.argtypes = [vp, vp, u64, ctypes.POINTER(Surfaces), vp, vp]
     lib.mpf_integrate_surfaces_dev.argtypes = [vp, vp, u64, ctypes.POINTER(Surfaces), vp, vp, i32]
+    i64 = ctypes.c_int64
+    lib.mpf_healpix_pix2vec.argtypes = [i64, vp, u64, vp]
+    lib.mpf_healpix_vec2pix.argtypes = [i64, vp, u64, vp]
+    lib.mpf_hmaps_launch_dev.argtypes = [vp, i64, vp, u64, vp]
+    lib.mpf_hmaps_fill_dev.argtypes = [vp, i64, vp, vp, u64, ctypes.c_uint32, vp]
+    lib.mpf_healpix_bin_dev.argtypes = [vp, i64, vp, vp, u64, ctypes.c_uint32, ctypes.c_uint32, vp]
     _lib = lib
     return lib
 
@@ -122,6 +130,24 @@ def validate_octree(rec):
     buf = ctypes.create_string_buffer(512)
     rc = lib.mpf_validate_octree(_p(rec), ctypes.c_uint64(rec.size), ctypes.byref(lo), ctypes.byref(hi), buf, 512)
     return rc == 0, lo.value, hi.value, buf.value.decode()
+
+
+def healpix_pix2vec(nside, pix):
+    lib = load_library()
+    pix = np.ascontiguousarray(pix, dtype=np.int64).ravel()
+    out = np.zeros((pix.size, 3))
+    if lib.mpf_healpix_pix2vec(int(nside), _p(pix), ctypes.c_uint64(pix.size), _p(out)):
+        raise MpfError("mpf_healpix_pix2vec: invalid arguments")
+    return out
+
+
+def healpix_vec2pix(nside, vec):
+    lib = load_library()
+    vec = np.ascontiguousarray(vec, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros(vec.shape[0], dtype=np.int64)
+    if lib.mpf_healpix_vec2pix(int(nside), _p(vec), ctypes.c_uint64(vec.shape[0]), _p(out)):
+        raise MpfError("mpf_healpix_vec2pix: invalid arguments")
+    return out
 
 
 def make_stop(kind="none", value=0.0, kitarget=(0.0, 0.0, 0.0)):
@@ -220,23 +246,51 @@ class Context:
                                            ctypes.byref(steps)), "mpf_integrate")
         return out, steps.value
 
-    def integrate_surfaces(self, in9, kind, values, jacobian=True):
-        """Rays to the last surface; returns (per-ray records, per-ray per-surface records)."""
+    def integrate_surfaces(self, in9, kind, values, jacobian=True, ray_value=None):
+        """Rays to the last surface; returns (per-ray records, per-ray per-surface records).
+        ray_value: optional per-ray surface/stop value (one surface per ray)."""
         in9 = np.ascontiguousarray(in9, dtype=np.float64).reshape(-1, 9)
         n = in9.shape[0]
-        sf = make_surfaces(kind, values, jacobian)
+        rv = None
+        if ray_value is not None:
+            rv = np.ascontiguousarray(ray_value, dtype=np.float64).ravel()
+            values = [0.0]
+        sf = make_surfaces(kind, values, jacobian, None if rv is None else rv.ctypes.data)
         out = np.zeros((n, NOUT))
         surf = np.zeros((n, sf.n, NSURF))
         self._check(self.lib.mpf_integrate_surfaces(self.h, _p(in9), ctypes.c_uint64(n), ctypes.byref(sf), _p(out),
                                                     _p(surf)), "mpf_integrate_surfaces")
         return out, surf
 
-    def integrate_surfaces_dev(self, in_ptr, n, out_ptr, surf_ptr, kind, values, jacobian=True, sync=False):
-        sf = make_surfaces(kind, values, jacobian)
+    def integrate_surfaces_dev(self, in_ptr, n, out_ptr, surf_ptr, kind, values, jacobian=True, sync=False,
+                               ray_value_ptr=None):
+        sf = make_surfaces(kind, values if ray_value_ptr is None else [0.0], jacobian, ray_value_ptr)
         self._check(self.lib.mpf_integrate_surfaces_dev(self.h, ctypes.c_void_p(in_ptr), ctypes.c_uint64(n),
                                                         ctypes.byref(sf), ctypes.c_void_p(out_ptr),
                                                         ctypes.c_void_p(surf_ptr), int(bool(sync))),
                     "mpf_integrate_surfaces_dev")
+
+    def hmaps_launch_dev(self, nside, pix_ptr, n, in9_ptr):
+        self._check(self.lib.mpf_hmaps_launch_dev(self.h, int(nside), ctypes.c_void_p(pix_ptr), ctypes.c_uint64(n),
+                                                  ctypes.c_void_p(in9_ptr)), "mpf_hmaps_launch_dev")
+
+    def hmaps_fill_dev(self, nside, pix_ptr, surf_ptr, n, nsurf, maps_ptr):
+        self._check(self.lib.mpf_hmaps_fill_dev(self.h, int(nside), ctypes.c_void_p(pix_ptr), ctypes.c_void_p(surf_ptr),
+                                                ctypes.c_uint64(n), ctypes.c_uint32(nsurf), ctypes.c_void_p(maps_ptr)),
+                    "mpf_hmaps_fill_dev")
+
+    def healpix_bin_dev(self, nside, in9_ptr, surf_ptr, n, nsurf, isurf, maps_ptr):
+        self._check(self.lib.mpf_healpix_bin_dev(self.h, int(nside), ctypes.c_void_p(in9_ptr),
+                                                 ctypes.c_void_p(surf_ptr), ctypes.c_uint64(n), ctypes.c_uint32(nsurf),
+                                                 ctypes.c_uint32(isurf), ctypes.c_void_p(maps_ptr)),
+                    "mpf_healpix_bin_dev")
+        return self.last_ms()
+
+    def last_ms(self):
+        ms = ctypes.c_double(0.0)
+        steps = ctypes.c_uint64(0)
+        self.lib.mpf_last_run(self.h, ctypes.byref(steps), ctypes.byref(ms))
+        return ms.value
 
     def integrate_dev(self, in_ptr, n, out_ptr, stop="none", value=0.0, kitarget=(0.0, 0.0, 0.0), sync=False):
         """Device-resident integration: in_ptr/out_ptr are device addresses (ints)."""
