@@ -154,6 +154,38 @@ int mpf_integrate_surfaces(mpf_ctx* ctx, const double* in9, uint64_t n, const mp
 int mpf_integrate_surfaces_dev(mpf_ctx* ctx, const double* dev_in9, uint64_t n, const mpf_surfaces* surf,
                                double* dev_out, double* dev_out_surf, int sync);
 
+/* Catalogue root-finding: Catalogues::relCat / iterateNewtonMethod /
+ * newtonMethod2d (catalogues.h:410-920) with the infinitesimal Jacobian
+ * (jacobiantype = infinitesimal), for a batch of sources on the device. */
+typedef struct {
+    double accuracy;          /* cat_accuracy (rad), e.g. 1e-8 */
+    uint32_t max_iterations;  /* 10 (catalogues.h:765) */
+    double vobs[3];           /* observer velocity (SI) */
+    double h;                 /* dimensionless Hubble parameter */
+    double mpc;               /* one megaparsec in metres (parameter file `mpc`) */
+} mpf_newton_cfg;
+/* Per source, MPF_NCAT doubles: the catalogue columns of catalogues.h:844-859
+ * plus status and the number of rounds run. */
+enum {
+    MPF_CAT_PHI, MPF_CAT_THETA, MPF_CAT_PHI_OBS, MPF_CAT_THETA_OBS, MPF_CAT_SEP, MPF_CAT_ITER,
+    MPF_CAT_Z0, MPF_CAT_Z1, MPF_CAT_Z2, MPF_CAT_Z3, MPF_CAT_Z4, MPF_CAT_Z5,
+    MPF_CAT_A11, MPF_CAT_A12, MPF_CAT_A21, MPF_CAT_A22, MPF_CAT_STATUS, MPF_CAT_NROUNDS, MPF_NCAT = 20
+};
+/* status: converged (catalogue file), outside (the ray never reaches the
+ * source radius: observed angles 42, the _err file), rejected (no convergence
+ * within max_iterations: iterations 42, the _reject file) */
+enum { MPF_CAT_OK = 0, MPF_CAT_OUTSIDE = 1, MPF_CAT_REJECTED = 2 };
+/* src6: n x (x, y, z, vx, vy, vz), positions in box units in the octree frame
+ * (already rotated for narrow cones, catalogues.h:818-824), velocities in
+ * simulation units; init2 (may be NULL): n x (phi, theta) first guesses, by
+ * default the unlensed direction (catalogues.h:832-836).  total_rays (may be
+ * NULL): rays integrated, summed over Newton rounds. */
+int mpf_catalogue_solve(mpf_ctx* ctx, const double* src6, const double* init2, uint64_t n,
+                        const mpf_newton_cfg* cfg, double* out, uint64_t* total_rays);
+/* Same on device buffers (sync on return). */
+int mpf_catalogue_solve_dev(mpf_ctx* ctx, const double* dev_src6, const double* dev_init2, uint64_t n,
+                            const mpf_newton_cfg* cfg, double* dev_out, uint64_t* total_rays);
+
 /* HEALPix RING scheme (host, no device needed), restated from the published
  * algorithm (Gorski et al. 2005): the reference links HEALPix C++ 3.70 /
  * chealpix (pix2vec_ring at hmaps.h:599, nside2npix at hmaps.cpp:185). */

@@ -440,6 +440,136 @@ int mpf_dev_copy(mpf_ctx* c, void* dev_dst, const void* dev_src, uint64_t bytes)
     return mpf_synchronize(c);
 }
 
+int mpf_catalogue_solve_dev(mpf_ctx* c, const double* dev_src6, const double* dev_init2, uint64_t n,
+                            const mpf_newton_cfg* cfg, double* dev_out, uint64_t* total_rays) {
+    if (!c || !cfg || (n && (!dev_src6 || !dev_out))) return ERR_INVAL;
+    if (!c->have_tree) return fail(c, ERR_STATE, "no octree loaded");
+    if (c->ncos <= 0) return fail(c, ERR_STATE, "no cosmology table set");
+    if (!(cfg->accuracy > 0) || cfg->max_iterations < 1 || !(cfg->h > 0) || !(cfg->mpc > 0))
+        return fail(c, ERR_INVAL, "newton configuration");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (total_rays) *total_rays = 0;
+    if (n == 0) return 0;
+    if (n >= 0xFFFFFFFFull) return fail(c, ERR_INVAL, "too many sources");
+    // potential and scale factor at the observer (the launch element, integrator.h:669-683)
+    double phi0 = 0.0, a0 = 1.0;
+    {
+        const double origin[3] = {0.0, 0.0, 0.0};
+        double d7[7];
+        const int prec = c->cfg.precision;
+        if (prec == MPF_PREC_SPLIT) c->cfg.precision = MPF_PREC_FP64;
+        const int ord = c->cfg.order == MPF_TSC ? 1 : c->cfg.order;  // tsc at the box centre is cic
+        int rc = mpf_interpolate(c, ord, origin, 1, d7);
+        c->cfg.precision = prec;
+        if (rc) return rc;
+        phi0 = d7[1];
+        std::vector<double> ct(c->ncos), ca(c->ncos);
+        HIPCHK(c, hipMemcpy(ct.data(), c->cosmo.p, c->ncos * 8, hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(ca.data(), c->cosmo.p + c->ncos, c->ncos * 8, hipMemcpyDeviceToHost));
+        // Utility::interpolate at t = 0
+        const long long nn = c->ncos;
+        long long i = std::upper_bound(ct.begin(), ct.end(), 0.0) - ct.begin();
+        i += (nn > 1) * ((i <= 0) - (i >= nn));
+        a0 = (nn > 1) ? (ca[i - 1] + ((ca[i] - ca[i - 1]) * (0.0 - ct[i - 1]) / (ct[i] - ct[i - 1]))) : ca[0];
+    }
+    DevBuf<double> dist, ang, in9, dact, surf, outb;
+    DevBuf<uint32_t> act, next, need;
+    DevBuf<unsigned int> cnt;
+    if (dist.ensure(n) || ang.ensure(2 * n) || in9.ensure(9 * n) || dact.ensure(n) ||
+        surf.ensure((uint64_t)MPF_NSURF * n) || act.ensure(n) || next.ensure(n) || need.ensure(n) || cnt.ensure(2) ||
+        outb.ensure((uint64_t)MPF_NOUT * n)) {
+        dist.release(); ang.release(); in9.release(); dact.release(); surf.release(); act.release(); next.release();
+        need.release(); cnt.release(); outb.release();
+        return fail(c, ERR_NOMEM, "catalogue buffers");
+    }
+    auto release = [&]() {
+        dist.release(); ang.release(); in9.release(); dact.release(); surf.release(); act.release(); next.release();
+        need.release(); cnt.release(); outb.release();
+    };
+#define CATCHK(expr)                                  \
+    do {                                              \
+        hipError_t e_ = (expr);                       \
+        if (e_ != hipSuccess) { release(); return hipfail(c, e_, #expr); } \
+    } while (0)
+    CATCHK(mpf::launch_cat_init(dev_src6, dev_init2, n, dist.p, ang.p, dev_out, c->stream));
+    std::vector<uint32_t> ids(n);
+    for (uint64_t i = 0; i < n; ++i) ids[i] = (uint32_t)i;
+    CATCHK(hipMemcpyAsync(act.p, ids.data(), n * 4, hipMemcpyHostToDevice, c->stream));
+    uint64_t nact = n, rays = 0;
+    mpf::CatRound R;
+    R.src = dev_src6; R.dist = dist.p; R.ang = ang.p; R.out = dev_out; R.surf = surf.p;
+    R.next = next.p; R.nnext = cnt.p; R.need_jac = need.p; R.nneed = cnt.p + 1;
+    R.max_iter = (int)cfg->max_iterations; R.acc = cfg->accuracy; R.length = c->cfg.length; R.h = cfg->h;
+    R.mpc = cfg->mpc; R.a0 = a0; R.phi0 = phi0;
+    for (int k = 0; k < 3; ++k) R.vobs[k] = cfg->vobs[k];
+    unsigned int zero2[2] = {0u, 0u};
+    CATCHK(hipMemcpyAsync(cnt.p + 1, zero2, 4, hipMemcpyHostToDevice, c->stream));
+    for (int round = 0; nact > 0 && round <= (int)cfg->max_iterations; ++round) {
+        const int jac = round >= 3 ? 1 : 0;
+        CATCHK(mpf::launch_cat_launch(ang.p, act.p, nact, in9.p, dact.p, dist.p, c->stream));
+        mpf_surfaces sf{};
+        sf.kind = MPF_STOP_RADIUS;
+        sf.n = 1;
+        sf.jacobian = jac;
+        sf.ray_value = dact.p;
+        int rc = mpf_integrate_surfaces_dev(c, in9.p, nact, &sf, outb.p, surf.p, 1);
+        if (rc) { release(); return rc; }
+        rays += nact;
+        CATCHK(hipMemcpyAsync(cnt.p, zero2, 4, hipMemcpyHostToDevice, c->stream));
+        R.act = act.p; R.next = next.p; R.nact = nact; R.round = round; R.jac = jac;
+        CATCHK(mpf::launch_newton(R, c->stream));
+        unsigned int nn = 0;
+        CATCHK(hipMemcpyAsync(&nn, cnt.p, 4, hipMemcpyDeviceToHost, c->stream));
+        CATCHK(hipStreamSynchronize(c->stream));
+        std::swap(act.p, next.p);
+        nact = nn;
+    }
+    // Jacobian of the sources that converged before the Jacobian rounds
+    unsigned int nneed = 0;
+    CATCHK(hipMemcpyAsync(&nneed, cnt.p + 1, 4, hipMemcpyDeviceToHost, c->stream));
+    CATCHK(hipStreamSynchronize(c->stream));
+    if (nneed) {
+        CATCHK(mpf::launch_cat_launch(ang.p, need.p, nneed, in9.p, dact.p, dist.p, c->stream));
+        mpf_surfaces sf{};
+        sf.kind = MPF_STOP_RADIUS;
+        sf.n = 1;
+        sf.jacobian = 1;
+        sf.ray_value = dact.p;
+        int rc = mpf_integrate_surfaces_dev(c, in9.p, nneed, &sf, outb.p, surf.p, 1);
+        if (rc) { release(); return rc; }
+        rays += nneed;
+        CATCHK(mpf::launch_cat_jac(need.p, nneed, surf.p, dev_out, c->stream));
+        CATCHK(hipStreamSynchronize(c->stream));
+    }
+#undef CATCHK
+    release();
+    if (total_rays) *total_rays = rays;
+    return 0;
+}
+
+int mpf_catalogue_solve(mpf_ctx* c, const double* src6, const double* init2, uint64_t n, const mpf_newton_cfg* cfg,
+                        double* out, uint64_t* total_rays) {
+    if (!c || !cfg || (n && (!src6 || !out))) return ERR_INVAL;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    DevBuf<double> s, i2, o;
+    if (s.ensure(6 * std::max<uint64_t>(n, 1)) || o.ensure((uint64_t)MPF_NCAT * std::max<uint64_t>(n, 1)) ||
+        (init2 && i2.ensure(2 * std::max<uint64_t>(n, 1)))) {
+        s.release(); i2.release(); o.release();
+        return fail(c, ERR_NOMEM, "catalogue buffers");
+    }
+    int rc = 0;
+    if (n) {
+        if (hipMemcpy(s.p, src6, 6 * n * 8, hipMemcpyHostToDevice) != hipSuccess ||
+            (init2 && hipMemcpy(i2.p, init2, 2 * n * 8, hipMemcpyHostToDevice) != hipSuccess))
+            rc = fail(c, ERR_HIP, "catalogue upload");
+    }
+    if (!rc) rc = mpf_catalogue_solve_dev(c, s.p, init2 ? i2.p : nullptr, n, cfg, o.p, total_rays);
+    if (!rc && n && hipMemcpy(out, o.p, (uint64_t)MPF_NCAT * n * 8, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = fail(c, ERR_HIP, "catalogue download");
+    s.release(); i2.release(); o.release();
+    return rc;
+}
+
 int mpf_healpix_pix2vec(int64_t nside, const int64_t* pix, uint64_t n, double* vec3) {
     if (nside < 1 || nside > (1 << 29) || (n && (!pix || !vec3))) return ERR_INVAL;
     const int64_t npix = 12 * nside * nside;

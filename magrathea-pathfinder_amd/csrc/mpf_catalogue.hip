@@ -1,0 +1,214 @@
+// Catalogue root-finding: Catalogues::iterateNewtonMethod / newtonMethod2d
+// (catalogues.h:410-800) for a batch of sources, infinitesimal Jacobian.
+//
+// Each Newton round integrates the still-active sources to their comoving
+// distance (a radius surface per ray), then newton_kernel, per source:
+//   * no crossing of the source radius -> failure (42, 42; catalogues.h:435-440)
+//   * separation |x_src - x_ray(D)| / D (catalogues.h:495-496) above the
+//     accuracy -> Newton update in the (e1, e2) plane at (phi, theta)
+//     (catalogues.h:627-660) with the identity for iterations 0-2 and the
+//     inverse infinitesimal Jacobian afterwards (catalogues.h:509-517);
+//     more than max_iterations rounds -> rejected (iterations = 42)
+//   * otherwise converged: observed angles, Jacobian and the six redshifts
+//     of catalogues.h:682-719.
+#include "mpf_kernels.h"
+
+namespace mpf {
+
+namespace {
+
+enum { CA = 0, CT, CX, CY, CZ, CKT, CKX, CKY, CKZ };
+
+// launch photons from the current angles (Integrator::launch, integrator.h:304-329)
+__global__ void cat_launch_kernel(const double* __restrict__ ang, const uint32_t* __restrict__ act, uint64_t nact,
+                                  double* __restrict__ in9, double* __restrict__ dist_act,
+                                  const double* __restrict__ dist) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nact) return;
+    const uint32_t i = act[k];
+    const double phi = ang[2 * i], theta = ang[2 * i + 1];
+    double* r = in9 + 9 * k;
+    r[CA] = 1.0; r[CT] = 0.0; r[CX] = 0.0; r[CY] = 0.0; r[CZ] = 0.0; r[CKT] = 1.0;
+    r[CKX] = cos(phi) * sin(theta);
+    r[CKY] = sin(phi) * sin(theta);
+    r[CKZ] = cos(theta);
+    dist_act[k] = dist[i];
+}
+
+// comoving distance and angles of each source (catalogues.h:834-836)
+__global__ void cat_init_kernel(const double* __restrict__ src, const double* __restrict__ init, uint64_t n,
+                                double* __restrict__ dist, double* __restrict__ ang, double* __restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double* s = src + 6 * i;
+    const double D = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+    const double theta = acos(s[2] / D);
+    const double phi = atan2(s[1], s[0]);
+    dist[i] = D;
+    ang[2 * i] = init ? init[2 * i] : phi;
+    ang[2 * i + 1] = init ? init[2 * i + 1] : theta;
+    double* o = out + (uint64_t)MPF_NCAT * i;
+    for (int k = 0; k < MPF_NCAT; ++k) o[k] = 0.0;
+    o[MPF_CAT_PHI] = phi;
+    o[MPF_CAT_THETA] = theta;
+}
+
+struct NewtonArgs {
+    const double* src;       // n x 6
+    const double* dist;      // n
+    double* ang;             // n x 2 current angles
+    double* out;             // n x MPF_NCAT
+    const double* surf;      // nact x SF_NREC
+    const uint32_t* act;     // active source ids
+    uint64_t nact;
+    uint32_t* next;          // next round's active ids
+    unsigned int* nnext;
+    uint32_t* need_jac;      // converged without a Jacobian: final pass
+    unsigned int* nneed;
+    int round, jac, max_iter;
+    double acc, length, h, mpc, vobs[3], a0, phi0;
+};
+
+__global__ void newton_kernel(NewtonArgs N) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N.nact) return;
+    const double c = C_LIGHT, c2 = C2;
+    const uint32_t i = N.act[k];
+    const double* s = N.surf + (uint64_t)SF_NREC * k;
+    const double* tt = N.src + 6 * i;
+    double* o = N.out + (uint64_t)MPF_NCAT * i;
+    const double D = N.dist[i];
+    const double phi = N.ang[2 * i], theta = N.ang[2 * i + 1];
+    if (N.jac) {
+        o[MPF_CAT_A11] = s[SF_A11]; o[MPF_CAT_A12] = s[SF_A12];
+        o[MPF_CAT_A21] = s[SF_A12]; o[MPF_CAT_A22] = s[SF_A22];
+    }
+    if (s[SF_CROSSED] == 0.0) {  // the ray never reached the source radius
+        o[MPF_CAT_PHI_OBS] = 42; o[MPF_CAT_THETA_OBS] = 42; o[MPF_CAT_SEP] = 0;
+        // firstid of catalogues.h:433-439: the last element minus one
+        o[MPF_CAT_ITER] = s[SF_FIRSTID] - (s[SF_FIRSTID] > 0.0 ? 1.0 : 0.0);
+        o[MPF_CAT_STATUS] = MPF_CAT_OUTSIDE;
+        o[MPF_CAT_NROUNDS] = N.round + 1;
+        return;
+    }
+    const double fx = s[SF_X], fy = s[SF_Y], fz = s[SF_Z];
+    const double sep = sqrt(pow(tt[0] - fx, 2) + pow(tt[1] - fy, 2) + pow(tt[2] - fz, 2)) / D;
+    o[MPF_CAT_SEP] = sep;
+    o[MPF_CAT_NROUNDS] = N.round + 1;
+    if (sep > N.acc) {
+        double j00 = 1, j01 = 0, j10 = 0, j11 = 1;
+        if (N.round >= 3) {  // inverse of the infinitesimal Jacobian (utility.h:687-698), a21 = a12
+            const double a11 = s[SF_A11], a12 = s[SF_A12], a21 = s[SF_A12], a22 = s[SF_A22];
+            const double invdet = 1. / (a11 * a22 - a12 * a21);
+            j00 = a22 * invdet; j01 = -a12 * invdet; j10 = -a21 * invdet; j11 = a11 * invdet;
+        }
+        const double cp = cos(phi), sp = sin(phi), ct = cos(theta), st = sin(theta);
+        const double e1[3] = {-sp, cp, 0.0};
+        const double e2[3] = {-cp * ct, -sp * ct, st};
+        const double tg[3] = {D * cp * st, D * sp * st, D * ct};
+        const double sv[3] = {fx - tt[0], fy - tt[1], fz - tt[2]};
+        const double pv[3] = {tg[0] - tt[0], tg[1] - tt[1], tg[2] - tt[2]};
+        const double sepx = sv[0] * e1[0] + sv[1] * e1[1];
+        const double sepy = sv[0] * e2[0] + sv[1] * e2[1] + sv[2] * e2[2];
+        const double posx = pv[0] * e1[0] + pv[1] * e1[1];
+        const double posy = pv[0] * e2[0] + pv[1] * e2[1] + pv[2] * e2[2];
+        const double sepxnew = j00 * sepx + j01 * sepy;
+        const double sepynew = j10 * sepx + j11 * sepy;
+        const double posxnew = posx - sepxnew;
+        const double posynew = posy - sepynew;
+        const double xn = tt[0] + (posxnew * e1[0] + posynew * e2[0]);
+        const double yn = tt[1] + (posxnew * e1[1] + posynew * e2[1]);
+        const double zn = tt[2] + posynew * e2[2];
+        const double nphi = atan2(yn, xn);
+        const double ntheta = acos(zn / sqrt(xn * xn + yn * yn + zn * zn));
+        N.ang[2 * i] = nphi;
+        N.ang[2 * i + 1] = ntheta;
+        if (N.round + 1 > N.max_iter) {  // rejected (catalogues.h:775-778)
+            o[MPF_CAT_PHI_OBS] = nphi; o[MPF_CAT_THETA_OBS] = ntheta;
+            o[MPF_CAT_ITER] = 42;
+            o[MPF_CAT_STATUS] = MPF_CAT_REJECTED;
+            return;
+        }
+        N.next[atomicAdd(N.nnext, 1u)] = i;
+        return;
+    }
+    // converged (catalogues.h:665-719)
+    o[MPF_CAT_PHI_OBS] = phi;
+    o[MPF_CAT_THETA_OBS] = theta;
+    o[MPF_CAT_ITER] = N.round + 1;
+    o[MPF_CAT_STATUS] = MPF_CAT_OK;
+    if (!N.jac) N.need_jac[atomicAdd(N.nneed, 1u)] = i;
+    const double as = s[SF_A];
+    const double z1 = s[SF_REDSHIFT];
+    const double potential = s[SF_PHI], isw = s[SF_ISW];
+    const double* v = tt + 3;
+    const double unit_t = as * as * N.mpc / (N.h * 100.0 * 1000.0);
+    const double unit_l = as * N.length * 100;
+    const double vx = v[0] * unit_l * 1e-2 / (unit_t * c);
+    const double vy = v[1] * unit_l * 1e-2 / (unit_t * c);
+    const double vz = v[2] * unit_l * 1e-2 / (unit_t * c);
+    // launch element: k normalised with the potential at the observer (integrator.h:673-676)
+    const double k0[3] = {cos(phi) * sin(theta), sin(phi) * sin(theta), cos(theta)};
+    const double norm = sqrt((c2 * (1.0 + 2.0 * (N.phi0 / c2)) * 1.0 * 1.0) /
+                             ((1.0 - 2.0 * (N.phi0 / c2)) * (k0[0] * k0[0] + k0[1] * k0[1] + k0[2] * k0[2])));
+    const double kx0 = k0[0] * norm, ky0 = k0[1] * norm, kz0 = k0[2] * norm, kt0 = 1.0;
+    const double* vo = N.vobs;
+    const double gref = -N.a0 * c * kt0 * (1. + N.phi0 / c2 + (vo[0] * kx0 + vo[1] * ky0 + vo[2] * kz0) / (c * kt0) +
+                                          0.5 * (vo[0] * vo[0] + vo[1] * vo[1] + vo[2] * vo[2]));
+    const double dtdl = s[SF_KT], dxdl = s[SF_KX], dydl = s[SF_KY], dzdl = s[SF_KZ];
+    const double gK = -as * c * dtdl * (1. + potential / c2 + (vx * dxdl + vy * dydl + vz * dzdl) / (c * dtdl) +
+                                        0.5 * (vx * vx + vy * vy + vz * vz));
+    o[MPF_CAT_Z0] = -1 + 1. / as;
+    o[MPF_CAT_Z1] = z1;
+    o[MPF_CAT_Z2] = z1 + (vx * dxdl + vy * dydl + vz * dzdl) / (as * c * dtdl) -
+                    (vo[0] * kx0 + vo[1] * ky0 + vo[2] * kz0) / (as * c * kt0);
+    o[MPF_CAT_Z3] = o[MPF_CAT_Z2] + 0.5 * (vx * vx + vy * vy + vz * vz - vo[0] * vo[0] - vo[1] * vo[1] - vo[2] * vo[2]) / as;
+    o[MPF_CAT_Z4] = o[MPF_CAT_Z3] + isw / as;
+    o[MPF_CAT_Z5] = gK / gref - 1;
+}
+
+// Jacobian of the converged ray, from a pass with the Jacobian on
+__global__ void cat_jac_kernel(const uint32_t* __restrict__ act, uint64_t nact, const double* __restrict__ surf,
+                               double* __restrict__ out) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nact) return;
+    const double* s = surf + (uint64_t)SF_NREC * k;
+    double* o = out + (uint64_t)MPF_NCAT * act[k];
+    o[MPF_CAT_A11] = s[SF_A11]; o[MPF_CAT_A12] = s[SF_A12];
+    o[MPF_CAT_A21] = s[SF_A12]; o[MPF_CAT_A22] = s[SF_A22];
+}
+
+inline unsigned blocks_for(uint64_t n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace
+
+hipError_t launch_cat_init(const double* src, const double* init, uint64_t n, double* dist, double* ang, double* out,
+                           hipStream_t st) {
+    if (n) hipLaunchKernelGGL(cat_init_kernel, dim3(blocks_for(n)), dim3(256), 0, st, src, init, n, dist, ang, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_cat_launch(const double* ang, const uint32_t* act, uint64_t nact, double* in9, double* dist_act,
+                             const double* dist, hipStream_t st) {
+    if (nact) hipLaunchKernelGGL(cat_launch_kernel, dim3(blocks_for(nact)), dim3(256), 0, st, ang, act, nact, in9,
+                                 dist_act, dist);
+    return hipGetLastError();
+}
+
+hipError_t launch_newton(const CatRound& R, hipStream_t st) {
+    NewtonArgs N;
+    N.src = R.src; N.dist = R.dist; N.ang = R.ang; N.out = R.out; N.surf = R.surf; N.act = R.act; N.nact = R.nact;
+    N.next = R.next; N.nnext = R.nnext; N.need_jac = R.need_jac; N.nneed = R.nneed;
+    N.round = R.round; N.jac = R.jac; N.max_iter = R.max_iter; N.acc = R.acc; N.length = R.length; N.h = R.h;
+    N.mpc = R.mpc; N.a0 = R.a0; N.phi0 = R.phi0;
+    for (int k = 0; k < 3; ++k) N.vobs[k] = R.vobs[k];
+    if (R.nact) hipLaunchKernelGGL(newton_kernel, dim3(blocks_for(R.nact)), dim3(256), 0, st, N);
+    return hipGetLastError();
+}
+
+hipError_t launch_cat_jac(const uint32_t* act, uint64_t nact, const double* surf, double* out, hipStream_t st) {
+    if (nact) hipLaunchKernelGGL(cat_jac_kernel, dim3(blocks_for(nact)), dim3(256), 0, st, act, nact, surf, out);
+    return hipGetLastError();
+}
+
+}  // namespace mpf

@@ -299,6 +299,7 @@ __device__ __forceinline__ void surf_finish(const IntegrateArgs& A, const RaySta
             o[SF_FIRSTID] = (double)size_minus_one;
         } else if (o[SF_FIRSTID] >= (double)size_minus_one) {
             o[SF_VALID] = 0.0;
+            o[SF_CROSSED] = 0.0;
         }
     }
 }
@@ -418,11 +419,13 @@ __global__ void surface_final_kernel(IntegrateArgs A) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= A.n * (uint64_t)A.nsurf) return;
     double* o = A.surf_out + (uint64_t)SF_NREC * i;
-    if (!A.jac || o[SF_VALID] == 0.0) {
+    if (!A.jac || o[SF_CROSSED] == 0.0) {
         o[SF_A11] = 1.0; o[SF_A12] = 0.0; o[SF_A22] = 1.0;
         return;
     }
-    const double D = o[SF_DIST];
+    // the catalogue's Jacobian is taken at the source distance
+    // (catalogues.h:516), the maps' at the interpolated position's (hmaps.h:667)
+    const double D = (A.ray_value && A.surf_kind == 0) ? o[SF_REF] : o[SF_DIST];
     double a[3];
     for (int c = 0; c < 3; ++c) a[c] = o[SF_S1 + c] - o[SF_S1 + 3 + c] / D;
     o[SF_A11] = 1 - 2 * a[0] / C2 * A.length;

@@ -48,6 +48,27 @@ hipError_t launch_pixel_fill(int64_t nside, const int64_t* pix, const double* su
 hipError_t launch_pixel_launch(int64_t nside, const int64_t* pix, uint64_t n, double* in9, hipStream_t st);
 hipError_t launch_vec2pix(int64_t nside, const double* vec, uint64_t n, int64_t* pix, hipStream_t st);
 hipError_t launch_pix2vec(int64_t nside, const int64_t* pix, uint64_t n, double* vec, hipStream_t st);
+struct CatRound {
+    const double* src = nullptr;
+    const double* dist = nullptr;
+    double* ang = nullptr;
+    double* out = nullptr;
+    const double* surf = nullptr;
+    const uint32_t* act = nullptr;
+    uint64_t nact = 0;
+    uint32_t* next = nullptr;
+    unsigned int* nnext = nullptr;
+    uint32_t* need_jac = nullptr;
+    unsigned int* nneed = nullptr;
+    int round = 0, jac = 0, max_iter = 10;
+    double acc = 1e-8, length = 1.0, h = 0.7, mpc = 3.08568e22, vobs[3] = {0, 0, 0}, a0 = 1.0, phi0 = 0.0;
+};
+hipError_t launch_cat_init(const double* src, const double* init, uint64_t n, double* dist, double* ang, double* out,
+                           hipStream_t st);
+hipError_t launch_cat_launch(const double* ang, const uint32_t* act, uint64_t nact, double* in9, double* dist_act,
+                             const double* dist, hipStream_t st);
+hipError_t launch_newton(const CatRound& R, hipStream_t st);
+hipError_t launch_cat_jac(const uint32_t* act, uint64_t nact, const double* surf, double* out, hipStream_t st);
 hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v, hipStream_t st);
 
 }  // namespace mpf

@@ -562,7 +562,10 @@ integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
                         for (int k = 0; k < SF_NREC; ++k) so[k] = 0.0;
                         so[SF_FIRSTID] = (double)last;
                     } else {
-                        if (so[SF_FIRSTID] >= (double)last) so[SF_VALID] = 0.0;
+                        if (so[SF_FIRSTID] >= (double)last) {
+                            so[SF_VALID] = 0.0;
+                            so[SF_CROSSED] = 0.0;
+                        }
                         so[SF_RHO] = __builtin_nan("");
                         so[SF_RHOMAX] = __builtin_nan("");
                     }

@@ -28,6 +28,8 @@ enum {
     SF_VALID = 0, SF_FIRSTID, SF_F, SF_X, SF_Y, SF_Z, SF_DIST, SF_CHI, SF_LAMBDA, SF_T, SF_A, SF_REDSHIFT, SF_S,
     SF_ISW, SF_RHO, SF_PHI, SF_RHOMAX, SF_KT, SF_KX, SF_KY, SF_KZ, SF_A11, SF_A12, SF_A22,
     SF_S1 = 24,  // 24..26: S1 snapshot (11, 12, 22); 27..29: S2 snapshot
+    SF_REF = 30,      // the surface value
+    SF_CROSSED = 31,  // the surface was passed (valid or not)
     SF_NREC = 32
 };
 
@@ -80,6 +82,8 @@ __device__ __forceinline__ void surf_append(SurfAcc& s, int kind, int nsurf, con
         double* o = rec + (uint64_t)SF_NREC * j;
         const double f = (vn - ref) / (vn - vp);
         o[SF_VALID] = (i >= 2) ? 1.0 : 0.0;   // firstid == 0 is rejected (hmaps.h:716)
+        o[SF_REF] = ref;
+        o[SF_CROSSED] = 1.0;
         o[SF_FIRSTID] = (double)(i - 1);
         o[SF_F] = f;
         o[SF_X] = pv.x * f + nw.x * (1 - f);

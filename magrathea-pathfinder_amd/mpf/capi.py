@@ -31,7 +31,14 @@ EXPORTS = ("mpf_create", "mpf_destroy", "mpf_last_error", "mpf_set_precision", "
            "mpf_integrate", "mpf_integrate_dev", "mpf_last_run", "mpf_synchronize", "mpf_validate_octree",
            "mpf_dev_alloc", "mpf_dev_free", "mpf_dev_upload", "mpf_dev_download", "mpf_dev_copy",
            "mpf_integrate_surfaces", "mpf_integrate_surfaces_dev", "mpf_healpix_pix2vec", "mpf_healpix_vec2pix",
-           "mpf_hmaps_launch_dev", "mpf_hmaps_fill_dev", "mpf_healpix_bin_dev")
+           "mpf_hmaps_launch_dev", "mpf_hmaps_fill_dev", "mpf_healpix_bin_dev", "mpf_catalogue_solve",
+           "mpf_catalogue_solve_dev")
+
+CAT_FIELDS = ("phi", "theta", "phi_obs", "theta_obs", "sep", "iter", "z0", "z1", "z2", "z3", "z4", "z5",
+              "a11", "a12", "a21", "a22", "status", "nrounds")
+CAT = {name: i for i, name in enumerate(CAT_FIELDS)}
+NCAT = 20
+CAT_OK, CAT_OUTSIDE, CAT_REJECTED = 0, 1, 2
 
 SURF_FIELDS = ("valid", "firstid", "f", "x", "y", "z", "dist", "chi", "lambda", "t", "a", "redshift", "s", "isw",
                "rho", "phi", "rhomax", "kt", "kx", "ky", "kz", "a11", "a12", "a22")
@@ -66,6 +73,22 @@ def make_surfaces(kind, values, jacobian=True, ray_value=None):
     s.jacobian = int(bool(jacobian))
     s.ray_value = ray_value
     return s
+
+
+class NewtonCfg(ctypes.Structure):
+    _fields_ = [("accuracy", ctypes.c_double), ("max_iterations", ctypes.c_uint32), ("vobs", ctypes.c_double * 3),
+                ("h", ctypes.c_double), ("mpc", ctypes.c_double)]
+
+
+def make_newton(accuracy=1e-8, max_iterations=10, vobs=(0.0, 0.0, 0.0), h=0.7, mpc=3.08568e22):
+    c = NewtonCfg()
+    c.accuracy = accuracy
+    c.max_iterations = max_iterations
+    for i in range(3):
+        c.vobs[i] = vobs[i]
+    c.h = h
+    c.mpc = mpc
+    return c
 
 
 class Stop(ctypes.Structure):
@@ -107,6 +130,8 @@ def load_library(path=LIB_PATH):
     lib.mpf_dev_copy.argtypes = [vp, vp, vp, u64]
     lib.mpf_integrate_surfaces.argtypes = [vp, vp, u64, ctypes.POINTER(Surfaces), vp, vp]
     lib.mpf_integrate_surfaces_dev.argtypes = [vp, vp, u64, ctypes.POINTER(Surfaces), vp, vp, i32]
+    lib.mpf_catalogue_solve.argtypes = [vp, vp, vp, u64, ctypes.POINTER(NewtonCfg), vp, vp]
+    lib.mpf_catalogue_solve_dev.argtypes = [vp, vp, vp, u64, ctypes.POINTER(NewtonCfg), vp, vp]
     i64 = ctypes.c_int64
     lib.mpf_healpix_pix2vec.argtypes = [i64, vp, u64, vp]
     lib.mpf_healpix_vec2pix.argtypes = [i64, vp, u64, vp]
@@ -269,6 +294,26 @@ class Context:
                                                         ctypes.byref(sf), ctypes.c_void_p(out_ptr),
                                                         ctypes.c_void_p(surf_ptr), int(bool(sync))),
                     "mpf_integrate_surfaces_dev")
+
+    def catalogue_solve(self, src6, init2=None, **newton):
+        """Newton root-finding per source; returns (n x NCAT records, rays integrated)."""
+        src6 = np.ascontiguousarray(src6, dtype=np.float64).reshape(-1, 6)
+        n = src6.shape[0]
+        i2 = None if init2 is None else np.ascontiguousarray(init2, dtype=np.float64).reshape(-1, 2)
+        cfg = make_newton(**newton)
+        out = np.zeros((n, NCAT))
+        rays = ctypes.c_uint64(0)
+        self._check(self.lib.mpf_catalogue_solve(self.h, _p(src6), _p(i2), ctypes.c_uint64(n), ctypes.byref(cfg),
+                                                 _p(out), ctypes.byref(rays)), "mpf_catalogue_solve")
+        return out, rays.value
+
+    def catalogue_solve_dev(self, src_ptr, init_ptr, n, out_ptr, **newton):
+        cfg = make_newton(**newton)
+        rays = ctypes.c_uint64(0)
+        self._check(self.lib.mpf_catalogue_solve_dev(self.h, ctypes.c_void_p(src_ptr), ctypes.c_void_p(init_ptr),
+                                                     ctypes.c_uint64(n), ctypes.byref(cfg), ctypes.c_void_p(out_ptr),
+                                                     ctypes.byref(rays)), "mpf_catalogue_solve_dev")
+        return rays.value
 
     def hmaps_launch_dev(self, nside, pix_ptr, n, in9_ptr):
         self._check(self.lib.mpf_hmaps_launch_dev(self.h, int(nside), ctypes.c_void_p(pix_ptr), ctypes.c_uint64(n),

@@ -965,6 +965,181 @@ uint64_t mpfo_surfaces(const mpfo_octree* o, int order, int precision, const dou
     return n;
 }
 
+/* one newtonMethod2d call (catalogues.h:410-720) */
+typedef struct {
+    double phi_new, theta_new, sep, firstid;
+    int fail, converged;
+    double jac[3];
+    int have_jac;
+    double z[6];
+} newton_res;
+
+static void newton_once(const mpfo_octree* o, int order, int precision, const double* tab, uint64_t ncos,
+                        double length, unsigned nsteps, const double* tt, const double* vel, double phi, double theta,
+                        unsigned iteration, double accuracy, const double* vobs, double h, double mpc,
+                        newton_res* r) {
+    const double c = C_LIGHT, c2 = C2;
+    memset(r, 0, sizeof(*r));
+    const double distTarget = sqrt(pow(tt[0], 2) + pow(tt[1], 2) + pow(tt[2], 2));
+    double in9[9] = {1.0, 0.0, 0.0, 0.0, 0.0, 1.0, cos(phi) * sin(theta), sin(phi) * sin(theta), cos(theta)};
+    traj_t t = {NULL, 0, 0, 1};
+    run_full(o, order, precision, tab, ncos, length, nsteps, 0, distTarget, NULL, in9, &t);
+    const uint64_t marked = t.n - 1;
+    const uint64_t firstid = marked - (marked > 0);
+    if (marked == 0 || t.p[marked].chi < distTarget) {
+        r->fail = 1;
+        r->firstid = (double)firstid;
+        free(t.p);
+        return;
+    }
+    const double previous = t.p[firstid].chi, next = t.p[firstid + 1].chi;
+    const double f = (next - distTarget) / (next - previous);
+    const photon* A = &t.p[firstid];
+    const photon* B = &t.p[firstid + 1];
+    const double z1 = A->redshift * f + B->redshift * (1 - f);
+    const double as = A->core[CA] * f + B->core[CA] * (1 - f);
+    double fp[3];
+    for (int k = 0; k < 3; ++k) fp[k] = A->core[CX + k] * f + B->core[CX + k] * (1 - f);
+    const double dist_sep = sqrt(pow(tt[0] - fp[0], 2) + pow(tt[1] - fp[1], 2) + pow(tt[2] - fp[2], 2));
+    r->sep = dist_sep / distTarget;
+    if (r->sep > accuracy) {
+        double ji[2][2] = {{1, 0}, {0, 1}};
+        if (iteration >= 3) {
+            jacobian_at(o, precision, &t, distTarget, length, r->jac);
+            r->have_jac = 1;
+            const double a11 = r->jac[0], a12 = r->jac[1], a21 = r->jac[1], a22 = r->jac[2];
+            const double invdet = 1. / (a11 * a22 - a12 * a21);   /* utility.h:687-698 */
+            ji[0][0] = a22 * invdet; ji[0][1] = -a12 * invdet; ji[1][0] = -a21 * invdet; ji[1][1] = a11 * invdet;
+        }
+        const double cp = cos(phi), sp = sin(phi), ct = cos(theta), st = sin(theta);
+        const double e1[3] = {-sp, cp, 0}, e2[3] = {-cp * ct, -sp * ct, st};
+        const double targeted[3] = {distTarget * cp * st, distTarget * sp * st, distTarget * ct};
+        double sep[3], pos[3];
+        for (int k = 0; k < 3; ++k) { sep[k] = fp[k] - tt[k]; pos[k] = targeted[k] - tt[k]; }
+        const double sepx = sep[0] * e1[0] + sep[1] * e1[1];
+        const double sepy = sep[0] * e2[0] + sep[1] * e2[1] + sep[2] * e2[2];
+        const double posx = pos[0] * e1[0] + pos[1] * e1[1];
+        const double posy = pos[0] * e2[0] + pos[1] * e2[1] + pos[2] * e2[2];
+        const double sepxnew = ji[0][0] * sepx + ji[0][1] * sepy;
+        const double sepynew = ji[1][0] * sepx + ji[1][1] * sepy;
+        const double posxnew = posx - sepxnew, posynew = posy - sepynew;
+        const double xnew = tt[0] + (posxnew * e1[0] + posynew * e2[0]);
+        const double ynew = tt[1] + (posxnew * e1[1] + posynew * e2[1]);
+        const double znew = tt[2] + posynew * e2[2];
+        r->phi_new = atan2(ynew, xnew);
+        r->theta_new = acos(znew / sqrt(xnew * xnew + ynew * ynew + znew * znew));
+        free(t.p);
+        return;
+    }
+    r->converged = 1;
+    jacobian_at(o, precision, &t, distTarget, length, r->jac);
+    r->have_jac = 1;
+    const double potential = A->phi * f + B->phi * (1 - f);
+    const double isw = A->isw * f + B->isw * (1 - f);
+    r->z[0] = -1 + 1. / (as);
+    r->z[1] = z1;
+    const double unit_t = as * as * mpc / (h * 100 * 1000);
+    const double unit_l = as * length * 100;
+    const double vx = vel[0] * unit_l * 1e-2 / (unit_t * c);
+    const double vy = vel[1] * unit_l * 1e-2 / (unit_t * c);
+    const double vz = vel[2] * unit_l * 1e-2 / (unit_t * c);
+    const photon* F = &t.p[0];
+    const double gref = -F->core[CA] * c * F->core[CKT] *
+                        (1. + F->phi / c2 + (vobs[0] * F->core[CKX] + vobs[1] * F->core[CKY] + vobs[2] * F->core[CKZ]) / (c * F->core[CKT]) +
+                         0.5 * (vobs[0] * vobs[0] + vobs[1] * vobs[1] + vobs[2] * vobs[2]));
+    const double dtdl = A->core[CKT] * f + B->core[CKT] * (1 - f);
+    const double dxdl = A->core[CKX] * f + B->core[CKX] * (1 - f);
+    const double dydl = A->core[CKY] * f + B->core[CKY] * (1 - f);
+    const double dzdl = A->core[CKZ] * f + B->core[CKZ] * (1 - f);
+    const double gK = -as * c * dtdl * (1. + potential / c2 + (vx * dxdl + vy * dydl + vz * dzdl) / (c * dtdl) +
+                                        0.5 * (vx * vx + vy * vy + vz * vz));
+    r->z[2] = r->z[1] + (vx * dxdl + vy * dydl + vz * dzdl) / (as * c * dtdl) -
+              (vobs[0] * F->core[CKX] + vobs[1] * F->core[CKY] + vobs[2] * F->core[CKZ]) / (as * c * F->core[CKT]);
+    r->z[3] = r->z[2] + 0.5 * (vx * vx + vy * vy + vz * vz - vobs[0] * vobs[0] - vobs[1] * vobs[1] - vobs[2] * vobs[2]) / as;
+    r->z[4] = r->z[3] + isw / as;
+    r->z[5] = gK / gref - 1;
+    free(t.p);
+}
+
+typedef struct {
+    const mpfo_octree* o;
+    int order, precision;
+    const double* tab;
+    uint64_t ncos;
+    double length;
+    unsigned nsteps;
+    const double *src6, *init2;
+    double accuracy;
+    unsigned max_it;
+    const double* vobs;
+    double h, mpc;
+    double* out;
+    uint64_t begin, end, rays;
+} cat_work;
+
+static void* cat_worker(void* arg) {
+    cat_work* w = (cat_work*)arg;
+    w->rays = 0;
+    for (uint64_t i = w->begin; i < w->end; ++i) {
+        const double* s = w->src6 + 6 * i;
+        double* o = w->out + 20 * i;
+        memset(o, 0, 20 * sizeof(double));
+        const double d = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+        o[0] = atan2(s[1], s[0]);
+        o[1] = acos(s[2] / d);
+        double phi = w->init2 ? w->init2[2 * i] : o[0];
+        double theta = w->init2 ? w->init2[2 * i + 1] : o[1];
+        unsigned iteration = 0;
+        for (;;) {
+            newton_res r;
+            newton_once(w->o, w->order, w->precision, w->tab, w->ncos, w->length, w->nsteps, s, s + 3, phi, theta,
+                        iteration, w->accuracy, w->vobs, w->h, w->mpc, &r);
+            w->rays++;
+            iteration++;
+            o[17] = iteration;
+            if (r.have_jac) { o[12] = r.jac[0]; o[13] = r.jac[1]; o[14] = r.jac[1]; o[15] = r.jac[2]; }
+            if (r.fail) { o[2] = 42; o[3] = 42; o[4] = 0; o[5] = r.firstid; o[16] = 1; break; }
+            o[4] = r.sep;
+            if (r.converged) {
+                o[2] = phi; o[3] = theta; o[5] = iteration; o[16] = 0;
+                for (int k = 0; k < 6; ++k) o[6 + k] = r.z[k];
+                break;
+            }
+            phi = r.phi_new;
+            theta = r.theta_new;
+            if (iteration > w->max_it) { o[2] = phi; o[3] = theta; o[5] = 42; o[16] = 2; break; }
+        }
+    }
+    return NULL;
+}
+
+uint64_t mpfo_catalogue(const mpfo_octree* o, int order, int precision, const double* cosmo_tab, uint64_t ncos,
+                        double length, unsigned nsteps, const double* src6, const double* init2, uint64_t n,
+                        double accuracy, unsigned max_iterations, const double* vobs, double h, double mpc,
+                        double* out, int nthreads) {
+    if (nthreads < 1) nthreads = 1;
+    if ((uint64_t)nthreads > n) nthreads = n ? (int)n : 1;
+    cat_work* w = (cat_work*)calloc((size_t)nthreads, sizeof(cat_work));
+    pthread_t* th = (pthread_t*)calloc((size_t)nthreads, sizeof(pthread_t));
+    const uint64_t chunk = (n + nthreads - 1) / nthreads;
+    for (int k = 0; k < nthreads; ++k) {
+        w[k] = (cat_work){o, order, precision, cosmo_tab, ncos, length, nsteps, src6, init2, accuracy, max_iterations,
+                          vobs, h, mpc, out, 0, 0, 0};
+        w[k].begin = chunk * k < n ? chunk * k : n;
+        w[k].end = chunk * (k + 1) < n ? chunk * (k + 1) : n;
+    }
+    for (int k = 1; k < nthreads; ++k) pthread_create(&th[k], NULL, cat_worker, &w[k]);
+    cat_worker(&w[0]);
+    uint64_t rays = w[0].rays;
+    for (int k = 1; k < nthreads; ++k) {
+        pthread_join(th[k], NULL);
+        rays += w[k].rays;
+    }
+    free(w);
+    free(th);
+    return rays;
+}
+
 void mpfo_interpolate(const double* x, const double* y, uint64_t n, const double* x0, uint64_t m, double* out) {
     for (uint64_t i = 0; i < m; ++i) out[i] = interpolate(x0[i], x, y, (int64_t)n);
 }

@@ -55,6 +55,15 @@ uint64_t mpfo_jacobian_infinitesimal(const mpfo_octree* o, int order, int precis
 uint64_t mpfo_surfaces(const mpfo_octree* o, int order, int precision, const double* cosmo_tab, uint64_t ncos,
                        double length, unsigned nsteps, int kind, const double* vals, uint32_t nsurf, int jacobian,
                        const double* in9, double* out);
+/* Catalogues::iterateNewtonMethod + newtonMethod2d (catalogues.h:410-800),
+ * jacobiantype = infinitesimal, use_previous_catalogues = 0, observer at the
+ * origin.  src6: x y z vx vy vz per source; init2 (may be NULL) first guesses;
+ * cfg: accuracy, max_iterations, vobs[3], h, mpc.  out: 20 doubles per source
+ * (libmpf's MPF_CAT_* layout).  Returns the rays integrated. */
+uint64_t mpfo_catalogue(const mpfo_octree* o, int order, int precision, const double* cosmo_tab, uint64_t ncos,
+                        double length, unsigned nsteps, const double* src6, const double* init2, uint64_t n,
+                        double accuracy, unsigned max_iterations, const double* vobs, double h, double mpc,
+                        double* out, int nthreads);
 void mpfo_interpolate(const double* x, const double* y, uint64_t n, const double* x0, uint64_t m, double* out);
 
 #ifdef __cplusplus
