@@ -163,6 +163,13 @@ typedef struct {
     double vobs[3];           /* observer velocity (SI) */
     double h;                 /* dimensionless Hubble parameter */
     double mpc;               /* one megaparsec in metres (parameter file `mpc`) */
+    /* 0: the Jacobian is integrated with the rays of rounds >= 3 (where the
+     * Newton update needs it, catalogues.h:509-517) and, for sources that
+     * converged earlier, by one more ray at the converged direction;
+     * 1: with every round's rays, no extra ray.  Same results; 1 integrates
+     * fewer rays when most sources converge within three rounds (a good
+     * first guess), more when they need many rounds. */
+    uint32_t jacobian_all_rounds;
 } mpf_newton_cfg;
 /* Per source, MPF_NCAT doubles: the catalogue columns of catalogues.h:844-859
  * plus status and the number of rounds run. */
@@ -185,6 +192,73 @@ int mpf_catalogue_solve(mpf_ctx* ctx, const double* src6, const double* init2, u
 /* Same on device buffers (sync on return). */
 int mpf_catalogue_solve_dev(mpf_ctx* ctx, const double* dev_src6, const double* dev_init2, uint64_t n,
                             const mpf_newton_cfg* cfg, double* dev_out, uint64_t* total_rays);
+
+/* Learned first guess of the catalogue root-finder.  No counterpart in the
+ * reference: it proposes the observed angles that seed the Newton
+ * iterations (init2 of mpf_catalogue_solve; the reference starts from the
+ * unlensed direction, catalogues.h:832-836), which still finish each source
+ * to the accuracy.  A 10 -> 128 -> 128 -> 2 GELU network (BF16 operands, FP32
+ * accumulation on the matrix cores, FP32 master weights, Adam), one per
+ * context, trained on pairs from the root-finder itself.
+ *
+ * Features per source (MPF_MLP_NIN floats), from the octree along the
+ * straight line of sight (TSC at chi_k = (k + 1/2) D / 128, k < 128: exact
+ * FP64 with one stencil cache per line, or the split tier's FP32 stencil
+ * blocks when the context precision is MPF_PREC_SPLIT), invariant under
+ * rotations of the sky:
+ *   0 comoving distance D (box units); 1, 2 Born deflection
+ *   2/c^2 sum (D - chi)/D grad_perp(phi) dchi along e_phi and e_theta
+ *   (arcsec); 3, 4 the same over the far half (k >= 64); 5, 6 over the first
+ *   quarter (k < 32); 7, 8 2 D grad_perp(phi) / c^2 at k = 127 (arcsec);
+ *   9 phi / c^2 at k = 63 (units of 1e-5).
+ * Targets (MPF_MLP_NOUT floats): the deflection left after the Born
+ * estimate, (phi_obs - phi) sin(theta) - f1 and theta_obs - theta - f2
+ * (arcsec).  The first guess is phi + (f1 + y0) / sin(theta),
+ * theta + f2 + y1, with y the network's output.
+ *
+ * Parameter vector (MPF_MLP_NPARAM floats), padded to matrix-core tiles:
+ * W1 [16][128] (rows 10..15 zero), b1 [128], W2 [128][128], b2 [128],
+ * W3 [128][16] (columns 2..15 zero), b3 [16] (2..15 zero), y = x W + b.
+ * The state (MPF_MLP_NSTATE floats) appends the normalisation: feature mean
+ * [10], feature std [10], target mean [2], target std [2]; the network sees
+ * (x - mean) / std and its output is scaled back by the target std and mean.
+ * The normalisation is taken from the first training set unless set. */
+#define MPF_MLP_NIN 10
+#define MPF_MLP_NOUT 2
+#define MPF_MLP_NHIDDEN 128
+#define MPF_MLP_KIN 16
+#define MPF_MLP_KOUT 16
+#define MPF_MLP_NPARAM 20752
+#define MPF_MLP_NNORM 24
+#define MPF_MLP_NSTATE (MPF_MLP_NPARAM + MPF_MLP_NNORM)
+/* (Re)initialise the context's network: Glorot-uniform hidden weights from
+ * seed, zero output layer and biases, Adam state cleared, normalisation
+ * unset. */
+int mpf_mlp_init(mpf_ctx* ctx, uint64_t seed);
+/* Read (get != 0) and/or replace (set != 0) the MPF_MLP_NSTATE-float state. */
+int mpf_mlp_state(mpf_ctx* ctx, float* get, const float* set);
+/* Adam (lr, beta 0.9 / 0.999, eps 1e-8) on the mean squared error of the
+ * normalised outputs, over `epochs` passes of shuffled mini-batches (shuffle
+ * from seed).  loss_per_epoch (may be null): mean loss of each epoch. */
+int mpf_mlp_train(mpf_ctx* ctx, const float* feat, const float* target, uint64_t n, uint32_t epochs,
+                  uint32_t batch, float lr, uint64_t seed, float* loss_per_epoch);
+/* Loss and gradient (MPF_MLP_NPARAM floats) of one batch, parameters
+ * unchanged: the training step without the update. */
+int mpf_mlp_gradient(mpf_ctx* ctx, const float* feat, const float* target, uint64_t n, float* grad, float* loss);
+/* Predicted targets (n x MPF_MLP_NOUT). */
+int mpf_mlp_infer(mpf_ctx* ctx, const float* feat, uint64_t n, float* out);
+int mpf_mlp_infer_dev(mpf_ctx* ctx, const float* dev_feat, uint64_t n, float* dev_out);
+/* Features of sources (n x 6 doubles as in mpf_catalogue_solve). */
+int mpf_catalogue_features(mpf_ctx* ctx, const double* src6, uint64_t n, float* feat);
+int mpf_catalogue_features_dev(mpf_ctx* ctx, const double* dev_src6, uint64_t n, float* dev_feat);
+/* First guesses init2 (n x 2: phi, theta) = unlensed direction + Born
+ * estimate + (use_network != 0) the network's correction. */
+int mpf_catalogue_seed(mpf_ctx* ctx, const double* src6, uint64_t n, int use_network, double* init2);
+int mpf_catalogue_seed_dev(mpf_ctx* ctx, const double* dev_src6, uint64_t n, int use_network, double* dev_init2);
+/* One 16 x 16 x 16 product on the matrix cores, c = bf16(a) bf16(b) with FP32
+ * accumulation (row-major 16 x 16 floats): checks the operand layout the
+ * network's kernels assume. */
+int mpf_mfma_probe(mpf_ctx* ctx, const float* a, const float* b, float* c);
 
 /* HEALPix RING scheme (host, no device needed), restated from the published
  * algorithm (Gorski et al. 2005): the reference links HEALPix C++ 3.70 /

@@ -11,6 +11,7 @@
 #include "mpf_split.h"
 #include "mpf_tree.h"
 #include "mpf_healpix.h"
+#include "mpf_mlp.h"
 
 namespace {
 
@@ -90,6 +91,8 @@ struct mpf_ctx {
     uint64_t last_steps = 0;
     float last_ms = 0.f;
     bool pending = false;
+    // learned first guess of the catalogue root-finder
+    mpf::MlpModel* mlp = nullptr;
 };
 
 static int fail(mpf_ctx* c, int code, const std::string& msg) {
@@ -106,6 +109,17 @@ static int hipfail(mpf_ctx* c, hipError_t e, const char* where) {
         hipError_t e_ = (expr);                        \
         if (e_ != hipSuccess) return hipfail(c, e_, #expr); \
     } while (0)
+
+// stencil blocks of the split tier, built on first use after a tree load
+static int ensure_split(mpf_ctx* c) {
+    if (c->split_ready) return 0;
+    std::string err;
+    if (!mpf::split_build(c->split, c->dt, c->lev.data(), c->kx.data(), c->ky.data(), c->kz.data(), c->cfg.order,
+                          c->stream, err))
+        return fail(c, ERR_NOMEM, err);
+    c->split_ready = true;
+    return 0;
+}
 
 extern "C" {
 
@@ -145,6 +159,7 @@ int mpf_destroy(mpf_ctx* c) {
     c->io_in.release(); c->io_out.release(); c->io_surf.release(); c->io_rv.release(); c->scratch.release();
     c->map_acc.release(); c->io_i64.release(); c->io_u32.release();
     mpf::split_release(c->split);
+    mpf::mlp_destroy(c->mlp);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -361,11 +376,7 @@ static int integrate_impl(mpf_ctx* c, const double* dev_in9, uint64_t n, const m
             if (A.nsurf && !mpf::split_supports_surfaces())
                 return fail(c, ERR_INVAL, "surfaces are not available in the split tier");
             if (!c->split_ready) {
-                std::string err;
-                if (!mpf::split_build(c->split, c->dt, c->lev.data(), c->kx.data(), c->ky.data(), c->kz.data(),
-                                      c->cfg.order, c->stream, err))
-                    return fail(c, ERR_NOMEM, err);
-                c->split_ready = true;
+                if (int rc = ensure_split(c)) return rc;
                 HIPCHK(c, hipEventRecord(c->ev0, c->stream));
             }
             if (c->scratch.ensure(16 * n) != hipSuccess) return fail(c, ERR_NOMEM, "launch-record buffer");
@@ -505,7 +516,7 @@ int mpf_catalogue_solve_dev(mpf_ctx* c, const double* dev_src6, const double* de
     unsigned int zero2[2] = {0u, 0u};
     CATCHK(hipMemcpyAsync(cnt.p + 1, zero2, 4, hipMemcpyHostToDevice, c->stream));
     for (int round = 0; nact > 0 && round <= (int)cfg->max_iterations; ++round) {
-        const int jac = round >= 3 ? 1 : 0;
+        const int jac = (round >= 3 || cfg->jacobian_all_rounds) ? 1 : 0;
         CATCHK(mpf::launch_cat_launch(ang.p, act.p, nact, in9.p, dact.p, dist.p, c->stream));
         mpf_surfaces sf{};
         sf.kind = MPF_STOP_RADIUS;
@@ -567,6 +578,187 @@ int mpf_catalogue_solve(mpf_ctx* c, const double* src6, const double* init2, uin
     if (!rc && n && hipMemcpy(out, o.p, (uint64_t)MPF_NCAT * n * 8, hipMemcpyDeviceToHost) != hipSuccess)
         rc = fail(c, ERR_HIP, "catalogue download");
     s.release(); i2.release(); o.release();
+    return rc;
+}
+
+int mpf_mlp_init(mpf_ctx* c, uint64_t seed) {
+    if (!c) return ERR_INVAL;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    mpf::mlp_destroy(c->mlp);
+    c->mlp = nullptr;
+    std::string err;
+    if (!mpf::mlp_create(&c->mlp, c->stream, seed, err)) return fail(c, ERR_HIP, err);
+    return 0;
+}
+
+static int need_mlp(mpf_ctx* c) {
+    if (!c) return ERR_INVAL;
+    if (!c->mlp) return fail(c, ERR_STATE, "no network: call mpf_mlp_init first");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    return 0;
+}
+
+int mpf_mlp_state(mpf_ctx* c, float* get, const float* set) {
+    if (int rc = need_mlp(c)) return rc;
+    if (set) {
+        for (int i = MPF_MLP_NPARAM + MPF_MLP_NIN; i < MPF_MLP_NPARAM + 2 * MPF_MLP_NIN; ++i)
+            if (!(set[i] > 0)) return fail(c, ERR_INVAL, "feature standard deviations must be positive");
+        for (int i = MPF_MLP_NPARAM + 2 * MPF_MLP_NIN + MPF_MLP_NOUT; i < MPF_MLP_NSTATE; ++i)
+            if (!(set[i] > 0)) return fail(c, ERR_INVAL, "target standard deviations must be positive");
+    }
+    std::string err;
+    if (!mpf::mlp_state(c->mlp, get, set, err)) return fail(c, ERR_HIP, err);
+    return 0;
+}
+
+static bool finite_all(const float* p, uint64_t n) {
+    for (uint64_t i = 0; i < n; ++i)
+        if (!std::isfinite(p[i])) return false;
+    return true;
+}
+
+int mpf_mlp_train(mpf_ctx* c, const float* feat, const float* target, uint64_t n, uint32_t epochs, uint32_t batch,
+                  float lr, uint64_t seed, float* loss_per_epoch) {
+    if (int rc = need_mlp(c)) return rc;
+    if (n == 0 || !feat || !target || batch == 0 || !(lr > 0)) return fail(c, ERR_INVAL, "training arguments");
+    if (!finite_all(feat, n * MPF_MLP_NIN) || !finite_all(target, n * MPF_MLP_NOUT))
+        return fail(c, ERR_INVAL, "non-finite training data");
+    std::string err;
+    if (!mpf::mlp_train(c->mlp, feat, target, n, epochs, batch, lr, seed, loss_per_epoch, err))
+        return fail(c, ERR_HIP, err);
+    return 0;
+}
+
+int mpf_mlp_gradient(mpf_ctx* c, const float* feat, const float* target, uint64_t n, float* grad, float* loss) {
+    if (int rc = need_mlp(c)) return rc;
+    if (n == 0 || !feat || !target || !grad) return fail(c, ERR_INVAL, "gradient arguments");
+    if (!finite_all(feat, n * MPF_MLP_NIN) || !finite_all(target, n * MPF_MLP_NOUT))
+        return fail(c, ERR_INVAL, "non-finite batch");
+    std::string err;
+    if (!mpf::mlp_gradient(c->mlp, feat, target, n, grad, loss, err)) return fail(c, ERR_HIP, err);
+    return 0;
+}
+
+int mpf_mlp_infer_dev(mpf_ctx* c, const float* dev_feat, uint64_t n, float* dev_out) {
+    if (int rc = need_mlp(c)) return rc;
+    if (n && (!dev_feat || !dev_out)) return ERR_INVAL;
+    std::string err;
+    if (!mpf::mlp_infer_dev(c->mlp, dev_feat, n, dev_out, err)) return fail(c, ERR_HIP, err);
+    return 0;
+}
+
+int mpf_mlp_infer(mpf_ctx* c, const float* feat, uint64_t n, float* out) {
+    if (int rc = need_mlp(c)) return rc;
+    if (n && (!feat || !out)) return ERR_INVAL;
+    if (n == 0) return 0;
+    DevBuf<float> f, o;
+    if (f.ensure(n * MPF_MLP_NIN) || o.ensure(n * MPF_MLP_NOUT)) {
+        f.release(); o.release();
+        return fail(c, ERR_NOMEM, "inference buffers");
+    }
+    int rc = 0;
+    if (hipMemcpy(f.p, feat, n * MPF_MLP_NIN * 4, hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(c, ERR_HIP, "inference upload");
+    if (!rc) rc = mpf_mlp_infer_dev(c, f.p, n, o.p);
+    if (!rc && hipMemcpy(out, o.p, n * MPF_MLP_NOUT * 4, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = fail(c, ERR_HIP, "inference download");
+    f.release(); o.release();
+    return rc;
+}
+
+int mpf_catalogue_features_dev(mpf_ctx* c, const double* dev_src6, uint64_t n, float* dev_feat) {
+    if (!c || (n && (!dev_src6 || !dev_feat))) return ERR_INVAL;
+    if (!c->have_tree) return fail(c, ERR_STATE, "no octree loaded");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (c->cfg.precision == MPF_PREC_SPLIT && c->cfg.order == MPF_TSC) {
+        if (int rc = ensure_split(c)) return rc;
+        HIPCHK(c, mpf::launch_features_split(c->dt, c->split, dev_src6, n, c->cfg.length, dev_feat, c->stream));
+    } else {
+        HIPCHK(c, mpf::launch_features(c->dt, dev_src6, n, c->cfg.length, dev_feat, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int mpf_catalogue_features(mpf_ctx* c, const double* src6, uint64_t n, float* feat) {
+    if (!c || (n && (!src6 || !feat))) return ERR_INVAL;
+    if (n == 0) return 0;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    DevBuf<double> s;
+    DevBuf<float> f;
+    if (s.ensure(6 * n) || f.ensure(n * MPF_MLP_NIN)) {
+        s.release(); f.release();
+        return fail(c, ERR_NOMEM, "feature buffers");
+    }
+    int rc = 0;
+    if (hipMemcpy(s.p, src6, 6 * n * 8, hipMemcpyHostToDevice) != hipSuccess) rc = fail(c, ERR_HIP, "feature upload");
+    if (!rc) rc = mpf_catalogue_features_dev(c, s.p, n, f.p);
+    if (!rc && hipMemcpy(feat, f.p, n * MPF_MLP_NIN * 4, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = fail(c, ERR_HIP, "feature download");
+    s.release(); f.release();
+    return rc;
+}
+
+int mpf_catalogue_seed_dev(mpf_ctx* c, const double* dev_src6, uint64_t n, int use_network, double* dev_init2) {
+    if (!c) return ERR_INVAL;
+    if (use_network)
+        if (int rc = need_mlp(c)) return rc;
+    if (n && (!dev_src6 || !dev_init2)) return ERR_INVAL;
+    if (n == 0) return 0;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    DevBuf<float> f, o;
+    if (f.ensure(n * MPF_MLP_NIN) || (use_network && o.ensure(n * MPF_MLP_NOUT))) {
+        f.release(); o.release();
+        return fail(c, ERR_NOMEM, "seed buffers");
+    }
+    int rc = mpf_catalogue_features_dev(c, dev_src6, n, f.p);
+    if (!rc && use_network) rc = mpf_mlp_infer_dev(c, f.p, n, o.p);
+    if (!rc) {
+        hipError_t e = mpf::launch_seed(dev_src6, f.p, use_network ? o.p : nullptr, n, dev_init2, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = hipfail(c, e, "seed");
+    }
+    f.release(); o.release();
+    return rc;
+}
+
+int mpf_catalogue_seed(mpf_ctx* c, const double* src6, uint64_t n, int use_network, double* init2) {
+    if (!c) return ERR_INVAL;
+    if (use_network)
+        if (int rc = need_mlp(c)) return rc;
+    if (n && (!src6 || !init2)) return ERR_INVAL;
+    if (n == 0) return 0;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    DevBuf<double> s, i2;
+    if (s.ensure(6 * n) || i2.ensure(2 * n)) {
+        s.release(); i2.release();
+        return fail(c, ERR_NOMEM, "seed buffers");
+    }
+    int rc = 0;
+    if (hipMemcpy(s.p, src6, 6 * n * 8, hipMemcpyHostToDevice) != hipSuccess) rc = fail(c, ERR_HIP, "seed upload");
+    if (!rc) rc = mpf_catalogue_seed_dev(c, s.p, n, use_network, i2.p);
+    if (!rc && hipMemcpy(init2, i2.p, 2 * n * 8, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = fail(c, ERR_HIP, "seed download");
+    s.release(); i2.release();
+    return rc;
+}
+
+int mpf_mfma_probe(mpf_ctx* c, const float* a, const float* b, float* out) {
+    if (!c || !a || !b || !out) return ERR_INVAL;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    DevBuf<float> d;
+    if (d.ensure(3 * 256)) return fail(c, ERR_NOMEM, "probe buffer");
+    int rc = 0;
+    if (hipMemcpy(d.p, a, 1024, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d.p + 256, b, 1024, hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(c, ERR_HIP, "probe upload");
+    if (!rc) {
+        hipError_t e = mpf::launch_mfma_probe(d.p, d.p + 256, d.p + 512, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess) e = hipMemcpy(out, d.p + 512, 1024, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = hipfail(c, e, "probe");
+    }
+    d.release();
     return rc;
 }
 

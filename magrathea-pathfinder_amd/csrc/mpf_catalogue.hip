@@ -79,10 +79,6 @@ __global__ void newton_kernel(NewtonArgs N) {
     double* o = N.out + (uint64_t)MPF_NCAT * i;
     const double D = N.dist[i];
     const double phi = N.ang[2 * i], theta = N.ang[2 * i + 1];
-    if (N.jac) {
-        o[MPF_CAT_A11] = s[SF_A11]; o[MPF_CAT_A12] = s[SF_A12];
-        o[MPF_CAT_A21] = s[SF_A12]; o[MPF_CAT_A22] = s[SF_A22];
-    }
     if (s[SF_CROSSED] == 0.0) {  // the ray never reached the source radius
         o[MPF_CAT_PHI_OBS] = 42; o[MPF_CAT_THETA_OBS] = 42; o[MPF_CAT_SEP] = 0;
         // firstid of catalogues.h:433-439: the last element minus one
@@ -90,6 +86,10 @@ __global__ void newton_kernel(NewtonArgs N) {
         o[MPF_CAT_STATUS] = MPF_CAT_OUTSIDE;
         o[MPF_CAT_NROUNDS] = N.round + 1;
         return;
+    }
+    if (N.jac) {  // the Jacobian of this round's ray at the source distance
+        o[MPF_CAT_A11] = s[SF_A11]; o[MPF_CAT_A12] = s[SF_A12];
+        o[MPF_CAT_A21] = s[SF_A12]; o[MPF_CAT_A22] = s[SF_A22];
     }
     const double fx = s[SF_X], fy = s[SF_Y], fz = s[SF_Z];
     const double sep = sqrt(pow(tt[0] - fx, 2) + pow(tt[1] - fy, 2) + pow(tt[2] - fz, 2)) / D;

@@ -20,6 +20,9 @@ bool split_build(SplitTree& s, const DevTree& t, const uint8_t* level, const uin
                  const uint32_t* kz, int order, hipStream_t st, std::string& err);
 void split_release(SplitTree& s);
 bool split_supports_surfaces();
+// features of catalogue sources (learned first guess) from the stencil blocks
+hipError_t launch_features_split(const DevTree& t, const SplitTree& s, const double* src, uint64_t n, double length,
+                                 float* feat, hipStream_t st);
 hipError_t launch_integrate_split(const IntegrateArgs& A, const DevTree& t, const SplitTree& s, int cus, hipStream_t st);
 
 }  // namespace mpf

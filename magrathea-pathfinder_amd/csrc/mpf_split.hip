@@ -25,6 +25,7 @@
 #include "mpf_split.h"
 
 #include "mpf_exact_dev.h"
+#include "mpf_features.h"
 #include "mpf_surfaces.h"
 
 #include <algorithm>
@@ -576,6 +577,34 @@ integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
     }
 }
 
+// Features of catalogue sources from the FP32 stencil blocks: one block
+// read per cell the line of sight crosses.
+struct SplitSampler {
+    const DevTree& t;
+    const SplitView& sv;
+    Interp<false> I;
+    __device__ SplitSampler(const DevTree& tree, const SplitView& view) : t(tree), sv(view) {
+        I.valid = false;
+        I.slot = NIL;
+    }
+    __device__ void operator()(double x, double y, double z, double* g) {
+        float o[NFIELD];
+        if (!split_eval<NFIELD, false>(t, sv, I, x, y, z, o)) {
+            g[0] = g[1] = g[2] = g[3] = 0.0;
+            return;
+        }
+        g[0] = o[4]; g[1] = o[1]; g[2] = o[2]; g[3] = o[3];
+    }
+};
+
+__global__ void features_split_kernel(DevTree t, SplitView sv, const double* __restrict__ src, uint64_t n,
+                                      double length, float* __restrict__ feat) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    SplitSampler sample(t, sv);
+    source_features(src + 6 * i, length, sample, feat + (uint64_t)MPF_MLP_NIN * i);
+}
+
 }  // namespace
 
 bool split_build(SplitTree& s, const DevTree& t, const uint8_t* level, const uint32_t* kx, const uint32_t* ky,
@@ -719,4 +748,15 @@ hipError_t launch_integrate_split(const IntegrateArgs& A, const DevTree& t, cons
     return hipGetLastError();
 }
 
+}  // namespace mpf
+
+namespace mpf {
+hipError_t launch_features_split(const DevTree& t, const SplitTree& s, const double* src, uint64_t n, double length,
+                                 float* feat, hipStream_t st) {
+    SplitView sv{s.tslot, s.blocks};
+    if (n)
+        hipLaunchKernelGGL(features_split_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, t, sv, src, n,
+                           length, feat);
+    return hipGetLastError();
+}
 }  // namespace mpf

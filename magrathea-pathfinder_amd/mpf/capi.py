@@ -32,7 +32,9 @@ EXPORTS = ("mpf_create", "mpf_destroy", "mpf_last_error", "mpf_set_precision", "
            "mpf_dev_alloc", "mpf_dev_free", "mpf_dev_upload", "mpf_dev_download", "mpf_dev_copy",
            "mpf_integrate_surfaces", "mpf_integrate_surfaces_dev", "mpf_healpix_pix2vec", "mpf_healpix_vec2pix",
            "mpf_hmaps_launch_dev", "mpf_hmaps_fill_dev", "mpf_healpix_bin_dev", "mpf_catalogue_solve",
-           "mpf_catalogue_solve_dev")
+           "mpf_catalogue_solve_dev", "mpf_mlp_init", "mpf_mlp_state", "mpf_mlp_train", "mpf_mlp_gradient",
+           "mpf_mlp_infer", "mpf_mlp_infer_dev", "mpf_catalogue_features", "mpf_catalogue_features_dev",
+           "mpf_catalogue_seed", "mpf_catalogue_seed_dev", "mpf_mfma_probe")
 
 CAT_FIELDS = ("phi", "theta", "phi_obs", "theta_obs", "sep", "iter", "z0", "z1", "z2", "z3", "z4", "z5",
               "a11", "a12", "a21", "a22", "status", "nrounds")
@@ -45,6 +47,13 @@ SURF_FIELDS = ("valid", "firstid", "f", "x", "y", "z", "dist", "chi", "lambda", 
 SURF = {name: i for i, name in enumerate(SURF_FIELDS)}
 NSURF = 32
 MAX_SURFACES = 16
+
+# learned first guess of the catalogue root-finder (sizes of include/mpf.h)
+MLP_NIN, MLP_NOUT, MLP_NHIDDEN, MLP_KIN, MLP_KOUT = 10, 2, 128, 16, 16
+MLP_NPARAM = MLP_KIN * MLP_NHIDDEN + MLP_NHIDDEN + MLP_NHIDDEN * MLP_NHIDDEN + MLP_NHIDDEN + \
+    MLP_NHIDDEN * MLP_KOUT + MLP_KOUT
+MLP_NNORM = 24
+MLP_NSTATE = MLP_NPARAM + MLP_NNORM
 
 
 class MpfError(RuntimeError):
@@ -77,11 +86,13 @@ def make_surfaces(kind, values, jacobian=True, ray_value=None):
 
 class NewtonCfg(ctypes.Structure):
     _fields_ = [("accuracy", ctypes.c_double), ("max_iterations", ctypes.c_uint32), ("vobs", ctypes.c_double * 3),
-                ("h", ctypes.c_double), ("mpc", ctypes.c_double)]
+                ("h", ctypes.c_double), ("mpc", ctypes.c_double), ("jacobian_all_rounds", ctypes.c_uint32)]
 
 
-def make_newton(accuracy=1e-8, max_iterations=10, vobs=(0.0, 0.0, 0.0), h=0.7, mpc=3.08568e22):
+def make_newton(accuracy=1e-8, max_iterations=10, vobs=(0.0, 0.0, 0.0), h=0.7, mpc=3.08568e22,
+                jacobian_all_rounds=False):
     c = NewtonCfg()
+    c.jacobian_all_rounds = int(bool(jacobian_all_rounds))
     c.accuracy = accuracy
     c.max_iterations = max_iterations
     for i in range(3):
@@ -138,6 +149,18 @@ def load_library(path=LIB_PATH):
     lib.mpf_hmaps_launch_dev.argtypes = [vp, i64, vp, u64, vp]
     lib.mpf_hmaps_fill_dev.argtypes = [vp, i64, vp, vp, u64, ctypes.c_uint32, vp]
     lib.mpf_healpix_bin_dev.argtypes = [vp, i64, vp, vp, u64, ctypes.c_uint32, ctypes.c_uint32, vp]
+    f32, u32 = ctypes.c_float, ctypes.c_uint32
+    lib.mpf_mlp_init.argtypes = [vp, u64]
+    lib.mpf_mlp_state.argtypes = [vp, vp, vp]
+    lib.mpf_mlp_train.argtypes = [vp, vp, vp, u64, u32, u32, f32, u64, vp]
+    lib.mpf_mlp_gradient.argtypes = [vp, vp, vp, u64, vp, vp]
+    lib.mpf_mlp_infer.argtypes = [vp, vp, u64, vp]
+    lib.mpf_mlp_infer_dev.argtypes = [vp, vp, u64, vp]
+    lib.mpf_catalogue_features.argtypes = [vp, vp, u64, vp]
+    lib.mpf_catalogue_features_dev.argtypes = [vp, vp, u64, vp]
+    lib.mpf_catalogue_seed.argtypes = [vp, vp, u64, i32, vp]
+    lib.mpf_catalogue_seed_dev.argtypes = [vp, vp, u64, i32, vp]
+    lib.mpf_mfma_probe.argtypes = [vp, vp, vp, vp]
     _lib = lib
     return lib
 
@@ -314,6 +337,78 @@ class Context:
                                                      ctypes.c_uint64(n), ctypes.byref(cfg), ctypes.c_void_p(out_ptr),
                                                      ctypes.byref(rays)), "mpf_catalogue_solve_dev")
         return rays.value
+
+    # learned first guess of the catalogue root-finder
+    def mlp_init(self, seed=0):
+        self._check(self.lib.mpf_mlp_init(self.h, ctypes.c_uint64(seed)), "mpf_mlp_init")
+
+    def mlp_state(self, set_state=None):
+        """Returns the MLP_NSTATE-float state; replaces it first when set_state is given."""
+        get = np.zeros(MLP_NSTATE, dtype=np.float32)
+        st = None if set_state is None else np.ascontiguousarray(set_state, dtype=np.float32).reshape(MLP_NSTATE)
+        self._check(self.lib.mpf_mlp_state(self.h, _p(get), _p(st)), "mpf_mlp_state")
+        return get
+
+    def mlp_train(self, feat, target, epochs=10, batch=1024, lr=1e-3, seed=0):
+        """Adam epochs over (feat, target); returns the mean loss of each epoch."""
+        feat = np.ascontiguousarray(feat, dtype=np.float32).reshape(-1, MLP_NIN)
+        target = np.ascontiguousarray(target, dtype=np.float32).reshape(-1, MLP_NOUT)
+        assert feat.shape[0] == target.shape[0]
+        loss = np.zeros(max(int(epochs), 1), dtype=np.float32)
+        self._check(self.lib.mpf_mlp_train(self.h, _p(feat), _p(target), ctypes.c_uint64(feat.shape[0]),
+                                           int(epochs), int(batch), float(lr), ctypes.c_uint64(seed), _p(loss)),
+                    "mpf_mlp_train")
+        return loss[:int(epochs)]
+
+    def mlp_gradient(self, feat, target):
+        feat = np.ascontiguousarray(feat, dtype=np.float32).reshape(-1, MLP_NIN)
+        target = np.ascontiguousarray(target, dtype=np.float32).reshape(-1, MLP_NOUT)
+        grad = np.zeros(MLP_NPARAM, dtype=np.float32)
+        loss = ctypes.c_float(0.0)
+        self._check(self.lib.mpf_mlp_gradient(self.h, _p(feat), _p(target), ctypes.c_uint64(feat.shape[0]),
+                                              _p(grad), ctypes.byref(loss)), "mpf_mlp_gradient")
+        return grad, loss.value
+
+    def mlp_infer(self, feat):
+        feat = np.ascontiguousarray(feat, dtype=np.float32).reshape(-1, MLP_NIN)
+        out = np.zeros((feat.shape[0], MLP_NOUT), dtype=np.float32)
+        self._check(self.lib.mpf_mlp_infer(self.h, _p(feat), ctypes.c_uint64(feat.shape[0]), _p(out)),
+                    "mpf_mlp_infer")
+        return out
+
+    def mlp_infer_dev(self, feat_ptr, n, out_ptr):
+        self._check(self.lib.mpf_mlp_infer_dev(self.h, ctypes.c_void_p(feat_ptr), ctypes.c_uint64(n),
+                                               ctypes.c_void_p(out_ptr)), "mpf_mlp_infer_dev")
+
+    def catalogue_features(self, src6):
+        src6 = np.ascontiguousarray(src6, dtype=np.float64).reshape(-1, 6)
+        feat = np.zeros((src6.shape[0], MLP_NIN), dtype=np.float32)
+        self._check(self.lib.mpf_catalogue_features(self.h, _p(src6), ctypes.c_uint64(src6.shape[0]), _p(feat)),
+                    "mpf_catalogue_features")
+        return feat
+
+    def catalogue_features_dev(self, src_ptr, n, feat_ptr):
+        self._check(self.lib.mpf_catalogue_features_dev(self.h, ctypes.c_void_p(src_ptr), ctypes.c_uint64(n),
+                                                        ctypes.c_void_p(feat_ptr)), "mpf_catalogue_features_dev")
+
+    def catalogue_seed(self, src6, use_network=True):
+        src6 = np.ascontiguousarray(src6, dtype=np.float64).reshape(-1, 6)
+        init2 = np.zeros((src6.shape[0], 2))
+        self._check(self.lib.mpf_catalogue_seed(self.h, _p(src6), ctypes.c_uint64(src6.shape[0]),
+                                                int(bool(use_network)), _p(init2)), "mpf_catalogue_seed")
+        return init2
+
+    def catalogue_seed_dev(self, src_ptr, n, init_ptr, use_network=True):
+        self._check(self.lib.mpf_catalogue_seed_dev(self.h, ctypes.c_void_p(src_ptr), ctypes.c_uint64(n),
+                                                    int(bool(use_network)), ctypes.c_void_p(init_ptr)),
+                    "mpf_catalogue_seed_dev")
+
+    def mfma_probe(self, a, b):
+        a = np.ascontiguousarray(a, dtype=np.float32).reshape(16, 16)
+        b = np.ascontiguousarray(b, dtype=np.float32).reshape(16, 16)
+        c = np.zeros((16, 16), dtype=np.float32)
+        self._check(self.lib.mpf_mfma_probe(self.h, _p(a), _p(b), _p(c)), "mpf_mfma_probe")
+        return c
 
     def hmaps_launch_dev(self, nside, pix_ptr, n, in9_ptr):
         self._check(self.lib.mpf_hmaps_launch_dev(self.h, int(nside), ctypes.c_void_p(pix_ptr), ctypes.c_uint64(n),

@@ -1,0 +1,110 @@
+"""Learned first guess of the catalogue root-finder (host side).
+
+The root-finder (Catalogues::iterateNewtonMethod, catalogues.h:757-800)
+starts every source from its unlensed direction (catalogues.h:832-836).
+Here a small network proposes the observed direction instead; it is trained
+on pairs that the root-finder itself produced on other cones of the same
+octree: the features of a source (a Born estimate of its deflection from the
+octree along the line of sight, and what else the network may use) and the
+deflection the root-finder found, less the Born estimate.  The first guess
+is the unlensed direction + the Born estimate + the network's correction,
+and the Newton iterations still take every source to the requested
+accuracy.  All compute goes through libmpf
+(features, training, inference, seeding and the root-finder on the device).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import capi
+
+ARCSEC = 206264.80624709636
+
+
+def cone_sources(n, axis_phi_deg, axis_theta_deg, aperture_deg, dmin, dmax, seed, vscale=1e-3):
+    """n sources uniform in comoving volume inside a circular cone of
+    half-angle aperture_deg around (axis_phi, axis_theta), at comoving
+    distances [dmin, dmax] (box units).  Returns n x 6 (x, y, z, vx, vy, vz)."""
+    rng = np.random.default_rng(seed)
+    r = np.cbrt(dmin ** 3 + rng.uniform(0, 1, n) * (dmax ** 3 - dmin ** 3))
+    ca = rng.uniform(math.cos(math.radians(aperture_deg)), 1.0, n)
+    sa = np.sqrt(1 - ca * ca)
+    az = rng.uniform(0, 2 * math.pi, n)
+    local = np.stack([sa * np.cos(az), sa * np.sin(az), ca], 1)   # around +z
+    ph, th = math.radians(axis_phi_deg), math.radians(axis_theta_deg)
+    # rotation taking +z to the axis: R = Rz(phi) Ry(theta)
+    ry = np.array([[math.cos(th), 0, math.sin(th)], [0, 1, 0], [-math.sin(th), 0, math.cos(th)]])
+    rz = np.array([[math.cos(ph), -math.sin(ph), 0], [math.sin(ph), math.cos(ph), 0], [0, 0, 1]])
+    d = local @ (rz @ ry).T
+    src = np.zeros((n, 6))
+    src[:, :3] = d * r[:, None]
+    src[:, 3:] = rng.normal(scale=vscale, size=(n, 3))
+    return src
+
+
+def deflections(out):
+    """observed - unlensed direction (arcsec along e_phi and e_theta) of
+    root-finder records (MPF_NCAT)."""
+    dphi = out[:, capi.CAT["phi_obs"]] - out[:, capi.CAT["phi"]]
+    dphi = (dphi + math.pi) % (2 * math.pi) - math.pi
+    dth = out[:, capi.CAT["theta_obs"]] - out[:, capi.CAT["theta"]]
+    return np.stack([dphi * np.sin(out[:, capi.CAT["theta"]]), dth], 1) * ARCSEC
+
+
+def targets(out, feat):
+    """What the network learns: the deflection left after the Born estimate
+    carried by the features (columns 1, 2), arcsec."""
+    return (deflections(out) - feat[:, 1:3]).astype(np.float32)
+
+
+def born_guess(src, feat):
+    """First guesses from the Born estimate alone (no network)."""
+    src = np.asarray(src, dtype=np.float64).reshape(-1, 6)
+    feat = np.asarray(feat, dtype=np.float64)
+    D = np.linalg.norm(src[:, :3], axis=1)
+    theta = np.arccos(src[:, 2] / D)
+    return np.stack([np.arctan2(src[:, 1], src[:, 0]) + feat[:, 1] / (ARCSEC * np.sin(theta)),
+                     theta + feat[:, 2] / ARCSEC], 1)
+
+
+def training_pairs(ctx, src, **newton):
+    """(features, targets, records, rays) from the root-finder on src: only
+    converged sources give pairs."""
+    out, rays = ctx.catalogue_solve(src, **newton)
+    ok = out[:, capi.CAT["status"]] == capi.CAT_OK
+    feat = ctx.catalogue_features(src[ok])
+    return feat, targets(out[ok], feat), out, rays
+
+
+def train(ctx, feat, target, val_feat, val_target, epochs=100, batch=256, lr=1e-3, seed=0, check_every=5):
+    """Fresh network trained on (feat, target), early-stopped on a held-out
+    set (another cone): the state with the lowest held-out residual is kept.
+    The network is worth using only if that residual is below the one of the
+    Born estimate alone (the held-out targets' rms).  Returns a dict: loss per
+    epoch, held-out rms without and with the network, the epoch kept and
+    use_network."""
+    ctx.mlp_init(seed)
+    val_target = np.asarray(val_target, dtype=np.float64)
+    rms0 = float(np.sqrt(np.mean(val_target ** 2)))
+    losses = []
+    done = 0
+    best = (math.inf, 0, None)
+    while done < epochs:
+        k = min(check_every, epochs - done)
+        losses += list(ctx.mlp_train(feat, target, epochs=k, batch=batch, lr=lr, seed=seed + done))
+        done += k
+        r = float(np.sqrt(np.mean((ctx.mlp_infer(val_feat) - val_target) ** 2)))
+        if r < best[0]:
+            best = (r, done, ctx.mlp_state())
+    if best[2] is not None:
+        ctx.mlp_state(best[2])
+    return dict(loss=np.array(losses), val_rms_born=rms0, val_rms_network=best[0], epoch=best[1],
+                use_network=best[0] < rms0)
+
+
+def solve_seeded(ctx, src, use_network=True, **newton):
+    """Root-finder started from the first guesses: (records, rays)."""
+    init = ctx.catalogue_seed(src, use_network)
+    return ctx.catalogue_solve(src, init, **newton)
