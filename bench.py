@@ -89,6 +89,116 @@ def cpu_baseline(rec, table, length, in9, w, nthreads, target_s=15.0):
                       f"{dt:.1f}s, {nthreads} threads"}
 
 
+def jacobian_maps(ctx, w, length, table, n=4194304, nside=2048, zmin=0.1, zmax=2.0, seed=7):
+    """Uneven batch with the lensing Jacobian and HEALPix maps: n rays uniform
+    over the cone, each stopped at its own source redshift z_s ~ U[zmin, zmax]
+    with the infinitesimal Jacobian there, then kappa, gamma1, gamma2 binned
+    into RING maps of the given nside (split tier)."""
+    from mpf import capi, synth
+    ph, th = synth.cone_uniform(w["phi0"], w["theta0"], w["aperture"], n, seed)
+    in9 = synth.launch_photons(ph, th)
+    zs = np.random.default_rng(seed + 1).uniform(zmin, zmax, n)
+    npix = 12 * nside * nside
+    d_in, d_rv = ctx.dev_alloc(in9.nbytes), ctx.dev_alloc(zs.nbytes)
+    d_out = ctx.dev_alloc(n * capi.NOUT * 8)
+    d_surf = ctx.dev_alloc(n * capi.NSURF * 8)
+    d_maps = ctx.dev_alloc(4 * npix * 8)
+    ctx.upload(d_in, in9)
+    ctx.upload(d_rv, zs)
+    res = []
+    for rep in range(2):   # the second pass is reported (the first also allocates the map accumulator)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.integrate_surfaces_dev(d_in, n, d_out, d_surf, "redshift", [0.0], jacobian=True, sync=True,
+                                   ray_value_ptr=d_rv)
+        steps, ms = ctx.last_run()
+        t1 = time.perf_counter()
+        ctx.healpix_bin_dev(nside, d_in, d_surf, n, 1, 0, d_maps)
+        ctx.synchronize()
+        t2 = time.perf_counter()
+        res.append((steps, ms, (t1 - t0) * 1e3, (t2 - t1) * 1e3))
+    steps, ms, wall_int, wall_bin = res[-1]
+    surf = ctx.download(d_surf, (n, capi.NSURF))
+    maps = ctx.download(d_maps, (4, npix))
+    for p in (d_in, d_rv, d_out, d_surf, d_maps):
+        ctx.dev_free(p)
+    valid = surf[:, capi.SURF["valid"]] != 0
+    b_step = B_STEP_SPLIT_TSC + 6 * 27 * 3 * 4   # + 6 offset samples of grad phi (FP32)
+    return {"workload": f"{n} rays uniform over {w['aperture']}x{w['aperture']} deg, per-ray z_s ~ U[{zmin}, {zmax}], "
+                        f"infinitesimal Jacobian at z_s, kappa/gamma1/gamma2 binned into nside {nside} RING maps",
+            "value": steps / (wall_int / 1e3 + wall_bin / 1e3), "unit": "ray-RK4-steps/s (integration + maps)",
+            "kernel_steps_per_s": steps / (ms / 1e3), "rk4_steps": int(steps), "integrate_ms": ms,
+            "integrate_wall_ms": wall_int, "bin_wall_ms": wall_bin, "rays_valid": int(valid.sum()),
+            "pixels_hit": int(np.count_nonzero(maps[3])), "kappa_rms": float(np.sqrt(np.mean(
+                maps[0][maps[3] > 0] ** 2))),
+            "roofline": {"bound": "hbm", "bytes_per_step": b_step,
+                         "achieved": steps / (ms / 1e3) * b_step / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                         "frac": steps / (ms / 1e3) * b_step / HBM_PEAK}}
+
+
+def catalogue(ctx, w, length, table, n=1048576, zmin=0.1, zmax=1.5, acc=1e-8, ntrain=16384, seed=11):
+    """Catalogue root-finding: n sources uniform in comoving volume in the
+    cone (circular, the area of the square aperture) at z in [zmin, zmax],
+    Newton to acc (rad) with the infinitesimal Jacobian, split tier: plain
+    (unlensed first guess) against the first guess from the Born features
+    (and the network when it beats them on a held-out cone; trained on pairs
+    from 4 other cones of the same octree)."""
+    import math
+    from mpf import capi, proposal, synth
+    box = w["box_mpch"]
+    dmin = synth.comoving_distance_of_z(zmin) * 0.7 / synth.MPC_SI / box
+    dmax = synth.comoving_distance_of_z(zmax) * 0.7 / synth.MPC_SI / box
+    half = math.degrees(math.acos(1 - (math.radians(w["aperture"]) ** 2) / (2 * math.pi)))
+    src = proposal.cone_sources(n, w["phi0"], w["theta0"], half, dmin, dmax, seed)
+    # training pairs on 4 other cones, held-out cone for the network's gate
+    t0 = time.perf_counter()
+    pairs = []
+    for k, (p, t) in enumerate(((80.0, 50.0), (150.0, 70.0), (230.0, 40.0), (300.0, 110.0))):
+        s = proposal.cone_sources(ntrain, p, t, half, dmin, dmax, seed + 100 + k)
+        fe, ta, _, _ = proposal.training_pairs(ctx, s, accuracy=acc)
+        pairs.append((fe, ta))
+    t_pairs = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    fit = proposal.train(ctx, np.concatenate([p[0] for p in pairs[:3]]), np.concatenate([p[1] for p in pairs[:3]]),
+                         pairs[3][0], pairs[3][1], epochs=20, batch=512, lr=1e-3, seed=seed, check_every=5)
+    t_train = time.perf_counter() - t0
+    d_src, d_init = ctx.dev_alloc(src.nbytes), ctx.dev_alloc(n * 16)
+    d_out = ctx.dev_alloc(n * capi.NCAT * 8)
+    ctx.upload(d_src, src)
+    st, it = capi.CAT["status"], capi.CAT["iter"]
+    runs = {}
+    for name in ("plain", "seeded"):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        if name == "plain":
+            rays = ctx.catalogue_solve_dev(d_src, 0, n, d_out, accuracy=acc)
+        else:
+            ctx.catalogue_seed_dev(d_src, n, d_init, use_network=fit["use_network"])
+            rays = ctx.catalogue_solve_dev(d_src, d_init, n, d_out, accuracy=acc, jacobian_all_rounds=True)
+        ctx.synchronize()
+        wall = time.perf_counter() - t0
+        out = ctx.download(d_out, (n, capi.NCAT))
+        ok = out[:, st] == capi.CAT_OK
+        runs[name] = {"sources_per_s": float(ok.sum() / wall), "wall_s": wall, "converged": int(ok.sum()),
+                      "rays": int(rays), "mean_iterations": float(out[ok, it].mean()),
+                      "rejected": int(np.sum(out[:, st] == capi.CAT_REJECTED))}
+        runs[name + "_angles"] = out[:, [capi.CAT["phi_obs"], capi.CAT["theta_obs"]]].copy()
+    ok_both = np.all(np.isfinite(runs["plain_angles"]), 1)
+    dmax_ang = float(np.max(np.abs(runs["plain_angles"][ok_both] - runs["seeded_angles"][ok_both])))
+    for p in (d_src, d_init, d_out):
+        ctx.dev_free(p)
+    return {"workload": f"{n} sources uniform in comoving volume, z in [{zmin}, {zmax}], cone of {half:.2f} deg "
+                        f"half-angle, Newton to {acc} rad, infinitesimal Jacobian, split tier",
+            "metric": "catalogue sources/s", "value": runs["seeded"]["sources_per_s"], "unit": "sources/s",
+            "plain": runs["plain"], "seeded": runs["seeded"],
+            "speedup_seeded_over_plain": runs["seeded"]["sources_per_s"] / runs["plain"]["sources_per_s"],
+            "max_angle_difference_rad": dmax_ang,
+            "first_guess": {"training_pairs": int(sum(len(p[0]) for p in pairs[:3])),
+                            "held_out_rms_arcsec_born": fit["val_rms_born"],
+                            "held_out_rms_arcsec_network": fit["val_rms_network"],
+                            "use_network": bool(fit["use_network"]), "pairs_s": t_pairs, "train_s": t_train}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,6 +207,8 @@ def main():
     ap.add_argument("--workload", default="cfg1")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-tier comparison")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the Jacobian/maps and catalogue workloads on the same octree")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="only run the plain copy kernel over MB megabytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -191,6 +303,13 @@ def main():
                                "step_count_mismatch_rays": int(np.sum(out64[:, capi.OUT["nsteps"]] !=
                                                                       out_split[:, capi.OUT["nsteps"]]))}
         ctx.set_precision(capi.PREC_SPLIT)
+    if not args.no_extra and order == 2:
+        t0 = time.time()
+        result["jacobian_maps"] = jacobian_maps(ctx, w, length, table)
+        log(f"jacobian/maps: {result['jacobian_maps']} ({time.time() - t0:.1f}s)")
+        t0 = time.time()
+        result["catalogue"] = catalogue(ctx, w, length, table)
+        log(f"catalogue: {result['catalogue']} ({time.time() - t0:.1f}s)")
     if not args.no_cpu:
         nth = min(16, os.cpu_count() or 1)
         t0 = time.time()
