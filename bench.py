@@ -26,6 +26,8 @@ sys.path.insert(0, os.path.join(ROOT, "magrathea-pathfinder_amd"))
 HBM_PEAK = 8.0e12       # MI355X HBM3E, bytes/s (AMD public spec)
 B_STEP_SPLIT_TSC = 540  # 27 cells x 5 fields x 4 bytes per RK4 step
 B_STEP_FP64_TSC = 1080  # 27 cells x 5 fields x 8 bytes
+BF16_PEAK = 2.5e15      # MI355X dense BF16 matrix-core peak, flop/s (AMD public spec)
+MLP_FLOP_PER_SAMPLE = 2 * (10 * 128 + 128 * 128 + 128 * 2)   # 35,840: the 10-128-128-2 network's forward
 
 
 def profiled_traffic(name="integrate_split_tsc_cfg1.txt"):
@@ -185,8 +187,24 @@ def catalogue(ctx, w, length, table, n=1048576, zmin=0.1, zmax=1.5, acc=1e-8, nt
         runs[name + "_angles"] = out[:, [capi.CAT["phi_obs"], capi.CAT["theta_obs"]]].copy()
     ok_both = np.all(np.isfinite(runs["plain_angles"]), 1)
     dmax_ang = float(np.max(np.abs(runs["plain_angles"][ok_both] - runs["seeded_angles"][ok_both])))
-    for p in (d_src, d_init, d_out):
+    # the network's throughput on every source (features from the split tier's stencil blocks)
+    d_feat, d_pred = ctx.dev_alloc(n * capi.MLP_NIN * 4), ctx.dev_alloc(n * capi.MLP_NOUT * 4)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.catalogue_features_dev(d_src, n, d_feat)
+    t_feat = time.perf_counter() - t0
+    t_inf = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        ctx.mlp_infer_dev(d_feat, n, d_pred)
+        t_inf.append(time.perf_counter() - t0)
+    flop = n * MLP_FLOP_PER_SAMPLE
+    for p in (d_src, d_init, d_out, d_feat, d_pred):
         ctx.dev_free(p)
+    network = {"features_s": t_feat, "inference_s": min(t_inf), "inference_sources_per_s": n / min(t_inf),
+               "flop_per_sample": MLP_FLOP_PER_SAMPLE, "achieved_TFLOP_s": flop / min(t_inf) / 1e12,
+               "peak_TFLOP_s": BF16_PEAK / 1e12, "frac": flop / min(t_inf) / BF16_PEAK,
+               "note": "wall time of one call (normalise, MFMA forward, denormalise, one scratch allocation)"}
     return {"workload": f"{n} sources uniform in comoving volume, z in [{zmin}, {zmax}], cone of {half:.2f} deg "
                         f"half-angle, Newton to {acc} rad, infinitesimal Jacobian, split tier",
             "metric": "catalogue sources/s", "value": runs["seeded"]["sources_per_s"], "unit": "sources/s",
@@ -196,7 +214,8 @@ def catalogue(ctx, w, length, table, n=1048576, zmin=0.1, zmax=1.5, acc=1e-8, nt
             "first_guess": {"training_pairs": int(sum(len(p[0]) for p in pairs[:3])),
                             "held_out_rms_arcsec_born": fit["val_rms_born"],
                             "held_out_rms_arcsec_network": fit["val_rms_network"],
-                            "use_network": bool(fit["use_network"]), "pairs_s": t_pairs, "train_s": t_train}}
+                            "use_network": bool(fit["use_network"]), "pairs_s": t_pairs, "train_s": t_train},
+            "network": network}
 
 
 def main():
@@ -209,6 +228,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-tier comparison")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the Jacobian/maps and catalogue workloads on the same octree")
+    ap.add_argument("--only", choices=("jacobian_maps", "catalogue"), default=None,
+                    help="run one of the extra workloads alone and print its record (profiling runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="only run the plain copy kernel over MB megabytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -241,6 +262,11 @@ def main():
     ctx.set_cosmology(table)
     ncells, lmin, lmax = ctx.octree_info()
     log(f"octree loaded in {time.time() - t0:.1f}s: {ncells} cells, levels {lmin}..{lmax}")
+    if args.only:
+        fn = jacobian_maps if args.only == "jacobian_maps" else catalogue
+        print(json.dumps({args.only: fn(ctx, w, length, table)}), flush=True)
+        ctx.close()
+        return
     # inputs and outputs resident in HBM (libmpf device buffers)
     n = in9.shape[0]
     d_in = ctx.dev_alloc(in9.nbytes)

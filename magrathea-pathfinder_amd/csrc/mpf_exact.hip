@@ -264,8 +264,15 @@ __device__ bool ray_step(const IntegrateArgs& A, const DevTree& t, TscCache& cac
         if (A.nsurf) {
             double T[3] = {0.0, 0.0, 0.0};
             if (A.jac) element_tidal<P>(t, s.back.v, T);
-            surf_append(s.surf, A.surf_kind, A.nsurf, A.surf_val, A.ray_value != nullptr, s.stopv, A.surf_out + (uint64_t)A.nsurf * SF_NREC * ray,
-                        s.appended, surf_elem(s.prev.v), surf_elem(s.back.v), T, A.jac != 0);
+            const double* pr = s.prev.v;
+            const double* bk2 = s.back.v;
+            const double vp = surf_param(A.surf_kind, pr[MPF_OUT_CHI], pr[MPF_OUT_LAMBDA], pr[MPF_OUT_REDSHIFT], pr[CT],
+                                         pr[CA], pr[MPF_OUT_S]);
+            const double vn = surf_param(A.surf_kind, bk2[MPF_OUT_CHI], bk2[MPF_OUT_LAMBDA], bk2[MPF_OUT_REDSHIFT],
+                                         bk2[CT], bk2[CA], bk2[MPF_OUT_S]);
+            surf_append(s.surf, A.surf_kind, A.nsurf, A.surf_val, A.ray_value != nullptr, s.stopv,
+                        A.surf_out + (uint64_t)A.nsurf * SF_NREC * ray, s.appended, surf_elem(s.prev.v),
+                        surf_elem(s.back.v), vp, vn, T, A.jac != 0);
         }
     }
     if (A.stop_kind >= 0) {

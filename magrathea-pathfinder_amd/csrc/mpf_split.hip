@@ -170,6 +170,7 @@ struct Interp {
     unsigned lvl;
     uint32_t kx, ky, kz;
     bool valid;          // the last point had data
+    bool reuse;          // ... and its cell stores no children: every point inside has the same finest cell
     uint32_t slot;
     double cx, cy, cz, inv;
     float c[RC ? NFIELD * NCOEF : 1];
@@ -204,7 +205,7 @@ template <int NF, bool RC>
 __device__ __forceinline__ bool split_eval(const DevTree& t, const SplitView& sv, Interp<RC>& I, double x, double y,
                                           double z, float* out) {
     bool same = false;
-    if (I.valid) {
+    if (I.valid && I.reuse) {
         same = axis_coord(x, I.lvl) == I.kx && axis_coord(y, I.lvl) == I.ky && axis_coord(z, I.lvl) == I.kz;
     }
     if (!same) {
@@ -221,6 +222,9 @@ __device__ __forceinline__ bool split_eval(const DevTree& t, const SplitView& sv
         I.ky = ky32 >> (32 - fl);
         I.kz = kz32 >> (32 - fl);
         I.valid = true;
+        // a cell with some children stored (a partial oct, e.g. at a cone's
+        // edge) is the finest cell only of the points outside those children
+        I.reuse = t.child_mask[g] == 0;
         if (slot != I.slot) load_block<RC>(sv, slot, I);
     }
     const float u = (float)((x - I.cx) * I.inv);
@@ -241,6 +245,50 @@ __device__ __forceinline__ bool split_eval(const DevTree& t, const SplitView& sv
             }
             out[f] = horner27(c, u, v, w);
         }
+    }
+    return true;
+}
+
+// Jacobian samples: grad phi components 1..NG (dphidx, dphidy, dphidz) at
+// (x, y, z), with the sample's finest cell cached in LDS (one entry per lane
+// and sample direction: cell coordinates + block slot, level | reuse flag).
+// The samples sit one cell away from the ray along each axis and stay in
+// the same neighbour cell for about as many steps as the ray stays in its
+// own; a hit costs three coordinate compares instead of a tree descent.
+template <int NG>
+__device__ __forceinline__ bool jac_eval(const DevTree& t, const SplitView& sv, uint4& cell, uint32_t& lw, double x,
+                                         double y, double z, float* out) {
+    const unsigned l0 = lw & 0x3Fu;
+    uint4 c = cell;
+    bool hit = (lw & 0x100u) && axis_coord(x, l0) == c.x && axis_coord(y, l0) == c.y && axis_coord(z, l0) == c.z;
+    if (!hit) {
+        unsigned fl = 0;
+        const uint32_t kx32 = axis_coord(x, 32), ky32 = axis_coord(y, 32), kz32 = axis_coord(z, 32);
+        const uint32_t g = find_cell(t, 32, kx32, ky32, kz32, &fl);
+        const uint32_t slot = (g == NIL) ? NIL : sv.tslot[g];
+        if (slot == NIL) {
+            lw = 0u;
+            return false;
+        }
+        c = make_uint4(kx32 >> (32 - fl), ky32 >> (32 - fl), kz32 >> (32 - fl), slot);
+        cell = c;
+        lw = fl | (t.child_mask[g] == 0 ? 0x100u : 0u);
+    }
+    const unsigned char* blk = sv.blocks + (uint64_t)c.w * BLOCK_BYTES;
+    const double4 h = *reinterpret_cast<const double4*>(blk);
+    const float u = (float)((x - h.x) * h.w);
+    const float v = (float)((y - h.y) * h.w);
+    const float w = (float)((z - h.z) * h.w);
+    const float4* cf = reinterpret_cast<const float4*>(blk + BLOCK_DOUBLES * 8);
+#pragma unroll
+    for (int f = 1; f <= NG; ++f) {
+        float cc[FSTRIDE];
+#pragma unroll
+        for (int i = 0; i < FSTRIDE / 4; ++i) {
+            const float4 q = cf[f * (FSTRIDE / 4) + i];
+            cc[4 * i] = q.x; cc[4 * i + 1] = q.y; cc[4 * i + 2] = q.z; cc[4 * i + 3] = q.w;
+        }
+        out[f - 1] = horner27(cc, u, v, w);
     }
     return true;
 }
@@ -354,9 +402,21 @@ integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
     I.slot = NIL;
     SplitRay R;
     SurfAcc SA;
-    Interp<RC> J;   // second cell cache for the Jacobian samples
-    J.valid = false;
-    J.slot = NIL;
+    // surface values in LDS: a runtime index into the by-value argument
+    // would make the compiler copy it to scratch
+    __shared__ double surf_vals[MAX_SURF];
+    // cells of the 6 Jacobian samples of each lane (jac_eval)
+    __shared__ uint4 jcell[SURF ? 6 : 1][SURF ? 256 : 1];
+    __shared__ uint32_t jlw[SURF ? 6 : 1][SURF ? 256 : 1];
+    if (SURF) {
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int k = 0; k < MAX_SURF; ++k) surf_vals[k] = A.surf_val[k];
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) jlw[k][threadIdx.x] = 0u;
+        __syncthreads();
+    }
     bool active = false, drained = false;
     uint64_t ray = 0;
     const unsigned lane = __lane_id();
@@ -481,25 +541,38 @@ integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
                 nw.kt = p[CKT]; nw.kx = p[CKX]; nw.ky = p[CKY]; nw.kz = p[CKZ];
                 double T[3] = {0.0, 0.0, 0.0};
                 if (A.jac) {
-                    // 6 samples of grad phi at +-dl/2 (lensing.h:429-436), FP32 interpolation
+                    // 6 samples of grad phi at +-dl/2 (lensing.h:429-436), FP32
+                    // interpolation; per axis pair only the components the tidal
+                    // tensor uses, differenced at once (+ sample minus - sample)
                     const double jdl = jacobian_step(I.lvl);
-                    double g[6][3];
-#pragma unroll
-                    for (int k = 0; k < 6; ++k) {
-                        const double off = (k & 1) ? 0.5 * jdl : -0.5 * jdl;
-                        const double sx = p[CX] + ((k >> 1) == 0 ? off : 0.0);
-                        const double sy = p[CY] + ((k >> 1) == 1 ? off : 0.0);
-                        const double sz = p[CZ] + ((k >> 1) == 2 ? off : 0.0);
-                        float G[5];
-                        const bool okj = split_eval<4, RC>(t, sv, J, sx, sy, sz, G);
-                        g[k][0] = okj ? (double)G[1] : 0.0;
-                        g[k][1] = okj ? (double)G[2] : 0.0;
-                        g[k][2] = okj ? (double)G[3] : 0.0;
-                    }
-                    tidal_from_samples(p[CX], p[CY], p[CZ], chi, jdl, g, T);
+                    const double hj = 0.5 * jdl;
+                    const unsigned tl = threadIdx.x;
+                    double D[6];
+                    float gm[3], gp[3];
+                    // x pair: d/dx of dphidx
+                    if (!jac_eval<1>(t, sv, jcell[0][tl], jlw[0][tl], p[CX] - hj, p[CY], p[CZ], gm)) gm[0] = 0.f;
+                    if (!jac_eval<1>(t, sv, jcell[1][tl], jlw[1][tl], p[CX] + hj, p[CY], p[CZ], gp)) gp[0] = 0.f;
+                    D[0] = (double)gp[0] - (double)gm[0];
+                    // y pair: d/dy of dphidx, dphidy
+                    if (!jac_eval<2>(t, sv, jcell[2][tl], jlw[2][tl], p[CX], p[CY] - hj, p[CZ], gm)) gm[0] = gm[1] = 0.f;
+                    if (!jac_eval<2>(t, sv, jcell[3][tl], jlw[3][tl], p[CX], p[CY] + hj, p[CZ], gp)) gp[0] = gp[1] = 0.f;
+                    D[1] = (double)gp[0] - (double)gm[0];
+                    D[3] = (double)gp[1] - (double)gm[1];
+                    // z pair: d/dz of dphidx, dphidy, dphidz
+                    if (!jac_eval<3>(t, sv, jcell[4][tl], jlw[4][tl], p[CX], p[CY], p[CZ] - hj, gm))
+                        gm[0] = gm[1] = gm[2] = 0.f;
+                    if (!jac_eval<3>(t, sv, jcell[5][tl], jlw[5][tl], p[CX], p[CY], p[CZ] + hj, gp))
+                        gp[0] = gp[1] = gp[2] = 0.f;
+                    D[2] = (double)gp[0] - (double)gm[0];
+                    D[4] = (double)gp[1] - (double)gm[1];
+                    D[5] = (double)gp[2] - (double)gm[2];
+                    tidal_from_differences(p[CX], p[CY], p[CZ], chi, jdl, D, T);
                 }
-                surf_append(SA, A.surf_kind, A.nsurf, A.surf_val, A.ray_value != nullptr, R.stopv, A.surf_out + (uint64_t)A.nsurf * SF_NREC * ray,
-                            R.appended + 1, pv, nw, T, A.jac != 0);
+                const double vp = surf_param(A.surf_kind, R.chi, R.lambda, R.redshift, R.core[CT], R.core[CA], R.s);
+                const double vn = surf_param(A.surf_kind, chi, lambda, redshift, p[CT], p[CA], s);
+                surf_append(SA, A.surf_kind, A.nsurf, surf_vals, A.ray_value != nullptr, R.stopv,
+                            A.surf_out + (uint64_t)A.nsurf * SF_NREC * ray, R.appended + 1, pv, nw, vp, vn, T,
+                            A.jac != 0);
             }
 #pragma unroll
             for (int i = 0; i < 9; ++i) R.core[i] = p[i];
@@ -717,6 +790,13 @@ static int split_variant() {
     return (v >= 1 && v <= 4) ? v : 2;
 }
 
+// waves per SIMD of the surfaces/Jacobian kernel: 2 (spills) or 1 (none)
+static int split_surf_waves() {
+    const char* e = getenv("MPF_SPLIT_SURF_WAVES");
+    const int v = e ? atoi(e) : 0;
+    return v == 1 ? 1 : 2;
+}
+
 static int split_waves_per_cu() {
     const char* e = getenv("MPF_SPLIT_BLOCKS_PER_CU");
     const int v = e ? atoi(e) : 0;
@@ -733,7 +813,10 @@ hipError_t launch_integrate_split(const IntegrateArgs& A, const DevTree& t, cons
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (A.nsurf) {
-        hipLaunchKernelGGL((integrate_split_kernel<false, 2, true>), dim3(grid), dim3(256), 0, st, A, t, sv);
+        if (split_surf_waves() == 1)
+            hipLaunchKernelGGL((integrate_split_kernel<false, 1, true>), dim3(grid), dim3(256), 0, st, A, t, sv);
+        else
+            hipLaunchKernelGGL((integrate_split_kernel<false, 2, true>), dim3(grid), dim3(256), 0, st, A, t, sv);
     } else {
         switch (split_variant()) {
             case 1: hipLaunchKernelGGL((integrate_split_kernel<true, 1, false>), dim3(grid), dim3(256), 0, st, A, t, sv); break;

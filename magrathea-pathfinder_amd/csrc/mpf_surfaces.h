@@ -53,26 +53,29 @@ __device__ __forceinline__ void surf_init(SurfAcc& s, double rho0) {
     s.crossed = 0;
 }
 
-__device__ __forceinline__ double surf_value(int kind, const SurfElem& e) {
-    switch (kind) {
-        case 0: return e.chi;
-        case 1: return e.lambda;
-        case 2: return e.redshift;
-        case 3: return e.t;
-        case 4: return e.a;
-        default: return e.s;
-    }
+// The surface parameter of an element, chosen among scalars: chosen among
+// the fields of an element struct with a runtime kind, the compiler reads it
+// at a variable offset and keeps the whole struct in scratch memory.
+__device__ __forceinline__ double surf_param(int kind, double chi, double lambda, double redshift, double t, double a,
+                                             double s) {
+    double v = s;
+    v = (kind == 4) ? a : v;
+    v = (kind == 3) ? t : v;
+    v = (kind == 2) ? redshift : v;
+    v = (kind == 1) ? lambda : v;
+    v = (kind == 0) ? chi : v;
+    return v;
 }
 
 // Called after element i (>= 1) is appended, with the previous element.
 // `tidal` (d1d1, d1d2, d2d2) of element i is added after the crossings are
 // recorded, so a snapshot holds the sums up to element i-1 = firstid.
-// vals: the surface values, or, when per_ray, the single value rv
+// vals: the surface values, or, when per_ray, the single value rv; vp, vn:
+// the surface parameter (surf_param) of the previous and the new element.
 __device__ __forceinline__ void surf_append(SurfAcc& s, int kind, int nsurf, const double* vals, bool per_ray,
                                             double rv, double* rec, uint32_t i, const SurfElem& pv,
-                                            const SurfElem& nw, const double* tidal, bool jac) {
-    const double vn = surf_value(kind, nw);
-    const double vp = surf_value(kind, pv);
+                                            const SurfElem& nw, double vp, double vn, const double* tidal,
+                                            bool jac) {
     for (int j = 0; j < nsurf; ++j) {
         if (s.crossed & (1u << j)) continue;
         const double ref = per_ray ? rv : vals[j];
@@ -123,25 +126,35 @@ __device__ __forceinline__ void surf_append(SurfAcc& s, int kind, int nsurf, con
     s.rmax_cur = (nw.rho > s.rmax_cur) ? nw.rho : s.rmax_cur;
 }
 
-// Angular tidal components from the 6 samples of grad phi (lensing.h:421-448).
-// g[k][d]: sample k in (x-, x+, y-, y+, z-, z+), component d of grad phi.
+// Angular tidal components (lensing.h:421-448) from the differences of the
+// 6 samples of grad phi at +-dl/2 along each axis: D = (d_x gx, d_y gx,
+// d_z gx, d_y gy, d_z gy, d_z gz), each the + sample minus the - sample.
 // The angles' sines and cosines come from the position itself.
-__device__ __forceinline__ void tidal_from_samples(double x, double y, double z, double r, double dl,
-                                                   const double (*g)[3], double* T) {
+__device__ __forceinline__ void tidal_from_differences(double x, double y, double z, double r, double dl,
+                                                       const double* D, double* T) {
     const double rho = sqrt(x * x + y * y);
     const double cp = (rho > 0.0) ? x / rho : 1.0, sp = (rho > 0.0) ? y / rho : 0.0;
     const double ct = z / r, st = rho / r;
     const double s2p = 2.0 * sp * cp, s2t = 2.0 * st * ct;
-    const double dxxp = (g[1][0] - g[0][0]) / dl;
-    const double dxyp = (g[3][0] - g[2][0]) / dl;
-    const double dxzp = (g[5][0] - g[4][0]) / dl;
-    const double dyyp = (g[3][1] - g[2][1]) / dl;
-    const double dyzp = (g[5][1] - g[4][1]) / dl;
-    const double dzzp = (g[5][2] - g[4][2]) / dl;
+    const double dxxp = D[0] / dl;
+    const double dxyp = D[1] / dl;
+    const double dxzp = D[2] / dl;
+    const double dyyp = D[3] / dl;
+    const double dyzp = D[4] / dl;
+    const double dzzp = D[5] / dl;
     T[0] = sp * sp * dxxp + cp * cp * dyyp - s2p * dxyp;
     T[2] = cp * cp * ct * ct * dxxp + sp * sp * ct * ct * dyyp + st * st * dzzp + s2p * ct * ct * dxyp -
            sp * s2t * dyzp - cp * s2t * dxzp;
     T[1] = ct * cp * sp * (dyyp - dxxp) + (cp * cp - sp * sp) * ct * dxyp + sp * st * dxzp - cp * st * dyzp;
+}
+
+// The same from the samples themselves: g[k][d], sample k in (x-, x+, y-, y+,
+// z-, z+), component d of grad phi.
+__device__ __forceinline__ void tidal_from_samples(double x, double y, double z, double r, double dl,
+                                                   const double (*g)[3], double* T) {
+    const double D[6] = {g[1][0] - g[0][0], g[3][0] - g[2][0], g[5][0] - g[4][0],
+                         g[3][1] - g[2][1], g[5][1] - g[4][1], g[5][2] - g[4][2]};
+    tidal_from_differences(x, y, z, r, dl, D, T);
 }
 
 // dl = 2^-(level - 1 - log2(EXTENT)) (lensing.h:429)

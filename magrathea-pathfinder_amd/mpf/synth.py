@@ -204,6 +204,27 @@ def amr_coords(base_level, half, refine, max_level):
     return np.concatenate(levels), np.concatenate(xs), np.concatenate(ys), np.concatenate(zs)
 
 
+def _parents_kept(level, kx, ky, kz, keep):
+    """Mask of the cells whose every stored ancestor is kept (a dropped cell
+    takes its descendants with it, so the tree stays parent-closed); cells
+    whose centre is outside a hole may keep only some of their children."""
+    ok = keep.copy()
+    lmin = int(level.min())
+
+    def code(l, x, y, z):
+        return (np.asarray(x, dtype=np.uint64) << np.uint64(42)) | (np.asarray(y, dtype=np.uint64) << np.uint64(21)) | \
+            np.asarray(z, dtype=np.uint64)
+    for l in range(lmin + 1, int(level.max()) + 1):
+        par = level == l - 1
+        kept_par = np.sort(code(l - 1, kx[par & ok], ky[par & ok], kz[par & ok]))
+        cur = np.nonzero(level == l)[0]
+        c = code(l - 1, kx[cur] >> np.uint64(1), ky[cur] >> np.uint64(1), kz[cur] >> np.uint64(1))
+        pos = np.searchsorted(kept_par, c)
+        found = (pos < kept_par.size) & (kept_par[np.minimum(pos, kept_par.size - 1)] == c)
+        ok[cur] &= found
+    return ok
+
+
 def analytic_tree(base_level=6, half=0.5, max_level=None, refine=None, length=1.0e25, seed=0,
                   holes=None):
     """A small AMR tree with analytic fields, for tests.  holes: optional
@@ -218,6 +239,7 @@ def analytic_tree(base_level=6, half=0.5, max_level=None, refine=None, length=1.
     cx, cy, cz = formats.cell_centers(level, kx, ky, kz)
     if holes is not None:
         keep = ~holes(level, cx, cy, cz)
+        keep &= _parents_kept(level, kx, ky, kz, keep)
         level, kx, ky, kz, cx, cy, cz = (v[keep] for v in (level, kx, ky, kz, cx, cy, cz))
     f = analytic_fields(cx, cy, cz, length, seed=seed)
     rec, order = formats.make_records(level, kx, ky, kz, f)

@@ -57,13 +57,31 @@ def tree(name):
         hole = _patch(hole_c, 0.05)
         return synth.analytic_tree(base_level=6, half=0.5, max_level=9, refine=refine, length=LENGTH, seed=2,
                                    holes=hole)
+    if name == "partial":
+        # the refined patch of "amr" cut by a hole: parents outside the hole
+        # keep only some of their children (partial octs, as a cone's edge
+        # leaves them in the reference's cone files)
+        inner = _patch(0.22 * DIR, 0.09)
+        deep = _patch(0.2 * DIR + np.array([0.02, 0.0, 0.0]), 0.04)
+
+        def refine(level, x, y, z):
+            m = inner(level, x, y, z)
+            if level >= 7:
+                m &= deep(level, x, y, z)
+            return m
+        box = _patch(0.2 * DIR + np.array([0.03, 0.01, -0.01]), 0.012)
+
+        def cut(level, x, y, z):   # cells of levels 8 and 9 only: their level-7 parents stay
+            return (level >= 8) & box(level, x, y, z)
+        return synth.analytic_tree(base_level=6, half=0.5, max_level=9, refine=refine, length=LENGTH, seed=4,
+                                   holes=cut)
     if name == "coarse":
         return synth.analytic_tree(base_level=5, half=0.5, max_level=7, refine=_patch(0.15 * DIR, 0.12),
                                    length=LENGTH, seed=3)
     raise KeyError(name)
 
 
-TREES = ("uniform", "amr", "coarse")
+TREES = ("uniform", "amr", "coarse", "partial")
 
 
 def records_digest(rec):

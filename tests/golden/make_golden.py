@@ -7,7 +7,7 @@ the deterministic inputs of tests/cases.py.  The fixtures are data only:
 inputs are regenerated by tests/cases.py and checked against the stored
 digest; outputs are the reference's numbers.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [tree ...]   (default: keys and every tree)
 """
 import os
 import sys
@@ -23,9 +23,11 @@ import cases  # noqa: E402
 from oracle import RefHarness  # noqa: E402
 
 
-def main():
+def main(names):
     R = RefHarness()
     tab = cases.cosmology()
+    if names:
+        return trees(R, tab, names)
     # keys: compute() at several levels on random and edge points
     rng = np.random.default_rng(11)
     kp = np.concatenate([rng.uniform(-2.2, 2.2, (400, 3)), rng.uniform(-1e-9, 1e-9, (20, 3)),
@@ -40,7 +42,11 @@ def main():
     np.savez_compressed(os.path.join(HERE, "keys.npz"), points=kp, levels=levels, keys=keys, info_level=info_lvl,
                         info_center=info_cen, info_extent=info_ext, info_parent=info_par, interp_x=xs,
                         interp_y=ys, interp_x0=x0, interp_out=R.interpolate(xs, ys, x0))
-    for name in cases.TREES:
+    trees(R, tab, cases.TREES)
+
+
+def trees(R, tab, names):
+    for name in names:
         rec, f64 = cases.tree(name)
         h = R.tree(rec)
         pts = cases.points(rec, n=1500)
@@ -74,4 +80,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])
