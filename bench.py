@@ -1,6 +1,6 @@
 """Benchmark: ray-RK4-steps/s of the photon-propagation path on one MI355X.
 
-Workload (config 1 of the benchmark set): a level-8 base grid (256^3) refined
+Workload "amr256_tsc": a level-8 base grid (256^3) refined
 to level 11 wherever the lognormal overdensity exceeds 8, FP64 fields; 1024 x
 1024 rays on a regular grid over a 10 x 10 degree cone, TSC interpolation,
 integrated to redshift 2.  One bench step = one pass of every ray to its stop
@@ -10,7 +10,7 @@ check (relative end-redshift error <= 1e-7) and its own throughput; the
 reference's CPU path (or the oracle when the reference is absent) is timed on
 a bounded subset of the rays.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload cfg1]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload amr256_tsc]
 """
 import argparse
 import json
@@ -24,13 +24,18 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "magrathea-pathfinder_amd"))
 
 HBM_PEAK = 8.0e12       # MI355X HBM3E, bytes/s (AMD public spec)
+WORKLOAD_TEXT = {
+    "amr256_tsc": "amr256_tsc: level-8 base (256^3, 8000 Mpc/h box) refined to level 11 where the lognormal "
+                  "overdensity > 8 (37.5 M cells), 1024x1024 rays over 10x10 deg, TSC, to z=2",
+    "uniform128_cic": "uniform128_cic: uniform 128^3 grid (3000 Mpc/h box), 64x64 rays over 5x5 deg, CIC, to z=0.5",
+}
 B_STEP_SPLIT_TSC = 540  # 27 cells x 5 fields x 4 bytes per RK4 step
 B_STEP_FP64_TSC = 1080  # 27 cells x 5 fields x 8 bytes
 BF16_PEAK = 2.5e15      # MI355X dense BF16 matrix-core peak, flop/s (AMD public spec)
 MLP_FLOP_PER_SAMPLE = 2 * (10 * 128 + 128 * 128 + 128 * 2)   # 35,840: the 10-128-128-2 network's forward
 
 
-def profiled_traffic(name="integrate_split_tsc_cfg1.txt"):
+def profiled_traffic(name="integrate_split_tsc_amr256_1m_rays.txt"):
     """HBM bytes per launch of the split kernel from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary (calibrated with a copy kernel), or None."""
     path = os.path.join(ROOT, "profiles", name)
@@ -223,7 +228,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="cfg1")
+    ap.add_argument("--workload", default="amr256_tsc", choices=("amr256_tsc", "uniform128_cic"))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-tier comparison")
     ap.add_argument("--no-extra", action="store_true",
@@ -298,8 +303,7 @@ def main():
         "value": value, "unit": "ray-RK4-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32-perturbation/fp64-background", "data": "synthetic (seeded GRF lightcone octree)",
-        "config": {"workload": f"{args.workload}: level-8 base (256^3) refined to level 11 where overdensity > 8, "
-                               f"1024x1024 rays over 10x10 deg, TSC, to z=2",
+        "config": {"workload": WORKLOAD_TEXT[args.workload],
                    "cells": int(ncells), "levels": [int(lmin), int(lmax)], "rays": int(n),
                    "rk4_steps_per_pass": int(steps)},
     }

@@ -43,12 +43,26 @@ constexpr int FSTRIDE = 28;  // floats per field (27 coefficients + pad): 7 x 16
 constexpr int BLOCK_DOUBLES = 4;
 constexpr int BLOCK_BYTES = BLOCK_DOUBLES * 8 + NFIELD * FSTRIDE * 4;  // 592
 
+// tslot word of a cell: NIL, or the block slot with PARTIAL_BIT set when the
+// cell stores some of its children (then it is the finest cell only of the
+// points outside them, and a lane may not reuse it for the next point)
+constexpr uint32_t PARTIAL_BIT = 0x80000000u;
+constexpr uint32_t SLOT_MASK = 0x7FFFFFFFu;
+
 struct SplitView {
-    const uint32_t* tslot;        // cell -> block slot or NIL
+    const uint32_t* tslot;        // cell -> block slot (| PARTIAL_BIT) or NIL
     const unsigned char* blocks;  // slot -> BLOCK_BYTES
 };
 
 // -------------------------------------------------------------- build
+
+// PARTIAL_BIT of the cells that store some (not all) of their children
+__global__ void partial_kernel(DevTree t, uint32_t* __restrict__ tslot) {
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= t.n) return;
+    const unsigned m = t.child_mask[g];
+    if (m != 0u && tslot[g] != NIL) tslot[g] |= PARTIAL_BIT;
+}
 
 __global__ void target_kernel(DevTree t, const uint8_t* __restrict__ lev, const uint32_t* __restrict__ kx,
                               const uint32_t* __restrict__ ky, const uint32_t* __restrict__ kz, uint32_t* target) {
@@ -212,11 +226,12 @@ __device__ __forceinline__ bool split_eval(const DevTree& t, const SplitView& sv
         unsigned fl = 0;
         const uint32_t kx32 = axis_coord(x, 32), ky32 = axis_coord(y, 32), kz32 = axis_coord(z, 32);
         const uint32_t g = find_cell(t, 32, kx32, ky32, kz32, &fl);
-        const uint32_t slot = (g == NIL) ? NIL : sv.tslot[g];
-        if (slot == NIL) {
+        const uint32_t word = (g == NIL) ? NIL : sv.tslot[g];
+        if (word == NIL) {
             I.valid = false;
             return false;
         }
+        const uint32_t slot = word & SLOT_MASK;
         I.lvl = fl;
         I.kx = kx32 >> (32 - fl);
         I.ky = ky32 >> (32 - fl);
@@ -224,7 +239,7 @@ __device__ __forceinline__ bool split_eval(const DevTree& t, const SplitView& sv
         I.valid = true;
         // a cell with some children stored (a partial oct, e.g. at a cone's
         // edge) is the finest cell only of the points outside those children
-        I.reuse = t.child_mask[g] == 0;
+        I.reuse = (word & PARTIAL_BIT) == 0;
         if (slot != I.slot) load_block<RC>(sv, slot, I);
     }
     const float u = (float)((x - I.cx) * I.inv);
@@ -265,14 +280,14 @@ __device__ __forceinline__ bool jac_eval(const DevTree& t, const SplitView& sv, 
         unsigned fl = 0;
         const uint32_t kx32 = axis_coord(x, 32), ky32 = axis_coord(y, 32), kz32 = axis_coord(z, 32);
         const uint32_t g = find_cell(t, 32, kx32, ky32, kz32, &fl);
-        const uint32_t slot = (g == NIL) ? NIL : sv.tslot[g];
-        if (slot == NIL) {
+        const uint32_t word = (g == NIL) ? NIL : sv.tslot[g];
+        if (word == NIL) {
             lw = 0u;
             return false;
         }
-        c = make_uint4(kx32 >> (32 - fl), ky32 >> (32 - fl), kz32 >> (32 - fl), slot);
+        c = make_uint4(kx32 >> (32 - fl), ky32 >> (32 - fl), kz32 >> (32 - fl), word & SLOT_MASK);
         cell = c;
-        lw = fl | (t.child_mask[g] == 0 ? 0x100u : 0u);
+        lw = fl | ((word & PARTIAL_BIT) ? 0u : 0x100u);
     }
     const unsigned char* blk = sv.blocks + (uint64_t)c.w * BLOCK_BYTES;
     const double4 h = *reinterpret_cast<const double4*>(blk);
@@ -738,9 +753,14 @@ bool split_build(SplitTree& s, const DevTree& t, const uint8_t* level, const uin
             slot_of[g] = (uint32_t)slot_cell.size();
             slot_cell.push_back((uint32_t)g);
         }
+    const uint64_t ns = slot_cell.size();
+    if (ns >= SLOT_MASK) {
+        err = "split tier: more than 2^31 - 1 stencil blocks";
+        cleanup();
+        return false;
+    }
     std::vector<uint32_t> tslot(n);
     for (uint64_t g = 0; g < n; ++g) tslot[g] = (target[g] == NIL) ? NIL : slot_of[target[g]];
-    const uint64_t ns = slot_cell.size();
     s.nslots = ns;
     s.bytes = n * 4 + std::max<uint64_t>(ns, 1) * BLOCK_BYTES;
     if (!chk(hipMalloc(&s.mem, s.bytes), "alloc blocks") ||
@@ -763,6 +783,10 @@ bool split_build(SplitTree& s, const DevTree& t, const uint8_t* level, const uin
     ok = chk(hipMemcpyAsync(s.tslot, tslot.data(), 4 * n, hipMemcpyHostToDevice, st), "copy") &&
          chk(hipMemcpyAsync(d_slot_cell, slot_cell.data(), 4 * std::max<uint64_t>(ns, 1), hipMemcpyHostToDevice, st),
              "copy");
+    if (ok && n) {
+        hipLaunchKernelGGL(partial_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, t, s.tslot);
+        ok = chk(hipGetLastError(), "partial-oct flags");
+    }
     if (ok && ns) {
         hipLaunchKernelGGL(block_kernel, dim3((unsigned)((ns + 127) / 128)), dim3(128), 0, st, t, d_lev, d_kx, d_ky,
                            d_kz, d_slot_cell, ns, s.blocks);

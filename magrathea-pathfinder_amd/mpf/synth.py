@@ -451,10 +451,10 @@ def lightcone_tree(base_input_level, box_mpch, sigma, threshold, max_input_level
 
 WORKLOADS = {
     # uniform level-7 grid, CIC, 64x64 rays over 5x5 deg to z = 0.5
-    "cfg0": dict(base_input_level=7, box_mpch=3000.0, sigma=1.0, threshold=1e30, max_input_level=7, seed=42,
+    "uniform128_cic": dict(base_input_level=7, box_mpch=3000.0, sigma=1.0, threshold=1e30, max_input_level=7, seed=42,
                  order=1, nside=64, aperture=5.0, z_stop=0.5, phi0=20.0, theta0=20.0),
     # level-8 base refined to level 11 where the overdensity exceeds 8, TSC,
     # 1024x1024 rays over 10x10 deg to z = 2
-    "cfg1": dict(base_input_level=8, box_mpch=8000.0, sigma=1.5, threshold=8.0, max_input_level=11, seed=42,
+    "amr256_tsc": dict(base_input_level=8, box_mpch=8000.0, sigma=1.5, threshold=8.0, max_input_level=11, seed=42,
                  noise=1.0, decay=0.75, order=2, nside=1024, aperture=10.0, z_stop=2.0, phi0=20.0, theta0=20.0),
 }
