@@ -21,8 +21,11 @@ extern "C" {
 
 typedef struct mpf_ctx mpf_ctx;
 
-/* interpolation order: the reference ORDER macro (Makefile:35-37) */
-enum { MPF_NGP = 0, MPF_CIC = 1, MPF_TSC = 2 };
+/* interpolation order: the reference ORDER macro (Makefile:35-37), and
+ * Order -1 of Integrator::integrate (integrator.h:505, 573): homogeneous
+ * data (a = 1, every other field 0) wherever the photon is inside the box,
+ * as for the reference photon of rays.cpp:211-214 (exact tiers only) */
+enum { MPF_HOMOGENEOUS = -1, MPF_NGP = 0, MPF_CIC = 1, MPF_TSC = 2 };
 
 /* precision tiers */
 enum {
@@ -93,6 +96,37 @@ int mpf_octree_info(const mpf_ctx* ctx, uint64_t* ncells, uint32_t* min_level, u
 int mpf_validate_octree(const void* rec48, uint64_t n, uint32_t* min_level, uint32_t* max_level, char* msg,
                         uint64_t msg_size);
 
+/* Input::acquire (input.h:1091-1233), host only: the table (t, a, da/dt,
+ * d2a/dt2; t the conformal time from the observer in seconds, increasing)
+ * derived from a RAMSES evolution file (rows: a, H/H0, tau H0, t_lookback H0,
+ * t_proper H0), with h and Omega_m from the parameter file ("key = value"
+ * lines) and the box length (Mpc/h) from the cell directory's name
+ * ("...boxlen<L>...", '-' read as the decimal point).  The same arithmetic
+ * as the reference, so the table is bit-identical.  n receives the number of
+ * rows; the columns are written when capacity >= n (call with capacity 0 to
+ * size them).  -EINVAL with a message when a file or a value is missing. */
+int mpf_cosmology_acquire(const char* evolution_file, const char* parameter_file, const char* cell_dir, double mpc,
+                          double* h, double* omega_m, double* lbox_mpch, double* t, double* a, double* dadt,
+                          double* d2adt2, uint64_t capacity, uint64_t* n);
+/* Input::correct(cosmology, trajectory) (input.h:1330-1378), host only: a is
+ * replaced, inside the trajectory's time range, by the a of a photon
+ * propagated in a homogeneous universe (traj_t, traj_a: m elements), and
+ * smoothed outside it (4 passes) when the trajectory is shorter than the
+ * table.  In place on a (n rows). */
+int mpf_cosmology_correct(const double* t, double* a, uint64_t n, const double* traj_t, const double* traj_a,
+                          uint64_t m);
+/* The cosmology preparation of rays.cpp:200-215: mpf_cosmology_acquire, then a
+ * photon from the box centre toward +x through a homogeneous level-5 octree
+ * (MPF_HOMOGENEOUS, on `device`, exact tier `precision`), then
+ * mpf_cosmology_correct with its trajectory.  Outputs as mpf_cosmology_acquire;
+ * d2adt2 is the uncorrected column, as in the reference. */
+int mpf_cosmology_prepare(int device, int precision, const char* evolution_file, const char* parameter_file,
+                          const char* cell_dir, double mpc, double* h, double* omega_m, double* lbox_mpch, double* t,
+                          double* a, double* dadt, double* d2adt2, uint64_t capacity, uint64_t* n);
+
+/* Message of the last failed mpf_cosmology_* call of this thread. */
+const char* mpf_cosmology_last_error(void);
+
 /* Cosmology table (t, a, da/dt), n rows, t increasing.  The table the
  * reference builds with Input::acquire + Input::correct (input.h:1091-1233,
  * 1330-1378) and reads with Utility::interpolate (utility.h:442-449). */
@@ -120,6 +154,14 @@ int mpf_integrate(mpf_ctx* ctx, const double* in9, uint64_t n, const mpf_stop* s
  * context stream unless sync != 0. */
 int mpf_integrate_dev(mpf_ctx* ctx, const double* dev_in9, uint64_t n, const mpf_stop* stop, double* dev_out,
                       int sync);
+/* Whole trajectories (the reference's Evolution<Photon>, integrator.h:605,
+ * 776, after the final pop), exact tiers: traj receives up to `capacity`
+ * elements per ray, element k of ray i at traj[(i*capacity + k)*MPF_NOUT]
+ * with k in its MPF_OUT_NSTEPS slot; ray i has out[i*MPF_NOUT +
+ * MPF_OUT_NSTEPS] + 1 elements.  -ENOSPC (and the first `capacity` elements)
+ * when a ray has more. */
+int mpf_integrate_trajectory(mpf_ctx* ctx, const double* in9, uint64_t n, const mpf_stop* stop, uint64_t capacity,
+                             double* out, double* traj);
 /* Constant-parameter surfaces: the stop criterion of hmaps (`stop_ray`,
  * surfaces `interpRefvec`, hmaps.cpp:182, hmaps.h:610-660) and of the
  * catalogue root-finder (radius = source distance, catalogues.h:431-456).

@@ -12,6 +12,7 @@
 #include "mpf_tree.h"
 #include "mpf_healpix.h"
 #include "mpf_mlp.h"
+#include "mpf_cosmology.h"
 
 namespace {
 
@@ -126,7 +127,7 @@ extern "C" {
 int mpf_create(mpf_ctx** out, const mpf_config* cfg) {
     if (!out || !cfg) return ERR_INVAL;
     *out = nullptr;
-    if (cfg->order < 0 || cfg->order > 2 || cfg->precision < 0 || cfg->precision > 2 || cfg->nsteps == 0 ||
+    if (cfg->order < MPF_HOMOGENEOUS || cfg->order > 2 || cfg->precision < 0 || cfg->precision > 2 || cfg->nsteps == 0 ||
         !(cfg->length > 0))
         return ERR_INVAL;
     int ndev = 0;
@@ -142,7 +143,7 @@ int mpf_create(mpf_ctx** out, const mpf_config* cfg) {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, cfg->device) == hipSuccess && cus > 0)
         c->cus = cus;
-    if (c->counters.ensure(2) != hipSuccess) {
+    if (c->counters.ensure(3) != hipSuccess) {
         delete c;
         return ERR_NOMEM;
     }
@@ -312,7 +313,8 @@ int mpf_interpolate(mpf_ctx* c, int order, const double* xyz, uint64_t n, double
 }
 
 static int integrate_impl(mpf_ctx* c, const double* dev_in9, uint64_t n, const mpf_stop* stop, double* dev_out,
-                          const mpf_surfaces* surf, double* dev_surf, int sync);
+                          const mpf_surfaces* surf, double* dev_surf, int sync, double* dev_traj = nullptr,
+                          uint64_t traj_cap = 0);
 
 int mpf_integrate_dev(mpf_ctx* c, const double* dev_in9, uint64_t n, const mpf_stop* stop, double* dev_out, int sync) {
     return integrate_impl(c, dev_in9, n, stop, dev_out, nullptr, nullptr, sync);
@@ -335,7 +337,7 @@ int mpf_integrate_surfaces_dev(mpf_ctx* c, const double* dev_in9, uint64_t n, co
 }
 
 static int integrate_impl(mpf_ctx* c, const double* dev_in9, uint64_t n, const mpf_stop* stop, double* dev_out,
-                          const mpf_surfaces* surf, double* dev_surf, int sync) {
+                          const mpf_surfaces* surf, double* dev_surf, int sync, double* dev_traj, uint64_t traj_cap) {
     if (!c || !stop || (n && !dev_in9)) return ERR_INVAL;
     if (n && !dev_out) {
         if (c->io_out.ensure((uint64_t)MPF_NOUT * n) != hipSuccess) return fail(c, ERR_NOMEM, "output buffer");
@@ -369,7 +371,16 @@ static int integrate_impl(mpf_ctx* c, const double* dev_in9, uint64_t n, const m
         A.jac = surf->jacobian ? 1 : 0;
         A.ray_value = surf->ray_value;
     }
-    HIPCHK(c, mpf::launch_fill_u64(c->counters.p, 2, 0ull, c->stream));
+    if (dev_traj) {
+        A.traj = dev_traj;
+        A.traj_cap = traj_cap;
+        A.traj_over = c->counters.p + 2;
+    }
+    if (c->cfg.order == MPF_HOMOGENEOUS && c->cfg.precision == MPF_PREC_SPLIT)
+        return fail(c, ERR_INVAL, "the homogeneous order runs in the exact tiers (REF, FP64)");
+    if (dev_traj && c->cfg.precision == MPF_PREC_SPLIT)
+        return fail(c, ERR_INVAL, "trajectories are written by the exact tiers (REF, FP64)");
+    HIPCHK(c, mpf::launch_fill_u64(c->counters.p, 3, 0ull, c->stream));
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     if (n) {
         if (c->cfg.precision == MPF_PREC_SPLIT) {
@@ -394,6 +405,139 @@ static int integrate_impl(mpf_ctx* c, const double* dev_in9, uint64_t n, const m
     if (sync) return mpf_synchronize(c);
     return 0;
 }
+
+int mpf_integrate_trajectory(mpf_ctx* c, const double* in9, uint64_t n, const mpf_stop* stop, uint64_t capacity,
+                             double* out, double* traj) {
+    if (!c || !stop || (n && (!in9 || !out || !traj)) || capacity == 0) return ERR_INVAL;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (n == 0) return 0;
+    const uint64_t tn = n * capacity * MPF_NOUT;
+    if (tn / capacity / MPF_NOUT != n) return fail(c, ERR_INVAL, "trajectory buffer size overflows");
+    DevBuf<double> din, dout, dtraj;
+    if (din.ensure(9 * n) || dout.ensure((uint64_t)MPF_NOUT * n) || dtraj.ensure(tn)) {
+        din.release(); dout.release(); dtraj.release();
+        return fail(c, ERR_NOMEM, "trajectory buffers (" + std::to_string(tn * 8) + " bytes)");
+    }
+    int rc = 0;
+    if (hipMemcpy(din.p, in9, 9 * n * 8, hipMemcpyHostToDevice) != hipSuccess) rc = fail(c, ERR_HIP, "upload");
+    if (!rc) rc = integrate_impl(c, din.p, n, stop, dout.p, nullptr, nullptr, 1, dtraj.p, capacity);
+    unsigned long long over = 0;
+    if (!rc && (hipMemcpy(out, dout.p, (uint64_t)MPF_NOUT * n * 8, hipMemcpyDeviceToHost) != hipSuccess ||
+                hipMemcpy(traj, dtraj.p, tn * 8, hipMemcpyDeviceToHost) != hipSuccess ||
+                hipMemcpy(&over, c->counters.p + 2, 8, hipMemcpyDeviceToHost) != hipSuccess))
+        rc = fail(c, ERR_HIP, "download");
+    din.release(); dout.release(); dtraj.release();
+    if (!rc && over) rc = fail(c, -28, std::to_string(over) + " rays have more than capacity elements");
+    return rc;
+}
+
+static int copy_table(const mpf::cosmo::Table& tab, double* t, double* a, double* dadt, double* d2adt2,
+                      uint64_t capacity, uint64_t* n) {
+    const uint64_t m = tab.t.size();
+    if (n) *n = m;
+    if (capacity >= m && m) {
+        if (!t || !a || !dadt || !d2adt2) return ERR_INVAL;
+        std::copy(tab.t.begin(), tab.t.end(), t);
+        std::copy(tab.a.begin(), tab.a.end(), a);
+        std::copy(tab.dadt.begin(), tab.dadt.end(), dadt);
+        std::copy(tab.d2adt2.begin(), tab.d2adt2.end(), d2adt2);
+    }
+    return 0;
+}
+
+static thread_local std::string g_cosmo_err;
+
+int mpf_cosmology_acquire(const char* evolution_file, const char* parameter_file, const char* cell_dir, double mpc,
+                          double* h, double* omega_m, double* lbox_mpch, double* t, double* a, double* dadt,
+                          double* d2adt2, uint64_t capacity, uint64_t* n) {
+    if (!evolution_file || !parameter_file || !cell_dir || !h || !omega_m || !lbox_mpch || !n || !(mpc > 0))
+        return ERR_INVAL;
+    mpf::cosmo::Table tab;
+    if (!mpf::cosmo::acquire(evolution_file, parameter_file, cell_dir, mpc, *h, *omega_m, *lbox_mpch, tab,
+                             g_cosmo_err)) {
+        *n = 0;
+        return ERR_INVAL;
+    }
+    return copy_table(tab, t, a, dadt, d2adt2, capacity, n);
+}
+
+int mpf_cosmology_correct(const double* t, double* a, uint64_t n, const double* traj_t, const double* traj_a,
+                          uint64_t m) {
+    if ((n && (!t || !a)) || (m && (!traj_t || !traj_a))) return ERR_INVAL;
+    mpf::cosmo::Table tab;
+    tab.t.assign(t, t + n);
+    tab.a.assign(a, a + n);
+    mpf::cosmo::correct(tab, std::vector<double>(traj_t, traj_t + m), std::vector<double>(traj_a, traj_a + m));
+    std::copy(tab.a.begin(), tab.a.end(), a);
+    return 0;
+}
+
+int mpf_cosmology_prepare(int device, int precision, const char* evolution_file, const char* parameter_file,
+                          const char* cell_dir, double mpc, double* h, double* omega_m, double* lbox_mpch, double* t,
+                          double* a, double* dadt, double* d2adt2, uint64_t capacity, uint64_t* n) {
+    if (!evolution_file || !parameter_file || !cell_dir || !h || !omega_m || !lbox_mpch || !n || !(mpc > 0) ||
+        (precision != MPF_PREC_REF && precision != MPF_PREC_FP64))
+        return ERR_INVAL;
+    mpf::cosmo::Table tab;
+    if (!mpf::cosmo::acquire(evolution_file, parameter_file, cell_dir, mpc, *h, *omega_m, *lbox_mpch, tab,
+                             g_cosmo_err)) {
+        *n = 0;
+        return ERR_INVAL;
+    }
+    // the homogeneous level-5 octree of rays.cpp:211 (assign(5) + homogenize)
+    const unsigned nref = 5, side = 1u << nref;
+    std::vector<std::pair<unsigned __int128, uint32_t>> keys;
+    keys.reserve((size_t)side * side * side);
+    for (uint32_t x = 0; x < side; ++x)
+        for (uint32_t y = 0; y < side; ++y)
+            for (uint32_t z = 0; z < side; ++z) {
+                unsigned __int128 k = 0;
+                for (unsigned l = 0; l < nref; ++l) {
+                    const unsigned b = nref - 1 - l;
+                    const unsigned nib = 8u | (((z >> b) & 1u) << 2) | (((y >> b) & 1u) << 1) | ((x >> b) & 1u);
+                    k |= (unsigned __int128)nib << (124 - 4 * l);
+                }
+                keys.push_back({k, 0u});
+            }
+    std::sort(keys.begin(), keys.end());
+    std::vector<unsigned char> rec(keys.size() * 48, 0);
+    const float one = 1.0f;
+    for (size_t i = 0; i < keys.size(); ++i) {
+        const uint64_t lo = (uint64_t)keys[i].first, hi = (uint64_t)(keys[i].first >> 64);
+        std::memcpy(&rec[48 * i], &lo, 8);
+        std::memcpy(&rec[48 * i + 8], &hi, 8);
+        std::memcpy(&rec[48 * i + 20], &one, 4);   // a (homogenize sets a = 1)
+    }
+    const double length = *lbox_mpch * mpc / *h;   // rays.cpp:208
+    mpf_config cfg{device, MPF_HOMOGENEOUS, precision, mpf::cosmo::reference_nsteps(tab.t.size(), nref), length};
+    mpf_ctx* c = nullptr;
+    int rc = mpf_create(&c, &cfg);
+    if (rc) {
+        g_cosmo_err = "reference photon: no device context";
+        return rc;
+    }
+    rc = mpf_load_octree_records(c, rec.data(), keys.size());
+    if (!rc) rc = mpf_set_cosmology(c, tab.t.data(), tab.a.data(), tab.dadt.data(), tab.t.size());
+    // steps to the edge of the box: EXTENT/2 over (cell size / nsteps), plus margin
+    const uint64_t cap = 2ull * side * cfg.nsteps + 64;
+    std::vector<double> out(MPF_NOUT), traj(cap * MPF_NOUT);
+    const double in9[9] = {1.0, 0.0, 0.0, 0.0, 0.0, 1.0, 1.0, 0.0, 0.0};   // launch (0,0,0) -> (2,0,0)
+    const mpf_stop none{MPF_STOP_NONE, 0.0, {0.0, 0.0, 0.0}};
+    if (!rc) rc = mpf_integrate_trajectory(c, in9, 1, &none, cap, out.data(), traj.data());
+    if (rc) g_cosmo_err = std::string("reference photon: ") + mpf_last_error(c);
+    mpf_destroy(c);
+    if (rc) return rc;
+    const uint64_t m = out[MPF_OUT_NSTEPS] < 0 ? 0 : (uint64_t)out[MPF_OUT_NSTEPS] + 1;
+    std::vector<double> tt(m), aa(m);
+    for (uint64_t k = 0; k < m; ++k) {
+        tt[k] = traj[k * MPF_NOUT + MPF_OUT_T];
+        aa[k] = traj[k * MPF_NOUT + MPF_OUT_A];
+    }
+    mpf::cosmo::correct(tab, tt, aa);
+    return copy_table(tab, t, a, dadt, d2adt2, capacity, n);
+}
+
+const char* mpf_cosmology_last_error(void) { return g_cosmo_err.c_str(); }
 
 int mpf_dev_alloc(mpf_ctx* c, uint64_t bytes, void** dev_ptr) {
     if (!c || !dev_ptr) return ERR_INVAL;

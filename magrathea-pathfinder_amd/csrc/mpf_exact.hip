@@ -78,6 +78,18 @@ struct RayState {
     bool has_prev;
 };
 
+// element k of a ray's trajectory, the element index in the NSTEPS slot
+__device__ __forceinline__ void traj_store(const IntegrateArgs& A, uint64_t ray, uint32_t k, const double* rec) {
+    if (k >= A.traj_cap) {
+        if (k == A.traj_cap) atomicAdd(A.traj_over, 1ull);
+        return;
+    }
+    double* o = A.traj + ((uint64_t)ray * A.traj_cap + k) * MPF_NOUT;
+#pragma unroll
+    for (int i = 0; i < MPF_NOUT; ++i) o[i] = rec[i];
+    o[MPF_OUT_NSTEPS] = (double)k;
+}
+
 template <int P>
 __device__ void ray_launch(const IntegrateArgs& A, const DevTree& t, TscCache& cache, const double* in9, RayState& s) {
     const double zero = 0.0, one = 1.0, two = 2.0;
@@ -261,6 +273,7 @@ __device__ bool ray_step(const IntegrateArgs& A, const DevTree& t, TscCache& cac
         s.back = ph;
         s.has_prev = true;
         s.appended++;
+        if (A.traj) traj_store(A, ray, s.appended, s.back.v);
         if (A.nsurf) {
             double T[3] = {0.0, 0.0, 0.0};
             if (A.jac) element_tidal<P>(t, s.back.v, T);
@@ -360,6 +373,7 @@ __global__ void __launch_bounds__(256) integrate_exact_kernel(IntegrateArgs A, D
                 if (my < A.n) {
                     ray = my;
                     ray_launch<P>(A, t, cache, A.in9 + 9 * my, s);
+                    if (A.traj) traj_store(A, my, 0, s.back.v);
                     if (A.ray_value) s.stopv = A.ray_value[my];
                     active = !empty7(s.data);
                     if (!active) ray_finish(A, s, ray);

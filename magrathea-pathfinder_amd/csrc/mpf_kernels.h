@@ -33,6 +33,12 @@ struct IntegrateArgs {
     double* surf_out = nullptr;      // n x nsurf x SF_NREC
     const double* ray_value = nullptr; // per-ray stop/surface value (one surface), or null
     int jac = 0;
+    // whole trajectories (exact tiers): element k of ray i at
+    // traj[(i * traj_cap + k) * MPF_NOUT], k < traj_cap; traj_over counts
+    // rays with more elements than that
+    double* traj = nullptr;
+    uint64_t traj_cap = 0;
+    unsigned long long* traj_over = nullptr;
 };
 
 hipError_t launch_integrate_exact(int precision, const IntegrateArgs& A, const DevTree& t, int grid, hipStream_t st);

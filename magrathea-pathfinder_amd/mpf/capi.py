@@ -13,7 +13,7 @@ import numpy as np
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_DIR, "libmpf.so")
 
-NGP, CIC, TSC = 0, 1, 2
+HOMOGENEOUS, NGP, CIC, TSC = -1, 0, 1, 2
 PREC_REF, PREC_FP64, PREC_SPLIT = 0, 1, 2
 STOP_NONE, STOP_RADIUS, STOP_LAMBDA, STOP_REDSHIFT, STOP_T, STOP_A, STOP_PLANE, STOP_S = -1, 0, 1, 2, 3, 4, 5, 6
 STOP_NAMES = {"none": -1, "r": 0, "radius": 0, "lambda": 1, "redshift": 2, "t": 3, "eta": 3, "a": 4,
@@ -34,7 +34,8 @@ EXPORTS = ("mpf_create", "mpf_destroy", "mpf_last_error", "mpf_set_precision", "
            "mpf_hmaps_launch_dev", "mpf_hmaps_fill_dev", "mpf_healpix_bin_dev", "mpf_catalogue_solve",
            "mpf_catalogue_solve_dev", "mpf_mlp_init", "mpf_mlp_state", "mpf_mlp_train", "mpf_mlp_gradient",
            "mpf_mlp_infer", "mpf_mlp_infer_dev", "mpf_catalogue_features", "mpf_catalogue_features_dev",
-           "mpf_catalogue_seed", "mpf_catalogue_seed_dev", "mpf_mfma_probe")
+           "mpf_catalogue_seed", "mpf_catalogue_seed_dev", "mpf_mfma_probe", "mpf_integrate_trajectory",
+           "mpf_cosmology_acquire", "mpf_cosmology_correct", "mpf_cosmology_prepare", "mpf_cosmology_last_error")
 
 CAT_FIELDS = ("phi", "theta", "phi_obs", "theta_obs", "sep", "iter", "z0", "z1", "z2", "z3", "z4", "z5",
               "a11", "a12", "a21", "a22", "status", "nrounds")
@@ -161,6 +162,12 @@ def load_library(path=LIB_PATH):
     lib.mpf_catalogue_seed.argtypes = [vp, vp, u64, i32, vp]
     lib.mpf_catalogue_seed_dev.argtypes = [vp, vp, u64, i32, vp]
     lib.mpf_mfma_probe.argtypes = [vp, vp, vp, vp]
+    lib.mpf_integrate_trajectory.argtypes = [vp, vp, u64, ctypes.POINTER(Stop), u64, vp, vp]
+    cp, dbl = ctypes.c_char_p, ctypes.c_double
+    lib.mpf_cosmology_acquire.argtypes = [cp, cp, cp, dbl, vp, vp, vp, vp, vp, vp, vp, u64, vp]
+    lib.mpf_cosmology_correct.argtypes = [vp, vp, u64, vp, vp, u64]
+    lib.mpf_cosmology_prepare.argtypes = [i32, i32, cp, cp, cp, dbl, vp, vp, vp, vp, vp, vp, vp, u64, vp]
+    lib.mpf_cosmology_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
 
@@ -196,6 +203,52 @@ def healpix_vec2pix(nside, vec):
     if lib.mpf_healpix_vec2pix(int(nside), _p(vec), ctypes.c_uint64(vec.shape[0]), _p(out)):
         raise MpfError("mpf_healpix_vec2pix: invalid arguments")
     return out
+
+
+def _cosmology_call(fn, args, name):
+    """Runs an mpf_cosmology_* table function twice (size, then fill)."""
+    lib = load_library()
+    h, om, lb = (ctypes.c_double(0.0) for _ in range(3))
+    n = ctypes.c_uint64(0)
+    null = ctypes.c_void_p(None)
+    rc = fn(*args, ctypes.byref(h), ctypes.byref(om), ctypes.byref(lb), null, null, null, null, ctypes.c_uint64(0),
+            ctypes.byref(n))
+    if rc != 0:
+        raise MpfError(f"{name}: {rc} {lib.mpf_cosmology_last_error().decode()}")
+    cols = np.zeros((4, n.value))
+    rc = fn(*args, ctypes.byref(h), ctypes.byref(om), ctypes.byref(lb), _p(cols[0]), _p(cols[1]), _p(cols[2]),
+            _p(cols[3]), ctypes.c_uint64(n.value), ctypes.byref(n))
+    if rc != 0:
+        raise MpfError(f"{name}: {rc} {lib.mpf_cosmology_last_error().decode()}")
+    return cols, h.value, om.value, lb.value
+
+
+def cosmology_acquire(evolution_file, parameter_file, cell_dir, mpc=3.08568e22):
+    """Input::acquire: ((t, a, dadt, d2adt2) rows, h, omega_m, lbox_mpch)."""
+    lib = load_library()
+    return _cosmology_call(lib.mpf_cosmology_acquire, (str(evolution_file).encode(), str(parameter_file).encode(),
+                                                       str(cell_dir).encode(), float(mpc)), "mpf_cosmology_acquire")
+
+
+def cosmology_correct(t, a, traj_t, traj_a):
+    """Input::correct: the corrected a column."""
+    lib = load_library()
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    a = np.array(a, dtype=np.float64)
+    tt = np.ascontiguousarray(traj_t, dtype=np.float64)
+    ta = np.ascontiguousarray(traj_a, dtype=np.float64)
+    rc = lib.mpf_cosmology_correct(_p(t), _p(a), ctypes.c_uint64(t.size), _p(tt), _p(ta), ctypes.c_uint64(tt.size))
+    if rc != 0:
+        raise MpfError(f"mpf_cosmology_correct: {rc}")
+    return a
+
+
+def cosmology_prepare(evolution_file, parameter_file, cell_dir, mpc=3.08568e22, device=0, precision=0):
+    """rays.cpp:200-215: acquire, the homogeneous reference photon, correct."""
+    lib = load_library()
+    return _cosmology_call(lib.mpf_cosmology_prepare, (int(device), int(precision), str(evolution_file).encode(),
+                                                       str(parameter_file).encode(), str(cell_dir).encode(),
+                                                       float(mpc)), "mpf_cosmology_prepare")
 
 
 def make_stop(kind="none", value=0.0, kitarget=(0.0, 0.0, 0.0)):
@@ -438,6 +491,19 @@ class Context:
         self._check(self.lib.mpf_integrate_dev(self.h, ctypes.c_void_p(in_ptr), ctypes.c_uint64(n),
                                                ctypes.byref(st), ctypes.c_void_p(out_ptr), int(bool(sync))),
                     "mpf_integrate_dev")
+
+    def trajectory(self, in9, stop="none", value=0.0, kitarget=(0.0, 0.0, 0.0), capacity=100000):
+        """Whole trajectories: (out n x NOUT, list of per-ray element arrays)."""
+        in9 = np.ascontiguousarray(in9, dtype=np.float64).reshape(-1, 9)
+        n = in9.shape[0]
+        st = make_stop(stop, value, kitarget)
+        out = np.zeros((n, NOUT))
+        traj = np.zeros((n, capacity, NOUT))
+        self._check(self.lib.mpf_integrate_trajectory(self.h, _p(in9), ctypes.c_uint64(n), ctypes.byref(st),
+                                                      ctypes.c_uint64(capacity), _p(out), _p(traj)),
+                    "mpf_integrate_trajectory")
+        m = np.maximum(out[:, OUT["nsteps"]], -1).astype(np.int64) + 1
+        return out, [traj[i, :m[i]] for i in range(n)]
 
     def last_run(self):
         steps = ctypes.c_uint64(0)

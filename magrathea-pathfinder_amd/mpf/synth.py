@@ -50,6 +50,40 @@ def flat_lcdm_table(h=0.7, omega_m=0.3, a_min=0.05, n=2048, mpc=MPC_SI):
     return np.stack([tau, a, dadt, d2])
 
 
+def ramses_evolution_files(directory, h=0.7, omega_m=0.3, n=2000, a_min=1e-3, box_mpch=2625.5, bad_rows=()):
+    """Writes a RAMSES-style evolution file (rows: a, H/H0 scaled as read by
+    the reference, tau H0, t_lookback H0, t_proper H0), a parameter file and
+    returns (evolution path, parameter path, cell directory name) for a flat
+    LCDM model.  Columns 1 and 4 are what the reference reads: da/deta / H0 =
+    a^2 E(a) and eta H0 (conformal time from today, negative in the past);
+    rows listed in bad_rows repeat the previous a (the reference drops them)."""
+    import os
+    a = np.linspace(a_min, 1.0, n)
+    E = np.sqrt(omega_m / a ** 3 + 1.0 - omega_m)
+    # eta(a) H0 = -int_a^1 da' / (a'^2 E(a')), on a fine grid
+    fine = np.linspace(a_min, 1.0, 200 * n)
+    Ef = np.sqrt(omega_m / fine ** 3 + 1.0 - omega_m)
+    f = 1.0 / (fine ** 2 * Ef)
+    cum = np.concatenate([[0.0], np.cumsum(0.5 * (f[1:] + f[:-1]) * np.diff(fine))])
+    eta = np.interp(a, fine, cum - cum[-1])
+    g = 1.0 / (fine * Ef)
+    cumt = np.concatenate([[0.0], np.cumsum(0.5 * (g[1:] + g[:-1]) * np.diff(fine))])
+    lookback = np.interp(a, fine, cumt[-1] - cumt)
+    rows = np.stack([a, a * a * E, eta, lookback, eta], 1)
+    for i in bad_rows:
+        rows[i, 0] = rows[i - 1, 0]
+    os.makedirs(directory, exist_ok=True)
+    evol = os.path.join(directory, "ramses_input_00001.dat")
+    with open(evol, "w") as fh:
+        for r in rows:
+            fh.write(" ".join(f"{v:.17e}" for v in r) + "\n")
+    par = os.path.join(directory, "param_00001.txt")
+    with open(par, "w") as fh:
+        fh.write(f"h={h:.8f}   \nOmega_m={omega_m:.8f}\nOmega_b=0.043557099\nn_s=0.96300000\nw=-1.0000000\n")
+    cell = os.path.join(directory, "boxlen" + f"{box_mpch:g}".replace(".", "-") + "_n1024_lcdmw7")
+    return evol, par, cell
+
+
 def comoving_distance_of_z(z, h=0.7, omega_m=0.3, mpc=MPC_SI, n=4096):
     """Comoving distance (metres) to redshift z in flat LCDM."""
     H0 = h * 1.0e5 / mpc
