@@ -302,6 +302,71 @@ int mpf_catalogue_seed_dev(mpf_ctx* ctx, const double* dev_src6, uint64_t n, int
  * network's kernels assume. */
 int mpf_mfma_probe(mpf_ctx* ctx, const float* a, const float* b, float* c);
 
+/* Cones and photon launch (host, no device needed).  A cone is MPF_CONE_N
+ * doubles: vertex x, y, z, base centre x, y, z, half-angle (rad), the three
+ * members of the reference's Cone (cone.h).  Results are bit-identical to the
+ * reference's; errors return a negative status with the message in
+ * mpf_cone_last_error(). */
+enum { MPF_CONE_N = 7 };
+const char* mpf_cone_last_error(void);
+/* Miscellaneous::GenerateFullskyCones (miscellaneous.h:200-218): cones with
+ * their vertex at the origin and bases on the spiral of
+ * HyperSphere::uniform (abstracthypersphere.h:365-418) on a sphere of the
+ * given radius (EXTENT / 2 = 2 in the reference programs, rays.cpp:106,145). */
+int mpf_cones_fullsky(uint32_t ncones, double radius, double* cones7);
+/* Miscellaneous::GenerateNarrowCones (miscellaneous.h:257-402): angles in
+ * degrees (phi, theta, thetay, thetaz of info_narrow_cone.txt).  cones7: the
+ * cones in the simulation frame (catalogue source selection); cones_rot7: the
+ * same cones with their base rotated into the octree frame (coneIfRot: ray
+ * launch, map pixels); rotm1: 3 x 3 row-major rotation into the octree frame
+ * (source positions, catalogues.h:818-824).  spiral_points (may be NULL):
+ * the spiral size that puts exactly ncones points in the region. */
+int mpf_cones_narrow(uint32_t ncones, double radius, double phi_deg, double theta_deg, double thetay_deg,
+                     double thetaz_deg, double* cones7, double* cones_rot7, double* rotm1, uint64_t* spiral_points);
+/* The ASCII branch of GenerateNarrowCones (miscellaneous.h:293-300): reads
+ * info_narrow_cone.txt with Input::parse; angles4 = phi (read as an integer,
+ * std::stoul), theta, thetay, thetaz, in degrees. */
+int mpf_narrow_cone_info(const char* path, double* angles4);
+/* Cone::inside (cone.h:359-373) for n points (x, y, z at every `stride`
+ * doubles). */
+int mpf_cone_inside(const double* cone7, const double* xyz, uint64_t n, uint64_t stride, uint8_t* out);
+/* Closest-cone ownership: out[i] = 1 when point i is inside cone icone and
+ * no other cone in front of it has its axis closer (the test of
+ * Miscellaneous::getTargets miscellaneous.h:523-578, by_value = 1: other
+ * cones are those whose members differ; of Hmaps::getPixels_per_cone2
+ * hmaps.h:261-315 and Integrator::launch integrator.h:166-249, by_value = 0:
+ * every other index). */
+int mpf_cone_select(const double* cones7, uint32_t ncones, uint32_t icone, int by_value, const double* xyz,
+                    uint64_t n, uint64_t stride, uint8_t* out);
+/* Miscellaneous::getTargets (miscellaneous.h:523-600): the indices, in input
+ * order, of the targets (positions at the start of every `stride` doubles,
+ * 8 in the reference) that cone icone owns. */
+int mpf_cone_targets(const double* cones7, uint32_t ncones, uint32_t icone, const double* targets, uint64_t n,
+                     uint64_t stride, uint64_t* index_out, uint64_t* nselected);
+/* Hmaps::getPixels_per_cone2 (hmaps.h:261-322): the RING pixels whose centre
+ * cone icone owns, in increasing order.  npix_out receives the count; with
+ * pix_out NULL only the count is returned. */
+int mpf_hmaps_cone_pixels(int64_t nside, const double* cones7, uint32_t ncones, uint32_t icone, int64_t* pix_out,
+                          uint64_t capacity, uint64_t* npix_out);
+/* Integrator::launch from a point towards angles (integrator.h:304-329):
+ * phi_theta n x 2 (rad) -> in9 n x 9. */
+int mpf_launch_angles(const double* observer3, const double* phi_theta, uint64_t n, double* in9);
+/* Integrator::launch from begin to end (integrator.h:264-290), n pairs. */
+int mpf_launch_towards(const double* begin3, const double* end3, uint64_t n, double* in9);
+/* rays.cpp:233-239 (ray_targets = random): one std::mt19937 seeded with
+ * `seed` (the reference seeds seed + rank), n photons from the origin towards
+ * random points of a sphere of radius micro_radius (1 / microcoeff) that
+ * cone icone owns (Integrator::launch into a series of cones,
+ * integrator.h:166-249, HyperSphere::random abstracthypersphere.h:323-345),
+ * each followed by one uniform draw times 2 pi: rotation (may be NULL), the
+ * angle by which the ray's bundle is turned. */
+int mpf_launch_cone(const double* cones7, uint32_t ncones, uint32_t icone, double micro_radius, uint32_t seed,
+                    uint64_t n, double* in9, double* rotation);
+/* Integrator::launch around one photon (integrator.h:345-382): count photons
+ * on a circle of half-angle `angle` turned by `rotation`, preceded by the
+ * photon itself when center != 0; out9 holds count + (center != 0) photons. */
+int mpf_launch_bundle(const double* in9, uint32_t count, double angle, double rotation, int center, double* out9);
+
 /* HEALPix RING scheme (host, no device needed), restated from the published
  * algorithm (Gorski et al. 2005): the reference links HEALPix C++ 3.70 /
  * chealpix (pix2vec_ring at hmaps.h:599, nside2npix at hmaps.cpp:185). */

@@ -35,7 +35,10 @@ EXPORTS = ("mpf_create", "mpf_destroy", "mpf_last_error", "mpf_set_precision", "
            "mpf_catalogue_solve_dev", "mpf_mlp_init", "mpf_mlp_state", "mpf_mlp_train", "mpf_mlp_gradient",
            "mpf_mlp_infer", "mpf_mlp_infer_dev", "mpf_catalogue_features", "mpf_catalogue_features_dev",
            "mpf_catalogue_seed", "mpf_catalogue_seed_dev", "mpf_mfma_probe", "mpf_integrate_trajectory",
-           "mpf_cosmology_acquire", "mpf_cosmology_correct", "mpf_cosmology_prepare", "mpf_cosmology_last_error")
+           "mpf_cosmology_acquire", "mpf_cosmology_correct", "mpf_cosmology_prepare", "mpf_cosmology_last_error",
+           "mpf_cone_last_error", "mpf_cones_fullsky", "mpf_cones_narrow", "mpf_narrow_cone_info", "mpf_cone_inside",
+           "mpf_cone_select", "mpf_cone_targets", "mpf_hmaps_cone_pixels", "mpf_launch_angles", "mpf_launch_towards",
+           "mpf_launch_cone", "mpf_launch_bundle")
 
 CAT_FIELDS = ("phi", "theta", "phi_obs", "theta_obs", "sep", "iter", "z0", "z1", "z2", "z3", "z4", "z5",
               "a11", "a12", "a21", "a22", "status", "nrounds")
@@ -47,6 +50,7 @@ SURF_FIELDS = ("valid", "firstid", "f", "x", "y", "z", "dist", "chi", "lambda", 
                "rho", "phi", "rhomax", "kt", "kx", "ky", "kz", "a11", "a12", "a22")
 SURF = {name: i for i, name in enumerate(SURF_FIELDS)}
 NSURF = 32
+CONE_N = 7          # vertex (3), base centre (3), half-angle: a cone of cone.h
 MAX_SURFACES = 16
 
 # learned first guess of the catalogue root-finder (sizes of include/mpf.h)
@@ -168,6 +172,19 @@ def load_library(path=LIB_PATH):
     lib.mpf_cosmology_correct.argtypes = [vp, vp, u64, vp, vp, u64]
     lib.mpf_cosmology_prepare.argtypes = [i32, i32, cp, cp, cp, dbl, vp, vp, vp, vp, vp, vp, vp, u64, vp]
     lib.mpf_cosmology_last_error.restype = ctypes.c_char_p
+    # cones and launch (host)
+    lib.mpf_cone_last_error.restype = ctypes.c_char_p
+    lib.mpf_cones_fullsky.argtypes = [u32, dbl, vp]
+    lib.mpf_cones_narrow.argtypes = [u32, dbl, dbl, dbl, dbl, dbl, vp, vp, vp, vp]
+    lib.mpf_narrow_cone_info.argtypes = [cp, vp]
+    lib.mpf_cone_inside.argtypes = [vp, vp, u64, u64, vp]
+    lib.mpf_cone_select.argtypes = [vp, u32, u32, i32, vp, u64, u64, vp]
+    lib.mpf_cone_targets.argtypes = [vp, u32, u32, vp, u64, u64, vp, vp]
+    lib.mpf_hmaps_cone_pixels.argtypes = [i64, vp, u32, u32, vp, u64, vp]
+    lib.mpf_launch_angles.argtypes = [vp, vp, u64, vp]
+    lib.mpf_launch_towards.argtypes = [vp, vp, u64, vp]
+    lib.mpf_launch_cone.argtypes = [vp, u32, u32, dbl, u32, u64, vp, vp]
+    lib.mpf_launch_bundle.argtypes = [vp, u32, dbl, dbl, i32, vp]
     _lib = lib
     return lib
 
@@ -202,6 +219,110 @@ def healpix_vec2pix(nside, vec):
     out = np.zeros(vec.shape[0], dtype=np.int64)
     if lib.mpf_healpix_vec2pix(int(nside), _p(vec), ctypes.c_uint64(vec.shape[0]), _p(out)):
         raise MpfError("mpf_healpix_vec2pix: invalid arguments")
+    return out
+
+
+def _cone_check(rc, name):
+    if rc != 0:
+        raise RuntimeError(f"{name}: {load_library().mpf_cone_last_error().decode()} ({rc})")
+
+
+def _f64(a, shape):
+    return np.ascontiguousarray(a, dtype=np.float64).reshape(shape)
+
+
+def cones_fullsky(ncones, radius=2.0):
+    """GenerateFullskyCones (miscellaneous.h:200-218): (ncones, 7) cones."""
+    out = np.zeros((ncones, CONE_N))
+    _cone_check(load_library().mpf_cones_fullsky(ncones, radius, _p(out)), "mpf_cones_fullsky")
+    return out
+
+
+def cones_narrow(ncones, phi_deg, theta_deg, thetay_deg, thetaz_deg, radius=2.0):
+    """GenerateNarrowCones (miscellaneous.h:257-402): (cones, cones rotated
+    into the octree frame, rotm1 3x3, spiral size)."""
+    c, cr, rot = np.zeros((ncones, CONE_N)), np.zeros((ncones, CONE_N)), np.zeros(9)
+    size = ctypes.c_uint64(0)
+    _cone_check(load_library().mpf_cones_narrow(ncones, radius, phi_deg, theta_deg, thetay_deg, thetaz_deg, _p(c), _p(cr),
+                                        _p(rot), ctypes.byref(size)), "mpf_cones_narrow")
+    return c, cr, rot.reshape(3, 3), size.value
+
+
+def narrow_cone_info(path):
+    """info_narrow_cone.txt -> (phi, theta, thetay, thetaz) in degrees."""
+    out = np.zeros(4)
+    _cone_check(load_library().mpf_narrow_cone_info(str(path).encode(), _p(out)), "mpf_narrow_cone_info")
+    return tuple(float(v) for v in out)
+
+
+def cone_inside(cone7, xyz):
+    xyz = _f64(xyz, (-1, 3))
+    out = np.zeros(xyz.shape[0], dtype=np.uint8)
+    _cone_check(load_library().mpf_cone_inside(_p(_f64(cone7, 7)), _p(xyz), xyz.shape[0], 3, _p(out)), "mpf_cone_inside")
+    return out.astype(bool)
+
+
+def cone_select(cones7, icone, xyz, by_value=False):
+    cones7 = _f64(cones7, (-1, CONE_N))
+    xyz = _f64(xyz, (-1, 3))
+    out = np.zeros(xyz.shape[0], dtype=np.uint8)
+    _cone_check(load_library().mpf_cone_select(_p(cones7), cones7.shape[0], icone, int(by_value), _p(xyz), xyz.shape[0], 3,
+                                       _p(out)), "mpf_cone_select")
+    return out.astype(bool)
+
+
+def cone_targets(cones7, icone, targets):
+    """getTargets (miscellaneous.h:523-600): indices of the owned targets
+    (rows whose first three columns are positions)."""
+    cones7 = _f64(cones7, (-1, CONE_N))
+    targets = np.ascontiguousarray(targets, dtype=np.float64)
+    n, stride = targets.shape
+    idx = np.zeros(n, dtype=np.uint64)
+    k = ctypes.c_uint64(0)
+    _cone_check(load_library().mpf_cone_targets(_p(cones7), cones7.shape[0], icone, _p(targets), n, stride, _p(idx),
+                                        ctypes.byref(k)), "mpf_cone_targets")
+    return idx[:k.value].astype(np.int64)
+
+
+def hmaps_cone_pixels(nside, cones7, icone):
+    """getPixels_per_cone2 (hmaps.h:261-322): owned RING pixels, ascending."""
+    cones7 = _f64(cones7, (-1, CONE_N))
+    k = ctypes.c_uint64(0)
+    _cone_check(load_library().mpf_hmaps_cone_pixels(nside, _p(cones7), cones7.shape[0], icone, None, 0, ctypes.byref(k)),
+                "mpf_hmaps_cone_pixels")
+    pix = np.zeros(max(1, k.value), dtype=np.int64)
+    _cone_check(load_library().mpf_hmaps_cone_pixels(nside, _p(cones7), cones7.shape[0], icone, _p(pix), k.value,
+                                             ctypes.byref(k)), "mpf_hmaps_cone_pixels")
+    return pix[:k.value]
+
+
+def launch_angles(phi_theta, observer=(0.0, 0.0, 0.0)):
+    pt = _f64(phi_theta, (-1, 2))
+    out = np.zeros((pt.shape[0], 9))
+    _cone_check(load_library().mpf_launch_angles(_p(_f64(observer, 3)), _p(pt), pt.shape[0], _p(out)), "mpf_launch_angles")
+    return out
+
+
+def launch_towards(begin, end):
+    b, e = _f64(begin, (-1, 3)), _f64(end, (-1, 3))
+    out = np.zeros((b.shape[0], 9))
+    _cone_check(load_library().mpf_launch_towards(_p(b), _p(e), b.shape[0], _p(out)), "mpf_launch_towards")
+    return out
+
+
+def launch_cone(cones7, icone, n, seed, micro_radius=1.0):
+    """rays.cpp:233-239: (in9, bundle rotation) of n random rays in cone icone."""
+    cones7 = _f64(cones7, (-1, CONE_N))
+    in9, rot = np.zeros((n, 9)), np.zeros(n)
+    _cone_check(load_library().mpf_launch_cone(_p(cones7), cones7.shape[0], icone, micro_radius, seed, n, _p(in9), _p(rot)),
+                "mpf_launch_cone")
+    return in9, rot
+
+
+def launch_bundle(in9, count, angle, rotation=0.0, center=True):
+    out = np.zeros((count + bool(center), 9))
+    _cone_check(load_library().mpf_launch_bundle(_p(_f64(in9, 9)), count, angle, rotation, int(bool(center)), _p(out)),
+                "mpf_launch_bundle")
     return out
 
 

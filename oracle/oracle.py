@@ -223,6 +223,59 @@ class RefHarness(_Base):
         self.lib.ref_interpolate(_p(x), _p(y), _u64(x.size), _p(x0), _u64(x0.size), _p(out))
         return out
 
+    # cones and launch (cone.h, abstracthypersphere.h, integrator.h:124-382, rays.cpp:233-239)
+    def sphere_uniform(self, n, radius):
+        xyz = np.zeros((n, 3))
+        mm = np.zeros(2)
+        self.lib.ref_sphere_uniform(_u64(n), ctypes.c_double(radius), _p(xyz), _p(mm))
+        return xyz, mm
+
+    def cone_inside(self, cone7, xyz):
+        cone7 = np.ascontiguousarray(cone7, dtype=np.float64).ravel()
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros(xyz.shape[0], dtype=np.uint8)
+        self.lib.ref_cone_inside(_p(cone7), _p(xyz), _u64(xyz.shape[0]), _p(out))
+        return out.astype(bool)
+
+    def launch_cone(self, cones7, icone, micro_radius, seed, n):
+        cones7 = np.ascontiguousarray(cones7, dtype=np.float64).reshape(-1, 7)
+        in9, rot = np.zeros((n, 9)), np.zeros(n)
+        self.lib.ref_launch_cone(_p(cones7), ctypes.c_uint32(cones7.shape[0]), ctypes.c_uint32(icone),
+                                 ctypes.c_double(micro_radius), ctypes.c_uint32(seed), _u64(n), _p(in9), _p(rot))
+        return in9, rot
+
+    def launch_angles(self, observer, phi_theta):
+        obs = np.ascontiguousarray(observer, dtype=np.float64)
+        pt = np.ascontiguousarray(phi_theta, dtype=np.float64).reshape(-1, 2)
+        out = np.zeros((pt.shape[0], 9))
+        self.lib.ref_launch_angles(_p(obs), _p(pt), _u64(pt.shape[0]), _p(out))
+        return out
+
+    def launch_towards(self, begin, end):
+        b = np.ascontiguousarray(begin, dtype=np.float64).reshape(-1, 3)
+        e = np.ascontiguousarray(end, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros((b.shape[0], 9))
+        self.lib.ref_launch_towards(_p(b), _p(e), _u64(b.shape[0]), _p(out))
+        return out
+
+    def launch_bundle(self, in9, count, angle, rotation, center):
+        in9 = np.ascontiguousarray(in9, dtype=np.float64).ravel()
+        out = np.zeros((count + bool(center), 9))
+        self.lib.ref_launch_bundle(_p(in9), ctypes.c_uint32(count), ctypes.c_double(angle),
+                                   ctypes.c_double(rotation), int(bool(center)), _p(out))
+        return out
+
+    def inv3(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(9)
+        r = np.zeros(9)
+        self.lib.ref_inv3(_p(a), _p(r))
+        return r.reshape(3, 3)
+
+    def constants(self):
+        self.lib.ref_deg.restype = ctypes.c_double
+        self.lib.ref_pi.restype = ctypes.c_double
+        return {"deg": self.lib.ref_deg(), "pi": self.lib.ref_pi()}
+
 
 def ref_available():
     return os.path.exists(REF_LIB)

@@ -262,4 +262,89 @@ void ref_interpolate(const double* x, const double* y, uint64_t n, const double*
     for (uint64_t i = 0; i < m; ++i) out[i] = Utility::interpolate(x0[i], vx, vy);
 }
 
+/* ---- cones and launch (rays.cpp:145-146, 233-239; cone.h; integrator.h:124-382) ---- */
+using Point = std::array<double, 3>;
+using Sphere = HyperSphere<3, Point>;
+
+static std::vector<Cone<Point>> make_cones(const double* c7, uint32_t n) {
+    std::vector<Cone<Point>> v(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        Point vx{{c7[7 * i + 0], c7[7 * i + 1], c7[7 * i + 2]}};
+        Point bs{{c7[7 * i + 3], c7[7 * i + 4], c7[7 * i + 5]}};
+        v[i].assign(vx, bs, c7[7 * i + 6]);
+    }
+    return v;
+}
+
+static void photon_to9(const Photon<real, 3>& p, double* o) {
+    o[0] = p.a(); o[1] = p.t(); o[2] = p.x(); o[3] = p.y(); o[4] = p.z();
+    o[5] = p.dtdl(); o[6] = p.dxdl(); o[7] = p.dydl(); o[8] = p.dzdl();
+}
+
+/* HyperSphere::uniform<2> on a sphere of the given radius at the origin:
+ * positions and the (min, max) pair distances it returns. */
+void ref_sphere_uniform(uint64_t n, double radius, double* xyz, double* minmax) {
+    Sphere sphere(Point{{0., 0., 0.}}, radius);
+    std::vector<Point> t(n);
+    auto r = sphere.uniform<2>(std::begin(t), std::end(t));
+    for (uint64_t i = 0; i < n; ++i) for (int d = 0; d < 3; ++d) xyz[3 * i + d] = t[i][d];
+    minmax[0] = r.first; minmax[1] = r.second;
+}
+
+/* Cone::inside for n points. */
+void ref_cone_inside(const double* cone7, const double* xyz, uint64_t n, uint8_t* out) {
+    auto c = make_cones(cone7, 1);
+    for (uint64_t i = 0; i < n; ++i) out[i] = c[0].inside(Point{{xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]}});
+}
+
+/* The launch loop of rays.cpp:233-239 with seed = parameters.seed + rank. */
+void ref_launch_cone(const double* cones7, uint32_t ncones, uint32_t icone, double micro_radius, uint32_t seed,
+                     uint64_t n, double* in9, double* rot) {
+    auto cones = make_cones(cones7, ncones);
+    Sphere micro(Point{{0., 0., 0.}}, micro_radius);
+    std::mt19937 engine1(seed);
+    std::uniform_real_distribution<real> distribution(0, 1);
+    const real pi = Constants<real>::pi();
+    const unsigned two = 2;
+    for (uint64_t i = 0; i < n; ++i) {
+        auto p = Integrator::launch(micro, cones[icone], cones, engine1, distribution);
+        photon_to9(p, in9 + 9 * i);
+        rot[i] = distribution(engine1) * two * pi;
+    }
+}
+
+void ref_launch_angles(const double* obs, const double* phi_theta, uint64_t n, double* in9) {
+    for (uint64_t i = 0; i < n; ++i)
+        photon_to9(Integrator::launch(obs[0], obs[1], obs[2], phi_theta[2 * i], phi_theta[2 * i + 1]), in9 + 9 * i);
+}
+
+void ref_launch_towards(const double* b, const double* e, uint64_t n, double* in9) {
+    for (uint64_t i = 0; i < n; ++i)
+        photon_to9(Integrator::launch(b[3 * i], b[3 * i + 1], b[3 * i + 2], e[3 * i], e[3 * i + 1], e[3 * i + 2]),
+                   in9 + 9 * i);
+}
+
+void ref_launch_bundle(const double* in9, uint32_t count, double angle, double rotation, int center, double* out9) {
+    Photon<real, 3> p;
+    p.a() = in9[0]; p.t() = in9[1]; p.x() = in9[2]; p.y() = in9[3]; p.z() = in9[4];
+    p.dtdl() = in9[5]; p.dxdl() = in9[6]; p.dydl() = in9[7]; p.dzdl() = in9[8];
+    if (center) {
+        auto v = Integrator::launch<true>(p, count, angle, rotation);
+        for (size_t i = 0; i < v.size(); ++i) photon_to9(v[i], out9 + 9 * i);
+    } else {
+        auto v = Integrator::launch<false>(p, count, angle, rotation);
+        for (size_t i = 0; i < v.size(); ++i) photon_to9(v[i], out9 + 9 * i);
+    }
+}
+
+/* Utility::invMatrix3d, row-major; Constants<double>::deg() and pi(). */
+void ref_inv3(const double* a, double* r) {
+    std::array<std::array<double, 3>, 3> m;
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) m[i][j] = a[3 * i + j];
+    auto x = Utility::invMatrix3d(m);
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) r[3 * i + j] = x[i][j];
+}
+double ref_deg() { return magrathea::Constants<double>::deg(); }
+double ref_pi() { return magrathea::Constants<double>::pi(); }
+
 }  // extern "C"

@@ -7,7 +7,7 @@ the deterministic inputs of tests/cases.py.  The fixtures are data only:
 inputs are regenerated by tests/cases.py and checked against the stored
 digest; outputs are the reference's numbers.
 
-    python tests/golden/make_golden.py [tree ...]   (default: keys and every tree)
+    python tests/golden/make_golden.py [tree ... | cones]   (default: keys, cones and every tree)
 """
 import os
 import sys
@@ -26,6 +26,8 @@ from oracle import RefHarness  # noqa: E402
 def main(names):
     R = RefHarness()
     tab = cases.cosmology()
+    if names == ["cones"]:
+        return cones(R)
     if names:
         return trees(R, tab, names)
     # keys: compute() at several levels on random and edge points
@@ -42,7 +44,42 @@ def main(names):
     np.savez_compressed(os.path.join(HERE, "keys.npz"), points=kp, levels=levels, keys=keys, info_level=info_lvl,
                         info_center=info_cen, info_extent=info_ext, info_parent=info_par, interp_x=xs,
                         interp_y=ys, interp_x0=x0, interp_out=R.interpolate(xs, ys, x0))
+    cones(R)
     trees(R, tab, cases.TREES)
+
+
+def cones(R):
+    """Spiral points and pair distances of HyperSphere::uniform, and photons
+    of Integrator::launch: into a series of cones with the rays.cpp:233-239
+    draw sequence, towards angles, towards points, and around a photon."""
+    out = {}
+    for n in (1, 2, 5, 12, 48, 300):
+        out[f"uniform_{n}"], out[f"uniform_minmax_{n}"] = R.sphere_uniform(n, 2.0)
+    # 12 cones built from the spiral (vertex at the origin), with their angle
+    # from the spiral's smallest pair distance
+    xyz, mm = R.sphere_uniform(12, 2.0)
+    fs = np.concatenate([np.zeros((12, 3)), xyz, np.full((12, 1), 1.8 * np.arcsin(mm[0] / 4.0))], axis=1)
+    out["launch_cones"] = fs
+    for ic, seed in ((0, 1), (7, 2021), (11, 5)):
+        out[f"launch_{ic}_{seed}_in9"], out[f"launch_{ic}_{seed}_rot"] = R.launch_cone(fs, ic, 1.0 / 3.0, seed, 500)
+    rng = np.random.default_rng(17)
+    pts = rng.normal(size=(4000, 3)) * rng.uniform(0.1, 3.0, (4000, 1))
+    out["inside_points"] = pts
+    out["inside"] = np.stack([R.cone_inside(fs[i], pts) for i in range(12)])
+    pt = np.c_[rng.uniform(-np.pi, np.pi, 2000), rng.uniform(0, np.pi, 2000)]
+    out["angles_phi_theta"] = pt
+    out["angles_in9"] = R.launch_angles([0.0, 0.0, 0.0], pt)
+    b, e = rng.normal(size=(500, 3)), rng.normal(size=(500, 3))
+    out["towards_begin"], out["towards_end"], out["towards_in9"] = b, e, R.launch_towards(b, e)
+    bundles = []
+    for k in range(200):
+        p9 = out["angles_in9"][k]
+        cnt, ang, rot = int(rng.integers(1, 17)), rng.uniform(1e-5, 0.3), rng.uniform(0, 2 * np.pi)
+        bundles.append((p9, cnt, ang, rot, k % 2 == 0, R.launch_bundle(p9, cnt, ang, rot, k % 2 == 0)))
+    out["bundle_params"] = np.array([[c, a, r, ce] for _, c, a, r, ce, _ in bundles])
+    out["bundle_out"] = np.concatenate([o for *_, o in bundles])
+    np.savez_compressed(os.path.join(HERE, "cones.npz"), **out)
+    print("cones written")
 
 
 def trees(R, tab, names):
