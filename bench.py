@@ -48,6 +48,16 @@ def profiled_traffic(name="integrate_split_tsc_amr256_1m_rays.txt"):
     return None
 
 
+def autotune_summary(trials, seconds):
+    """Compact record of the autotuner's trials: (refill, sort_bucket,
+    blocks_per_cu, waves_per_simd, kernel ms) per run."""
+    return {"seconds": seconds, "trials": [[int(r[0]), int(r[1]), int(r[2]), int(r[3]), round(float(r[4]), 2)]
+                                           for r in trials]}
+
+
+FIXED_LAUNCH = None   # launch configuration given on the command line (no autotuning)
+
+
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
@@ -113,18 +123,28 @@ def jacobian_maps(ctx, w, length, table, n=4194304, nside=2048, zmin=0.1, zmax=2
     ctx.upload(d_in, in9)
     ctx.upload(d_rv, zs)
     res = []
+    # launch configuration tuned on this batch (ray order by source redshift
+    # and direction, refill granularity, grid, occupancy)
+    tuned, trials, t_tune = ctx.launch_config(surfaces=True), np.zeros((0, 6)), 0.0
+    if FIXED_LAUNCH is None:
+        t0 = time.perf_counter()
+        tuned, trials = ctx.autotune_dev(d_in, n, d_out, surf_ptr=d_surf, kind="redshift", values=[0.0],
+                                         ray_value_ptr=d_rv)
+        t_tune = time.perf_counter() - t0
+        log(f"jacobian/maps autotune ({t_tune:.1f}s): {tuned}; trials ms {np.round(trials[:, 4], 1).tolist()}")
     for rep in range(2):   # the second pass is reported (the first also allocates the map accumulator)
         ctx.synchronize()
         t0 = time.perf_counter()
         ctx.integrate_surfaces_dev(d_in, n, d_out, d_surf, "redshift", [0.0], jacobian=True, sync=True,
                                    ray_value_ptr=d_rv)
         steps, ms = ctx.last_run()
+        kms = ctx.last_kernel_ms()
         t1 = time.perf_counter()
         ctx.healpix_bin_dev(nside, d_in, d_surf, n, 1, 0, d_maps)
         ctx.synchronize()
         t2 = time.perf_counter()
-        res.append((steps, ms, (t1 - t0) * 1e3, (t2 - t1) * 1e3))
-    steps, ms, wall_int, wall_bin = res[-1]
+        res.append((steps, ms, kms, (t1 - t0) * 1e3, (t2 - t1) * 1e3))
+    steps, ms, kms, wall_int, wall_bin = res[-1]
     surf = ctx.download(d_surf, (n, capi.NSURF))
     maps = ctx.download(d_maps, (4, npix))
     for p in (d_in, d_rv, d_out, d_surf, d_maps):
@@ -134,13 +154,16 @@ def jacobian_maps(ctx, w, length, table, n=4194304, nside=2048, zmin=0.1, zmax=2
     return {"workload": f"{n} rays uniform over {w['aperture']}x{w['aperture']} deg, per-ray z_s ~ U[{zmin}, {zmax}], "
                         f"infinitesimal Jacobian at z_s, kappa/gamma1/gamma2 binned into nside {nside} RING maps",
             "value": steps / (wall_int / 1e3 + wall_bin / 1e3), "unit": "ray-RK4-steps/s (integration + maps)",
-            "kernel_steps_per_s": steps / (ms / 1e3), "rk4_steps": int(steps), "integrate_ms": ms,
+            "kernel_steps_per_s": steps / (kms / 1e3), "rk4_steps": int(steps), "integrate_ms": ms,
+            "kernel_ms": kms,
             "integrate_wall_ms": wall_int, "bin_wall_ms": wall_bin, "rays_valid": int(valid.sum()),
             "pixels_hit": int(np.count_nonzero(maps[3])), "kappa_rms": float(np.sqrt(np.mean(
                 maps[0][maps[3] > 0] ** 2))),
+            "launch_config": tuned, "autotune": autotune_summary(trials, t_tune),
             "roofline": {"bound": "hbm", "bytes_per_step": b_step,
-                         "achieved": steps / (ms / 1e3) * b_step / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                         "frac": steps / (ms / 1e3) * b_step / HBM_PEAK}}
+                         "achieved": steps / (kms / 1e3) * b_step / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                         "frac": steps / (kms / 1e3) * b_step / HBM_PEAK,
+                         "traffic": profiled_traffic("integrate_split_jacobian_amr256_4m_rays.txt")}}
 
 
 def catalogue(ctx, w, length, table, n=1048576, zmin=0.1, zmax=1.5, acc=1e-8, ntrain=16384, seed=11):
@@ -235,6 +258,9 @@ def main():
                     help="skip the Jacobian/maps and catalogue workloads on the same octree")
     ap.add_argument("--only", choices=("jacobian_maps", "catalogue"), default=None,
                     help="run one of the extra workloads alone and print its record (profiling runs)")
+    ap.add_argument("--launch-config", default=None, metavar="R,B,C,W",
+                    help="use this launch configuration (refill, sort bucket, blocks per CU, waves per SIMD) for "
+                         "every workload instead of the autotuner's (profiling runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="only run the plain copy kernel over MB megabytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -267,6 +293,14 @@ def main():
     ctx.set_cosmology(table)
     ncells, lmin, lmax = ctx.octree_info()
     log(f"octree loaded in {time.time() - t0:.1f}s: {ncells} cells, levels {lmin}..{lmax}")
+    fixed = None
+    if args.launch_config:
+        r, b, cpu, wv = (int(v) for v in args.launch_config.split(","))
+        fixed = dict(refill=r, sort_bucket=b, blocks_per_cu=cpu, waves_per_simd=wv)
+        ctx.set_launch_config(**fixed)
+        ctx.set_launch_config(surfaces=True, **dict(fixed, waves_per_simd=min(wv, 2)))
+    global FIXED_LAUNCH
+    FIXED_LAUNCH = fixed
     if args.only:
         fn = jacobian_maps if args.only == "jacobian_maps" else catalogue
         print(json.dumps({args.only: fn(ctx, w, length, table)}), flush=True)
@@ -283,13 +317,21 @@ def main():
         ctx.integrate_dev(d_in, n, d_out, *stop, sync=True)
         steps, ms = ctx.last_run()
         log(f"warmup {i}: {steps} steps, kernel {ms:.1f} ms, wall {time.time() - t0:.2f}s")
+    # launch configuration tuned on this batch (untimed, like the warmup)
+    tuned, trials, t_tune = ctx.launch_config(), np.zeros((0, 6)), 0.0
+    if FIXED_LAUNCH is None:
+        t0 = time.perf_counter()
+        tuned, trials = ctx.autotune_dev(d_in, n, d_out, *stop)
+        t_tune = time.perf_counter() - t0
+        log(f"autotune ({t_tune:.1f}s): {tuned}; trials ms {np.round(trials[:, 4], 2).tolist()}")
     ctx.synchronize()
-    kernel_ms = []
+    kernel_ms, run_ms = [], []
     t0 = time.perf_counter()
     for i in range(args.steps):
         ctx.integrate_dev(d_in, n, d_out, *stop, sync=False)
         steps, ms = ctx.last_run()   # synchronizes the context stream
-        kernel_ms.append(ms)
+        run_ms.append(ms)
+        kernel_ms.append(ctx.last_kernel_ms())
     ctx.synchronize()
     wall = time.perf_counter() - t0
     total_steps = steps * args.steps
@@ -306,13 +348,20 @@ def main():
         "config": {"workload": WORKLOAD_TEXT[args.workload],
                    "cells": int(ncells), "levels": [int(lmin), int(lmax)], "rays": int(n),
                    "rk4_steps_per_pass": int(steps)},
+        "launch_config": tuned, "autotune": autotune_summary(trials, t_tune),
     }
     bstep = B_STEP_SPLIT_TSC if order == 2 else 320 // 2
     achieved = kernel_rate * bstep
+    traffic = profiled_traffic() if order == 2 else None
     result["roofline"] = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                          "frac": achieved / HBM_PEAK,
-                          "traffic": profiled_traffic() if order == 2 else None,
-                          "bytes_per_step": bstep, "kernel_ms": kms}
+                          "frac": achieved / HBM_PEAK, "traffic": traffic,
+                          "bytes_per_step": bstep, "kernel_ms": kms,
+                          "integration_ms": float(np.mean(run_ms)),
+                          "traffic_GB_per_s": traffic / (kms / 1e3) / 1e9 if traffic else None,
+                          "note": "achieved = algorithmic stencil bytes (27 cells x 5 fields x 4 B per step) over the "
+                                  "persistent kernel's HIP-event time; traffic = HBM bytes per launch from the "
+                                  "FETCH_SIZE/WRITE_SIZE counters (profiles/): the stencil blocks are served from "
+                                  "L2/MALL, so achieved can approach or pass the HBM peak"}
     if not args.no_fp64:
         ctx.set_precision(capi.PREC_FP64)
         t0 = time.time()

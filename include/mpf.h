@@ -400,9 +400,49 @@ int mpf_dev_download(mpf_ctx* ctx, void* host_dst, const void* dev_src, uint64_t
  * counters (FETCH_SIZE / WRITE_SIZE) against the integration kernels.  No
  * counterpart in the reference. */
 int mpf_dev_copy(mpf_ctx* ctx, void* dev_dst, const void* dev_src, uint64_t bytes);
+/* Launch configuration of the persistent integrators (mpf_integrate*,
+ * mpf_catalogue_solve*).  No counterpart in the reference, which splits rays
+ * over threads in contiguous chunks (Utility::parallelize,
+ * utility.h:127-143).  It changes no result, only the time.  kind 0: runs
+ * without surfaces; kind 1: runs with surfaces (maps, catalogue rounds). */
+typedef struct {
+    /* idle lanes a wavefront gathers before it takes rays from the queue
+     * (1..64; 64: whole wavefronts of rays at a time) */
+    uint32_t refill;
+    /* 0: rays in input order; else rays ordered by decreasing path length
+     * (per-ray stop or surface value), cut into buckets of this many rays,
+     * each bucket ordered by direction (a bucket of n rays: direction only) */
+    uint64_t sort_bucket;
+    /* workgroups of 256 lanes per compute unit in the persistent grid
+     * (0: 8 for the split tier, 4 for the exact tiers; at most 32) */
+    uint32_t blocks_per_cu;
+    /* split tier without surfaces: 1 = coefficient block kept in registers
+     * (1 wavefront per SIMD), 2..4 = coefficients re-read, at least that
+     * many wavefronts per SIMD; with surfaces 1 or 2; 0: 2 */
+    uint32_t waves_per_simd;
+} mpf_launch_cfg;
+int mpf_set_launch_config(mpf_ctx* ctx, int kind, const mpf_launch_cfg* cfg);
+int mpf_get_launch_config(mpf_ctx* ctx, int kind, mpf_launch_cfg* cfg);
+/* Autotuner of the launch configuration: runs the given device-resident
+ * batch (stop as for mpf_integrate_dev, or surfaces as for
+ * mpf_integrate_surfaces_dev when surf is not NULL) under candidate
+ * configurations, one knob at a time (sort bucket, refill, waves per SIMD,
+ * blocks per CU), keeps a change that saves at least 1 % of the kernel time
+ * (HIP events, ray ordering included), stores the result in the context for
+ * that kind and returns it in best.  trials (may be NULL): up to max_trials
+ * rows of 6 doubles (refill, sort_bucket, blocks_per_cu, waves_per_simd,
+ * ms, accepted steps); ntrials: the number of runs.  dev_out / dev_out_surf
+ * receive the results of the last run. */
+int mpf_autotune(mpf_ctx* ctx, const double* dev_in9, uint64_t n, const mpf_stop* stop, const mpf_surfaces* surf,
+                 double* dev_out, double* dev_out_surf, mpf_launch_cfg* best, double* trials, uint32_t max_trials,
+                 uint32_t* ntrials);
 /* Accepted steps and duration (HIP events on the context stream) of the
  * last integration kernel. */
 int mpf_last_run(mpf_ctx* ctx, uint64_t* total_steps, double* kernel_ms);
+/* Duration (HIP events on the context stream) of the persistent integration
+ * kernel alone in the last integration: without the ray ordering, the launch
+ * records and the final values that mpf_last_run's time includes. */
+int mpf_last_kernel_ms(mpf_ctx* ctx, double* ms);
 int mpf_synchronize(mpf_ctx* ctx);
 
 #ifdef __cplusplus

@@ -11,6 +11,7 @@
 #include "mpf_split.h"
 #include "mpf_tree.h"
 #include "mpf_healpix.h"
+#include "mpf_order.h"
 #include "mpf_mlp.h"
 #include "mpf_cosmology.h"
 
@@ -64,6 +65,7 @@ struct mpf_ctx {
     mpf_config cfg{};
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t evk0 = nullptr, evk1 = nullptr;  // the persistent kernel alone
     int cus = 256;
     std::string err;
     // octree
@@ -90,10 +92,16 @@ struct mpf_ctx {
     DevBuf<int64_t> io_i64;
     DevBuf<uint32_t> io_u32;
     uint64_t last_steps = 0;
-    float last_ms = 0.f;
+    float last_ms = 0.f, last_kernel_ms = 0.f;
+    bool kernel_timed = false;  // the last integration recorded evk0/evk1
     bool pending = false;
     // learned first guess of the catalogue root-finder
     mpf::MlpModel* mlp = nullptr;
+    // launch configuration of the persistent integrators: [0] without
+    // surfaces, [1] with surfaces; the ray order they read
+    mpf_launch_cfg launch[2] = {{1, 0, 0, 0}, {1, 0, 0, 0}};
+    DevBuf<uint32_t> order;
+    DevBuf<unsigned char> order_work;
 };
 
 static int fail(mpf_ctx* c, int code, const std::string& msg) {
@@ -136,7 +144,8 @@ int mpf_create(mpf_ctx** out, const mpf_config* cfg) {
     mpf_ctx* c = new mpf_ctx;
     c->cfg = *cfg;
     if (hipSetDevice(cfg->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipEventCreate(&c->evk0) != hipSuccess || hipEventCreate(&c->evk1) != hipSuccess) {
         delete c;
         return ERR_HIP;
     }
@@ -158,11 +167,14 @@ int mpf_destroy(mpf_ctx* c) {
     c->base.release(); c->child_first.release(); c->elem.release(); c->child_mask.release();
     c->f32.release(); c->f64.release(); c->cosmo.release(); c->counters.release();
     c->io_in.release(); c->io_out.release(); c->io_surf.release(); c->io_rv.release(); c->scratch.release();
+    c->order.release(); c->order_work.release();
     c->map_acc.release(); c->io_i64.release(); c->io_u32.release();
     mpf::split_release(c->split);
     mpf::mlp_destroy(c->mlp);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->evk0) (void)hipEventDestroy(c->evk0);
+    if (c->evk1) (void)hipEventDestroy(c->evk1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return 0;
@@ -380,8 +392,11 @@ static int integrate_impl(mpf_ctx* c, const double* dev_in9, uint64_t n, const m
         return fail(c, ERR_INVAL, "the homogeneous order runs in the exact tiers (REF, FP64)");
     if (dev_traj && c->cfg.precision == MPF_PREC_SPLIT)
         return fail(c, ERR_INVAL, "trajectories are written by the exact tiers (REF, FP64)");
+    const mpf_launch_cfg& L = c->launch[surf ? 1 : 0];
+    A.refill = L.refill;
     HIPCHK(c, mpf::launch_fill_u64(c->counters.p, 3, 0ull, c->stream));
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    c->kernel_timed = n > 0;
     if (n) {
         if (c->cfg.precision == MPF_PREC_SPLIT) {
             if (A.nsurf && !mpf::split_supports_surfaces())
@@ -390,13 +405,28 @@ static int integrate_impl(mpf_ctx* c, const double* dev_in9, uint64_t n, const m
                 if (int rc = ensure_split(c)) return rc;
                 HIPCHK(c, hipEventRecord(c->ev0, c->stream));
             }
+        }
+        // the queue order (timed with the integration)
+        if (L.sort_bucket && n > 1 && n < (1ull << 31)) {
+            const size_t wb = mpf::ray_order_work_bytes(n);
+            if (c->order.ensure(n) != hipSuccess || c->order_work.ensure(wb) != hipSuccess)
+                return fail(c, ERR_NOMEM, "ray-order buffers");
+            HIPCHK(c, mpf::launch_ray_order(dev_in9, A.ray_value, stop->kind == MPF_STOP_A ? 1 : 0, n,
+                                            L.sort_bucket, c->order.p, c->order_work.p, wb, c->stream));
+            A.queue_order = c->order.p;
+        }
+        if (c->cfg.precision == MPF_PREC_SPLIT) {
             if (c->scratch.ensure(16 * n) != hipSuccess) return fail(c, ERR_NOMEM, "launch-record buffer");
             A.scratch = c->scratch.p;
-            HIPCHK(c, mpf::launch_integrate_split(A, c->dt, c->split, c->cus, c->stream));
+            HIPCHK(c, mpf::launch_integrate_split(A, c->dt, c->split, c->cus, L.blocks_per_cu ? L.blocks_per_cu : 8,
+                                                  L.waves_per_simd ? L.waves_per_simd : 2, c->stream, c->evk0,
+                                                  c->evk1));
         } else {
             const uint64_t blocks = (n + 255) / 256;
-            const int grid = (int)std::min<uint64_t>(blocks, (uint64_t)c->cus * 4);
+            const int grid = (int)std::min<uint64_t>(blocks, (uint64_t)c->cus * (L.blocks_per_cu ? L.blocks_per_cu : 4));
+            HIPCHK(c, hipEventRecord(c->evk0, c->stream));
             HIPCHK(c, mpf::launch_integrate_exact(c->cfg.precision, A, c->dt, grid, c->stream));
+            HIPCHK(c, hipEventRecord(c->evk1, c->stream));
         }
         if (A.nsurf) HIPCHK(c, mpf::launch_surface_final(A, c->stream));
     }
@@ -971,8 +1001,101 @@ int mpf_synchronize(mpf_ctx* c) {
         float ms = 0.f;
         HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
         c->last_ms = ms;
+        ms = 0.f;
+        if (c->kernel_timed) HIPCHK(c, hipEventElapsedTime(&ms, c->evk0, c->evk1));
+        c->last_kernel_ms = ms;
         c->pending = false;
     }
+    return 0;
+}
+
+static int check_launch_cfg(mpf_ctx* c, const mpf_launch_cfg& L) {
+    if (L.refill < 1 || L.refill > 64) return fail(c, ERR_INVAL, "refill must be 1..64");
+    if (L.blocks_per_cu > 32) return fail(c, ERR_INVAL, "blocks_per_cu must be 0..32");
+    if (L.waves_per_simd > 4) return fail(c, ERR_INVAL, "waves_per_simd must be 0..4");
+    return 0;
+}
+
+int mpf_set_launch_config(mpf_ctx* c, int kind, const mpf_launch_cfg* cfg) {
+    if (!c || !cfg || (kind != 0 && kind != 1)) return ERR_INVAL;
+    if (int rc = check_launch_cfg(c, *cfg)) return rc;
+    c->launch[kind] = *cfg;
+    return 0;
+}
+
+int mpf_get_launch_config(mpf_ctx* c, int kind, mpf_launch_cfg* cfg) {
+    if (!c || !cfg || (kind != 0 && kind != 1)) return ERR_INVAL;
+    *cfg = c->launch[kind];
+    return 0;
+}
+
+int mpf_autotune(mpf_ctx* c, const double* dev_in9, uint64_t n, const mpf_stop* stop, const mpf_surfaces* surf,
+                 double* dev_out, double* dev_out_surf, mpf_launch_cfg* best, double* trials, uint32_t max_trials,
+                 uint32_t* ntrials) {
+    if (!c || (!stop && !surf) || n == 0 || !dev_in9) return ERR_INVAL;
+    const int kind = surf ? 1 : 0;
+    const bool split = c->cfg.precision == MPF_PREC_SPLIT;
+    uint32_t nt = 0;
+    auto run = [&](const mpf_launch_cfg& L, double& ms) -> int {
+        c->launch[kind] = L;
+        const int rc = surf ? mpf_integrate_surfaces_dev(c, dev_in9, n, surf, dev_out, dev_out_surf, 1)
+                            : integrate_impl(c, dev_in9, n, stop, dev_out, nullptr, nullptr, 1);
+        if (rc) return rc;
+        ms = c->last_ms;
+        if (trials && nt < max_trials) {
+            double* r = trials + 6 * (uint64_t)nt;
+            r[0] = L.refill; r[1] = (double)L.sort_bucket; r[2] = L.blocks_per_cu; r[3] = L.waves_per_simd;
+            r[4] = ms; r[5] = (double)c->last_steps;
+        }
+        ++nt;
+        return 0;
+    };
+    mpf_launch_cfg cur = c->launch[kind];
+    if (!cur.blocks_per_cu) cur.blocks_per_cu = split ? 8 : 4;
+    if (!cur.waves_per_simd && split) cur.waves_per_simd = 2;
+    double ms = 0.0, best_ms = 0.0;
+    // the first run also builds what the tier needs once (stencil blocks); time the second
+    if (int rc = run(cur, ms)) return rc;
+    if (int rc = run(cur, best_ms)) return rc;
+    // coordinate descent, one knob at a time; a change is kept when it saves
+    // at least 1 % (the run-to-run spread of the kernel time)
+    const uint64_t buckets[] = {0, 4096, 65536, n};
+    const uint32_t refills[] = {1, 16, 32, 64};
+    const uint32_t waves_plain[] = {1, 2, 3}, waves_surf[] = {1, 2};
+    const uint32_t blocks_split[] = {4, 8, 16}, blocks_exact[] = {2, 4, 8};
+    auto try_knob = [&](auto setter, const auto& values) -> int {
+        mpf_launch_cfg keep = cur;
+        for (auto v : values) {
+            mpf_launch_cfg L = cur;
+            setter(L, v);
+            if (std::memcmp(&L, &cur, sizeof(L)) == 0) continue;
+            if (int rc = run(L, ms)) return rc;
+            if (ms < 0.99 * best_ms) {
+                best_ms = ms;
+                keep = L;
+            }
+        }
+        cur = keep;
+        return 0;
+    };
+    int rc = try_knob([](mpf_launch_cfg& L, uint64_t v) { L.sort_bucket = v; }, buckets);
+    if (!rc) rc = try_knob([](mpf_launch_cfg& L, uint32_t v) { L.refill = v; }, refills);
+    if (!rc && split && !surf) rc = try_knob([](mpf_launch_cfg& L, uint32_t v) { L.waves_per_simd = v; }, waves_plain);
+    if (!rc && split && surf) rc = try_knob([](mpf_launch_cfg& L, uint32_t v) { L.waves_per_simd = v; }, waves_surf);
+    if (!rc && split) rc = try_knob([](mpf_launch_cfg& L, uint32_t v) { L.blocks_per_cu = v; }, blocks_split);
+    if (!rc && !split) rc = try_knob([](mpf_launch_cfg& L, uint32_t v) { L.blocks_per_cu = v; }, blocks_exact);
+    if (rc) return rc;
+    c->launch[kind] = cur;
+    if (best) *best = cur;
+    if (ntrials) *ntrials = nt;
+    return 0;
+}
+
+int mpf_last_kernel_ms(mpf_ctx* c, double* ms) {
+    if (!c || !ms) return ERR_INVAL;
+    int rc = mpf_synchronize(c);
+    if (rc) return rc;
+    *ms = c->last_kernel_ms;
     return 0;
 }
 

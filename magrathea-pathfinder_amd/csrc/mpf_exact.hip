@@ -363,7 +363,7 @@ __global__ void __launch_bounds__(256) integrate_exact_kernel(IntegrateArgs A, D
         // refill idle lanes from the queue, one atomic per wave
         const bool need = !active && !drained;
         const unsigned long long m = __ballot(need);
-        if (m) {
+        if (m && ((unsigned)__popcll(m) >= A.refill || !__any(active))) {
             const int leader = __ffsll((long long)m) - 1;
             unsigned long long base = 0;
             if ((int)lane == leader) base = atomicAdd(A.queue, (unsigned long long)__popcll(m));
@@ -371,10 +371,10 @@ __global__ void __launch_bounds__(256) integrate_exact_kernel(IntegrateArgs A, D
             if (need) {
                 const uint64_t my = base + (uint64_t)__popcll(m & ((1ull << lane) - 1ull));
                 if (my < A.n) {
-                    ray = my;
-                    ray_launch<P>(A, t, cache, A.in9 + 9 * my, s);
-                    if (A.traj) traj_store(A, my, 0, s.back.v);
-                    if (A.ray_value) s.stopv = A.ray_value[my];
+                    ray = A.queue_order ? (uint64_t)A.queue_order[my] : my;
+                    ray_launch<P>(A, t, cache, A.in9 + 9 * ray, s);
+                    if (A.traj) traj_store(A, ray, 0, s.back.v);
+                    if (A.ray_value) s.stopv = A.ray_value[ray];
                     active = !empty7(s.data);
                     if (!active) ray_finish(A, s, ray);
                 } else {

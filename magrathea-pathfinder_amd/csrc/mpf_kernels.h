@@ -14,6 +14,11 @@ struct IntegrateArgs {
     double* out = nullptr;           // n x MPF_NOUT
     uint64_t n = 0;
     unsigned long long* queue = nullptr;  // work-queue counter (zeroed before launch)
+    // queue position -> ray index (null: input order), and the number of idle
+    // lanes a wavefront gathers before it takes rays from the queue (1..64;
+    // a wavefront with no active lane always takes rays)
+    const uint32_t* queue_order = nullptr;
+    unsigned refill = 1;
     unsigned long long* steps = nullptr;  // accepted-step counter
     const double* ct = nullptr;      // cosmology table columns
     const double* ca = nullptr;

@@ -23,6 +23,14 @@ bool split_supports_surfaces();
 // features of catalogue sources (learned first guess) from the stencil blocks
 hipError_t launch_features_split(const DevTree& t, const SplitTree& s, const double* src, uint64_t n, double length,
                                  float* feat, hipStream_t st);
-hipError_t launch_integrate_split(const IntegrateArgs& A, const DevTree& t, const SplitTree& s, int cus, hipStream_t st);
+// grid = cus x blocks_per_cu workgroups of 256 lanes; waves_per_simd picks
+// the kernel variant: without surfaces 1 = coefficient block kept in
+// registers (1 wave per SIMD), 2..4 = coefficients re-read, at least that
+// many waves per SIMD; with surfaces 1 or 2 waves per SIMD
+// ev0/ev1 (may be null) are recorded just before and after the persistent
+// kernel, which the launch records and final values bracket
+hipError_t launch_integrate_split(const IntegrateArgs& A, const DevTree& t, const SplitTree& s, int cus,
+                                  unsigned blocks_per_cu, unsigned waves_per_simd, hipStream_t st,
+                                  hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 
 }  // namespace mpf

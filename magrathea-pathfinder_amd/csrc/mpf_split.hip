@@ -360,13 +360,17 @@ __device__ __forceinline__ double level_extent(unsigned l) { return EXTENT / (do
 // dphidt, level of the located cell.
 constexpr int LAUNCH_DOUBLES = 16;
 
+// Records are written in queue order (record q is the ray the queue hands
+// out at position q), so that lanes that take rays together read adjacent
+// records.
 __global__ void __launch_bounds__(256) launch_kernel(IntegrateArgs A, DevTree t) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= A.n) return;
+    const uint64_t ray = A.queue_order ? (uint64_t)A.queue_order[i] : i;
     const double c2 = C2;
     double core[9];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) core[k] = A.in9[9 * i + k];
+    for (int k = 0; k < 9; ++k) core[k] = A.in9[9 * ray + k];
     exact::TscCache cache;
     exact::cache_init(cache);
     double d[7];
@@ -438,7 +442,7 @@ integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
     while (true) {
         const bool need = !active && !drained;
         const unsigned long long m = __ballot(need);
-        if (m) {
+        if (m && ((unsigned)__popcll(m) >= A.refill || !__any(active))) {
             const int leader = __ffsll((long long)m) - 1;
             unsigned long long base = 0;
             if ((int)lane == leader) base = atomicAdd(A.queue, (unsigned long long)__popcll(m));
@@ -446,7 +450,7 @@ integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
             if (need) {
                 const uint64_t my = base + (uint64_t)__popcll(m & ((1ull << lane) - 1ull));
                 if (my < A.n) {
-                    ray = my;
+                    ray = A.queue_order ? (uint64_t)A.queue_order[my] : my;
                     // launch record prepared by launch_kernel (integrator.h:669-705)
                     const double* L = A.scratch + (uint64_t)LAUNCH_DOUBLES * my;
 #pragma unroll
@@ -462,7 +466,7 @@ integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
                     R.dt = (float)L[14];
                     R.F[0] = R.dt; R.F[1] = R.dx; R.F[2] = R.dy; R.F[3] = R.dz;
                     R.appended = 0;
-                    R.stopv = A.ray_value ? A.ray_value[my] : A.stop_value;
+                    R.stopv = A.ray_value ? A.ray_value[ray] : A.stop_value;
                     if (SURF) surf_init(SA, 0.0);
                     const unsigned l0 = (unsigned)L[15];
                     R.dl = level_extent(l0) * (R.core[CA] * R.core[CA] * ratio_k);
@@ -805,44 +809,24 @@ void split_release(SplitTree& s) {
     s = SplitTree();
 }
 
-// Kernel variant (MPF_SPLIT_VARIANT): 1 = coefficient block resident in
-// registers (1 wave/SIMD); 2, 3, 4 = coefficients read per evaluation, with
-// at least that many waves per SIMD.
-static int split_variant() {
-    const char* e = getenv("MPF_SPLIT_VARIANT");
-    const int v = e ? atoi(e) : 0;
-    return (v >= 1 && v <= 4) ? v : 2;
-}
-
-// waves per SIMD of the surfaces/Jacobian kernel: 2 (spills) or 1 (none)
-static int split_surf_waves() {
-    const char* e = getenv("MPF_SPLIT_SURF_WAVES");
-    const int v = e ? atoi(e) : 0;
-    return v == 1 ? 1 : 2;
-}
-
-static int split_waves_per_cu() {
-    const char* e = getenv("MPF_SPLIT_BLOCKS_PER_CU");
-    const int v = e ? atoi(e) : 0;
-    return v > 0 ? v : 8;
-}
-
 hipError_t launch_integrate_split(const IntegrateArgs& A, const DevTree& t, const SplitTree& s, int cus,
-                                  hipStream_t st) {
+                                  unsigned blocks_per_cu, unsigned waves_per_simd, hipStream_t st,
+                                  hipEvent_t ev0, hipEvent_t ev1) {
     SplitView sv{s.tslot, s.blocks};
     const uint64_t blocks = (A.n + 255) / 256;
-    const int grid = (int)std::min<uint64_t>(blocks, (uint64_t)cus * split_waves_per_cu());
+    const int grid = (int)std::min<uint64_t>(blocks, (uint64_t)cus * blocks_per_cu);
     const unsigned pgrid = (unsigned)((A.n + 255) / 256);
     hipLaunchKernelGGL(launch_kernel, dim3(pgrid), dim3(256), 0, st, A, t);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    if (ev0 && (e = hipEventRecord(ev0, st)) != hipSuccess) return e;
     if (A.nsurf) {
-        if (split_surf_waves() == 1)
+        if (waves_per_simd == 1)
             hipLaunchKernelGGL((integrate_split_kernel<false, 1, true>), dim3(grid), dim3(256), 0, st, A, t, sv);
         else
             hipLaunchKernelGGL((integrate_split_kernel<false, 2, true>), dim3(grid), dim3(256), 0, st, A, t, sv);
     } else {
-        switch (split_variant()) {
+        switch (waves_per_simd) {
             case 1: hipLaunchKernelGGL((integrate_split_kernel<true, 1, false>), dim3(grid), dim3(256), 0, st, A, t, sv); break;
             case 3: hipLaunchKernelGGL((integrate_split_kernel<false, 3, false>), dim3(grid), dim3(256), 0, st, A, t, sv); break;
             case 4: hipLaunchKernelGGL((integrate_split_kernel<false, 4, false>), dim3(grid), dim3(256), 0, st, A, t, sv); break;
@@ -851,6 +835,7 @@ hipError_t launch_integrate_split(const IntegrateArgs& A, const DevTree& t, cons
     }
     e = hipGetLastError();
     if (e != hipSuccess) return e;
+    if (ev1 && (e = hipEventRecord(ev1, st)) != hipSuccess) return e;
     hipLaunchKernelGGL(final_kernel, dim3(pgrid), dim3(256), 0, st, A, t);
     return hipGetLastError();
 }

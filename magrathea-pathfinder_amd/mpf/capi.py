@@ -38,7 +38,8 @@ EXPORTS = ("mpf_create", "mpf_destroy", "mpf_last_error", "mpf_set_precision", "
            "mpf_cosmology_acquire", "mpf_cosmology_correct", "mpf_cosmology_prepare", "mpf_cosmology_last_error",
            "mpf_cone_last_error", "mpf_cones_fullsky", "mpf_cones_narrow", "mpf_narrow_cone_info", "mpf_cone_inside",
            "mpf_cone_select", "mpf_cone_targets", "mpf_hmaps_cone_pixels", "mpf_launch_angles", "mpf_launch_towards",
-           "mpf_launch_cone", "mpf_launch_bundle")
+           "mpf_launch_cone", "mpf_launch_bundle", "mpf_set_launch_config", "mpf_get_launch_config", "mpf_autotune",
+           "mpf_last_kernel_ms")
 
 CAT_FIELDS = ("phi", "theta", "phi_obs", "theta_obs", "sep", "iter", "z0", "z1", "z2", "z3", "z4", "z5",
               "a11", "a12", "a21", "a22", "status", "nrounds")
@@ -107,6 +108,14 @@ def make_newton(accuracy=1e-8, max_iterations=10, vobs=(0.0, 0.0, 0.0), h=0.7, m
     return c
 
 
+class LaunchCfg(ctypes.Structure):
+    _fields_ = [("refill", ctypes.c_uint32), ("sort_bucket", ctypes.c_uint64), ("blocks_per_cu", ctypes.c_uint32),
+                ("waves_per_simd", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
 class Stop(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int), ("value", ctypes.c_double), ("kitarget", ctypes.c_double * 3)]
 
@@ -172,6 +181,11 @@ def load_library(path=LIB_PATH):
     lib.mpf_cosmology_correct.argtypes = [vp, vp, u64, vp, vp, u64]
     lib.mpf_cosmology_prepare.argtypes = [i32, i32, cp, cp, cp, dbl, vp, vp, vp, vp, vp, vp, vp, u64, vp]
     lib.mpf_cosmology_last_error.restype = ctypes.c_char_p
+    lib.mpf_last_kernel_ms.argtypes = [vp, ctypes.POINTER(dbl)]
+    lib.mpf_set_launch_config.argtypes = [vp, i32, ctypes.POINTER(LaunchCfg)]
+    lib.mpf_get_launch_config.argtypes = [vp, i32, ctypes.POINTER(LaunchCfg)]
+    lib.mpf_autotune.argtypes = [vp, vp, u64, ctypes.POINTER(Stop), ctypes.POINTER(Surfaces), vp, vp,
+                                 ctypes.POINTER(LaunchCfg), vp, u32, ctypes.POINTER(u32)]
     # cones and launch (host)
     lib.mpf_cone_last_error.restype = ctypes.c_char_p
     lib.mpf_cones_fullsky.argtypes = [u32, dbl, vp]
@@ -626,11 +640,53 @@ class Context:
         m = np.maximum(out[:, OUT["nsteps"]], -1).astype(np.int64) + 1
         return out, [traj[i, :m[i]] for i in range(n)]
 
+    def set_launch_config(self, surfaces=False, **kw):
+        """Set knobs of the persistent integrators' launch (refill, sort_bucket,
+        blocks_per_cu, waves_per_simd); the others keep their value."""
+        cfg = LaunchCfg()
+        self._check(self.lib.mpf_get_launch_config(self.h, int(bool(surfaces)), ctypes.byref(cfg)),
+                    "mpf_get_launch_config")
+        for k, v in kw.items():
+            setattr(cfg, k, int(v))
+        self._check(self.lib.mpf_set_launch_config(self.h, int(bool(surfaces)), ctypes.byref(cfg)),
+                    "mpf_set_launch_config")
+
+    def launch_config(self, surfaces=False):
+        cfg = LaunchCfg()
+        self._check(self.lib.mpf_get_launch_config(self.h, int(bool(surfaces)), ctypes.byref(cfg)),
+                    "mpf_get_launch_config")
+        return cfg.as_dict()
+
+    def autotune_dev(self, in_ptr, n, out_ptr, stop="none", value=0.0, surf_ptr=None, kind=None, values=None,
+                     jacobian=True, ray_value_ptr=None, max_trials=64):
+        """Tune the launch configuration on a device-resident batch (with
+        surfaces when surf_ptr is given); returns (best config dict, trials
+        array of rows refill, sort_bucket, blocks_per_cu, waves_per_simd, ms, steps)."""
+        st = make_stop(stop, value)
+        sf = None
+        if surf_ptr is not None:
+            sf = make_surfaces(kind, values if ray_value_ptr is None else [0.0], jacobian, ray_value_ptr)
+        best = LaunchCfg()
+        trials = np.zeros((max_trials, 6))
+        nt = ctypes.c_uint32(0)
+        self._check(self.lib.mpf_autotune(self.h, ctypes.c_void_p(in_ptr), ctypes.c_uint64(n), ctypes.byref(st),
+                                          ctypes.byref(sf) if sf is not None else None, ctypes.c_void_p(out_ptr),
+                                          ctypes.c_void_p(surf_ptr) if surf_ptr is not None else None,
+                                          ctypes.byref(best), _p(trials), max_trials, ctypes.byref(nt)),
+                    "mpf_autotune")
+        return best.as_dict(), trials[:min(nt.value, max_trials)]
+
     def last_run(self):
         steps = ctypes.c_uint64(0)
         ms = ctypes.c_double(0.0)
         self._check(self.lib.mpf_last_run(self.h, ctypes.byref(steps), ctypes.byref(ms)), "mpf_last_run")
         return steps.value, ms.value
+
+    def last_kernel_ms(self):
+        """Duration of the persistent integration kernel alone in the last run."""
+        ms = ctypes.c_double(0.0)
+        self._check(self.lib.mpf_last_kernel_ms(self.h, ctypes.byref(ms)), "mpf_last_kernel_ms")
+        return ms.value
 
     def dev_alloc(self, nbytes):
         """Device buffer on the context's device; returns its address (int)."""

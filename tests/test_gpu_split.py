@@ -17,25 +17,26 @@ Z_TOL = 1e-7
 POS_TOL = 1e-9
 
 
-def run(name, precision, in9, stop, val, variant=None, monkeypatch=None):
+def run(name, precision, in9, stop, val, variant=None):
     rec, f64 = cases.tree(name)
     c = capi.Context(cases.LENGTH, order=capi.TSC, precision=precision, nsteps=cases.NSTEPS)
     c.load_records(rec)
     c.load_fields64(f64)
     c.set_cosmology(cases.cosmology())
+    if variant is not None and precision == capi.PREC_SPLIT:
+        c.set_launch_config(waves_per_simd=variant)
     out, steps = c.integrate(in9, stop, val)
     c.close()
     return out, steps
 
 
-@pytest.mark.parametrize("variant", ("1", "2", "3", "4"))
+@pytest.mark.parametrize("variant", (1, 2, 3, 4))
 @pytest.mark.parametrize("name", cases.TREES)
-def test_split_vs_fp64(name, variant, monkeypatch):
-    monkeypatch.setenv("MPF_SPLIT_VARIANT", variant)
+def test_split_vs_fp64(name, variant):
     print(f"[case] split-vs-fp64 tree={name} variant={variant}", flush=True)
     in9 = cases.rays(n_side=32, aperture=50.0)
     for stop, val in (("redshift", 0.35), ("none", 0.0), ("radius", 0.3)):
-        a, sa = run(name, capi.PREC_SPLIT, in9, stop, val)
+        a, sa = run(name, capi.PREC_SPLIT, in9, stop, val, variant)
         b, sb = run(name, capi.PREC_FP64, in9, stop, val)
         na, nb = a[:, capi.OUT["nsteps"]], b[:, capi.OUT["nsteps"]]
         same = na == nb
