@@ -1143,3 +1143,142 @@ uint64_t mpfo_catalogue(const mpfo_octree* o, int order, int precision, const do
 void mpfo_interpolate(const double* x, const double* y, uint64_t n, const double* x0, uint64_t m, double* out) {
     for (uint64_t i = 0; i < m; ++i) out[i] = interpolate(x0[i], x, y, (int64_t)n);
 }
+
+/* ------------------------------------------------------------------ propagate */
+
+/* Integrator::launch around one photon (integrator.h:345-382): count photons
+ * on a circle of half-angle `angle` turned by `rotation`, after the photon
+ * itself.  sin and cos of one variable come from one sincos call, as the
+ * reference build computes them; the two trigonometric calls of atan2(...)
+ * stay separate. */
+void mpfo_launch_bundle(const double* in9, unsigned count, double angle, double rotation, double* out9) {
+    const double pi = 4.0 * atan(1.0);
+    const double step = (pi + pi) / (double)(count + (!count));
+    const double kx = in9[6], ky = in9[7], kz = in9[8];
+    const double r = sqrt((kx * kx) + (ky * ky) + (kz * kz));
+    double sa, ca, sth, cth, sps, cps;
+    sincos(angle, &sa, &ca);
+    sincos(acos(kz / r), &sth, &cth);
+    const double cph = cos(atan2(ky, kx));
+    const double sph = sin(atan2(ky, kx));
+    sincos(rotation, &sps, &cps);
+    const double rcos = r * ca, rsin = r * sa;
+    for (unsigned i = 0; i <= count; ++i)
+        for (int k = 0; k < 9; ++k) out9[9 * i + k] = in9[k];
+    if (!(r > 0.0)) return;
+    for (unsigned i = 0; i < count; ++i) {
+        double si, ci;
+        sincos(i * step, &si, &ci);
+        const double x = rsin * ci, y = rsin * si, z = rcos;
+        double* o = out9 + 9 * (i + 1);
+        o[6] = -cph * sps * cth * x - cph * cps * cth * y + cph * sth * z + sph * sps * y - sph * cps * x;
+        o[7] = -sph * sps * cth * x - sph * cps * cth * y - cph * sps * y + cph * cps * x + sth * sph * z;
+        o[8] = sth * sps * x + sth * cps * y + cth * z;
+    }
+}
+
+/* interpolation parameter of propagate (integrator.h:916-971): kind 2
+ * redshift, 4 a (as 1/a - 1), 3 t, 1 lambda, 0 r (distance from the first
+ * element); anything else redshift */
+static double prop_ref(const traj_t* t, uint64_t k, int kind) {
+    const photon* p = &t->p[k];
+    switch (kind) {
+        case 4: return 1 / p->core[CA] - 1;
+        case 3: return p->core[CT];
+        case 1: return p->lambda;
+        case 0: {
+            const double x = p->core[CX] - t->p[0].core[CX];
+            const double y = p->core[CY] - t->p[0].core[CY];
+            const double z = p->core[CZ] - t->p[0].core[CZ];
+            return sqrt(x * x + y * y + z * z);
+        }
+        default: return p->redshift;
+    }
+}
+
+/* Utility::interpolate (utility.h:442-449) of coordinate `dim` of ray t at
+ * parameter x0: upper_bound on the parameter sequence (the bisection of
+ * std::upper_bound, so a sequence that is not sorted is searched the same
+ * way), clamped to the ends */
+static double prop_interp(const traj_t* t, int kind, double x0, int dim) {
+    const long long n = (long long)t->n;
+    long long first = 0, len = n;
+    while (len > 0) {
+        const long long half = len >> 1;
+        if (x0 < prop_ref(t, first + half, kind)) len = half;
+        else { first = first + half + 1; len = len - half - 1; }
+    }
+    long long i = first;
+    i += (n > 1) * ((i <= 0) - (i >= n));
+    if (n <= 1) return n > 0 ? t->p[0].core[dim] : 0.0;
+    const double xa = prop_ref(t, i - 1, kind), xb = prop_ref(t, i, kind);
+    const double ya = t->p[i - 1].core[dim], yb = t->p[i].core[dim];
+    return ya + ((yb - ya) * (x0 - xa) / (xb - xa));
+}
+
+/* Integrator::propagate (integrator.h:826-1015): the bundle of `count` rays
+ * around the central photon, every ray integrated until the data ends, the
+ * validity checks, and the angular diameter distance along the central ray.
+ * rows/dist receive up to maxlen elements of the central trajectory; returns
+ * its element count (0 when the bundle is rejected). */
+uint64_t mpfo_propagate(const mpfo_octree* o, int order, int precision, const double* cosmo_tab, uint64_t ncos,
+                        double length, unsigned nsteps, const double* in9, unsigned count, double angle,
+                        double rotation, int kind, double amin, double* rows, double* dist, uint64_t maxlen) {
+    const unsigned nb = count + 1;
+    double* init = (double*)malloc(sizeof(double) * 9 * nb);
+    traj_t* t = (traj_t*)calloc(nb, sizeof(traj_t));
+    double* last = (double*)calloc(nb, sizeof(double));
+    mpfo_launch_bundle(in9, count, angle, rotation, init);
+    const double quarter = (4.0 / (2.0 * 1.0)) / 2.0;  /* EXTENT / 4 */
+    const double limit = 1.0 / (2.0 * 2.0 * 2.0);
+    unsigned ntraj = nb;
+    for (unsigned j = 0; j < ntraj; ++j) {
+        t[j].keep = 1;
+        run_full(o, order, precision, cosmo_tab, ncos, length, nsteps, -1, 0.0, NULL, init + 9 * j, &t[j]);
+        if (!t[j].n) {
+            ntraj = 0;
+        } else {
+            const photon* b = &t[j].p[t[j].n - 1];
+            const photon* f = &t[j].p[0];
+            const double cx = b->core[CX] - f->core[CX], cy = b->core[CY] - f->core[CY], cz = b->core[CZ] - f->core[CZ];
+            last[j] = sqrt(cx * cx + cy * cy + cz * cz);
+            ntraj *= (last[j] > quarter);
+            ntraj *= (!(isnormal(amin) && isnormal(b->ah) && (b->ah < 1.0))) || (!(b->ah > amin));
+        }
+    }
+    double mx = last[0], mn = last[0];
+    for (unsigned j = 1; j < nb; ++j) {
+        if (last[j] > mx) mx = last[j];
+        if (last[j] < mn) mn = last[j];
+    }
+    ntraj *= (fabs(mx - mn) / mx < limit);
+    uint64_t size = 0;
+    if (ntraj > 0) {
+        traj_t* c = &t[0];
+        size = c->n;
+        if (ntraj > 1) {
+            for (uint64_t k = 0; k < size; ++k) {
+                const double x0 = prop_ref(c, k, kind);
+                double d = c->p[k].distance;
+                for (unsigned j = 1; j < ntraj; ++j) {
+                    const double dx = prop_interp(&t[j], kind, x0, CX) - c->p[k].core[CX];
+                    const double dy = prop_interp(&t[j], kind, x0, CY) - c->p[k].core[CY];
+                    const double dz = prop_interp(&t[j], kind, x0, CZ) - c->p[k].core[CZ];
+                    d += sqrt(dx * dx + dy * dy + dz * dz);
+                }
+                d /= (ntraj - 1);
+                c->p[k].distance = d;
+            }
+            for (uint64_t k = 0; k < size; ++k) c->p[k].distance *= (length * c->p[k].core[CA]) / angle;
+        }
+        for (uint64_t k = 0; k < size && k < maxlen; ++k) {
+            photon_to_rec(&c->p[k], rows + MPFO_NREC * k, (double)k);
+            dist[k] = c->p[k].distance;
+        }
+    }
+    for (unsigned j = 0; j < nb; ++j) free(t[j].p);
+    free(t);
+    free(last);
+    free(init);
+    return size;
+}

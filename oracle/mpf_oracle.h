@@ -64,6 +64,17 @@ uint64_t mpfo_catalogue(const mpfo_octree* o, int order, int precision, const do
                         double length, unsigned nsteps, const double* src6, const double* init2, uint64_t n,
                         double accuracy, unsigned max_iterations, const double* vobs, double h, double mpc,
                         double* out, int nthreads);
+/* Integrator::launch around one photon (integrator.h:345-382): out9 holds
+ * the photon and its count bundle photons. */
+void mpfo_launch_bundle(const double* in9, unsigned count, double angle, double rotation, double* out9);
+/* Integrator::propagate (integrator.h:826-1015): central trajectory (rows,
+ * MPFO_NREC doubles per element) and its angular diameter distance (dist) at
+ * every element, up to maxlen elements; returns the element count, 0 when
+ * the bundle is rejected.  kind: interpolation parameter (2 redshift, 4 a,
+ * 3 t, 1 lambda, 0 r). */
+uint64_t mpfo_propagate(const mpfo_octree* o, int order, int precision, const double* cosmo_tab, uint64_t ncos,
+                        double length, unsigned nsteps, const double* in9, unsigned count, double angle,
+                        double rotation, int kind, double amin, double* rows, double* dist, uint64_t maxlen);
 void mpfo_interpolate(const double* x, const double* y, uint64_t n, const double* x0, uint64_t m, double* out);
 
 #ifdef __cplusplus

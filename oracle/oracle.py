@@ -43,6 +43,7 @@ class _Base:
         getattr(L, pf + "integrate").restype = _u64
         getattr(L, pf + "trajectory").restype = _u64
         getattr(L, pf + "jacobian_infinitesimal").restype = _u64
+        getattr(L, pf + "propagate").restype = _u64
         if pf == "mpfo_":
             L.mpfo_surfaces.restype = _u64
             L.mpfo_catalogue.restype = _u64
@@ -124,6 +125,25 @@ class Oracle(_Base):
                                      ctypes.c_double(stop_value), _p(kit), _p(in9), _p(rows), _u64(maxlen))
         return rows[:min(n, maxlen)]
 
+    def propagate(self, h, order, table, length, nsteps, in9, count, angle, rotation, kind=2, amin=0.0,
+                  precision=0, maxlen=100000):
+        """Integrator::propagate: (central rows, distance per element)."""
+        tab = np.ascontiguousarray(table, dtype=np.float64)
+        in9 = np.ascontiguousarray(in9, dtype=np.float64).reshape(9)
+        rows, dist = np.zeros((maxlen, NREC)), np.zeros(maxlen)
+        n = self.lib.mpfo_propagate(_vp(h), int(order), int(precision), _p(tab), _u64(tab.shape[1]),
+                                    ctypes.c_double(length), ctypes.c_uint(nsteps), _p(in9), ctypes.c_uint(count),
+                                    ctypes.c_double(angle), ctypes.c_double(rotation), int(kind),
+                                    ctypes.c_double(amin), _p(rows), _p(dist), _u64(maxlen))
+        return rows[:min(n, maxlen)], dist[:min(n, maxlen)]
+
+    def launch_bundle(self, in9, count, angle, rotation):
+        in9 = np.ascontiguousarray(in9, dtype=np.float64).reshape(9)
+        out = np.zeros((count + 1, 9))
+        self.lib.mpfo_launch_bundle(_p(in9), ctypes.c_uint(count), ctypes.c_double(angle),
+                                    ctypes.c_double(rotation), _p(out))
+        return out
+
     def jacobian(self, h, order, table, length, nsteps, stop_kind, stop_value, in9, dist, precision=0,
                  kitarget=None):
         tab = np.ascontiguousarray(table, dtype=np.float64)
@@ -203,6 +223,17 @@ class RefHarness(_Base):
                                     ctypes.c_uint(nsteps), int(stop_kind), ctypes.c_double(stop_value), _p(kit),
                                     _p(in9), _p(rows), _u64(maxlen))
         return rows[:min(n, maxlen)]
+
+    def propagate(self, h, order, table, length, nsteps, in9, count, angle, rotation, interp="redshift", amin=0.0,
+                  maxlen=100000):
+        tab = np.ascontiguousarray(table, dtype=np.float64)
+        in9 = np.ascontiguousarray(in9, dtype=np.float64).reshape(9)
+        rows, dist = np.zeros((maxlen, NREC)), np.zeros(maxlen)
+        n = self.lib.ref_propagate(_vp(h), int(order), _p(tab), _u64(tab.shape[1]), ctypes.c_double(length),
+                                   ctypes.c_uint(nsteps), _p(in9), ctypes.c_uint(count), ctypes.c_double(angle),
+                                   ctypes.c_double(rotation), interp.encode(), ctypes.c_double(amin), _p(rows),
+                                   _p(dist), _u64(maxlen))
+        return rows[:min(n, maxlen)], dist[:min(n, maxlen)]
 
     def jacobian(self, h, order, table, length, nsteps, stop_kind, stop_value, in9, dist, kitarget=None):
         tab = np.ascontiguousarray(table, dtype=np.float64)

@@ -337,6 +337,35 @@ void ref_launch_bundle(const double* in9, uint32_t count, double angle, double r
     }
 }
 
+/* Integrator::propagate<ORDER> (integrator.h:826-1015) of one central photon:
+ * rows (O_NREC doubles) and distance per element of the central trajectory,
+ * up to maxlen; returns the element count. */
+uint64_t ref_propagate(void* h, int order, const double* cosmo_tab, uint64_t ncos, double length, unsigned nsteps,
+                       const double* in9, unsigned count, double angle, double rotation, const char* interp,
+                       double amin, double* rows, double* dist, uint64_t maxlen) {
+    const Octree& oc = static_cast<RefOctree*>(h)->octree;
+    const Cosmo cosmo = make_cosmo(cosmo_tab, ncos);
+    Photon<real, 3> p;
+    p.a() = in9[0]; p.t() = in9[1]; p.x() = in9[2]; p.y() = in9[3]; p.z() = in9[4];
+    p.dtdl() = in9[5]; p.dxdl() = in9[6]; p.dydl() = in9[7]; p.dzdl() = in9[8];
+    const std::array<double, 3> vobs{{0., 0., 0.}};
+    Traj homogeneous;
+    Traj r;
+    switch (order) {
+        case 0: r = Integrator::propagate<0>(p, count, angle, rotation, std::string(interp), cosmo, oc, vobs, length,
+                                             nsteps, amin, std::string(), homogeneous); break;
+        case 1: r = Integrator::propagate<1>(p, count, angle, rotation, std::string(interp), cosmo, oc, vobs, length,
+                                             nsteps, amin, std::string(), homogeneous); break;
+        default: r = Integrator::propagate<2>(p, count, angle, rotation, std::string(interp), cosmo, oc, vobs, length,
+                                              nsteps, amin, std::string(), homogeneous); break;
+    }
+    for (uint64_t i = 0; i < r.size() && i < maxlen; ++i) {
+        photon_to_rec(r[i], rows + O_NREC * i, double(i));
+        dist[i] = r[i].distance();
+    }
+    return r.size();
+}
+
 /* Utility::invMatrix3d, row-major; Constants<double>::deg() and pi(). */
 void ref_inv3(const double* a, double* r) {
     std::array<std::array<double, 3>, 3> m;

@@ -78,6 +78,11 @@ def tree(name):
     if name == "coarse":
         return synth.analytic_tree(base_level=5, half=0.5, max_level=7, refine=_patch(0.15 * DIR, 0.12),
                                    length=LENGTH, seed=3)
+    if name == "wide":
+        # reaches past EXTENT / 4 = 1 from the observer, as the bundles of
+        # Integrator::propagate must (integrator.h:907-909)
+        return synth.analytic_tree(base_level=5, half=1.25, max_level=7, refine=_patch(0.6 * DIR, 0.15),
+                                   length=LENGTH, seed=5)
     raise KeyError(name)
 
 
