@@ -154,6 +154,24 @@ int mpf_integrate(mpf_ctx* ctx, const double* in9, uint64_t n, const mpf_stop* s
  * context stream unless sync != 0. */
 int mpf_integrate_dev(mpf_ctx* ctx, const double* dev_in9, uint64_t n, const mpf_stop* stop, double* dev_out,
                       int sync);
+/* Integrator::propagate (integrator.h:826-1015), the per-ray computation of
+ * the rays program (rays.cpp:276-284), exact tiers, any order: around each
+ * central photon i (in9, n x 9) a bundle of `count` rays on a circle of
+ * half-angle `angle` turned by rotation[i] (NULL: 0) (Integrator::launch,
+ * integrator.h:345-382), every ray integrated until the data ends.  A bundle
+ * is rejected when one of its rays is empty, ends within EXTENT / 4 of its
+ * start, fails the amin test on the data's scale factor at its end (amin 0:
+ * no test), or when the end distances differ by 1/8 or more.  Accepted:
+ * traj receives the central trajectory (capacity elements x MPF_NOUT doubles
+ * per photon, element k with k in its MPF_OUT_NSTEPS slot) and dist its
+ * angular diameter distance (Photon::distance, metres) at each element: the
+ * mean distance of the other rays at the same value of `interpolation`
+ * (MPF_STOP_REDSHIFT, _A (as 1/a - 1), _T, _LAMBDA, _RADIUS) times
+ * length a / angle.  nelem[i]: elements of the central trajectory, 0 when
+ * the bundle is rejected (the reference returns an empty trajectory).
+ * -ENOSPC when a ray has more than capacity elements. */
+int mpf_propagate(mpf_ctx* ctx, const double* in9, uint64_t n, uint32_t count, double angle, const double* rotation,
+                  int interpolation, double amin, uint64_t capacity, double* traj, double* dist, uint64_t* nelem);
 /* Whole trajectories (the reference's Evolution<Photon>, integrator.h:605,
  * 776, after the final pop), exact tiers: traj receives up to `capacity`
  * elements per ray, element k of ray i at traj[(i*capacity + k)*MPF_NOUT]

@@ -12,6 +12,8 @@
 #include "mpf_tree.h"
 #include "mpf_healpix.h"
 #include "mpf_order.h"
+#include "mpf_propagate.h"
+#include "mpf_cone.h"
 #include "mpf_mlp.h"
 #include "mpf_cosmology.h"
 
@@ -434,6 +436,91 @@ static int integrate_impl(mpf_ctx* c, const double* dev_in9, uint64_t n, const m
     c->pending = true;
     if (sync) return mpf_synchronize(c);
     return 0;
+}
+
+int mpf_propagate(mpf_ctx* c, const double* in9, uint64_t n, uint32_t count, double angle, const double* rotation,
+                  int interpolation, double amin, uint64_t capacity, double* traj, double* dist, uint64_t* nelem) {
+    if (!c || (n && (!in9 || !traj || !dist || !nelem)) || capacity == 0) return ERR_INVAL;
+    if (c->cfg.precision == MPF_PREC_SPLIT)
+        return fail(c, ERR_INVAL, "propagate needs whole trajectories: exact tiers (REF, FP64)");
+    if (interpolation != MPF_STOP_REDSHIFT && interpolation != MPF_STOP_A && interpolation != MPF_STOP_T &&
+        interpolation != MPF_STOP_LAMBDA && interpolation != MPF_STOP_RADIUS)
+        return fail(c, ERR_INVAL, "interpolation must be redshift, a, t, lambda or radius");
+    if (count > 4096) return fail(c, ERR_INVAL, "at most 4096 bundle rays");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (n == 0) return 0;
+    const unsigned nb = count + 1;
+    const uint64_t per_ray = capacity * MPF_NOUT * 8;
+    if (per_ray / MPF_NOUT / 8 != capacity) return fail(c, ERR_INVAL, "capacity too large");
+    // bundles per pass: trajectories of at most 16 GiB on the device
+    const uint64_t budget = 16ull << 30;
+    const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(n, budget / (per_ray * nb)));
+    DevBuf<double> din, dout, dtraj, ddist;
+    DevBuf<unsigned> dacc;
+    if (din.ensure(9 * nb * chunk) || dout.ensure((uint64_t)MPF_NOUT * nb * chunk) ||
+        dtraj.ensure(capacity * MPF_NOUT * nb * chunk) || ddist.ensure(capacity * chunk) || dacc.ensure(chunk)) {
+        din.release(); dout.release(); dtraj.release(); ddist.release(); dacc.release();
+        return fail(c, ERR_NOMEM, "propagate buffers");
+    }
+    std::vector<double> bundle(9 * (uint64_t)nb * chunk), last((uint64_t)MPF_NOUT * chunk);
+    std::vector<unsigned> acc(chunk);
+    const mpf_stop none{MPF_STOP_NONE, 0.0, {0.0, 0.0, 0.0}};
+    int rc = 0;
+    for (uint64_t s0 = 0; s0 < n && !rc; s0 += chunk) {
+        const uint64_t m = std::min(chunk, n - s0);
+        for (uint64_t i = 0; i < m; ++i)
+            mpf::cone::launch_bundle(in9 + 9 * (s0 + i), count, angle, rotation ? rotation[s0 + i] : 0.0, true,
+                                     bundle.data() + 9 * (uint64_t)nb * i);
+        if (hipMemcpy(din.p, bundle.data(), 9 * 8 * (uint64_t)nb * m, hipMemcpyHostToDevice) != hipSuccess) {
+            rc = fail(c, ERR_HIP, "upload");
+            break;
+        }
+        rc = integrate_impl(c, din.p, m * nb, &none, dout.p, nullptr, nullptr, 1, dtraj.p, capacity);
+        if (rc) break;
+        unsigned long long over = 0;
+        if (hipMemcpy(&over, c->counters.p + 2, 8, hipMemcpyDeviceToHost) != hipSuccess) {
+            rc = fail(c, ERR_HIP, "download");
+            break;
+        }
+        if (over) {
+            rc = fail(c, -28, std::to_string(over) + " rays have more than capacity elements");
+            break;
+        }
+        mpf::PropagateArgs P;
+        P.traj = dtraj.p;
+        P.out = dout.p;
+        P.n = m;
+        P.cap = capacity;
+        P.nb = nb;
+        P.kind = interpolation;
+        P.amin = amin;
+        P.length = c->cfg.length;
+        P.angle = angle;
+        P.nacc = dacc.p;
+        P.dist = ddist.p;
+        hipError_t e = mpf::launch_bundle_accept(P, c->stream);
+        if (e == hipSuccess) e = mpf::launch_bundle_distance(P, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        // central rays: trajectory, last element and distances
+        const size_t w = capacity * MPF_NOUT * 8;
+        if (e == hipSuccess)
+            e = hipMemcpy2D(traj + s0 * capacity * MPF_NOUT, w, dtraj.p, w * nb, w, m, hipMemcpyDeviceToHost);
+        if (e == hipSuccess)
+            e = hipMemcpy2D(last.data(), MPF_NOUT * 8, dout.p, (size_t)MPF_NOUT * 8 * nb, MPF_NOUT * 8, m,
+                            hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(dist + s0 * capacity, ddist.p, capacity * 8 * m, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(acc.data(), dacc.p, 4 * m, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) {
+            rc = hipfail(c, e, "propagate");
+            break;
+        }
+        for (uint64_t i = 0; i < m; ++i) {
+            const double s = last[(uint64_t)MPF_NOUT * i + MPF_OUT_NSTEPS];
+            nelem[s0 + i] = (acc[i] && s >= 0.0) ? (uint64_t)s + 1 : 0;
+        }
+    }
+    din.release(); dout.release(); dtraj.release(); ddist.release(); dacc.release();
+    return rc;
 }
 
 int mpf_integrate_trajectory(mpf_ctx* c, const double* in9, uint64_t n, const mpf_stop* stop, uint64_t capacity,

@@ -39,7 +39,7 @@ EXPORTS = ("mpf_create", "mpf_destroy", "mpf_last_error", "mpf_set_precision", "
            "mpf_cone_last_error", "mpf_cones_fullsky", "mpf_cones_narrow", "mpf_narrow_cone_info", "mpf_cone_inside",
            "mpf_cone_select", "mpf_cone_targets", "mpf_hmaps_cone_pixels", "mpf_launch_angles", "mpf_launch_towards",
            "mpf_launch_cone", "mpf_launch_bundle", "mpf_set_launch_config", "mpf_get_launch_config", "mpf_autotune",
-           "mpf_last_kernel_ms")
+           "mpf_last_kernel_ms", "mpf_propagate")
 
 CAT_FIELDS = ("phi", "theta", "phi_obs", "theta_obs", "sep", "iter", "z0", "z1", "z2", "z3", "z4", "z5",
               "a11", "a12", "a21", "a22", "status", "nrounds")
@@ -181,6 +181,7 @@ def load_library(path=LIB_PATH):
     lib.mpf_cosmology_correct.argtypes = [vp, vp, u64, vp, vp, u64]
     lib.mpf_cosmology_prepare.argtypes = [i32, i32, cp, cp, cp, dbl, vp, vp, vp, vp, vp, vp, vp, u64, vp]
     lib.mpf_cosmology_last_error.restype = ctypes.c_char_p
+    lib.mpf_propagate.argtypes = [vp, vp, u64, u32, dbl, vp, i32, dbl, u64, vp, vp, vp]
     lib.mpf_last_kernel_ms.argtypes = [vp, ctypes.POINTER(dbl)]
     lib.mpf_set_launch_config.argtypes = [vp, i32, ctypes.POINTER(LaunchCfg)]
     lib.mpf_get_launch_config.argtypes = [vp, i32, ctypes.POINTER(LaunchCfg)]
@@ -681,6 +682,22 @@ class Context:
         ms = ctypes.c_double(0.0)
         self._check(self.lib.mpf_last_run(self.h, ctypes.byref(steps), ctypes.byref(ms)), "mpf_last_run")
         return steps.value, ms.value
+
+    def propagate(self, in9, count, angle, rotation=None, interpolation="redshift", amin=0.0, capacity=20000):
+        """Integrator::propagate for each central photon: (list of central
+        trajectories, list of their angular diameter distances); an empty
+        pair for a rejected bundle."""
+        in9 = np.ascontiguousarray(in9, dtype=np.float64).reshape(-1, 9)
+        n = in9.shape[0]
+        kind = STOP_NAMES[interpolation] if isinstance(interpolation, str) else int(interpolation)
+        rot = None if rotation is None else np.ascontiguousarray(rotation, dtype=np.float64).ravel()
+        traj = np.zeros((n, capacity, NOUT))
+        dist = np.zeros((n, capacity))
+        ne = np.zeros(n, dtype=np.uint64)
+        self._check(self.lib.mpf_propagate(self.h, _p(in9), ctypes.c_uint64(n), ctypes.c_uint32(count),
+                                           ctypes.c_double(angle), _p(rot), int(kind), ctypes.c_double(amin),
+                                           ctypes.c_uint64(capacity), _p(traj), _p(dist), _p(ne)), "mpf_propagate")
+        return [traj[i, :ne[i]] for i in range(n)], [dist[i, :ne[i]] for i in range(n)]
 
     def last_kernel_ms(self):
         """Duration of the persistent integration kernel alone in the last run."""

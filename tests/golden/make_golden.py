@@ -7,7 +7,8 @@ the deterministic inputs of tests/cases.py.  The fixtures are data only:
 inputs are regenerated by tests/cases.py and checked against the stored
 digest; outputs are the reference's numbers.
 
-    python tests/golden/make_golden.py [tree ... | cones]   (default: keys, cones and every tree)
+    python tests/golden/make_golden.py [tree ... | cones | propagate]
+    (default: keys, cones, propagate and every tree)
 """
 import os
 import sys
@@ -28,6 +29,8 @@ def main(names):
     tab = cases.cosmology()
     if names == ["cones"]:
         return cones(R)
+    if names == ["propagate"]:
+        return propagate(R, tab)
     if names:
         return trees(R, tab, names)
     # keys: compute() at several levels on random and edge points
@@ -45,7 +48,30 @@ def main(names):
                         info_center=info_cen, info_extent=info_ext, info_parent=info_par, interp_x=xs,
                         interp_y=ys, interp_x0=x0, interp_out=R.interpolate(xs, ys, x0))
     cones(R)
+    propagate(R, tab)
     trees(R, tab, cases.TREES)
+
+
+PROP_KINDS = ("redshift", "a", "t", "lambda", "r")
+
+
+def propagate(R, tab):
+    """Integrator::propagate: central trajectories and angular diameter
+    distances of bundles on the tree that reaches past EXTENT / 4."""
+    rec, f64 = cases.tree("wide")
+    h = R.tree(rec)
+    in9 = cases.rays(n_side=3, aperture=30.0)
+    out = dict(digest=np.array(cases.records_digest(rec)), in9=in9)
+    for order in (2, 1):
+        for kind in PROP_KINDS if order == 2 else ("redshift",):
+            for i, r in enumerate(in9):
+                rows, dist = R.propagate(h, order, tab, cases.LENGTH, cases.NSTEPS, r, 6, 0.01, 0.37 * i,
+                                         interp=kind)
+                out[f"o{order}_{kind}_{i}_rows"] = rows
+                out[f"o{order}_{kind}_{i}_dist"] = dist
+    R.free(h)
+    np.savez_compressed(os.path.join(HERE, "propagate.npz"), **out)
+    print("propagate written")
 
 
 def cones(R):
