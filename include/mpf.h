@@ -214,6 +214,22 @@ int mpf_integrate_surfaces(mpf_ctx* ctx, const double* in9, uint64_t n, const mp
 int mpf_integrate_surfaces_dev(mpf_ctx* ctx, const double* dev_in9, uint64_t n, const mpf_surfaces* surf,
                                double* dev_out, double* dev_out_surf, int sync);
 
+/* Lensing::dbetadtheta (lensing.h:140-364), the bundle Jacobian
+ * (jacobiantype = bundle): for each direction i (phi_theta, n x 2) four rays
+ * from observer3 towards the unit direction plus or minus tan(opening) along
+ * its screen vectors e1, e2, integrated on the device to the last of the nd
+ * values of the stop parameter `kind` (MPF_STOP_RADIUS, _LAMBDA, _REDSHIFT,
+ * _T, _A: stop_bundle), each ray's position interpolated at every value, and
+ * the finite differences projected on the screen normal to ki (n x nd x 3,
+ * the reference's kiTargets) over 2 dist tan(opening) (dist: n x nd).  ref:
+ * n x nd values, the same for every direction (up to 16, monotonic as the
+ * surfaces of mpf_integrate_surfaces), or one per direction (nd = 1).  out4:
+ * n x nd x (a11, a12, a21, a22); when a ray ends before a value: every entry
+ * of that direction 42, 42, 1, 0 (for `a`: that value's entry 42, 42, 1, 3
+ * and the others 0), as the reference returns them. */
+int mpf_jacobian_bundle(mpf_ctx* ctx, const double* phi_theta, uint64_t n, const double* observer3, int kind,
+                        const double* ref, uint32_t nd, const double* ki, const double* dist, double opening,
+                        double* out4);
 /* Catalogue root-finding: Catalogues::relCat / iterateNewtonMethod /
  * newtonMethod2d (catalogues.h:410-920) with the infinitesimal Jacobian
  * (jacobiantype = infinitesimal), for a batch of sources on the device. */

@@ -1219,6 +1219,112 @@ int mpf_integrate_surfaces(mpf_ctx* c, const double* in9, uint64_t n, const mpf_
     return 0;
 }
 
+// value of the stop parameter in a per-ray output record (lensing.h:205-301)
+static double bundle_param(const double* o, int kind) {
+    switch (kind) {
+        case MPF_STOP_LAMBDA: return o[MPF_OUT_LAMBDA];
+        case MPF_STOP_RADIUS: return o[MPF_OUT_CHI];
+        case MPF_STOP_A: return o[MPF_OUT_A];
+        case MPF_STOP_T: return o[MPF_OUT_T];
+        default: return o[MPF_OUT_REDSHIFT];
+    }
+}
+
+int mpf_jacobian_bundle(mpf_ctx* c, const double* phi_theta, uint64_t n, const double* observer3, int kind,
+                        const double* ref, uint32_t nd, const double* ki, const double* dist, double opening,
+                        double* out4) {
+    if (!c || !observer3 || (n && (!phi_theta || !ref || !ki || !dist || !out4))) return ERR_INVAL;
+    if (kind != MPF_STOP_RADIUS && kind != MPF_STOP_LAMBDA && kind != MPF_STOP_REDSHIFT && kind != MPF_STOP_T &&
+        kind != MPF_STOP_A)
+        return fail(c, ERR_INVAL, "stop_bundle must be radius, lambda, redshift, t or a");
+    if (nd < 1 || nd > MPF_MAX_SURFACES) return fail(c, ERR_INVAL, "1..16 values per direction");
+    if (n == 0) return 0;
+    // the same values for every direction: surfaces; else one value per direction
+    bool same = true;
+    for (uint64_t i = 1; i < n && same; ++i)
+        for (uint32_t r = 0; r < nd; ++r)
+            if (ref[i * nd + r] != ref[r]) { same = false; break; }
+    if (!same && nd != 1) return fail(c, ERR_INVAL, "values differing between directions need one value each");
+    // four rays per direction at +-tan(opening) along e1, e2 (lensing.h:147-190); sin and cos of one variable
+    // come from one sincos call, as the reference build computes them
+    const double to = std::tan(opening);
+    std::vector<double> in9(36 * n), rv(same ? 0 : 4 * n);
+    for (uint64_t i = 0; i < n; ++i) {
+        double sp, cp, st, ct;
+        ::sincos(phi_theta[2 * i], &sp, &cp);
+        ::sincos(phi_theta[2 * i + 1], &st, &ct);
+        const double t[3] = {cp * st, sp * st, ct};
+        const double e1[3] = {-sp, cp, 0}, e2[3] = {-cp * ct, -sp * ct, st};
+        const double rini[4][3] = {{t[0] + to * e1[0], t[1] + to * e1[1], t[2]},
+                                   {t[0] - to * e1[0], t[1] - to * e1[1], t[2]},
+                                   {t[0] - to * e2[0], t[1] - to * e2[1], t[2] - to * e2[2]},
+                                   {t[0] + to * e2[0], t[1] + to * e2[1], t[2] + to * e2[2]}};
+        for (int k = 0; k < 4; ++k) {
+            mpf::cone::launch_towards(observer3, rini[k], in9.data() + 9 * (4 * i + k));
+            if (!same) rv[4 * i + k] = ref[i];
+        }
+    }
+    mpf_surfaces S{};
+    S.kind = kind;
+    S.n = same ? nd : 1;
+    for (uint32_t r = 0; r < S.n; ++r) S.value[r] = same ? ref[r] : 0.0;
+    S.jacobian = 0;
+    S.ray_value = same ? nullptr : rv.data();
+    std::vector<double> out((uint64_t)MPF_NOUT * 4 * n), surf((uint64_t)MPF_NSURF * S.n * 4 * n);
+    if (int rc = mpf_integrate_surfaces(c, in9.data(), 4 * n, &S, out.data(), surf.data())) return rc;
+    // combine (lensing.h:195-360): failures in the reference's loop order, then finite differences
+    for (uint64_t i = 0; i < n; ++i) {
+        double* J = out4 + 4 * (uint64_t)nd * i;
+        for (uint64_t q = 0; q < 4ull * nd; ++q) J[q] = 0.0;
+        double fp[MPF_MAX_SURFACES][4][3];
+        bool failed = false;
+        for (int it = 0; it < 4 && !failed; ++it) {
+            const uint64_t ray = 4 * i + it;
+            const double back = bundle_param(out.data() + (uint64_t)MPF_NOUT * ray, kind);
+            for (uint32_t r = 0; r < nd && !failed; ++r) {
+                const double v = ref[i * nd + r];
+                const double* sr = surf.data() + ((uint64_t)S.n * ray + (same ? r : 0)) * MPF_NSURF;
+                const bool not_reached = (kind == MPF_STOP_A) ? (back > v) : (back < v);
+                if (not_reached || sr[0] == 0.0) {
+                    failed = true;
+                    if (kind == MPF_STOP_A) {
+                        J[4 * r] = 42; J[4 * r + 1] = 42; J[4 * r + 2] = 1; J[4 * r + 3] = 3;
+                    } else {
+                        for (uint32_t q = 0; q < nd; ++q) {
+                            J[4 * q] = 42; J[4 * q + 1] = 42; J[4 * q + 2] = 1; J[4 * q + 3] = 0;
+                        }
+                    }
+                    break;
+                }
+                fp[r][it][0] = sr[3];
+                fp[r][it][1] = sr[4];
+                fp[r][it][2] = sr[5];
+            }
+        }
+        if (failed) continue;
+        for (uint32_t r = 0; r < nd; ++r) {
+            double rh[3], rvv[3];
+            for (int d = 0; d < 3; ++d) {
+                rh[d] = fp[r][0][d] - fp[r][1][d];
+                rvv[d] = fp[r][2][d] - fp[r][3][d];
+            }
+            const double* k3 = ki + 3 * (i * nd + r);
+            const double phik = std::atan2(k3[1], k3[0]);
+            const double thetak = std::acos(k3[2] / std::sqrt(k3[0] * k3[0] + k3[1] * k3[1] + k3[2] * k3[2]));
+            double spk, cpk, stk, ctk;
+            ::sincos(phik, &spk, &cpk);
+            ::sincos(thetak, &stk, &ctk);
+            const double f1[3] = {-spk, cpk, 0}, f2[3] = {-cpk * ctk, -spk * ctk, stk};
+            const double dd = dist[i * nd + r];
+            J[4 * r + 0] = (rh[0] * f1[0] + rh[1] * f1[1]) / (2 * dd * to);
+            J[4 * r + 1] = (rvv[0] * f1[0] + rvv[1] * f1[1]) / (2 * dd * to);
+            J[4 * r + 2] = (rh[0] * f2[0] + rh[1] * f2[1] + rh[2] * f2[2]) / (-2 * dd * to);
+            J[4 * r + 3] = (rvv[0] * f2[0] + rvv[1] * f2[1] + rvv[2] * f2[2]) / (-2 * dd * to);
+        }
+    }
+    return 0;
+}
+
 int mpf_integrate(mpf_ctx* c, const double* in9, uint64_t n, const mpf_stop* stop, double* out,
                   uint64_t* total_steps) {
     if (!c || !stop || (n && (!in9 || !out))) return ERR_INVAL;

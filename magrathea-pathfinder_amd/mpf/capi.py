@@ -39,7 +39,7 @@ EXPORTS = ("mpf_create", "mpf_destroy", "mpf_last_error", "mpf_set_precision", "
            "mpf_cone_last_error", "mpf_cones_fullsky", "mpf_cones_narrow", "mpf_narrow_cone_info", "mpf_cone_inside",
            "mpf_cone_select", "mpf_cone_targets", "mpf_hmaps_cone_pixels", "mpf_launch_angles", "mpf_launch_towards",
            "mpf_launch_cone", "mpf_launch_bundle", "mpf_set_launch_config", "mpf_get_launch_config", "mpf_autotune",
-           "mpf_last_kernel_ms", "mpf_propagate")
+           "mpf_last_kernel_ms", "mpf_propagate", "mpf_jacobian_bundle")
 
 CAT_FIELDS = ("phi", "theta", "phi_obs", "theta_obs", "sep", "iter", "z0", "z1", "z2", "z3", "z4", "z5",
               "a11", "a12", "a21", "a22", "status", "nrounds")
@@ -181,6 +181,7 @@ def load_library(path=LIB_PATH):
     lib.mpf_cosmology_correct.argtypes = [vp, vp, u64, vp, vp, u64]
     lib.mpf_cosmology_prepare.argtypes = [i32, i32, cp, cp, cp, dbl, vp, vp, vp, vp, vp, vp, vp, u64, vp]
     lib.mpf_cosmology_last_error.restype = ctypes.c_char_p
+    lib.mpf_jacobian_bundle.argtypes = [vp, vp, u64, vp, i32, vp, u32, vp, vp, dbl, vp]
     lib.mpf_propagate.argtypes = [vp, vp, u64, u32, dbl, vp, i32, dbl, u64, vp, vp, vp]
     lib.mpf_last_kernel_ms.argtypes = [vp, ctypes.POINTER(dbl)]
     lib.mpf_set_launch_config.argtypes = [vp, i32, ctypes.POINTER(LaunchCfg)]
@@ -698,6 +699,25 @@ class Context:
                                            ctypes.c_double(angle), _p(rot), int(kind), ctypes.c_double(amin),
                                            ctypes.c_uint64(capacity), _p(traj), _p(dist), _p(ne)), "mpf_propagate")
         return [traj[i, :ne[i]] for i in range(n)], [dist[i, :ne[i]] for i in range(n)]
+
+    def jacobian_bundle(self, phi_theta, kind, ref, ki, dist, opening, observer=(0.0, 0.0, 0.0)):
+        """Lensing::dbetadtheta per direction: (n, nd, 4) = a11, a12, a21, a22.
+        ref: (nd,) values shared by every direction or (n, nd); ki (n, nd, 3);
+        dist (n, nd)."""
+        pt = np.ascontiguousarray(phi_theta, dtype=np.float64).reshape(-1, 2)
+        n = pt.shape[0]
+        ref = np.asarray(ref, dtype=np.float64)
+        ref = np.ascontiguousarray(np.broadcast_to(ref, (n, ref.shape[-1])) if ref.ndim == 1 else ref)
+        nd = ref.shape[1]
+        ki = np.ascontiguousarray(ki, dtype=np.float64).reshape(n, nd, 3)
+        dist = np.ascontiguousarray(dist, dtype=np.float64).reshape(n, nd)
+        k = STOP_NAMES[kind] if isinstance(kind, str) else int(kind)
+        out = np.zeros((n, nd, 4))
+        self._check(self.lib.mpf_jacobian_bundle(self.h, _p(pt), ctypes.c_uint64(n),
+                                                 _p(np.ascontiguousarray(observer, dtype=np.float64)), int(k),
+                                                 _p(ref), ctypes.c_uint32(nd), _p(ki), _p(dist),
+                                                 ctypes.c_double(opening), _p(out)), "mpf_jacobian_bundle")
+        return out
 
     def last_kernel_ms(self):
         """Duration of the persistent integration kernel alone in the last run."""

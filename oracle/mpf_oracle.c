@@ -1282,3 +1282,101 @@ uint64_t mpfo_propagate(const mpfo_octree* o, int order, int precision, const do
     free(init);
     return size;
 }
+
+/* -------------------------------------------------------------- bundle Jacobian */
+
+/* value of the stop parameter of element p (lensing.h:205-301) */
+static double bundle_param(const photon* p, int kind) {
+    switch (kind) {
+        case 1: return p->lambda;
+        case 0: return p->chi;
+        case 4: return p->core[CA];
+        case 3: return p->core[CT];
+        default: return p->redshift;
+    }
+}
+
+/* Lensing::dbetadtheta (lensing.h:140-364) for one direction: four rays at
+ * +-tan(opening) along the screen vectors e1, e2 of (phi, theta), integrated
+ * to the last value with the stop criterion `kind` (0 radius, 1 lambda,
+ * 2 redshift, 3 t, 4 a), each interpolated at every value (upper_bound on the
+ * stored trajectory, f = (next - ref) / (next - previous)), then finite
+ * differences projected on the screen of each ki.  out4: nd x (a11, a12,
+ * a21, a22); on a ray that ends before a value: every entry 42, 42, 1, 0
+ * (for `a`: that entry 42, 42, 1, 3 and the others 0). */
+void mpfo_dbetadtheta(const mpfo_octree* o, int order, int precision, const double* cosmo_tab, uint64_t ncos,
+                      double length, unsigned nsteps, int kind, double opening, const double* observer, double phi,
+                      double theta, const double* ref, const double* ki, const double* dist, uint64_t nd,
+                      double* out4) {
+    double cp, sp, ct, st;
+    sincos(phi, &sp, &cp);
+    sincos(theta, &st, &ct);
+    const double target[3] = {cp * st, sp * st, ct};
+    const double e1[3] = {-sp, cp, 0}, e2[3] = {-cp * ct, -sp * ct, st};
+    const double to = tan(opening);
+    double rini[4][3];
+    rini[0][0] = target[0] + to * e1[0]; rini[0][1] = target[1] + to * e1[1]; rini[0][2] = target[2];
+    rini[1][0] = target[0] - to * e1[0]; rini[1][1] = target[1] - to * e1[1]; rini[1][2] = target[2];
+    rini[2][0] = target[0] - to * e2[0]; rini[2][1] = target[1] - to * e2[1]; rini[2][2] = target[2] - to * e2[2];
+    rini[3][0] = target[0] + to * e2[0]; rini[3][1] = target[1] + to * e2[1]; rini[3][2] = target[2] + to * e2[2];
+    double* fp = (double*)calloc(nd * 12, sizeof(double));
+    for (uint64_t k = 0; k < 4 * nd; ++k) out4[k] = 0.0;
+    for (int it = 0; it < 4; ++it) {
+        /* Integrator::launch from the observer towards rini (integrator.h:264-290) */
+        const double dx = rini[it][0] - observer[0], dy = rini[it][1] - observer[1], dz = rini[it][2] - observer[2];
+        const double norm = sqrt(dx * dx + dy * dy + dz * dz);
+        const double in9[9] = {1.0, 0.0, observer[0], observer[1], observer[2], 1.0, dx / norm, dy / norm, dz / norm};
+        traj_t t = {NULL, 0, 0, 1};
+        run_full(o, order, precision, cosmo_tab, ncos, length, nsteps, kind, ref[nd - 1], NULL, in9, &t);
+        for (uint64_t r = 0; r < nd; ++r) {
+            const double back = t.n ? bundle_param(&t.p[t.n - 1], kind) : 0.0;
+            const int fail = (kind == 4) ? (back > ref[r]) : (back < ref[r]);
+            if (fail) {
+                if (kind == 4) {
+                    out4[4 * r] = 42; out4[4 * r + 1] = 42; out4[4 * r + 2] = 1; out4[4 * r + 3] = 3;
+                } else {
+                    for (uint64_t q = 0; q < nd; ++q) {
+                        out4[4 * q] = 42; out4[4 * q + 1] = 42; out4[4 * q + 2] = 1; out4[4 * q + 3] = 0;
+                    }
+                }
+                free(t.p);
+                free(fp);
+                return;
+            }
+            /* upper_bound with the criterion's order (decreasing for a) */
+            uint64_t first = 0, len = t.n;
+            while (len > 0) {
+                const uint64_t half = len >> 1;
+                const double v = bundle_param(&t.p[first + half], kind);
+                const int before = (kind == 4) ? (ref[r] > v) : (ref[r] < v);
+                if (before) len = half;
+                else { first = first + half + 1; len = len - half - 1; }
+            }
+            const uint64_t id = first - (first > 0);
+            const double next = bundle_param(&t.p[id + 1], kind), prev = bundle_param(&t.p[id], kind);
+            const double f = (next - ref[r]) / (next - prev);
+            for (int d = 0; d < 3; ++d)
+                fp[12 * r + 3 * it + d] = t.p[id].core[CX + d] * f + t.p[id + 1].core[CX + d] * (1 - f);
+        }
+        free(t.p);
+    }
+    for (uint64_t r = 0; r < nd; ++r) {
+        double rh[3], rv[3];
+        for (int d = 0; d < 3; ++d) {
+            rh[d] = fp[12 * r + d] - fp[12 * r + 3 + d];
+            rv[d] = fp[12 * r + 6 + d] - fp[12 * r + 9 + d];
+        }
+        const double kx = ki[3 * r], ky = ki[3 * r + 1], kz = ki[3 * r + 2];
+        const double phik = atan2(ky, kx);
+        const double thetak = acos(kz / sqrt(kx * kx + ky * ky + kz * kz));
+        double cpk, spk, ctk, stk;
+        sincos(phik, &spk, &cpk);
+        sincos(thetak, &stk, &ctk);
+        const double f1[3] = {-spk, cpk, 0}, f2[3] = {-cpk * ctk, -spk * ctk, stk};
+        out4[4 * r + 0] = (rh[0] * f1[0] + rh[1] * f1[1]) / (2 * dist[r] * to);
+        out4[4 * r + 1] = (rv[0] * f1[0] + rv[1] * f1[1]) / (2 * dist[r] * to);
+        out4[4 * r + 2] = (rh[0] * f2[0] + rh[1] * f2[1] + rh[2] * f2[2]) / (-2 * dist[r] * to);
+        out4[4 * r + 3] = (rv[0] * f2[0] + rv[1] * f2[1] + rv[2] * f2[2]) / (-2 * dist[r] * to);
+    }
+    free(fp);
+}

@@ -75,6 +75,12 @@ void mpfo_launch_bundle(const double* in9, unsigned count, double angle, double 
 uint64_t mpfo_propagate(const mpfo_octree* o, int order, int precision, const double* cosmo_tab, uint64_t ncos,
                         double length, unsigned nsteps, const double* in9, unsigned count, double angle,
                         double rotation, int kind, double amin, double* rows, double* dist, uint64_t maxlen);
+/* Lensing::dbetadtheta (lensing.h:140-364), the bundle Jacobian of one
+ * direction at nd values of the stop parameter `kind`. */
+void mpfo_dbetadtheta(const mpfo_octree* o, int order, int precision, const double* cosmo_tab, uint64_t ncos,
+                      double length, unsigned nsteps, int kind, double opening, const double* observer, double phi,
+                      double theta, const double* ref, const double* ki, const double* dist, uint64_t nd,
+                      double* out4);
 void mpfo_interpolate(const double* x, const double* y, uint64_t n, const double* x0, uint64_t m, double* out);
 
 #ifdef __cplusplus

@@ -137,6 +137,21 @@ class Oracle(_Base):
                                     ctypes.c_double(amin), _p(rows), _p(dist), _u64(maxlen))
         return rows[:min(n, maxlen)], dist[:min(n, maxlen)]
 
+    def dbetadtheta(self, h, order, table, length, nsteps, kind, opening, observer, phi, theta, ref, ki, dist,
+                    precision=0):
+        """Lensing::dbetadtheta: nd x (a11, a12, a21, a22)."""
+        tab = np.ascontiguousarray(table, dtype=np.float64)
+        ref = np.ascontiguousarray(ref, dtype=np.float64).ravel()
+        ki = np.ascontiguousarray(ki, dtype=np.float64).reshape(-1, 3)
+        dist = np.ascontiguousarray(dist, dtype=np.float64).ravel()
+        obs = np.ascontiguousarray(observer, dtype=np.float64)
+        out = np.zeros((ref.size, 4))
+        self.lib.mpfo_dbetadtheta(_vp(h), int(order), int(precision), _p(tab), _u64(tab.shape[1]),
+                                  ctypes.c_double(length), ctypes.c_uint(nsteps), int(kind), ctypes.c_double(opening),
+                                  _p(obs), ctypes.c_double(phi), ctypes.c_double(theta), _p(ref), _p(ki), _p(dist),
+                                  _u64(ref.size), _p(out))
+        return out
+
     def launch_bundle(self, in9, count, angle, rotation):
         in9 = np.ascontiguousarray(in9, dtype=np.float64).reshape(9)
         out = np.zeros((count + 1, 9))
@@ -234,6 +249,19 @@ class RefHarness(_Base):
                                    ctypes.c_double(rotation), interp.encode(), ctypes.c_double(amin), _p(rows),
                                    _p(dist), _u64(maxlen))
         return rows[:min(n, maxlen)], dist[:min(n, maxlen)]
+
+    def dbetadtheta(self, h, order, table, length, nsteps, stop, opening, observer, phi, theta, ref, ki, dist):
+        tab = np.ascontiguousarray(table, dtype=np.float64)
+        ref = np.ascontiguousarray(ref, dtype=np.float64).ravel()
+        ki = np.ascontiguousarray(ki, dtype=np.float64).reshape(-1, 3)
+        dist = np.ascontiguousarray(dist, dtype=np.float64).ravel()
+        obs = np.ascontiguousarray(observer, dtype=np.float64)
+        out = np.zeros((ref.size, 4))
+        self.lib.ref_dbetadtheta(_vp(h), int(order), _p(tab), _u64(tab.shape[1]), ctypes.c_double(length),
+                                 ctypes.c_uint(nsteps), stop.encode(), ctypes.c_double(opening), _p(obs),
+                                 ctypes.c_double(phi), ctypes.c_double(theta), _p(ref), _p(ki), _p(dist),
+                                 _u64(ref.size), _p(out))
+        return out
 
     def jacobian(self, h, order, table, length, nsteps, stop_kind, stop_value, in9, dist, kitarget=None):
         tab = np.ascontiguousarray(table, dtype=np.float64)

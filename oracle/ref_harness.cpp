@@ -366,6 +366,35 @@ uint64_t ref_propagate(void* h, int order, const double* cosmo_tab, uint64_t nco
     return r.size();
 }
 
+/* Lensing::dbetadtheta (lensing.h:140-364) with the parameters it reads
+ * (stop_bundle, openingmin, nsteps). */
+struct BundleParams {
+    std::string stop_bundle;
+    double openingmin;
+    unsigned nsteps;
+};
+
+void ref_dbetadtheta(void* h, int order, const double* cosmo_tab, uint64_t ncos, double length, unsigned nsteps,
+                     const char* stop, double opening, const double* observer, double phi, double theta,
+                     const double* ref, const double* ki, const double* dist, uint64_t nd, double* out4) {
+    const Octree& oc = static_cast<RefOctree*>(h)->octree;
+    const Cosmo cosmo = make_cosmo(cosmo_tab, ncos);
+    BundleParams P{std::string(stop), opening, nsteps};
+    std::vector<Point> kis(nd);
+    std::vector<double> refs(ref, ref + nd), dists(dist, dist + nd);
+    for (uint64_t r = 0; r < nd; ++r) kis[r] = Point{{ki[3 * r], ki[3 * r + 1], ki[3 * r + 2]}};
+    const Point obs{{observer[0], observer[1], observer[2]}};
+    const Point vobs{{0., 0., 0.}};
+    std::vector<std::array<std::array<double, 2>, 2>> j;
+    switch (order) {
+        case 1: j = Lensing::dbetadtheta<1>(P, kis, refs, obs, phi, theta, dists, cosmo, oc, vobs, length); break;
+        default: j = Lensing::dbetadtheta<2>(P, kis, refs, obs, phi, theta, dists, cosmo, oc, vobs, length); break;
+    }
+    for (uint64_t r = 0; r < nd; ++r) {
+        out4[4 * r] = j[r][0][0]; out4[4 * r + 1] = j[r][0][1]; out4[4 * r + 2] = j[r][1][0]; out4[4 * r + 3] = j[r][1][1];
+    }
+}
+
 /* Utility::invMatrix3d, row-major; Constants<double>::deg() and pi(). */
 void ref_inv3(const double* a, double* r) {
     std::array<std::array<double, 3>, 3> m;

@@ -7,8 +7,8 @@ the deterministic inputs of tests/cases.py.  The fixtures are data only:
 inputs are regenerated by tests/cases.py and checked against the stored
 digest; outputs are the reference's numbers.
 
-    python tests/golden/make_golden.py [tree ... | cones | propagate]
-    (default: keys, cones, propagate and every tree)
+    python tests/golden/make_golden.py [tree ... | cones | propagate | bundle]
+    (default: keys, cones, propagate, bundle and every tree)
 """
 import os
 import sys
@@ -31,6 +31,8 @@ def main(names):
         return cones(R)
     if names == ["propagate"]:
         return propagate(R, tab)
+    if names == ["bundle"]:
+        return bundle(R, tab)
     if names:
         return trees(R, tab, names)
     # keys: compute() at several levels on random and edge points
@@ -49,10 +51,39 @@ def main(names):
                         interp_y=ys, interp_x0=x0, interp_out=R.interpolate(xs, ys, x0))
     cones(R)
     propagate(R, tab)
+    bundle(R, tab)
     trees(R, tab, cases.TREES)
 
 
 PROP_KINDS = ("redshift", "a", "t", "lambda", "r")
+
+
+BUNDLE_VALUES = {"r": [0.15, 0.3, 0.42], "lambda": [2e16, 5e16], "redshift": [0.1, 0.25, 0.33],
+                 "t": [4e16, 1e17], "a": [0.95, 0.85, 0.78]}
+
+
+def bundle(R, tab):
+    """Lensing::dbetadtheta (the bundle Jacobian) for directions around the
+    test cone, every stop parameter, on a tree where some rays end early
+    (failure entries) and on one where every ray reaches the values."""
+    rng = np.random.default_rng(21)
+    out = {}
+    for name in ("amr", "wide"):
+        rec, f64 = cases.tree(name)
+        h = R.tree(rec)
+        out[f"{name}_digest"] = np.array(cases.records_digest(rec))
+        pt = np.c_[np.radians(20) + rng.uniform(-0.2, 0.2, 10), np.radians(20) + rng.uniform(-0.2, 0.2, 10)]
+        out[f"{name}_phi_theta"] = pt
+        for kind, vals in BUNDLE_VALUES.items():
+            ki = rng.normal(size=(10, len(vals), 3))
+            dist = rng.uniform(0.1, 0.4, (10, len(vals)))
+            out[f"{name}_{kind}_ki"], out[f"{name}_{kind}_dist"] = ki, dist
+            out[f"{name}_{kind}_jac"] = np.stack([
+                R.dbetadtheta(h, 2, tab, cases.LENGTH, cases.NSTEPS, kind, 0.005, [0.0, 0.0, 0.0], p, t, vals,
+                              ki[i], dist[i]) for i, (p, t) in enumerate(pt)])
+        R.free(h)
+    np.savez_compressed(os.path.join(HERE, "bundle.npz"), **out)
+    print("bundle written")
 
 
 def propagate(R, tab):

@@ -1,0 +1,80 @@
+"""Lensing::dbetadtheta (the bundle Jacobian, lensing.h:140-364) on the GPU
+(mpf_jacobian_bundle) against fixtures written by the reference's own code
+and against the oracle's restatement (pinned to it in
+tests/test_bundle_jacobian.py): bit-identical entries, failure entries
+included, in the exact tiers, with values shared by every direction and with
+one value per direction; the split tier against the FP64 tier within
+1e-9 + 1e-5 |A - I|."""
+import os
+
+import numpy as np
+import pytest
+
+import cases
+from mpf import capi
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+KIND = {"r": 0, "lambda": 1, "redshift": 2, "t": 3, "a": 4}
+VALUES = {"r": [0.15, 0.3, 0.42], "lambda": [2e16, 5e16], "redshift": [0.1, 0.25, 0.33], "t": [4e16, 1e17],
+          "a": [0.95, 0.85, 0.78]}
+
+
+def ctx(name, precision, order=capi.TSC):
+    rec, f64 = cases.tree(name)
+    c = capi.Context(cases.LENGTH, order=order, precision=precision, nsteps=cases.NSTEPS)
+    c.load_records(rec)
+    c.load_fields64(f64)
+    c.set_cosmology(cases.cosmology())
+    return c, rec, f64
+
+
+def test_bundle_jacobian_matches_golden():
+    g = np.load(os.path.join(GOLDEN, "bundle.npz"))
+    for name in ("amr", "wide"):
+        c, rec, _ = ctx(name, capi.PREC_REF)
+        for kind, vals in VALUES.items():
+            print(f"[case] bundle jacobian fixtures tree={name} kind={kind}", flush=True)
+            a = c.jacobian_bundle(g[f"{name}_phi_theta"], kind, vals, g[f"{name}_{kind}_ki"],
+                                  g[f"{name}_{kind}_dist"], 0.005)
+            np.testing.assert_array_equal(a, g[f"{name}_{kind}_jac"], err_msg=f"{name} {kind}")
+        c.close()
+
+
+@pytest.mark.parametrize("precision", (capi.PREC_REF, capi.PREC_FP64))
+def test_bundle_jacobian_per_direction_matches_oracle(precision):
+    c, rec, f64 = ctx("wide", precision)
+    O = Oracle()
+    h = O.tree(rec, f64)
+    rng = np.random.default_rng(8)
+    n = 40
+    pt = np.c_[np.radians(20) + rng.uniform(-0.3, 0.3, n), np.radians(20) + rng.uniform(-0.3, 0.3, n)]
+    for kind, lo, hi in (("redshift", 0.05, 0.4), ("r", 0.1, 0.6), ("a", 0.75, 0.97)):
+        print(f"[case] bundle jacobian per direction precision={precision} kind={kind}", flush=True)
+        ref = rng.uniform(lo, hi, (n, 1))
+        ki = rng.normal(size=(n, 1, 3))
+        dist = rng.uniform(0.1, 0.5, (n, 1))
+        a = c.jacobian_bundle(pt, kind, ref, ki, dist, 0.003)
+        for i in range(n):
+            b = O.dbetadtheta(h, 2, cases.cosmology(), cases.LENGTH, cases.NSTEPS, KIND[kind], 0.003,
+                              [0.0, 0.0, 0.0], pt[i, 0], pt[i, 1], ref[i], ki[i], dist[i], precision=precision)
+            np.testing.assert_array_equal(a[i], b, err_msg=f"{kind} direction {i}")
+    O.free(h)
+    c.close()
+
+
+def test_bundle_jacobian_split_vs_fp64():
+    print("[case] bundle jacobian split vs fp64", flush=True)
+    g = np.load(os.path.join(GOLDEN, "bundle.npz"))
+    cs, _, _ = ctx("wide", capi.PREC_SPLIT)
+    cf, _, _ = ctx("wide", capi.PREC_FP64)
+    for kind, vals in VALUES.items():
+        pt, ki, dist = g["wide_phi_theta"], g[f"wide_{kind}_ki"], g[f"wide_{kind}_dist"]
+        a = cs.jacobian_bundle(pt, kind, vals, ki, dist, 0.005)
+        b = cf.jacobian_bundle(pt, kind, vals, ki, dist, 0.005)
+        ident = np.array([1.0, 0.0, 0.0, 1.0])
+        tol = 1e-9 + 1e-5 * np.abs(b - ident)
+        assert np.all(np.abs(a - b) <= tol), f"{kind}: max excess {np.max(np.abs(a - b) - tol):.3e}"
+    cs.close()
+    cf.close()
