@@ -172,6 +172,29 @@ int mpf_integrate_dev(mpf_ctx* ctx, const double* dev_in9, uint64_t n, const mpf
  * -ENOSPC when a ray has more than capacity elements. */
 int mpf_propagate(mpf_ctx* ctx, const double* in9, uint64_t n, uint32_t count, double angle, const double* rotation,
                   int interpolation, double amin, uint64_t capacity, double* traj, double* dist, uint64_t* nelem);
+/* Statistics of the rays program (rays.cpp:300-456) over the central
+ * trajectories of mpf_propagate (rows, dist, nelem, capacity as it writes
+ * them; rejected bundles, nelem 0, are skipped), host only.  Each trajectory
+ * gives a curve of an abscissa (interp: MPF_STOP_REDSHIFT redshift,
+ * MPF_STOP_A t and MPF_STOP_T 1/a - 1 (the reference's mapping of `a` and
+ * `t`), MPF_STOP_RADIUS distance from the first element, MPF_STOP_LAMBDA)
+ * against a statistic (MPF_STAT_*, distances in units of `length`),
+ * re-interpolated at the abscissae of every statmod-th element of the
+ * homogeneous reference ray (ref_rows m x MPF_NOUT, ref_dist m; its elements
+ * with a < amin dropped when amin is normal; statmod = max(1, m' / nstat)).
+ * out4: per abscissa (redshift of the reference element, its statistic, the
+ * mean over rays, their standard deviation); nout receives their count (out4
+ * may be NULL to size it), nused the number of rays used.  One process's
+ * numbers, which are the reference's rank output and, with one rank, its
+ * global output. */
+enum {
+    MPF_STAT_DISTANCE = 0, MPF_STAT_DISTANCE2 = 1, MPF_STAT_HOMOGENEOUS = 2, MPF_STAT_INHOMOGENEOUS = 3,
+    MPF_STAT_INVDISTANCE = 4, MPF_STAT_INVDISTANCE2 = 5
+};
+int mpf_ray_statistics(const double* rows, const double* dist, const uint64_t* nelem, uint64_t n, uint64_t capacity,
+                       const double* ref_rows, const double* ref_dist, uint64_t m, int interp, int stat,
+                       uint32_t nstat, double amin, double length, double* out4, uint64_t cap_out, uint64_t* nout,
+                       uint64_t* nused);
 /* Whole trajectories (the reference's Evolution<Photon>, integrator.h:605,
  * 776, after the final pop), exact tiers: traj receives up to `capacity`
  * elements per ray, element k of ray i at traj[(i*capacity + k)*MPF_NOUT]

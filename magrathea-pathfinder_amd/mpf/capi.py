@@ -39,7 +39,8 @@ EXPORTS = ("mpf_create", "mpf_destroy", "mpf_last_error", "mpf_set_precision", "
            "mpf_cone_last_error", "mpf_cones_fullsky", "mpf_cones_narrow", "mpf_narrow_cone_info", "mpf_cone_inside",
            "mpf_cone_select", "mpf_cone_targets", "mpf_hmaps_cone_pixels", "mpf_launch_angles", "mpf_launch_towards",
            "mpf_launch_cone", "mpf_launch_bundle", "mpf_set_launch_config", "mpf_get_launch_config", "mpf_autotune",
-           "mpf_last_kernel_ms", "mpf_propagate", "mpf_jacobian_bundle")
+           "mpf_last_kernel_ms", "mpf_propagate", "mpf_jacobian_bundle",
+           "mpf_ray_statistics")
 
 CAT_FIELDS = ("phi", "theta", "phi_obs", "theta_obs", "sep", "iter", "z0", "z1", "z2", "z3", "z4", "z5",
               "a11", "a12", "a21", "a22", "status", "nrounds")
@@ -181,6 +182,7 @@ def load_library(path=LIB_PATH):
     lib.mpf_cosmology_correct.argtypes = [vp, vp, u64, vp, vp, u64]
     lib.mpf_cosmology_prepare.argtypes = [i32, i32, cp, cp, cp, dbl, vp, vp, vp, vp, vp, vp, vp, u64, vp]
     lib.mpf_cosmology_last_error.restype = ctypes.c_char_p
+    lib.mpf_ray_statistics.argtypes = [vp, vp, vp, u64, u64, vp, vp, u64, i32, i32, u32, dbl, dbl, vp, u64, vp, vp]
     lib.mpf_jacobian_bundle.argtypes = [vp, vp, u64, vp, i32, vp, u32, vp, vp, dbl, vp]
     lib.mpf_propagate.argtypes = [vp, vp, u64, u32, dbl, vp, i32, dbl, u64, vp, vp, vp]
     lib.mpf_last_kernel_ms.argtypes = [vp, ctypes.POINTER(dbl)]
@@ -310,6 +312,39 @@ def hmaps_cone_pixels(nside, cones7, icone):
     _cone_check(load_library().mpf_hmaps_cone_pixels(nside, _p(cones7), cones7.shape[0], icone, _p(pix), k.value,
                                              ctypes.byref(k)), "mpf_hmaps_cone_pixels")
     return pix[:k.value]
+
+
+STAT_NAMES = {"distance": 0, "distance2": 1, "homogeneous": 2, "inhomogeneous": 3, "invdistance": 4,
+              "invdistance2": 5}
+
+
+def ray_statistics(trajs, dists, ref_rows, ref_dist, interp="redshift", stat="distance", nstat=100, amin=0.0,
+                   length=1.0):
+    """rays.cpp statistics over propagate's central trajectories: (rows of
+    (z, reference statistic, mean, std), rays used)."""
+    n = len(trajs)
+    cap = max([len(t) for t in trajs] + [1])
+    rows = np.zeros((n, cap, NOUT))
+    dist = np.zeros((n, cap))
+    ne = np.zeros(n, dtype=np.uint64)
+    for i, (t, d) in enumerate(zip(trajs, dists)):
+        rows[i, :len(t)] = t
+        dist[i, :len(t)] = d
+        ne[i] = len(t)
+    ref_rows = np.ascontiguousarray(ref_rows, dtype=np.float64).reshape(-1, NOUT)
+    ref_dist = np.ascontiguousarray(ref_dist, dtype=np.float64).ravel()
+    ik = STOP_NAMES[interp] if isinstance(interp, str) else int(interp)
+    sk = STAT_NAMES[stat] if isinstance(stat, str) else int(stat)
+    nout, nused = ctypes.c_uint64(0), ctypes.c_uint64(0)
+    args = (_p(rows), _p(dist), _p(ne), n, cap, _p(ref_rows), _p(ref_dist), ref_rows.shape[0], ik, sk, nstat, amin,
+            length)
+    lib = load_library()
+    if lib.mpf_ray_statistics(*args, None, 0, ctypes.byref(nout), ctypes.byref(nused)):
+        raise MpfError("mpf_ray_statistics: invalid arguments")
+    out = np.zeros((max(1, nout.value), 4))
+    if lib.mpf_ray_statistics(*args, _p(out), nout.value, ctypes.byref(nout), ctypes.byref(nused)):
+        raise MpfError("mpf_ray_statistics: invalid arguments")
+    return out[:nout.value], nused.value
 
 
 def launch_angles(phi_theta, observer=(0.0, 0.0, 0.0)):
