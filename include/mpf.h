@@ -438,6 +438,50 @@ int mpf_hmaps_launch_dev(mpf_ctx* ctx, int64_t nside, const int64_t* dev_pix, ui
  * not in dev_pix are left untouched. */
 int mpf_hmaps_fill_dev(mpf_ctx* ctx, int64_t nside, const int64_t* dev_pix, const double* dev_surf, uint64_t n,
                        uint32_t nsurf, double* dev_maps);
+/* Map components of hmaps (map_components, hmaps.cpp:187-268): codes of
+ * Hmaps::FillMap's index_components.  lensing: kappa, gamma1, gamma2, 1/mu
+ * (infinitesimal Jacobian) or kappa, gamma1, gamma2, w, 1/mu (bundle); dr,
+ * dl, dt, da, dz, ds: the surface's chi, lambda, t, a, redshift, s over the
+ * homogeneous value minus 1 (ds over chi); isw, dens, dens_max, steps, phi:
+ * the interpolated value, the largest rho before the surface, the element
+ * index; deflection: phi - beta1, theta - beta2 of the pixel centre and the
+ * ray's position at the surface.  lensing_born is not provided. */
+enum {
+    MPF_MAP_LENSING = 0, MPF_MAP_LENSING_BORN = 1, MPF_MAP_DR = 2, MPF_MAP_DL = 3, MPF_MAP_DT = 4, MPF_MAP_DA = 5,
+    MPF_MAP_DZ = 6, MPF_MAP_DS = 7, MPF_MAP_ISW = 8, MPF_MAP_DENS = 9, MPF_MAP_DENS_MAX = 10, MPF_MAP_STEPS = 11,
+    MPF_MAP_PHI = 12, MPF_MAP_DEFLECTION = 13
+};
+/* The homogeneous quantities of hmaps.cpp:297-325 at each map redshift
+ * z_stop[iz]: along the homogeneous reference ray (ref_rows, m x MPF_OUT
+ * records of mpf_integrate_trajectory with MPF_HOMOGENEOUS), the element
+ * before the redshift (upper_bound) and the interpolation factor give t, chi
+ * and lambda; a = 1 / (1 + z); homo5: nz x (t, chi, lambda, redshift, a).
+ * interp_ref (may be NULL): the surface values of stop_ray (MPF_STOP_T
+ * thomo, _RADIUS rhomo, _LAMBDA lambdahomo, _A ahomo, _REDSHIFT z, _S
+ * rhomo).  Host only. */
+int mpf_hmaps_homogeneous(const double* ref_rows, uint64_t m, const double* z_stop, uint32_t nz, int stop_ray,
+                          double* homo5, double* interp_ref);
+/* Every map component of Hmaps::FillMap (hmaps.h:716-836) for rays launched
+ * at pixel centres (mpf_hmaps_launch_dev) and integrated with nz surfaces
+ * (dev_surf, n x nz x MPF_NSURF): dev_maps[(nmaps * iz + k) * npix + pix] in
+ * FP32, the layout of the reference's map vector; pixels of rejected
+ * surfaces (or of failed bundle Jacobians) are 0 in every map of that
+ * surface; pixels not in dev_pix are untouched.  dev_jac: n x nz x 4 bundle
+ * Jacobians (mpf_jacobian_bundle, copied to the device), or NULL for the
+ * infinitesimal one of the surface records.  homo5: host, nz x 5 of
+ * mpf_hmaps_homogeneous.  nmaps receives the maps per surface; -EINVAL,
+ * writing nothing, when nz x nmaps x 12 nside^2 exceeds maps_capacity
+ * (floats; dev_maps may then be NULL, to size the buffer). */
+int mpf_hmaps_components_dev(mpf_ctx* ctx, int64_t nside, const int64_t* dev_pix, const double* dev_surf,
+                             const double* dev_jac, uint64_t n, uint32_t nz, const int* components, uint32_t ncomp,
+                             const double* homo5, float* dev_maps, uint64_t maps_capacity, uint32_t* nmaps);
+/* Write one RING (nest = 0) or NESTED map as a HEALPix FITS file: a binary
+ * table of one FP32 column with the PIXTYPE, ORDERING, NSIDE, FIRSTPIX,
+ * LASTPIX and COORDSYS keywords (write_healpix_map of chealpix, called at
+ * hmaps.cpp:510).  clean != 0 first replaces every value that is not normal
+ * (zero, subnormal, infinite, NaN) by Healpix_undef (-1.6375e30), as
+ * hmaps.cpp:494-499 does before writing.  Host only; the file is replaced. */
+int mpf_healpix_write_fits(const char* path, const float* map, int64_t nside, int nest, char coordsys, int clean);
 /* Bin rays by the pixel of their observed direction (dev_in9 dx/dl, dy/dl,
  * dz/dl): dev_maps[k*npix + pix], k = mean kappa, mean gamma1, mean gamma2,
  * ray count, from surface `isurf` of dev_surf (n x nsurf x MPF_NSURF).

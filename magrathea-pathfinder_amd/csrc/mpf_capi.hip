@@ -103,6 +103,7 @@ struct mpf_ctx {
     // surfaces, [1] with surfaces; the ray order they read
     mpf_launch_cfg launch[2] = {{1, 0, 0, 0}, {1, 0, 0, 0}};
     DevBuf<uint32_t> order;
+    DevBuf<double> homo;   // homogeneous values of the map surfaces
     DevBuf<unsigned char> order_work;
 };
 
@@ -169,7 +170,7 @@ int mpf_destroy(mpf_ctx* c) {
     c->base.release(); c->child_first.release(); c->elem.release(); c->child_mask.release();
     c->f32.release(); c->f64.release(); c->cosmo.release(); c->counters.release();
     c->io_in.release(); c->io_out.release(); c->io_surf.release(); c->io_rv.release(); c->scratch.release();
-    c->order.release(); c->order_work.release();
+    c->order.release(); c->order_work.release(); c->homo.release();
     c->map_acc.release(); c->io_i64.release(); c->io_u32.release();
     mpf::split_release(c->split);
     mpf::mlp_destroy(c->mlp);
@@ -1055,6 +1056,35 @@ int mpf_hmaps_fill_dev(mpf_ctx* c, int64_t nside, const int64_t* dev_pix, const 
         return ERR_INVAL;
     HIPCHK(c, hipSetDevice(c->cfg.device));
     HIPCHK(c, mpf::launch_pixel_fill(nside, dev_pix, dev_surf, (int)nsurf, n, dev_maps, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int mpf_hmaps_components_dev(mpf_ctx* c, int64_t nside, const int64_t* dev_pix, const double* dev_surf,
+                             const double* dev_jac, uint64_t n, uint32_t nz, const int* components, uint32_t ncomp,
+                             const double* homo5, float* dev_maps, uint64_t maps_capacity, uint32_t* nmaps_out) {
+    if (!c || nside < 1 || nside > (1 << 29) || nz < 1 || nz > MPF_MAX_SURFACES || ncomp < 1 || ncomp > 16 ||
+        !components || !homo5 || (n && (!dev_pix || !dev_surf)))
+        return ERR_INVAL;
+    int nmaps = 0;
+    bool lensing = false;
+    for (uint32_t j = 0; j < ncomp; ++j) {
+        const int k = components[j];
+        if (k == MPF_MAP_LENSING_BORN) return fail(c, ERR_INVAL, "the lensing_born maps are not provided");
+        if (k < MPF_MAP_LENSING || k > MPF_MAP_DEFLECTION) return fail(c, ERR_INVAL, "unknown map component");
+        lensing |= k == MPF_MAP_LENSING;
+        nmaps += (k == MPF_MAP_LENSING) ? (dev_jac ? 5 : 4) : (k == MPF_MAP_DEFLECTION ? 2 : 1);
+    }
+    if (nmaps_out) *nmaps_out = (uint32_t)nmaps;
+    const uint64_t need = (uint64_t)nz * (uint64_t)nmaps * 12ull * (uint64_t)nside * (uint64_t)nside;
+    if (!dev_maps || need > maps_capacity)
+        return fail(c, ERR_INVAL, "map buffer holds " + std::to_string(maps_capacity) + " floats, " +
+                                      std::to_string(need) + " needed");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (c->homo.ensure(5 * nz) != hipSuccess) return fail(c, ERR_NOMEM, "homogeneous values");
+    HIPCHK(c, hipMemcpy(c->homo.p, homo5, 5 * 8 * (uint64_t)nz, hipMemcpyHostToDevice));
+    HIPCHK(c, mpf::launch_component_fill(nside, dev_pix, dev_surf, lensing ? dev_jac : nullptr, c->homo.p, n, (int)nz,
+                                         components, (int)ncomp, nmaps, dev_maps, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }

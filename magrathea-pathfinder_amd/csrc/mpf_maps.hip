@@ -104,6 +104,92 @@ __global__ void pixel_fill_kernel(int64_t nside, const int64_t* __restrict__ pix
     }
 }
 
+// Every map component of Hmaps::FillMap (hmaps.h:716-836) for the pixels of
+// the rays, from their surface records: maps[(nmaps * iz + icomp) * npix + p]
+// in FP32 (the reference's Healpix_Map<float>).  comps: component codes
+// (MPF_MAP_*); homo: per surface (t, chi, lambda, redshift, a) of the
+// homogeneous reference ray; jac: bundle Jacobians (n x nz x 4) or null for
+// the infinitesimal one carried by the surface records.
+struct CompArgs {
+    int64_t nside = 0;
+    const int64_t* pix = nullptr;
+    const double* surf = nullptr;
+    const double* jac = nullptr;
+    const double* homo = nullptr;
+    uint64_t n = 0, npix = 0;
+    int nz = 0, ncomp = 0, nmaps = 0;
+    int comps[16] = {0};
+    float* maps = nullptr;
+};
+
+__global__ void component_fill_kernel(CompArgs A) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n) return;
+    const int64_t p = A.pix[i];
+    if (p < 0 || (uint64_t)p >= A.npix) return;
+    double v[3];
+    healpix::pix2vec_ring(A.nside, p, v);
+    const double phi = atan2(v[1], v[0]);
+    const double theta = acos(v[2]);
+    for (int iz = 0; iz < A.nz; ++iz) {
+        const double* s = A.surf + ((uint64_t)A.nz * i + iz) * SF_NREC;
+        float* m0 = A.maps + (uint64_t)A.nmaps * iz * A.npix + (uint64_t)p;
+        bool zero = s[SF_VALID] == 0.0;   // hmaps.h:716-720
+        if (!zero && A.jac) zero = A.jac[4 * ((uint64_t)A.nz * i + iz)] == 42.0;
+        if (zero) {
+            for (int k = 0; k < A.nmaps; ++k) m0[(uint64_t)k * A.npix] = 0.0f;
+            continue;
+        }
+        const double* h = A.homo + 5 * iz;   // t, chi, lambda, redshift, a
+        int icomp = 0;
+        for (int j = 0; j < A.ncomp; ++j) {
+            float* m = m0 + (uint64_t)icomp * A.npix;
+            switch (A.comps[j]) {
+                case MPF_MAP_LENSING:
+                    if (A.jac) {
+                        const double* a = A.jac + 4 * ((uint64_t)A.nz * i + iz);
+                        const double a11 = a[0], a12 = a[1], a21 = a[2], a22 = a[3];
+                        const double w = -atan2(a12 - a21, a11 + a22);
+                        const double cw = cos(w), sw = sin(w);
+                        m[0] = (float)(1 - (a11 + a22) * 0.5 / cw);
+                        m[A.npix] = (float)(-0.5 * (cw * (a11 - a22) + sw * (a12 + a21)));
+                        m[2 * A.npix] = (float)(-0.5 * (cw * (a12 + a21) + sw * (a22 - a11)));
+                        m[3 * A.npix] = (float)w;
+                        m[4 * A.npix] = (float)(a11 * a22 - a12 * a21);
+                        icomp += 4;
+                    } else {
+                        double q[4];
+                        lensing_from_jacobian(s, q);
+                        for (int k = 0; k < 4; ++k) m[(uint64_t)k * A.npix] = (float)q[k];
+                        icomp += 3;
+                    }
+                    break;
+                case MPF_MAP_DR: m[0] = (float)(s[SF_CHI] / h[1] - 1); break;
+                case MPF_MAP_DL: m[0] = (float)(s[SF_LAMBDA] / h[2] - 1); break;
+                case MPF_MAP_DT: m[0] = (float)(s[SF_T] / h[0] - 1); break;
+                case MPF_MAP_DA: m[0] = (float)(s[SF_A] / h[4] - 1); break;
+                case MPF_MAP_DZ: m[0] = (float)(s[SF_REDSHIFT] / h[3] - 1); break;
+                case MPF_MAP_DS: m[0] = (float)(s[SF_S] / h[1] - 1); break;
+                case MPF_MAP_ISW: m[0] = (float)s[SF_ISW]; break;
+                case MPF_MAP_DENS: m[0] = (float)s[SF_RHO]; break;
+                case MPF_MAP_DENS_MAX: m[0] = (float)s[SF_RHOMAX]; break;
+                case MPF_MAP_STEPS: m[0] = (float)s[SF_FIRSTID]; break;
+                case MPF_MAP_PHI: m[0] = (float)s[SF_PHI]; break;
+                case MPF_MAP_DEFLECTION: {
+                    const double x = s[SF_X], y = s[SF_Y], z = s[SF_Z];
+                    const double r = sqrt(x * x + y * y + z * z);
+                    m[0] = (float)(phi - atan2(y, x));
+                    m[A.npix] = (float)(theta - acos(z / r));
+                    icomp++;
+                    break;
+                }
+                default: break;
+            }
+            icomp++;
+        }
+    }
+}
+
 // launch photons at pixel centres (hmaps.h:599, 607; integrator.h:264-290)
 __global__ void pixel_launch_kernel(int64_t nside, const int64_t* __restrict__ pix, uint64_t n,
                                     double* __restrict__ in9) {
@@ -150,6 +236,26 @@ hipError_t launch_pixel_fill(int64_t nside, const int64_t* pix, const double* su
     const uint64_t npix = 12ull * (uint64_t)nside * (uint64_t)nside;
     if (n) hipLaunchKernelGGL(pixel_fill_kernel, dim3(blocks_for(n)), dim3(256), 0, st, nside, pix, surf, nsurf, n,
                               maps, npix);
+    return hipGetLastError();
+}
+
+hipError_t launch_component_fill(int64_t nside, const int64_t* pix, const double* surf, const double* jac,
+                                 const double* homo, uint64_t n, int nz, const int* comps, int ncomp, int nmaps,
+                                 float* maps, hipStream_t st) {
+    CompArgs A;
+    A.nside = nside;
+    A.pix = pix;
+    A.surf = surf;
+    A.jac = jac;
+    A.homo = homo;
+    A.n = n;
+    A.npix = 12ull * (uint64_t)nside * (uint64_t)nside;
+    A.nz = nz;
+    A.ncomp = ncomp;
+    A.nmaps = nmaps;
+    for (int j = 0; j < ncomp && j < 16; ++j) A.comps[j] = comps[j];
+    A.maps = maps;
+    if (n) hipLaunchKernelGGL(component_fill_kernel, dim3(blocks_for(n)), dim3(256), 0, st, A);
     return hipGetLastError();
 }
 

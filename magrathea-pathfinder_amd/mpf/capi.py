@@ -40,7 +40,7 @@ EXPORTS = ("mpf_create", "mpf_destroy", "mpf_last_error", "mpf_set_precision", "
            "mpf_cone_select", "mpf_cone_targets", "mpf_hmaps_cone_pixels", "mpf_launch_angles", "mpf_launch_towards",
            "mpf_launch_cone", "mpf_launch_bundle", "mpf_set_launch_config", "mpf_get_launch_config", "mpf_autotune",
            "mpf_last_kernel_ms", "mpf_propagate", "mpf_jacobian_bundle",
-           "mpf_ray_statistics")
+           "mpf_ray_statistics", "mpf_hmaps_homogeneous", "mpf_hmaps_components_dev", "mpf_healpix_write_fits")
 
 CAT_FIELDS = ("phi", "theta", "phi_obs", "theta_obs", "sep", "iter", "z0", "z1", "z2", "z3", "z4", "z5",
               "a11", "a12", "a21", "a22", "status", "nrounds")
@@ -182,6 +182,10 @@ def load_library(path=LIB_PATH):
     lib.mpf_cosmology_correct.argtypes = [vp, vp, u64, vp, vp, u64]
     lib.mpf_cosmology_prepare.argtypes = [i32, i32, cp, cp, cp, dbl, vp, vp, vp, vp, vp, vp, vp, u64, vp]
     lib.mpf_cosmology_last_error.restype = ctypes.c_char_p
+    lib.mpf_hmaps_homogeneous.argtypes = [vp, u64, vp, u32, i32, vp, vp]
+    lib.mpf_hmaps_components_dev.argtypes = [vp, i64, vp, vp, vp, u64, u32, vp, u32, vp, vp, u64,
+                                             ctypes.POINTER(u32)]
+    lib.mpf_healpix_write_fits.argtypes = [cp, vp, i64, i32, ctypes.c_char, i32]
     lib.mpf_ray_statistics.argtypes = [vp, vp, vp, u64, u64, vp, vp, u64, i32, i32, u32, dbl, dbl, vp, u64, vp, vp]
     lib.mpf_jacobian_bundle.argtypes = [vp, vp, u64, vp, i32, vp, u32, vp, vp, dbl, vp]
     lib.mpf_propagate.argtypes = [vp, vp, u64, u32, dbl, vp, i32, dbl, u64, vp, vp, vp]
@@ -312,6 +316,30 @@ def hmaps_cone_pixels(nside, cones7, icone):
     _cone_check(load_library().mpf_hmaps_cone_pixels(nside, _p(cones7), cones7.shape[0], icone, _p(pix), k.value,
                                              ctypes.byref(k)), "mpf_hmaps_cone_pixels")
     return pix[:k.value]
+
+
+MAP_NAMES = {"lensing": 0, "lensing_born": 1, "dr": 2, "dl": 3, "dt": 4, "da": 5, "dz": 6, "ds": 7, "isw": 8,
+             "dens": 9, "dens_max": 10, "steps": 11, "phi": 12, "deflection": 13}
+
+
+def hmaps_homogeneous(ref_rows, z_stop, stop_ray="redshift"):
+    """hmaps.cpp:297-325: (nz x (t, chi, lambda, redshift, a), surface values of stop_ray)."""
+    ref_rows = np.ascontiguousarray(ref_rows, dtype=np.float64).reshape(-1, NOUT)
+    z = np.ascontiguousarray(z_stop, dtype=np.float64).ravel()
+    homo, iref = np.zeros((z.size, 5)), np.zeros(z.size)
+    k = STOP_NAMES[stop_ray] if isinstance(stop_ray, str) else int(stop_ray)
+    if load_library().mpf_hmaps_homogeneous(_p(ref_rows), ref_rows.shape[0], _p(z), z.size, k, _p(homo), _p(iref)):
+        raise MpfError("mpf_hmaps_homogeneous: invalid arguments (a redshift beyond the reference ray?)")
+    return homo, iref
+
+
+def healpix_write_fits(path, hmap, nside, nest=False, coordsys="G", clean=True):
+    hmap = np.ascontiguousarray(hmap, dtype=np.float32).ravel()
+    if hmap.size != 12 * nside * nside:
+        raise ValueError("map size is not 12 nside^2")
+    if load_library().mpf_healpix_write_fits(str(path).encode(), _p(hmap), int(nside), int(bool(nest)),
+                                             coordsys.encode(), int(bool(clean))):
+        raise MpfError("mpf_healpix_write_fits failed")
 
 
 STAT_NAMES = {"distance": 0, "distance2": 1, "homogeneous": 2, "inhomogeneous": 3, "invdistance": 4,
@@ -753,6 +781,27 @@ class Context:
                                                  _p(ref), ctypes.c_uint32(nd), _p(ki), _p(dist),
                                                  ctypes.c_double(opening), _p(out)), "mpf_jacobian_bundle")
         return out
+
+    def hmaps_nmaps(self, components, bundle=False):
+        """Maps per surface for a component list (lensing: 4, or 5 with the bundle Jacobian; deflection: 2)."""
+        return sum((5 if bundle else 4) if c in ("lensing", 0) else (2 if c in ("deflection", 13) else 1)
+                   for c in components)
+
+    def hmaps_components_dev(self, nside, pix_ptr, surf_ptr, n, nz, components, homo5, maps_ptr, maps_capacity,
+                             jac_ptr=None):
+        """Every map component of FillMap into FP32 maps (maps_capacity floats
+        at maps_ptr); returns maps per surface."""
+        comps = np.array([MAP_NAMES[c] if isinstance(c, str) else int(c) for c in components], dtype=np.int32)
+        homo5 = np.ascontiguousarray(homo5, dtype=np.float64).reshape(nz, 5)
+        nm = ctypes.c_uint32(0)
+        self._check(self.lib.mpf_hmaps_components_dev(self.h, int(nside), ctypes.c_void_p(pix_ptr),
+                                                      ctypes.c_void_p(surf_ptr),
+                                                      ctypes.c_void_p(jac_ptr) if jac_ptr else None,
+                                                      ctypes.c_uint64(n), nz, _p(comps), comps.size, _p(homo5),
+                                                      ctypes.c_void_p(maps_ptr), ctypes.c_uint64(maps_capacity),
+                                                      ctypes.byref(nm)),
+                    "mpf_hmaps_components_dev")
+        return nm.value
 
     def last_kernel_ms(self):
         """Duration of the persistent integration kernel alone in the last run."""
