@@ -253,6 +253,16 @@ int mpf_integrate_surfaces_dev(mpf_ctx* ctx, const double* dev_in9, uint64_t n, 
 int mpf_jacobian_bundle(mpf_ctx* ctx, const double* phi_theta, uint64_t n, const double* observer3, int kind,
                         const double* ref, uint32_t nd, const double* ki, const double* dist, double opening,
                         double* out4);
+/* The Born approximation of the surfaces (lensing_born maps, hmaps.h:705-713;
+ * infinitesimal_born catalogues, catalogues.h:670-674): the rays follow the
+ * path of Integrator::integrate<-1> (homogeneous data: straight lines of the
+ * background) while the fused infinitesimal Jacobian samples the octree's
+ * data along them.  Runs in the context's exact tier (FP64 for a split
+ * context).  Arguments as for mpf_integrate_surfaces[_dev]. */
+int mpf_integrate_surfaces_born(mpf_ctx* ctx, const double* in9, uint64_t n, const mpf_surfaces* surf, double* out,
+                                double* out_surf);
+int mpf_integrate_surfaces_born_dev(mpf_ctx* ctx, const double* dev_in9, uint64_t n, const mpf_surfaces* surf,
+                                    double* dev_out, double* dev_out_surf, int sync);
 /* Catalogue root-finding: Catalogues::relCat / iterateNewtonMethod /
  * newtonMethod2d (catalogues.h:410-920) with the infinitesimal Jacobian
  * (jacobiantype = infinitesimal), for a batch of sources on the device. */
@@ -269,6 +279,12 @@ typedef struct {
      * fewer rays when most sources converge within three rounds (a good
      * first guess), more when they need many rounds. */
     uint32_t jacobian_all_rounds;
+    /* 1: jacobiantype = infinitesimal_born (catalogues.h:670-674): the
+     * Jacobian columns of converged sources come from the homogeneous path
+     * (Integrator::integrate<-1>) in the converged direction to the source's
+     * distance, sampled on the octree's data; the Newton updates still use
+     * the ray's own Jacobian.  0: jacobiantype = infinitesimal. */
+    uint32_t jacobian_born;
 } mpf_newton_cfg;
 /* Per source, MPF_NCAT doubles: the catalogue columns of catalogues.h:844-859
  * plus status and the number of rounds run. */
@@ -445,7 +461,9 @@ int mpf_hmaps_fill_dev(mpf_ctx* ctx, int64_t nside, const int64_t* dev_pix, cons
  * homogeneous value minus 1 (ds over chi); isw, dens, dens_max, steps, phi:
  * the interpolated value, the largest rho before the surface, the element
  * index; deflection: phi - beta1, theta - beta2 of the pixel centre and the
- * ray's position at the surface.  lensing_born is not provided. */
+ * ray's position at the surface; lensing_born: kappa, gamma1, gamma2, 1/mu
+ * from the Born surfaces (mpf_integrate_surfaces_born_dev, radius surfaces
+ * at the homogeneous distances, hmaps.h:705-713). */
 enum {
     MPF_MAP_LENSING = 0, MPF_MAP_LENSING_BORN = 1, MPF_MAP_DR = 2, MPF_MAP_DL = 3, MPF_MAP_DT = 4, MPF_MAP_DA = 5,
     MPF_MAP_DZ = 6, MPF_MAP_DS = 7, MPF_MAP_ISW = 8, MPF_MAP_DENS = 9, MPF_MAP_DENS_MAX = 10, MPF_MAP_STEPS = 11,
@@ -466,14 +484,17 @@ int mpf_hmaps_homogeneous(const double* ref_rows, uint64_t m, const double* z_st
  * (dev_surf, n x nz x MPF_NSURF): dev_maps[(nmaps * iz + k) * npix + pix] in
  * FP32, the layout of the reference's map vector; pixels of rejected
  * surfaces (or of failed bundle Jacobians) are 0 in every map of that
- * surface; pixels not in dev_pix are untouched.  dev_jac: n x nz x 4 bundle
+ * surface; pixels not in dev_pix are untouched.  dev_surf_born: the Born
+ * surface records (n x nz x MPF_NSURF), needed by lensing_born only, as
+ * dev_surf is needed only by the other components.  dev_jac: n x nz x 4 bundle
  * Jacobians (mpf_jacobian_bundle, copied to the device), or NULL for the
  * infinitesimal one of the surface records.  homo5: host, nz x 5 of
  * mpf_hmaps_homogeneous.  nmaps receives the maps per surface; -EINVAL,
  * writing nothing, when nz x nmaps x 12 nside^2 exceeds maps_capacity
  * (floats; dev_maps may then be NULL, to size the buffer). */
 int mpf_hmaps_components_dev(mpf_ctx* ctx, int64_t nside, const int64_t* dev_pix, const double* dev_surf,
-                             const double* dev_jac, uint64_t n, uint32_t nz, const int* components, uint32_t ncomp,
+                             const double* dev_surf_born, const double* dev_jac, uint64_t n, uint32_t nz,
+                             const int* components, uint32_t ncomp,
                              const double* homo5, float* dev_maps, uint64_t maps_capacity, uint32_t* nmaps);
 /* Write one RING (nest = 0) or NESTED map as a HEALPix FITS file: a binary
  * table of one FP32 column with the PIXTYPE, ORDERING, NSIDE, FIRSTPIX,

@@ -439,6 +439,32 @@ static int integrate_impl(mpf_ctx* c, const double* dev_in9, uint64_t n, const m
     return 0;
 }
 
+int mpf_integrate_surfaces_born_dev(mpf_ctx* c, const double* dev_in9, uint64_t n, const mpf_surfaces* surf,
+                                    double* dev_out, double* dev_out_surf, int sync) {
+    if (!c) return ERR_INVAL;
+    // the path of Integrator::integrate<-1> (homogeneous data) in an exact tier; the Jacobian's samples
+    // still come from the octree's data (lensing.h:421-448)
+    const int ord = c->cfg.order, prec = c->cfg.precision;
+    c->cfg.order = MPF_HOMOGENEOUS;
+    if (prec == MPF_PREC_SPLIT) c->cfg.precision = MPF_PREC_FP64;
+    const int rc = mpf_integrate_surfaces_dev(c, dev_in9, n, surf, dev_out, dev_out_surf, sync);
+    c->cfg.order = ord;
+    c->cfg.precision = prec;
+    return rc;
+}
+
+int mpf_integrate_surfaces_born(mpf_ctx* c, const double* in9, uint64_t n, const mpf_surfaces* surf, double* out,
+                                double* out_surf) {
+    if (!c) return ERR_INVAL;
+    const int ord = c->cfg.order, prec = c->cfg.precision;
+    c->cfg.order = MPF_HOMOGENEOUS;
+    if (prec == MPF_PREC_SPLIT) c->cfg.precision = MPF_PREC_FP64;
+    const int rc = mpf_integrate_surfaces(c, in9, n, surf, out, out_surf);
+    c->cfg.order = ord;
+    c->cfg.precision = prec;
+    return rc;
+}
+
 int mpf_propagate(mpf_ctx* c, const double* in9, uint64_t n, uint32_t count, double angle, const double* rotation,
                   int interpolation, double amin, uint64_t capacity, double* traj, double* dist, uint64_t* nelem) {
     if (!c || (n && (!in9 || !traj || !dist || !nelem)) || capacity == 0) return ERR_INVAL;
@@ -814,6 +840,22 @@ int mpf_catalogue_solve_dev(mpf_ctx* c, const double* dev_src6, const double* de
         CATCHK(mpf::launch_cat_jac(need.p, nneed, surf.p, dev_out, c->stream));
         CATCHK(hipStreamSynchronize(c->stream));
     }
+    // jacobiantype = infinitesimal_born: the final Jacobian from the homogeneous path in the converged
+    // direction to the source's distance (catalogues.h:670-674)
+    if (cfg->jacobian_born) {
+        CATCHK(hipMemcpyAsync(need.p, ids.data(), n * 4, hipMemcpyHostToDevice, c->stream));
+        CATCHK(mpf::launch_cat_launch(ang.p, need.p, n, in9.p, dact.p, dist.p, c->stream));
+        mpf_surfaces sf{};
+        sf.kind = MPF_STOP_RADIUS;
+        sf.n = 1;
+        sf.jacobian = 1;
+        sf.ray_value = dact.p;
+        int rc = mpf_integrate_surfaces_born_dev(c, in9.p, n, &sf, outb.p, surf.p, 1);
+        if (rc) { release(); return rc; }
+        rays += n;
+        CATCHK(mpf::launch_cat_jac_born(n, surf.p, dev_out, c->stream));
+        CATCHK(hipStreamSynchronize(c->stream));
+    }
 #undef CATCHK
     release();
     if (total_rays) *total_rays = rays;
@@ -1061,20 +1103,25 @@ int mpf_hmaps_fill_dev(mpf_ctx* c, int64_t nside, const int64_t* dev_pix, const 
 }
 
 int mpf_hmaps_components_dev(mpf_ctx* c, int64_t nside, const int64_t* dev_pix, const double* dev_surf,
-                             const double* dev_jac, uint64_t n, uint32_t nz, const int* components, uint32_t ncomp,
+                             const double* dev_surf_born, const double* dev_jac, uint64_t n, uint32_t nz,
+                             const int* components, uint32_t ncomp,
                              const double* homo5, float* dev_maps, uint64_t maps_capacity, uint32_t* nmaps_out) {
     if (!c || nside < 1 || nside > (1 << 29) || nz < 1 || nz > MPF_MAX_SURFACES || ncomp < 1 || ncomp > 16 ||
-        !components || !homo5 || (n && (!dev_pix || !dev_surf)))
+        !components || !homo5 || (n && !dev_pix))
         return ERR_INVAL;
     int nmaps = 0;
-    bool lensing = false;
+    bool lensing = false, born = false, real = false;
     for (uint32_t j = 0; j < ncomp; ++j) {
         const int k = components[j];
-        if (k == MPF_MAP_LENSING_BORN) return fail(c, ERR_INVAL, "the lensing_born maps are not provided");
         if (k < MPF_MAP_LENSING || k > MPF_MAP_DEFLECTION) return fail(c, ERR_INVAL, "unknown map component");
         lensing |= k == MPF_MAP_LENSING;
-        nmaps += (k == MPF_MAP_LENSING) ? (dev_jac ? 5 : 4) : (k == MPF_MAP_DEFLECTION ? 2 : 1);
+        born |= k == MPF_MAP_LENSING_BORN;
+        real |= k != MPF_MAP_LENSING_BORN;
+        nmaps += (k == MPF_MAP_LENSING) ? (dev_jac ? 5 : 4)
+                                         : (k == MPF_MAP_LENSING_BORN ? 4 : (k == MPF_MAP_DEFLECTION ? 2 : 1));
     }
+    if (n && ((real && !dev_surf) || (born && !dev_surf_born)))
+        return fail(c, ERR_INVAL, "the components need the ray's surfaces and, for lensing_born, the Born surfaces");
     if (nmaps_out) *nmaps_out = (uint32_t)nmaps;
     const uint64_t need = (uint64_t)nz * (uint64_t)nmaps * 12ull * (uint64_t)nside * (uint64_t)nside;
     if (!dev_maps || need > maps_capacity)
@@ -1083,8 +1130,9 @@ int mpf_hmaps_components_dev(mpf_ctx* c, int64_t nside, const int64_t* dev_pix, 
     HIPCHK(c, hipSetDevice(c->cfg.device));
     if (c->homo.ensure(5 * nz) != hipSuccess) return fail(c, ERR_NOMEM, "homogeneous values");
     HIPCHK(c, hipMemcpy(c->homo.p, homo5, 5 * 8 * (uint64_t)nz, hipMemcpyHostToDevice));
-    HIPCHK(c, mpf::launch_component_fill(nside, dev_pix, dev_surf, lensing ? dev_jac : nullptr, c->homo.p, n, (int)nz,
-                                         components, (int)ncomp, nmaps, dev_maps, c->stream));
+    HIPCHK(c, mpf::launch_component_fill(nside, dev_pix, real ? dev_surf : nullptr, born ? dev_surf_born : nullptr,
+                                         lensing ? dev_jac : nullptr, c->homo.p, n, (int)nz, components, (int)ncomp,
+                                         nmaps, dev_maps, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }

@@ -178,9 +178,26 @@ __global__ void cat_jac_kernel(const uint32_t* __restrict__ act, uint64_t nact, 
     o[MPF_CAT_A21] = s[SF_A12]; o[MPF_CAT_A22] = s[SF_A22];
 }
 
+// the Born Jacobian of converged sources (jacobiantype = infinitesimal_born,
+// catalogues.h:670-674): k runs over every source
+__global__ void cat_jac_born_kernel(uint64_t n, const double* __restrict__ surf, double* __restrict__ out) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    double* o = out + (uint64_t)MPF_NCAT * k;
+    if (o[MPF_CAT_STATUS] != (double)MPF_CAT_OK) return;
+    const double* s = surf + (uint64_t)SF_NREC * k;
+    o[MPF_CAT_A11] = s[SF_A11]; o[MPF_CAT_A12] = s[SF_A12];
+    o[MPF_CAT_A21] = s[SF_A12]; o[MPF_CAT_A22] = s[SF_A22];
+}
+
 inline unsigned blocks_for(uint64_t n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace
+
+hipError_t launch_cat_jac_born(uint64_t n, const double* surf, double* out, hipStream_t st) {
+    if (n) hipLaunchKernelGGL(cat_jac_born_kernel, dim3(blocks_for(n)), dim3(256), 0, st, n, surf, out);
+    return hipGetLastError();
+}
 
 hipError_t launch_cat_init(const double* src, const double* init, uint64_t n, double* dist, double* ang, double* out,
                            hipStream_t st) {

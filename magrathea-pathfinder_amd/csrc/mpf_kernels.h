@@ -56,9 +56,9 @@ hipError_t launch_bin(int64_t nside, const double* in9, const double* surf, int 
                       unsigned long long* acc, double* maps, hipStream_t st);
 hipError_t launch_pixel_fill(int64_t nside, const int64_t* pix, const double* surf, int nsurf, uint64_t n,
                              double* maps, hipStream_t st);
-hipError_t launch_component_fill(int64_t nside, const int64_t* pix, const double* surf, const double* jac,
-                                 const double* homo, uint64_t n, int nz, const int* comps, int ncomp, int nmaps,
-                                 float* maps, hipStream_t st);
+hipError_t launch_component_fill(int64_t nside, const int64_t* pix, const double* surf, const double* surf_born,
+                                 const double* jac, const double* homo, uint64_t n, int nz, const int* comps,
+                                 int ncomp, int nmaps, float* maps, hipStream_t st);
 hipError_t launch_pixel_launch(int64_t nside, const int64_t* pix, uint64_t n, double* in9, hipStream_t st);
 hipError_t launch_vec2pix(int64_t nside, const double* vec, uint64_t n, int64_t* pix, hipStream_t st);
 hipError_t launch_pix2vec(int64_t nside, const int64_t* pix, uint64_t n, double* vec, hipStream_t st);
@@ -83,6 +83,7 @@ hipError_t launch_cat_launch(const double* ang, const uint32_t* act, uint64_t na
                              const double* dist, hipStream_t st);
 hipError_t launch_newton(const CatRound& R, hipStream_t st);
 hipError_t launch_cat_jac(const uint32_t* act, uint64_t nact, const double* surf, double* out, hipStream_t st);
+hipError_t launch_cat_jac_born(uint64_t n, const double* surf, double* out, hipStream_t st);
 hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v, hipStream_t st);
 
 }  // namespace mpf

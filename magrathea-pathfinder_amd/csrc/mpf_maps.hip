@@ -114,8 +114,10 @@ struct CompArgs {
     int64_t nside = 0;
     const int64_t* pix = nullptr;
     const double* surf = nullptr;
+    const double* surf_born = nullptr;   // Born surface records (lensing_born), or null
     const double* jac = nullptr;
     const double* homo = nullptr;
+    bool real = true;                    // a component needs the ray's own surface records
     uint64_t n = 0, npix = 0;
     int nz = 0, ncomp = 0, nmaps = 0;
     int comps[16] = {0};
@@ -133,8 +135,9 @@ __global__ void component_fill_kernel(CompArgs A) {
     const double theta = acos(v[2]);
     for (int iz = 0; iz < A.nz; ++iz) {
         const double* s = A.surf + ((uint64_t)A.nz * i + iz) * SF_NREC;
+        const double* sb = A.surf_born ? A.surf_born + ((uint64_t)A.nz * i + iz) * SF_NREC : nullptr;
         float* m0 = A.maps + (uint64_t)A.nmaps * iz * A.npix + (uint64_t)p;
-        bool zero = s[SF_VALID] == 0.0;   // hmaps.h:716-720
+        bool zero = (A.real && s[SF_VALID] == 0.0) || (sb && sb[SF_VALID] == 0.0);   // hmaps.h:716-720
         if (!zero && A.jac) zero = A.jac[4 * ((uint64_t)A.nz * i + iz)] == 42.0;
         if (zero) {
             for (int k = 0; k < A.nmaps; ++k) m0[(uint64_t)k * A.npix] = 0.0f;
@@ -164,6 +167,13 @@ __global__ void component_fill_kernel(CompArgs A) {
                         icomp += 3;
                     }
                     break;
+                case MPF_MAP_LENSING_BORN: {
+                    double q[4];
+                    lensing_from_jacobian(sb, q);
+                    for (int k = 0; k < 4; ++k) m[(uint64_t)k * A.npix] = (float)q[k];
+                    icomp += 3;
+                    break;
+                }
                 case MPF_MAP_DR: m[0] = (float)(s[SF_CHI] / h[1] - 1); break;
                 case MPF_MAP_DL: m[0] = (float)(s[SF_LAMBDA] / h[2] - 1); break;
                 case MPF_MAP_DT: m[0] = (float)(s[SF_T] / h[0] - 1); break;
@@ -239,13 +249,16 @@ hipError_t launch_pixel_fill(int64_t nside, const int64_t* pix, const double* su
     return hipGetLastError();
 }
 
-hipError_t launch_component_fill(int64_t nside, const int64_t* pix, const double* surf, const double* jac,
-                                 const double* homo, uint64_t n, int nz, const int* comps, int ncomp, int nmaps,
-                                 float* maps, hipStream_t st) {
+hipError_t launch_component_fill(int64_t nside, const int64_t* pix, const double* surf, const double* surf_born,
+                                 const double* jac, const double* homo, uint64_t n, int nz, const int* comps,
+                                 int ncomp, int nmaps, float* maps, hipStream_t st) {
     CompArgs A;
     A.nside = nside;
     A.pix = pix;
     A.surf = surf;
+    A.surf_born = surf_born;
+    A.real = false;
+    for (int j = 0; j < ncomp; ++j) A.real |= comps[j] != MPF_MAP_LENSING_BORN;
     A.jac = jac;
     A.homo = homo;
     A.n = n;

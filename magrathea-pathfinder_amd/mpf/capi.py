@@ -40,7 +40,8 @@ EXPORTS = ("mpf_create", "mpf_destroy", "mpf_last_error", "mpf_set_precision", "
            "mpf_cone_select", "mpf_cone_targets", "mpf_hmaps_cone_pixels", "mpf_launch_angles", "mpf_launch_towards",
            "mpf_launch_cone", "mpf_launch_bundle", "mpf_set_launch_config", "mpf_get_launch_config", "mpf_autotune",
            "mpf_last_kernel_ms", "mpf_propagate", "mpf_jacobian_bundle",
-           "mpf_ray_statistics", "mpf_hmaps_homogeneous", "mpf_hmaps_components_dev", "mpf_healpix_write_fits")
+           "mpf_ray_statistics", "mpf_hmaps_homogeneous", "mpf_hmaps_components_dev", "mpf_healpix_write_fits",
+           "mpf_integrate_surfaces_born", "mpf_integrate_surfaces_born_dev")
 
 CAT_FIELDS = ("phi", "theta", "phi_obs", "theta_obs", "sep", "iter", "z0", "z1", "z2", "z3", "z4", "z5",
               "a11", "a12", "a21", "a22", "status", "nrounds")
@@ -93,13 +94,15 @@ def make_surfaces(kind, values, jacobian=True, ray_value=None):
 
 class NewtonCfg(ctypes.Structure):
     _fields_ = [("accuracy", ctypes.c_double), ("max_iterations", ctypes.c_uint32), ("vobs", ctypes.c_double * 3),
-                ("h", ctypes.c_double), ("mpc", ctypes.c_double), ("jacobian_all_rounds", ctypes.c_uint32)]
+                ("h", ctypes.c_double), ("mpc", ctypes.c_double), ("jacobian_all_rounds", ctypes.c_uint32),
+                ("jacobian_born", ctypes.c_uint32)]
 
 
 def make_newton(accuracy=1e-8, max_iterations=10, vobs=(0.0, 0.0, 0.0), h=0.7, mpc=3.08568e22,
-                jacobian_all_rounds=False):
+                jacobian_all_rounds=False, jacobian_born=False):
     c = NewtonCfg()
     c.jacobian_all_rounds = int(bool(jacobian_all_rounds))
+    c.jacobian_born = int(bool(jacobian_born))
     c.accuracy = accuracy
     c.max_iterations = max_iterations
     for i in range(3):
@@ -183,8 +186,10 @@ def load_library(path=LIB_PATH):
     lib.mpf_cosmology_prepare.argtypes = [i32, i32, cp, cp, cp, dbl, vp, vp, vp, vp, vp, vp, vp, u64, vp]
     lib.mpf_cosmology_last_error.restype = ctypes.c_char_p
     lib.mpf_hmaps_homogeneous.argtypes = [vp, u64, vp, u32, i32, vp, vp]
-    lib.mpf_hmaps_components_dev.argtypes = [vp, i64, vp, vp, vp, u64, u32, vp, u32, vp, vp, u64,
+    lib.mpf_hmaps_components_dev.argtypes = [vp, i64, vp, vp, vp, vp, u64, u32, vp, u32, vp, vp, u64,
                                              ctypes.POINTER(u32)]
+    lib.mpf_integrate_surfaces_born.argtypes = [vp, vp, u64, ctypes.POINTER(Surfaces), vp, vp]
+    lib.mpf_integrate_surfaces_born_dev.argtypes = [vp, vp, u64, ctypes.POINTER(Surfaces), vp, vp, i32]
     lib.mpf_healpix_write_fits.argtypes = [cp, vp, i64, i32, ctypes.c_char, i32]
     lib.mpf_ray_statistics.argtypes = [vp, vp, vp, u64, u64, vp, vp, u64, i32, i32, u32, dbl, dbl, vp, u64, vp, vp]
     lib.mpf_jacobian_bundle.argtypes = [vp, vp, u64, vp, i32, vp, u32, vp, vp, dbl, vp]
@@ -563,6 +568,29 @@ class Context:
                                                     _p(surf)), "mpf_integrate_surfaces")
         return out, surf
 
+    def integrate_surfaces_born(self, in9, kind, values, jacobian=True, ray_value=None):
+        """Surfaces in the Born approximation (homogeneous path, Jacobian on the octree's data)."""
+        in9 = np.ascontiguousarray(in9, dtype=np.float64).reshape(-1, 9)
+        n = in9.shape[0]
+        rv = None
+        if ray_value is not None:
+            rv = np.ascontiguousarray(ray_value, dtype=np.float64).ravel()
+            values = [0.0]
+        sf = make_surfaces(kind, values, jacobian, None if rv is None else rv.ctypes.data)
+        out = np.zeros((n, NOUT))
+        surf = np.zeros((n, sf.n, NSURF))
+        self._check(self.lib.mpf_integrate_surfaces_born(self.h, _p(in9), ctypes.c_uint64(n), ctypes.byref(sf),
+                                                         _p(out), _p(surf)), "mpf_integrate_surfaces_born")
+        return out, surf
+
+    def integrate_surfaces_born_dev(self, in_ptr, n, out_ptr, surf_ptr, kind, values, jacobian=True, sync=False,
+                                    ray_value_ptr=None):
+        sf = make_surfaces(kind, values if ray_value_ptr is None else [0.0], jacobian, ray_value_ptr)
+        self._check(self.lib.mpf_integrate_surfaces_born_dev(self.h, ctypes.c_void_p(in_ptr), ctypes.c_uint64(n),
+                                                             ctypes.byref(sf), ctypes.c_void_p(out_ptr),
+                                                             ctypes.c_void_p(surf_ptr), int(bool(sync))),
+                    "mpf_integrate_surfaces_born_dev")
+
     def integrate_surfaces_dev(self, in_ptr, n, out_ptr, surf_ptr, kind, values, jacobian=True, sync=False,
                                ray_value_ptr=None):
         sf = make_surfaces(kind, values if ray_value_ptr is None else [0.0], jacobian, ray_value_ptr)
@@ -784,18 +812,19 @@ class Context:
 
     def hmaps_nmaps(self, components, bundle=False):
         """Maps per surface for a component list (lensing: 4, or 5 with the bundle Jacobian; deflection: 2)."""
-        return sum((5 if bundle else 4) if c in ("lensing", 0) else (2 if c in ("deflection", 13) else 1)
-                   for c in components)
+        return sum((5 if bundle else 4) if c in ("lensing", 0) else
+                   (4 if c in ("lensing_born", 1) else (2 if c in ("deflection", 13) else 1)) for c in components)
 
     def hmaps_components_dev(self, nside, pix_ptr, surf_ptr, n, nz, components, homo5, maps_ptr, maps_capacity,
-                             jac_ptr=None):
+                             jac_ptr=None, surf_born_ptr=None):
         """Every map component of FillMap into FP32 maps (maps_capacity floats
         at maps_ptr); returns maps per surface."""
         comps = np.array([MAP_NAMES[c] if isinstance(c, str) else int(c) for c in components], dtype=np.int32)
         homo5 = np.ascontiguousarray(homo5, dtype=np.float64).reshape(nz, 5)
         nm = ctypes.c_uint32(0)
         self._check(self.lib.mpf_hmaps_components_dev(self.h, int(nside), ctypes.c_void_p(pix_ptr),
-                                                      ctypes.c_void_p(surf_ptr),
+                                                      ctypes.c_void_p(surf_ptr) if surf_ptr else None,
+                                                      ctypes.c_void_p(surf_born_ptr) if surf_born_ptr else None,
                                                       ctypes.c_void_p(jac_ptr) if jac_ptr else None,
                                                       ctypes.c_uint64(n), nz, _p(comps), comps.size, _p(homo5),
                                                       ctypes.c_void_p(maps_ptr), ctypes.c_uint64(maps_capacity),

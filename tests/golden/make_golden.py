@@ -7,8 +7,8 @@ the deterministic inputs of tests/cases.py.  The fixtures are data only:
 inputs are regenerated by tests/cases.py and checked against the stored
 digest; outputs are the reference's numbers.
 
-    python tests/golden/make_golden.py [tree ... | cones | propagate | bundle]
-    (default: keys, cones, propagate, bundle and every tree)
+    python tests/golden/make_golden.py [tree ... | cones | propagate | bundle | born]
+    (default: keys, cones, propagate, bundle, born and every tree)
 """
 import os
 import sys
@@ -33,6 +33,8 @@ def main(names):
         return propagate(R, tab)
     if names == ["bundle"]:
         return bundle(R, tab)
+    if names == ["born"]:
+        return born(R, tab)
     if names:
         return trees(R, tab, names)
     # keys: compute() at several levels on random and edge points
@@ -52,6 +54,7 @@ def main(names):
     cones(R)
     propagate(R, tab)
     bundle(R, tab)
+    born(R, tab)
     trees(R, tab, cases.TREES)
 
 
@@ -84,6 +87,26 @@ def bundle(R, tab):
         R.free(h)
     np.savez_compressed(os.path.join(HERE, "bundle.npz"), **out)
     print("bundle written")
+
+
+def born(R, tab):
+    """The Born Jacobian of the reference: Integrator::integrate<-1> to a
+    radius, then Lensing::dbetadtheta_infinitesimal on the octree's data
+    (catalogues.h:670-674)."""
+    out = {}
+    for name in ("amr", "wide"):
+        rec, f64 = cases.tree(name)
+        h = R.tree(rec)
+        in9 = cases.rays(n_side=5, aperture=24.0)
+        out[f"{name}_digest"] = np.array(cases.records_digest(rec))
+        out[f"{name}_in9"] = in9
+        # stop before the hole of the amr tree at 0.38 (inside a missing region
+        # the homogeneous path has no cell level to take its step from)
+        out[f"{name}_jac"] = np.stack([R.jacobian(h, -1, tab, cases.LENGTH, cases.NSTEPS, 0, 0.32, r, [0.15, 0.25, 0.3])
+                                       for r in in9])
+        R.free(h)
+    np.savez_compressed(os.path.join(HERE, "born.npz"), **out)
+    print("born written")
 
 
 def propagate(R, tab):
