@@ -287,10 +287,15 @@ int mpf_set_cosmology(mpf_ctx* c, const double* t, const double* a, const double
     for (uint64_t i = 1; i < n; ++i)
         if (!(t[i - 1] <= t[i])) return fail(c, ERR_INVAL, "cosmology t column is not sorted");
     HIPCHK(c, hipSetDevice(c->cfg.device));
-    if (c->cosmo.ensure(3 * n) != hipSuccess) return fail(c, ERR_NOMEM, "cosmology allocation");
+    if (c->cosmo.ensure(4 * n) != hipSuccess) return fail(c, ERR_NOMEM, "cosmology allocation");
+    // fourth column: the slope of da/dt on each segment (the split tier's interpolation)
+    std::vector<double> slope(n, 0.0);
+    for (uint64_t i = 1; i < n; ++i)
+        slope[i] = (t[i] > t[i - 1]) ? (dadt[i] - dadt[i - 1]) / (t[i] - t[i - 1]) : 0.0;
     HIPCHK(c, hipMemcpyAsync(c->cosmo.p, t, n * 8, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->cosmo.p + n, a, n * 8, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->cosmo.p + 2 * n, dadt, n * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->cosmo.p + 3 * n, slope.data(), n * 8, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->ncos = (int)n;
     return 0;
@@ -371,6 +376,7 @@ static int integrate_impl(mpf_ctx* c, const double* dev_in9, uint64_t n, const m
     A.ct = c->cosmo.p;
     A.ca = c->cosmo.p + c->ncos;
     A.cdadt = c->cosmo.p + 2 * c->ncos;
+    A.cslope = c->cosmo.p + 3 * c->ncos;
     A.ncos = c->ncos;
     A.order = c->cfg.order;
     A.nsteps = c->cfg.nsteps;

@@ -23,6 +23,7 @@ struct IntegrateArgs {
     const double* ct = nullptr;      // cosmology table columns
     const double* ca = nullptr;
     const double* cdadt = nullptr;
+    const double* cslope = nullptr;  // d(da/dt)/dt of each table segment (split tier)
     int ncos = 0;
     int order = 2;
     unsigned nsteps = 10;

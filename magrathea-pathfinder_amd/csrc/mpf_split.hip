@@ -318,6 +318,30 @@ __device__ __forceinline__ double cosmo_lin(double x0, const double* __restrict_
     return y[i - 1] + ((y[i] - y[i - 1]) * (x0 - x[i - 1]) / (x[i] - x[i - 1]));
 }
 
+// da/dt of the table at x0 with the segment slopes computed once per table
+// (no division per call; the exact tiers keep Utility::interpolate's)
+__device__ __forceinline__ double cosmo_slope(double x0, const double* __restrict__ x, const double* __restrict__ y,
+                                              const double* __restrict__ slope, int n, int& hint) {
+    int i = hint;
+    while (i < n && !(x0 < x[i])) ++i;
+    while (i > 0 && x0 < x[i - 1]) --i;
+    hint = i;
+    i += (i <= 0) - (i >= n);
+    return fma(slope[i], x0 - x[i - 1], y[i - 1]);
+}
+
+// 1 / a: the hardware reciprocal refined by one Newton step (within an ulp
+// or two of the quotient; the FP64 background of the split tier does not need
+// the correctly rounded division)
+__device__ __forceinline__ double recip(double a) {
+    double r = __builtin_amdgcn_rcp(a);
+    const double e = fma(-a, r, 1.0);
+    r = fma(r, e, r);
+    return r;
+}
+
+constexpr double INV_C2 = 1.0 / C2;
+
 enum { CA = 0, CT, CX, CY, CZ, CKT, CKX, CKY, CKZ };
 
 // geodesic right-hand side: FP64 background, FP32 perturbation
@@ -330,7 +354,7 @@ __device__ __forceinline__ void rhs_split(const double* in, const float* F, doub
     const float kt2 = ktf * ktf;
     const float pert_t = -(c2inv2 * ktf) * grad;                 // -(2/c^2) kt (dphidl - dphidt kt)
     const float s = c2inv2 * dphidl;
-    const double bg = -(2.0 * dadt / in[CA]) * in[CKT];
+    const double bg = -((2.0 * dadt) * recip(in[CA])) * in[CKT];
     out[CA] = in[CKT] * dadt;
     out[CT] = in[CKT];
     out[CX] = in[CKX] * inv_scale;
@@ -353,7 +377,7 @@ struct SplitRay {
     double stopv;     // this ray's stop value (and its surface when per-ray)
 };
 
-__device__ __forceinline__ double level_extent(unsigned l) { return EXTENT / (double)(1ull << l); }
+__device__ __forceinline__ double level_extent(unsigned l) { return ldexp((double)EXTENT, -(int)l); }
 
 // Launch records (integrator.h:669-705) with exact FP64 fields at the start
 // point: normalised core (9), empty flag, phi, dphidx, dphidy, dphidz,
@@ -492,30 +516,30 @@ integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
         // ---------------------------------------------------------- step
         const double dl = R.dl;
         double ksum[9], kin[9], kout[9];
-        double dadt = cosmo_lin(R.core[CT], A.ct, A.cdadt, A.ncos, R.hint);
+        double dadt = cosmo_slope(R.core[CT], A.ct, A.cdadt, A.cslope, A.ncos, R.hint);
         rhs_split(R.core, R.F, dadt, inv_scale, kout);
 #pragma unroll
         for (int i = 0; i < 9; ++i) { ksum[i] = kout[i]; kin[i] = R.core[i] + (0.5 * dl) * kout[i]; }
         float F[5];
         bool ok = split_eval<4, RC>(t, sv, I, kin[CX], kin[CY], kin[CZ], F);
         if (!ok) { F[0] = F[1] = F[2] = F[3] = 0.f; }
-        dadt = cosmo_lin(kin[CT], A.ct, A.cdadt, A.ncos, R.hint);
+        dadt = cosmo_slope(kin[CT], A.ct, A.cdadt, A.cslope, A.ncos, R.hint);
         rhs_split(kin, F, dadt, inv_scale, kout);
 #pragma unroll
         for (int i = 0; i < 9; ++i) { ksum[i] += 2.0 * kout[i]; kin[i] = R.core[i] + (0.5 * dl) * kout[i]; }
         ok = split_eval<4, RC>(t, sv, I, kin[CX], kin[CY], kin[CZ], F);
         if (!ok) { F[0] = F[1] = F[2] = F[3] = 0.f; }
-        dadt = cosmo_lin(kin[CT], A.ct, A.cdadt, A.ncos, R.hint);
+        dadt = cosmo_slope(kin[CT], A.ct, A.cdadt, A.cslope, A.ncos, R.hint);
         rhs_split(kin, F, dadt, inv_scale, kout);
 #pragma unroll
         for (int i = 0; i < 9; ++i) { ksum[i] += 2.0 * kout[i]; kin[i] = R.core[i] + dl * kout[i]; }
         ok = split_eval<4, RC>(t, sv, I, kin[CX], kin[CY], kin[CZ], F);
         if (!ok) { F[0] = F[1] = F[2] = F[3] = 0.f; }
-        dadt = cosmo_lin(kin[CT], A.ct, A.cdadt, A.ncos, R.hint);
+        dadt = cosmo_slope(kin[CT], A.ct, A.cdadt, A.cslope, A.ncos, R.hint);
         rhs_split(kin, F, dadt, inv_scale, kout);
         double p[9];
 #pragma unroll
-        for (int i = 0; i < 9; ++i) p[i] = R.core[i] + (dl / 6.0) * (ksum[i] + kout[i]);
+        for (int i = 0; i < 9; ++i) p[i] = R.core[i] + (dl * (1.0 / 6.0)) * (ksum[i] + kout[i]);
         // fields at the new point
         const bool inside = !(p[CA] < 0.0) && p[CX] > -2.0 && p[CX] < 2.0 && p[CY] > -2.0 && p[CY] < 2.0 &&
                             p[CZ] > -2.0 && p[CZ] < 2.0;
@@ -525,15 +549,15 @@ integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
         const double nphi = (double)F[4];
         const double ndt = -(double)F[0];
         const double pdx = (double)F[1], pdy = (double)F[2], pdz = (double)F[3];
-        const double isw = R.isw + ((double)R.dt + ndt) * (p[CT] - R.core[CT]) / c2;
+        const double isw = R.isw + ((double)R.dt + ndt) * (p[CT] - R.core[CT]) * INV_C2;
         const double ddx = p[CX] - R.core[CX], ddy = p[CY] - R.core[CY], ddz = p[CZ] - R.core[CZ];
-        const double iswold = R.iswold - (2.0 * (nphi - (double)R.phi) / c2 -
+        const double iswold = R.iswold - (2.0 * (nphi - (double)R.phi) * INV_C2 -
                                           ((((double)R.dx + pdx) * ddx + ((double)R.dy + pdy) * ddy +
-                                            ((double)R.dz + pdz) * ddz) * A.length) / c2);
+                                            ((double)R.dz + pdz) * ddz) * A.length) * INV_C2);
         const double chi = sqrt(p[CX] * p[CX] + p[CY] * p[CY] + p[CZ] * p[CZ]);
         const double lambda = R.lambda + dl;
         const double s = R.s + sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
-        const double redshift = R.a0 / p[CA] * (1.0 + R.phi0c2 - nphi / c2) - 1.0;
+        const double redshift = R.a0 * recip(p[CA]) * (1.0 + R.phi0c2 - nphi * INV_C2) - 1.0;
         bool provisional = false;
         if (cont) {
             // a photon that the final pop could remove: keep its predecessor as the provisional output
