@@ -44,20 +44,12 @@ __device__ __forceinline__ double cell_center(unsigned L, uint32_t k) {
     return (-EXTENT / 2.0) + (EXTENT * ((double)(2ull * k + 1ull) / (double)(1ull << (L + 1))));
 }
 
-// Exact replacement of upper_bound(key(L, k)) - 1 followed by containing():
-// returns the level-sorted cell index, or NIL when the element found does not
-// contain the key.  *flvl receives the level of the cell found.
-__device__ __forceinline__ uint32_t find_cell(const DevTree& t, unsigned L, uint32_t kx, uint32_t ky, uint32_t kz,
-                                              unsigned* flvl) {
-    if (L < t.B) return NIL;
-    const unsigned sh = L - t.B;
-    const uint32_t bx = (uint32_t)((uint64_t)kx >> sh) - t.bx0;
-    const uint32_t by = (uint32_t)((uint64_t)ky >> sh) - t.by0;
-    const uint32_t bz = (uint32_t)((uint64_t)kz >> sh) - t.bz0;
-    if (bx >= t.bnx || by >= t.bny || bz >= t.bnz) return NIL;
-    uint32_t g = t.base[((uint64_t)bz * t.bny + by) * t.bnx + bx];
-    if (g == NIL) return NIL;
-    unsigned lvl = t.B;
+// The descent of find_cell below from cell g, stored at level lvl, whose
+// region contains the level-L key (kx, ky, kz): the finest stored cell that
+// contains the key, or NIL when the element upper_bound would find does not
+// contain it.  *flvl receives the level of the cell found.
+__device__ __forceinline__ uint32_t descend_cell(const DevTree& t, uint32_t g, unsigned lvl, unsigned L, uint32_t kx,
+                                                 uint32_t ky, uint32_t kz, unsigned* flvl) {
     while (lvl < L) {
         const unsigned m = t.child_mask[g];
         const unsigned b = L - lvl - 1;
@@ -74,6 +66,28 @@ __device__ __forceinline__ uint32_t find_cell(const DevTree& t, unsigned L, uint
     }
     *flvl = lvl;
     return g;
+}
+
+// cell of the dense base table (level B) whose region holds the level-L
+// key, or NIL
+__device__ __forceinline__ uint32_t base_cell(const DevTree& t, unsigned L, uint32_t kx, uint32_t ky, uint32_t kz) {
+    if (L < t.B) return NIL;
+    const unsigned sh = L - t.B;
+    const uint32_t bx = (uint32_t)((uint64_t)kx >> sh) - t.bx0;
+    const uint32_t by = (uint32_t)((uint64_t)ky >> sh) - t.by0;
+    const uint32_t bz = (uint32_t)((uint64_t)kz >> sh) - t.bz0;
+    if (bx >= t.bnx || by >= t.bny || bz >= t.bnz) return NIL;
+    return t.base[((uint64_t)bz * t.bny + by) * t.bnx + bx];
+}
+
+// Exact replacement of upper_bound(key(L, k)) - 1 followed by containing():
+// returns the level-sorted cell index, or NIL when the element found does not
+// contain the key.  *flvl receives the level of the cell found.
+__device__ __forceinline__ uint32_t find_cell(const DevTree& t, unsigned L, uint32_t kx, uint32_t ky, uint32_t kz,
+                                              unsigned* flvl) {
+    const uint32_t g = base_cell(t, L, kx, ky, kz);
+    if (g == NIL) return NIL;
+    return descend_cell(t, g, t.B, L, kx, ky, kz, flvl);
 }
 
 __device__ __forceinline__ uint32_t locate_point(const DevTree& t, double x, double y, double z, unsigned* flvl) {

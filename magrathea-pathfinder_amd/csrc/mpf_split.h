@@ -13,6 +13,8 @@ struct SplitTree {
     uint64_t nslots = 0;
     uint32_t* tslot = nullptr;        // cell -> stencil block slot, NIL when the stencil is missing
     unsigned char* blocks = nullptr;  // per slot: FP64 centre, 1/size, 5 x 27 FP32 coefficients
+    uint2* nbr = nullptr;             // cell x 6 faces -> same-level neighbour (see neighbour_kernel)
+    size_t nbr_bytes = 0;
 };
 
 // level/kx/ky/kz: host arrays of the level-sorted cells

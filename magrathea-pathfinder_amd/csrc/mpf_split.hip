@@ -52,7 +52,17 @@ constexpr uint32_t SLOT_MASK = 0x7FFFFFFFu;
 struct SplitView {
     const uint32_t* tslot;        // cell -> block slot (| PARTIAL_BIT) or NIL
     const unsigned char* blocks;  // slot -> BLOCK_BYTES
+    const uint2* nbr;             // cell x 6 faces -> neighbour entry
 };
+
+// Face-neighbour entries (neighbour_kernel): for cell g at level l and face
+// d = 2 * axis + (0 for -1, 1 for +1), the level-l cell h across that face:
+//   {h, tslot word}  h is stored at level l without children (word < PARTIAL_BIT)
+//   {h, NBR_DESCEND} h is stored at level l with children: the lookup
+//                    continues the descent from h
+//   {NIL, NIL}       anything else (a coarser neighbour, outside the tree or
+//                    the domain, no stencil): the lookup starts from the root
+constexpr uint32_t NBR_DESCEND = PARTIAL_BIT;
 
 // -------------------------------------------------------------- build
 
@@ -62,6 +72,29 @@ __global__ void partial_kernel(DevTree t, uint32_t* __restrict__ tslot) {
     if (g >= t.n) return;
     const unsigned m = t.child_mask[g];
     if (m != 0u && tslot[g] != NIL) tslot[g] |= PARTIAL_BIT;
+}
+
+__global__ void neighbour_kernel(DevTree t, const uint8_t* __restrict__ lev, const uint32_t* __restrict__ kx,
+                                 const uint32_t* __restrict__ ky, const uint32_t* __restrict__ kz,
+                                 const uint32_t* __restrict__ tslot, uint2* __restrict__ nbr) {
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= t.n) return;
+    const unsigned l = lev[g];
+    const long long lim = (long long)((1ull << l) - 1ull);
+    for (unsigned d = 0; d < 6; ++d) {
+        long long c[3] = {(long long)kx[g], (long long)ky[g], (long long)kz[g]};
+        c[d >> 1] += (d & 1u) ? 1 : -1;
+        uint2 e = make_uint2(NIL, NIL);
+        if (c[d >> 1] >= 0 && c[d >> 1] <= lim) {
+            unsigned fl = 0;
+            const uint32_t h = find_cell(t, l, (uint32_t)c[0], (uint32_t)c[1], (uint32_t)c[2], &fl);
+            if (h != NIL && fl == l) {
+                if (t.child_mask[h] != 0u) e = make_uint2(h, NBR_DESCEND);
+                else if (tslot[h] != NIL) e = make_uint2(h, tslot[h]);
+            }
+        }
+        nbr[6 * g + d] = e;
+    }
 }
 
 __global__ void target_kernel(DevTree t, const uint8_t* __restrict__ lev, const uint32_t* __restrict__ kx,
@@ -185,6 +218,7 @@ struct Interp {
     uint32_t kx, ky, kz;
     bool valid;          // the last point had data
     bool reuse;          // ... and its cell stores no children: every point inside has the same finest cell
+    uint32_t cell;       // ... that cell
     uint32_t slot;
     double cx, cy, cz, inv;
     float c[RC ? NFIELD * NCOEF : 1];
@@ -225,12 +259,47 @@ __device__ __forceinline__ bool split_eval(const DevTree& t, const SplitView& sv
     if (!same) {
         unsigned fl = 0;
         const uint32_t kx32 = axis_coord(x, 32), ky32 = axis_coord(y, 32), kz32 = axis_coord(z, 32);
-        const uint32_t g = find_cell(t, 32, kx32, ky32, kz32, &fl);
-        const uint32_t word = (g == NIL) ? NIL : sv.tslot[g];
+        uint32_t g = NIL, word = NIL, from = NIL;
+        unsigned from_lvl = t.B;
+        bool have_word = false;
+        if (I.valid) {
+            // the last cell's region or one face across from it: the descent
+            // starts from the last cell or its neighbour entry instead of the
+            // root, and ends in the same cell as the root descent
+            const unsigned sh = 32u - I.lvl;
+            const uint32_t ax = (uint32_t)((uint64_t)kx32 >> sh) - I.kx + 1u;
+            const uint32_t ay = (uint32_t)((uint64_t)ky32 >> sh) - I.ky + 1u;
+            const uint32_t az = (uint32_t)((uint64_t)kz32 >> sh) - I.kz + 1u;
+            if (ax <= 2u && ay <= 2u && az <= 2u) {
+                const unsigned moved = (ax != 1u) + (ay != 1u) + (az != 1u);
+                if (moved == 0u) {
+                    from = I.cell;
+                    from_lvl = I.lvl;
+                } else if (moved == 1u && sv.nbr) {
+                    const unsigned d = (ax != 1u) ? (ax >> 1) : (ay != 1u) ? 2u + (ay >> 1) : 4u + (az >> 1);
+                    const uint2 e = sv.nbr[(uint64_t)I.cell * 6u + d];
+                    if (e.y < PARTIAL_BIT) {
+                        g = e.x;
+                        fl = I.lvl;
+                        word = e.y;
+                        have_word = true;
+                    } else if (e.y == NBR_DESCEND) {
+                        from = e.x;
+                        from_lvl = I.lvl;
+                    }
+                }
+            }
+        }
+        if (!have_word) {
+            if (from == NIL) from = base_cell(t, 32, kx32, ky32, kz32);
+            if (from != NIL) g = descend_cell(t, from, from_lvl, 32, kx32, ky32, kz32, &fl);
+            word = (g == NIL) ? NIL : sv.tslot[g];
+        }
         if (word == NIL) {
             I.valid = false;
             return false;
         }
+        I.cell = g;
         const uint32_t slot = word & SLOT_MASK;
         I.lvl = fl;
         I.kx = kx32 >> (32 - fl);
@@ -790,20 +859,11 @@ bool split_build(SplitTree& s, const DevTree& t, const uint8_t* level, const uin
     std::vector<uint32_t> tslot(n);
     for (uint64_t g = 0; g < n; ++g) tslot[g] = (target[g] == NIL) ? NIL : slot_of[target[g]];
     s.nslots = ns;
-    s.bytes = n * 4 + std::max<uint64_t>(ns, 1) * BLOCK_BYTES;
+    s.bytes = ((n * 4 + 255) / 256) * 256 + std::max<uint64_t>(ns, 1) * BLOCK_BYTES;
     if (!chk(hipMalloc(&s.mem, s.bytes), "alloc blocks") ||
         !chk(hipMalloc((void**)&d_slot_cell, 4 * std::max<uint64_t>(ns, 1)), "alloc")) {
         cleanup();
         split_release(s);
-        return false;
-    }
-    s.tslot = reinterpret_cast<uint32_t*>(s.mem);
-    s.blocks = reinterpret_cast<unsigned char*>(s.mem) + ((n * 4 + 255) / 256) * 256;
-    s.bytes = ((n * 4 + 255) / 256) * 256 + std::max<uint64_t>(ns, 1) * BLOCK_BYTES;
-    (void)hipFree(s.mem);
-    s.mem = nullptr;
-    if (!chk(hipMalloc(&s.mem, s.bytes), "alloc blocks")) {
-        cleanup();
         return false;
     }
     s.tslot = reinterpret_cast<uint32_t*>(s.mem);
@@ -814,6 +874,19 @@ bool split_build(SplitTree& s, const DevTree& t, const uint8_t* level, const uin
     if (ok && n) {
         hipLaunchKernelGGL(partial_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, t, s.tslot);
         ok = chk(hipGetLastError(), "partial-oct flags");
+    }
+    if (ok && n) {
+        // the face-neighbour table is an accelerator: without the memory for
+        // it every lookup of a new cell starts from the root
+        if (hipMalloc((void**)&s.nbr, 6 * sizeof(uint2) * n) == hipSuccess) {
+            s.nbr_bytes = 6 * sizeof(uint2) * n;
+            hipLaunchKernelGGL(neighbour_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, t, d_lev, d_kx,
+                               d_ky, d_kz, s.tslot, s.nbr);
+            ok = chk(hipGetLastError(), "neighbour table");
+        } else {
+            (void)hipGetLastError();
+            s.nbr = nullptr;
+        }
     }
     if (ok && ns) {
         hipLaunchKernelGGL(block_kernel, dim3((unsigned)((ns + 127) / 128)), dim3(128), 0, st, t, d_lev, d_kx, d_ky,
@@ -830,13 +903,14 @@ bool split_supports_surfaces() { return true; }
 
 void split_release(SplitTree& s) {
     if (s.mem) (void)hipFree(s.mem);
+    if (s.nbr) (void)hipFree(s.nbr);
     s = SplitTree();
 }
 
 hipError_t launch_integrate_split(const IntegrateArgs& A, const DevTree& t, const SplitTree& s, int cus,
                                   unsigned blocks_per_cu, unsigned waves_per_simd, hipStream_t st,
                                   hipEvent_t ev0, hipEvent_t ev1) {
-    SplitView sv{s.tslot, s.blocks};
+    SplitView sv{s.tslot, s.blocks, s.nbr};
     const uint64_t blocks = (A.n + 255) / 256;
     const int grid = (int)std::min<uint64_t>(blocks, (uint64_t)cus * blocks_per_cu);
     const unsigned pgrid = (unsigned)((A.n + 255) / 256);
@@ -869,7 +943,7 @@ hipError_t launch_integrate_split(const IntegrateArgs& A, const DevTree& t, cons
 namespace mpf {
 hipError_t launch_features_split(const DevTree& t, const SplitTree& s, const double* src, uint64_t n, double length,
                                  float* feat, hipStream_t st) {
-    SplitView sv{s.tslot, s.blocks};
+    SplitView sv{s.tslot, s.blocks, s.nbr};
     if (n)
         hipLaunchKernelGGL(features_split_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, t, sv, src, n,
                            length, feat);
