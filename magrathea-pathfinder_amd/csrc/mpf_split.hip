@@ -247,54 +247,75 @@ __device__ __forceinline__ void load_block(const SplitView& sv, uint32_t slot, I
     I.slot = slot;
 }
 
+// axis_coord(x, 32): inside the domain (0 <= v < 2^32) the truncation is one
+// conversion; outside it the full signed truncation and wrap of axis_coord
+__device__ __forceinline__ uint32_t axis_key32(double x) {
+    const double v = ((1.0 / EXTENT) * 4294967296.0) * (x - (-EXTENT / 2.0));
+    if (v >= 0.0 && v < 4294967296.0) return (uint32_t)v;
+    return axis_coord(x, 32);
+}
+
+// Tslot word of the finest cell of the level-32 key (kx32, ky32, kz32)
+// (NIL: no data), with the cell in *g and its level in *fl.  (ax, ay, az) is
+// the key's level-l prefix minus the last cell's coordinates plus one, when
+// the lane has a last cell (valid); the lookup then starts from that cell or
+// its face neighbour when the key lies in their regions.
+__device__ __forceinline__ uint32_t lookup_near(const DevTree& t, const SplitView& sv, bool valid, uint32_t cell,
+                                                unsigned lvl, uint32_t ax, uint32_t ay, uint32_t az, uint32_t kx32,
+                                                uint32_t ky32, uint32_t kz32, uint32_t* g, unsigned* fl) {
+    uint32_t from = NIL;
+    unsigned from_lvl = t.B;
+    *g = NIL;
+    if (valid && ax <= 2u && ay <= 2u && az <= 2u) {
+        const unsigned moved = (ax != 1u) + (ay != 1u) + (az != 1u);
+        if (moved == 0u) {
+            from = cell;
+            from_lvl = lvl;
+        } else if (moved == 1u && sv.nbr) {
+            const unsigned d = (ax != 1u) ? (ax >> 1) : (ay != 1u) ? 2u + (ay >> 1) : 4u + (az >> 1);
+            const uint2 e = sv.nbr[(uint64_t)cell * 6u + d];
+            if (e.y < PARTIAL_BIT) {
+                *g = e.x;
+                *fl = lvl;
+                return e.y;
+            }
+            if (e.y == NBR_DESCEND) {
+                from = e.x;
+                from_lvl = lvl;
+            }
+        }
+    }
+    if (from == NIL) from = base_cell(t, 32, kx32, ky32, kz32);
+    if (from != NIL) *g = descend_cell(t, from, from_lvl, 32, kx32, ky32, kz32, fl);
+    return (*g == NIL) ? NIL : sv.tslot[*g];
+}
+
 // Fields at (x, y, z): out[0..3] = dphidt, dphidx, dphidy, dphidz; out[4] = phi
 // when NF == 5.  Returns false when the reference would return Data().
+//
+// The finest cell is the one find_cell returns for the level-32 key of the
+// point.  A lane keeps its last cell (level l, coordinates k): a key whose
+// level-l prefix is k lies in that cell's region, and one whose prefix is
+// k +- 1 along one axis in the region of the neighbour across that face.
+// The lookup then starts from the last cell (it stores no children: it is
+// the answer; else the descent continues from it) or from the neighbour
+// entry, and ends in the cell the root descent finds.
 template <int NF, bool RC>
 __device__ __forceinline__ bool split_eval(const DevTree& t, const SplitView& sv, Interp<RC>& I, double x, double y,
                                           double z, float* out) {
-    bool same = false;
-    if (I.valid && I.reuse) {
-        same = axis_coord(x, I.lvl) == I.kx && axis_coord(y, I.lvl) == I.ky && axis_coord(z, I.lvl) == I.kz;
+    const uint32_t kx32 = axis_key32(x), ky32 = axis_key32(y), kz32 = axis_key32(z);
+    uint32_t ax = 0u, ay = 0u, az = 0u;
+    if (I.valid) {
+        const unsigned sh = 32u - I.lvl;
+        ax = (uint32_t)((uint64_t)kx32 >> sh) - I.kx + 1u;
+        ay = (uint32_t)((uint64_t)ky32 >> sh) - I.ky + 1u;
+        az = (uint32_t)((uint64_t)kz32 >> sh) - I.kz + 1u;
     }
+    const bool same = I.valid && I.reuse && ax == 1u && ay == 1u && az == 1u;
     if (!same) {
         unsigned fl = 0;
-        const uint32_t kx32 = axis_coord(x, 32), ky32 = axis_coord(y, 32), kz32 = axis_coord(z, 32);
-        uint32_t g = NIL, word = NIL, from = NIL;
-        unsigned from_lvl = t.B;
-        bool have_word = false;
-        if (I.valid) {
-            // the last cell's region or one face across from it: the descent
-            // starts from the last cell or its neighbour entry instead of the
-            // root, and ends in the same cell as the root descent
-            const unsigned sh = 32u - I.lvl;
-            const uint32_t ax = (uint32_t)((uint64_t)kx32 >> sh) - I.kx + 1u;
-            const uint32_t ay = (uint32_t)((uint64_t)ky32 >> sh) - I.ky + 1u;
-            const uint32_t az = (uint32_t)((uint64_t)kz32 >> sh) - I.kz + 1u;
-            if (ax <= 2u && ay <= 2u && az <= 2u) {
-                const unsigned moved = (ax != 1u) + (ay != 1u) + (az != 1u);
-                if (moved == 0u) {
-                    from = I.cell;
-                    from_lvl = I.lvl;
-                } else if (moved == 1u && sv.nbr) {
-                    const unsigned d = (ax != 1u) ? (ax >> 1) : (ay != 1u) ? 2u + (ay >> 1) : 4u + (az >> 1);
-                    const uint2 e = sv.nbr[(uint64_t)I.cell * 6u + d];
-                    if (e.y < PARTIAL_BIT) {
-                        g = e.x;
-                        fl = I.lvl;
-                        word = e.y;
-                        have_word = true;
-                    } else if (e.y == NBR_DESCEND) {
-                        from = e.x;
-                        from_lvl = I.lvl;
-                    }
-                }
-            }
-        }
-        if (!have_word) {
-            if (from == NIL) from = base_cell(t, 32, kx32, ky32, kz32);
-            if (from != NIL) g = descend_cell(t, from, from_lvl, 32, kx32, ky32, kz32, &fl);
-            word = (g == NIL) ? NIL : sv.tslot[g];
-        }
+        uint32_t g;
+        const uint32_t word = lookup_near(t, sv, I.valid, I.cell, I.lvl, ax, ay, az, kx32, ky32, kz32, &g, &fl);
         if (word == NIL) {
             I.valid = false;
             return false;
@@ -302,9 +323,9 @@ __device__ __forceinline__ bool split_eval(const DevTree& t, const SplitView& sv
         I.cell = g;
         const uint32_t slot = word & SLOT_MASK;
         I.lvl = fl;
-        I.kx = kx32 >> (32 - fl);
-        I.ky = ky32 >> (32 - fl);
-        I.kz = kz32 >> (32 - fl);
+        I.kx = (uint32_t)((uint64_t)kx32 >> (32 - fl));
+        I.ky = (uint32_t)((uint64_t)ky32 >> (32 - fl));
+        I.kz = (uint32_t)((uint64_t)kz32 >> (32 - fl));
         I.valid = true;
         // a cell with some children stored (a partial oct, e.g. at a cone's
         // edge) is the finest cell only of the points outside those children
@@ -335,28 +356,38 @@ __device__ __forceinline__ bool split_eval(const DevTree& t, const SplitView& sv
 
 // Jacobian samples: grad phi components 1..NG (dphidx, dphidy, dphidz) at
 // (x, y, z), with the sample's finest cell cached in LDS (one entry per lane
-// and sample direction: cell coordinates + block slot, level | reuse flag).
-// The samples sit one cell away from the ray along each axis and stay in
-// the same neighbour cell for about as many steps as the ray stays in its
-// own; a hit costs three coordinate compares instead of a tree descent.
+// and sample direction: cell coordinates + block slot, the cell, and its
+// level | JC_REUSE | JC_VALID).  The samples sit one cell away from the ray
+// along each axis and stay in the same neighbour cell for about as many steps
+// as the ray stays in its own; a hit costs three key compares, a miss starts
+// from the cached cell's face neighbour (lookup_near).
+constexpr uint32_t JC_REUSE = 0x100u, JC_VALID = 0x200u;
+
 template <int NG>
-__device__ __forceinline__ bool jac_eval(const DevTree& t, const SplitView& sv, uint4& cell, uint32_t& lw, double x,
-                                         double y, double z, float* out) {
+__device__ __forceinline__ bool jac_eval(const DevTree& t, const SplitView& sv, uint4& cell, uint32_t& cid,
+                                         uint32_t& lw, double x, double y, double z, float* out) {
+    const uint32_t kx32 = axis_key32(x), ky32 = axis_key32(y), kz32 = axis_key32(z);
     const unsigned l0 = lw & 0x3Fu;
+    const unsigned sh = 32u - l0;
     uint4 c = cell;
-    bool hit = (lw & 0x100u) && axis_coord(x, l0) == c.x && axis_coord(y, l0) == c.y && axis_coord(z, l0) == c.z;
+    const uint32_t ax = (uint32_t)((uint64_t)kx32 >> sh) - c.x + 1u;
+    const uint32_t ay = (uint32_t)((uint64_t)ky32 >> sh) - c.y + 1u;
+    const uint32_t az = (uint32_t)((uint64_t)kz32 >> sh) - c.z + 1u;
+    const bool hit = (lw & JC_REUSE) && ax == 1u && ay == 1u && az == 1u;
     if (!hit) {
         unsigned fl = 0;
-        const uint32_t kx32 = axis_coord(x, 32), ky32 = axis_coord(y, 32), kz32 = axis_coord(z, 32);
-        const uint32_t g = find_cell(t, 32, kx32, ky32, kz32, &fl);
-        const uint32_t word = (g == NIL) ? NIL : sv.tslot[g];
+        uint32_t g;
+        const uint32_t word = lookup_near(t, sv, (lw & JC_VALID) != 0u, cid, l0, ax, ay, az, kx32, ky32, kz32, &g,
+                                          &fl);
         if (word == NIL) {
             lw = 0u;
             return false;
         }
-        c = make_uint4(kx32 >> (32 - fl), ky32 >> (32 - fl), kz32 >> (32 - fl), word & SLOT_MASK);
+        c = make_uint4((uint32_t)((uint64_t)kx32 >> (32 - fl)), (uint32_t)((uint64_t)ky32 >> (32 - fl)),
+                       (uint32_t)((uint64_t)kz32 >> (32 - fl)), word & SLOT_MASK);
         cell = c;
-        lw = fl | ((word & PARTIAL_BIT) ? 0u : 0x100u);
+        cid = g;
+        lw = fl | JC_VALID | ((word & PARTIAL_BIT) ? 0u : JC_REUSE);
     }
     const unsigned char* blk = sv.blocks + (uint64_t)c.w * BLOCK_BYTES;
     const double4 h = *reinterpret_cast<const double4*>(blk);
@@ -388,15 +419,34 @@ __device__ __forceinline__ double cosmo_lin(double x0, const double* __restrict_
 }
 
 // da/dt of the table at x0 with the segment slopes computed once per table
-// (no division per call; the exact tiers keep Utility::interpolate's)
+// (no division per call; the exact tiers keep Utility::interpolate's).  The
+// segment of the last call is kept in registers: the hint walk ends at
+// upper_bound(x0) whatever the hint, so a point inside that segment takes it
+// without reading the table.
+struct CosmoSeg {
+    double lo, hi, y, slope;
+    int hint;
+};
+
+__device__ __forceinline__ void cosmo_seg_init(CosmoSeg& S) {
+    S.lo = 1.0;
+    S.hi = 0.0;  // empty: the first call walks
+    S.hint = 0;
+}
+
 __device__ __forceinline__ double cosmo_slope(double x0, const double* __restrict__ x, const double* __restrict__ y,
-                                              const double* __restrict__ slope, int n, int& hint) {
-    int i = hint;
+                                              const double* __restrict__ slope, int n, CosmoSeg& S) {
+    if (!(x0 < S.lo) && x0 < S.hi) return fma(S.slope, x0 - S.lo, S.y);
+    int i = S.hint;
     while (i < n && !(x0 < x[i])) ++i;
     while (i > 0 && x0 < x[i - 1]) --i;
-    hint = i;
+    S.hint = i;
     i += (i <= 0) - (i >= n);
-    return fma(slope[i], x0 - x[i - 1], y[i - 1]);
+    S.lo = x[i - 1];
+    S.hi = x[i];
+    S.y = y[i - 1];
+    S.slope = slope[i];
+    return fma(S.slope, x0 - S.lo, S.y);
 }
 
 // 1 / a: the hardware reciprocal refined by one Newton step (within an ulp
@@ -442,7 +492,7 @@ struct SplitRay {
     float phi, dx, dy, dz, dt;   // fields at the current point (dt: +dphidt at launch, -dphidt after)
     float F[4];                  // interpolated dphidt, dphidx, dphidy, dphidz at the current point
     uint32_t appended;
-    int hint;
+    CosmoSeg cs;
     double stopv;     // this ray's stop value (and its surface when per-ray)
 };
 
@@ -520,6 +570,7 @@ integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
     // cells of the 6 Jacobian samples of each lane (jac_eval)
     __shared__ uint4 jcell[SURF ? 6 : 1][SURF ? 256 : 1];
     __shared__ uint32_t jlw[SURF ? 6 : 1][SURF ? 256 : 1];
+    __shared__ uint32_t jcid[SURF ? 6 : 1][SURF ? 256 : 1];
     if (SURF) {
         if (threadIdx.x == 0) {
 #pragma unroll
@@ -550,7 +601,7 @@ integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
                     for (int k = 0; k < 9; ++k) R.core[k] = L[k];
                     const bool empty = L[9] != 0.0;
                     const double phi = L[10];
-                    R.hint = 0;
+                    cosmo_seg_init(R.cs);
                     R.lambda = R.s = R.isw = R.iswold = R.redshift = R.chi = 0.0;
                     R.a0 = R.core[CA];
                     R.phi0c2 = phi / c2;
@@ -585,26 +636,26 @@ integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
         // ---------------------------------------------------------- step
         const double dl = R.dl;
         double ksum[9], kin[9], kout[9];
-        double dadt = cosmo_slope(R.core[CT], A.ct, A.cdadt, A.cslope, A.ncos, R.hint);
+        double dadt = cosmo_slope(R.core[CT], A.ct, A.cdadt, A.cslope, A.ncos, R.cs);
         rhs_split(R.core, R.F, dadt, inv_scale, kout);
 #pragma unroll
         for (int i = 0; i < 9; ++i) { ksum[i] = kout[i]; kin[i] = R.core[i] + (0.5 * dl) * kout[i]; }
         float F[5];
         bool ok = split_eval<4, RC>(t, sv, I, kin[CX], kin[CY], kin[CZ], F);
         if (!ok) { F[0] = F[1] = F[2] = F[3] = 0.f; }
-        dadt = cosmo_slope(kin[CT], A.ct, A.cdadt, A.cslope, A.ncos, R.hint);
+        dadt = cosmo_slope(kin[CT], A.ct, A.cdadt, A.cslope, A.ncos, R.cs);
         rhs_split(kin, F, dadt, inv_scale, kout);
 #pragma unroll
         for (int i = 0; i < 9; ++i) { ksum[i] += 2.0 * kout[i]; kin[i] = R.core[i] + (0.5 * dl) * kout[i]; }
         ok = split_eval<4, RC>(t, sv, I, kin[CX], kin[CY], kin[CZ], F);
         if (!ok) { F[0] = F[1] = F[2] = F[3] = 0.f; }
-        dadt = cosmo_slope(kin[CT], A.ct, A.cdadt, A.cslope, A.ncos, R.hint);
+        dadt = cosmo_slope(kin[CT], A.ct, A.cdadt, A.cslope, A.ncos, R.cs);
         rhs_split(kin, F, dadt, inv_scale, kout);
 #pragma unroll
         for (int i = 0; i < 9; ++i) { ksum[i] += 2.0 * kout[i]; kin[i] = R.core[i] + dl * kout[i]; }
         ok = split_eval<4, RC>(t, sv, I, kin[CX], kin[CY], kin[CZ], F);
         if (!ok) { F[0] = F[1] = F[2] = F[3] = 0.f; }
-        dadt = cosmo_slope(kin[CT], A.ct, A.cdadt, A.cslope, A.ncos, R.hint);
+        dadt = cosmo_slope(kin[CT], A.ct, A.cdadt, A.cslope, A.ncos, R.cs);
         rhs_split(kin, F, dadt, inv_scale, kout);
         double p[9];
 #pragma unroll
@@ -662,18 +713,18 @@ integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
                     double D[6];
                     float gm[3], gp[3];
                     // x pair: d/dx of dphidx
-                    if (!jac_eval<1>(t, sv, jcell[0][tl], jlw[0][tl], p[CX] - hj, p[CY], p[CZ], gm)) gm[0] = 0.f;
-                    if (!jac_eval<1>(t, sv, jcell[1][tl], jlw[1][tl], p[CX] + hj, p[CY], p[CZ], gp)) gp[0] = 0.f;
+                    if (!jac_eval<1>(t, sv, jcell[0][tl], jcid[0][tl], jlw[0][tl], p[CX] - hj, p[CY], p[CZ], gm)) gm[0] = 0.f;
+                    if (!jac_eval<1>(t, sv, jcell[1][tl], jcid[1][tl], jlw[1][tl], p[CX] + hj, p[CY], p[CZ], gp)) gp[0] = 0.f;
                     D[0] = (double)gp[0] - (double)gm[0];
                     // y pair: d/dy of dphidx, dphidy
-                    if (!jac_eval<2>(t, sv, jcell[2][tl], jlw[2][tl], p[CX], p[CY] - hj, p[CZ], gm)) gm[0] = gm[1] = 0.f;
-                    if (!jac_eval<2>(t, sv, jcell[3][tl], jlw[3][tl], p[CX], p[CY] + hj, p[CZ], gp)) gp[0] = gp[1] = 0.f;
+                    if (!jac_eval<2>(t, sv, jcell[2][tl], jcid[2][tl], jlw[2][tl], p[CX], p[CY] - hj, p[CZ], gm)) gm[0] = gm[1] = 0.f;
+                    if (!jac_eval<2>(t, sv, jcell[3][tl], jcid[3][tl], jlw[3][tl], p[CX], p[CY] + hj, p[CZ], gp)) gp[0] = gp[1] = 0.f;
                     D[1] = (double)gp[0] - (double)gm[0];
                     D[3] = (double)gp[1] - (double)gm[1];
                     // z pair: d/dz of dphidx, dphidy, dphidz
-                    if (!jac_eval<3>(t, sv, jcell[4][tl], jlw[4][tl], p[CX], p[CY], p[CZ] - hj, gm))
+                    if (!jac_eval<3>(t, sv, jcell[4][tl], jcid[4][tl], jlw[4][tl], p[CX], p[CY], p[CZ] - hj, gm))
                         gm[0] = gm[1] = gm[2] = 0.f;
-                    if (!jac_eval<3>(t, sv, jcell[5][tl], jlw[5][tl], p[CX], p[CY], p[CZ] + hj, gp))
+                    if (!jac_eval<3>(t, sv, jcell[5][tl], jcid[5][tl], jlw[5][tl], p[CX], p[CY], p[CZ] + hj, gp))
                         gp[0] = gp[1] = gp[2] = 0.f;
                     D[2] = (double)gp[0] - (double)gm[0];
                     D[4] = (double)gp[1] - (double)gm[1];
