@@ -339,14 +339,23 @@ __device__ __forceinline__ bool split_eval(const DevTree& t, const SplitView& sv
 #pragma unroll
         for (int f = 0; f < NF; ++f) out[f] = horner27(I.c + f * NCOEF, u, v, w);
     } else {
+        // the coefficients of field f + 1 are in flight while field f is
+        // evaluated: two round trips to L2 per evaluation instead of NF
         const float4* cf = reinterpret_cast<const float4*>(sv.blocks + (uint64_t)I.slot * BLOCK_BYTES + BLOCK_DOUBLES * 8);
+        float4 q[2][FSTRIDE / 4];
+#pragma unroll
+        for (int i = 0; i < FSTRIDE / 4; ++i) q[0][i] = cf[i];
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
+            if (f + 1 < NF) {
+#pragma unroll
+                for (int i = 0; i < FSTRIDE / 4; ++i) q[(f + 1) & 1][i] = cf[(f + 1) * (FSTRIDE / 4) + i];
+            }
             float c[FSTRIDE];
 #pragma unroll
             for (int i = 0; i < FSTRIDE / 4; ++i) {
-                const float4 q = cf[f * (FSTRIDE / 4) + i];
-                c[4 * i] = q.x; c[4 * i + 1] = q.y; c[4 * i + 2] = q.z; c[4 * i + 3] = q.w;
+                const float4 e = q[f & 1][i];
+                c[4 * i] = e.x; c[4 * i + 1] = e.y; c[4 * i + 2] = e.z; c[4 * i + 3] = e.w;
             }
             out[f] = horner27(c, u, v, w);
         }
