@@ -49,6 +49,16 @@ constexpr int BLOCK_BYTES = BLOCK_DOUBLES * 8 + NFIELD * FSTRIDE * 4;  // 592
 constexpr uint32_t PARTIAL_BIT = 0x80000000u;
 constexpr uint32_t SLOT_MASK = 0x7FFFFFFFu;
 
+// the stencil blocks through the constant address space: loads at an
+// address uniform across the wave become scalar loads (the blocks are not
+// written while the integrators run)
+typedef const __attribute__((address_space(4))) float ConstF;
+// blocks per evaluation the integrator reads with scalar loads
+#ifndef MPF_SCALAR_BLOCKS
+#define MPF_SCALAR_BLOCKS 1
+#endif
+constexpr int SCALAR_BLOCKS = MPF_SCALAR_BLOCKS;
+
 struct SplitView {
     const uint32_t* tslot;        // cell -> block slot (| PARTIAL_BIT) or NIL
     const unsigned char* blocks;  // slot -> BLOCK_BYTES
@@ -300,7 +310,8 @@ __device__ __forceinline__ uint32_t lookup_near(const DevTree& t, const SplitVie
 // The lookup then starts from the last cell (it stores no children: it is
 // the answer; else the descent continues from it) or from the neighbour
 // entry, and ends in the cell the root descent finds.
-template <int NF, bool RC>
+// SB: distinct blocks of the wave read with scalar loads (0: vector loads only)
+template <int NF, bool RC, int SB = 0>
 __device__ __forceinline__ bool split_eval(const DevTree& t, const SplitView& sv, Interp<RC>& I, double x, double y,
                                           double z, float* out) {
     const uint32_t kx32 = axis_key32(x), ky32 = axis_key32(y), kz32 = axis_key32(z);
@@ -339,25 +350,47 @@ __device__ __forceinline__ bool split_eval(const DevTree& t, const SplitView& sv
 #pragma unroll
         for (int f = 0; f < NF; ++f) out[f] = horner27(I.c + f * NCOEF, u, v, w);
     } else {
-        // the coefficients of field f + 1 are in flight while field f is
-        // evaluated: two round trips to L2 per evaluation instead of NF
-        const float4* cf = reinterpret_cast<const float4*>(sv.blocks + (uint64_t)I.slot * BLOCK_BYTES + BLOCK_DOUBLES * 8);
-        float4 q[2][FSTRIDE / 4];
+        // The wave walks the distinct blocks of its lanes (SB at most) with
+        // scalar loads of the coefficients: the vector memory path returns
+        // 1 KiB per wave and 16-byte load, and its address and data units
+        // bound the kernel when every lane reads its block.  The lanes of a
+        // ray patch mostly share one or two blocks.
+        unsigned long long pend = SB ? __ballot(true) : 0ull;
+        for (int it = 0; it < SB && pend; ++it) {
+            const uint32_t s0 = __builtin_amdgcn_readlane(I.slot, __builtin_ctzll(pend));
+            ConstF* cs = (ConstF*)(sv.blocks + (uint64_t)s0 * BLOCK_BYTES + BLOCK_DOUBLES * 8);
+            const bool mine = I.slot == s0;
 #pragma unroll
-        for (int i = 0; i < FSTRIDE / 4; ++i) q[0][i] = cf[i];
+            for (int f = 0; f < NF; ++f) {
+                float c[NCOEF];
 #pragma unroll
-        for (int f = 0; f < NF; ++f) {
-            if (f + 1 < NF) {
-#pragma unroll
-                for (int i = 0; i < FSTRIDE / 4; ++i) q[(f + 1) & 1][i] = cf[(f + 1) * (FSTRIDE / 4) + i];
+                for (int i = 0; i < NCOEF; ++i) c[i] = cs[f * FSTRIDE + i];
+                const float r = horner27(c, u, v, w);
+                out[f] = mine ? r : out[f];
             }
-            float c[FSTRIDE];
+            pend &= ~__ballot(mine);
+        }
+        if (!SB || ((pend >> __lane_id()) & 1ull)) {
+            // the coefficients of field f + 1 are in flight while field f is
+            // evaluated: two round trips to L2 per evaluation instead of NF
+            const float4* cf = reinterpret_cast<const float4*>(sv.blocks + (uint64_t)I.slot * BLOCK_BYTES + BLOCK_DOUBLES * 8);
+            float4 q[2][FSTRIDE / 4];
 #pragma unroll
-            for (int i = 0; i < FSTRIDE / 4; ++i) {
-                const float4 e = q[f & 1][i];
-                c[4 * i] = e.x; c[4 * i + 1] = e.y; c[4 * i + 2] = e.z; c[4 * i + 3] = e.w;
+            for (int i = 0; i < FSTRIDE / 4; ++i) q[0][i] = cf[i];
+#pragma unroll
+            for (int f = 0; f < NF; ++f) {
+                if (f + 1 < NF) {
+#pragma unroll
+                    for (int i = 0; i < FSTRIDE / 4; ++i) q[(f + 1) & 1][i] = cf[(f + 1) * (FSTRIDE / 4) + i];
+                }
+                float c[FSTRIDE];
+#pragma unroll
+                for (int i = 0; i < FSTRIDE / 4; ++i) {
+                    const float4 e = q[f & 1][i];
+                    c[4 * i] = e.x; c[4 * i + 1] = e.y; c[4 * i + 2] = e.z; c[4 * i + 3] = e.w;
+                }
+                out[f] = horner27(c, u, v, w);
             }
-            out[f] = horner27(c, u, v, w);
         }
     }
     return true;
@@ -650,19 +683,19 @@ integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
 #pragma unroll
         for (int i = 0; i < 9; ++i) { ksum[i] = kout[i]; kin[i] = R.core[i] + (0.5 * dl) * kout[i]; }
         float F[5];
-        bool ok = split_eval<4, RC>(t, sv, I, kin[CX], kin[CY], kin[CZ], F);
+        bool ok = split_eval<4, RC, SURF ? 0 : SCALAR_BLOCKS>(t, sv, I, kin[CX], kin[CY], kin[CZ], F);
         if (!ok) { F[0] = F[1] = F[2] = F[3] = 0.f; }
         dadt = cosmo_slope(kin[CT], A.ct, A.cdadt, A.cslope, A.ncos, R.cs);
         rhs_split(kin, F, dadt, inv_scale, kout);
 #pragma unroll
         for (int i = 0; i < 9; ++i) { ksum[i] += 2.0 * kout[i]; kin[i] = R.core[i] + (0.5 * dl) * kout[i]; }
-        ok = split_eval<4, RC>(t, sv, I, kin[CX], kin[CY], kin[CZ], F);
+        ok = split_eval<4, RC, SURF ? 0 : SCALAR_BLOCKS>(t, sv, I, kin[CX], kin[CY], kin[CZ], F);
         if (!ok) { F[0] = F[1] = F[2] = F[3] = 0.f; }
         dadt = cosmo_slope(kin[CT], A.ct, A.cdadt, A.cslope, A.ncos, R.cs);
         rhs_split(kin, F, dadt, inv_scale, kout);
 #pragma unroll
         for (int i = 0; i < 9; ++i) { ksum[i] += 2.0 * kout[i]; kin[i] = R.core[i] + dl * kout[i]; }
-        ok = split_eval<4, RC>(t, sv, I, kin[CX], kin[CY], kin[CZ], F);
+        ok = split_eval<4, RC, SURF ? 0 : SCALAR_BLOCKS>(t, sv, I, kin[CX], kin[CY], kin[CZ], F);
         if (!ok) { F[0] = F[1] = F[2] = F[3] = 0.f; }
         dadt = cosmo_slope(kin[CT], A.ct, A.cdadt, A.cslope, A.ncos, R.cs);
         rhs_split(kin, F, dadt, inv_scale, kout);
@@ -672,7 +705,7 @@ integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
         // fields at the new point
         const bool inside = !(p[CA] < 0.0) && p[CX] > -2.0 && p[CX] < 2.0 && p[CY] > -2.0 && p[CY] < 2.0 &&
                             p[CZ] > -2.0 && p[CZ] < 2.0;
-        bool cont = inside && split_eval<5, RC>(t, sv, I, p[CX], p[CY], p[CZ], F);
+        bool cont = inside && split_eval<5, RC, SURF ? 0 : SCALAR_BLOCKS>(t, sv, I, p[CX], p[CY], p[CZ], F);
         if (!cont) { F[0] = F[1] = F[2] = F[3] = F[4] = 0.f; }
         // extras (integrator.h:756-776) with FP64 accumulation
         const double nphi = (double)F[4];
