@@ -124,7 +124,7 @@ struct CompArgs {
     float* maps = nullptr;
 };
 
-__global__ void component_fill_kernel(CompArgs A) {
+__global__ void __launch_bounds__(256) component_fill_kernel(CompArgs A) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= A.n) return;
     const int64_t p = A.pix[i];

@@ -147,7 +147,7 @@ __device__ __forceinline__ double field_value(const DevTree& t, uint32_t g, int 
     return t.f64 ? t.f64[(uint64_t)slot * t.n + g] : (double)t.f32[(uint64_t)slot * t.n + g];
 }
 
-__global__ void block_kernel(DevTree t, const uint8_t* __restrict__ lev, const uint32_t* __restrict__ kx,
+__global__ void __launch_bounds__(128) block_kernel(DevTree t, const uint8_t* __restrict__ lev, const uint32_t* __restrict__ kx,
                              const uint32_t* __restrict__ ky, const uint32_t* __restrict__ kz,
                              const uint32_t* __restrict__ slot_cell, uint64_t nslots, unsigned char* blocks) {
     const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -216,6 +216,32 @@ __device__ __forceinline__ float horner27(const float* c, float u, float v, floa
         g[p] = __builtin_fmaf(__builtin_fmaf(h[2], v, h[1]), v, h[0]);
     }
     return __builtin_fmaf(__builtin_fmaf(g[2], u, g[1]), u, g[0]);
+}
+
+// sum_pqr C_pqr u^p v^q w^r with the coefficients in scalar registers and
+// m[3q + r] = v^q w^r: each product takes one coefficient operand (a VALU
+// instruction reads one scalar register), the constant terms are added last.
+// 29 instructions against Horner's 26 FMA plus the 9 copies a two-scalar
+// first FMA of each inner triple needs; the 8 products m are shared by the
+// fields.
+__device__ __forceinline__ float mono27(const float* c, const float* m, float u) {
+    float s[3];
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        const float* cp = c + 9 * p;
+        float a = cp[1] * m[1];
+#pragma unroll
+        for (int i = 2; i < 9; ++i) a = __builtin_fmaf(cp[i], m[i], a);
+        s[p] = a + cp[0];
+    }
+    return __builtin_fmaf(__builtin_fmaf(s[2], u, s[1]), u, s[0]);
+}
+
+__device__ __forceinline__ void monomials9(float v, float w, float* m) {
+    const float w2 = w * w, v2 = v * v;
+    m[0] = 1.0f; m[1] = w; m[2] = w2;
+    m[3] = v; m[4] = v * w; m[5] = v * w2;
+    m[6] = v2; m[7] = v2 * w; m[8] = v2 * w2;
 }
 
 // RC = true keeps the current coefficient block in registers; RC = false
@@ -346,9 +372,13 @@ __device__ __forceinline__ bool split_eval(const DevTree& t, const SplitView& sv
     const float u = (float)((x - I.cx) * I.inv);
     const float v = (float)((y - I.cy) * I.inv);
     const float w = (float)((z - I.cz) * I.inv);
+    // every path sums in the order of mono27, so that a lane's fields do not
+    // depend on which path its wave takes
     if (RC) {
+        float m[9];
+        monomials9(v, w, m);
 #pragma unroll
-        for (int f = 0; f < NF; ++f) out[f] = horner27(I.c + f * NCOEF, u, v, w);
+        for (int f = 0; f < NF; ++f) out[f] = mono27(I.c + f * NCOEF, m, u);
     } else {
         // The wave walks the distinct blocks of its lanes (SB at most) with
         // scalar loads of the coefficients: the vector memory path returns
@@ -360,12 +390,14 @@ __device__ __forceinline__ bool split_eval(const DevTree& t, const SplitView& sv
             const uint32_t s0 = __builtin_amdgcn_readlane(I.slot, __builtin_ctzll(pend));
             ConstF* cs = (ConstF*)(sv.blocks + (uint64_t)s0 * BLOCK_BYTES + BLOCK_DOUBLES * 8);
             const bool mine = I.slot == s0;
+            float m[9];
+            monomials9(v, w, m);
 #pragma unroll
             for (int f = 0; f < NF; ++f) {
                 float c[NCOEF];
 #pragma unroll
                 for (int i = 0; i < NCOEF; ++i) c[i] = cs[f * FSTRIDE + i];
-                const float r = horner27(c, u, v, w);
+                const float r = mono27(c, m, u);
                 out[f] = mine ? r : out[f];
             }
             pend &= ~__ballot(mine);
@@ -377,6 +409,8 @@ __device__ __forceinline__ bool split_eval(const DevTree& t, const SplitView& sv
             float4 q[2][FSTRIDE / 4];
 #pragma unroll
             for (int i = 0; i < FSTRIDE / 4; ++i) q[0][i] = cf[i];
+            float m[9];
+            monomials9(v, w, m);
 #pragma unroll
             for (int f = 0; f < NF; ++f) {
                 if (f + 1 < NF) {
@@ -389,7 +423,7 @@ __device__ __forceinline__ bool split_eval(const DevTree& t, const SplitView& sv
                     const float4 e = q[f & 1][i];
                     c[4 * i] = e.x; c[4 * i + 1] = e.y; c[4 * i + 2] = e.z; c[4 * i + 3] = e.w;
                 }
-                out[f] = horner27(c, u, v, w);
+                out[f] = mono27(c, m, u);
             }
         }
     }
@@ -535,7 +569,7 @@ struct SplitRay {
     float F[4];                  // interpolated dphidt, dphidx, dphidy, dphidz at the current point
     uint32_t appended;
     CosmoSeg cs;
-    double stopv;     // this ray's stop value (and its surface when per-ray)
+    double stopv;     // this ray's stop value (kept with surfaces only; re-read otherwise)
 };
 
 __device__ __forceinline__ double level_extent(unsigned l) { return ldexp((double)EXTENT, -(int)l); }
@@ -652,7 +686,7 @@ integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
                     R.dt = (float)L[14];
                     R.F[0] = R.dt; R.F[1] = R.dx; R.F[2] = R.dy; R.F[3] = R.dz;
                     R.appended = 0;
-                    R.stopv = A.ray_value ? A.ray_value[ray] : A.stop_value;
+                    if (SURF) R.stopv = A.ray_value ? A.ray_value[ray] : A.stop_value;
                     if (SURF) surf_init(SA, 0.0);
                     const unsigned l0 = (unsigned)L[15];
                     R.dl = level_extent(l0) * (R.core[CA] * R.core[CA] * ratio_k);
@@ -788,14 +822,15 @@ integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
             R.appended++;
         }
         bool brk = false;
+        const double stopv = SURF ? R.stopv : (A.ray_value ? A.ray_value[ray] : A.stop_value);
         switch (A.stop_kind) {
-            case 0: brk = R.chi > R.stopv; break;
-            case 1: brk = R.lambda > R.stopv; break;
-            case 2: brk = R.redshift > R.stopv; break;
-            case 3: brk = R.core[CT] > R.stopv; break;
-            case 4: brk = R.core[CA] < R.stopv; break;
-            case 5: brk = (p[CX] * A.kit[0] + p[CY] * A.kit[1] + p[CZ] * A.kit[2]) > R.stopv; break;
-            case 6: brk = R.s > R.stopv; break;
+            case 0: brk = R.chi > stopv; break;
+            case 1: brk = R.lambda > stopv; break;
+            case 2: brk = R.redshift > stopv; break;
+            case 3: brk = R.core[CT] > stopv; break;
+            case 4: brk = R.core[CA] < stopv; break;
+            case 5: brk = (p[CX] * A.kit[0] + p[CY] * A.kit[1] + p[CZ] * A.kit[2]) > stopv; break;
+            case 6: brk = R.s > stopv; break;
             default: brk = false; break;
         }
         if (!cont || brk) {
@@ -875,7 +910,7 @@ struct SplitSampler {
     }
 };
 
-__global__ void features_split_kernel(DevTree t, SplitView sv, const double* __restrict__ src, uint64_t n,
+__global__ void __launch_bounds__(256) features_split_kernel(DevTree t, SplitView sv, const double* __restrict__ src, uint64_t n,
                                       double length, float* __restrict__ feat) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
