@@ -168,6 +168,24 @@ def test_features_vs_oracle(precision):
     c.close()
 
 
+@pytest.mark.parametrize("precision", (capi.PREC_FP64, capi.PREC_SPLIT))
+def test_features_large_batch_stable(precision):
+    """A batch of several workgroups per compute unit: two calls agree bit for
+    bit, and with the same sources computed in two halves."""
+    print(f"[case] catalogue features, 131072 sources, precision={precision}", flush=True)
+    rec, f64 = cases.tree("amr")
+    c = context(precision)
+    c.load_records(rec)
+    c.load_fields64(f64)
+    src = proposal.cone_sources(131072, 20.0, 20.0, 8.0, 0.05, 0.45, seed=13)
+    a = c.catalogue_features(src)
+    b = c.catalogue_features(src)
+    h = np.concatenate([c.catalogue_features(src[:65536]), c.catalogue_features(src[65536:])])
+    c.close()
+    np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(a, h)
+
+
 def test_jacobian_all_rounds_same_results():
     print("[case] catalogue jacobian_all_rounds", flush=True)
     rec, f64 = cases.tree("amr")
