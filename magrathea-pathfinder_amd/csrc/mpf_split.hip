@@ -639,7 +639,12 @@ integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
     I.valid = false;
     I.slot = NIL;
     SplitRay R;
-    SurfAcc SA;
+    // per-lane Jacobian sums in LDS (SurfAccLds)
+    __shared__ double surf_sums[SURF ? 6 : 1][SURF ? 256 : 1];
+    SurfAccLds SA;
+    SA.S = &surf_sums[0][SURF ? threadIdx.x : 0];
+    SA.stride = SURF ? 256u : 0u;
+    SA.crossed = 0u;
     // surface values in LDS: a runtime index into the by-value argument
     // would make the compiler copy it to scratch
     __shared__ double surf_vals[MAX_SURF];
@@ -687,7 +692,7 @@ integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
                     R.F[0] = R.dt; R.F[1] = R.dx; R.F[2] = R.dy; R.F[3] = R.dz;
                     R.appended = 0;
                     if (SURF) R.stopv = A.ray_value ? A.ray_value[ray] : A.stop_value;
-                    if (SURF) surf_init(SA, 0.0);
+                    if (SURF) surf_init(SA);
                     const unsigned l0 = (unsigned)L[15];
                     R.dl = level_extent(l0) * (R.core[CA] * R.core[CA] * ratio_k);
                     I.valid = false;

@@ -38,11 +38,20 @@ struct SurfElem {
     double x, y, z, chi, lambda, t, a, redshift, s, isw, rho, phi, kt, kx, ky, kz;
 };
 
+// Accumulators of one ray in registers, with the running maximum of rho
 struct SurfAcc {
     double S1[3], S2[3];
     double rmax_prev;    // max rho over elements 0 .. i-2
     double rmax_cur;     // max rho over elements 0 .. i-1
     uint32_t crossed;    // bit j: surface j passed
+    __device__ double s1(int c) const { return S1[c]; }
+    __device__ double s2(int c) const { return S2[c]; }
+    __device__ void add(int c, double d1, double d2) { S1[c] += d1; S2[c] += d2; }
+    __device__ double rho_max() const { return rmax_prev; }
+    __device__ void push_rho(double rho) {
+        rmax_prev = rmax_cur;
+        rmax_cur = (rho > rmax_cur) ? rho : rmax_cur;
+    }
 };
 
 __device__ __forceinline__ void surf_init(SurfAcc& s, double rho0) {
@@ -50,6 +59,25 @@ __device__ __forceinline__ void surf_init(SurfAcc& s, double rho0) {
     for (int c = 0; c < 3; ++c) { s.S1[c] = 0.0; s.S2[c] = 0.0; }
     s.rmax_prev = -1.0e300;
     s.rmax_cur = rho0;
+    s.crossed = 0;
+}
+
+// The same with the sums in LDS (S[k * stride], k = S1 0..2, S2 0..2) and no
+// rho (a tier that does not carry rho writes its own values later)
+struct SurfAccLds {
+    double* S;
+    unsigned stride;
+    uint32_t crossed;
+    __device__ double s1(int c) const { return S[c * stride]; }
+    __device__ double s2(int c) const { return S[(3 + c) * stride]; }
+    __device__ void add(int c, double d1, double d2) { S[c * stride] += d1; S[(3 + c) * stride] += d2; }
+    __device__ double rho_max() const { return 0.0; }
+    __device__ void push_rho(double) {}
+};
+
+__device__ __forceinline__ void surf_init(SurfAccLds& s) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) s.S[k * s.stride] = 0.0;
     s.crossed = 0;
 }
 
@@ -72,7 +100,8 @@ __device__ __forceinline__ double surf_param(int kind, double chi, double lambda
 // recorded, so a snapshot holds the sums up to element i-1 = firstid.
 // vals: the surface values, or, when per_ray, the single value rv; vp, vn:
 // the surface parameter (surf_param) of the previous and the new element.
-__device__ __forceinline__ void surf_append(SurfAcc& s, int kind, int nsurf, const double* vals, bool per_ray,
+template <class Acc>
+__device__ __forceinline__ void surf_append(Acc& s, int kind, int nsurf, const double* vals, bool per_ray,
                                             double rv, double* rec, uint32_t i, const SurfElem& pv,
                                             const SurfElem& nw, double vp, double vn, const double* tidal,
                                             bool jac) {
@@ -102,15 +131,15 @@ __device__ __forceinline__ void surf_append(SurfAcc& s, int kind, int nsurf, con
         o[SF_ISW] = pv.isw * f + nw.isw * (1 - f);
         o[SF_RHO] = pv.rho * f + nw.rho * (1 - f);
         o[SF_PHI] = pv.phi * f + nw.phi * (1 - f);
-        o[SF_RHOMAX] = s.rmax_prev;
+        o[SF_RHOMAX] = s.rho_max();
         o[SF_KT] = pv.kt * f + nw.kt * (1 - f);
         o[SF_KX] = pv.kx * f + nw.kx * (1 - f);
         o[SF_KY] = pv.ky * f + nw.ky * (1 - f);
         o[SF_KZ] = pv.kz * f + nw.kz * (1 - f);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            o[SF_S1 + c] = s.S1[c];
-            o[SF_S1 + 3 + c] = s.S2[c];
+            o[SF_S1 + c] = s.s1(c);
+            o[SF_S1 + 3 + c] = s.s2(c);
         }
     }
     if (jac) {
@@ -118,12 +147,10 @@ __device__ __forceinline__ void surf_append(SurfAcc& s, int kind, int nsurf, con
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const double td = tidal[c] * dr;
-            s.S1[c] += nw.chi * td;
-            s.S2[c] += nw.chi * nw.chi * td;
+            s.add(c, nw.chi * td, nw.chi * nw.chi * td);
         }
     }
-    s.rmax_prev = s.rmax_cur;
-    s.rmax_cur = (nw.rho > s.rmax_cur) ? nw.rho : s.rmax_cur;
+    s.push_rho(nw.rho);
 }
 
 // Angular tidal components (lensing.h:421-448) from the differences of the
