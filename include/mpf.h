@@ -440,6 +440,70 @@ int mpf_launch_cone(const double* cones7, uint32_t ncones, uint32_t icone, doubl
  * photon itself when center != 0; out9 holds count + (center != 0) photons. */
 int mpf_launch_bundle(const double* in9, uint32_t count, double angle, double rotation, int center, double* out9);
 
+/* Octree preparation (host, no device needed): the ASCII branch of the
+ * reference's create_octree program, which turns cell files into cone files.
+ * A builder holds one octree as the reference's flat container of (key,
+ * Gravity<float>) cells; errors return a negative status with the message in
+ * mpf_prep_last_error().  The operation order, types and searches follow the
+ * reference's, so the cone files match its output byte for byte, except where
+ * the reference reads outside its container (the last cell of
+ * SimpleHyperOctree::update, a missing parent) or races (the coarse-density
+ * mean), where the builder does what the loop states, cell by cell. */
+typedef struct mpf_prep mpf_prep;
+typedef struct mpf_prep_cfg {
+    int32_t isfullsky;        /* 0: narrow cone, positions and forces rotated by rotm1 */
+    int32_t coarseonly;       /* keep only cells of level ncoarse + log2(EXTENT) */
+    uint32_t ncoarse;         /* coarse level of the simulation */
+    int32_t correction;       /* complete partial octs by interpolation (else coarsen them) */
+    int32_t coarsecorrection; /* non-normal coarse densities from their neighbours */
+    int32_t acorrection;      /* must be 0 (the Full Universe Run a snapping is not supported) */
+    int32_t order;            /* interpolation of the completed children: 0 NGP, 1 CIC, 2 TSC */
+    double microcoeff;        /* central sphere of radius 1 / microcoeff */
+    double h, omegam, lboxmpch, mpc;
+    uint64_t allocation;      /* cells to reserve */
+} mpf_prep_cfg;
+const char* mpf_prep_last_error(void);
+/* Create_octree::ReadParamFile's parameters (create_octree.h:120-146) */
+int mpf_prep_create(mpf_prep** prep, const mpf_prep_cfg* cfg);
+int mpf_prep_destroy(mpf_prep* prep);
+int mpf_prep_clear(mpf_prep* prep);
+/* Input::importascii (input.h:997-1069): appends the cells of one 10-column
+ * file (level, x, y, z, rho, phi, dphidx, dphidy, dphidz, a; RAMSES units)
+ * whose circumscribed sphere touches the central sphere or the cone
+ * (Input::collide, input.h:216-226; cone7 as for the cone functions, the
+ * octree-frame cone); rotm1 (narrow cones): 3 x 3 row-major. */
+int mpf_prep_import_ascii(mpf_prep* prep, const char* path, const double* cone7, const double* rotm1,
+                          uint64_t* added);
+/* Replaces the container by n 48-byte records in the given order. */
+int mpf_prep_load_records(mpf_prep* prep, const void* rec48, uint64_t n);
+/* SimpleHyperOctree::update (simplehyperoctree.h:1272-1307). */
+int mpf_prep_update(mpf_prep* prep);
+/* SimpleHyperOctree::ngp / cic / tsc (order 0 / 1 / 2, the uncached forms,
+ * simplehyperoctree.h:1396-1541) on the container as it is: out7 n x 7 in
+ * the order rho, phi, dphidx, dphidy, dphidz, a, dphidt. */
+int mpf_prep_interpolate(mpf_prep* prep, int order, const double* xyz, uint64_t n, float* out7);
+/* Miscellaneous::correctOctree (miscellaneous.h:467-479): SI units
+ * (Input::sistemize, input.h:382-394), Input::correct (input.h:1382-1520),
+ * dphi/da to dphi/dt with the corrected cosmology table (t, a, da/dt). */
+int mpf_prep_correct(mpf_prep* prep, const double* t, const double* a, const double* dadt, uint64_t n);
+int mpf_prep_size(mpf_prep* prep, uint64_t* n);
+int mpf_prep_records(mpf_prep* prep, void* rec48, uint64_t capacity);
+/* Input::save (input.h:1276-1289): refuses an existing file. */
+int mpf_prep_save(mpf_prep* prep, const char* path);
+/* Keys of cells from float positions at their levels (the importer's
+ * Index::compute<float>), key2 n x 2 (low, high 64 bits), and the
+ * importer's filter for them. */
+int mpf_prep_keys(const uint32_t* level, const float* xyz, uint64_t n, uint64_t* key2);
+int mpf_prep_collide(const uint64_t* key2, uint64_t n, double micro_radius, const double* cone7, uint8_t* out);
+/* Create_octree::PreparationASCII (create_octree.h:509-585): the .dat files
+ * of celldir (and minicone, narrow cones) in the order of the reference's
+ * shuffles (std::mt19937 seeded seed + rank), one cone file for each cone
+ * icone with icone % ntasks == rank; cones_rot7: the octree-frame cones. */
+int mpf_prepare_ascii(const mpf_prep_cfg* cfg, const char* celldir, const char* minicone, const double* cones_rot7,
+                      uint32_t ncones, const double* rotm1, uint32_t ntasks, uint32_t rank, uint64_t seed,
+                      const double* t, const double* a, const double* dadt, uint64_t ncos,
+                      const char* const* conefiles);
+
 /* HEALPix RING scheme (host, no device needed), restated from the published
  * algorithm (Gorski et al. 2005): the reference links HEALPix C++ 3.70 /
  * chealpix (pix2vec_ring at hmaps.h:599, nside2npix at hmaps.cpp:185). */

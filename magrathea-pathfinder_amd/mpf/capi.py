@@ -41,7 +41,10 @@ EXPORTS = ("mpf_create", "mpf_destroy", "mpf_last_error", "mpf_set_precision", "
            "mpf_launch_cone", "mpf_launch_bundle", "mpf_set_launch_config", "mpf_get_launch_config", "mpf_autotune",
            "mpf_last_kernel_ms", "mpf_propagate", "mpf_jacobian_bundle",
            "mpf_ray_statistics", "mpf_hmaps_homogeneous", "mpf_hmaps_components_dev", "mpf_healpix_write_fits",
-           "mpf_integrate_surfaces_born", "mpf_integrate_surfaces_born_dev")
+           "mpf_integrate_surfaces_born", "mpf_integrate_surfaces_born_dev",
+           "mpf_prep_last_error", "mpf_prep_create", "mpf_prep_destroy", "mpf_prep_clear", "mpf_prep_import_ascii",
+           "mpf_prep_load_records", "mpf_prep_update", "mpf_prep_interpolate", "mpf_prep_correct", "mpf_prep_size",
+           "mpf_prep_records", "mpf_prep_save", "mpf_prep_keys", "mpf_prep_collide", "mpf_prepare_ascii")
 
 CAT_FIELDS = ("phi", "theta", "phi_obs", "theta_obs", "sep", "iter", "z0", "z1", "z2", "z3", "z4", "z5",
               "a11", "a12", "a21", "a22", "status", "nrounds")
@@ -212,6 +215,23 @@ def load_library(path=LIB_PATH):
     lib.mpf_launch_towards.argtypes = [vp, vp, u64, vp]
     lib.mpf_launch_cone.argtypes = [vp, u32, u32, dbl, u32, u64, vp, vp]
     lib.mpf_launch_bundle.argtypes = [vp, u32, dbl, dbl, i32, vp]
+    # octree preparation (host)
+    lib.mpf_prep_last_error.restype = ctypes.c_char_p
+    lib.mpf_prep_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(PrepConfig)]
+    lib.mpf_prep_destroy.argtypes = [vp]
+    lib.mpf_prep_clear.argtypes = [vp]
+    lib.mpf_prep_import_ascii.argtypes = [vp, cp, vp, vp, ctypes.POINTER(u64)]
+    lib.mpf_prep_load_records.argtypes = [vp, vp, u64]
+    lib.mpf_prep_update.argtypes = [vp]
+    lib.mpf_prep_interpolate.argtypes = [vp, i32, vp, u64, vp]
+    lib.mpf_prep_correct.argtypes = [vp, vp, vp, vp, u64]
+    lib.mpf_prep_size.argtypes = [vp, ctypes.POINTER(u64)]
+    lib.mpf_prep_records.argtypes = [vp, vp, u64]
+    lib.mpf_prep_save.argtypes = [vp, cp]
+    lib.mpf_prep_keys.argtypes = [vp, vp, u64, vp]
+    lib.mpf_prep_collide.argtypes = [vp, u64, dbl, vp, vp]
+    lib.mpf_prepare_ascii.argtypes = [ctypes.POINTER(PrepConfig), cp, cp, vp, u32, vp, u32, u32, u64, vp, vp, vp, u64,
+                                      ctypes.POINTER(cp)]
     _lib = lib
     return lib
 
@@ -247,6 +267,129 @@ def healpix_vec2pix(nside, vec):
     if lib.mpf_healpix_vec2pix(int(nside), _p(vec), ctypes.c_uint64(vec.shape[0]), _p(out)):
         raise MpfError("mpf_healpix_vec2pix: invalid arguments")
     return out
+
+
+class PrepConfig(ctypes.Structure):
+    """mpf_prep_cfg: the create_octree parameters the ASCII branch reads."""
+    _fields_ = [("isfullsky", ctypes.c_int32), ("coarseonly", ctypes.c_int32), ("ncoarse", ctypes.c_uint32),
+                ("correction", ctypes.c_int32), ("coarsecorrection", ctypes.c_int32),
+                ("acorrection", ctypes.c_int32), ("order", ctypes.c_int32), ("microcoeff", ctypes.c_double),
+                ("h", ctypes.c_double), ("omegam", ctypes.c_double), ("lboxmpch", ctypes.c_double),
+                ("mpc", ctypes.c_double), ("allocation", ctypes.c_uint64)]
+
+
+MPC_SI = 3.0856775814913673e22
+
+
+def prep_config(isfullsky=1, coarseonly=0, ncoarse=0, correction=1, coarsecorrection=0, acorrection=0, order=TSC,
+                microcoeff=16.0, h=0.7, omegam=0.3, lboxmpch=1000.0, mpc=MPC_SI, allocation=0):
+    return PrepConfig(int(isfullsky), int(coarseonly), int(ncoarse), int(correction), int(coarsecorrection),
+                      int(acorrection), int(order), float(microcoeff), float(h), float(omegam), float(lboxmpch),
+                      float(mpc), int(allocation))
+
+
+def _prep_check(rc, name):
+    if rc != 0:
+        raise MpfError(f"{name}: {load_library().mpf_prep_last_error().decode()} ({rc})")
+
+
+class Prep:
+    """Octree builder of the create_octree ASCII branch (mpf_prep_*)."""
+
+    def __init__(self, **cfg):
+        self.lib = load_library()
+        self.cfg = prep_config(**cfg)
+        self.h = ctypes.c_void_p()
+        _prep_check(self.lib.mpf_prep_create(ctypes.byref(self.h), ctypes.byref(self.cfg)), "mpf_prep_create")
+
+    def close(self):
+        if self.h:
+            self.lib.mpf_prep_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def clear(self):
+        _prep_check(self.lib.mpf_prep_clear(self.h), "mpf_prep_clear")
+
+    def import_ascii(self, path, cone7, rotm1=None):
+        n = ctypes.c_uint64(0)
+        rot = None if rotm1 is None else _f64(rotm1, 9)
+        c7 = _f64(cone7, 7)
+        _prep_check(self.lib.mpf_prep_import_ascii(self.h, str(path).encode(), _p(c7), _p(rot), ctypes.byref(n)),
+                    "mpf_prep_import_ascii")
+        return n.value
+
+    def load_records(self, rec):
+        rec = np.ascontiguousarray(rec)
+        _prep_check(self.lib.mpf_prep_load_records(self.h, _p(rec), rec.size), "mpf_prep_load_records")
+
+    def update(self):
+        _prep_check(self.lib.mpf_prep_update(self.h), "mpf_prep_update")
+
+    def interpolate(self, order, xyz):
+        xyz = _f64(xyz, (-1, 3))
+        out = np.zeros((xyz.shape[0], 7), dtype=np.float32)
+        _prep_check(self.lib.mpf_prep_interpolate(self.h, int(order), _p(xyz), xyz.shape[0], _p(out)),
+                    "mpf_prep_interpolate")
+        return out
+
+    def correct(self, table):
+        t, a, dadt = (_f64(table[k], -1) for k in range(3))
+        _prep_check(self.lib.mpf_prep_correct(self.h, _p(t), _p(a), _p(dadt), t.size), "mpf_prep_correct")
+
+    def records(self):
+        from . import formats
+        n = ctypes.c_uint64(0)
+        _prep_check(self.lib.mpf_prep_size(self.h, ctypes.byref(n)), "mpf_prep_size")
+        rec = np.zeros(n.value, dtype=formats.RECORD_DTYPE)
+        if n.value:
+            _prep_check(self.lib.mpf_prep_records(self.h, _p(rec), n.value), "mpf_prep_records")
+        return rec
+
+    def save(self, path):
+        _prep_check(self.lib.mpf_prep_save(self.h, str(path).encode()), "mpf_prep_save")
+
+
+def prep_keys(level, xyz):
+    """Keys of cells from float positions (the ASCII importer's Index::compute<float>): n x 2 uint64 (lo, hi)."""
+    level = np.ascontiguousarray(level, dtype=np.uint32)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float32).reshape(-1, 3)
+    out = np.zeros((xyz.shape[0], 2), dtype=np.uint64)
+    _prep_check(load_library().mpf_prep_keys(_p(level), _p(xyz), xyz.shape[0], _p(out)), "mpf_prep_keys")
+    return out
+
+
+def prep_collide(key2, micro_radius, cone7):
+    key2 = np.ascontiguousarray(key2, dtype=np.uint64).reshape(-1, 2)
+    out = np.zeros(key2.shape[0], dtype=np.uint8)
+    c7 = _f64(cone7, 7)
+    _prep_check(load_library().mpf_prep_collide(_p(key2), key2.shape[0], float(micro_radius), _p(c7), _p(out)),
+                "mpf_prep_collide")
+    return out.astype(bool)
+
+
+def prepare_ascii(celldir, cones_rot7, conefiles, table, minicone=None, rotm1=None, ntasks=1, rank=0, seed=42,
+                  **cfg):
+    """Create_octree::PreparationASCII: one cone file per cone of this rank."""
+    c = prep_config(**cfg)
+    cones = _f64(cones_rot7, (-1, CONE_N))
+    t, a, dadt = (_f64(table[k], -1) for k in range(3))
+    names = (ctypes.c_char_p * len(conefiles))(*[str(f).encode() for f in conefiles])
+    rot = None if rotm1 is None else _f64(rotm1, 9)
+    cdir, mdir = _dir(celldir), (_dir(minicone) if minicone else None)
+    _prep_check(load_library().mpf_prepare_ascii(ctypes.byref(c), cdir, mdir, _p(cones), cones.shape[0], _p(rot), ntasks,
+                                                 rank, seed, _p(t), _p(a), _p(dadt), t.size, names),
+                "mpf_prepare_ascii")
+
+
+def _dir(d):
+    d = str(d)
+    return (d if d.endswith("/") else d + "/").encode()
 
 
 def _cone_check(rc, name):

@@ -212,6 +212,57 @@ class RefHarness(_Base):
 
     def __init__(self, path=REF_LIB):
         super().__init__(path)
+        self.lib.ref_prep_update.restype = _u64
+        self.lib.ref_prep_coarsen.restype = _u64
+
+    # ---- octree preparation (create_octree)
+    def prep_keys(self, level, xyz):
+        level = np.ascontiguousarray(level, dtype=np.uint32)
+        xyz = np.ascontiguousarray(xyz, dtype=np.float32).reshape(-1, 3)
+        out = np.zeros((xyz.shape[0], 2), dtype=np.uint64)
+        self.lib.ref_prep_keys(_p(level), _p(xyz), _u64(xyz.shape[0]), _p(out))
+        return out
+
+    def prep_collide(self, key2, micro_radius, cone7):
+        key2 = np.ascontiguousarray(key2, dtype=np.uint64).reshape(-1, 2)
+        c7 = np.ascontiguousarray(cone7, dtype=np.float64).reshape(7)
+        out = np.zeros(key2.shape[0], dtype=np.uint8)
+        self.lib.ref_prep_collide(_p(key2), _u64(key2.shape[0]), ctypes.c_double(micro_radius), _p(c7), _p(out))
+        return out.astype(bool)
+
+    def prep_update(self, rec):
+        rec = np.array(rec, copy=True)
+        n = self.lib.ref_prep_update(_p(rec), _u64(rec.size))
+        return rec[:n]
+
+    def prep_coarsen(self, rec, idx):
+        rec = np.array(rec, copy=True)
+        idx = np.ascontiguousarray(idx, dtype=np.uint64)
+        n = self.lib.ref_prep_coarsen(_p(rec), _u64(rec.size), _p(idx), _u64(idx.size))
+        return rec[:n]
+
+    def prep_find(self, rec, key2):
+        rec = np.ascontiguousarray(rec)
+        key2 = np.ascontiguousarray(key2, dtype=np.uint64).reshape(-1, 2)
+        found = np.zeros(key2.shape[0], dtype=np.uint64)
+        leaf = np.zeros(rec.size, dtype=np.uint8)
+        self.lib.ref_prep_find(_p(rec), _u64(rec.size), _p(key2), _u64(key2.shape[0]), _p(found), _p(leaf))
+        return found, leaf.astype(bool)
+
+    def prep_interpolate(self, rec, order, xyz):
+        rec = np.ascontiguousarray(rec)
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros((xyz.shape[0], 7), dtype=np.float32)
+        self.lib.ref_prep_interpolate(_p(rec), _u64(rec.size), int(order), _p(xyz), _u64(xyz.shape[0]), _p(out))
+        return out
+
+    def prep_interpolate2(self, x0, x, y):
+        x0 = np.ascontiguousarray(x0, dtype=np.float32)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        out = np.zeros(x0.size, dtype=np.float32)
+        self.lib.ref_prep_interpolate2(_p(x0), _u64(x0.size), _p(x), _p(y), _u64(x.size), _p(out))
+        return out
 
     def interp(self, h, order, xyz):
         xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)

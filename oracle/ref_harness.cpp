@@ -405,4 +405,99 @@ void ref_inv3(const double* a, double* r) {
 double ref_deg() { return magrathea::Constants<double>::deg(); }
 double ref_pi() { return magrathea::Constants<double>::pi(); }
 
+/* ---- octree preparation (create_octree) ---- */
+
+/* Index::compute<float> of the ASCII importer (input.h:1058-1059). */
+void ref_prep_keys(const uint32_t* level, const float* xyz, uint64_t n, uint64_t* key2) {
+    for (uint64_t i = 0; i < n; ++i) {
+        const Index k = Index::template compute<float, position, extent>(level[i], xyz[3 * i], xyz[3 * i + 1],
+                                                                          xyz[3 * i + 2]);
+        indexing v;
+        std::memcpy(&v, &k, sizeof(v));
+        key2[2 * i] = (uint64_t)v;
+        key2[2 * i + 1] = (uint64_t)(v >> 64);
+    }
+}
+
+/* Utility::collide of a cell's circumscribed sphere (its position and
+ * extent * sqrt(3) / 2, as Input::collide builds it) with the central sphere
+ * and with the cone. */
+void ref_prep_collide(const uint64_t* key2, uint64_t n, double micro_radius, const double* cone7, uint8_t* out) {
+    const std::vector<Cone<Point>> cone = make_cones(cone7, 1);
+    const Sphere micro(Point{{0.0, 0.0, 0.0}}, micro_radius);
+    for (uint64_t i = 0; i < n; ++i) {
+        const indexing v = (((indexing)key2[2 * i + 1]) << 64) | key2[2 * i];
+        Index k;
+        std::memcpy((void*)&k, &v, sizeof(v));
+        Sphere s;
+        for (unsigned d = 0; d < 3; ++d) s.position(d) = k.template position<real, position, extent>(d);
+        s.extent() = k.template extent<real, position, extent>() * std::sqrt(real(3)) / real(2);
+        out[i] = (Utility::collide(s, micro) || Utility::collide(s, cone[0], micro.extent())) ? 1 : 0;
+    }
+}
+
+/* A container of n records in the given order with one more record past its
+ * end (a copy of the last one, so the update's scan past the end reads a
+ * duplicate of the last cell). */
+static void load_container(Octree& o, const void* rec, uint64_t n) {
+    o.reserve(n + 1);
+    o.resize(n + 1);
+    if (n) {
+        std::memcpy((void*)o.data(), rec, n * sizeof(Element));
+        std::memcpy((void*)(o.data() + n), (const char*)rec + (n - 1) * sizeof(Element), sizeof(Element));
+    }
+    o.resize(n);
+}
+
+/* SimpleHyperOctree::update on the records; returns the new count. */
+uint64_t ref_prep_update(void* rec, uint64_t n) {
+    Octree o;
+    load_container(o, rec, n);
+    o.update();
+    std::memcpy(rec, (const void*)o.data(), o.size() * sizeof(Element));
+    return o.size();
+}
+
+/* coarsen(begin + i) for each listed i, then update. */
+uint64_t ref_prep_coarsen(void* rec, uint64_t n, const uint64_t* idx, uint64_t m) {
+    Octree o;
+    load_container(o, rec, n);
+    for (uint64_t j = 0; j < m; ++j)
+        if (!std::get<0>(o[idx[j]]).invalidated()) o.coarsen(o.begin() + idx[j]);
+    o.update();
+    std::memcpy(rec, (const void*)o.data(), o.size() * sizeof(Element));
+    return o.size();
+}
+
+/* find / leaf on the container as given: found[i] = position or n. */
+void ref_prep_find(const void* rec, uint64_t n, const uint64_t* key2, uint64_t m, uint64_t* found, uint8_t* leaf) {
+    Octree o;
+    load_container(o, rec, n);
+    for (uint64_t i = 0; i < m; ++i) {
+        const indexing v = (((indexing)key2[2 * i + 1]) << 64) | key2[2 * i];
+        Index k;
+        std::memcpy((void*)&k, &v, sizeof(v));
+        found[i] = (uint64_t)std::distance(o.begin(), o.find(k));
+    }
+    for (uint64_t i = 0; i < n; ++i) leaf[i] = o.leaf(o.begin() + i) ? 1 : 0;
+}
+
+/* Uncached ngp / cic / tsc (order 0 / 1 / 2) on the container as given. */
+void ref_prep_interpolate(const void* rec, uint64_t n, int order, const double* xyz, uint64_t m, float* out7) {
+    Octree o;
+    load_container(o, rec, n);
+    for (uint64_t i = 0; i < m; ++i) {
+        const double* p = xyz + 3 * i;
+        const Data d = (order == 0) ? o.ngp(p[0], p[1], p[2]) : (order == 1) ? o.cic(p[0], p[1], p[2])
+                                                                              : o.tsc(p[0], p[1], p[2]);
+        data_to7(d, out7 + 7 * i);
+    }
+}
+
+/* Utility::interpolate2 with a float abscissa (miscellaneous.h:477). */
+void ref_prep_interpolate2(const float* x0, uint64_t m, const double* x, const double* y, uint64_t n, float* out) {
+    const std::vector<double> vx(x, x + n), vy(y, y + n);
+    for (uint64_t i = 0; i < m; ++i) out[i] = Utility::interpolate2(x0[i], vx, vy);
+}
+
 }  // extern "C"
