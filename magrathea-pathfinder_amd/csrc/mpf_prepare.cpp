@@ -10,9 +10,14 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
 #include <fstream>
 #include <iterator>
 #include <random>
+#include <sstream>
+#include <thread>
 
 namespace mpf {
 namespace prep {
@@ -264,23 +269,146 @@ Grav Builder::tsc(double x, double y, double z) const {
 
 // ------------------------------------------------------------------ input
 
+namespace {
+
+// a token the stream's extraction of an unsigned or a float would read in
+// full: optional sign, digits with one optional point, optional exponent
+bool plain_number(const char* b, const char* e, bool integral) {
+    const char* p = b;
+    if (p < e && (*p == '+' || *p == '-')) ++p;
+    bool digits = false, point = false;
+    for (; p < e && ((*p >= '0' && *p <= '9') || (!integral && *p == '.' && !point)); ++p) {
+        if (*p == '.') point = true;
+        else digits = true;
+    }
+    if (!digits) return false;
+    if (p < e && !integral && (*p == 'e' || *p == 'E')) {
+        ++p;
+        if (p < e && (*p == '+' || *p == '-')) ++p;
+        const char* q = p;
+        while (p < e && *p >= '0' && *p <= '9') ++p;
+        if (p == q) return false;
+    }
+    return p == e && !(integral && *b == '-');
+}
+
+// one line: exactly 10 plain tokens; the values strtoul / strtof give them
+// (correctly rounded, as the stream's extraction)
+template <class Store>
+bool parse_line(const char* b, const char* e, unsigned row, Store& store) {
+    const char* tok[10][2];
+    int n = 0;
+    const char* p = b;
+    while (p < e) {
+        while (p < e && (*p == ' ' || *p == '\t' || *p == '\r' || *p == '\v' || *p == '\f')) ++p;
+        if (p >= e) break;
+        const char* q = p;
+        while (q < e && !(*q == ' ' || *q == '\t' || *q == '\r' || *q == '\v' || *q == '\f')) ++q;
+        if (n == 10) return false;
+        tok[n][0] = p;
+        tok[n][1] = q;
+        ++n;
+        p = q;
+    }
+    if (n != 10) return false;
+    char tmp[64];
+    unsigned lv = 0;
+    float v[9];
+    for (int k = 0; k < 10; ++k) {
+        const size_t len = (size_t)(tok[k][1] - tok[k][0]);
+        if (len >= sizeof(tmp) || !plain_number(tok[k][0], tok[k][1], k == 0)) return false;
+        std::memcpy(tmp, tok[k][0], len);
+        tmp[len] = 0;
+        if (k == 0) {
+            const unsigned long x = std::strtoul(tmp, nullptr, 10);
+            if (x > 0xFFFFFFFFul) return false;
+            lv = (unsigned)x;
+        } else {
+            v[k - 1] = std::strtof(tmp, nullptr);
+        }
+    }
+    store(row, lv, v);
+    return true;
+}
+
+// f(i) for i in [0, n) over up to 16 threads
+template <class F>
+void parallel_rows(unsigned n, F&& f) {
+    const unsigned nthreads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (n < 4096 || nthreads == 1) {
+        for (unsigned i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (unsigned w = 0; w < nthreads; ++w)
+        pool.emplace_back([&, w]() {
+            const unsigned lo = (unsigned)((uint64_t)n * w / nthreads), hi = (unsigned)((uint64_t)n * (w + 1) / nthreads);
+            for (unsigned i = lo; i < hi; ++i) f(i);
+        });
+    for (std::thread& t : pool) t.join();
+}
+
+// every line of the buffer parsed by parse_line, in threads over line
+// ranges; false when a line is not one well-formed row
+template <class Store>
+bool parse_rows_parallel(const std::vector<char>& buf, unsigned rows, Store& store) {
+    if (rows == 0) return true;
+    std::vector<size_t> starts;
+    starts.reserve(rows + 1);
+    starts.push_back(0);
+    for (size_t i = 0; i < buf.size(); ++i)
+        if (buf[i] == '\n') starts.push_back(i + 1);
+    // a trailing line without a newline is not a row
+    const unsigned nthreads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<int> ok(nthreads, 1);
+    std::vector<std::thread> pool;
+    for (unsigned w = 0; w < nthreads; ++w)
+        pool.emplace_back([&, w]() {
+            const unsigned lo = (unsigned)((uint64_t)rows * w / nthreads), hi = (unsigned)((uint64_t)rows * (w + 1) / nthreads);
+            for (unsigned r = lo; r < hi && ok[w]; ++r)
+                if (!parse_line(buf.data() + starts[r], buf.data() + starts[r + 1] - 1, r, store)) ok[w] = 0;
+        });
+    for (std::thread& t : pool) t.join();
+    return std::all_of(ok.begin(), ok.end(), [](int x) { return x != 0; });
+}
+
+}  // namespace
+
 bool Builder::import_ascii(const std::string& path, const cone::Cone& conic, const double* rotm1, uint64_t& added,
                            std::string& err) {
     added = 0;
-    std::ifstream counting(path.c_str());
-    if (!counting) {
-        err = "cannot open " + path;
-        return false;
+    std::vector<char> buf;
+    {
+        FILE* f = std::fopen(path.c_str(), "rb");
+        if (!f) {
+            err = "cannot open " + path;
+            return false;
+        }
+        char chunk[1 << 16];
+        size_t got;
+        while ((got = std::fread(chunk, 1, sizeof(chunk), f)) > 0) buf.insert(buf.end(), chunk, chunk + got);
+        std::fclose(f);
     }
-    counting.unsetf(std::ios_base::skipws);
-    const unsigned size = (unsigned)std::count(std::istream_iterator<char>(counting), std::istream_iterator<char>(), '\n');
-    counting.close();
-    std::ifstream stream(path.c_str());
+    // as many rows as newline characters, 10 values each (input.h:1008-1030)
+    const unsigned size = (unsigned)std::count(buf.begin(), buf.end(), '\n');
     std::vector<unsigned> level(size);
     std::vector<float> center(3ull * size), force(3ull * size), a(size), phi(size), rho(size);
-    for (unsigned i = 0; i < size; ++i)
-        stream >> level[i] >> center[3 * i] >> center[3 * i + 1] >> center[3 * i + 2] >> rho[i] >> phi[i] >>
-            force[3 * i] >> force[3 * i + 1] >> force[3 * i + 2] >> a[i];
+    auto store = [&](unsigned i, unsigned lv, const float* v) {
+        level[i] = lv;
+        center[3 * i] = v[0]; center[3 * i + 1] = v[1]; center[3 * i + 2] = v[2];
+        rho[i] = v[3]; phi[i] = v[4];
+        force[3 * i] = v[5]; force[3 * i + 1] = v[6]; force[3 * i + 2] = v[7];
+        a[i] = v[8];
+    };
+    if (!parse_rows_parallel(buf, size, store)) {
+        // not one well-formed row per line: the stream extraction itself
+        std::fill(level.begin(), level.end(), 0u);
+        for (std::vector<float>* v : {&center, &force, &a, &phi, &rho}) std::fill(v->begin(), v->end(), 0.0f);
+        std::istringstream stream(std::string(buf.begin(), buf.end()));
+        for (unsigned i = 0; i < size; ++i)
+            stream >> level[i] >> center[3 * i] >> center[3 * i + 1] >> center[3 * i + 2] >> rho[i] >> phi[i] >>
+                force[3 * i] >> force[3 * i + 1] >> force[3 * i + 2] >> a[i];
+    }
     if (!cfg_.isfullsky) {
         if (!rotm1) {
             err = "a narrow cone needs the rotation rotm1";
@@ -295,12 +423,18 @@ bool Builder::import_ascii(const std::string& path, const cone::Cone& conic, con
             }
         }
     }
+    // keys and the filter per row in threads, then the selected rows
+    // appended in file order
     const double micro = 1.0 / cfg_.microcoeff;
+    std::vector<u128> keys(size);
+    std::vector<uint8_t> keep(size);
+    parallel_rows(size, [&](unsigned i) {
+        keys[i] = compute_f(level[i] + LEVEL_SHIFT, center[3 * i], center[3 * i + 1], center[3 * i + 2]);
+        keep[i] = collide(keys[i], micro, conic) ? 1 : 0;
+    });
     for (unsigned i = 0; i < size; ++i) {
-        const unsigned l = level[i] + LEVEL_SHIFT;
-        const u128 k = compute_f(l, center[3 * i], center[3 * i + 1], center[3 * i + 2]);
-        if (!collide(k, micro, conic)) continue;
-        v_.push_back(Elem{k, Grav{rho[i], phi[i], force[3 * i], force[3 * i + 1], force[3 * i + 2], a[i], 0.0f}});
+        if (!keep[i]) continue;
+        v_.push_back(Elem{keys[i], Grav{rho[i], phi[i], force[3 * i], force[3 * i + 1], force[3 * i + 2], a[i], 0.0f}});
         ++added;
     }
     if (cfg_.coarseonly) {

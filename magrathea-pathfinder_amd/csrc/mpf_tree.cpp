@@ -1,5 +1,7 @@
 #include "mpf_tree.h"
 
+#include "mpf_key.h"
+
 #include <algorithm>
 #include <cstring>
 #include <thread>
@@ -7,48 +9,11 @@
 namespace mpf {
 
 namespace {
-using u128 = unsigned __int128;
-const u128 MARK = (((u128)0x8888888888888888ULL) << 64) | (u128)0x8888888888888888ULL;
-
-inline u128 load_key(const unsigned char* r) {
-    uint64_t lo, hi;
-    std::memcpy(&lo, r, 8);
-    std::memcpy(&hi, r + 8, 8);
-    return (((u128)hi) << 64) | lo;
-}
-
-// level(): number of leading nibbles, from the trailing marker bit
-// (simplehyperoctreeindex.h:572-575)
-inline unsigned key_level(u128 k) {
-    const u128 m = k & MARK;
-    unsigned tz = 128;
-    if (m) {
-        const uint64_t lo = (uint64_t)m, hi = (uint64_t)(m >> 64);
-        tz = lo ? (unsigned)__builtin_ctzll(lo) : 64u + (unsigned)__builtin_ctzll(hi);
-    }
-    return (128u - tz) / 4u + (k != 0);
-}
-
-// a well-formed key of level l: the l leading nibbles carry their marker bit
-// and every bit below them is zero
-inline bool key_wellformed(u128 k, unsigned l) {
-    if (l > 32) return false;
-    const u128 below = (l >= 32) ? (u128)0 : ((((u128)1) << (128 - 4 * l)) - 1);
-    if (k & below) return false;
-    for (unsigned i = 0; i < l; ++i)
-        if (!((k >> (127 - 4 * i)) & 1)) return false;
-    return true;
-}
-
-inline void key_coords(u128 k, unsigned l, uint32_t& x, uint32_t& y, uint32_t& z) {
-    x = y = z = 0;
-    for (unsigned i = 0; i < l; ++i) {
-        const unsigned nib = (unsigned)((k >> (124 - 4 * i)) & 0xF);
-        x = (x << 1) | (nib & 1u);
-        y = (y << 1) | ((nib >> 1) & 1u);
-        z = (z << 1) | ((nib >> 2) & 1u);
-    }
-}
+using key::u128;
+inline u128 load_key(const unsigned char* r) { return key::load(r); }
+inline unsigned key_level(u128 k) { return key::level(k); }
+inline bool key_wellformed(u128 k, unsigned l) { return key::wellformed(k, l); }
+inline void key_coords(u128 k, unsigned l, uint32_t& x, uint32_t& y, uint32_t& z) { key::coords(k, l, x, y, z); }
 }  // namespace
 
 bool build_tree(const void* rec48, uint64_t n, HostTree& t, std::string& err, uint64_t max_base_entries) {
