@@ -536,3 +536,22 @@ def test_prepare_ascii_writes_loadable_cone_files(tmp_path):
     # an existing output is refused (Input::save)
     with pytest.raises(capi.MpfError):
         capi.prepare_ascii(celldir, cones, out, table, seed=42, **cfg)
+
+
+def test_import_malformed_file_follows_the_stream(tmp_path):
+    """A token the stream cannot read (here `nan`) stops the reading: that
+    value and every later one stay 0, as with std::ifstream >> (input.h:1028)."""
+    path = tmp_path / "bad.dat"
+    path.write_text("3 -0.4375 -0.4375 -0.4375 1.5 2e-06 1e-05 2e-05 3e-05 0.8\n"
+                    "3 0.0625 0.0625 0.0625 nan 4e-06 1e-05 2e-05 3e-05 0.8\n"
+                    "3 0.1875 0.1875 0.1875 1.5 2e-06 1e-05 2e-05 3e-05 0.8\n")
+    cone = np.array([0, 0, 0, 2.0, 0, 0, math.radians(60.0)])
+    P = capi.Prep(microcoeff=0.5)
+    assert P.import_ascii(path, cone) == 3
+    got = P.records()
+    keys = capi.prep_keys(np.array([5, 5, 2], dtype=np.uint32),
+                          np.array([[-0.4375] * 3, [0.0625] * 3, [0.0, 0.0, 0.0]], dtype=np.float32))
+    np.testing.assert_array_equal(key_words(got), keys)
+    assert got[0]["rho"] == np.float32(1.5) and got[0]["dphidz"] == np.float32(3e-05)
+    assert got[1]["rho"] == 0 and got[1]["phi"] == 0 and got[1]["a"] == 0
+    assert got[2]["rho"] == 0 and got[2]["a"] == 0
