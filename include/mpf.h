@@ -237,6 +237,41 @@ int mpf_integrate_surfaces(mpf_ctx* ctx, const double* in9, uint64_t n, const mp
 int mpf_integrate_surfaces_dev(mpf_ctx* ctx, const double* dev_in9, uint64_t n, const mpf_surfaces* surf,
                                double* dev_out, double* dev_out_surf, int sync);
 
+/* Velocity field of the reference's velocity-field build (VELOCITYFIELD;
+ * hmaps.cpp:359-441).  The cells then carry a velocity (gravity2.h:45-90)
+ * that the exact tiers interpolate with the other fields, and the redshift
+ * of every step takes the Doppler and ISW terms
+ *   z = a0/a (1 + phi0/c^2 - phi/c^2 + isw + v.k/(c k_t) - v0n) - 1,
+ *   v0n = vobs.k0/(c k0_t)                     (integrator.h:537-539, 594-595).
+ * The split tier refuses a tree with a velocity field.
+ *
+ * mpf_velocity_begin: rho of every cell becomes the deposited weight, so
+ *   the sums start from zero (hmaps.cpp:391).
+ * mpf_velocity_deposit: Hmaps::CreateOctreeVelocityWithCIC (scheme
+ *   MPF_VELOCITY_CIC, hmaps.h:439-478) or WithTSC (MPF_VELOCITY_TSC,
+ *   hmaps.h:497-541) of n particles (pos3: box units in the octree frame, as
+ *   Hmaps::fill_particles_vectors leaves them, hmaps.h:345-401; vel3: SI),
+ *   at every level from the coarsest to the finest stored; may be called once
+ *   per particle file.  The weights are summed in FP64 (the reference: FP32 in
+ *   thread order) and rounded once to FP32.
+ * mpf_velocity_finish: v /= (w + (w == 0)), rho = w, and a cell whose w is
+ *   not a normal number takes its parent's velocity, level by level
+ *   (hmaps.cpp:422-441); the tree then carries the field.
+ * mpf_velocity_set / _get: the field in record order (rho may be NULL: kept).
+ * mpf_velocity_clear: the tree carries no field (rho keeps its values).
+ * mpf_set_observer_velocity: vobs (SI; the parameters v0x, v0y, v0z of
+ *   hmaps.cpp:97-99), used with a velocity field. */
+#define MPF_VELOCITY_CIC 1
+#define MPF_VELOCITY_TSC 2
+int mpf_velocity_begin(mpf_ctx* ctx);
+int mpf_velocity_deposit(mpf_ctx* ctx, const float* pos3, const float* vel3, uint64_t n, int scheme);
+int mpf_velocity_deposit_dev(mpf_ctx* ctx, const float* dev_pos3, const float* dev_vel3, uint64_t n, int scheme);
+int mpf_velocity_finish(mpf_ctx* ctx);
+int mpf_velocity_set(mpf_ctx* ctx, const float* rho, const float* vel3, uint64_t n);
+int mpf_velocity_get(mpf_ctx* ctx, float* rho, float* vel3, uint64_t n);
+int mpf_velocity_clear(mpf_ctx* ctx);
+int mpf_set_observer_velocity(mpf_ctx* ctx, const double* vobs3);
+
 /* Lensing::dbetadtheta (lensing.h:140-364), the bundle Jacobian
  * (jacobiantype = bundle): for each direction i (phi_theta, n x 2) four rays
  * from observer3 towards the unit direction plus or minus tan(opening) along

@@ -84,6 +84,13 @@ struct mpf_ctx {
     mpf::DevTree dt;
     mpf::SplitTree split;
     bool split_ready = false;
+    // velocity field: FP64 sums (4 x n) between begin and finish, the field
+    // (3 x n) the tree carries after it, particle staging, parent work array
+    DevBuf<double> vacc;
+    DevBuf<float> vel, vstage;
+    DevBuf<uint32_t> vparent;
+    bool vel_open = false;
+    double vobs[3] = {0.0, 0.0, 0.0};
     // cosmology
     DevBuf<double> cosmo;
     int ncos = 0;
@@ -172,6 +179,7 @@ int mpf_destroy(mpf_ctx* c) {
     c->io_in.release(); c->io_out.release(); c->io_surf.release(); c->io_rv.release(); c->scratch.release();
     c->order.release(); c->order_work.release(); c->homo.release();
     c->map_acc.release(); c->io_i64.release(); c->io_u32.release();
+    c->vacc.release(); c->vel.release(); c->vstage.release(); c->vparent.release();
     mpf::split_release(c->split);
     mpf::mlp_destroy(c->mlp);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -213,7 +221,8 @@ int mpf_load_octree_records(mpf_ctx* c, const void* rec48, uint64_t n) {
     HIPCHK(c, hipMemcpyAsync(c->f32.p, soa.data(), soa.size() * 4, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     mpf::DevTree& d = c->dt;
-    d = mpf::DevTree();
+    d = mpf::DevTree();  // no velocity field
+    c->vel_open = false;
     d.B = t.min_level;
     d.maxL = t.max_level;
     d.bx0 = t.bx0; d.by0 = t.by0; d.bz0 = t.bz0;
@@ -260,6 +269,137 @@ int mpf_octree_info(const mpf_ctx* c, uint64_t* ncells, uint32_t* min_level, uin
     if (ncells) *ncells = c->have_tree ? c->ncells : 0;
     if (min_level) *min_level = c->min_level;
     if (max_level) *max_level = c->max_level;
+    return 0;
+}
+
+// ---- velocity field
+
+int mpf_set_observer_velocity(mpf_ctx* c, const double* vobs3) {
+    if (!c || !vobs3) return ERR_INVAL;
+    for (int k = 0; k < 3; ++k) c->vobs[k] = vobs3[k];
+    return 0;
+}
+
+int mpf_velocity_begin(mpf_ctx* c) {
+    if (!c) return ERR_INVAL;
+    if (!c->have_tree) return fail(c, ERR_STATE, "no octree loaded");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (c->vacc.ensure(4 * c->ncells) != hipSuccess) return fail(c, ERR_NOMEM, "velocity sums");
+    HIPCHK(c, hipMemsetAsync(c->vacc.p, 0, 4 * c->ncells * sizeof(double), c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->vel_open = true;
+    return 0;
+}
+
+static int velocity_deposit_impl(mpf_ctx* c, const float* dpos, const float* dvel, uint64_t n, int scheme) {
+    for (uint32_t l = c->min_level; l <= c->max_level; ++l)
+        HIPCHK(c, mpf::launch_velocity_deposit(c->dt, scheme, dpos, dvel, n, l, c->vacc.p, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+static int velocity_check(mpf_ctx* c, uint64_t n, const void* pos, const void* vel, int scheme) {
+    if (!c->vel_open) return fail(c, ERR_STATE, "call mpf_velocity_begin first");
+    if (n && (!pos || !vel)) return fail(c, ERR_INVAL, "null particle arrays");
+    if (scheme != MPF_VELOCITY_CIC && scheme != MPF_VELOCITY_TSC)
+        return fail(c, ERR_INVAL, "velocity field scheme must be cic (1) or tsc (2)");
+    return 0;
+}
+
+int mpf_velocity_deposit(mpf_ctx* c, const float* pos3, const float* vel3, uint64_t n, int scheme) {
+    if (!c) return ERR_INVAL;
+    if (int rc = velocity_check(c, n, pos3, vel3, scheme)) return rc;
+    if (n == 0) return 0;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (c->vstage.ensure(6 * n) != hipSuccess) return fail(c, ERR_NOMEM, "particle staging");
+    HIPCHK(c, hipMemcpyAsync(c->vstage.p, pos3, 3 * n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->vstage.p + 3 * n, vel3, 3 * n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    return velocity_deposit_impl(c, c->vstage.p, c->vstage.p + 3 * n, n, scheme);
+}
+
+int mpf_velocity_deposit_dev(mpf_ctx* c, const float* dev_pos3, const float* dev_vel3, uint64_t n, int scheme) {
+    if (!c) return ERR_INVAL;
+    if (int rc = velocity_check(c, n, dev_pos3, dev_vel3, scheme)) return rc;
+    if (n == 0) return 0;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    return velocity_deposit_impl(c, dev_pos3, dev_vel3, n, scheme);
+}
+
+int mpf_velocity_finish(mpf_ctx* c) {
+    if (!c) return ERR_INVAL;
+    if (!c->vel_open) return fail(c, ERR_STATE, "call mpf_velocity_begin first");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    const uint64_t n = c->ncells;
+    if (c->vel.ensure(3 * n) != hipSuccess || c->vparent.ensure(n) != hipSuccess)
+        return fail(c, ERR_NOMEM, "velocity field");
+    // first level-sorted cell of every level (the levels are stored in order)
+    std::vector<uint64_t> first(c->max_level - c->min_level + 2, n);
+    for (uint32_t l = c->min_level; l <= c->max_level + 1; ++l)
+        first[l - c->min_level] =
+            (uint64_t)(std::lower_bound(c->lev.begin(), c->lev.end(), (uint8_t)std::min<uint32_t>(l, 255)) -
+                       c->lev.begin());
+    first.back() = n;
+    HIPCHK(c, hipMemsetAsync(c->vparent.p, 0xFF, n * sizeof(uint32_t), c->stream));
+    HIPCHK(c, mpf::launch_velocity_finish(c->dt, c->vacc.p, c->f32.p, c->dt.f64 ? c->f64.p : nullptr, c->vel.p,
+                                          c->vparent.p, first.data(), c->min_level, c->max_level, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->dt.vel = c->vel.p;
+    c->vel_open = false;
+    c->split_ready = false;  // rho changed
+    return 0;
+}
+
+int mpf_velocity_set(mpf_ctx* c, const float* rho, const float* vel3, uint64_t n) {
+    if (!c || !vel3) return ERR_INVAL;
+    if (!c->have_tree) return fail(c, ERR_STATE, "no octree loaded");
+    if (n != c->ncells) return fail(c, ERR_INVAL, "count differs from the octree cell count");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    std::vector<float> soa(3 * n);
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint64_t g = c->where[i];
+        for (int k = 0; k < 3; ++k) soa[k * n + g] = vel3[3 * i + k];
+    }
+    if (c->vel.ensure(3 * n) != hipSuccess) return fail(c, ERR_NOMEM, "velocity field");
+    HIPCHK(c, hipMemcpyAsync(c->vel.p, soa.data(), 3 * n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    if (rho) {
+        std::vector<float> r(n);
+        std::vector<double> r64(c->dt.f64 ? n : 0);
+        for (uint64_t i = 0; i < n; ++i) r[c->where[i]] = rho[i];
+        for (uint64_t g = 0; g < r64.size(); ++g) r64[g] = (double)r[g];
+        HIPCHK(c, hipMemcpyAsync(c->f32.p, r.data(), n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        if (c->dt.f64)
+            HIPCHK(c, hipMemcpyAsync(c->f64.p, r64.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        c->split_ready = false;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->dt.vel = c->vel.p;
+    c->vel_open = false;
+    return 0;
+}
+
+int mpf_velocity_get(mpf_ctx* c, float* rho, float* vel3, uint64_t n) {
+    if (!c) return ERR_INVAL;
+    if (!c->have_tree) return fail(c, ERR_STATE, "no octree loaded");
+    if (!c->dt.vel) return fail(c, ERR_STATE, "the tree carries no velocity field");
+    if (n != c->ncells) return fail(c, ERR_INVAL, "count differs from the octree cell count");
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    std::vector<float> soa(3 * n), r(n);
+    HIPCHK(c, hipMemcpyAsync(soa.data(), c->vel.p, 3 * n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(r.data(), c->f32.p, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint64_t g = c->where[i];
+        if (rho) rho[i] = r[g];
+        if (vel3)
+            for (int k = 0; k < 3; ++k) vel3[3 * i + k] = soa[k * n + g];
+    }
+    return 0;
+}
+
+int mpf_velocity_clear(mpf_ctx* c) {
+    if (!c) return ERR_INVAL;
+    c->dt.vel = nullptr;
+    c->vel_open = false;
     return 0;
 }
 
@@ -401,6 +541,9 @@ static int integrate_impl(mpf_ctx* c, const double* dev_in9, uint64_t n, const m
         return fail(c, ERR_INVAL, "the homogeneous order runs in the exact tiers (REF, FP64)");
     if (dev_traj && c->cfg.precision == MPF_PREC_SPLIT)
         return fail(c, ERR_INVAL, "trajectories are written by the exact tiers (REF, FP64)");
+    if (c->dt.vel && c->cfg.precision == MPF_PREC_SPLIT)
+        return fail(c, ERR_INVAL, "a tree with a velocity field is integrated by the exact tiers (REF, FP64)");
+    for (int k = 0; k < 3; ++k) A.vobs[k] = c->vobs[k];
     const mpf_launch_cfg& L = c->launch[surf ? 1 : 0];
     A.refill = L.refill;
     HIPCHK(c, mpf::launch_fill_u64(c->counters.p, 3, 0ull, c->stream));

@@ -18,12 +18,12 @@ namespace {
 
 using namespace exact;
 
-template <int P>
+template <int P, int NF = 7>
 __device__ __forceinline__ void interp(int order, const DevTree& t, TscCache& cache, double x, double y, double z,
                                        double* d, int* loc) {
-    if (order == 0) ngp<P>(t, x, y, z, d, loc);
-    else if (order == 1) cic<P>(t, x, y, z, d, loc);
-    else tsc<P, true>(t, cache, x, y, z, d, loc);
+    if (order == 0) ngp<P, NF>(t, x, y, z, d, loc);
+    else if (order == 1) cic<P, NF>(t, x, y, z, d, loc);
+    else tsc<P, true, NF>(t, cache, x, y, z, d, loc);
 }
 
 // Utility::interpolate (utility.h:442-449) with a per-lane search hint
@@ -67,12 +67,16 @@ __device__ __forceinline__ double level_extent(int lvl) {
     return EXTENT / (1.0 * (double)(1ull << l));
 }
 
+// NF = 10: the tree carries a velocity field and the redshift takes the
+// Doppler and ISW terms of the velocity-field build
+template <int NF>
 struct RayState {
     SurfAcc surf;
     double stopv;     // this ray's stop value (and its surface when per-ray)
     Rec back, prev;
-    double data[7];   // fields at back's position
+    double data[NF];  // fields at back's position
     double dl, a0, phi0c2;
+    double v0n;       // observer velocity along the launch direction / (c dt/dl)
     int hint;
     uint32_t appended;
     bool has_prev;
@@ -90,8 +94,9 @@ __device__ __forceinline__ void traj_store(const IntegrateArgs& A, uint64_t ray,
     o[MPF_OUT_NSTEPS] = (double)k;
 }
 
-template <int P>
-__device__ void ray_launch(const IntegrateArgs& A, const DevTree& t, TscCache& cache, const double* in9, RayState& s) {
+template <int P, int NF>
+__device__ void ray_launch(const IntegrateArgs& A, const DevTree& t, TscCache& cache, const double* in9,
+                           RayState<NF>& s) {
     const double zero = 0.0, one = 1.0, two = 2.0;
     const double c = C_LIGHT, c2 = C2;
     const double scale = A.length;
@@ -103,8 +108,8 @@ __device__ void ray_launch(const IntegrateArgs& A, const DevTree& t, TscCache& c
     cache_init(cache);
     s.hint = 0;
     int loc = -1;
-    if (A.order >= 0) interp<P>(A.order, t, cache, b[CX], b[CY], b[CZ], s.data, &loc);
-    else { zero7(s.data); s.data[F_A] = one; }
+    if (A.order >= 0) interp<P, NF>(A.order, t, cache, b[CX], b[CY], b[CZ], s.data, &loc);
+    else { zerof<NF>(s.data); s.data[F_A] = one; }
     double* d = s.data;
     const double norm = sqrt((c2 * (one + two * (d[F_PHI] / c2)) * b[CKT] * b[CKT]) /
                              ((one - two * (d[F_PHI] / c2)) * (b[CKX] * b[CKX] + b[CKY] * b[CKY] + b[CKZ] * b[CKZ])));
@@ -135,6 +140,8 @@ __device__ void ray_launch(const IntegrateArgs& A, const DevTree& t, TscCache& c
     s.dl = level_extent(lvl0) * ratio;
     s.a0 = b[CA];
     s.phi0c2 = phi / c2;
+    // integrator.h:537-539 (velocity-field build)
+    s.v0n = (NF == 10) ? (A.vobs[0] * b[CKX] + A.vobs[1] * b[CKY] + A.vobs[2] * b[CKZ]) / (c * b[CKT]) : 0.0;
     s.appended = 0;
     s.has_prev = false;
     if (A.nsurf) surf_init(s.surf, d[F_RHO]);
@@ -178,8 +185,8 @@ __device__ void element_tidal(const DevTree& t, const double* r, double* T) {
 
 // One iteration of the integration loop (integrator.h:683-812).  Returns
 // true while the ray continues.
-template <int P>
-__device__ bool ray_step(const IntegrateArgs& A, const DevTree& t, TscCache& cache, RayState& s, uint64_t ray) {
+template <int P, int NF>
+__device__ bool ray_step(const IntegrateArgs& A, const DevTree& t, TscCache& cache, RayState<NF>& s, uint64_t ray) {
     const double zero = 0.0, one = 1.0, two = 2.0, six = 6.0;
     const double c = C_LIGHT, c2 = C2;
     const double min = 0.0 - (EXTENT / two), max = 0.0 + (EXTENT / two);
@@ -221,9 +228,9 @@ __device__ bool ray_step(const IntegrateArgs& A, const DevTree& t, TscCache& cac
     for (int i = 0; i < 9; ++i) p[i] = bk[i] + (dl / six) * (ksum[i] + kout[i]);
     // fields at the new position
     double* data = s.data;
-    if (A.order >= 0) interp<P>(A.order, t, cache, p[CX], p[CY], p[CZ], data, &loc);
+    if (A.order >= 0) interp<P, NF>(A.order, t, cache, p[CX], p[CY], p[CZ], data, &loc);
     else {
-        zero7(data);
+        zerof<NF>(data);
         data[F_A] = one;
         unsigned fl = 0;
         const uint32_t g = locate_point(t, p[CX], p[CY], p[CZ], &fl);
@@ -236,7 +243,7 @@ __device__ bool ray_step(const IntegrateArgs& A, const DevTree& t, TscCache& cac
     }
     if (!(!(p[CA] < zero) && ((p[CX] > min) && (p[CX] < max) && (p[CY] > min) && (p[CY] < max) && (p[CZ] > min) &&
                               (p[CZ] < max))))
-        zero7(data);
+        zerof<NF>(data);
     p[MPF_OUT_LEVEL] = (double)(loc < 0 ? 0 : loc);
     p[MPF_OUT_AH] = data[F_A];
     p[MPF_OUT_RHO] = data[F_RHO];
@@ -263,8 +270,14 @@ __device__ bool ray_step(const IntegrateArgs& A, const DevTree& t, TscCache& cac
     p[MPF_OUT_LAMBDA] = bk[MPF_OUT_LAMBDA] + dl;
     p[MPF_OUT_S] = bk[MPF_OUT_S] + sqrt((p[CX] - bk[CX]) * (p[CX] - bk[CX]) + (p[CY] - bk[CY]) * (p[CY] - bk[CY]) +
                                         (p[CZ] - bk[CZ]) * (p[CZ] - bk[CZ]));
-    p[MPF_OUT_REDSHIFT] = s.a0 / p[CA] * (one + s.phi0c2 - p[MPF_OUT_PHI] / c2) - one;
-    const bool cont = !empty7(data);
+    if (NF == 10)  // Doppler and ISW terms (integrator.h:594-595, 765-766)
+        p[MPF_OUT_REDSHIFT] =
+            s.a0 / p[CA] *
+                (one + s.phi0c2 - p[MPF_OUT_PHI] / c2 + p[MPF_OUT_ISW] +
+                 (data[F_VX] * p[CKX] + data[F_VY] * p[CKY] + data[F_VZ] * p[CKZ]) / (c * p[CKT]) - s.v0n) -
+            one;
+    else p[MPF_OUT_REDSHIFT] = s.a0 / p[CA] * (one + s.phi0c2 - p[MPF_OUT_PHI] / c2) - one;
+    const bool cont = !emptyf<NF>(data);
     if (cont) {
         const double ratio = p[CA] * p[CA] * (scale / c) / A.nsteps;
         s.dl = level_extent(loc) * ratio;
@@ -306,7 +319,8 @@ __device__ bool ray_step(const IntegrateArgs& A, const DevTree& t, TscCache& cac
     return cont;
 }
 
-__device__ __forceinline__ void surf_finish(const IntegrateArgs& A, const RayState& s, uint64_t ray,
+template <int NF>
+__device__ __forceinline__ void surf_finish(const IntegrateArgs& A, const RayState<NF>& s, uint64_t ray,
                                             uint32_t size_minus_one) {
     // surfaces never passed point past the end (firstid = size - 1), and a
     // surface whose next element was popped ends on the last element: both
@@ -324,7 +338,8 @@ __device__ __forceinline__ void surf_finish(const IntegrateArgs& A, const RaySta
     }
 }
 
-__device__ __forceinline__ void ray_finish(const IntegrateArgs& A, RayState& s, uint64_t ray) {
+template <int NF>
+__device__ __forceinline__ void ray_finish(const IntegrateArgs& A, RayState<NF>& s, uint64_t ray) {
     // erase the last element if its redshift or a is negative (integrator.h:816-821)
     const Rec* r = &s.back;
     uint32_t steps = s.appended;
@@ -351,9 +366,9 @@ __device__ __forceinline__ void ray_finish(const IntegrateArgs& A, RayState& s, 
     atomicAdd(A.steps, (unsigned long long)steps);
 }
 
-template <int P>
+template <int P, int NF>
 __global__ void __launch_bounds__(256) integrate_exact_kernel(IntegrateArgs A, DevTree t) {
-    RayState s;
+    RayState<NF> s;
     TscCache cache;
     cache_init(cache);
     bool active = false, drained = false;
@@ -372,10 +387,10 @@ __global__ void __launch_bounds__(256) integrate_exact_kernel(IntegrateArgs A, D
                 const uint64_t my = base + (uint64_t)__popcll(m & ((1ull << lane) - 1ull));
                 if (my < A.n) {
                     ray = A.queue_order ? (uint64_t)A.queue_order[my] : my;
-                    ray_launch<P>(A, t, cache, A.in9 + 9 * ray, s);
+                    ray_launch<P, NF>(A, t, cache, A.in9 + 9 * ray, s);
                     if (A.traj) traj_store(A, ray, 0, s.back.v);
                     if (A.ray_value) s.stopv = A.ray_value[ray];
-                    active = !empty7(s.data);
+                    active = !emptyf<NF>(s.data);
                     if (!active) ray_finish(A, s, ray);
                 } else {
                     drained = true;
@@ -387,7 +402,7 @@ __global__ void __launch_bounds__(256) integrate_exact_kernel(IntegrateArgs A, D
             continue;
         }
         if (active) {
-            if (!ray_step<P>(A, t, cache, s, ray)) {
+            if (!ray_step<P, NF>(A, t, cache, s, ray)) {
                 active = false;
                 ray_finish(A, s, ray);
             }
@@ -464,8 +479,12 @@ hipError_t launch_surface_final(const IntegrateArgs& A, hipStream_t st) {
 }
 
 hipError_t launch_integrate_exact(int precision, const IntegrateArgs& A, const DevTree& t, int grid, hipStream_t st) {
-    if (precision == 0) hipLaunchKernelGGL(integrate_exact_kernel<0>, dim3(grid), dim3(256), 0, st, A, t);
-    else hipLaunchKernelGGL(integrate_exact_kernel<1>, dim3(grid), dim3(256), 0, st, A, t);
+    // a tree with a velocity field: 10 interpolated fields and the Doppler redshift
+    if (t.vel) {
+        if (precision == 0) hipLaunchKernelGGL((integrate_exact_kernel<0, 10>), dim3(grid), dim3(256), 0, st, A, t);
+        else hipLaunchKernelGGL((integrate_exact_kernel<1, 10>), dim3(grid), dim3(256), 0, st, A, t);
+    } else if (precision == 0) hipLaunchKernelGGL((integrate_exact_kernel<0, 7>), dim3(grid), dim3(256), 0, st, A, t);
+    else hipLaunchKernelGGL((integrate_exact_kernel<1, 7>), dim3(grid), dim3(256), 0, st, A, t);
     return hipGetLastError();
 }
 

@@ -1,14 +1,20 @@
 // Exact NGP / CIC / TSC interpolation (reference arithmetic, operation by
 // operation).  Included by the kernels that need the reference semantics;
 // those translation units are compiled with -ffp-contract=off.
+//
+// NF = 7 interpolates the cell fields of gravity.h; NF = 10 adds the cell
+// velocity of the velocity-field build (gravity2.h:45-90), read from the
+// tree's velocity arrays (zero where the tree carries none).
 #pragma once
 #include "mpf_device.h"
 
 namespace mpf {
 namespace exact {
 
-template <int P>
-__device__ __forceinline__ void load7(const DevTree& t, uint32_t g, double* d) {
+enum { F_VX = 7, F_VY = 8, F_VZ = 9 };
+
+template <int P, int NF>
+__device__ __forceinline__ void loadf(const DevTree& t, uint32_t g, double* d) {
     if (P == 1 && t.f64) {
 #pragma unroll
         for (int k = 0; k < 7; ++k) d[k] = t.f64[(uint64_t)k * t.n + g];
@@ -16,42 +22,51 @@ __device__ __forceinline__ void load7(const DevTree& t, uint32_t g, double* d) {
 #pragma unroll
         for (int k = 0; k < 7; ++k) d[k] = (double)t.f32[(uint64_t)k * t.n + g];
     }
+    if (NF == 10) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) d[7 + k] = t.vel ? (double)t.vel[(uint64_t)k * t.n + g] : 0.0;
+    }
 }
 
 // mac() (simplehyperoctree.h:1870-1910): accumulator += factor * coefficient
-template <int P>
-__device__ __forceinline__ void mac7(double* acc, const double* f, double w) {
+template <int P, int NF>
+__device__ __forceinline__ void macf(double* acc, const double* f, double w) {
 #pragma unroll
-    for (int k = 0; k < 7; ++k) {
+    for (int k = 0; k < NF; ++k) {
         if (P == 0) acc[k] = (double)(float)(acc[k] + f[k] * w);
         else acc[k] = acc[k] + f[k] * w;
     }
 }
 
-__device__ __forceinline__ void zero7(double* d) {
+template <int NF>
+__device__ __forceinline__ void zerof(double* d) {
 #pragma unroll
-    for (int k = 0; k < 7; ++k) d[k] = 0.0;
+    for (int k = 0; k < NF; ++k) d[k] = 0.0;
 }
 
-__device__ __forceinline__ bool empty7(const double* d) {
+template <int NF>
+__device__ __forceinline__ bool emptyf(const double* d) {
     bool e = true;
 #pragma unroll
-    for (int k = 0; k < 7; ++k) e = e && (d[k] == 0.0);
+    for (int k = 0; k < NF; ++k) e = e && (d[k] == 0.0);
     return e;
 }
 
+__device__ __forceinline__ void zero7(double* d) { zerof<7>(d); }
+__device__ __forceinline__ bool empty7(const double* d) { return emptyf<7>(d); }
+
 // ngp() (simplehyperoctree.h:1396-1401); *loc = level of the located cell or -1
-template <int P>
+template <int P, int NF = 7>
 __device__ void ngp(const DevTree& t, double x, double y, double z, double* r, int* loc) {
     unsigned fl = 0;
     const uint32_t g = locate_point(t, x, y, z, &fl);
     *loc = (g == NIL) ? -1 : (int)fl;
-    if (g == NIL) zero7(r);
-    else load7<P>(t, g, r);
+    if (g == NIL) zerof<NF>(r);
+    else loadf<P, NF>(t, g, r);
 }
 
 // cic() (simplehyperoctree.h:1412-1469)
-template <int P>
+template <int P, int NF = 7>
 __device__ void cic(const DevTree& t, double x, double y, double z, double* result, int* loc) {
     const double one = 1.0, two = 2.0;
     const double point[3] = {x, y, z};
@@ -67,8 +82,8 @@ __device__ void cic(const DevTree& t, double x, double y, double z, double* resu
     unsigned ilvl = idl, lvl = idl;
     double half = EXTENT / 1.0 * (1.0 / (1.0 * (double)(1ull << idl))) / two;
     unsigned site = 0;
-    double cell[7];
-    zero7(result);
+    double cell[NF];
+    zerof<NF>(result);
     do {
         ilvl = lvl;
         if (!invalid) {
@@ -95,10 +110,10 @@ __device__ void cic(const DevTree& t, double x, double y, double z, double* resu
                                               axis_coord(vertex[2], ilvl), &vl);
                 if (gv != NIL) {
                     lvl = lvl < vl ? lvl : vl;
-                    load7<P>(t, gv, cell);
-                    mac7<P>(result, cell, weight);
+                    loadf<P, NF>(t, gv, cell);
+                    macf<P, NF>(result, cell, weight);
                 } else {
-                    zero7(result);
+                    zerof<NF>(result);
                     break;
                 }
             }
@@ -110,7 +125,7 @@ __device__ void cic(const DevTree& t, double x, double y, double z, double* resu
             idl = lvl;
             idk[0] = ek[0] >> 1; idk[1] = ek[1] >> 1; idk[2] = ek[2] >> 1;
             invalid = false;
-            zero7(result);
+            zerof<NF>(result);
             half = EXTENT / 1.0 * (1.0 / (1.0 * (double)(1ull << idl))) / two;
         }
     } while (lvl < ilvl);
@@ -139,12 +154,12 @@ __device__ __forceinline__ double tsc_axis_w(int ai, double d) {
 
 // tsc() with the cache (simplehyperoctree.h:1554-1644), or without it
 // (UseCache = false, simplehyperoctree.h:1481-1541).
-template <int P, bool UseCache>
+template <int P, bool UseCache, int NF = 7>
 __device__ void tsc(const DevTree& t, TscCache& cache, double x, double y, double z, double* result, int* loc) {
     const double two = 2.0;
     const double point[3] = {x, y, z};
     if (point[0] == 0.0 && point[1] == 0.0 && point[2] == 0.0) {
-        cic<P>(t, 0., 0., 0., result, loc);
+        cic<P, NF>(t, 0., 0., 0., result, loc);
         return;
     }
     uint32_t ek[3] = {axis_coord(x, 32), axis_coord(y, 32), axis_coord(z, 32)};
@@ -156,8 +171,8 @@ __device__ void tsc(const DevTree& t, TscCache& cache, double x, double y, doubl
     uint32_t idk[3] = {ek[0] >> (32 - idl), ek[1] >> (32 - idl), ek[2] >> (32 - idl)};
     unsigned ilvl = idl, lvl = idl;
     double half = EXTENT / 1.0 * (1.0 / (1.0 * (double)(1ull << idl))) / two;
-    double cell[7];
-    zero7(result);
+    double cell[NF];
+    zerof<NF>(result);
     do {
         ilvl = lvl;
         const double twohalves = two * half;
@@ -169,7 +184,7 @@ __device__ void tsc(const DevTree& t, TscCache& cache, double x, double y, doubl
         }
         if (hit) {
             if (cache.dflt) {  // default elements carry Data(): the sum stays zero
-                zero7(result);
+                zerof<NF>(result);
                 return;
             }
             double wx[3], wy[3], wz[3];
@@ -183,8 +198,8 @@ __device__ void tsc(const DevTree& t, TscCache& cache, double x, double y, doubl
             for (int ic = 0; ic < 27; ++ic) {
                 const int ix = ic / 9, iy = (ic / 3) % 3, iz = ic % 3;
                 const double w = wx[ix] * wy[iy] * wz[iz];
-                load7<P>(t, cache.g[ic], cell);
-                mac7<P>(result, cell, w);
+                loadf<P, NF>(t, cache.g[ic], cell);
+                macf<P, NF>(result, cell, w);
             }
             return;
         }
@@ -208,17 +223,17 @@ __device__ void tsc(const DevTree& t, TscCache& cache, double x, double y, doubl
                 unsigned vl = 0;
                 const uint32_t gv = find_cell(t, ilvl, nx[ix], ny[iy], nz[iz], &vl);
                 if (gv == NIL) {
-                    zero7(result);
+                    zerof<NF>(result);
                     break;
                 }
                 lvl = lvl < vl ? lvl : vl;
                 if (lvl < ilvl) {
-                    zero7(result);
+                    zerof<NF>(result);
                     break;
                 }
                 const double w = wx[ix] * wy[iy] * wz[iz];
-                load7<P>(t, gv, cell);
-                mac7<P>(result, cell, w);
+                loadf<P, NF>(t, gv, cell);
+                macf<P, NF>(result, cell, w);
                 if (UseCache) gl[ic] = gv;
             }
             if (UseCache && ic == 27) {
@@ -237,7 +252,7 @@ __device__ void tsc(const DevTree& t, TscCache& cache, double x, double y, doubl
             idk[1] = axis_coord(y, el) >> 1;
             idk[2] = axis_coord(z, el) >> 1;
             invalid = false;
-            zero7(result);
+            zerof<NF>(result);
             half = EXTENT / 1.0 * (1.0 / (1.0 * (double)(1ull << idl))) / two;
         }
     } while (lvl < ilvl);

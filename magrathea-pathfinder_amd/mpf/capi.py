@@ -44,7 +44,9 @@ EXPORTS = ("mpf_create", "mpf_destroy", "mpf_last_error", "mpf_set_precision", "
            "mpf_integrate_surfaces_born", "mpf_integrate_surfaces_born_dev",
            "mpf_prep_last_error", "mpf_prep_create", "mpf_prep_destroy", "mpf_prep_clear", "mpf_prep_import_ascii",
            "mpf_prep_load_records", "mpf_prep_update", "mpf_prep_interpolate", "mpf_prep_correct", "mpf_prep_size",
-           "mpf_prep_records", "mpf_prep_save", "mpf_prep_keys", "mpf_prep_collide", "mpf_prepare_ascii")
+           "mpf_prep_records", "mpf_prep_save", "mpf_prep_keys", "mpf_prep_collide", "mpf_prepare_ascii",
+           "mpf_velocity_begin", "mpf_velocity_deposit", "mpf_velocity_deposit_dev", "mpf_velocity_finish",
+           "mpf_velocity_set", "mpf_velocity_get", "mpf_velocity_clear", "mpf_set_observer_velocity")
 
 CAT_FIELDS = ("phi", "theta", "phi_obs", "theta_obs", "sep", "iter", "z0", "z1", "z2", "z3", "z4", "z5",
               "a11", "a12", "a21", "a22", "status", "nrounds")
@@ -146,6 +148,14 @@ def load_library(path=LIB_PATH):
     lib.mpf_set_precision.argtypes = [vp, i32]
     lib.mpf_load_octree_records.argtypes = [vp, vp, u64]
     lib.mpf_load_octree_fields64.argtypes = [vp, vp, u64]
+    lib.mpf_velocity_begin.argtypes = [vp]
+    lib.mpf_velocity_deposit.argtypes = [vp, vp, vp, u64, i32]
+    lib.mpf_velocity_deposit_dev.argtypes = [vp, vp, vp, u64, i32]
+    lib.mpf_velocity_finish.argtypes = [vp]
+    lib.mpf_velocity_set.argtypes = [vp, vp, vp, u64]
+    lib.mpf_velocity_get.argtypes = [vp, vp, vp, u64]
+    lib.mpf_velocity_clear.argtypes = [vp]
+    lib.mpf_set_observer_velocity.argtypes = [vp, vp]
     lib.mpf_octree_info.argtypes = [vp, vp, vp, vp]
     lib.mpf_set_cosmology.argtypes = [vp, vp, vp, vp, u64]
     lib.mpf_locate.argtypes = [vp, vp, u64, vp, vp]
@@ -653,6 +663,46 @@ class Context:
         f7 = np.ascontiguousarray(f7, dtype=np.float64)
         self._check(self.lib.mpf_load_octree_fields64(self.h, _p(f7), ctypes.c_uint64(f7.shape[0])),
                     "mpf_load_octree_fields64")
+
+    # ---- velocity field (the reference's velocity-field build, hmaps.cpp:359-441)
+    VELOCITY_SCHEMES = {"cic": 1, "tsc": 2}
+
+    def velocity_field(self, particles, scheme="cic"):
+        """Deposits particle sets into the tree and finishes the field:
+        `particles` is an iterable of (pos3, vel3) float arrays (one per
+        particle file), `scheme` "cic" or "tsc" (the parameter velocity_field)."""
+        if scheme not in self.VELOCITY_SCHEMES:
+            raise ValueError("velocity_field must be cic or tsc")
+        self._check(self.lib.mpf_velocity_begin(self.h), "mpf_velocity_begin")
+        for pos, vel in particles:
+            pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+            vel = np.ascontiguousarray(vel, dtype=np.float32).reshape(-1, 3)
+            if pos.shape != vel.shape:
+                raise ValueError("positions and velocities differ in count")
+            self._check(self.lib.mpf_velocity_deposit(self.h, _p(pos), _p(vel), ctypes.c_uint64(pos.shape[0]),
+                                                      self.VELOCITY_SCHEMES[scheme]), "mpf_velocity_deposit")
+        self._check(self.lib.mpf_velocity_finish(self.h), "mpf_velocity_finish")
+
+    def velocity_set(self, vel3, rho=None):
+        vel3 = np.ascontiguousarray(vel3, dtype=np.float32).reshape(-1, 3)
+        r = None if rho is None else np.ascontiguousarray(rho, dtype=np.float32)
+        self._check(self.lib.mpf_velocity_set(self.h, _p(r), _p(vel3), ctypes.c_uint64(vel3.shape[0])),
+                    "mpf_velocity_set")
+
+    def velocity_get(self):
+        """(rho, vel3) in record order."""
+        n = self.octree_info()[0]
+        rho = np.zeros(n, dtype=np.float32)
+        vel = np.zeros((n, 3), dtype=np.float32)
+        self._check(self.lib.mpf_velocity_get(self.h, _p(rho), _p(vel), ctypes.c_uint64(n)), "mpf_velocity_get")
+        return rho, vel
+
+    def velocity_clear(self):
+        self._check(self.lib.mpf_velocity_clear(self.h), "mpf_velocity_clear")
+
+    def set_observer_velocity(self, vobs):
+        v = np.ascontiguousarray(vobs, dtype=np.float64).reshape(3)
+        self._check(self.lib.mpf_set_observer_velocity(self.h, _p(v)), "mpf_set_observer_velocity")
 
     def octree_info(self):
         n = ctypes.c_uint64(0)

@@ -50,16 +50,22 @@ static const u128 INVALID = (((u128)0x7fffffffffffffffULL) << 64) | (u128)0xffff
 static const double C_LIGHT = 299792458.0;          /* constants.h:441-444 */
 static const double C2 = 299792458.0 * 299792458.0; /* constants.h:481-484 */
 
+/* rho, phi, dphidx, dphidy, dphidz, a, dphidt (gravity.h:45-87 tuple
+ * order), then the velocity of the velocity-field build (gravity2.h:45-90),
+ * zero when the octree carries none */
+#define NFD 10
 typedef struct {
-    double v[7]; /* rho, phi, dphidx, dphidy, dphidz, a, dphidt (gravity.h:45-87 tuple order) */
+    double v[NFD];
 } fdata;
-enum { F_RHO, F_PHI, F_DX, F_DY, F_DZ, F_A, F_DT };
+enum { F_RHO, F_PHI, F_DX, F_DY, F_DZ, F_A, F_DT, F_VX, F_VY, F_VZ };
 
 struct mpfo_octree {
     uint64_t n;
     u128* key;
     float* f32;  /* n x 7 */
     double* f64; /* n x 7, optional */
+    float* vel;  /* n x 3 (SI), optional: the velocity-field build */
+    double vobs[3];
 };
 
 /* ------------------------------------------------------------------ keys */
@@ -161,11 +167,12 @@ static inline void load_data(const mpfo_octree* o, int prec, uint64_t i, fdata* 
     } else {
         for (int k = 0; k < 7; ++k) d->v[k] = (double)o->f32[7 * i + k];
     }
+    for (int k = 0; k < 3; ++k) d->v[F_VX + k] = o->vel ? (double)o->vel[3 * i + k] : 0.0;
 }
 
 static inline void zero_data(fdata* d) { memset(d, 0, sizeof(*d)); }
 static inline int data_empty(const fdata* d) {
-    for (int k = 0; k < 7; ++k) if (d->v[k] != 0.0) return 0;
+    for (int k = 0; k < NFD; ++k) if (d->v[k] != 0.0) return 0;
     return 1;
 }
 
@@ -173,9 +180,9 @@ static inline int data_empty(const fdata* d) {
  * precision 0: accumulator is float, so (float)((double)acc + (double)f*w). */
 static inline void mac(fdata* acc, const fdata* f, double w, int prec) {
     if (prec == 0) {
-        for (int k = 0; k < 7; ++k) acc->v[k] = (double)(float)(acc->v[k] + f->v[k] * w);
+        for (int k = 0; k < NFD; ++k) acc->v[k] = (double)(float)(acc->v[k] + f->v[k] * w);
     } else {
-        for (int k = 0; k < 7; ++k) acc->v[k] = acc->v[k] + f->v[k] * w;
+        for (int k = 0; k < NFD; ++k) acc->v[k] = acc->v[k] + f->v[k] * w;
     }
 }
 
@@ -499,6 +506,10 @@ static void integrate(integ_ctx* g, traj_t* traj, int stop_kind, double ref, con
     double ratio = b0->core[CA] * b0->core[CA] * (scale / c) / g->nsteps;
     double dl = locate_extent(g->o, b0->core[CX], b0->core[CY], b0->core[CZ]) * ratio;
     const double a0 = b0->core[CA];
+    /* the observer's velocity along the first direction, velocity-field
+     * build only (integrator.h:537-539, 707-709) */
+    const double* vobs = g->o->vobs;
+    const double v0n = (vobs[0] * b0->core[CKX] + vobs[1] * b0->core[CKY] + vobs[2] * b0->core[CKZ]) / (c * b0->core[CKT]);
     const double phi0c2 = b0->phi / c2;
     const int order_step = g->order;
     while (!data_empty(&data)) {
@@ -546,7 +557,14 @@ static void integrate(integ_ctx* g, traj_t* traj, int stop_kind, double ref, con
         ph.s += sqrt((ph.core[CX] - back->core[CX]) * (ph.core[CX] - back->core[CX]) +
                      (ph.core[CY] - back->core[CY]) * (ph.core[CY] - back->core[CY]) +
                      (ph.core[CZ] - back->core[CZ]) * (ph.core[CZ] - back->core[CZ]));
-        ph.redshift = a0 / ph.core[CA] * (one + phi0c2 - ph.phi / c2) - one;
+        if (g->o->vel) /* Doppler and ISW terms (integrator.h:594-595, 765-766) */
+            ph.redshift = a0 / ph.core[CA] *
+                              (one + phi0c2 - ph.phi / c2 + ph.isw +
+                               (data.v[F_VX] * ph.core[CKX] + data.v[F_VY] * ph.core[CKY] + data.v[F_VZ] * ph.core[CKZ]) /
+                                   (c * ph.core[CKT]) -
+                               v0n) -
+                          one;
+        else ph.redshift = a0 / ph.core[CA] * (one + phi0c2 - ph.phi / c2) - one;
         if (!data_empty(&data)) {
             ratio = ph.core[CA] * ph.core[CA] * (scale / c) / g->nsteps;
             dl = locate_extent(g->o, ph.core[CX], ph.core[CY], ph.core[CZ]) * ratio;
@@ -639,7 +657,121 @@ void mpfo_octree_free(mpfo_octree* o) {
     free(o->key);
     free(o->f32);
     free(o->f64);
+    free(o->vel);
     free(o);
+}
+
+/* The velocity field of the velocity-field build: the octree's rho replaced
+ * (rho, when given) and a velocity per record (vel3, SI), with the
+ * observer's velocity vobs3; vel3 = NULL removes the field. */
+int mpfo_octree_set_velocity(mpfo_octree* o, const float* rho, const float* vel3, uint64_t n, const double* vobs3) {
+    if (n != o->n) return -1;
+    free(o->vel);
+    o->vel = NULL;
+    o->vobs[0] = o->vobs[1] = o->vobs[2] = 0.0;
+    if (rho)
+        for (uint64_t i = 0; i < n; ++i) {
+            o->f32[7 * i + F_RHO] = rho[i];
+            if (o->f64) o->f64[7 * i + F_RHO] = (double)rho[i];
+        }
+    if (vel3) {
+        o->vel = (float*)malloc((n ? n : 1) * 3 * sizeof(float));
+        memcpy(o->vel, vel3, n * 3 * sizeof(float));
+        if (vobs3) for (int k = 0; k < 3; ++k) o->vobs[k] = vobs3[k];
+    }
+    return 0;
+}
+
+/* Hmaps::CreateOctreeVelocityWithCIC / WithTSC (hmaps.h:439-478, 497-541)
+ * for one particle set, then the normalisation by the deposited weight and
+ * the parent fill of cells without weight (hmaps.cpp:389-441).  Particles:
+ * positions in box units and velocities (SI), 3 floats each.  rho and vel3
+ * (record order) hold the running sums on entry when accumulate != 0 (the
+ * sums of earlier particle files, before normalisation) and are cleared
+ * otherwise (rho = 0 in every cell, hmaps.cpp:391).  finish != 0 normalises
+ * and fills from the parents.  One thread, particles in order within each
+ * level and the levels in increasing order: the reference's float
+ * accumulation run sequentially.  scheme: 1 CIC, 2 TSC.  Returns -1 on a
+ * bad scheme. */
+int mpfo_velocity_deposit(const mpfo_octree* o, int scheme, const float* pos, const float* vel, uint64_t np,
+                          int accumulate, int finish, float* rho, float* vel3) {
+    if (scheme != 1 && scheme != 2) return -1;
+    const uint64_t n = o->n;
+    if (!accumulate) {
+        memset(rho, 0, n * sizeof(float));
+        memset(vel3, 0, 3 * n * sizeof(float));
+    }
+    if (n == 0) return 0;
+    unsigned lmin = 64, lmax = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        const unsigned l = key_level(o->key[i]);
+        if (l < lmin) lmin = l;
+        if (l > lmax) lmax = l;
+    }
+    for (unsigned ilvl = lmin; ilvl <= lmax; ++ilvl) {
+        const double half = 0.5 * EXTENT / pow(2, ilvl);
+        const double invextension = 1 / (EXTENT / pow(2, ilvl));
+        const double twohalves = 2 * half;
+        for (uint64_t i = 0; i < np; ++i) {
+            const double p[3] = {(double)pos[3 * i], (double)pos[3 * i + 1], (double)pos[3 * i + 2]};
+            /* CIC: the 8 cells at +-half a cell; TSC: the 27 at 0, +-one cell */
+            const int step = (scheme == 1) ? 2 : 1;
+            const double off = (scheme == 1) ? half : twohalves;
+            for (int iz = -1; iz <= 1; iz += step)
+                for (int iy = -1; iy <= 1; iy += step)
+                    for (int ix = -1; ix <= 1; ix += step) {
+                        const u128 k = key_compute(ilvl, p[0] + off * ix, p[1] + off * iy, p[2] + off * iz);
+                        const uint64_t m = upper_bound(o, k);
+                        const uint64_t c = m - (m > 0);
+                        if (o->key[c] != k) continue;
+                        double vratio;
+                        if (scheme == 1) {
+                            vratio = (1 - fabs(p[0] - key_center(k, 0)) * invextension) *
+                                     (1 - fabs(p[1] - key_center(k, 1)) * invextension) *
+                                     (1 - fabs(p[2] - key_center(k, 2)) * invextension);
+                        } else {
+                            const int ai[3] = {abs(ix), abs(iy), abs(iz)};
+                            double w[3];
+                            for (int d = 0; d < 3; ++d) {
+                                const double dist = fabs((key_center(o->key[c], d) - p[d]) / (twohalves));
+                                w[d] = ai[d] * 0.5 * (1.5 - dist) * (1.5 - dist) + (1 - ai[d]) * (0.75 - dist * dist);
+                            }
+                            vratio = w[0] * w[1] * w[2];
+                        }
+                        rho[c] += vratio;
+                        for (int d = 0; d < 3; ++d) vel3[3 * c + d] += vel[3 * i + d] * vratio;
+                    }
+        }
+    }
+    if (!finish) return 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        double mass = rho[i];
+        mass = mass + (mass == 0);
+        for (int d = 0; d < 3; ++d) vel3[3 * i + d] /= mass;
+    }
+    for (unsigned ilvl = lmin + 1; ilvl <= lmax; ++ilvl)
+        for (uint64_t i = 0; i < n; ++i) {
+            if (key_level(o->key[i]) != ilvl || isnormal(rho[i])) continue;
+            const u128 pk = key_parent(o->key[i]);
+            const uint64_t m = upper_bound(o, pk);
+            const uint64_t c = m - (m > 0);
+            if (!key_containing(o->key[c], pk)) continue; /* find() == end(): not on a parent-closed octree */
+            for (int d = 0; d < 3; ++d) vel3[3 * i + d] = vel3[3 * c + d];
+        }
+    return 0;
+}
+
+/* order 0 ngp, 1 cic, 2 uncached tsc: 10 values per point (the 7 fields,
+ * then the velocity) */
+void mpfo_interp10(const mpfo_octree* o, int order, int precision, const double* xyz, uint64_t m, double* out10) {
+    for (uint64_t i = 0; i < m; ++i) {
+        fdata d;
+        const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+        if (order == 0) ngp(o, precision, x, y, z, &d);
+        else if (order == 1) cic(o, precision, x, y, z, &d);
+        else tsc(o, precision, NULL, x, y, z, &d);
+        for (int k = 0; k < NFD; ++k) out10[NFD * i + k] = d.v[k];
+    }
 }
 
 void mpfo_compute_key(unsigned level, const double* xyz, uint64_t m, uint64_t* lohi) {

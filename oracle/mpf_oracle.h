@@ -30,6 +30,14 @@ enum {
 mpfo_octree* mpfo_octree_new(const void* rec48, uint64_t n);
 int mpfo_octree_set_fields64(mpfo_octree* o, const double* f7, uint64_t n);
 void mpfo_octree_free(mpfo_octree* o);
+/* velocity-field build (gravity2.h cells): rho replaced, velocity per record
+ * (SI), observer velocity for the Doppler term of the redshift */
+int mpfo_octree_set_velocity(mpfo_octree* o, const float* rho, const float* vel3, uint64_t n, const double* vobs3);
+/* particle deposit (scheme 1 CIC, 2 TSC), normalisation and parent fill of
+ * hmaps.h:439-541 and hmaps.cpp:389-441, on record-order rho / vel3 */
+int mpfo_velocity_deposit(const mpfo_octree* o, int scheme, const float* pos, const float* vel, uint64_t np,
+                          int accumulate, int finish, float* rho, float* vel3);
+void mpfo_interp10(const mpfo_octree* o, int order, int precision, const double* xyz, uint64_t m, double* out10);
 
 void mpfo_compute_key(unsigned level, const double* xyz, uint64_t m, uint64_t* lohi);
 void mpfo_key_info(const uint64_t* lohi, uint64_t m, uint32_t* level, double* center3, double* ext, uint64_t* parent);

@@ -7,6 +7,8 @@ cpu_baseline leg of bench.py, as the checker or the timed CPU baseline.
   RefHarness  -- oracle/_ref/libmpfref.so, the reference's own headers
                  instantiated by oracle/ref_harness.cpp (built only where the
                  reference sources exist; travels as a built library)
+  RefVelocity -- oracle/_ref/libmpfrefvel.so, the same integrator compiled
+                 with VELOCITYFIELD (oracle/ref_harness_vel.cpp)
 """
 from __future__ import annotations
 
@@ -18,6 +20,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_LIB = os.path.join(HERE, "libmpforacle.so")
 REF_LIB = os.path.join(HERE, "_ref", "libmpfref.so")
+REFV_LIB = os.path.join(HERE, "_ref", "libmpfrefvel.so")
 NREC = 28
 
 _vp = ctypes.c_void_p
@@ -100,6 +103,38 @@ class Oracle(_Base):
         xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
         out = np.zeros((xyz.shape[0], 7))
         self.lib.mpfo_interp(_vp(h), int(order), int(precision), _p(xyz), _u64(xyz.shape[0]), _p(out))
+        return out
+
+    # ---- velocity field (velocity-field build)
+    def set_velocity(self, h, rho, vel3, vobs=(0.0, 0.0, 0.0)):
+        """Replaces rho (if given) and sets the per-record velocity; vel3 None removes it."""
+        r = None if rho is None else np.ascontiguousarray(rho, dtype=np.float32)
+        v = None if vel3 is None else np.ascontiguousarray(vel3, dtype=np.float32).reshape(-1, 3)
+        vo = np.ascontiguousarray(vobs, dtype=np.float64)
+        n = (r.size if r is not None else v.shape[0])
+        self.lib.mpfo_octree_set_velocity.argtypes = [_vp, _vp, _vp, _u64, _vp]
+        rc = self.lib.mpfo_octree_set_velocity(_vp(h), _p(r), _p(v), _u64(n), _p(vo))
+        if rc != 0:
+            raise ValueError("count differs from the octree's")
+
+    def velocity_deposit(self, h, n, scheme, pos, vel, rho=None, vel3=None, finish=True):
+        """Deposit of one particle set into record-order sums (rho, vel3);
+        with rho None the sums start from zero.  Returns (rho, vel3)."""
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+        vel = np.ascontiguousarray(vel, dtype=np.float32).reshape(-1, 3)
+        acc = rho is not None
+        r = np.array(rho, dtype=np.float32) if acc else np.zeros(n, dtype=np.float32)
+        v = np.array(vel3, dtype=np.float32).reshape(-1, 3) if acc else np.zeros((n, 3), dtype=np.float32)
+        rc = self.lib.mpfo_velocity_deposit(_vp(h), int(scheme), _p(pos), _p(vel), _u64(pos.shape[0]), int(acc),
+                                            int(bool(finish)), _p(r), _p(v))
+        if rc != 0:
+            raise ValueError("scheme must be 1 (CIC) or 2 (TSC)")
+        return r, v
+
+    def interp10(self, h, order, xyz, precision=0):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros((xyz.shape[0], 10))
+        self.lib.mpfo_interp10(_vp(h), int(order), int(precision), _p(xyz), _u64(xyz.shape[0]), _p(out))
         return out
 
     def integrate(self, h, order, table, length, nsteps, stop_kind, stop_value, in9, precision=0, nthreads=1,
@@ -387,5 +422,49 @@ class RefHarness(_Base):
         return {"deg": self.lib.ref_deg(), "pi": self.lib.ref_pi()}
 
 
+class RefVelocity:
+    """The reference's integrator compiled with VELOCITYFIELD (oracle/_ref)."""
+
+    def __init__(self, path=REFV_LIB):
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        L = self.lib = ctypes.CDLL(path)
+        L.refv_octree_new.restype = _vp
+        L.refv_octree_new.argtypes = [_vp, _u64, _vp, _vp]
+        L.refv_octree_free.argtypes = [_vp]
+        L.refv_integrate.restype = _u64
+
+    def tree(self, rec, rho=None, vel3=None):
+        rec = np.ascontiguousarray(rec)
+        r = None if rho is None else np.ascontiguousarray(rho, dtype=np.float32)
+        v = None if vel3 is None else np.ascontiguousarray(vel3, dtype=np.float32).reshape(-1, 3)
+        return self.lib.refv_octree_new(_p(rec), _u64(rec.size), _p(r), _p(v))
+
+    def free(self, h):
+        self.lib.refv_octree_free(_vp(h))
+
+    def interp10(self, h, order, xyz):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros((xyz.shape[0], 10), dtype=np.float32)
+        self.lib.refv_interp(_vp(h), int(order), _p(xyz), _u64(xyz.shape[0]), _p(out))
+        return out
+
+    def integrate(self, h, order, table, length, nsteps, stop_kind, stop_value, in9, vobs=(0.0, 0.0, 0.0),
+                  kitarget=None):
+        tab = np.ascontiguousarray(table, dtype=np.float64)
+        in9 = np.ascontiguousarray(in9, dtype=np.float64).reshape(-1, 9)
+        out = np.zeros((in9.shape[0], NREC))
+        kit = None if kitarget is None else np.ascontiguousarray(kitarget, dtype=np.float64)
+        vo = np.ascontiguousarray(vobs, dtype=np.float64)
+        steps = self.lib.refv_integrate(_vp(h), int(order), _p(tab), _u64(tab.shape[1]), ctypes.c_double(length),
+                                        ctypes.c_uint(nsteps), int(stop_kind), ctypes.c_double(stop_value), _p(kit),
+                                        _p(vo), _p(in9), _u64(in9.shape[0]), _p(out))
+        return out, steps
+
+
 def ref_available():
     return os.path.exists(REF_LIB)
+
+
+def refv_available():
+    return os.path.exists(REFV_LIB)

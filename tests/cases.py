@@ -117,3 +117,31 @@ def points(rec, n=3000, seed=0):
 STOPS = (("redshift", 0.3), ("radius", 0.42), ("none", 0.0), ("lambda", 5e16), ("t", 1.2e17),
          ("a", 0.8), ("plane", 0.3), ("s", 0.33))
 STOP_KIND = {"none": -1, "radius": 0, "lambda": 1, "redshift": 2, "t": 3, "a": 4, "plane": 5, "s": 6}
+
+
+# ---- velocity field (the reference's velocity-field build)
+# Velocities are in units of c: the reference converts the particle
+# velocities with unit_l 1e-2 / (c unit_t) (hmaps.h:378-380) and its Doppler
+# term v.k / (c k_t) is then the line-of-sight velocity over c.
+VOBS = (1.2e-3, -0.7e-3, 0.4e-3)   # observer velocity / c
+
+
+def velocity_field(rec, seed=7):
+    """(rho, vel3) per record: weights in [0, 3) with a fifth of the cells at
+    zero, Gaussian cell velocities of 1e-3 c per axis (float32)."""
+    rng = np.random.default_rng(seed)
+    n = rec.size
+    rho = rng.uniform(0, 3, n).astype(np.float32)
+    rho[rng.random(n) < 0.2] = 0
+    vel = rng.normal(0, 1e-3, (n, 3)).astype(np.float32)
+    return rho, vel
+
+
+def particles(n, seed=0, center=None, spread=0.12):
+    """(pos3, vel3) float32 particles around the refined patch of the test
+    trees: a bulk flow plus a dispersion of 7e-4 c per axis."""
+    rng = np.random.default_rng(seed)
+    c = 0.22 * DIR if center is None else np.asarray(center)
+    pos = (c + rng.uniform(-spread, spread, (n, 3))).astype(np.float32)
+    vel = (np.array([5e-4, -2.7e-4, 1.3e-4]) + rng.normal(0, 7e-4, (n, 3))).astype(np.float32)
+    return pos, vel

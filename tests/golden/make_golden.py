@@ -7,8 +7,11 @@ the deterministic inputs of tests/cases.py.  The fixtures are data only:
 inputs are regenerated by tests/cases.py and checked against the stored
 digest; outputs are the reference's numbers.
 
-    python tests/golden/make_golden.py [tree ... | cones | propagate | bundle | born]
-    (default: keys, cones, propagate, bundle, born and every tree)
+    python tests/golden/make_golden.py [tree ... | cones | propagate | bundle | born | velocity]
+    (default: keys, cones, propagate, bundle, born, velocity and every tree)
+
+The velocity fixture comes from oracle/_ref/libmpfrefvel.so, the reference's
+integrator compiled with VELOCITYFIELD (oracle/ref_harness_vel.cpp).
 """
 import os
 import sys
@@ -21,7 +24,7 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 import cases  # noqa: E402
-from oracle import RefHarness  # noqa: E402
+from oracle import RefHarness, RefVelocity  # noqa: E402
 
 
 def main(names):
@@ -35,6 +38,8 @@ def main(names):
         return bundle(R, tab)
     if names == ["born"]:
         return born(R, tab)
+    if names == ["velocity"]:
+        return velocity(tab)
     if names:
         return trees(R, tab, names)
     # keys: compute() at several levels on random and edge points
@@ -55,10 +60,36 @@ def main(names):
     propagate(R, tab)
     bundle(R, tab)
     born(R, tab)
+    velocity(tab)
     trees(R, tab, cases.TREES)
 
 
 PROP_KINDS = ("redshift", "a", "t", "lambda", "r")
+
+VEL_TREES = ("amr", "partial")
+VEL_STOPS = (("redshift", 0.3), ("none", 0.0), ("radius", 0.42))
+
+
+def velocity(tab):
+    """Integrations of the velocity-field build: the cell velocities and
+    weights of cases.velocity_field and the observer velocity cases.VOBS,
+    every interpolation order and three stop criteria."""
+    V = RefVelocity()
+    out = {}
+    in9 = cases.rays(n_side=5, aperture=6.0)
+    out["in9"] = in9
+    for name in VEL_TREES:
+        rec, _ = cases.tree(name)
+        rho, vel = cases.velocity_field(rec)
+        out[f"{name}_digest"] = np.array(cases.records_digest(rec))
+        h = V.tree(rec, rho, vel)
+        for order in (0, 1, 2):
+            for stop, value in VEL_STOPS:
+                o, steps = V.integrate(h, order, tab, cases.LENGTH, cases.NSTEPS, cases.STOP_KIND[stop], value, in9,
+                                       vobs=cases.VOBS)
+                out[f"{name}_{order}_{stop}"] = o
+        V.free(h)
+    np.savez_compressed(os.path.join(HERE, "velocity.npz"), **out)
 
 
 BUNDLE_VALUES = {"r": [0.15, 0.3, 0.42], "lambda": [2e16, 5e16], "redshift": [0.1, 0.25, 0.33],
