@@ -166,6 +166,45 @@ def jacobian_maps(ctx, w, length, table, n=4194304, nside=2048, zmin=0.1, zmax=2
                          "traffic": profiled_traffic("integrate_split_jacobian_amr256_4m_rays.txt")}}
 
 
+def velocity_field(ctx, rec, n=1 << 24, seed=13):
+    """Velocity field of the velocity-field build: n particles inside random
+    cells of the octree (denser where it refines) deposited with CIC and with
+    TSC at every stored level, then normalised and filled from the parents.
+    Particles resident in HBM; the deposit is the timed part."""
+    from mpf import synth
+    pos, vel = synth.particles_in_cells(rec, n, seed)
+    ncells, lmin, lmax = ctx.octree_info()
+    nlev = lmax - lmin + 1
+    d_pos, d_vel = ctx.dev_alloc(pos.nbytes), ctx.dev_alloc(vel.nbytes)
+    ctx.upload(d_pos, pos)
+    ctx.upload(d_vel, vel)
+    out = {"workload": f"{n} particles in random cells of the octree ({ncells} cells, {nlev} levels), "
+                       f"deposit at every level, normalisation and parent fill"}
+    for scheme, cells in (("cic", 8), ("tsc", 27)):
+        res = []
+        for rep in range(2):   # the second pass is reported (the first also allocates)
+            ctx.synchronize()
+            ctx.velocity_begin()
+            t0 = time.perf_counter()
+            ctx.velocity_deposit_dev(d_pos, d_vel, n, scheme)
+            t1 = time.perf_counter()
+            ctx.velocity_finish()
+            t2 = time.perf_counter()
+            res.append(((t1 - t0) * 1e3, (t2 - t1) * 1e3))
+        dep_ms, fin_ms = res[-1]
+        b_pl = 24 + cells * 4 * 8     # particle read + 4 FP64 atomic adds per cell
+        out[scheme] = {"value": n / (dep_ms / 1e3), "unit": "particles/s (all levels)",
+                       "particle_levels_per_s": n * nlev / (dep_ms / 1e3), "deposit_ms": dep_ms,
+                       "finish_ms": fin_ms, "bytes_per_particle_level": b_pl,
+                       "achieved_GB_per_s": n * nlev * b_pl / (dep_ms / 1e3) / 1e9}
+    rho, _ = ctx.velocity_get()
+    out["cells_with_weight"] = int(np.count_nonzero(rho))
+    ctx.velocity_clear()
+    ctx.dev_free(d_pos)
+    ctx.dev_free(d_vel)
+    return out
+
+
 def catalogue(ctx, w, length, table, n=1048576, zmin=0.1, zmax=1.5, acc=1e-8, ntrain=16384, seed=11):
     """Catalogue root-finding: n sources uniform in comoving volume in the
     cone (circular, the area of the square aperture) at z in [zmin, zmax],
@@ -256,7 +295,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-tier comparison")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the Jacobian/maps and catalogue workloads on the same octree")
-    ap.add_argument("--only", choices=("jacobian_maps", "catalogue"), default=None,
+    ap.add_argument("--only", choices=("jacobian_maps", "catalogue", "velocity"), default=None,
                     help="run one of the extra workloads alone and print its record (profiling runs)")
     ap.add_argument("--launch-config", default=None, metavar="R,B,C,W",
                     help="use this launch configuration (refill, sort bucket, blocks per CU, waves per SIMD) for "
@@ -301,6 +340,10 @@ def main():
         ctx.set_launch_config(surfaces=True, **dict(fixed, waves_per_simd=min(wv, 2)))
     global FIXED_LAUNCH
     FIXED_LAUNCH = fixed
+    if args.only == "velocity":
+        print(json.dumps({"velocity_field": velocity_field(ctx, rec)}), flush=True)
+        ctx.close()
+        return
     if args.only:
         fn = jacobian_maps if args.only == "jacobian_maps" else catalogue
         print(json.dumps({args.only: fn(ctx, w, length, table)}), flush=True)

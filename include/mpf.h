@@ -250,7 +250,8 @@ int mpf_integrate_surfaces_dev(mpf_ctx* ctx, const double* dev_in9, uint64_t n, 
  * mpf_velocity_deposit: Hmaps::CreateOctreeVelocityWithCIC (scheme
  *   MPF_VELOCITY_CIC, hmaps.h:439-478) or WithTSC (MPF_VELOCITY_TSC,
  *   hmaps.h:497-541) of n particles (pos3: box units in the octree frame, as
- *   Hmaps::fill_particles_vectors leaves them, hmaps.h:345-401; vel3: SI),
+ *   Hmaps::fill_particles_vectors leaves them, hmaps.h:345-401; vel3: in units
+ *   of c, its conversion unit_l 1e-2 / (c unit_t), hmaps.h:378-380),
  *   at every level from the coarsest to the finest stored; may be called once
  *   per particle file.  The weights are summed in FP64 (the reference: FP32 in
  *   thread order) and rounded once to FP32.
@@ -259,7 +260,8 @@ int mpf_integrate_surfaces_dev(mpf_ctx* ctx, const double* dev_in9, uint64_t n, 
  *   (hmaps.cpp:422-441); the tree then carries the field.
  * mpf_velocity_set / _get: the field in record order (rho may be NULL: kept).
  * mpf_velocity_clear: the tree carries no field (rho keeps its values).
- * mpf_set_observer_velocity: vobs (SI; the parameters v0x, v0y, v0z of
+ * mpf_set_observer_velocity: vobs in units of c like the cell velocities
+ *   (the Doppler terms divide v.k by c k_t, and |k| = c k_t); the parameters v0x, v0y, v0z of
  *   hmaps.cpp:97-99), used with a velocity field. */
 #define MPF_VELOCITY_CIC 1
 #define MPF_VELOCITY_TSC 2

@@ -84,11 +84,12 @@ struct mpf_ctx {
     mpf::DevTree dt;
     mpf::SplitTree split;
     bool split_ready = false;
-    // velocity field: FP64 sums (4 x n) between begin and finish, the field
+    // velocity field: FP64 sums (n x 4) between begin and finish, the field
     // (3 x n) the tree carries after it, particle staging, parent work array
     DevBuf<double> vacc;
     DevBuf<float> vel, vstage;
     DevBuf<uint32_t> vparent;
+    DevBuf<unsigned char> vwork;
     bool vel_open = false;
     double vobs[3] = {0.0, 0.0, 0.0};
     // cosmology
@@ -179,7 +180,7 @@ int mpf_destroy(mpf_ctx* c) {
     c->io_in.release(); c->io_out.release(); c->io_surf.release(); c->io_rv.release(); c->scratch.release();
     c->order.release(); c->order_work.release(); c->homo.release();
     c->map_acc.release(); c->io_i64.release(); c->io_u32.release();
-    c->vacc.release(); c->vel.release(); c->vstage.release(); c->vparent.release();
+    c->vacc.release(); c->vel.release(); c->vstage.release(); c->vparent.release(); c->vwork.release();
     mpf::split_release(c->split);
     mpf::mlp_destroy(c->mlp);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -291,9 +292,19 @@ int mpf_velocity_begin(mpf_ctx* c) {
     return 0;
 }
 
+// particles in chunks of at most 2^27, each put in Morton order (locality of
+// the tree descent and of the atomic adds; the sums do not depend on it)
 static int velocity_deposit_impl(mpf_ctx* c, const float* dpos, const float* dvel, uint64_t n, int scheme) {
-    for (uint32_t l = c->min_level; l <= c->max_level; ++l)
-        HIPCHK(c, mpf::launch_velocity_deposit(c->dt, scheme, dpos, dvel, n, l, c->vacc.p, c->stream));
+    const uint64_t chunk = std::min<uint64_t>(n, 1ull << 27);
+    const size_t wb = mpf::velocity_order_work_bytes(chunk);
+    if (c->vwork.ensure(wb) != hipSuccess) return fail(c, ERR_NOMEM, "particle order work area");
+    for (uint64_t first = 0; first < n; first += chunk) {
+        const uint64_t m = std::min<uint64_t>(chunk, n - first);
+        const float *ps = nullptr, *vs = nullptr;
+        HIPCHK(c, mpf::launch_velocity_order(dpos + 3 * first, dvel + 3 * first, m, c->vwork.p, &ps, &vs, c->stream));
+        for (uint32_t l = c->min_level; l <= c->max_level; ++l)
+            HIPCHK(c, mpf::launch_velocity_deposit(c->dt, scheme, ps, vs, m, l, c->vacc.p, c->stream));
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }

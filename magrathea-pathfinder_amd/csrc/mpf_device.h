@@ -24,7 +24,7 @@ struct DevTree {
     const uint32_t* elem = nullptr;                  // cell -> record index
     const float* f32 = nullptr;                      // 7 x n, structure of arrays
     const double* f64 = nullptr;                     // 7 x n or null
-    const float* vel = nullptr;                      // 3 x n cell velocity (SI) or null
+    const float* vel = nullptr;                      // 3 x n cell velocity / c, or null
     uint64_t n = 0;
 };
 

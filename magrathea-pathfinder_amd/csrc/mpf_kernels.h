@@ -31,7 +31,7 @@ struct IntegrateArgs {
     int stop_kind = -1;
     double stop_value = 0.0;
     double kit[3] = {0.0, 0.0, 0.0};
-    double vobs[3] = {0.0, 0.0, 0.0};  // observer velocity (SI), trees with a velocity field
+    double vobs[3] = {0.0, 0.0, 0.0};  // observer velocity / c, trees with a velocity field
     double* scratch = nullptr;       // per-ray launch records (split tier)
     // constant-parameter surfaces (hmaps.h:615-660) and the infinitesimal Jacobian
     int surf_kind = 0;               // 0 r, 1 lambda, 2 redshift, 3 t, 4 a, 6 s
@@ -50,9 +50,14 @@ struct IntegrateArgs {
 
 hipError_t launch_integrate_exact(int precision, const IntegrateArgs& A, const DevTree& t, int grid, hipStream_t st);
 // velocity field (mpf_velocity.hip): one deposit pass at level lvl into the
-// 4 x n FP64 sums acc; then the normalisation and the parent fill, with
+// n x 4 FP64 sums acc; then the normalisation and the parent fill, with
 // level_first[l - min_level] the first level-sorted cell of level l
 // (max_level - min_level + 2 entries) and parent an n-entry work array
+// particles in Morton order of their position: sorted copies of pos and vel
+// in the work area (velocity_order_work_bytes(n) bytes, n < 2^31)
+size_t velocity_order_work_bytes(uint64_t n);
+hipError_t launch_velocity_order(const float* pos, const float* vel, uint64_t n, void* work, const float** pos_s,
+                                 const float** vel_s, hipStream_t st);
 hipError_t launch_velocity_deposit(const DevTree& t, int scheme, const float* pos, const float* vel, uint64_t np,
                                    unsigned lvl, double* acc, hipStream_t st);
 hipError_t launch_velocity_finish(const DevTree& t, const double* acc, float* rho32, double* rho64, float* vel,

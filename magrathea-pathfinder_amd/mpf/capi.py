@@ -683,6 +683,18 @@ class Context:
                                                       self.VELOCITY_SCHEMES[scheme]), "mpf_velocity_deposit")
         self._check(self.lib.mpf_velocity_finish(self.h), "mpf_velocity_finish")
 
+    def velocity_begin(self):
+        self._check(self.lib.mpf_velocity_begin(self.h), "mpf_velocity_begin")
+
+    def velocity_deposit_dev(self, pos_ptr, vel_ptr, n, scheme="cic"):
+        """One particle set already in device memory (3 floats per particle each)."""
+        self._check(self.lib.mpf_velocity_deposit_dev(self.h, ctypes.c_void_p(pos_ptr), ctypes.c_void_p(vel_ptr),
+                                                      ctypes.c_uint64(n), self.VELOCITY_SCHEMES[scheme]),
+                    "mpf_velocity_deposit_dev")
+
+    def velocity_finish(self):
+        self._check(self.lib.mpf_velocity_finish(self.h), "mpf_velocity_finish")
+
     def velocity_set(self, vel3, rho=None):
         vel3 = np.ascontiguousarray(vel3, dtype=np.float32).reshape(-1, 3)
         r = None if rho is None else np.ascontiguousarray(rho, dtype=np.float32)

@@ -280,6 +280,39 @@ def analytic_tree(base_level=6, half=0.5, max_level=None, refine=None, length=1.
     return rec, f[order]
 
 
+# ------------------------------------------------------------ particles
+
+def record_cells(rec):
+    """(level, kx, ky, kz) of cone records from their 128-bit keys."""
+    level = np.zeros(rec.size, dtype=np.int64)
+    k = [np.zeros(rec.size, dtype=np.uint64) for _ in range(3)]
+    for i in range(32):
+        word = rec["hi"] if i < 16 else rec["lo"]
+        nib = (word >> np.uint64(60 - 4 * (i % 16))) & np.uint64(0xF)
+        present = (nib & np.uint64(8)) != 0
+        level += present
+        for d in range(3):
+            bit = (nib >> np.uint64(d)) & np.uint64(1)
+            k[d] = np.where(present, (k[d] << np.uint64(1)) | bit, k[d])
+    return level, k[0], k[1], k[2]
+
+
+def particles_in_cells(rec, n, seed, bulk=(5e-4, -2.7e-4, 1.3e-4), sigma=7e-4):
+    """n particles (float32 positions in box units and velocities in units
+    of c) placed uniformly inside randomly chosen cells of a cone octree, so
+    that they crowd where the tree refines, with a bulk flow plus a Gaussian
+    dispersion per axis."""
+    rng = np.random.default_rng(seed)
+    level, kx, ky, kz = record_cells(rec)
+    pick = rng.integers(0, rec.size, n)
+    size = 4.0 / np.exp2(level[pick].astype(np.float64))
+    pos = np.empty((n, 3), dtype=np.float32)
+    for d, kd in enumerate((kx, ky, kz)):
+        pos[:, d] = -2.0 + (kd[pick].astype(np.float64) + rng.uniform(0.0, 1.0, n)) * size
+    vel = (np.asarray(bulk) + rng.normal(0.0, sigma, (n, 3))).astype(np.float32)
+    return pos, vel
+
+
 # ----------------------------------------------------- benchmark workloads
 
 def _part1by3(v):

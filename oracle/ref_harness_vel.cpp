@@ -9,7 +9,7 @@
  * instantiates the reference templates with the types hmaps.cpp uses in that
  * build (hmaps.cpp:97-99) and marshals plain arrays in and out.  It pins the
  * velocity terms of the C restatement in oracle/mpf_oracle.c and writes the
- * fixture tests/golden/velocity_integrate.npz.  Product code never links it.
+ * fixture tests/golden/velocity.npz.  Product code never links it.
  */
 #include <cstdint>
 #include <cstring>
@@ -62,7 +62,7 @@ extern "C" {
 
 /* n 48-byte cone records (key, then dphidt, a, dphidx, dphidy, dphidz, phi,
  * rho, pad as floats) with rho replaced by rho_override (when given) and the
- * velocity (3 floats per record, SI) set from vel3. */
+ * velocity (3 floats per record, units of c) set from vel3. */
 void* refv_octree_new(const void* rec, uint64_t n, const float* rho_override, const float* vel3) {
     Octree* o = new Octree;
     o->resize(n);
