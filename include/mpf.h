@@ -322,6 +322,22 @@ typedef struct {
      * distance, sampled on the octree's data; the Newton updates still use
      * the ray's own Jacobian.  0: jacobiantype = infinitesimal. */
     uint32_t jacobian_born;
+    /* 1: jacobiantype = bundle (catalogues.h:455-491, 509-523, 666-681):
+     * iterations 0-2 the identity, 3-4 the infinitesimal Jacobian, then and
+     * for the converged source Lensing::dbetadtheta (mpf_jacobian_bundle)
+     * from the observer at the origin, with stop_bundle bundle_kind
+     * (MPF_STOP_RADIUS, _LAMBDA, _REDSHIFT, _T or _A; interpRef taken at the
+     * crossing of the source radius), plane bundle_plane (0 sachs: the ray's
+     * direction there; 1 normal: the crossing point) and half-opening
+     * bundle_opening (rad).  A failed bundle ends the source with observed
+     * angles 42, 42 and separation 22 (status MPF_CAT_OUTSIDE, the _err
+     * file; iterations 0 and redshifts 0 where the reference leaves them
+     * unset).  Runs the bundles' rays through the host (four per source and
+     * round). */
+    uint32_t jacobian_bundle;
+    int32_t bundle_kind;
+    int32_t bundle_plane;
+    double bundle_opening;
 } mpf_newton_cfg;
 /* Per source, MPF_NCAT doubles: the catalogue columns of catalogues.h:844-859
  * plus status and the number of rounds run. */

@@ -906,6 +906,17 @@ int mpf_catalogue_solve_dev(mpf_ctx* c, const double* dev_src6, const double* de
     if (c->ncos <= 0) return fail(c, ERR_STATE, "no cosmology table set");
     if (!(cfg->accuracy > 0) || cfg->max_iterations < 1 || !(cfg->h > 0) || !(cfg->mpc > 0))
         return fail(c, ERR_INVAL, "newton configuration");
+    const bool bundle = cfg->jacobian_bundle != 0;
+    if (bundle) {
+        const int k = cfg->bundle_kind;
+        if (k != MPF_STOP_RADIUS && k != MPF_STOP_LAMBDA && k != MPF_STOP_REDSHIFT && k != MPF_STOP_T &&
+            k != MPF_STOP_A)
+            return fail(c, ERR_INVAL, "stop_bundle must be radius, lambda, redshift, t or a");
+        if (cfg->bundle_plane != 0 && cfg->bundle_plane != 1) return fail(c, ERR_INVAL, "plane must be sachs or normal");
+        if (!(cfg->bundle_opening > 0) || !(cfg->bundle_opening < 1.5))
+            return fail(c, ERR_INVAL, "bundle opening out of range");
+        if (cfg->jacobian_born) return fail(c, ERR_INVAL, "jacobiantype is bundle or infinitesimal_born, not both");
+    }
     HIPCHK(c, hipSetDevice(c->cfg.device));
     if (total_rays) *total_rays = 0;
     if (n == 0) return 0;
@@ -931,20 +942,20 @@ int mpf_catalogue_solve_dev(mpf_ctx* c, const double* dev_src6, const double* de
         i += (nn > 1) * ((i <= 0) - (i >= nn));
         a0 = (nn > 1) ? (ca[i - 1] + ((ca[i] - ca[i - 1]) * (0.0 - ct[i - 1]) / (ct[i] - ct[i - 1]))) : ca[0];
     }
-    DevBuf<double> dist, ang, in9, dact, surf, outb;
+    DevBuf<double> dist, ang, in9, dact, surf, outb, bctx, jb;
     DevBuf<uint32_t> act, next, need;
     DevBuf<unsigned int> cnt;
-    if (dist.ensure(n) || ang.ensure(2 * n) || in9.ensure(9 * n) || dact.ensure(n) ||
-        surf.ensure((uint64_t)MPF_NSURF * n) || act.ensure(n) || next.ensure(n) || need.ensure(n) || cnt.ensure(2) ||
-        outb.ensure((uint64_t)MPF_NOUT * n)) {
-        dist.release(); ang.release(); in9.release(); dact.release(); surf.release(); act.release(); next.release();
-        need.release(); cnt.release(); outb.release();
-        return fail(c, ERR_NOMEM, "catalogue buffers");
-    }
     auto release = [&]() {
         dist.release(); ang.release(); in9.release(); dact.release(); surf.release(); act.release(); next.release();
-        need.release(); cnt.release(); outb.release();
+        need.release(); cnt.release(); outb.release(); bctx.release(); jb.release();
     };
+    if (dist.ensure(n) || ang.ensure(2 * n) || in9.ensure(9 * n) || dact.ensure(n) ||
+        surf.ensure((uint64_t)MPF_NSURF * n) || act.ensure(n) || next.ensure(n) || need.ensure(n) || cnt.ensure(3) ||
+        outb.ensure((uint64_t)MPF_NOUT * n) ||
+        (bundle && (bctx.ensure((uint64_t)mpf::CAT_BCTX * n) || jb.ensure(4 * n)))) {
+        release();
+        return fail(c, ERR_NOMEM, "catalogue buffers");
+    }
 #define CATCHK(expr)                                  \
     do {                                              \
         hipError_t e_ = (expr);                       \
@@ -961,10 +972,45 @@ int mpf_catalogue_solve_dev(mpf_ctx* c, const double* dev_src6, const double* de
     R.max_iter = (int)cfg->max_iterations; R.acc = cfg->accuracy; R.length = c->cfg.length; R.h = cfg->h;
     R.mpc = cfg->mpc; R.a0 = a0; R.phi0 = phi0;
     for (int k = 0; k < 3; ++k) R.vobs[k] = cfg->vobs[k];
+    R.bundle = bundle ? 1 : 0; R.bkind = cfg->bundle_kind; R.bplane = cfg->bundle_plane; R.bctx = bctx.p;
+    // the bundle Jacobians (jacobiantype = bundle) of the sources in ids[0, m): their current
+    // angles and crossing context read back, mpf_jacobian_bundle on the host, the result in jb
+    std::vector<double> h_dist;
+    auto bundle_jacobians = [&](const uint32_t* d_ids, uint64_t m) -> int {
+        std::vector<uint32_t> hid(m);
+        std::vector<double> hang(2 * n), hctx((uint64_t)mpf::CAT_BCTX * n);
+        if (hipMemcpyAsync(hid.data(), d_ids, m * 4, hipMemcpyDeviceToHost, c->stream) ||
+            hipMemcpyAsync(hang.data(), ang.p, 2 * n * 8, hipMemcpyDeviceToHost, c->stream) ||
+            hipMemcpyAsync(hctx.data(), bctx.p, (uint64_t)mpf::CAT_BCTX * n * 8, hipMemcpyDeviceToHost, c->stream) ||
+            hipStreamSynchronize(c->stream))
+            return fail(c, ERR_HIP, "bundle context download");
+        if (h_dist.empty()) {
+            h_dist.resize(n);
+            if (hipMemcpy(h_dist.data(), dist.p, n * 8, hipMemcpyDeviceToHost)) return fail(c, ERR_HIP, "distances");
+        }
+        std::vector<double> pt(2 * m), ki(3 * m), ref(m), dd(m), J(4 * m);
+        for (uint64_t q = 0; q < m; ++q) {
+            const uint64_t i = hid[q];
+            const double* b = hctx.data() + (uint64_t)mpf::CAT_BCTX * i;
+            pt[2 * q] = hang[2 * i]; pt[2 * q + 1] = hang[2 * i + 1];
+            for (int k = 0; k < 3; ++k) ki[3 * q + k] = b[3 + k];
+            ref[q] = b[6];
+            dd[q] = h_dist[i];
+        }
+        const double origin[3] = {0.0, 0.0, 0.0};
+        if (int rc = mpf_jacobian_bundle(c, pt.data(), m, origin, cfg->bundle_kind, ref.data(), 1, ki.data(),
+                                         dd.data(), cfg->bundle_opening, J.data()))
+            return rc;
+        if (hipMemcpyAsync(jb.p, J.data(), 4 * m * 8, hipMemcpyHostToDevice, c->stream))
+            return fail(c, ERR_HIP, "bundle Jacobian upload");
+        rays += 4 * m;
+        return 0;
+    };
     unsigned int zero2[2] = {0u, 0u};
     CATCHK(hipMemcpyAsync(cnt.p + 1, zero2, 4, hipMemcpyHostToDevice, c->stream));
     for (int round = 0; nact > 0 && round <= (int)cfg->max_iterations; ++round) {
-        const int jac = (round >= 3 || cfg->jacobian_all_rounds) ? 1 : 0;
+        // the bundle replaces the ray's own Jacobian from iteration 5 (catalogues.h:509-523)
+        const int jac = ((round >= 3 || cfg->jacobian_all_rounds) && !(bundle && round >= 5)) ? 1 : 0;
         CATCHK(mpf::launch_cat_launch(ang.p, act.p, nact, in9.p, dact.p, dist.p, c->stream));
         mpf_surfaces sf{};
         sf.kind = MPF_STOP_RADIUS;
@@ -982,11 +1028,28 @@ int mpf_catalogue_solve_dev(mpf_ctx* c, const double* dev_src6, const double* de
         CATCHK(hipStreamSynchronize(c->stream));
         std::swap(act.p, next.p);
         nact = nn;
+        if (bundle && round >= 5 && nact) {  // the pending updates with the bundle Jacobian
+            if (int rc = bundle_jacobians(act.p, nact)) { release(); return rc; }
+            CATCHK(hipMemcpyAsync(cnt.p + 2, zero2, 4, hipMemcpyHostToDevice, c->stream));
+            CATCHK(mpf::launch_newton_bundle(R, act.p, nact, jb.p, next.p, cnt.p + 2, c->stream));
+            CATCHK(hipMemcpyAsync(&nn, cnt.p + 2, 4, hipMemcpyDeviceToHost, c->stream));
+            CATCHK(hipStreamSynchronize(c->stream));
+            std::swap(act.p, next.p);
+            nact = nn;
+        }
     }
     // Jacobian of the sources that converged before the Jacobian rounds
     unsigned int nneed = 0;
     CATCHK(hipMemcpyAsync(&nneed, cnt.p + 1, 4, hipMemcpyDeviceToHost, c->stream));
     CATCHK(hipStreamSynchronize(c->stream));
+    if (bundle) {  // every converged source: the bundle at its converged direction (catalogues.h:666-681)
+        if (nneed) {
+            if (int rc = bundle_jacobians(need.p, nneed)) { release(); return rc; }
+            CATCHK(mpf::launch_cat_jac_bundle(need.p, nneed, jb.p, dev_out, c->stream));
+            CATCHK(hipStreamSynchronize(c->stream));
+        }
+        nneed = 0;
+    }
     if (nneed) {
         CATCHK(mpf::launch_cat_launch(ang.p, need.p, nneed, in9.p, dact.p, dist.p, c->stream));
         mpf_surfaces sf{};

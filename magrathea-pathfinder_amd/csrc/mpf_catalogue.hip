@@ -67,7 +67,68 @@ struct NewtonArgs {
     unsigned int* nneed;
     int round, jac, max_iter;
     double acc, length, h, mpc, vobs[3], a0, phi0;
+    // jacobiantype = bundle: stop_bundle kind, plane (0 sachs, 1 normal) and
+    // per source the crossing's finalPoint, kiTarget and interpRef
+    int bundle, bkind, bplane;
+    double* bctx;            // n x BCTX
 };
+
+enum { BCTX_FP = 0, BCTX_KI = 3, BCTX_REF = 6, BCTX = 8 };
+
+// kiTarget and interpRef of the crossing (catalogues.h:458-491)
+__device__ void bundle_context(const NewtonArgs& N, uint32_t i, const double* s, double D) {
+    double* b = N.bctx + (uint64_t)BCTX * i;
+    b[BCTX_FP] = s[SF_X]; b[BCTX_FP + 1] = s[SF_Y]; b[BCTX_FP + 2] = s[SF_Z];
+    if (N.bplane == 0) { b[BCTX_KI] = s[SF_KX]; b[BCTX_KI + 1] = s[SF_KY]; b[BCTX_KI + 2] = s[SF_KZ]; }
+    else { b[BCTX_KI] = s[SF_X]; b[BCTX_KI + 1] = s[SF_Y]; b[BCTX_KI + 2] = s[SF_Z]; }
+    double ref;
+    switch (N.bkind) {
+        case MPF_STOP_REDSHIFT: ref = s[SF_REDSHIFT]; break;
+        case MPF_STOP_A: ref = s[SF_A]; break;
+        case MPF_STOP_T: ref = s[SF_T]; break;
+        case MPF_STOP_LAMBDA: ref = s[SF_LAMBDA]; break;
+        default: ref = D; break;
+    }
+    b[BCTX_REF] = ref;
+}
+
+// the Newton update in the (e1, e2) plane at (phi, theta) (catalogues.h:627-660)
+// with the inverse Jacobian j; false when the source is rejected
+__device__ bool newton_update(const NewtonArgs& N, uint32_t i, double fx, double fy, double fz, double j00,
+                              double j01, double j10, double j11) {
+    const double* tt = N.src + 6 * i;
+    double* o = N.out + (uint64_t)MPF_NCAT * i;
+    const double D = N.dist[i];
+    const double phi = N.ang[2 * i], theta = N.ang[2 * i + 1];
+    const double cp = cos(phi), sp = sin(phi), ct = cos(theta), st = sin(theta);
+    const double e1[3] = {-sp, cp, 0.0};
+    const double e2[3] = {-cp * ct, -sp * ct, st};
+    const double tg[3] = {D * cp * st, D * sp * st, D * ct};
+    const double sv[3] = {fx - tt[0], fy - tt[1], fz - tt[2]};
+    const double pv[3] = {tg[0] - tt[0], tg[1] - tt[1], tg[2] - tt[2]};
+    const double sepx = sv[0] * e1[0] + sv[1] * e1[1];
+    const double sepy = sv[0] * e2[0] + sv[1] * e2[1] + sv[2] * e2[2];
+    const double posx = pv[0] * e1[0] + pv[1] * e1[1];
+    const double posy = pv[0] * e2[0] + pv[1] * e2[1] + pv[2] * e2[2];
+    const double sepxnew = j00 * sepx + j01 * sepy;
+    const double sepynew = j10 * sepx + j11 * sepy;
+    const double posxnew = posx - sepxnew;
+    const double posynew = posy - sepynew;
+    const double xn = tt[0] + (posxnew * e1[0] + posynew * e2[0]);
+    const double yn = tt[1] + (posxnew * e1[1] + posynew * e2[1]);
+    const double zn = tt[2] + posynew * e2[2];
+    const double nphi = atan2(yn, xn);
+    const double ntheta = acos(zn / sqrt(xn * xn + yn * yn + zn * zn));
+    N.ang[2 * i] = nphi;
+    N.ang[2 * i + 1] = ntheta;
+    if (N.round + 1 > N.max_iter) {  // rejected (catalogues.h:775-778)
+        o[MPF_CAT_PHI_OBS] = nphi; o[MPF_CAT_THETA_OBS] = ntheta;
+        o[MPF_CAT_ITER] = 42;
+        o[MPF_CAT_STATUS] = MPF_CAT_REJECTED;
+        return false;
+    }
+    return true;
+}
 
 __global__ void newton_kernel(NewtonArgs N) {
     const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -96,40 +157,19 @@ __global__ void newton_kernel(NewtonArgs N) {
     o[MPF_CAT_SEP] = sep;
     o[MPF_CAT_NROUNDS] = N.round + 1;
     if (sep > N.acc) {
+        if (N.bundle && N.round >= 5) {
+            // the bundle Jacobian of this crossing, computed on the host, then newton_bundle_kernel
+            bundle_context(N, i, s, D);
+            N.next[atomicAdd(N.nnext, 1u)] = i;
+            return;
+        }
         double j00 = 1, j01 = 0, j10 = 0, j11 = 1;
         if (N.round >= 3) {  // inverse of the infinitesimal Jacobian (utility.h:687-698), a21 = a12
             const double a11 = s[SF_A11], a12 = s[SF_A12], a21 = s[SF_A12], a22 = s[SF_A22];
             const double invdet = 1. / (a11 * a22 - a12 * a21);
             j00 = a22 * invdet; j01 = -a12 * invdet; j10 = -a21 * invdet; j11 = a11 * invdet;
         }
-        const double cp = cos(phi), sp = sin(phi), ct = cos(theta), st = sin(theta);
-        const double e1[3] = {-sp, cp, 0.0};
-        const double e2[3] = {-cp * ct, -sp * ct, st};
-        const double tg[3] = {D * cp * st, D * sp * st, D * ct};
-        const double sv[3] = {fx - tt[0], fy - tt[1], fz - tt[2]};
-        const double pv[3] = {tg[0] - tt[0], tg[1] - tt[1], tg[2] - tt[2]};
-        const double sepx = sv[0] * e1[0] + sv[1] * e1[1];
-        const double sepy = sv[0] * e2[0] + sv[1] * e2[1] + sv[2] * e2[2];
-        const double posx = pv[0] * e1[0] + pv[1] * e1[1];
-        const double posy = pv[0] * e2[0] + pv[1] * e2[1] + pv[2] * e2[2];
-        const double sepxnew = j00 * sepx + j01 * sepy;
-        const double sepynew = j10 * sepx + j11 * sepy;
-        const double posxnew = posx - sepxnew;
-        const double posynew = posy - sepynew;
-        const double xn = tt[0] + (posxnew * e1[0] + posynew * e2[0]);
-        const double yn = tt[1] + (posxnew * e1[1] + posynew * e2[1]);
-        const double zn = tt[2] + posynew * e2[2];
-        const double nphi = atan2(yn, xn);
-        const double ntheta = acos(zn / sqrt(xn * xn + yn * yn + zn * zn));
-        N.ang[2 * i] = nphi;
-        N.ang[2 * i + 1] = ntheta;
-        if (N.round + 1 > N.max_iter) {  // rejected (catalogues.h:775-778)
-            o[MPF_CAT_PHI_OBS] = nphi; o[MPF_CAT_THETA_OBS] = ntheta;
-            o[MPF_CAT_ITER] = 42;
-            o[MPF_CAT_STATUS] = MPF_CAT_REJECTED;
-            return;
-        }
-        N.next[atomicAdd(N.nnext, 1u)] = i;
+        if (newton_update(N, i, fx, fy, fz, j00, j01, j10, j11)) N.next[atomicAdd(N.nnext, 1u)] = i;
         return;
     }
     // converged (catalogues.h:665-719)
@@ -137,7 +177,10 @@ __global__ void newton_kernel(NewtonArgs N) {
     o[MPF_CAT_THETA_OBS] = theta;
     o[MPF_CAT_ITER] = N.round + 1;
     o[MPF_CAT_STATUS] = MPF_CAT_OK;
-    if (!N.jac) N.need_jac[atomicAdd(N.nneed, 1u)] = i;
+    if (N.bundle) {  // the converged Jacobian is the bundle's (catalogues.h:666-667): final pass
+        bundle_context(N, i, s, D);
+        N.need_jac[atomicAdd(N.nneed, 1u)] = i;
+    } else if (!N.jac) N.need_jac[atomicAdd(N.nneed, 1u)] = i;
     const double as = s[SF_A];
     const double z1 = s[SF_REDSHIFT];
     const double potential = s[SF_PHI], isw = s[SF_ISW];
@@ -165,6 +208,48 @@ __global__ void newton_kernel(NewtonArgs N) {
     o[MPF_CAT_Z3] = o[MPF_CAT_Z2] + 0.5 * (vx * vx + vy * vy + vz * vz - vo[0] * vo[0] - vo[1] * vo[1] - vo[2] * vo[2]) / as;
     o[MPF_CAT_Z4] = o[MPF_CAT_Z3] + isw / as;
     o[MPF_CAT_Z5] = gK / gref - 1;
+}
+
+// rounds >= 5 with jacobiantype = bundle: the update with the inverse of the
+// bundle Jacobian J (4 per pending source, from mpf_jacobian_bundle), or the
+// failure 42, 42, 22 (catalogues.h:518-523)
+__global__ void newton_bundle_kernel(NewtonArgs N, const uint32_t* __restrict__ pend, uint64_t npend,
+                                     const double* __restrict__ J, uint32_t* __restrict__ next2,
+                                     unsigned int* __restrict__ nnext2) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= npend) return;
+    const uint32_t i = pend[k];
+    const double* j = J + 4 * k;
+    double* o = N.out + (uint64_t)MPF_NCAT * i;
+    o[MPF_CAT_A11] = j[0]; o[MPF_CAT_A12] = j[1]; o[MPF_CAT_A21] = j[2]; o[MPF_CAT_A22] = j[3];
+    if (j[0] == 42 && j[1] == 42) {
+        o[MPF_CAT_PHI_OBS] = 42; o[MPF_CAT_THETA_OBS] = 42; o[MPF_CAT_SEP] = 22;
+        o[MPF_CAT_ITER] = 0;  // left unset by the reference
+        o[MPF_CAT_STATUS] = MPF_CAT_OUTSIDE;
+        return;
+    }
+    const double a11 = j[0], a12 = j[1], a21 = j[2], a22 = j[3];
+    const double invdet = 1. / (a11 * a22 - a12 * a21);
+    const double* b = N.bctx + (uint64_t)BCTX * i;
+    if (newton_update(N, i, b[BCTX_FP], b[BCTX_FP + 1], b[BCTX_FP + 2], a22 * invdet, -a12 * invdet, -a21 * invdet,
+                      a11 * invdet))
+        next2[atomicAdd(nnext2, 1u)] = i;
+}
+
+// the converged sources' bundle Jacobian (catalogues.h:666-681)
+__global__ void cat_jac_bundle_kernel(const uint32_t* __restrict__ ids, uint64_t m, const double* __restrict__ J,
+                                      double* __restrict__ out) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    const double* j = J + 4 * k;
+    double* o = out + (uint64_t)MPF_NCAT * ids[k];
+    o[MPF_CAT_A11] = j[0]; o[MPF_CAT_A12] = j[1]; o[MPF_CAT_A21] = j[2]; o[MPF_CAT_A22] = j[3];
+    if (j[0] == 42 && j[1] == 42) {
+        o[MPF_CAT_PHI_OBS] = 42; o[MPF_CAT_THETA_OBS] = 42; o[MPF_CAT_SEP] = 22;
+        o[MPF_CAT_ITER] = 0;  // left unset by the reference
+        o[MPF_CAT_STATUS] = MPF_CAT_OUTSIDE;
+        for (int q = MPF_CAT_Z0; q <= MPF_CAT_Z5; ++q) o[q] = 0.0;
+    }
 }
 
 // Jacobian of the converged ray, from a pass with the Jacobian on
@@ -212,14 +297,34 @@ hipError_t launch_cat_launch(const double* ang, const uint32_t* act, uint64_t na
     return hipGetLastError();
 }
 
-hipError_t launch_newton(const CatRound& R, hipStream_t st) {
+static NewtonArgs newton_args(const CatRound& R) {
     NewtonArgs N;
     N.src = R.src; N.dist = R.dist; N.ang = R.ang; N.out = R.out; N.surf = R.surf; N.act = R.act; N.nact = R.nact;
     N.next = R.next; N.nnext = R.nnext; N.need_jac = R.need_jac; N.nneed = R.nneed;
     N.round = R.round; N.jac = R.jac; N.max_iter = R.max_iter; N.acc = R.acc; N.length = R.length; N.h = R.h;
     N.mpc = R.mpc; N.a0 = R.a0; N.phi0 = R.phi0;
     for (int k = 0; k < 3; ++k) N.vobs[k] = R.vobs[k];
+    N.bundle = R.bundle; N.bkind = R.bkind; N.bplane = R.bplane; N.bctx = R.bctx;
+    return N;
+}
+
+hipError_t launch_newton(const CatRound& R, hipStream_t st) {
+    const NewtonArgs N = newton_args(R);
     if (R.nact) hipLaunchKernelGGL(newton_kernel, dim3(blocks_for(R.nact)), dim3(256), 0, st, N);
+    return hipGetLastError();
+}
+
+hipError_t launch_newton_bundle(const CatRound& R, const uint32_t* pend, uint64_t npend, const double* J,
+                                uint32_t* next2, unsigned int* nnext2, hipStream_t st) {
+    const NewtonArgs N = newton_args(R);
+    if (npend)
+        hipLaunchKernelGGL(newton_bundle_kernel, dim3(blocks_for(npend)), dim3(256), 0, st, N, pend, npend, J, next2,
+                           nnext2);
+    return hipGetLastError();
+}
+
+hipError_t launch_cat_jac_bundle(const uint32_t* ids, uint64_t m, const double* J, double* out, hipStream_t st) {
+    if (m) hipLaunchKernelGGL(cat_jac_bundle_kernel, dim3(blocks_for(m)), dim3(256), 0, st, ids, m, J, out);
     return hipGetLastError();
 }
 

@@ -92,12 +92,21 @@ struct CatRound {
     unsigned int* nneed = nullptr;
     int round = 0, jac = 0, max_iter = 10;
     double acc = 1e-8, length = 1.0, h = 0.7, mpc = 3.08568e22, vobs[3] = {0, 0, 0}, a0 = 1.0, phi0 = 0.0;
+    // jacobiantype = bundle: stop_bundle (MPF_STOP_*), plane (0 sachs, 1 normal), and the
+    // per-source crossing context (n x 8: finalPoint, kiTarget, interpRef) the host reads
+    int bundle = 0, bkind = 0, bplane = 0;
+    double* bctx = nullptr;
 };
+constexpr int CAT_BCTX = 8;
 hipError_t launch_cat_init(const double* src, const double* init, uint64_t n, double* dist, double* ang, double* out,
                            hipStream_t st);
 hipError_t launch_cat_launch(const double* ang, const uint32_t* act, uint64_t nact, double* in9, double* dist_act,
                              const double* dist, hipStream_t st);
 hipError_t launch_newton(const CatRound& R, hipStream_t st);
+// rounds >= 5 with the bundle Jacobian: J (4 per pending source), survivors to next2
+hipError_t launch_newton_bundle(const CatRound& R, const uint32_t* pend, uint64_t npend, const double* J,
+                                uint32_t* next2, unsigned int* nnext2, hipStream_t st);
+hipError_t launch_cat_jac_bundle(const uint32_t* ids, uint64_t m, const double* J, double* out, hipStream_t st);
 hipError_t launch_cat_jac(const uint32_t* act, uint64_t nact, const double* surf, double* out, hipStream_t st);
 hipError_t launch_cat_jac_born(uint64_t n, const double* surf, double* out, hipStream_t st);
 hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v, hipStream_t st);

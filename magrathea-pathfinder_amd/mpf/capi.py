@@ -100,14 +100,25 @@ def make_surfaces(kind, values, jacobian=True, ray_value=None):
 class NewtonCfg(ctypes.Structure):
     _fields_ = [("accuracy", ctypes.c_double), ("max_iterations", ctypes.c_uint32), ("vobs", ctypes.c_double * 3),
                 ("h", ctypes.c_double), ("mpc", ctypes.c_double), ("jacobian_all_rounds", ctypes.c_uint32),
-                ("jacobian_born", ctypes.c_uint32)]
+                ("jacobian_born", ctypes.c_uint32), ("jacobian_bundle", ctypes.c_uint32),
+                ("bundle_kind", ctypes.c_int32), ("bundle_plane", ctypes.c_int32), ("bundle_opening", ctypes.c_double)]
+
+BUNDLE_PLANES = {"sachs": 0, "normal": 1}
 
 
 def make_newton(accuracy=1e-8, max_iterations=10, vobs=(0.0, 0.0, 0.0), h=0.7, mpc=3.08568e22,
-                jacobian_all_rounds=False, jacobian_born=False):
+                jacobian_all_rounds=False, jacobian_born=False, bundle=None):
+    """bundle: None, or (stop_bundle, plane, opening) for jacobiantype = bundle,
+    e.g. ("redshift", "sachs", 1e-4)."""
     c = NewtonCfg()
     c.jacobian_all_rounds = int(bool(jacobian_all_rounds))
     c.jacobian_born = int(bool(jacobian_born))
+    if bundle is not None:
+        kind, plane, opening = bundle
+        c.jacobian_bundle = 1
+        c.bundle_kind = STOP_NAMES[kind] if isinstance(kind, str) else int(kind)
+        c.bundle_plane = BUNDLE_PLANES[plane] if isinstance(plane, str) else int(plane)
+        c.bundle_opening = float(opening)
     c.accuracy = accuracy
     c.max_iterations = max_iterations
     for i in range(3):

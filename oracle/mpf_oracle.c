@@ -64,7 +64,7 @@ struct mpfo_octree {
     u128* key;
     float* f32;  /* n x 7 */
     double* f64; /* n x 7, optional */
-    float* vel;  /* n x 3 (SI), optional: the velocity-field build */
+    float* vel;  /* n x 3 (units of c), optional: the velocity-field build */
     double vobs[3];
 };
 
@@ -662,7 +662,7 @@ void mpfo_octree_free(mpfo_octree* o) {
 }
 
 /* The velocity field of the velocity-field build: the octree's rho replaced
- * (rho, when given) and a velocity per record (vel3, SI), with the
+ * (rho, when given) and a velocity per record (vel3, units of c), with the
  * observer's velocity vobs3; vel3 = NULL removes the field. */
 int mpfo_octree_set_velocity(mpfo_octree* o, const float* rho, const float* vel3, uint64_t n, const double* vobs3) {
     if (n != o->n) return -1;
@@ -685,7 +685,7 @@ int mpfo_octree_set_velocity(mpfo_octree* o, const float* rho, const float* vel3
 /* Hmaps::CreateOctreeVelocityWithCIC / WithTSC (hmaps.h:439-478, 497-541)
  * for one particle set, then the normalisation by the deposited weight and
  * the parent fill of cells without weight (hmaps.cpp:389-441).  Particles:
- * positions in box units and velocities (SI), 3 floats each.  rho and vel3
+ * positions in box units and velocities (units of c), 3 floats each.  rho and vel3
  * (record order) hold the running sums on entry when accumulate != 0 (the
  * sums of earlier particle files, before normalisation) and are cleared
  * otherwise (rho = 0 in every cell, hmaps.cpp:391).  finish != 0 normalises
@@ -1100,16 +1100,52 @@ uint64_t mpfo_surfaces(const mpfo_octree* o, int order, int precision, const dou
 /* one newtonMethod2d call (catalogues.h:410-720) */
 typedef struct {
     double phi_new, theta_new, sep, firstid;
-    int fail, converged;
-    double jac[3];
+    int fail, converged, bundle_fail;
+    double jac[4]; /* a11, a12, a21, a22 */
     int have_jac;
     double z[6];
 } newton_res;
 
+/* jacobiantype = bundle: stop_bundle kind (0 radius, 1 lambda, 2 redshift,
+ * 3 t, 4 a; -1: jacobiantype = infinitesimal), plane 0 sachs / 1 normal, the
+ * bundle's half-opening */
+typedef struct {
+    int kind, plane;
+    double opening;
+} bundle_cfg;
+
+void mpfo_dbetadtheta(const mpfo_octree* o, int order, int precision, const double* cosmo_tab, uint64_t ncos,
+                      double length, unsigned nsteps, int kind, double opening, const double* observer, double phi,
+                      double theta, const double* ref, const double* ki, const double* dist, uint64_t nd,
+                      double* out4);
+
+/* Lensing::dbetadtheta at the crossing of the source radius, with kiTarget
+ * and interpRef of catalogues.h:458-491; returns 1 on its failure entry */
+static int bundle_at(const mpfo_octree* o, int order, int precision, const double* tab, uint64_t ncos, double length,
+                     unsigned nsteps, const bundle_cfg* b, double phi, double theta, double distTarget,
+                     const photon* A, const photon* B, double f, const double* fp, double* jac4) {
+    double ki[3], ref;
+    if (b->plane == 0)
+        for (int k = 0; k < 3; ++k) ki[k] = A->core[CKX + k] * f + B->core[CKX + k] * (1 - f);
+    else
+        for (int k = 0; k < 3; ++k) ki[k] = fp[k];
+    switch (b->kind) {
+        case 2: ref = A->redshift * f + B->redshift * (1 - f); break;
+        case 4: ref = A->core[CA] * f + B->core[CA] * (1 - f); break;
+        case 3: ref = A->core[CT] * f + B->core[CT] * (1 - f); break;
+        case 1: ref = A->lambda * f + B->lambda * (1 - f); break;
+        default: ref = distTarget; break;
+    }
+    const double observer[3] = {0.0, 0.0, 0.0};
+    mpfo_dbetadtheta(o, order, precision, tab, ncos, length, nsteps, b->kind, b->opening, observer, phi, theta, &ref,
+                     ki, &distTarget, 1, jac4);
+    return jac4[0] == 42 && jac4[1] == 42;
+}
+
 static void newton_once(const mpfo_octree* o, int order, int precision, const double* tab, uint64_t ncos,
                         double length, unsigned nsteps, const double* tt, const double* vel, double phi, double theta,
                         unsigned iteration, double accuracy, const double* vobs, double h, double mpc,
-                        newton_res* r) {
+                        const bundle_cfg* bundle, newton_res* r) {
     const double c = C_LIGHT, c2 = C2;
     memset(r, 0, sizeof(*r));
     const double distTarget = sqrt(pow(tt[0], 2) + pow(tt[1], 2) + pow(tt[2], 2));
@@ -1137,9 +1173,20 @@ static void newton_once(const mpfo_octree* o, int order, int precision, const do
     if (r->sep > accuracy) {
         double ji[2][2] = {{1, 0}, {0, 1}};
         if (iteration >= 3) {
-            jacobian_at(o, precision, &t, distTarget, length, r->jac);
+            if (iteration < 5 || bundle->kind < 0) {
+                double j3[3];
+                jacobian_at(o, precision, &t, distTarget, length, j3);
+                r->jac[0] = j3[0]; r->jac[1] = j3[1]; r->jac[2] = j3[1]; r->jac[3] = j3[2];
+            } else if (bundle_at(o, order, precision, tab, ncos, length, nsteps, bundle, phi, theta, distTarget, A, B,
+                                 f, fp, r->jac)) {
+                /* catalogues.h:518-523 */
+                r->have_jac = 1;
+                r->bundle_fail = 1;
+                free(t.p);
+                return;
+            }
             r->have_jac = 1;
-            const double a11 = r->jac[0], a12 = r->jac[1], a21 = r->jac[1], a22 = r->jac[2];
+            const double a11 = r->jac[0], a12 = r->jac[1], a21 = r->jac[2], a22 = r->jac[3];
             const double invdet = 1. / (a11 * a22 - a12 * a21);   /* utility.h:687-698 */
             ji[0][0] = a22 * invdet; ji[0][1] = -a12 * invdet; ji[1][0] = -a21 * invdet; ji[1][1] = a11 * invdet;
         }
@@ -1164,8 +1211,18 @@ static void newton_once(const mpfo_octree* o, int order, int precision, const do
         return;
     }
     r->converged = 1;
-    jacobian_at(o, precision, &t, distTarget, length, r->jac);
     r->have_jac = 1;
+    if (bundle->kind < 0) {
+        double j3[3];
+        jacobian_at(o, precision, &t, distTarget, length, j3);
+        r->jac[0] = j3[0]; r->jac[1] = j3[1]; r->jac[2] = j3[1]; r->jac[3] = j3[2];
+    } else if (bundle_at(o, order, precision, tab, ncos, length, nsteps, bundle, phi, theta, distTarget, A, B, f, fp,
+                         r->jac)) {
+        r->converged = 0; /* catalogues.h:676-681 */
+        r->bundle_fail = 1;
+        free(t.p);
+        return;
+    }
     const double potential = A->phi * f + B->phi * (1 - f);
     const double isw = A->isw * f + B->isw * (1 - f);
     r->z[0] = -1 + 1. / (as);
@@ -1207,6 +1264,7 @@ typedef struct {
     double h, mpc;
     double* out;
     uint64_t begin, end, rays;
+    bundle_cfg bundle;
 } cat_work;
 
 static void* cat_worker(void* arg) {
@@ -1225,12 +1283,15 @@ static void* cat_worker(void* arg) {
         for (;;) {
             newton_res r;
             newton_once(w->o, w->order, w->precision, w->tab, w->ncos, w->length, w->nsteps, s, s + 3, phi, theta,
-                        iteration, w->accuracy, w->vobs, w->h, w->mpc, &r);
+                        iteration, w->accuracy, w->vobs, w->h, w->mpc, &w->bundle, &r);
             w->rays++;
             iteration++;
             o[17] = iteration;
-            if (r.have_jac) { o[12] = r.jac[0]; o[13] = r.jac[1]; o[14] = r.jac[1]; o[15] = r.jac[2]; }
+            if (r.have_jac) { o[12] = r.jac[0]; o[13] = r.jac[1]; o[14] = r.jac[2]; o[15] = r.jac[3]; }
             if (r.fail) { o[2] = 42; o[3] = 42; o[4] = 0; o[5] = r.firstid; o[16] = 1; break; }
+            /* a failed bundle: 42, 42, 22 (catalogues.h:518-523, 676-681); the
+             * reference leaves result[1][1] unset, written 0 here */
+            if (r.bundle_fail) { o[2] = 42; o[3] = 42; o[4] = 22; o[5] = 0; o[16] = 1; break; }
             o[4] = r.sep;
             if (r.converged) {
                 o[2] = phi; o[3] = theta; o[5] = iteration; o[16] = 0;
@@ -1245,10 +1306,10 @@ static void* cat_worker(void* arg) {
     return NULL;
 }
 
-uint64_t mpfo_catalogue(const mpfo_octree* o, int order, int precision, const double* cosmo_tab, uint64_t ncos,
-                        double length, unsigned nsteps, const double* src6, const double* init2, uint64_t n,
-                        double accuracy, unsigned max_iterations, const double* vobs, double h, double mpc,
-                        double* out, int nthreads) {
+uint64_t mpfo_catalogue_bundle(const mpfo_octree* o, int order, int precision, const double* cosmo_tab,
+                               uint64_t ncos, double length, unsigned nsteps, const double* src6, const double* init2,
+                               uint64_t n, double accuracy, unsigned max_iterations, const double* vobs, double h,
+                               double mpc, int bundle_kind, int plane, double opening, double* out, int nthreads) {
     if (nthreads < 1) nthreads = 1;
     if ((uint64_t)nthreads > n) nthreads = n ? (int)n : 1;
     cat_work* w = (cat_work*)calloc((size_t)nthreads, sizeof(cat_work));
@@ -1256,7 +1317,7 @@ uint64_t mpfo_catalogue(const mpfo_octree* o, int order, int precision, const do
     const uint64_t chunk = (n + nthreads - 1) / nthreads;
     for (int k = 0; k < nthreads; ++k) {
         w[k] = (cat_work){o, order, precision, cosmo_tab, ncos, length, nsteps, src6, init2, accuracy, max_iterations,
-                          vobs, h, mpc, out, 0, 0, 0};
+                          vobs, h, mpc, out, 0, 0, 0, {bundle_kind, plane, opening}};
         w[k].begin = chunk * k < n ? chunk * k : n;
         w[k].end = chunk * (k + 1) < n ? chunk * (k + 1) : n;
     }
@@ -1270,6 +1331,14 @@ uint64_t mpfo_catalogue(const mpfo_octree* o, int order, int precision, const do
     free(w);
     free(th);
     return rays;
+}
+
+uint64_t mpfo_catalogue(const mpfo_octree* o, int order, int precision, const double* cosmo_tab, uint64_t ncos,
+                        double length, unsigned nsteps, const double* src6, const double* init2, uint64_t n,
+                        double accuracy, unsigned max_iterations, const double* vobs, double h, double mpc,
+                        double* out, int nthreads) {
+    return mpfo_catalogue_bundle(o, order, precision, cosmo_tab, ncos, length, nsteps, src6, init2, n, accuracy,
+                                 max_iterations, vobs, h, mpc, -1, 0, 0.0, out, nthreads);
 }
 
 void mpfo_interpolate(const double* x, const double* y, uint64_t n, const double* x0, uint64_t m, double* out) {

@@ -31,7 +31,7 @@ mpfo_octree* mpfo_octree_new(const void* rec48, uint64_t n);
 int mpfo_octree_set_fields64(mpfo_octree* o, const double* f7, uint64_t n);
 void mpfo_octree_free(mpfo_octree* o);
 /* velocity-field build (gravity2.h cells): rho replaced, velocity per record
- * (SI), observer velocity for the Doppler term of the redshift */
+ * (units of c), observer velocity for the Doppler term of the redshift */
 int mpfo_octree_set_velocity(mpfo_octree* o, const float* rho, const float* vel3, uint64_t n, const double* vobs3);
 /* particle deposit (scheme 1 CIC, 2 TSC), normalisation and parent fill of
  * hmaps.h:439-541 and hmaps.cpp:389-441, on record-order rho / vel3 */
@@ -72,6 +72,15 @@ uint64_t mpfo_catalogue(const mpfo_octree* o, int order, int precision, const do
                         double length, unsigned nsteps, const double* src6, const double* init2, uint64_t n,
                         double accuracy, unsigned max_iterations, const double* vobs, double h, double mpc,
                         double* out, int nthreads);
+/* The same with jacobiantype = bundle (catalogues.h:455-491, 509-523,
+ * 666-681): iterations >= 5 and the converged Jacobian from
+ * Lensing::dbetadtheta with stop_bundle bundle_kind (0 radius, 1 lambda,
+ * 2 redshift, 3 t, 4 a), plane 0 sachs / 1 normal and the bundle opening;
+ * bundle_kind -1 is jacobiantype = infinitesimal. */
+uint64_t mpfo_catalogue_bundle(const mpfo_octree* o, int order, int precision, const double* cosmo_tab,
+                               uint64_t ncos, double length, unsigned nsteps, const double* src6, const double* init2,
+                               uint64_t n, double accuracy, unsigned max_iterations, const double* vobs, double h,
+                               double mpc, int bundle_kind, int plane, double opening, double* out, int nthreads);
 /* Integrator::launch around one photon (integrator.h:345-382): out9 holds
  * the photon and its count bundle photons. */
 void mpfo_launch_bundle(const double* in9, unsigned count, double angle, double rotation, double* out9);

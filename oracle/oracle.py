@@ -219,17 +219,23 @@ class Oracle(_Base):
         return out
 
     def catalogue(self, h, order, table, length, nsteps, src6, init2=None, accuracy=1e-8, max_iterations=10,
-                  vobs=(0.0, 0.0, 0.0), hubble=0.7, mpc=3.08568e22, precision=0, nthreads=8):
-        """Newton root-finding of catalogues.h:410-800; returns (n x 20 records, rays integrated)."""
+                  vobs=(0.0, 0.0, 0.0), hubble=0.7, mpc=3.08568e22, precision=0, nthreads=8, bundle=None):
+        """Newton root-finding of catalogues.h:410-800; returns (n x 20 records, rays integrated).
+        bundle: None (jacobiantype = infinitesimal) or (stop_bundle kind, plane 0 sachs / 1 normal,
+        opening) for jacobiantype = bundle."""
         tab = np.ascontiguousarray(table, dtype=np.float64)
         src6 = np.ascontiguousarray(src6, dtype=np.float64).reshape(-1, 6)
         i2 = None if init2 is None else np.ascontiguousarray(init2, dtype=np.float64).reshape(-1, 2)
         vo = np.ascontiguousarray(vobs, dtype=np.float64)
         out = np.zeros((src6.shape[0], 20))
-        rays = self.lib.mpfo_catalogue(_vp(h), int(order), int(precision), _p(tab), _u64(tab.shape[1]),
-                                       ctypes.c_double(length), ctypes.c_uint(nsteps), _p(src6), _p(i2),
-                                       _u64(src6.shape[0]), ctypes.c_double(accuracy), ctypes.c_uint(max_iterations),
-                                       _p(vo), ctypes.c_double(hubble), ctypes.c_double(mpc), _p(out), int(nthreads))
+        kind, plane, opening = bundle if bundle is not None else (-1, 0, 0.0)
+        self.lib.mpfo_catalogue_bundle.restype = _u64
+        rays = self.lib.mpfo_catalogue_bundle(_vp(h), int(order), int(precision), _p(tab), _u64(tab.shape[1]),
+                                              ctypes.c_double(length), ctypes.c_uint(nsteps), _p(src6), _p(i2),
+                                              _u64(src6.shape[0]), ctypes.c_double(accuracy),
+                                              ctypes.c_uint(max_iterations), _p(vo), ctypes.c_double(hubble),
+                                              ctypes.c_double(mpc), int(kind), int(plane), ctypes.c_double(opening),
+                                              _p(out), int(nthreads))
         return out, rays
 
     def interpolate(self, x, y, x0):

@@ -104,3 +104,34 @@ def test_catalogue_flrw_first_iteration():
     np.testing.assert_array_equal(out[:, capi.CAT["theta_obs"]], out[:, capi.CAT["theta"]])
     np.testing.assert_allclose(out[:, capi.CAT["a11"]], 1.0, atol=1e-15)
     c.close()
+
+
+@pytest.mark.parametrize("bundle", [("redshift", "sachs", 1e-4), ("radius", "normal", 2e-4)])
+def test_catalogue_bundle_vs_oracle(bundle):
+    """jacobiantype = bundle: converged sources (their Jacobian is the
+    bundle's) and sources run to max_iterations (rounds 5 on update with the
+    bundle Jacobian), against the oracle's restatement in the FP64 tier."""
+    print(f"[case] catalogue bundle={bundle}", flush=True)
+    c, rec, f64 = ctx("amr", capi.PREC_FP64)
+    O = Oracle()
+    h = O.tree(rec, f64)
+    kind = capi.STOP_NAMES[bundle[0]]
+    plane = capi.BUNDLE_PLANES[bundle[1]]
+    for acc, n in ((1e-8, 120), (1e-300, 40)):
+        src = sources(n, seed=13)
+        a, rays = c.catalogue_solve(src, accuracy=acc, bundle=bundle)
+        b, _ = O.catalogue(h, 2, cases.cosmology(), cases.LENGTH, cases.NSTEPS, src, accuracy=acc, precision=1,
+                           bundle=(kind, plane, bundle[2]))
+        np.testing.assert_array_equal(a[:, capi.CAT["status"]], b[:, capi.CAT["status"]])
+        np.testing.assert_array_equal(a[:, capi.CAT["iter"]], b[:, capi.CAT["iter"]])
+        np.testing.assert_array_equal(a[:, capi.CAT["nrounds"]], b[:, 17])
+        done = b[:, capi.CAT["status"]] != capi.CAT_OUTSIDE
+        assert done.sum() > 0.5 * n
+        for f in ("phi_obs", "theta_obs"):
+            np.testing.assert_allclose(a[done, capi.CAT[f]], b[done, capi.CAT[f]], rtol=0, atol=1e-12)
+        for f in ("a11", "a12", "a21", "a22"):
+            np.testing.assert_allclose(a[done, capi.CAT[f]], b[done, capi.CAT[f]], rtol=0, atol=1e-9)
+        if acc < 1e-100:
+            assert np.all(a[done, capi.CAT["status"]] == capi.CAT_REJECTED)
+    O.free(h)
+    c.close()
