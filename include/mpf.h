@@ -279,7 +279,9 @@ int mpf_set_observer_velocity(mpf_ctx* ctx, const double* vobs3);
  * from observer3 towards the unit direction plus or minus tan(opening) along
  * its screen vectors e1, e2, integrated on the device to the last of the nd
  * values of the stop parameter `kind` (MPF_STOP_RADIUS, _LAMBDA, _REDSHIFT,
- * _T, _A: stop_bundle), each ray's position interpolated at every value, and
+ * _T, _A, or _PLANE: the position along the last ki, whole trajectories in an
+ * exact tier, FP64 for a split context), each ray's position interpolated at
+ * every value (along each ki for _PLANE), and
  * the finite differences projected on the screen normal to ki (n x nd x 3,
  * the reference's kiTargets) over 2 dist tan(opening) (dist: n x nd).  ref:
  * n x nd values, the same for every direction (up to 16, monotonic as the

@@ -23,6 +23,7 @@ constexpr int ERR_INVAL = -22;
 constexpr int ERR_NOMEM = -12;
 constexpr int ERR_HIP = -5;
 constexpr int ERR_STATE = -71;
+constexpr int ERR_NOSPC = -28;
 
 __global__ void fill_u64_kernel(unsigned long long* p, uint64_t n, unsigned long long v) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1531,15 +1532,112 @@ static double bundle_param(const double* o, int kind) {
     }
 }
 
+// stop_bundle = plane (lensing.h:201, 281-293): per direction, the four rays'
+// whole trajectories in an exact tier, stopped on the last value along the
+// last ki, each interpolated along every ki; then the finite differences
+static int jacobian_bundle_plane(mpf_ctx* c, const double* phi_theta, uint64_t n, const double* observer3,
+                                 const double* ref, uint32_t nd, const double* ki, const double* dist, double opening,
+                                 double* out4) {
+    const double to = std::tan(opening);
+    uint64_t cap = 1ull << 14;
+    std::vector<double> out((uint64_t)MPF_NOUT * 4), traj;
+    const int prec = c->cfg.precision;
+    if (prec == MPF_PREC_SPLIT) c->cfg.precision = MPF_PREC_FP64;
+    int rc = 0;
+    for (uint64_t i = 0; i < n && !rc; ++i) {
+        double sp, cp, st, ct;
+        ::sincos(phi_theta[2 * i], &sp, &cp);
+        ::sincos(phi_theta[2 * i + 1], &st, &ct);
+        const double t[3] = {cp * st, sp * st, ct};
+        const double e1[3] = {-sp, cp, 0}, e2[3] = {-cp * ct, -sp * ct, st};
+        const double rini[4][3] = {{t[0] + to * e1[0], t[1] + to * e1[1], t[2]},
+                                   {t[0] - to * e1[0], t[1] - to * e1[1], t[2]},
+                                   {t[0] - to * e2[0], t[1] - to * e2[1], t[2] - to * e2[2]},
+                                   {t[0] + to * e2[0], t[1] + to * e2[1], t[2] + to * e2[2]}};
+        double in9[36];
+        for (int k = 0; k < 4; ++k) mpf::cone::launch_towards(observer3, rini[k], in9 + 9 * k);
+        const double* klast = ki + 3 * (i * nd + nd - 1);
+        mpf_stop stop{MPF_STOP_PLANE, ref[i * nd + nd - 1], {klast[0], klast[1], klast[2]}};
+        for (;;) {
+            traj.resize((uint64_t)MPF_NOUT * 4 * cap);
+            rc = mpf_integrate_trajectory(c, in9, 4, &stop, cap, out.data(), traj.data());
+            if (rc != ERR_NOSPC || cap >= (1ull << 22)) break;
+            cap <<= 2;
+        }
+        if (rc) break;
+        double* J = out4 + 4 * (uint64_t)nd * i;
+        for (uint64_t q = 0; q < 4ull * nd; ++q) J[q] = 0.0;
+        double fp[MPF_MAX_SURFACES][4][3];
+        bool failed = false;
+        for (int it = 0; it < 4 && !failed; ++it) {
+            const int64_t len = (int64_t)out[(uint64_t)MPF_NOUT * it + MPF_OUT_NSTEPS] + 1;
+            const double* tr = traj.data() + (uint64_t)MPF_NOUT * cap * it;
+            for (uint32_t r = 0; r < nd && !failed; ++r) {
+                const double* kr = ki + 3 * (i * nd + r);
+                const double v = ref[i * nd + r];
+                auto along = [&](int64_t e) {
+                    const double* x = tr + (uint64_t)MPF_NOUT * e;
+                    return x[MPF_OUT_X] * kr[0] + x[MPF_OUT_Y] * kr[1] + x[MPF_OUT_Z] * kr[2];
+                };
+                if (len < 2 || along(len - 1) < v) {
+                    failed = true;
+                    for (uint32_t q = 0; q < nd; ++q) {
+                        J[4 * q] = 42; J[4 * q + 1] = 42; J[4 * q + 2] = 1; J[4 * q + 3] = 0;
+                    }
+                    break;
+                }
+                int64_t first = 0, cnt = len;  // upper_bound of v along kr
+                while (cnt > 0) {
+                    const int64_t half = cnt / 2;
+                    if (v < along(first + half)) cnt = half;
+                    else { first += half + 1; cnt -= half + 1; }
+                }
+                const int64_t id = first - (first > 0);
+                const double next = along(id + 1), prev = along(id);
+                const double f = (next - v) / (next - prev);
+                const double* a = tr + (uint64_t)MPF_NOUT * id;
+                const double* b = a + MPF_NOUT;
+                fp[r][it][0] = a[MPF_OUT_X] * f + b[MPF_OUT_X] * (1 - f);
+                fp[r][it][1] = a[MPF_OUT_Y] * f + b[MPF_OUT_Y] * (1 - f);
+                fp[r][it][2] = a[MPF_OUT_Z] * f + b[MPF_OUT_Z] * (1 - f);
+            }
+        }
+        if (failed) continue;
+        for (uint32_t r = 0; r < nd; ++r) {
+            double rh[3], rvv[3];
+            for (int d = 0; d < 3; ++d) {
+                rh[d] = fp[r][0][d] - fp[r][1][d];
+                rvv[d] = fp[r][2][d] - fp[r][3][d];
+            }
+            const double* k3 = ki + 3 * (i * nd + r);
+            const double phik = std::atan2(k3[1], k3[0]);
+            const double thetak = std::acos(k3[2] / std::sqrt(k3[0] * k3[0] + k3[1] * k3[1] + k3[2] * k3[2]));
+            double spk, cpk, stk, ctk;
+            ::sincos(phik, &spk, &cpk);
+            ::sincos(thetak, &stk, &ctk);
+            const double f1[3] = {-spk, cpk, 0}, f2[3] = {-cpk * ctk, -spk * ctk, stk};
+            const double dd = dist[i * nd + r];
+            J[4 * r + 0] = (rh[0] * f1[0] + rh[1] * f1[1]) / (2 * dd * to);
+            J[4 * r + 1] = (rvv[0] * f1[0] + rvv[1] * f1[1]) / (2 * dd * to);
+            J[4 * r + 2] = (rh[0] * f2[0] + rh[1] * f2[1] + rh[2] * f2[2]) / (-2 * dd * to);
+            J[4 * r + 3] = (rvv[0] * f2[0] + rvv[1] * f2[1] + rvv[2] * f2[2]) / (-2 * dd * to);
+        }
+    }
+    c->cfg.precision = prec;
+    return rc;
+}
+
 int mpf_jacobian_bundle(mpf_ctx* c, const double* phi_theta, uint64_t n, const double* observer3, int kind,
                         const double* ref, uint32_t nd, const double* ki, const double* dist, double opening,
                         double* out4) {
     if (!c || !observer3 || (n && (!phi_theta || !ref || !ki || !dist || !out4))) return ERR_INVAL;
     if (kind != MPF_STOP_RADIUS && kind != MPF_STOP_LAMBDA && kind != MPF_STOP_REDSHIFT && kind != MPF_STOP_T &&
-        kind != MPF_STOP_A)
-        return fail(c, ERR_INVAL, "stop_bundle must be radius, lambda, redshift, t or a");
+        kind != MPF_STOP_A && kind != MPF_STOP_PLANE)
+        return fail(c, ERR_INVAL, "stop_bundle must be radius, lambda, redshift, t, a or plane");
     if (nd < 1 || nd > MPF_MAX_SURFACES) return fail(c, ERR_INVAL, "1..16 values per direction");
     if (n == 0) return 0;
+    if (kind == MPF_STOP_PLANE)
+        return jacobian_bundle_plane(c, phi_theta, n, observer3, ref, nd, ki, dist, opening, out4);
     // the same values for every direction: surfaces; else one value per direction
     bool same = true;
     for (uint64_t i = 1; i < n && same; ++i)

@@ -1486,8 +1486,10 @@ uint64_t mpfo_propagate(const mpfo_octree* o, int order, int precision, const do
 
 /* -------------------------------------------------------------- bundle Jacobian */
 
-/* value of the stop parameter of element p (lensing.h:205-301) */
-static double bundle_param(const photon* p, int kind) {
+/* value of the stop parameter of element p (lensing.h:205-301); plane
+ * (kind 5): the position along ki (lensing.h:281-293) */
+static double bundle_param_ki(const photon* p, int kind, const double* ki) {
+    if (kind == 5) return p->core[CX] * ki[0] + p->core[CY] * ki[1] + p->core[CZ] * ki[2];
     switch (kind) {
         case 1: return p->lambda;
         case 0: return p->chi;
@@ -1496,6 +1498,7 @@ static double bundle_param(const photon* p, int kind) {
         default: return p->redshift;
     }
 }
+static double bundle_param(const photon* p, int kind) { return bundle_param_ki(p, kind, NULL); }
 
 /* Lensing::dbetadtheta (lensing.h:140-364) for one direction: four rays at
  * +-tan(opening) along the screen vectors e1, e2 of (phi, theta), integrated
@@ -1528,9 +1531,12 @@ void mpfo_dbetadtheta(const mpfo_octree* o, int order, int precision, const doub
         const double norm = sqrt(dx * dx + dy * dy + dz * dz);
         const double in9[9] = {1.0, 0.0, observer[0], observer[1], observer[2], 1.0, dx / norm, dy / norm, dz / norm};
         traj_t t = {NULL, 0, 0, 1};
-        run_full(o, order, precision, cosmo_tab, ncos, length, nsteps, kind, ref[nd - 1], NULL, in9, &t);
+        /* the plane criterion stops at the last value along the last ki (lensing.h:201) */
+        run_full(o, order, precision, cosmo_tab, ncos, length, nsteps, kind, ref[nd - 1],
+                 kind == 5 ? ki + 3 * (nd - 1) : NULL, in9, &t);
         for (uint64_t r = 0; r < nd; ++r) {
-            const double back = t.n ? bundle_param(&t.p[t.n - 1], kind) : 0.0;
+            const double* kr = ki + 3 * r;
+            const double back = t.n ? bundle_param_ki(&t.p[t.n - 1], kind, kr) : 0.0;
             const int fail = (kind == 4) ? (back > ref[r]) : (back < ref[r]);
             if (fail) {
                 if (kind == 4) {
@@ -1548,13 +1554,13 @@ void mpfo_dbetadtheta(const mpfo_octree* o, int order, int precision, const doub
             uint64_t first = 0, len = t.n;
             while (len > 0) {
                 const uint64_t half = len >> 1;
-                const double v = bundle_param(&t.p[first + half], kind);
+                const double v = bundle_param_ki(&t.p[first + half], kind, kr);
                 const int before = (kind == 4) ? (ref[r] > v) : (ref[r] < v);
                 if (before) len = half;
                 else { first = first + half + 1; len = len - half - 1; }
             }
             const uint64_t id = first - (first > 0);
-            const double next = bundle_param(&t.p[id + 1], kind), prev = bundle_param(&t.p[id], kind);
+            const double next = bundle_param_ki(&t.p[id + 1], kind, kr), prev = bundle_param_ki(&t.p[id], kind, kr);
             const double f = (next - ref[r]) / (next - prev);
             for (int d = 0; d < 3; ++d)
                 fp[12 * r + 3 * it + d] = t.p[id].core[CX + d] * f + t.p[id + 1].core[CX + d] * (1 - f);

@@ -77,3 +77,31 @@ def test_dbetadtheta_homogeneous_closed_form():
         expect = np.array([1.0, 0.0, 0.0, 1.0]) / math.sqrt(1 + math.tan(opening) ** 2)
         np.testing.assert_allclose(a, np.tile(expect, (3, 1)), atol=1e-9)
     O.free(h)
+
+
+@pytest.mark.skipif(not ref_available(), reason="reference harness not built")
+def test_dbetadtheta_plane_matches_reference():
+    """stop_bundle = plane (lensing.h:201, 281-293): the four rays stop on the
+    last value along the last ki and are interpolated along each ki; bit for
+    bit the reference's, failure entries included."""
+    O, R = Oracle(), RefHarness()
+    tab = cases.cosmology()
+    rng = np.random.default_rng(3)
+    fails = 0
+    for name, ref in (("amr", [0.1, 0.2, 0.3]), ("wide", [0.2, 0.5, 0.9])):
+        rec, _ = cases.tree(name)
+        ho, hr = O.tree(rec), R.tree(rec)
+        for _ in range(8):
+            phi = math.radians(20) + rng.uniform(-0.2, 0.2)
+            th = math.radians(20) + rng.uniform(-0.2, 0.2)
+            n = np.array([math.cos(phi) * math.sin(th), math.sin(phi) * math.sin(th), math.cos(th)])
+            ki = np.tile(n, (3, 1)) + rng.normal(0, 0.01, (3, 3))
+            for order in (1, 2):
+                a = O.dbetadtheta(ho, order, tab, cases.LENGTH, cases.NSTEPS, 5, 1e-4, [0, 0, 0], phi, th, ref, ki, ref)
+                b = R.dbetadtheta(hr, order, tab, cases.LENGTH, cases.NSTEPS, "plane", 1e-4, [0, 0, 0], phi, th, ref,
+                                  ki, ref)
+                assert np.array_equal(a, b), (name, order)
+                fails += a[0, 0] == 42
+        O.free(ho)
+        R.free(hr)
+    assert 0 < fails < 32

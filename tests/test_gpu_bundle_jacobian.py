@@ -78,3 +78,30 @@ def test_bundle_jacobian_split_vs_fp64():
         assert np.all(np.abs(a - b) <= tol), f"{kind}: max excess {np.max(np.abs(a - b) - tol):.3e}"
     cs.close()
     cf.close()
+
+
+def test_bundle_jacobian_plane_vs_oracle():
+    """stop_bundle = plane: bit for bit the oracle's (itself the reference's,
+    tests/test_bundle_jacobian.py), failure entries included."""
+    print("[case] bundle jacobian plane", flush=True)
+    import math
+    from mpf import capi
+    rec, f64 = cases.tree("wide")
+    c = capi.Context(cases.LENGTH, order=capi.TSC, precision=capi.PREC_REF, nsteps=cases.NSTEPS)
+    c.load_records(rec)
+    c.set_cosmology(cases.cosmology())
+    O = Oracle()
+    h = O.tree(rec)
+    rng = np.random.default_rng(8)
+    n, ref = 10, np.array([0.2, 0.5, 0.9])
+    pt = np.c_[math.radians(20) + rng.uniform(-0.2, 0.2, n), math.radians(20) + rng.uniform(-0.2, 0.2, n)]
+    dirs = np.c_[np.cos(pt[:, 0]) * np.sin(pt[:, 1]), np.sin(pt[:, 0]) * np.sin(pt[:, 1]), np.cos(pt[:, 1])]
+    ki = np.repeat(dirs[:, None, :], 3, axis=1) + rng.normal(0, 0.01, (n, 3, 3))
+    refs = np.tile(ref, (n, 1))
+    got = c.jacobian_bundle(pt, "plane", refs, ki, refs, 1e-4)
+    for i in range(n):
+        want = O.dbetadtheta(h, 2, cases.cosmology(), cases.LENGTH, cases.NSTEPS, 5, 1e-4, [0, 0, 0], pt[i, 0],
+                             pt[i, 1], ref, ki[i], ref)
+        np.testing.assert_array_equal(got[i], want, err_msg=f"direction {i}")
+    O.free(h)
+    c.close()
