@@ -154,6 +154,20 @@ int mpf_integrate(mpf_ctx* ctx, const double* in9, uint64_t n, const mpf_stop* s
  * context stream unless sync != 0. */
 int mpf_integrate_dev(mpf_ctx* ctx, const double* dev_in9, uint64_t n, const mpf_stop* stop, double* dev_out,
                       int sync);
+/* mpf_integrate_dev without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Same kernels, launch
+ * configuration and results, bit for bit; synchronous.  Its workspace (queue
+ * and step counters, the ray order and its sort, the split tier's launch
+ * records) is a fixed device array per GPU, held until the process ends:
+ * at most MPF_STATIC_MAX_RAYS rays per call (else -EINVAL).  Every call on a
+ * GPU shares that array, so calls that overlap on one GPU race.  The split
+ * tier needs its stencil blocks already built (any mpf_integrate_dev call on
+ * the context builds them; else -EPROTO).  total_steps (may be NULL): as
+ * mpf_integrate. */
+#define MPF_STATIC_MAX_RAYS 262144
+int mpf_integrate_dev_static(mpf_ctx* ctx, const double* dev_in9, uint64_t n, const mpf_stop* stop, double* dev_out,
+                             uint64_t* total_steps);
 /* Integrator::propagate (integrator.h:826-1015), the per-ray computation of
  * the rays program (rays.cpp:276-284), exact tiers, any order: around each
  * central photon i (in9, n x 9) a bundle of `count` rays on a circle of

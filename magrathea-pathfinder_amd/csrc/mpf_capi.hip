@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -53,6 +54,62 @@ struct DevBuf {
         return e;
     }
 };
+
+// Workspace of one integration: the queue and step counters, the ray order
+// and the work area of its sort, the split tier's launch records.  The
+// context's own buffers grow on demand (mpf_integrate_dev); the
+// allocation-free entry point (mpf_integrate_dev_static) carves them out of
+// the file-scope array below instead.
+struct Workspace {
+    unsigned long long* counters = nullptr;  // [0] queue, [1] steps, [2] trajectory overflow
+    uint32_t* order = nullptr;
+    void* order_work = nullptr;
+    size_t order_work_bytes = 0;
+    double* records = nullptr;
+};
+
+// The allocation-free integration's workspace: one array per GPU, held until
+// the process ends.  In host code `scratch` names a host-side stand-in, not
+// device memory: the device address of each GPU's copy comes from
+// hipGetSymbolAddress, taken once per device by mpf_create with that device
+// current, and only that address is ever passed to kernels.  Every call on a
+// GPU shares its copy, so calls that overlap on one GPU race.
+constexpr size_t STATIC_SCRATCH_BYTES = 48ull << 20;
+static __device__ __align__(256) unsigned char scratch[STATIC_SCRATCH_BYTES];
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[MAX_DEVICES] = {};
+hipError_t g_scratch_err[MAX_DEVICES] = {};
+
+// the device address of this device's workspace array (device current)
+void scratch_address(int device) {
+    if (device < 0 || device >= MAX_DEVICES) return;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e == hipSuccess && p) g_scratch_ptr[device] = p;
+    else g_scratch_err[device] = (e == hipSuccess) ? hipErrorInvalidSymbol : e;
+}
+
+// Layout of n rays in the static workspace: counters (256 B), the order
+// (4 B per ray), its sort's work area, the launch records (128 B per ray);
+// false when it does not fit.
+bool static_layout(void* base, uint64_t n, bool split, Workspace& w) {
+    unsigned char* p = static_cast<unsigned char*>(base);
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        unsigned char* q = p + off;
+        off += (bytes + 255) / 256 * 256;
+        return q;
+    };
+    w.counters = reinterpret_cast<unsigned long long*>(take(256));
+    w.order = reinterpret_cast<uint32_t*>(take(4 * std::max<uint64_t>(n, 1)));
+    w.order_work_bytes = mpf::ray_order_work_bytes(std::max<uint64_t>(n, 2));
+    w.order_work = take(w.order_work_bytes);
+    w.records = split ? reinterpret_cast<double*>(take(8ull * mpf::split_launch_doubles() * n)) : nullptr;
+    return off <= STATIC_SCRATCH_BYTES;
+}
 
 }  // namespace
 
@@ -164,6 +221,7 @@ int mpf_create(mpf_ctx** out, const mpf_config* cfg) {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, cfg->device) == hipSuccess && cus > 0)
         c->cus = cus;
+    scratch_address(cfg->device);
     if (c->counters.ensure(3) != hipSuccess) {
         delete c;
         return ERR_NOMEM;
@@ -486,7 +544,8 @@ int mpf_interpolate(mpf_ctx* c, int order, const double* xyz, uint64_t n, double
 
 static int integrate_impl(mpf_ctx* c, const double* dev_in9, uint64_t n, const mpf_stop* stop, double* dev_out,
                           const mpf_surfaces* surf, double* dev_surf, int sync, double* dev_traj = nullptr,
-                          uint64_t traj_cap = 0);
+                          uint64_t traj_cap = 0, const Workspace* ws = nullptr);
+static int finish_run(mpf_ctx* c, const unsigned long long* counters);
 
 int mpf_integrate_dev(mpf_ctx* c, const double* dev_in9, uint64_t n, const mpf_stop* stop, double* dev_out, int sync) {
     return integrate_impl(c, dev_in9, n, stop, dev_out, nullptr, nullptr, sync);
@@ -509,8 +568,13 @@ int mpf_integrate_surfaces_dev(mpf_ctx* c, const double* dev_in9, uint64_t n, co
 }
 
 static int integrate_impl(mpf_ctx* c, const double* dev_in9, uint64_t n, const mpf_stop* stop, double* dev_out,
-                          const mpf_surfaces* surf, double* dev_surf, int sync, double* dev_traj, uint64_t traj_cap) {
+                          const mpf_surfaces* surf, double* dev_surf, int sync, double* dev_traj, uint64_t traj_cap,
+                          const Workspace* ws) {
     if (!c || !stop || (n && !dev_in9)) return ERR_INVAL;
+    if (ws && (surf || dev_traj || (n && !dev_out))) return ERR_INVAL;
+    if (ws && c->cfg.precision == MPF_PREC_SPLIT && !c->split_ready)
+        return fail(c, ERR_STATE, "split-tier stencil blocks not built (mpf_integrate_dev builds them on first use)");
+    unsigned long long* counters = ws ? ws->counters : c->counters.p;
     if (n && !dev_out) {
         if (c->io_out.ensure((uint64_t)MPF_NOUT * n) != hipSuccess) return fail(c, ERR_NOMEM, "output buffer");
         dev_out = c->io_out.p;
@@ -523,8 +587,8 @@ static int integrate_impl(mpf_ctx* c, const double* dev_in9, uint64_t n, const m
     A.in9 = dev_in9;
     A.out = dev_out;
     A.n = n;
-    A.queue = c->counters.p;
-    A.steps = c->counters.p + 1;
+    A.queue = counters;
+    A.steps = counters + 1;
     A.ct = c->cosmo.p;
     A.ca = c->cosmo.p + c->ncos;
     A.cdadt = c->cosmo.p + 2 * c->ncos;
@@ -547,7 +611,7 @@ static int integrate_impl(mpf_ctx* c, const double* dev_in9, uint64_t n, const m
     if (dev_traj) {
         A.traj = dev_traj;
         A.traj_cap = traj_cap;
-        A.traj_over = c->counters.p + 2;
+        A.traj_over = counters + 2;
     }
     if (c->cfg.order == MPF_HOMOGENEOUS && c->cfg.precision == MPF_PREC_SPLIT)
         return fail(c, ERR_INVAL, "the homogeneous order runs in the exact tiers (REF, FP64)");
@@ -558,7 +622,7 @@ static int integrate_impl(mpf_ctx* c, const double* dev_in9, uint64_t n, const m
     for (int k = 0; k < 3; ++k) A.vobs[k] = c->vobs[k];
     const mpf_launch_cfg& L = c->launch[surf ? 1 : 0];
     A.refill = L.refill;
-    HIPCHK(c, mpf::launch_fill_u64(c->counters.p, 3, 0ull, c->stream));
+    HIPCHK(c, mpf::launch_fill_u64(counters, 3, 0ull, c->stream));
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     c->kernel_timed = n > 0;
     if (n) {
@@ -572,16 +636,32 @@ static int integrate_impl(mpf_ctx* c, const double* dev_in9, uint64_t n, const m
         }
         // the queue order (timed with the integration)
         if (L.sort_bucket && n > 1 && n < (1ull << 31)) {
-            const size_t wb = mpf::ray_order_work_bytes(n);
-            if (c->order.ensure(n) != hipSuccess || c->order_work.ensure(wb) != hipSuccess)
-                return fail(c, ERR_NOMEM, "ray-order buffers");
+            uint32_t* order;
+            void* work;
+            size_t wb;
+            if (ws) {
+                order = ws->order;
+                work = ws->order_work;
+                wb = ws->order_work_bytes;
+            } else {
+                wb = mpf::ray_order_work_bytes(n);
+                if (c->order.ensure(n) != hipSuccess || c->order_work.ensure(wb) != hipSuccess)
+                    return fail(c, ERR_NOMEM, "ray-order buffers");
+                order = c->order.p;
+                work = c->order_work.p;
+            }
             HIPCHK(c, mpf::launch_ray_order(dev_in9, A.ray_value, stop->kind == MPF_STOP_A ? 1 : 0, n,
-                                            L.sort_bucket, c->order.p, c->order_work.p, wb, c->stream));
-            A.queue_order = c->order.p;
+                                            L.sort_bucket, order, work, wb, c->stream));
+            A.queue_order = order;
         }
         if (c->cfg.precision == MPF_PREC_SPLIT) {
-            if (c->scratch.ensure(16 * n) != hipSuccess) return fail(c, ERR_NOMEM, "launch-record buffer");
-            A.scratch = c->scratch.p;
+            if (ws) {
+                A.scratch = ws->records;
+            } else {
+                if (c->scratch.ensure((uint64_t)mpf::split_launch_doubles() * n) != hipSuccess)
+                    return fail(c, ERR_NOMEM, "launch-record buffer");
+                A.scratch = c->scratch.p;
+            }
             HIPCHK(c, mpf::launch_integrate_split(A, c->dt, c->split, c->cus, L.blocks_per_cu ? L.blocks_per_cu : 8,
                                                   L.waves_per_simd ? L.waves_per_simd : 2, c->stream, c->evk0,
                                                   c->evk1));
@@ -596,8 +676,29 @@ static int integrate_impl(mpf_ctx* c, const double* dev_in9, uint64_t n, const m
     }
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     c->pending = true;
+    if (ws) return finish_run(c, counters);
     if (sync) return mpf_synchronize(c);
     return 0;
+}
+
+int mpf_integrate_dev_static(mpf_ctx* c, const double* dev_in9, uint64_t n, const mpf_stop* stop, double* dev_out,
+                             uint64_t* total_steps) {
+    if (!c || !stop || (n && (!dev_in9 || !dev_out))) return ERR_INVAL;
+    if (n > MPF_STATIC_MAX_RAYS)
+        return fail(c, ERR_INVAL, "mpf_integrate_dev_static takes at most " + std::to_string(MPF_STATIC_MAX_RAYS) +
+                                      " rays per call (its workspace is a fixed device array)");
+    const int dev = c->cfg.device;
+    void* base = (dev >= 0 && dev < MAX_DEVICES) ? g_scratch_ptr[dev] : nullptr;
+    if (!base)
+        return fail(c, ERR_HIP, std::string("static workspace address of this device unavailable: ") +
+                                    hipGetErrorString(dev >= 0 && dev < MAX_DEVICES ? g_scratch_err[dev]
+                                                                                    : hipErrorInvalidDevice));
+    Workspace w;
+    if (!static_layout(base, n, c->cfg.precision == MPF_PREC_SPLIT, w))
+        return fail(c, ERR_NOSPC, "static workspace too small for this batch");
+    const int rc = integrate_impl(c, dev_in9, n, stop, dev_out, nullptr, nullptr, 1, nullptr, 0, &w);
+    if (!rc && total_steps) *total_steps = c->last_steps;
+    return rc;
 }
 
 int mpf_integrate_surfaces_born_dev(mpf_ctx* c, const double* dev_in9, uint64_t n, const mpf_surfaces* surf,
@@ -1381,11 +1482,17 @@ int mpf_healpix_bin_dev(mpf_ctx* c, int64_t nside, const double* dev_in9, const 
 
 int mpf_synchronize(mpf_ctx* c) {
     if (!c) return ERR_INVAL;
+    return finish_run(c, c->counters.p);
+}
+
+// waits for the context stream; after an integration, reads its step counter
+// and timings
+static int finish_run(mpf_ctx* c, const unsigned long long* counters) {
     HIPCHK(c, hipSetDevice(c->cfg.device));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (c->pending) {
         unsigned long long h[2] = {0, 0};
-        HIPCHK(c, hipMemcpy(h, c->counters.p, sizeof(h), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(h, counters, sizeof(h), hipMemcpyDeviceToHost));
         c->last_steps = h[1];
         float ms = 0.f;
         HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));

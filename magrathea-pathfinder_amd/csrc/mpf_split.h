@@ -22,6 +22,9 @@ bool split_build(SplitTree& s, const DevTree& t, const uint8_t* level, const uin
                  const uint32_t* kz, int order, hipStream_t st, std::string& err);
 void split_release(SplitTree& s);
 bool split_supports_surfaces();
+// doubles of launch record per ray that the split integrator's caller
+// provides in IntegrateArgs::scratch
+int split_launch_doubles();
 // features of catalogue sources (learned first guess) from the stencil blocks
 hipError_t launch_features_split(const DevTree& t, const SplitTree& s, const double* src, uint64_t n, double length,
                                  float* feat, hipStream_t st);

@@ -1034,6 +1034,8 @@ bool split_build(SplitTree& s, const DevTree& t, const uint8_t* level, const uin
 
 bool split_supports_surfaces() { return true; }
 
+int split_launch_doubles() { return LAUNCH_DOUBLES; }
+
 void split_release(SplitTree& s) {
     if (s.mem) (void)hipFree(s.mem);
     if (s.nbr) (void)hipFree(s.nbr);

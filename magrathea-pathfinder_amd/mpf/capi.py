@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(PKG_DIR, "libmpf.so")
 
 HOMOGENEOUS, NGP, CIC, TSC = -1, 0, 1, 2
 PREC_REF, PREC_FP64, PREC_SPLIT = 0, 1, 2
+STATIC_MAX_RAYS = 262144   # MPF_STATIC_MAX_RAYS of include/mpf.h
 STOP_NONE, STOP_RADIUS, STOP_LAMBDA, STOP_REDSHIFT, STOP_T, STOP_A, STOP_PLANE, STOP_S = -1, 0, 1, 2, 3, 4, 5, 6
 STOP_NAMES = {"none": -1, "r": 0, "radius": 0, "lambda": 1, "redshift": 2, "t": 3, "eta": 3, "a": 4,
               "plane": 5, "s": 6}
@@ -28,7 +29,7 @@ OUT = {name: i for i, name in enumerate(OUT_FIELDS)}
 # every symbol declared in include/mpf.h
 EXPORTS = ("mpf_create", "mpf_destroy", "mpf_last_error", "mpf_set_precision", "mpf_load_octree_records",
            "mpf_load_octree_fields64", "mpf_octree_info", "mpf_set_cosmology", "mpf_locate", "mpf_interpolate",
-           "mpf_integrate", "mpf_integrate_dev", "mpf_last_run", "mpf_synchronize", "mpf_validate_octree",
+           "mpf_integrate", "mpf_integrate_dev", "mpf_integrate_dev_static", "mpf_last_run", "mpf_synchronize", "mpf_validate_octree",
            "mpf_dev_alloc", "mpf_dev_free", "mpf_dev_upload", "mpf_dev_download", "mpf_dev_copy",
            "mpf_integrate_surfaces", "mpf_integrate_surfaces_dev", "mpf_healpix_pix2vec", "mpf_healpix_vec2pix",
            "mpf_hmaps_launch_dev", "mpf_hmaps_fill_dev", "mpf_healpix_bin_dev", "mpf_catalogue_solve",
@@ -173,6 +174,7 @@ def load_library(path=LIB_PATH):
     lib.mpf_interpolate.argtypes = [vp, i32, vp, u64, vp]
     lib.mpf_integrate.argtypes = [vp, vp, u64, ctypes.POINTER(Stop), vp, vp]
     lib.mpf_integrate_dev.argtypes = [vp, vp, u64, ctypes.POINTER(Stop), vp, i32]
+    lib.mpf_integrate_dev_static.argtypes = [vp, vp, u64, ctypes.POINTER(Stop), vp, ctypes.POINTER(u64)]
     lib.mpf_last_run.argtypes = [vp, vp, vp]
     lib.mpf_synchronize.argtypes = [vp]
     lib.mpf_validate_octree.argtypes = [vp, u64, vp, vp, ctypes.c_char_p, u64]
@@ -935,6 +937,17 @@ class Context:
         self._check(self.lib.mpf_integrate_dev(self.h, ctypes.c_void_p(in_ptr), ctypes.c_uint64(n),
                                                ctypes.byref(st), ctypes.c_void_p(out_ptr), int(bool(sync))),
                     "mpf_integrate_dev")
+
+    def integrate_dev_static(self, in_ptr, n, out_ptr, stop="none", value=0.0, kitarget=(0.0, 0.0, 0.0)):
+        """mpf_integrate_dev_static: the device-resident integration with no
+        device allocation (workspace in a fixed per-GPU array); synchronous,
+        returns the total step count."""
+        st = make_stop(stop, value, kitarget)
+        steps = ctypes.c_uint64(0)
+        self._check(self.lib.mpf_integrate_dev_static(self.h, ctypes.c_void_p(in_ptr), ctypes.c_uint64(n),
+                                                      ctypes.byref(st), ctypes.c_void_p(out_ptr),
+                                                      ctypes.byref(steps)), "mpf_integrate_dev_static")
+        return steps.value
 
     def trajectory(self, in9, stop="none", value=0.0, kitarget=(0.0, 0.0, 0.0), capacity=100000):
         """Whole trajectories: (out n x NOUT, list of per-ray element arrays)."""
