@@ -260,8 +260,9 @@ def _parents_kept(level, kx, ky, kz, keep):
 
 
 def analytic_tree(base_level=6, half=0.5, max_level=None, refine=None, length=1.0e25, seed=0,
-                  holes=None):
-    """A small AMR tree with analytic fields, for tests.  holes: optional
+                  holes=None, amp=1.0e-5):
+    """A small AMR tree with analytic fields, for tests (amp: amplitude of
+    phi / c^2).  holes: optional
     callable (level, cx, cy, cz) -> bool mask of cells to drop (and with them
     every descendant, so the tree stays parent-closed).  Returns (records,
     fields64 in record order)."""
@@ -275,7 +276,7 @@ def analytic_tree(base_level=6, half=0.5, max_level=None, refine=None, length=1.
         keep = ~holes(level, cx, cy, cz)
         keep &= _parents_kept(level, kx, ky, kz, keep)
         level, kx, ky, kz, cx, cy, cz = (v[keep] for v in (level, kx, ky, kz, cx, cy, cz))
-    f = analytic_fields(cx, cy, cz, length, seed=seed)
+    f = analytic_fields(cx, cy, cz, length, amp=amp, seed=seed)
     rec, order = formats.make_records(level, kx, ky, kz, f)
     return rec, f[order]
 

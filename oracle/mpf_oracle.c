@@ -1097,142 +1097,266 @@ uint64_t mpfo_surfaces(const mpfo_octree* o, int order, int precision, const dou
     return n;
 }
 
-/* one newtonMethod2d call (catalogues.h:410-720) */
-typedef struct {
-    double phi_new, theta_new, sep, firstid;
-    int fail, converged, bundle_fail;
-    double jac[4]; /* a11, a12, a21, a22 */
-    int have_jac;
-    double z[6];
-} newton_res;
-
-/* jacobiantype = bundle: stop_bundle kind (0 radius, 1 lambda, 2 redshift,
- * 3 t, 4 a; -1: jacobiantype = infinitesimal), plane 0 sachs / 1 normal, the
- * bundle's half-opening */
-typedef struct {
-    int kind, plane;
-    double opening;
-} bundle_cfg;
-
 void mpfo_dbetadtheta(const mpfo_octree* o, int order, int precision, const double* cosmo_tab, uint64_t ncos,
                       double length, unsigned nsteps, int kind, double opening, const double* observer, double phi,
                       double theta, const double* ref, const double* ki, const double* dist, uint64_t nd,
                       double* out4);
 
-/* Lensing::dbetadtheta at the crossing of the source radius, with kiTarget
- * and interpRef of catalogues.h:458-491; returns 1 on its failure entry */
-static int bundle_at(const mpfo_octree* o, int order, int precision, const double* tab, uint64_t ncos, double length,
-                     unsigned nsteps, const bundle_cfg* b, double phi, double theta, double distTarget,
-                     const photon* A, const photon* B, double f, const double* fp, double* jac4) {
-    double ki[3], ref;
-    if (b->plane == 0)
+/* ------------------------------------------------- catalogue root-finding */
+
+/* Parameters of the root-finder that change its path: jacobiantype (0
+ * infinitesimal, 1 infinitesimal_born, 2 bundle), for the bundle the
+ * stop_bundle kind (0 radius, 1 lambda, 2 redshift, 3 t, 4 a, 5 plane), the
+ * plane (0 sachs, 1 normal) and the half-opening; use_previous_catalogues
+ * (0, or 2: the grid re-run of rejected sources). */
+typedef struct {
+    int jactype, kind, plane;
+    double opening;
+    int use_prev;
+} cat_cfg;
+
+typedef struct {
+    const mpfo_octree* o;
+    int order, precision;
+    const double* tab;
+    uint64_t ncos;
+    double length;
+    unsigned nsteps;
+    double accuracy;
+    const double* vobs;
+    double h, mpc;
+    cat_cfg cfg;
+} cat_ctx;
+
+/* What persists across the newtonMethod2d calls of one source in the
+ * reference: the jacobian, redshifts and interpRef out-parameters
+ * (catalogues.h:811-814), plus counts kept here: rays integrated, and grid
+ * rays of the re-run mode that ended before the source radius. */
+typedef struct {
+    double jac[4];
+    double z[6];
+    double interp;
+    uint64_t rays, short_grid;
+} cat_state;
+
+static void launch9(double phi, double theta, double* in9) {
+    /* Integrator::launch(0, 0, 0, phi, theta) (integrator.h:304-329) */
+    in9[0] = 1.0; in9[1] = 0.0; in9[2] = 0.0; in9[3] = 0.0; in9[4] = 0.0; in9[5] = 1.0;
+    in9[6] = cos(phi) * sin(theta);
+    in9[7] = sin(phi) * sin(theta);
+    in9[8] = cos(theta);
+}
+
+/* kiTarget and interpRef of the bundle at a crossing (catalogues.h:458-491):
+ * A, B the elements around it, f its weight, fp its position; z1 and as the
+ * redshift and scale factor the reference uses there (the central ray's, also
+ * when a grid ray is chosen, catalogues.h:586-589) */
+static void bundle_ref(const cat_ctx* g, const photon* A, const photon* B, double f, const double* fp,
+                       double distTarget, double z1, double as, double* ki, double* ref) {
+    if (g->cfg.plane == 0)
         for (int k = 0; k < 3; ++k) ki[k] = A->core[CKX + k] * f + B->core[CKX + k] * (1 - f);
     else
         for (int k = 0; k < 3; ++k) ki[k] = fp[k];
-    switch (b->kind) {
-        case 2: ref = A->redshift * f + B->redshift * (1 - f); break;
-        case 4: ref = A->core[CA] * f + B->core[CA] * (1 - f); break;
-        case 3: ref = A->core[CT] * f + B->core[CT] * (1 - f); break;
-        case 1: ref = A->lambda * f + B->lambda * (1 - f); break;
-        default: ref = distTarget; break;
+    switch (g->cfg.kind) {
+        case 2: *ref = z1; break;
+        case 4: *ref = as; break;
+        case 3: *ref = A->core[CT] * f + B->core[CT] * (1 - f); break;
+        case 1: *ref = A->lambda * f + B->lambda * (1 - f); break;
+        case 5: *ref = ki[0] * fp[0] + ki[1] * fp[1] + ki[2] * fp[2]; break;
+        default: *ref = distTarget; break;
     }
-    const double observer[3] = {0.0, 0.0, 0.0};
-    mpfo_dbetadtheta(o, order, precision, tab, ncos, length, nsteps, b->kind, b->opening, observer, phi, theta, &ref,
-                     ki, &distTarget, 1, jac4);
-    return jac4[0] == 42 && jac4[1] == 42;
 }
 
-static void newton_once(const mpfo_octree* o, int order, int precision, const double* tab, uint64_t ncos,
-                        double length, unsigned nsteps, const double* tt, const double* vel, double phi, double theta,
-                        unsigned iteration, double accuracy, const double* vobs, double h, double mpc,
-                        const bundle_cfg* bundle, newton_res* r) {
+/* Lensing::dbetadtheta from the observer at the origin for one direction */
+static void bundle_jac(const cat_ctx* g, const double* ki, double ref, double phi, double theta, double dist,
+                       double* jac4) {
+    const double observer[3] = {0.0, 0.0, 0.0};
+    mpfo_dbetadtheta(g->o, g->order, g->precision, g->tab, g->ncos, g->length, g->nsteps, g->cfg.kind,
+                     g->cfg.opening, observer, phi, theta, &ref, ki, &dist, 1, jac4);
+}
+
+static void jac_inf(const cat_ctx* g, const traj_t* t, double dist, double* jac4) {
+    double j3[3];
+    jacobian_at(g->o, g->precision, t, dist, g->length, j3);
+    jac4[0] = j3[0]; jac4[1] = j3[1]; jac4[2] = j3[1]; jac4[3] = j3[2];
+}
+
+static void invert2(const double* j, double* ji) {   /* Utility::invMatrix2d (utility.h:687-698) */
+    const double invdet = 1. / (j[0] * j[3] - j[1] * j[2]);
+    ji[0] = j[3] * invdet; ji[1] = -j[1] * invdet; ji[2] = -j[2] * invdet; ji[3] = j[0] * invdet;
+}
+
+/* One Catalogues::newtonMethod2d call (catalogues.h:410-754): res receives
+ * result[0][0], result[0][1], result[1][0], result[1][1] (the last is left 0
+ * where the reference leaves it unset). */
+static void newton2d(const cat_ctx* g, const double* tt, const double* vel, double phi, double theta,
+                     unsigned iteration, cat_state* st, double* res) {
     const double c = C_LIGHT, c2 = C2;
-    memset(r, 0, sizeof(*r));
+    const double acc = g->accuracy;
+    res[0] = res[1] = res[2] = res[3] = 0.0;
     const double distTarget = sqrt(pow(tt[0], 2) + pow(tt[1], 2) + pow(tt[2], 2));
-    double in9[9] = {1.0, 0.0, 0.0, 0.0, 0.0, 1.0, cos(phi) * sin(theta), sin(phi) * sin(theta), cos(theta)};
+    double photon9[9];
+    launch9(phi, theta, photon9);
     traj_t t = {NULL, 0, 0, 1};
-    run_full(o, order, precision, tab, ncos, length, nsteps, 0, distTarget, NULL, in9, &t);
+    run_full(g->o, g->order, g->precision, g->tab, g->ncos, g->length, g->nsteps, 0, distTarget, NULL, photon9, &t);
+    st->rays++;
     const uint64_t marked = t.n - 1;
-    const uint64_t firstid = marked - (marked > 0);
-    if (marked == 0 || t.p[marked].chi < distTarget) {
-        r->fail = 1;
-        r->firstid = (double)firstid;
+    uint64_t firstid = marked - (marked > 0);
+    if (marked == 0 || t.p[marked].chi < distTarget) {   /* catalogues.h:433-440 */
+        res[0] = 42; res[1] = 42; res[2] = 0; res[3] = (double)firstid;
         free(t.p);
         return;
     }
-    const double previous = t.p[firstid].chi, next = t.p[firstid + 1].chi;
-    const double f = (next - distTarget) / (next - previous);
-    const photon* A = &t.p[firstid];
-    const photon* B = &t.p[firstid + 1];
-    const double z1 = A->redshift * f + B->redshift * (1 - f);
-    const double as = A->core[CA] * f + B->core[CA] * (1 - f);
+    double f = (t.p[firstid + 1].chi - distTarget) / (t.p[firstid + 1].chi - t.p[firstid].chi);
+    {
+        const photon* A = &t.p[firstid];
+        const photon* B = &t.p[firstid + 1];
+        st->z[1] = A->redshift * f + B->redshift * (1 - f);
+    }
+    const double as = t.p[firstid].core[CA] * f + t.p[firstid + 1].core[CA] * (1 - f);
     double fp[3];
-    for (int k = 0; k < 3; ++k) fp[k] = A->core[CX + k] * f + B->core[CX + k] * (1 - f);
-    const double dist_sep = sqrt(pow(tt[0] - fp[0], 2) + pow(tt[1] - fp[1], 2) + pow(tt[2] - fp[2], 2));
-    r->sep = dist_sep / distTarget;
-    if (r->sep > accuracy) {
-        double ji[2][2] = {{1, 0}, {0, 1}};
-        if (iteration >= 3) {
-            if (iteration < 5 || bundle->kind < 0) {
-                double j3[3];
-                jacobian_at(o, precision, &t, distTarget, length, j3);
-                r->jac[0] = j3[0]; r->jac[1] = j3[1]; r->jac[2] = j3[1]; r->jac[3] = j3[2];
-            } else if (bundle_at(o, order, precision, tab, ncos, length, nsteps, bundle, phi, theta, distTarget, A, B,
-                                 f, fp, r->jac)) {
-                /* catalogues.h:518-523 */
-                r->have_jac = 1;
-                r->bundle_fail = 1;
+    for (int k = 0; k < 3; ++k) fp[k] = t.p[firstid].core[CX + k] * f + t.p[firstid + 1].core[CX + k] * (1 - f);
+    double ki[3] = {0, 0, 0};
+    if (g->cfg.jactype == 2)
+        bundle_ref(g, &t.p[firstid], &t.p[firstid + 1], f, fp, distTarget, st->z[1], as, ki, &st->interp);
+    res[2] = sqrt(pow(tt[0] - fp[0], 2) + pow(tt[1] - fp[1], 2) + pow(tt[2] - fp[2], 2)) / distTarget;
+    if (g->cfg.use_prev == 2) { res[0] = phi; res[1] = theta; }
+    traj_t* T = &t;      /* the trajectory the rest reads: the central ray or the chosen grid ray */
+    traj_t best = {NULL, 0, 0, 1};
+    if (res[2] > acc) {
+        double ji[4] = {1, 0, 0, 1};
+        if (g->cfg.use_prev == 0) {
+            if (iteration < 3) {
+            } else if (iteration < 5 || g->cfg.jactype != 2) {
+                jac_inf(g, T, distTarget, st->jac);
+                invert2(st->jac, ji);
+            } else {
+                bundle_jac(g, ki, st->interp, phi, theta, distTarget, st->jac);
+                if (st->jac[0] == 42 && st->jac[1] == 42) {   /* catalogues.h:518-523 */
+                    res[0] = 42; res[1] = 42; res[2] = 22; res[3] = 0;
+                    free(t.p);
+                    return;
+                }
+                invert2(st->jac, ji);
+            }
+        } else {
+            /* grid around the last angles, its size and step from the
+             * separation (catalogues.h:527-623) */
+            const double err = 5 * res[2] / (iteration + 1);
+            double phi_tmp = phi, theta_tmp = theta;
+            double sep_tmp = res[2];
+            for (double i = -err; i <= err; i += 0.2 * err) {
+                for (double j = -err; j <= err; j += 0.2 * err) {
+                    double in9[9];
+                    launch9(phi + i, theta + j, in9);
+                    memcpy(photon9, in9, sizeof(in9));
+                    traj_t u = {NULL, 0, 0, 1};
+                    run_full(g->o, g->order, g->precision, g->tab, g->ncos, g->length, g->nsteps, 0, distTarget, NULL,
+                             in9, &u);
+                    st->rays++;
+                    const uint64_t marked2 = u.n - 1;
+                    if (marked2 == 0) {
+                        /* the reference reads past this one-element
+                         * trajectory: here the ray is no candidate */
+                        st->short_grid++;
+                        sep_tmp = INFINITY;
+                        free(u.p);
+                        continue;
+                    }
+                    if (u.p[marked2].chi < distTarget) st->short_grid++;   /* extrapolated, as the reference does */
+                    firstid = marked2 - 1;
+                    f = (u.p[firstid + 1].chi - distTarget) / (u.p[firstid + 1].chi - u.p[firstid].chi);
+                    for (int k = 0; k < 3; ++k)
+                        fp[k] = u.p[firstid].core[CX + k] * f + u.p[firstid + 1].core[CX + k] * (1 - f);
+                    sep_tmp = sqrt(pow(tt[0] - fp[0], 2) + pow(tt[1] - fp[1], 2) + pow(tt[2] - fp[2], 2)) / distTarget;
+                    if (sep_tmp < acc) {
+                        res[2] = sep_tmp;
+                        phi_tmp = phi + i;
+                        theta_tmp = theta + j;
+                        free(best.p);
+                        best = u;
+                        T = &best;
+                        if (g->cfg.jactype == 2)
+                            bundle_ref(g, &u.p[firstid], &u.p[firstid + 1], f, fp, distTarget, st->z[1], as, ki,
+                                       &st->interp);
+                        break;
+                    } else if (sep_tmp < res[2]) {
+                        res[2] = sep_tmp;
+                        phi_tmp = phi + i;
+                        theta_tmp = theta + j;
+                        free(best.p);
+                        best = u;
+                        T = &best;
+                    } else {
+                        free(u.p);
+                    }
+                }
+                if (sep_tmp < acc) break;
+            }
+            res[0] = phi_tmp;
+            res[1] = theta_tmp;
+            if (res[2] > acc) {
+                free(best.p);
                 free(t.p);
                 return;
             }
-            r->have_jac = 1;
-            const double a11 = r->jac[0], a12 = r->jac[1], a21 = r->jac[2], a22 = r->jac[3];
-            const double invdet = 1. / (a11 * a22 - a12 * a21);   /* utility.h:687-698 */
-            ji[0][0] = a22 * invdet; ji[0][1] = -a12 * invdet; ji[1][0] = -a21 * invdet; ji[1][1] = a11 * invdet;
         }
-        const double cp = cos(phi), sp = sin(phi), ct = cos(theta), st = sin(theta);
-        const double e1[3] = {-sp, cp, 0}, e2[3] = {-cp * ct, -sp * ct, st};
-        const double targeted[3] = {distTarget * cp * st, distTarget * sp * st, distTarget * ct};
-        double sep[3], pos[3];
-        for (int k = 0; k < 3; ++k) { sep[k] = fp[k] - tt[k]; pos[k] = targeted[k] - tt[k]; }
-        const double sepx = sep[0] * e1[0] + sep[1] * e1[1];
-        const double sepy = sep[0] * e2[0] + sep[1] * e2[1] + sep[2] * e2[2];
-        const double posx = pos[0] * e1[0] + pos[1] * e1[1];
-        const double posy = pos[0] * e2[0] + pos[1] * e2[1] + pos[2] * e2[2];
-        const double sepxnew = ji[0][0] * sepx + ji[0][1] * sepy;
-        const double sepynew = ji[1][0] * sepx + ji[1][1] * sepy;
-        const double posxnew = posx - sepxnew, posynew = posy - sepynew;
-        const double xnew = tt[0] + (posxnew * e1[0] + posynew * e2[0]);
-        const double ynew = tt[1] + (posxnew * e1[1] + posynew * e2[1]);
-        const double znew = tt[2] + posynew * e2[2];
-        r->phi_new = atan2(ynew, xnew);
-        r->theta_new = acos(znew / sqrt(xnew * xnew + ynew * ynew + znew * znew));
+        if (g->cfg.use_prev == 0) {
+            /* the update in the (e1, e2) plane at (phi, theta) (catalogues.h:627-660) */
+            const double cp = cos(phi), sp = sin(phi), ct = cos(theta), st_ = sin(theta);
+            const double e1[3] = {-sp, cp, 0}, e2[3] = {-cp * ct, -sp * ct, st_};
+            const double targeted[3] = {distTarget * cp * st_, distTarget * sp * st_, distTarget * ct};
+            double sep[3], pos[3];
+            for (int k = 0; k < 3; ++k) { sep[k] = fp[k] - tt[k]; pos[k] = targeted[k] - tt[k]; }
+            const double sepx = sep[0] * e1[0] + sep[1] * e1[1];
+            const double sepy = sep[0] * e2[0] + sep[1] * e2[1] + sep[2] * e2[2];
+            const double posx = pos[0] * e1[0] + pos[1] * e1[1];
+            const double posy = pos[0] * e2[0] + pos[1] * e2[1] + pos[2] * e2[2];
+            const double sepxnew = ji[0] * sepx + ji[1] * sepy;
+            const double sepynew = ji[2] * sepx + ji[3] * sepy;
+            const double posxnew = posx - sepxnew, posynew = posy - sepynew;
+            const double xnew = tt[0] + (posxnew * e1[0] + posynew * e2[0]);
+            const double ynew = tt[1] + (posxnew * e1[1] + posynew * e2[1]);
+            const double znew = tt[2] + posynew * e2[2];
+            res[0] = atan2(ynew, xnew);
+            res[1] = acos(znew / sqrt(xnew * xnew + ynew * ynew + znew * znew));
+            free(best.p);
+            free(t.p);
+            return;
+        }
+    }
+    /* the Jacobian of the converged source (catalogues.h:665-681) */
+    if (g->cfg.jactype == 2) {
+        bundle_jac(g, ki, st->interp, phi, theta, distTarget, st->jac);
+    } else if (g->cfg.jactype == 0) {
+        jac_inf(g, T, distTarget, st->jac);
+    } else {
+        /* the last photon launched, on the homogeneous path (Integrator::integrate<-1>) */
+        traj_t b = {NULL, 0, 0, 1};
+        run_full(g->o, -1, g->precision, g->tab, g->ncos, g->length, g->nsteps, 0, distTarget, NULL, photon9, &b);
+        jac_inf(g, &b, distTarget, st->jac);
+        free(b.p);
+    }
+    if (st->jac[0] == 42 && st->jac[1] == 42) {
+        res[0] = 42; res[1] = 42; res[2] = 22; res[3] = 0;
+        free(best.p);
         free(t.p);
         return;
     }
-    r->converged = 1;
-    r->have_jac = 1;
-    if (bundle->kind < 0) {
-        double j3[3];
-        jacobian_at(o, precision, &t, distTarget, length, j3);
-        r->jac[0] = j3[0]; r->jac[1] = j3[1]; r->jac[2] = j3[1]; r->jac[3] = j3[2];
-    } else if (bundle_at(o, order, precision, tab, ncos, length, nsteps, bundle, phi, theta, distTarget, A, B, f, fp,
-                         r->jac)) {
-        r->converged = 0; /* catalogues.h:676-681 */
-        r->bundle_fail = 1;
-        free(t.p);
-        return;
-    }
+    /* redshifts (catalogues.h:682-719): potential, ISW and direction on the
+     * trajectory read, the scale factor and redshift of the central ray */
+    const photon* A = &T->p[firstid];
+    const photon* B = &T->p[firstid + 1];
     const double potential = A->phi * f + B->phi * (1 - f);
     const double isw = A->isw * f + B->isw * (1 - f);
-    r->z[0] = -1 + 1. / (as);
-    r->z[1] = z1;
-    const double unit_t = as * as * mpc / (h * 100 * 1000);
-    const double unit_l = as * length * 100;
+    st->z[0] = -1 + 1. / (as);
+    const double unit_t = as * as * g->mpc / (g->h * 100 * 1000);
+    const double unit_l = as * g->length * 100;
     const double vx = vel[0] * unit_l * 1e-2 / (unit_t * c);
     const double vy = vel[1] * unit_l * 1e-2 / (unit_t * c);
     const double vz = vel[2] * unit_l * 1e-2 / (unit_t * c);
-    const photon* F = &t.p[0];
+    const double* vobs = g->vobs;
+    const photon* F = &T->p[0];
     const double gref = -F->core[CA] * c * F->core[CKT] *
                         (1. + F->phi / c2 + (vobs[0] * F->core[CKX] + vobs[1] * F->core[CKY] + vobs[2] * F->core[CKZ]) / (c * F->core[CKT]) +
                          0.5 * (vobs[0] * vobs[0] + vobs[1] * vobs[1] + vobs[2] * vobs[2]));
@@ -1242,82 +1366,76 @@ static void newton_once(const mpfo_octree* o, int order, int precision, const do
     const double dzdl = A->core[CKZ] * f + B->core[CKZ] * (1 - f);
     const double gK = -as * c * dtdl * (1. + potential / c2 + (vx * dxdl + vy * dydl + vz * dzdl) / (c * dtdl) +
                                         0.5 * (vx * vx + vy * vy + vz * vz));
-    r->z[2] = r->z[1] + (vx * dxdl + vy * dydl + vz * dzdl) / (as * c * dtdl) -
-              (vobs[0] * F->core[CKX] + vobs[1] * F->core[CKY] + vobs[2] * F->core[CKZ]) / (as * c * F->core[CKT]);
-    r->z[3] = r->z[2] + 0.5 * (vx * vx + vy * vy + vz * vz - vobs[0] * vobs[0] - vobs[1] * vobs[1] - vobs[2] * vobs[2]) / as;
-    r->z[4] = r->z[3] + isw / as;
-    r->z[5] = gK / gref - 1;
+    st->z[2] = st->z[1] + (vx * dxdl + vy * dydl + vz * dzdl) / (as * c * dtdl) -
+               (vobs[0] * F->core[CKX] + vobs[1] * F->core[CKY] + vobs[2] * F->core[CKZ]) / (as * c * F->core[CKT]);
+    st->z[3] = st->z[2] + 0.5 * (vx * vx + vy * vy + vz * vz - vobs[0] * vobs[0] - vobs[1] * vobs[1] - vobs[2] * vobs[2]) / as;
+    st->z[4] = st->z[3] + isw / as;
+    st->z[5] = gK / gref - 1;
+    if (g->cfg.use_prev == 0) { res[0] = phi; res[1] = theta; }
+    free(best.p);
     free(t.p);
 }
 
 typedef struct {
-    const mpfo_octree* o;
-    int order, precision;
-    const double* tab;
-    uint64_t ncos;
-    double length;
-    unsigned nsteps;
+    const cat_ctx* g;
     const double *src6, *init2;
-    double accuracy;
     unsigned max_it;
-    const double* vobs;
-    double h, mpc;
     double* out;
     uint64_t begin, end, rays;
-    bundle_cfg bundle;
 } cat_work;
 
+/* Catalogues::iterateNewtonMethod (catalogues.h:757-800) per source, and the
+ * routing of relCat (catalogues.h:876-903) as a status */
 static void* cat_worker(void* arg) {
     cat_work* w = (cat_work*)arg;
+    const cat_ctx* g = w->g;
     w->rays = 0;
     for (uint64_t i = w->begin; i < w->end; ++i) {
         const double* s = w->src6 + 6 * i;
         double* o = w->out + 20 * i;
         memset(o, 0, 20 * sizeof(double));
         const double d = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
-        o[0] = atan2(s[1], s[0]);
         o[1] = acos(s[2] / d);
-        double phi = w->init2 ? w->init2[2 * i] : o[0];
-        double theta = w->init2 ? w->init2[2 * i + 1] : o[1];
+        o[0] = atan2(s[1], s[0]);
+        cat_state st;
+        memset(&st, 0, sizeof(st));
+        double res[4] = {w->init2 ? w->init2[2 * i] : o[0], w->init2 ? w->init2[2 * i + 1] : o[1], 0, 0};
         unsigned iteration = 0;
+        int status;
         for (;;) {
-            newton_res r;
-            newton_once(w->o, w->order, w->precision, w->tab, w->ncos, w->length, w->nsteps, s, s + 3, phi, theta,
-                        iteration, w->accuracy, w->vobs, w->h, w->mpc, &w->bundle, &r);
-            w->rays++;
+            double r[4];
+            newton2d(g, s, s + 3, res[0], res[1], iteration, &st, r);
+            memcpy(res, r, sizeof(r));
             iteration++;
-            o[17] = iteration;
-            if (r.have_jac) { o[12] = r.jac[0]; o[13] = r.jac[1]; o[14] = r.jac[2]; o[15] = r.jac[3]; }
-            if (r.fail) { o[2] = 42; o[3] = 42; o[4] = 0; o[5] = r.firstid; o[16] = 1; break; }
-            /* a failed bundle: 42, 42, 22 (catalogues.h:518-523, 676-681); the
-             * reference leaves result[1][1] unset, written 0 here */
-            if (r.bundle_fail) { o[2] = 42; o[3] = 42; o[4] = 22; o[5] = 0; o[16] = 1; break; }
-            o[4] = r.sep;
-            if (r.converged) {
-                o[2] = phi; o[3] = theta; o[5] = iteration; o[16] = 0;
-                for (int k = 0; k < 6; ++k) o[6 + k] = r.z[k];
-                break;
-            }
-            phi = r.phi_new;
-            theta = r.theta_new;
-            if (iteration > w->max_it) { o[2] = phi; o[3] = theta; o[5] = 42; o[16] = 2; break; }
+            if (res[0] == 42 && res[1] == 42) { status = 1; break; }
+            if (iteration > w->max_it && res[2] > g->accuracy) { res[3] = 42; status = 2; break; }
+            if (!(res[2] > g->accuracy)) { res[3] = iteration; status = 0; break; }
         }
+        o[2] = res[0]; o[3] = res[1]; o[4] = res[2]; o[5] = res[3];
+        for (int k = 0; k < 6; ++k) o[6 + k] = st.z[k];
+        for (int k = 0; k < 4; ++k) o[12 + k] = st.jac[k];
+        o[16] = status;
+        o[17] = iteration;
+        o[18] = (double)st.short_grid;
+        w->rays += st.rays;
     }
     return NULL;
 }
 
-uint64_t mpfo_catalogue_bundle(const mpfo_octree* o, int order, int precision, const double* cosmo_tab,
-                               uint64_t ncos, double length, unsigned nsteps, const double* src6, const double* init2,
-                               uint64_t n, double accuracy, unsigned max_iterations, const double* vobs, double h,
-                               double mpc, int bundle_kind, int plane, double opening, double* out, int nthreads) {
+uint64_t mpfo_catalogue_ex(const mpfo_octree* o, int order, int precision, const double* cosmo_tab, uint64_t ncos,
+                           double length, unsigned nsteps, const double* src6, const double* init2, uint64_t n,
+                           double accuracy, unsigned max_iterations, const double* vobs, double h, double mpc,
+                           int jactype, int bundle_kind, int plane, double opening, int use_prev, double* out,
+                           int nthreads) {
     if (nthreads < 1) nthreads = 1;
     if ((uint64_t)nthreads > n) nthreads = n ? (int)n : 1;
+    const cat_ctx g = {o, order, precision, cosmo_tab, ncos, length, nsteps, accuracy, vobs, h, mpc,
+                       {jactype, bundle_kind, plane, opening, use_prev}};
     cat_work* w = (cat_work*)calloc((size_t)nthreads, sizeof(cat_work));
     pthread_t* th = (pthread_t*)calloc((size_t)nthreads, sizeof(pthread_t));
     const uint64_t chunk = (n + nthreads - 1) / nthreads;
     for (int k = 0; k < nthreads; ++k) {
-        w[k] = (cat_work){o, order, precision, cosmo_tab, ncos, length, nsteps, src6, init2, accuracy, max_iterations,
-                          vobs, h, mpc, out, 0, 0, 0, {bundle_kind, plane, opening}};
+        w[k] = (cat_work){&g, src6, init2, max_iterations, out, 0, 0, 0};
         w[k].begin = chunk * k < n ? chunk * k : n;
         w[k].end = chunk * (k + 1) < n ? chunk * (k + 1) : n;
     }
@@ -1333,12 +1451,78 @@ uint64_t mpfo_catalogue_bundle(const mpfo_octree* o, int order, int precision, c
     return rays;
 }
 
+uint64_t mpfo_catalogue_bundle(const mpfo_octree* o, int order, int precision, const double* cosmo_tab,
+                               uint64_t ncos, double length, unsigned nsteps, const double* src6, const double* init2,
+                               uint64_t n, double accuracy, unsigned max_iterations, const double* vobs, double h,
+                               double mpc, int bundle_kind, int plane, double opening, double* out, int nthreads) {
+    return mpfo_catalogue_ex(o, order, precision, cosmo_tab, ncos, length, nsteps, src6, init2, n, accuracy,
+                             max_iterations, vobs, h, mpc, bundle_kind < 0 ? 0 : 2, bundle_kind, plane, opening, 0,
+                             out, nthreads);
+}
+
 uint64_t mpfo_catalogue(const mpfo_octree* o, int order, int precision, const double* cosmo_tab, uint64_t ncos,
                         double length, unsigned nsteps, const double* src6, const double* init2, uint64_t n,
                         double accuracy, unsigned max_iterations, const double* vobs, double h, double mpc,
                         double* out, int nthreads) {
-    return mpfo_catalogue_bundle(o, order, precision, cosmo_tab, ncos, length, nsteps, src6, init2, n, accuracy,
-                                 max_iterations, vobs, h, mpc, -1, 0, 0.0, out, nthreads);
+    return mpfo_catalogue_ex(o, order, precision, cosmo_tab, ncos, length, nsteps, src6, init2, n, accuracy,
+                             max_iterations, vobs, h, mpc, 0, -1, 0, 0.0, 0, out, nthreads);
+}
+
+/* Catalogues::relCat_with_previous_cat (catalogues.h:922-1044), per row of a
+ * previous catalogue (18 columns): the ray from its observed angles (the
+ * comoving ones for infinitesimal_born) to the scale factor of its FLRW
+ * redshift, the Jacobian at the interpolated distance; jac4 receives the new
+ * columns 13-16.  flag[i] = 1 where the reference reads past a one-element
+ * trajectory (jac4 then 42, 42, 42, 42), 2 where the ray ends before the
+ * scale factor (the reference extrapolates its last segment; so does this),
+ * else 0. */
+void mpfo_catalogue_prev(const mpfo_octree* o, int order, int precision, const double* cosmo_tab, uint64_t ncos,
+                         double length, unsigned nsteps, const double* prev18, uint64_t n, int jactype,
+                         int bundle_kind, int plane, double opening, double* jac4, int* flag) {
+    const double zero3[3] = {0, 0, 0};
+    const cat_ctx g = {o, order, precision, cosmo_tab, ncos, length, nsteps, 0.0, zero3, 0.7, 3.08568e22,
+                       {jactype, bundle_kind, plane, opening, 1}};
+    for (uint64_t i = 0; i < n; ++i) {
+        const double* p = prev18 + 18 * i;
+        double* J = jac4 + 4 * i;
+        flag[i] = 0;
+        const double sf = 1. / (1. + p[7]);
+        const double phi = (jactype == 1) ? p[1] : p[3];
+        const double theta = (jactype == 1) ? p[2] : p[4];
+        double in9[9];
+        launch9(phi, theta, in9);
+        traj_t t = {NULL, 0, 0, 1};
+        run_full(o, order, precision, cosmo_tab, ncos, length, nsteps, 4, sf, NULL, in9, &t);
+        const uint64_t marked = t.n - 1;
+        if (marked == 0) {
+            flag[i] = 1;
+            J[0] = J[1] = J[2] = J[3] = 42;
+            free(t.p);
+            continue;
+        }
+        if (t.p[marked].core[CA] > sf) flag[i] = 2;
+        const uint64_t firstid = marked - 1;
+        const photon* A = &t.p[firstid];
+        const photon* B = &t.p[firstid + 1];
+        const double f = (B->core[CA] - sf) / (B->core[CA] - A->core[CA]);
+        double fp[3];
+        for (int k = 0; k < 3; ++k) fp[k] = A->core[CX + k] * f + B->core[CX + k] * (1 - f);
+        const double distTarget = A->chi * f + B->chi * (1 - f);
+        if (jactype == 2) {
+            double ki[3], ref;
+            const double z1 = A->redshift * f + B->redshift * (1 - f);
+            bundle_ref(&g, A, B, f, fp, distTarget, z1, sf, ki, &ref);
+            bundle_jac(&g, ki, ref, phi, theta, distTarget, J);
+        } else if (jactype == 0) {
+            jac_inf(&g, &t, distTarget, J);
+        } else {
+            traj_t b = {NULL, 0, 0, 1};
+            run_full(o, -1, precision, cosmo_tab, ncos, length, nsteps, 4, sf, NULL, in9, &b);
+            jac_inf(&g, &b, distTarget, J);
+            free(b.p);
+        }
+        free(t.p);
+    }
 }
 
 void mpfo_interpolate(const double* x, const double* y, uint64_t n, const double* x0, uint64_t m, double* out) {

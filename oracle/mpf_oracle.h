@@ -81,6 +81,26 @@ uint64_t mpfo_catalogue_bundle(const mpfo_octree* o, int order, int precision, c
                                uint64_t ncos, double length, unsigned nsteps, const double* src6, const double* init2,
                                uint64_t n, double accuracy, unsigned max_iterations, const double* vobs, double h,
                                double mpc, int bundle_kind, int plane, double opening, double* out, int nthreads);
+/* Every mode of the root-finder: jactype 0 infinitesimal, 1
+ * infinitesimal_born (the converged Jacobian on the homogeneous path of the
+ * last photon launched, catalogues.h:670-674), 2 bundle (bundle_kind 0
+ * radius, 1 lambda, 2 redshift, 3 t, 4 a, 5 plane); use_prev 0, or 2: the
+ * grid re-run of rejected sources (catalogues.h:527-623; max_iterations 100
+ * in the reference).  out[18]: grid rays that ended before the source
+ * radius (extrapolated as the reference does; one-element trajectories are
+ * no candidate). */
+uint64_t mpfo_catalogue_ex(const mpfo_octree* o, int order, int precision, const double* cosmo_tab, uint64_t ncos,
+                           double length, unsigned nsteps, const double* src6, const double* init2, uint64_t n,
+                           double accuracy, unsigned max_iterations, const double* vobs, double h, double mpc,
+                           int jactype, int bundle_kind, int plane, double opening, int use_prev, double* out,
+                           int nthreads);
+/* Catalogues::relCat_with_previous_cat (catalogues.h:922-1044): new Jacobian
+ * columns (n x 4) of n previous-catalogue rows (18 columns); flag per row:
+ * 1 one-element trajectory (42s), 2 the ray ended before the scale factor
+ * (extrapolated), 0 otherwise. */
+void mpfo_catalogue_prev(const mpfo_octree* o, int order, int precision, const double* cosmo_tab, uint64_t ncos,
+                         double length, unsigned nsteps, const double* prev18, uint64_t n, int jactype,
+                         int bundle_kind, int plane, double opening, double* jac4, int* flag);
 /* Integrator::launch around one photon (integrator.h:345-382): out9 holds
  * the photon and its count bundle photons. */
 void mpfo_launch_bundle(const double* in9, unsigned count, double angle, double rotation, double* out9);

@@ -9,6 +9,9 @@ cpu_baseline leg of bench.py, as the checker or the timed CPU baseline.
                  reference sources exist; travels as a built library)
   RefVelocity -- oracle/_ref/libmpfrefvel.so, the same integrator compiled
                  with VELOCITYFIELD (oracle/ref_harness_vel.cpp)
+  RefCatalogue -- oracle/_ref/libmpfrefcat.so, catalogues.h, input.h and
+                 miscellaneous.h instantiated by oracle/ref_harness_cat.cpp
+                 (needs the HDF5 library it was linked against)
 """
 from __future__ import annotations
 
@@ -21,6 +24,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_LIB = os.path.join(HERE, "libmpforacle.so")
 REF_LIB = os.path.join(HERE, "_ref", "libmpfref.so")
 REFV_LIB = os.path.join(HERE, "_ref", "libmpfrefvel.so")
+REFC_LIB = os.path.join(HERE, "_ref", "libmpfrefcat.so")
+HDF5_LIB = "/opt/conda/lib/libhdf5.so.103"
 NREC = 28
 
 _vp = ctypes.c_void_p
@@ -218,25 +223,44 @@ class Oracle(_Base):
                                ctypes.c_uint32(vals.size), int(bool(jacobian)), _p(in9), _p(out))
         return out
 
-    def catalogue(self, h, order, table, length, nsteps, src6, init2=None, accuracy=1e-8, max_iterations=10,
-                  vobs=(0.0, 0.0, 0.0), hubble=0.7, mpc=3.08568e22, precision=0, nthreads=8, bundle=None):
+    def catalogue(self, h, order, table, length, nsteps, src6, init2=None, accuracy=1e-8, max_iterations=None,
+                  vobs=(0.0, 0.0, 0.0), hubble=0.7, mpc=3.08568e22, precision=0, nthreads=8, bundle=None,
+                  born=False, use_prev=0):
         """Newton root-finding of catalogues.h:410-800; returns (n x 20 records, rays integrated).
-        bundle: None (jacobiantype = infinitesimal) or (stop_bundle kind, plane 0 sachs / 1 normal,
-        opening) for jacobiantype = bundle."""
+        bundle: None (jacobiantype = infinitesimal, or infinitesimal_born with born) or (stop_bundle
+        kind, plane 0 sachs / 1 normal, opening) for jacobiantype = bundle.  use_prev 2: the grid
+        re-run (max_iterations 100 by default, else 10).  Column 18: grid rays short of the source."""
         tab = np.ascontiguousarray(table, dtype=np.float64)
         src6 = np.ascontiguousarray(src6, dtype=np.float64).reshape(-1, 6)
         i2 = None if init2 is None else np.ascontiguousarray(init2, dtype=np.float64).reshape(-1, 2)
         vo = np.ascontiguousarray(vobs, dtype=np.float64)
         out = np.zeros((src6.shape[0], 20))
         kind, plane, opening = bundle if bundle is not None else (-1, 0, 0.0)
-        self.lib.mpfo_catalogue_bundle.restype = _u64
-        rays = self.lib.mpfo_catalogue_bundle(_vp(h), int(order), int(precision), _p(tab), _u64(tab.shape[1]),
-                                              ctypes.c_double(length), ctypes.c_uint(nsteps), _p(src6), _p(i2),
-                                              _u64(src6.shape[0]), ctypes.c_double(accuracy),
-                                              ctypes.c_uint(max_iterations), _p(vo), ctypes.c_double(hubble),
-                                              ctypes.c_double(mpc), int(kind), int(plane), ctypes.c_double(opening),
-                                              _p(out), int(nthreads))
+        jt = 2 if bundle is not None else (1 if born else 0)
+        if max_iterations is None:
+            max_iterations = 100 if use_prev else 10
+        self.lib.mpfo_catalogue_ex.restype = _u64
+        rays = self.lib.mpfo_catalogue_ex(_vp(h), int(order), int(precision), _p(tab), _u64(tab.shape[1]),
+                                          ctypes.c_double(length), ctypes.c_uint(nsteps), _p(src6), _p(i2),
+                                          _u64(src6.shape[0]), ctypes.c_double(accuracy),
+                                          ctypes.c_uint(max_iterations), _p(vo), ctypes.c_double(hubble),
+                                          ctypes.c_double(mpc), int(jt), int(kind), int(plane),
+                                          ctypes.c_double(opening), int(use_prev), _p(out), int(nthreads))
         return out, rays
+
+    def catalogue_prev(self, h, order, table, length, nsteps, prev18, precision=0, jacobiantype=0, bundle=None):
+        """relCat_with_previous_cat: (n x 4 new Jacobian columns, n flags)."""
+        tab = np.ascontiguousarray(table, dtype=np.float64)
+        pv = np.ascontiguousarray(prev18, dtype=np.float64).reshape(-1, 18)
+        n = pv.shape[0]
+        jac = np.zeros((n, 4))
+        flag = np.zeros(n, dtype=np.int32)
+        kind, plane, opening = bundle if bundle is not None else (-1, 0, 0.0)
+        self.lib.mpfo_catalogue_prev(_vp(h), int(order), int(precision), _p(tab), _u64(tab.shape[1]),
+                                     ctypes.c_double(length), ctypes.c_uint(nsteps), _p(pv), _u64(n),
+                                     int(jacobiantype), int(kind), int(plane), ctypes.c_double(opening), _p(jac),
+                                     _p(flag))
+        return jac, flag
 
     def interpolate(self, x, y, x0):
         x = np.ascontiguousarray(x, dtype=np.float64)
@@ -468,8 +492,143 @@ class RefVelocity:
         return out, steps
 
 
+JACTYPES = {"infinitesimal": 0, "infinitesimal_born": 1, "bundle": 2}
+PLANES = {"sachs": 0, "normal": 1}
+
+
+class RefCatalogue:
+    """The reference's catalogue root-finder, relCat, Input::acquire/correct,
+    the cone generators and getTargets (oracle/_ref/libmpfrefcat.so)."""
+
+    def __init__(self, path=REFC_LIB):
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        # the library names libhdf5.so.103; map it first from where it was
+        # linked, so the loader finds it without a search path
+        ctypes.CDLL(HDF5_LIB, mode=ctypes.RTLD_GLOBAL)
+        L = self.lib = ctypes.CDLL(path)
+        L.refc_octree_new.restype = _vp
+        L.refc_octree_new.argtypes = [_vp, _u64]
+        L.refc_octree_free.argtypes = [_vp]
+        L.refc_get_targets.restype = _u64
+
+    def tree(self, rec):
+        rec = np.ascontiguousarray(rec)
+        return self.lib.refc_octree_new(_p(rec), _u64(rec.size))
+
+    def free(self, h):
+        self.lib.refc_octree_free(_vp(h))
+
+    def newton(self, h, table, length, nsteps, src6, init2=None, accuracy=1e-8, jacobiantype="infinitesimal",
+               plane="sachs", stop_bundle="redshift", opening=1e-4, use_prev=0, vobs=(0.0, 0.0, 0.0), hubble=0.7,
+               mpc=3.08568e22, order=2, nthreads=0):
+        """Catalogues::iterateNewtonMethod per source: n x 16 (result 2 x 2,
+        jacobian 2 x 2, redshifts 6, interpRef, 0)."""
+        tab = np.ascontiguousarray(table, dtype=np.float64)
+        src6 = np.ascontiguousarray(src6, dtype=np.float64).reshape(-1, 6)
+        n = src6.shape[0]
+        i2 = None if init2 is None else np.ascontiguousarray(init2, dtype=np.float64).reshape(-1, 2)
+        vo = np.ascontiguousarray(vobs, dtype=np.float64)
+        out = np.zeros((n, 16))
+        self.lib.refc_newton(_vp(h), int(order), _p(tab), _u64(tab.shape[1]), ctypes.c_double(length),
+                             ctypes.c_double(hubble), ctypes.c_double(mpc), ctypes.c_uint(nsteps),
+                             ctypes.c_double(accuracy), int(JACTYPES[jacobiantype]), int(PLANES[plane]),
+                             stop_bundle.encode(), ctypes.c_double(opening), int(use_prev), _p(vo), _p(src6),
+                             _p(i2), _u64(n), _p(out), int(nthreads))
+        return out
+
+    def newton_once(self, h, table, length, nsteps, src6, ang2, iteration, jac4=None, accuracy=1e-8,
+                    jacobiantype="infinitesimal", plane="sachs", stop_bundle="redshift", opening=1e-4, use_prev=0,
+                    vobs=(0.0, 0.0, 0.0), hubble=0.7, mpc=3.08568e22):
+        """One Catalogues::newtonMethod2d call per source: n x 16 as newton()."""
+        tab = np.ascontiguousarray(table, dtype=np.float64)
+        src6 = np.ascontiguousarray(src6, dtype=np.float64).reshape(-1, 6)
+        a2 = np.ascontiguousarray(ang2, dtype=np.float64).reshape(-1, 2)
+        j4 = None if jac4 is None else np.ascontiguousarray(jac4, dtype=np.float64).reshape(-1, 4)
+        vo = np.ascontiguousarray(vobs, dtype=np.float64)
+        out = np.zeros((src6.shape[0], 16))
+        self.lib.refc_newton_once(_vp(h), _p(tab), _u64(tab.shape[1]), ctypes.c_double(length),
+                                  ctypes.c_double(hubble), ctypes.c_double(mpc), ctypes.c_uint(nsteps),
+                                  ctypes.c_double(accuracy), int(JACTYPES[jacobiantype]), int(PLANES[plane]),
+                                  stop_bundle.encode(), ctypes.c_double(opening), int(use_prev), _p(vo), _p(src6),
+                                  _p(a2), _p(j4), ctypes.c_uint(iteration), _u64(src6.shape[0]), _p(out))
+        return out
+
+    def relcat(self, h, table, length, nsteps, targets8, filename, accuracy=1e-8, jacobiantype="infinitesimal",
+               plane="sachs", stop_bundle="redshift", opening=1e-4, use_prev=0, vobs=(0.0, 0.0, 0.0), rotm1=None,
+               prev18=None, hubble=0.7, mpc=3.08568e22):
+        """Catalogues::relCat: writes filename.txt, _err.txt, _reject.txt."""
+        tab = np.ascontiguousarray(table, dtype=np.float64)
+        t8 = np.ascontiguousarray(targets8, dtype=np.float64).reshape(-1, 8)
+        rot = None if rotm1 is None else np.ascontiguousarray(rotm1, dtype=np.float64).reshape(3, 3)
+        pv = None if prev18 is None else np.ascontiguousarray(prev18, dtype=np.float64).reshape(-1, 18)
+        vo = np.ascontiguousarray(vobs, dtype=np.float64)
+        self.lib.refc_relcat(_vp(h), _p(tab), _u64(tab.shape[1]), ctypes.c_double(length), ctypes.c_double(hubble),
+                             ctypes.c_double(mpc), ctypes.c_uint(nsteps), ctypes.c_double(accuracy),
+                             int(JACTYPES[jacobiantype]), int(PLANES[plane]), stop_bundle.encode(),
+                             ctypes.c_double(opening), int(use_prev), _p(vo), _p(rot), _p(t8), _u64(t8.shape[0]),
+                             _p(pv), _u64(0 if pv is None else pv.shape[0]), filename.encode())
+
+    def relcat_prev(self, h, table, length, nsteps, prev18, filename, jacobiantype="infinitesimal", plane="sachs",
+                    stop_bundle="redshift", opening=1e-4, vobs=(0.0, 0.0, 0.0), hubble=0.7, mpc=3.08568e22):
+        """Catalogues::relCat_with_previous_cat: returns the updated rows."""
+        tab = np.ascontiguousarray(table, dtype=np.float64)
+        pv = np.array(prev18, dtype=np.float64).reshape(-1, 18)
+        vo = np.ascontiguousarray(vobs, dtype=np.float64)
+        self.lib.refc_relcat_prev(_vp(h), _p(tab), _u64(tab.shape[1]), ctypes.c_double(length),
+                                  ctypes.c_double(hubble), ctypes.c_double(mpc), ctypes.c_uint(nsteps),
+                                  int(JACTYPES[jacobiantype]), int(PLANES[plane]), stop_bundle.encode(),
+                                  ctypes.c_double(opening), _p(vo), _p(pv), _u64(pv.shape[0]), filename.encode())
+        return pv
+
+    def acquire(self, evolfile, paramfile, celldir, mpc=3.08568e22, capacity=200000):
+        """Input::acquire: (h, omega_m, lboxmpch), 4 x n table."""
+        hol = np.zeros(3)
+        tab = np.zeros((4, capacity))
+        n = ctypes.c_uint64(0)
+        self.lib.refc_acquire(evolfile.encode(), paramfile.encode(), celldir.encode(), ctypes.c_double(mpc),
+                              _p(hol), _p(tab), _u64(capacity), ctypes.byref(n))
+        if n.value > capacity:
+            raise ValueError("table larger than capacity")
+        return tuple(hol), np.ascontiguousarray(tab[:, :n.value])
+
+    def correct(self, table4, traj_t, traj_a):
+        tab = np.ascontiguousarray(table4, dtype=np.float64)
+        t = np.ascontiguousarray(traj_t, dtype=np.float64)
+        a = np.ascontiguousarray(traj_a, dtype=np.float64)
+        out = np.zeros_like(tab)
+        self.lib.refc_correct(_p(tab), _u64(tab.shape[1]), _p(t), _p(a), _u64(t.size), _p(out))
+        return out
+
+    def fullsky_cones(self, ncones):
+        out = np.zeros((ncones, 7))
+        self.lib.refc_fullsky_cones(ctypes.c_uint32(ncones), _p(out))
+        return out
+
+    def narrow_cones(self, celldir, ncones):
+        """(cones7, cones_rot7, rotm1 3 x 3, (thetay, thetaz) rad)."""
+        c = np.zeros((ncones, 7))
+        r = np.zeros((ncones, 7))
+        m = np.zeros((3, 3))
+        tyz = np.zeros(2)
+        self.lib.refc_narrow_cones(celldir.encode(), ctypes.c_uint32(ncones), _p(c), _p(r), _p(m), _p(tyz))
+        return c, r, m, tyz
+
+    def get_targets(self, src8, cones7, icone):
+        s = np.ascontiguousarray(src8, dtype=np.float64).reshape(-1, 8)
+        cn = np.ascontiguousarray(cones7, dtype=np.float64).reshape(-1, 7)
+        out = np.zeros_like(s)
+        m = self.lib.refc_get_targets(_p(s), _u64(s.shape[0]), _p(cn), ctypes.c_uint32(cn.shape[0]),
+                                      ctypes.c_uint32(icone), _p(out))
+        return out[:m]
+
+
 def ref_available():
     return os.path.exists(REF_LIB)
+
+
+def refc_available():
+    return os.path.exists(REFC_LIB) and os.path.exists(HDF5_LIB)
 
 
 def refv_available():

@@ -78,6 +78,12 @@ def tree(name):
     if name == "coarse":
         return synth.analytic_tree(base_level=5, half=0.5, max_level=7, refine=_patch(0.15 * DIR, 0.12),
                                    length=LENGTH, seed=3)
+    if name == "strong":
+        # the refined patch of "amr" with a potential a hundred times deeper:
+        # catalogue sources need the Jacobian-based Newton updates, and at
+        # tight accuracies some are rejected
+        return synth.analytic_tree(base_level=6, half=0.5, max_level=8, refine=_patch(0.22 * DIR, 0.09),
+                                   length=LENGTH, seed=2, amp=1.0e-3)
     if name == "wide":
         # reaches past EXTENT / 4 = 1 from the observer, as the bundles of
         # Integrator::propagate must (integrator.h:907-909)
@@ -91,6 +97,19 @@ TREES = ("uniform", "amr", "coarse", "partial")
 
 def records_digest(rec):
     return hashlib.sha256(np.ascontiguousarray(rec).tobytes()).hexdigest()
+
+
+def sources(n, seed=5, rmin=0.1, rmax=0.35, spread=0.15):
+    """Catalogue sources (x, y, z, vx, vy, vz) around the cone axis at
+    distances in [rmin, rmax) (box units), velocities in simulation units."""
+    rng = np.random.default_rng(seed)
+    d = rng.uniform(rmin, rmax, n)
+    dirs = DIR + rng.normal(scale=spread, size=(n, 3))
+    dirs /= np.linalg.norm(dirs, axis=1)[:, None]
+    src = np.zeros((n, 6))
+    src[:, :3] = dirs * d[:, None]
+    src[:, 3:] = rng.normal(scale=1e-3, size=(n, 3))
+    return src
 
 
 RAYSETS = {"narrow": dict(n_side=8, aperture=6.0), "wide": dict(n_side=8, aperture=40.0)}

@@ -7,11 +7,14 @@ the deterministic inputs of tests/cases.py.  The fixtures are data only:
 inputs are regenerated by tests/cases.py and checked against the stored
 digest; outputs are the reference's numbers.
 
-    python tests/golden/make_golden.py [tree ... | cones | propagate | bundle | born | velocity]
-    (default: keys, cones, propagate, bundle, born, velocity and every tree)
+    python tests/golden/make_golden.py [tree ... | cones | propagate | bundle | born | velocity |
+                                        catalogue | cosmology | targets]
+    (default: all of them and every tree)
 
 The velocity fixture comes from oracle/_ref/libmpfrefvel.so, the reference's
-integrator compiled with VELOCITYFIELD (oracle/ref_harness_vel.cpp).
+integrator compiled with VELOCITYFIELD (oracle/ref_harness_vel.cpp); the
+catalogue, cosmology and targets fixtures from oracle/_ref/libmpfrefcat.so
+(catalogues.h, input.h and miscellaneous.h, oracle/ref_harness_cat.cpp).
 """
 import os
 import sys
@@ -24,7 +27,7 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 import cases  # noqa: E402
-from oracle import RefHarness, RefVelocity  # noqa: E402
+from oracle import RefCatalogue, RefHarness, RefVelocity  # noqa: E402
 
 
 def main(names):
@@ -40,6 +43,12 @@ def main(names):
         return born(R, tab)
     if names == ["velocity"]:
         return velocity(tab)
+    if names == ["catalogue"]:
+        return catalogue(tab)
+    if names == ["cosmology"]:
+        return cosmology()
+    if names == ["targets"]:
+        return targets()
     if names:
         return trees(R, tab, names)
     # keys: compute() at several levels on random and edge points
@@ -61,7 +70,160 @@ def main(names):
     bundle(R, tab)
     born(R, tab)
     velocity(tab)
+    catalogue(tab)
+    cosmology()
+    targets()
     trees(R, tab, cases.TREES)
+
+
+# catalogue root-finder configurations: (tree, accuracy, jacobiantype, bundle (stop_bundle, plane,
+# openingmin) or None, number of sources[, arguments of cases.sources]); the last two bundles are wide
+# enough for some of their rays to leave the tree (the failure entries 42, 42, 22)
+CAT_VOBS = (3e-4, -2e-4, 1e-4)
+CAT_RUNS = (("amr", 1e-8, "infinitesimal", None, 200), ("amr", 1e-12, "infinitesimal", None, 200),
+            ("partial", 1e-12, "infinitesimal", None, 200), ("strong", 1e-8, "infinitesimal", None, 200),
+            ("strong", 1e-12, "infinitesimal", None, 200), ("strong", 1e-12, "infinitesimal_born", None, 200),
+            ("strong", 1e-12, "bundle", ("redshift", "sachs", 1e-4), 80),
+            ("strong", 1e-12, "bundle", ("radius", "normal", 2e-4), 80),
+            ("strong", 1e-12, "bundle", ("plane", "sachs", 1e-4), 80),
+            ("strong", 1e-12, "bundle", ("a", "normal", 1e-4), 80),
+            ("strong", 1e-12, "bundle", ("t", "sachs", 1e-4), 80),
+            ("strong", 1e-12, "bundle", ("lambda", "normal", 1e-4), 80),
+            ("strong", 1e-8, "bundle", ("redshift", "sachs", 0.5), 80, dict(seed=9, rmin=0.3, rmax=0.45, spread=0.25)),
+            ("strong", 1e-8, "bundle", ("a", "sachs", 0.8), 80, dict(seed=9, rmin=0.3, rmax=0.45, spread=0.25)))
+
+
+def cat_name(tree, acc, jt, bundle):
+    b = "" if bundle is None else "_" + "_".join(str(v) for v in bundle)
+    return f"{tree}_{acc:g}_{jt}{b}"
+
+
+def catalogue(tab):
+    """Catalogues::iterateNewtonMethod on the sources of cases.sources (the
+    16 outputs of refc_newton per source), the grid re-run of rejected
+    sources (use_previous_catalogues = 2) and relCat_with_previous_cat
+    (use_previous_catalogues = 1), with an observer velocity."""
+    RC = RefCatalogue()
+    out = {"vobs": np.array(CAT_VOBS)}
+    handles = {}
+    for run in CAT_RUNS:
+        tree, acc, jt, bundle, n = run[:5]
+        if tree not in handles:
+            rec, _ = cases.tree(tree)
+            handles[tree] = RC.tree(rec)
+            out[f"digest_{tree}"] = np.array(cases.records_digest(rec))
+        src = cases.sources(n, **(run[5] if len(run) > 5 else dict(seed=5)))
+        kw = dict(jacobiantype=jt)
+        if bundle is not None:
+            kw.update(stop_bundle=bundle[0], plane=bundle[1], opening=bundle[2])
+        name = cat_name(tree, acc, jt, bundle)
+        out[f"src_{name}"] = src
+        out[f"ref_{name}"] = RC.newton(handles[tree], tab, cases.LENGTH, cases.NSTEPS, src, accuracy=acc,
+                                       vobs=CAT_VOBS, **kw)
+        print(name, "rejected", int(np.sum(out[f"ref_{name}"][:, 3] == 42)), "failed",
+              int(np.sum(out[f"ref_{name}"][:, 0] == 42)))
+    # the grid re-run of the first rejected sources of the strong tree, from
+    # their last angles (catalogues.cpp:240-257, catalogues.h:826-829)
+    h = handles["strong"]
+    base = out["ref_" + cat_name("strong", 1e-12, "infinitesimal", None)]
+    src = out["src_" + cat_name("strong", 1e-12, "infinitesimal", None)]
+    rej = np.flatnonzero(base[:, 3] == 42)[:8]
+    for jt, bundle in (("infinitesimal", None), ("bundle", ("redshift", "sachs", 1e-4))):
+        kw = dict(jacobiantype=jt)
+        if bundle is not None:
+            kw.update(stop_bundle=bundle[0], plane=bundle[1], opening=bundle[2])
+        name = "rerun_" + cat_name("strong", 3e-12, jt, bundle)
+        out[f"src_{name}"] = src[rej]
+        out[f"init_{name}"] = base[rej, :2]
+        out[f"ref_{name}"] = RC.newton(h, tab, cases.LENGTH, cases.NSTEPS, src[rej], init2=base[rej, :2],
+                                       accuracy=3e-12, vobs=CAT_VOBS, use_prev=2, **kw)
+        print(name, "converged", int(np.sum(out[f"ref_{name}"][:, 3] != 42)))
+    # relCat_with_previous_cat on catalogue rows of converged sources
+    conv = out["ref_" + cat_name("strong", 1e-8, "infinitesimal", None)]
+    src = out["src_" + cat_name("strong", 1e-8, "infinitesimal", None)]
+    ok = np.flatnonzero(conv[:, 3] != 42)[:60]
+    d = np.linalg.norm(src[ok, :3], axis=1)
+    prev = np.zeros((ok.size, 18))
+    prev[:, 0] = ok
+    prev[:, 1] = np.arctan2(src[ok, 1], src[ok, 0])
+    prev[:, 2] = np.arccos(src[ok, 2] / d)
+    prev[:, 3:7] = conv[ok, 0:4]
+    prev[:, 7:13] = conv[ok, 8:14]
+    prev[:, 13:17] = conv[ok, 4:8]
+    prev[:, 17] = 1
+    out["prev18"] = prev
+    for jt, bundle in (("infinitesimal", None), ("infinitesimal_born", None), ("bundle", ("redshift", "sachs", 1e-4)),
+                       ("bundle", ("a", "normal", 1e-4)), ("bundle", ("plane", "sachs", 1e-4))):
+        kw = dict(jacobiantype=jt)
+        if bundle is not None:
+            kw.update(stop_bundle=bundle[0], plane=bundle[1], opening=bundle[2])
+        name = "prev_" + cat_name("strong", 0, jt, bundle)
+        out[f"ref_{name}"] = RC.relcat_prev(h, tab, cases.LENGTH, cases.NSTEPS, prev, "/tmp/mpf_golden_relprev", **kw)
+    for hh in handles.values():
+        RC.free(hh)
+    np.savez_compressed(os.path.join(HERE, "catalogue.npz"), **out)
+    print("catalogue written")
+
+
+COSMO_CASES = ((2000, (700, 1301)), (400, ()), (1200, (10,)))
+
+
+def cosmology():
+    """Input::acquire on RAMSES-style evolution files (the files' text is
+    stored with the table), and Input::correct with a perturbed photon."""
+    import tempfile
+    from mpf import synth
+    RC = RefCatalogue()
+    out = {}
+    for n, bad in COSMO_CASES:
+        d = tempfile.mkdtemp()
+        evol, par, cell = synth.ramses_evolution_files(d, h=0.7, omega_m=0.3, n=n, bad_rows=bad)
+        hol, tab = RC.acquire(evol, par, cell, mpc=3.08568e22)
+        key = f"n{n}"
+        out[f"{key}_evol"] = np.frombuffer(open(evol, "rb").read(), dtype=np.uint8)
+        out[f"{key}_param"] = np.frombuffer(open(par, "rb").read(), dtype=np.uint8)
+        out[f"{key}_celldir"] = np.array(os.path.basename(cell))
+        out[f"{key}_hol"] = np.array(hol)
+        out[f"{key}_table"] = tab
+        t, a = tab[0], tab[1]
+        tt = np.linspace(t[0], t[min(1200, len(t) - 1)], 600)
+        ta = np.interp(tt, t, a) * (1 + 1e-3 * np.sin(tt / tt[-1] * 7))
+        out[f"{key}_traj_t"], out[f"{key}_traj_a"] = tt, ta
+        out[f"{key}_corrected"] = RC.correct(tab, tt, ta)
+    np.savez_compressed(os.path.join(HERE, "cosmology.npz"), **out)
+    print("cosmology written")
+
+
+NARROW_CASES = ((16, 20, 70.0, 5.0, 5.0), (4, 33, 60.0, 10.0, 10.0), (50, 0, 90.0, 20.0, 15.0),
+                (1, 200, 45.5, 3.0, 2.0), (9, 20, 70.0, 30.0, 30.0))
+
+
+def targets():
+    """Miscellaneous::GenerateNarrowCones from info_narrow_cone.txt (ASCII
+    input), GenerateFullskyCones, and getTargets of random targets for the
+    first cones of each series."""
+    import tempfile
+    RC = RefCatalogue()
+    out = {}
+    for k, (n, phi, theta, ty, tz) in enumerate(NARROW_CASES):
+        d = tempfile.mkdtemp() + "/"
+        with open(d + "info_narrow_cone.txt", "w") as fh:
+            fh.write(f"phi = {phi}\ntheta = {theta}\nthetay = {ty}\nthetaz = {tz}\n")
+        c, cr, rm, tyz = RC.narrow_cones(d, n)
+        out[f"narrow{k}_params"] = np.array([n, phi, theta, ty, tz], dtype=np.float64)
+        out[f"narrow{k}_cones"], out[f"narrow{k}_rot"], out[f"narrow{k}_rotm1"], out[f"narrow{k}_tyz"] = c, cr, rm, tyz
+        rng = np.random.default_rng(n)
+        pos = rng.normal(size=(1500, 3))
+        pos /= np.linalg.norm(pos, axis=1)[:, None]
+        pos *= rng.uniform(0.05, 1.9, (1500, 1))
+        t8 = np.concatenate([pos, rng.normal(size=(1500, 3)), np.arange(1500)[:, None], np.ones((1500, 1))], axis=1)
+        out[f"narrow{k}_targets"] = t8
+        for ic in range(min(n, 3)):
+            out[f"narrow{k}_sel{ic}"] = RC.get_targets(t8, c, ic)[:, 6]
+    for n in (1, 12, 48):
+        out[f"fullsky{n}"] = RC.fullsky_cones(n)
+    np.savez_compressed(os.path.join(HERE, "targets.npz"), **out)
+    print("targets written")
 
 
 PROP_KINDS = ("redshift", "a", "t", "lambda", "r")

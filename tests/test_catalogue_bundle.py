@@ -1,9 +1,8 @@
 """The oracle's catalogue with jacobiantype = bundle (catalogues.h:455-491,
-509-523, 666-681), CPU only.  catalogues.h needs MPI and is not built here;
-its bundle branch is pinned through its parts: the bundle Jacobian itself is
-the oracle's Lensing::dbetadtheta, bit-identical to the reference's
-(tests/test_bundle_jacobian.py), and the Newton loop around it by closed
-forms and by a second, independent estimate of the same matrix:
+509-523, 666-681), CPU only.  The oracle's bundle branch is pinned bit for
+bit to the reference's own catalogues.h (compiled in oracle/_ref,
+tests/test_ref_catalogue.py); this file adds what the mathematics fixes,
+by closed forms and by a second, independent estimate of the same matrix:
   * phi == 0: every source converges at the first iteration on its unlensed
     direction and the bundle Jacobian is cos(opening) times the identity (the
     chord over the tangent offset of the four rays);

@@ -2,9 +2,9 @@
 mpf_cones_* / mpf_cone_* / mpf_launch_* against fixtures written by the
 reference's own headers (HyperSphere::uniform, Cone::inside,
 Integrator::launch, tests/golden/make_golden.py), against the compiled
-reference where it is present, and against closed forms for the parts the
-reference cannot run here (GenerateNarrowCones and getTargets live in
-miscellaneous.h, which needs MPI and HDF5)."""
+reference where it is present, and against closed forms.  GenerateNarrowCones
+and getTargets (miscellaneous.h) are pinned bit for bit to the reference's
+own code in tests/test_ref_catalogue.py."""
 import math
 import os
 

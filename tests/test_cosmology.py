@@ -1,9 +1,9 @@
 """The cosmology table of the reference, restated on the host
 (Input::acquire input.h:1091-1233, Input::correct input.h:1330-1378).
 
-input.h includes TReadHDF5.h, which needs the HDF5 library: the reference's
-own acquire/correct cannot be built here, so the restatement is pinned by
-what the mathematics fixes: a flat LCDM evolution file gives back the
+The tables are bit-identical to the reference's own acquire/correct
+(input.h compiled in oracle/_ref, tests/test_ref_catalogue.py); this file
+adds what the mathematics fixes: a flat LCDM evolution file gives back the
 conformal lookback time, da/dtau and d2a/dtau2 of the model; the parsed h,
 Omega_m and box length; dropped rows; and correct() is the identity when the
 homogeneous photon reproduces the table, and the interpolant of the photon

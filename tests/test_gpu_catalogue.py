@@ -1,9 +1,8 @@
 """Batched catalogue root-finding on the GPU (catalogues.h:410-920) against
-the oracle's restatement of iterateNewtonMethod/newtonMethod2d on its pinned
-trajectories and Jacobians.  catalogues.h itself needs MPI and is not built
-here: the restatement is pinned through its parts (integrate, the Jacobian)
-and by the closed form phi == 0 (converged at the first iteration on the
-unlensed direction).
+the oracle's restatement of iterateNewtonMethod/newtonMethod2d, which is
+pinned bit for bit to the reference's own catalogues.h (compiled in
+oracle/_ref, tests/test_ref_catalogue.py), and by the closed form phi == 0
+(converged at the first iteration on the unlensed direction).
 
 Tolerances: same status and iteration count; observed angles within 1e-12 rad
 (the device's sines, cosines, atan2 and acos differ from libm in the last

@@ -1,9 +1,10 @@
 """Statistics of the rays program (rays.cpp:300-456) on the host
 (mpf_ray_statistics), over central trajectories and distances from the
 oracle's propagate (pinned to the reference's own propagate in
-tests/test_propagate.py).  rays.cpp itself needs MPI and cannot run here:
-the reduction is pinned by re-interpolating each curve with the reference's
-own Utility::interpolate (the compiled reference harness) and by closed forms
+tests/test_propagate.py).  rays.cpp computes its statistics inside main(),
+which no harness can call: the reduction is pinned by re-interpolating each
+curve with the reference's own Utility::interpolate (the compiled reference
+harness) and by closed forms
 (identical rays: zero spread; the redshift against itself: the identity).
 CPU only."""
 import numpy as np
