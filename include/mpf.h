@@ -354,6 +354,19 @@ typedef struct {
     int32_t bundle_kind;
     int32_t bundle_plane;
     double bundle_opening;
+    /* use_previous_catalogues (catalogues.h:501-623, catalogues.cpp:240-257):
+     * 0 the catalogue; 2 the re-run of rejected sources from their last
+     * angles (init2): a source not within the accuracy sends a grid of up to
+     * 11 x 11 rays at offsets from -err to err in steps of err / 5,
+     * err = 5 sep / (iteration + 1), around its angles, and takes the first
+     * within the accuracy (converged: its angles, with z1 and the scale factor
+     * of the central ray and, for the bundle, the bundle around the central
+     * direction) or else the closest if closer than the central ray; no
+     * Jacobian-based update, and no Jacobian for a rejected source.  The
+     * reference allows 100 iterations there (set max_iterations).  A grid ray
+     * that ends before the source radius is no candidate (the reference
+     * extrapolates it). */
+    uint32_t use_previous;
 } mpf_newton_cfg;
 /* Per source, MPF_NCAT doubles: the catalogue columns of catalogues.h:844-859
  * plus status and the number of rounds run. */
@@ -376,6 +389,30 @@ int mpf_catalogue_solve(mpf_ctx* ctx, const double* src6, const double* init2, u
 /* Same on device buffers (sync on return). */
 int mpf_catalogue_solve_dev(mpf_ctx* ctx, const double* dev_src6, const double* dev_init2, uint64_t n,
                             const mpf_newton_cfg* cfg, double* dev_out, uint64_t* total_rays);
+/* Catalogues::relCat_with_previous_cat (catalogues.h:922-1044), the re-run
+ * of a computed catalogue with use_previous_catalogues = 1: for each row of
+ * rows18 (n x 18: the catalogue file's columns id, phi, theta, phi_obs,
+ * theta_obs, separation, iterations, z0..z5, a11, a12, a21, a22, npart) a
+ * ray from its observed angles (its comoving ones for jacobian_born) to the
+ * scale factor 1 / (1 + z0), and the lensing Jacobian of cfg's
+ * jacobiantype at the distance interpolated there replaces columns 13-16,
+ * in place.  A row whose ray never reaches that scale factor gets 42, 42,
+ * 42, 42 (the reference extrapolates the ray's last segment); a failed
+ * bundle gives the reference's entries.  Of cfg only jacobian_born,
+ * jacobian_bundle and the bundle fields are read. */
+int mpf_catalogue_previous(mpf_ctx* ctx, double* rows18, uint64_t n, const mpf_newton_cfg* cfg);
+/* The files of Catalogues::relCat (catalogues.h:862-912) from n rows of
+ * mpf_catalogue_solve, host only: <prefix>.txt (converged sources: id, the
+ * 16 catalogue columns, npart, 17 significant digits), <prefix>_err.txt (the
+ * ray never reached the source, or its bundle failed: id, phi, theta,
+ * phi_obs, theta_obs and the separation twice, at the stream's default 6
+ * digits) and <prefix>_reject.txt (not converged: as <prefix>.txt), each
+ * replaced.  id_npart: n x 2 (the targets' id and npart). */
+int mpf_catalogue_write(const char* prefix, const double* out, uint64_t n, const double* id_npart);
+/* The files of relCat_with_previous_cat (catalogues.h:1008-1040), host
+ * only: the 18 columns of each row with 17 significant digits, in
+ * <prefix>.txt when a11 and a12 are not 42, else in <prefix>_err.txt. */
+int mpf_catalogue_write_previous(const char* prefix, const double* rows18, uint64_t n);
 
 /* Learned first guess of the catalogue root-finder.  No counterpart in the
  * reference: it proposes the observed angles that seed the Newton

@@ -162,6 +162,7 @@ struct mpf_ctx {
     uint64_t last_steps = 0;
     float last_ms = 0.f, last_kernel_ms = 0.f;
     bool kernel_timed = false;  // the last integration recorded evk0/evk1
+    bool surf_jac_chi = false;  // surfaces take the Jacobian at the interpolated chi
     bool pending = false;
     // learned first guess of the catalogue root-finder
     mpf::MlpModel* mlp = nullptr;
@@ -607,6 +608,7 @@ static int integrate_impl(mpf_ctx* c, const double* dev_in9, uint64_t n, const m
         A.surf_out = dev_surf;
         A.jac = surf->jacobian ? 1 : 0;
         A.ray_value = surf->ray_value;
+        A.jac_at_chi = c->surf_jac_chi ? 1 : 0;
     }
     if (dev_traj) {
         A.traj = dev_traj;
@@ -1012,13 +1014,16 @@ int mpf_catalogue_solve_dev(mpf_ctx* c, const double* dev_src6, const double* de
     if (bundle) {
         const int k = cfg->bundle_kind;
         if (k != MPF_STOP_RADIUS && k != MPF_STOP_LAMBDA && k != MPF_STOP_REDSHIFT && k != MPF_STOP_T &&
-            k != MPF_STOP_A)
-            return fail(c, ERR_INVAL, "stop_bundle must be radius, lambda, redshift, t or a");
+            k != MPF_STOP_A && k != MPF_STOP_PLANE)
+            return fail(c, ERR_INVAL, "stop_bundle must be radius, lambda, redshift, t, a or plane");
         if (cfg->bundle_plane != 0 && cfg->bundle_plane != 1) return fail(c, ERR_INVAL, "plane must be sachs or normal");
         if (!(cfg->bundle_opening > 0) || !(cfg->bundle_opening < 1.5))
             return fail(c, ERR_INVAL, "bundle opening out of range");
         if (cfg->jacobian_born) return fail(c, ERR_INVAL, "jacobiantype is bundle or infinitesimal_born, not both");
     }
+    if (cfg->use_previous != 0 && cfg->use_previous != 2)
+        return fail(c, ERR_INVAL, "use_previous must be 0 or 2 (mode 1 is mpf_catalogue_previous)");
+    const bool rerun = cfg->use_previous == 2;
     HIPCHK(c, hipSetDevice(c->cfg.device));
     if (total_rays) *total_rays = 0;
     if (n == 0) return 0;
@@ -1044,17 +1049,20 @@ int mpf_catalogue_solve_dev(mpf_ctx* c, const double* dev_src6, const double* de
         i += (nn > 1) * ((i <= 0) - (i >= nn));
         a0 = (nn > 1) ? (ca[i - 1] + ((ca[i] - ca[i - 1]) * (0.0 - ct[i - 1]) / (ct[i] - ct[i - 1]))) : ca[0];
     }
-    DevBuf<double> dist, ang, in9, dact, surf, outb, bctx, jb;
-    DevBuf<uint32_t> act, next, need;
+    DevBuf<double> dist, ang, in9, dact, surf, outb, bctx, jb, gerr, gas, gin9, gdist, gsurf, gout;
+    DevBuf<uint32_t> act, next, need, gsrc, gcnt, goff;
     DevBuf<unsigned int> cnt;
     auto release = [&]() {
         dist.release(); ang.release(); in9.release(); dact.release(); surf.release(); act.release(); next.release();
         need.release(); cnt.release(); outb.release(); bctx.release(); jb.release();
+        gerr.release(); gas.release(); gin9.release(); gdist.release(); gsurf.release(); gout.release();
+        gsrc.release(); gcnt.release(); goff.release();
     };
     if (dist.ensure(n) || ang.ensure(2 * n) || in9.ensure(9 * n) || dact.ensure(n) ||
         surf.ensure((uint64_t)MPF_NSURF * n) || act.ensure(n) || next.ensure(n) || need.ensure(n) || cnt.ensure(3) ||
         outb.ensure((uint64_t)MPF_NOUT * n) ||
-        (bundle && (bctx.ensure((uint64_t)mpf::CAT_BCTX * n) || jb.ensure(4 * n)))) {
+        (bundle && (bctx.ensure((uint64_t)mpf::CAT_BCTX * n) || jb.ensure(4 * n))) ||
+        (rerun && (gerr.ensure(n) || gas.ensure(n) || gsrc.ensure(n) || gcnt.ensure(n) || goff.ensure(n)))) {
         release();
         return fail(c, ERR_NOMEM, "catalogue buffers");
     }
@@ -1075,14 +1083,15 @@ int mpf_catalogue_solve_dev(mpf_ctx* c, const double* dev_src6, const double* de
     R.mpc = cfg->mpc; R.a0 = a0; R.phi0 = phi0;
     for (int k = 0; k < 3; ++k) R.vobs[k] = cfg->vobs[k];
     R.bundle = bundle ? 1 : 0; R.bkind = cfg->bundle_kind; R.bplane = cfg->bundle_plane; R.bctx = bctx.p;
+    R.born = cfg->jacobian_born ? 1 : 0;
+    R.rerun = rerun ? 1 : 0; R.gsrc = gsrc.p; R.ngrid = cnt.p + 2; R.gerr = gerr.p; R.gas = gas.p;
     // the bundle Jacobians (jacobiantype = bundle) of the sources in ids[0, m): their current
     // angles and crossing context read back, mpf_jacobian_bundle on the host, the result in jb
     std::vector<double> h_dist;
     auto bundle_jacobians = [&](const uint32_t* d_ids, uint64_t m) -> int {
         std::vector<uint32_t> hid(m);
-        std::vector<double> hang(2 * n), hctx((uint64_t)mpf::CAT_BCTX * n);
+        std::vector<double> hctx((uint64_t)mpf::CAT_BCTX * n);
         if (hipMemcpyAsync(hid.data(), d_ids, m * 4, hipMemcpyDeviceToHost, c->stream) ||
-            hipMemcpyAsync(hang.data(), ang.p, 2 * n * 8, hipMemcpyDeviceToHost, c->stream) ||
             hipMemcpyAsync(hctx.data(), bctx.p, (uint64_t)mpf::CAT_BCTX * n * 8, hipMemcpyDeviceToHost, c->stream) ||
             hipStreamSynchronize(c->stream))
             return fail(c, ERR_HIP, "bundle context download");
@@ -1094,7 +1103,7 @@ int mpf_catalogue_solve_dev(mpf_ctx* c, const double* dev_src6, const double* de
         for (uint64_t q = 0; q < m; ++q) {
             const uint64_t i = hid[q];
             const double* b = hctx.data() + (uint64_t)mpf::CAT_BCTX * i;
-            pt[2 * q] = hang[2 * i]; pt[2 * q + 1] = hang[2 * i + 1];
+            pt[2 * q] = b[7]; pt[2 * q + 1] = b[8];   // the direction the bundle is launched around
             for (int k = 0; k < 3; ++k) ki[3 * q + k] = b[3 + k];
             ref[q] = b[6];
             dd[q] = h_dist[i];
@@ -1111,8 +1120,10 @@ int mpf_catalogue_solve_dev(mpf_ctx* c, const double* dev_src6, const double* de
     unsigned int zero2[2] = {0u, 0u};
     CATCHK(hipMemcpyAsync(cnt.p + 1, zero2, 4, hipMemcpyHostToDevice, c->stream));
     for (int round = 0; nact > 0 && round <= (int)cfg->max_iterations; ++round) {
-        // the bundle replaces the ray's own Jacobian from iteration 5 (catalogues.h:509-523)
-        const int jac = ((round >= 3 || cfg->jacobian_all_rounds) && !(bundle && round >= 5)) ? 1 : 0;
+        // the bundle replaces the ray's own Jacobian from iteration 5 (catalogues.h:509-523); the
+        // re-run mode updates without one
+        const int jac =
+            ((round >= 3 || cfg->jacobian_all_rounds) && !(bundle && round >= 5) && !rerun) ? 1 : 0;
         CATCHK(mpf::launch_cat_launch(ang.p, act.p, nact, in9.p, dact.p, dist.p, c->stream));
         mpf_surfaces sf{};
         sf.kind = MPF_STOP_RADIUS;
@@ -1123,14 +1134,47 @@ int mpf_catalogue_solve_dev(mpf_ctx* c, const double* dev_src6, const double* de
         if (rc) { release(); return rc; }
         rays += nact;
         CATCHK(hipMemcpyAsync(cnt.p, zero2, 4, hipMemcpyHostToDevice, c->stream));
+        if (rerun) CATCHK(hipMemcpyAsync(cnt.p + 2, zero2, 4, hipMemcpyHostToDevice, c->stream));
         R.act = act.p; R.next = next.p; R.nact = nact; R.round = round; R.jac = jac;
         CATCHK(mpf::launch_newton(R, c->stream));
+        if (rerun) {  // the grids of the sources not within the accuracy, in one batch
+            unsigned int ng = 0;
+            CATCHK(hipMemcpyAsync(&ng, cnt.p + 2, 4, hipMemcpyDeviceToHost, c->stream));
+            CATCHK(hipStreamSynchronize(c->stream));
+            if (ng) {
+                CATCHK(mpf::launch_cat_grid_count(R, ng, gcnt.p, c->stream));
+                std::vector<uint32_t> hc(ng), ho(ng);
+                CATCHK(hipMemcpyAsync(hc.data(), gcnt.p, 4ull * ng, hipMemcpyDeviceToHost, c->stream));
+                CATCHK(hipStreamSynchronize(c->stream));
+                uint64_t tot = 0;
+                for (unsigned g = 0; g < ng; ++g) {
+                    ho[g] = (uint32_t)tot;
+                    tot += hc[g];
+                }
+                if (gin9.ensure(9 * tot) || gdist.ensure(tot) || gsurf.ensure((uint64_t)MPF_NSURF * tot) ||
+                    gout.ensure((uint64_t)MPF_NOUT * tot)) {
+                    release();
+                    return fail(c, ERR_NOMEM, "re-run grid buffers");
+                }
+                CATCHK(hipMemcpyAsync(goff.p, ho.data(), 4ull * ng, hipMemcpyHostToDevice, c->stream));
+                CATCHK(mpf::launch_cat_grid_launch(R, ng, goff.p, gcnt.p, gin9.p, gdist.p, c->stream));
+                mpf_surfaces gs{};
+                gs.kind = MPF_STOP_RADIUS;
+                gs.n = 1;
+                gs.jacobian = 0;
+                gs.ray_value = gdist.p;
+                int rc = mpf_integrate_surfaces_dev(c, gin9.p, tot, &gs, gout.p, gsurf.p, 1);
+                if (rc) { release(); return rc; }
+                rays += tot;
+                CATCHK(mpf::launch_cat_grid_select(R, ng, goff.p, gcnt.p, gsurf.p, c->stream));
+            }
+        }
         unsigned int nn = 0;
         CATCHK(hipMemcpyAsync(&nn, cnt.p, 4, hipMemcpyDeviceToHost, c->stream));
         CATCHK(hipStreamSynchronize(c->stream));
         std::swap(act.p, next.p);
         nact = nn;
-        if (bundle && round >= 5 && nact) {  // the pending updates with the bundle Jacobian
+        if (bundle && !rerun && round >= 5 && nact) {  // the pending updates with the bundle Jacobian
             if (int rc = bundle_jacobians(act.p, nact)) { release(); return rc; }
             CATCHK(hipMemcpyAsync(cnt.p + 2, zero2, 4, hipMemcpyHostToDevice, c->stream));
             CATCHK(mpf::launch_newton_bundle(R, act.p, nact, jb.p, next.p, cnt.p + 2, c->stream));
@@ -1207,6 +1251,134 @@ int mpf_catalogue_solve(mpf_ctx* c, const double* src6, const double* init2, uin
     if (!rc && n && hipMemcpy(out, o.p, (uint64_t)MPF_NCAT * n * 8, hipMemcpyDeviceToHost) != hipSuccess)
         rc = fail(c, ERR_HIP, "catalogue download");
     s.release(); i2.release(); o.release();
+    return rc;
+}
+
+int mpf_catalogue_previous(mpf_ctx* c, double* rows18, uint64_t n, const mpf_newton_cfg* cfg) {
+    if (!c || !cfg || (n && !rows18)) return ERR_INVAL;
+    if (!c->have_tree) return fail(c, ERR_STATE, "no octree loaded");
+    if (c->ncos <= 0) return fail(c, ERR_STATE, "no cosmology table set");
+    const bool bundle = cfg->jacobian_bundle != 0, born = cfg->jacobian_born != 0;
+    if (bundle && born) return fail(c, ERR_INVAL, "jacobiantype is bundle or infinitesimal_born, not both");
+    if (bundle) {
+        const int k = cfg->bundle_kind;
+        if (k != MPF_STOP_RADIUS && k != MPF_STOP_LAMBDA && k != MPF_STOP_REDSHIFT && k != MPF_STOP_T &&
+            k != MPF_STOP_A && k != MPF_STOP_PLANE)
+            return fail(c, ERR_INVAL, "stop_bundle must be radius, lambda, redshift, t, a or plane");
+        if (cfg->bundle_plane != 0 && cfg->bundle_plane != 1) return fail(c, ERR_INVAL, "plane must be sachs or normal");
+        if (!(cfg->bundle_opening > 0) || !(cfg->bundle_opening < 1.5))
+            return fail(c, ERR_INVAL, "bundle opening out of range");
+    }
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    if (n == 0) return 0;
+    // the launch from the observed angles (the comoving ones for the Born
+    // approximation) to the scale factor of the FLRW redshift (catalogues.h:936-946),
+    // launched here on the host as Integrator::launch does
+    std::vector<double> h9(9 * n), sf(n), ang(2 * n);
+    for (uint64_t i = 0; i < n; ++i) {
+        const double* r = rows18 + 18 * i;
+        sf[i] = 1. / (1. + r[7]);
+        const double phi = born ? r[1] : r[3], theta = born ? r[2] : r[4];
+        ang[2 * i] = phi;
+        ang[2 * i + 1] = theta;
+        double* p = h9.data() + 9 * i;
+        p[0] = 1.0; p[1] = 0.0; p[2] = 0.0; p[3] = 0.0; p[4] = 0.0; p[5] = 1.0;
+        p[6] = std::cos(phi) * std::sin(theta);
+        p[7] = std::sin(phi) * std::sin(theta);
+        p[8] = std::cos(theta);
+    }
+    DevBuf<double> din, dsf, dout, dsurf;
+    auto release = [&]() { din.release(); dsf.release(); dout.release(); dsurf.release(); };
+    if (din.ensure(9 * n) || dsf.ensure(n) || dout.ensure((uint64_t)MPF_NOUT * n) || dsurf.ensure((uint64_t)MPF_NSURF * n)) {
+        release();
+        return fail(c, ERR_NOMEM, "previous-catalogue buffers");
+    }
+    std::vector<double> hs((uint64_t)MPF_NSURF * n);
+    int rc = 0;
+    if (hipMemcpy(din.p, h9.data(), 9 * n * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dsf.p, sf.data(), n * 8, hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(c, ERR_HIP, "previous-catalogue upload");
+    if (!rc) {
+        mpf_surfaces s{};
+        s.kind = MPF_STOP_A;
+        s.n = 1;
+        s.jacobian = (!bundle && !born) ? 1 : 0;
+        s.ray_value = dsf.p;
+        c->surf_jac_chi = true;
+        const int prec = c->cfg.precision;
+        rc = mpf_integrate_surfaces_dev(c, din.p, n, &s, dout.p, dsurf.p, 1);
+        c->surf_jac_chi = false;
+        c->cfg.precision = prec;
+    }
+    if (!rc && hipMemcpy(hs.data(), dsurf.p, (uint64_t)MPF_NSURF * n * 8, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = fail(c, ERR_HIP, "previous-catalogue download");
+    // rows whose ray reached the scale factor; the others get 42 entries
+    std::vector<uint64_t> ok;
+    for (uint64_t i = 0; !rc && i < n; ++i) {
+        const double* r = hs.data() + (uint64_t)MPF_NSURF * i;
+        double* J = rows18 + 18 * i + 13;
+        if (r[mpf::SF_CROSSED] == 0.0) {
+            J[0] = J[1] = J[2] = J[3] = 42;
+            continue;
+        }
+        ok.push_back(i);
+        if (!bundle && !born) { J[0] = r[MPF_SURF_A11]; J[1] = r[MPF_SURF_A12]; J[2] = r[MPF_SURF_A12]; J[3] = r[MPF_SURF_A22]; }
+    }
+    const uint64_t m = ok.size();
+    if (!rc && m && born) {
+        // the homogeneous path to the crossing's distance, sampled on the data (catalogues.h:995-999)
+        std::vector<double> b9(9 * m), bd(m);
+        for (uint64_t q = 0; q < m; ++q) {
+            std::copy(h9.begin() + 9 * ok[q], h9.begin() + 9 * ok[q] + 9, b9.begin() + 9 * q);
+            bd[q] = hs[(uint64_t)MPF_NSURF * ok[q] + MPF_SURF_CHI];
+        }
+        if (hipMemcpy(din.p, b9.data(), 9 * m * 8, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(dsf.p, bd.data(), m * 8, hipMemcpyHostToDevice) != hipSuccess)
+            rc = fail(c, ERR_HIP, "Born upload");
+        if (!rc) {
+            mpf_surfaces s{};
+            s.kind = MPF_STOP_RADIUS;
+            s.n = 1;
+            s.jacobian = 1;
+            s.ray_value = dsf.p;
+            rc = mpf_integrate_surfaces_born_dev(c, din.p, m, &s, dout.p, dsurf.p, 1);
+        }
+        std::vector<double> bs((uint64_t)MPF_NSURF * m);
+        if (!rc && hipMemcpy(bs.data(), dsurf.p, (uint64_t)MPF_NSURF * m * 8, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = fail(c, ERR_HIP, "Born download");
+        for (uint64_t q = 0; !rc && q < m; ++q) {
+            const double* r = bs.data() + (uint64_t)MPF_NSURF * q;
+            double* J = rows18 + 18 * ok[q] + 13;
+            J[0] = r[MPF_SURF_A11]; J[1] = r[MPF_SURF_A12]; J[2] = r[MPF_SURF_A12]; J[3] = r[MPF_SURF_A22];
+        }
+    }
+    if (!rc && m && bundle) {
+        // Lensing::dbetadtheta with kiTarget and interpRef of the crossing (catalogues.h:950-992)
+        std::vector<double> pt(2 * m), ki(3 * m), ref(m), dd(m), J(4 * m);
+        for (uint64_t q = 0; q < m; ++q) {
+            const uint64_t i = ok[q];
+            const double* r = hs.data() + (uint64_t)MPF_NSURF * i;
+            pt[2 * q] = ang[2 * i];
+            pt[2 * q + 1] = ang[2 * i + 1];
+            const double fp[3] = {r[MPF_SURF_X], r[MPF_SURF_Y], r[MPF_SURF_Z]};
+            for (int k = 0; k < 3; ++k) ki[3 * q + k] = cfg->bundle_plane == 0 ? r[MPF_SURF_KX + k] : fp[k];
+            dd[q] = r[MPF_SURF_CHI];
+            switch (cfg->bundle_kind) {
+                case MPF_STOP_REDSHIFT: ref[q] = r[MPF_SURF_REDSHIFT]; break;
+                case MPF_STOP_A: ref[q] = sf[i]; break;
+                case MPF_STOP_T: ref[q] = r[MPF_SURF_T]; break;
+                case MPF_STOP_LAMBDA: ref[q] = r[MPF_SURF_LAMBDA]; break;
+                case MPF_STOP_PLANE: ref[q] = ki[3 * q] * fp[0] + ki[3 * q + 1] * fp[1] + ki[3 * q + 2] * fp[2]; break;
+                default: ref[q] = dd[q]; break;
+            }
+        }
+        const double origin[3] = {0.0, 0.0, 0.0};
+        rc = mpf_jacobian_bundle(c, pt.data(), m, origin, cfg->bundle_kind, ref.data(), 1, ki.data(), dd.data(),
+                                 cfg->bundle_opening, J.data());
+        for (uint64_t q = 0; !rc && q < m; ++q)
+            for (int k = 0; k < 4; ++k) rows18[18 * ok[q] + 13 + k] = J[4 * q + k];
+    }
+    release();
     return rc;
 }
 

@@ -460,8 +460,9 @@ __global__ void surface_final_kernel(IntegrateArgs A) {
         return;
     }
     // the catalogue's Jacobian is taken at the source distance
-    // (catalogues.h:516), the maps' at the interpolated position's (hmaps.h:667)
-    const double D = (A.ray_value && A.surf_kind == 0) ? o[SF_REF] : o[SF_DIST];
+    // (catalogues.h:516), the maps' at the interpolated position's (hmaps.h:667),
+    // the re-run catalogue's at the interpolated chi (catalogues.h:955)
+    const double D = A.jac_at_chi ? o[SF_CHI] : (A.ray_value && A.surf_kind == 0) ? o[SF_REF] : o[SF_DIST];
     double a[3];
     for (int c = 0; c < 3; ++c) a[c] = o[SF_S1 + c] - o[SF_S1 + 3 + c] / D;
     o[SF_A11] = 1 - 2 * a[0] / C2 * A.length;

@@ -40,6 +40,10 @@ struct IntegrateArgs {
     double* surf_out = nullptr;      // n x nsurf x SF_NREC
     const double* ray_value = nullptr; // per-ray stop/surface value (one surface), or null
     int jac = 0;
+    // the distance the Jacobian is taken at: 0 the source distance of a
+    // per-ray radius (catalogues.h:516), else the interpolated position's
+    // (hmaps.h:667); 1 the interpolated chi (catalogues.h:955)
+    int jac_at_chi = 0;
     // whole trajectories (exact tiers): element k of ray i at
     // traj[(i * traj_cap + k) * MPF_NOUT], k < traj_cap; traj_over counts
     // rays with more elements than that
@@ -93,11 +97,29 @@ struct CatRound {
     int round = 0, jac = 0, max_iter = 10;
     double acc = 1e-8, length = 1.0, h = 0.7, mpc = 3.08568e22, vobs[3] = {0, 0, 0}, a0 = 1.0, phi0 = 0.0;
     // jacobiantype = bundle: stop_bundle (MPF_STOP_*), plane (0 sachs, 1 normal), and the
-    // per-source crossing context (n x 8: finalPoint, kiTarget, interpRef) the host reads
+    // per-source crossing context (n x CAT_BCTX: finalPoint, kiTarget, interpRef, the bundle's
+    // direction) the host reads
     int bundle = 0, bkind = 0, bplane = 0;
     double* bctx = nullptr;
+    // jacobiantype = infinitesimal_born: the converged Jacobian comes from a Born pass
+    int born = 0;
+    // use_previous_catalogues = 2: non-converged sources go to the grid list (gsrc, ngrid) with
+    // their grid half-size gerr and the central ray's scale factor gas, per source
+    int rerun = 0;
+    uint32_t* gsrc = nullptr;
+    unsigned int* ngrid = nullptr;
+    double* gerr = nullptr;
+    double* gas = nullptr;
 };
-constexpr int CAT_BCTX = 8;
+constexpr int CAT_BCTX = 10;
+// the grid of the re-run mode: rays per grid source (cnt), their launch
+// photons and radii from the slot offsets (off, an exclusive sum of cnt)
+hipError_t launch_cat_grid_count(const CatRound& R, uint64_t ngrid, uint32_t* cnt, hipStream_t st);
+hipError_t launch_cat_grid_launch(const CatRound& R, uint64_t ngrid, const uint32_t* off, const uint32_t* cnt,
+                                  double* in9, double* dist_slot, hipStream_t st);
+// the choice among each source's grid rays (surf: their radius surfaces), survivors to R.next
+hipError_t launch_cat_grid_select(const CatRound& R, uint64_t ngrid, const uint32_t* off, const uint32_t* cnt,
+                                  const double* gsurf, hipStream_t st);
 hipError_t launch_cat_init(const double* src, const double* init, uint64_t n, double* dist, double* ang, double* out,
                            hipStream_t st);
 hipError_t launch_cat_launch(const double* ang, const uint32_t* act, uint64_t nact, double* in9, double* dist_act,

@@ -33,7 +33,8 @@ EXPORTS = ("mpf_create", "mpf_destroy", "mpf_last_error", "mpf_set_precision", "
            "mpf_dev_alloc", "mpf_dev_free", "mpf_dev_upload", "mpf_dev_download", "mpf_dev_copy",
            "mpf_integrate_surfaces", "mpf_integrate_surfaces_dev", "mpf_healpix_pix2vec", "mpf_healpix_vec2pix",
            "mpf_hmaps_launch_dev", "mpf_hmaps_fill_dev", "mpf_healpix_bin_dev", "mpf_catalogue_solve",
-           "mpf_catalogue_solve_dev", "mpf_mlp_init", "mpf_mlp_state", "mpf_mlp_train", "mpf_mlp_gradient",
+           "mpf_catalogue_solve_dev", "mpf_catalogue_previous", "mpf_catalogue_write",
+           "mpf_catalogue_write_previous", "mpf_mlp_init", "mpf_mlp_state", "mpf_mlp_train", "mpf_mlp_gradient",
            "mpf_mlp_infer", "mpf_mlp_infer_dev", "mpf_catalogue_features", "mpf_catalogue_features_dev",
            "mpf_catalogue_seed", "mpf_catalogue_seed_dev", "mpf_mfma_probe", "mpf_integrate_trajectory",
            "mpf_cosmology_acquire", "mpf_cosmology_correct", "mpf_cosmology_prepare", "mpf_cosmology_last_error",
@@ -102,16 +103,19 @@ class NewtonCfg(ctypes.Structure):
     _fields_ = [("accuracy", ctypes.c_double), ("max_iterations", ctypes.c_uint32), ("vobs", ctypes.c_double * 3),
                 ("h", ctypes.c_double), ("mpc", ctypes.c_double), ("jacobian_all_rounds", ctypes.c_uint32),
                 ("jacobian_born", ctypes.c_uint32), ("jacobian_bundle", ctypes.c_uint32),
-                ("bundle_kind", ctypes.c_int32), ("bundle_plane", ctypes.c_int32), ("bundle_opening", ctypes.c_double)]
+                ("bundle_kind", ctypes.c_int32), ("bundle_plane", ctypes.c_int32), ("bundle_opening", ctypes.c_double),
+                ("use_previous", ctypes.c_uint32)]
 
 BUNDLE_PLANES = {"sachs": 0, "normal": 1}
 
 
 def make_newton(accuracy=1e-8, max_iterations=10, vobs=(0.0, 0.0, 0.0), h=0.7, mpc=3.08568e22,
-                jacobian_all_rounds=False, jacobian_born=False, bundle=None):
+                jacobian_all_rounds=False, jacobian_born=False, bundle=None, use_previous=0):
     """bundle: None, or (stop_bundle, plane, opening) for jacobiantype = bundle,
-    e.g. ("redshift", "sachs", 1e-4)."""
+    e.g. ("redshift", "sachs", 1e-4); use_previous 2: the grid re-run of
+    rejected sources."""
     c = NewtonCfg()
+    c.use_previous = int(use_previous)
     c.jacobian_all_rounds = int(bool(jacobian_all_rounds))
     c.jacobian_born = int(bool(jacobian_born))
     if bundle is not None:
@@ -187,6 +191,9 @@ def load_library(path=LIB_PATH):
     lib.mpf_integrate_surfaces_dev.argtypes = [vp, vp, u64, ctypes.POINTER(Surfaces), vp, vp, i32]
     lib.mpf_catalogue_solve.argtypes = [vp, vp, vp, u64, ctypes.POINTER(NewtonCfg), vp, vp]
     lib.mpf_catalogue_solve_dev.argtypes = [vp, vp, vp, u64, ctypes.POINTER(NewtonCfg), vp, vp]
+    lib.mpf_catalogue_previous.argtypes = [vp, vp, u64, ctypes.POINTER(NewtonCfg)]
+    lib.mpf_catalogue_write.argtypes = [ctypes.c_char_p, vp, u64, vp]
+    lib.mpf_catalogue_write_previous.argtypes = [ctypes.c_char_p, vp, u64]
     i64 = ctypes.c_int64
     lib.mpf_healpix_pix2vec.argtypes = [i64, vp, u64, vp]
     lib.mpf_healpix_vec2pix.argtypes = [i64, vp, u64, vp]
@@ -408,6 +415,23 @@ def prepare_ascii(celldir, cones_rot7, conefiles, table, minicone=None, rotm1=No
     _prep_check(load_library().mpf_prepare_ascii(ctypes.byref(c), cdir, mdir, _p(cones), cones.shape[0], _p(rot), ntasks,
                                                  rank, seed, _p(t), _p(a), _p(dadt), t.size, names),
                 "mpf_prepare_ascii")
+
+
+def catalogue_write(prefix, out, id_npart):
+    """relCat's three files (<prefix>.txt, _err.txt, _reject.txt) from
+    mpf_catalogue_solve rows and the targets' (id, npart)."""
+    out = np.ascontiguousarray(out, dtype=np.float64).reshape(-1, NCAT)
+    idn = np.ascontiguousarray(id_npart, dtype=np.float64).reshape(-1, 2)
+    rc = load_library().mpf_catalogue_write(str(prefix).encode(), _p(out), ctypes.c_uint64(out.shape[0]), _p(idn))
+    if rc:
+        raise MpfError(f"mpf_catalogue_write failed ({rc})")
+
+
+def catalogue_write_previous(prefix, rows18):
+    rows = np.ascontiguousarray(rows18, dtype=np.float64).reshape(-1, 18)
+    rc = load_library().mpf_catalogue_write_previous(str(prefix).encode(), _p(rows), ctypes.c_uint64(rows.shape[0]))
+    if rc:
+        raise MpfError(f"mpf_catalogue_write_previous failed ({rc})")
 
 
 def _dir(d):
@@ -836,6 +860,15 @@ class Context:
                                                      ctypes.c_uint64(n), ctypes.byref(cfg), ctypes.c_void_p(out_ptr),
                                                      ctypes.byref(rays)), "mpf_catalogue_solve_dev")
         return rays.value
+
+    def catalogue_previous(self, rows18, **newton):
+        """relCat_with_previous_cat: returns a copy of the rows with the
+        Jacobian columns 13-16 recomputed."""
+        rows = np.array(rows18, dtype=np.float64).reshape(-1, 18)
+        cfg = make_newton(**newton)
+        self._check(self.lib.mpf_catalogue_previous(self.h, _p(rows), ctypes.c_uint64(rows.shape[0]),
+                                                    ctypes.byref(cfg)), "mpf_catalogue_previous")
+        return rows
 
     # learned first guess of the catalogue root-finder
     def mlp_init(self, seed=0):

@@ -159,6 +159,26 @@ def catalogue(tab):
             kw.update(stop_bundle=bundle[0], plane=bundle[1], opening=bundle[2])
         name = "prev_" + cat_name("strong", 0, jt, bundle)
         out[f"ref_{name}"] = RC.relcat_prev(h, tab, cases.LENGTH, cases.NSTEPS, prev, "/tmp/mpf_golden_relprev", **kw)
+    # the files relCat writes (catalogue, _err, _reject) for three runs, and
+    # those of relCat_with_previous_cat
+    import tempfile
+    tmp = tempfile.mkdtemp()
+    for run in (CAT_RUNS[0], CAT_RUNS[4], CAT_RUNS[13]):
+        tree, acc, jt, bundle = run[:4]
+        name = cat_name(tree, acc, jt, bundle)
+        src = out[f"src_{name}"]
+        t8 = np.concatenate([src, np.arange(len(src))[:, None] + 1000.0, np.full((len(src), 1), 3.0)], axis=1)
+        kw = dict(jacobiantype=jt)
+        if bundle is not None:
+            kw.update(stop_bundle=bundle[0], plane=bundle[1], opening=bundle[2])
+        base = os.path.join(tmp, name)
+        RC.relcat(handles[tree], tab, cases.LENGTH, cases.NSTEPS, t8, base, accuracy=acc, vobs=CAT_VOBS, **kw)
+        for suffix in ("", "_err", "_reject"):
+            out[f"file_{name}{suffix}"] = np.frombuffer(open(base + suffix + ".txt", "rb").read(), dtype=np.uint8)
+    base = os.path.join(tmp, "previous")
+    RC.relcat_prev(h, tab, cases.LENGTH, cases.NSTEPS, prev, base)
+    for suffix in ("", "_err"):
+        out[f"file_previous{suffix}"] = np.frombuffer(open(base + suffix + ".txt", "rb").read(), dtype=np.uint8)
     for hh in handles.values():
         RC.free(hh)
     np.savez_compressed(os.path.join(HERE, "catalogue.npz"), **out)

@@ -199,3 +199,36 @@ def test_narrow_cones_live_against_harness():
         np.testing.assert_array_equal(c, rc)
         np.testing.assert_array_equal(cr, rr)
         np.testing.assert_array_equal(rotm1, rm)
+
+
+# ------------------------------------------------------- catalogue files
+
+
+@pytest.mark.parametrize("idx", (0, 4, 13))
+def test_catalogue_files_match_reference(idx, tmp_path):
+    """mpf_catalogue_write on rows equal to the reference's (the oracle's,
+    pinned above) writes relCat's three files byte for byte."""
+    tree, acc, jt, bundle = make_golden.CAT_RUNS[idx][:4]
+    name = make_golden.cat_name(tree, acc, jt, bundle)
+    O, h, _ = _tree(tree)
+    src = CAT[f"src_{name}"]
+    rows, _ = O.catalogue(h, 2, cases.cosmology(), cases.LENGTH, cases.NSTEPS, src, accuracy=acc,
+                          vobs=tuple(CAT["vobs"]), **_oracle_kw(jt, bundle))
+    idn = np.stack([np.arange(len(src)) + 1000.0, np.full(len(src), 3.0)], axis=1)
+    capi.catalogue_write(str(tmp_path / "cat"), rows, idn)
+    for suffix in ("", "_err", "_reject"):
+        got = (tmp_path / f"cat{suffix}.txt").read_bytes()
+        assert got == CAT[f"file_{name}{suffix}"].tobytes(), suffix
+
+
+def test_previous_catalogue_files_match_reference(tmp_path):
+    ref = CAT["ref_prev_" + make_golden.cat_name("strong", 0, "infinitesimal", None)]
+    bad = ref.copy()
+    bad[::7, 13:15] = 42          # rows routed to the error file
+    capi.catalogue_write_previous(str(tmp_path / "prev"), ref)
+    assert (tmp_path / "prev.txt").read_bytes() == CAT["file_previous"].tobytes()
+    assert (tmp_path / "prev_err.txt").read_bytes() == CAT["file_previous_err"].tobytes()
+    capi.catalogue_write_previous(str(tmp_path / "prev2"), bad)
+    main = (tmp_path / "prev2.txt").read_text().splitlines()
+    err = (tmp_path / "prev2_err.txt").read_text().splitlines()
+    assert len(err) == len(range(0, len(ref), 7)) and len(main) + len(err) == len(ref)
