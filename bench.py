@@ -300,6 +300,8 @@ def main():
     ap.add_argument("--launch-config", default=None, metavar="R,B,C,W",
                     help="use this launch configuration (refill, sort bucket, blocks per CU, waves per SIMD) for "
                          "every workload instead of the autotuner's (profiling runs)")
+    ap.add_argument("--tier", choices=("split", "fp64", "ref"), default="split",
+                    help="precision tier of the timed passes (profiling runs of the exact tiers)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="only run the plain copy kernel over MB megabytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -324,8 +326,10 @@ def main():
 
     w, rec, f64, length, table, in9 = build_workload(args.workload)
     order = w["order"]
-    ctx = capi.Context(length, order=order, precision=capi.PREC_SPLIT if order == 2 else capi.PREC_FP64,
-                       nsteps=10, device=0)
+    tier = {"split": capi.PREC_SPLIT, "fp64": capi.PREC_FP64, "ref": capi.PREC_REF}[args.tier]
+    if order != 2 and tier == capi.PREC_SPLIT:
+        tier = capi.PREC_FP64
+    ctx = capi.Context(length, order=order, precision=tier, nsteps=10, device=0)
     t0 = time.time()
     ctx.load_records(rec)
     ctx.load_fields64(f64)

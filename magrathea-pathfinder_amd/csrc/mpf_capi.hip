@@ -668,8 +668,11 @@ static int integrate_impl(mpf_ctx* c, const double* dev_in9, uint64_t n, const m
                                                   L.waves_per_simd ? L.waves_per_simd : 2, c->stream, c->evk0,
                                                   c->evk1));
         } else {
+            // plain integration runs the lean kernel, two 256-lane blocks per CU (its TSC caches
+            // take 55 KB of LDS per block); the kernel with surfaces or trajectories four
             const uint64_t blocks = (n + 255) / 256;
-            const int grid = (int)std::min<uint64_t>(blocks, (uint64_t)c->cus * (L.blocks_per_cu ? L.blocks_per_cu : 4));
+            const unsigned dflt = (A.nsurf == 0 && !A.traj) ? 2u : 4u;
+            const int grid = (int)std::min<uint64_t>(blocks, (uint64_t)c->cus * (L.blocks_per_cu ? L.blocks_per_cu : dflt));
             HIPCHK(c, hipEventRecord(c->evk0, c->stream));
             HIPCHK(c, mpf::launch_integrate_exact(c->cfg.precision, A, c->dt, grid, c->stream));
             HIPCHK(c, hipEventRecord(c->evk1, c->stream));

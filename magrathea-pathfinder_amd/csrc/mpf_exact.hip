@@ -410,6 +410,462 @@ __global__ void __launch_bounds__(256) integrate_exact_kernel(IntegrateArgs A, D
     }
 }
 
+// ---- the lean integrator: plain integration (no surfaces, no stored
+// trajectory), the path of mpf_integrate[_dev].  Only the recurrence of the
+// trajectory stays in registers; the record of the last element is derived
+// from it when the ray ends.  The final pop (integrator.h:816-821) needs the
+// element before the last one only when the last one has a negative
+// redshift or scale factor: that element is written to the ray's output slot
+// at the step that appends such an element, and the last one overwrites it
+// at the end when it is compatible.  The four interpolations of a step run
+// in one loop (one copy of the TSC code), and the 27 cell indices of the TSC
+// cache live in LDS (two sets per lane: a refill that fails keeps the old
+// cache, as the reference's does).  Every expression is ray_step's.
+
+constexpr int LEAN_BLOCK = 256;
+
+// TscCache with its 27 cell indices in LDS: set `cur` of this lane is
+// g[(cur * 27 + i) * LEAN_BLOCK]
+struct LeanCache {
+    unsigned lvl;
+    uint32_t cx[3], cy[3], cz[3];
+    bool dflt;
+    bool leaf;      // the central cell (13) stores no children
+    unsigned cur;
+    uint32_t* g;
+};
+
+__device__ __forceinline__ void lean_cache_init(LeanCache& c) {
+    c.dflt = true;
+    c.leaf = false;
+    c.lvl = 0;
+    c.cur = 0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) { c.cx[i] = 0; c.cy[i] = 0; c.cz[i] = 0; }
+}
+
+template <typename T>
+__device__ __forceinline__ T pick3(const T* a, int i) { return (i == 0) ? a[0] : ((i == 1) ? a[1] : a[2]); }
+
+// the fields a stage of the RK4 step reads (Integrator::dphotondl,
+// integrator.h:409-445): grad phi and dphi/dt; each field is summed on its
+// own (mac(), simplehyperoctree.h:1870-1910), so leaving the others out
+// changes none of these
+__device__ __forceinline__ bool stage_field(int k) { return k == F_DX || k == F_DY || k == F_DZ || k == F_DT; }
+
+template <int P, int NF>
+__device__ __forceinline__ void loadf_sel(const DevTree& t, uint32_t g, double* d, bool full) {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        if (full || stage_field(k)) {
+            if (P == 1 && t.f64) d[k] = t.f64[(uint64_t)k * t.n + g];
+            else d[k] = (double)t.f32[(uint64_t)k * t.n + g];
+        }
+    }
+    if (NF == 10 && full) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) d[7 + k] = t.vel ? (double)t.vel[(uint64_t)k * t.n + g] : 0.0;
+    }
+}
+
+template <int P, int NF>
+__device__ __forceinline__ void macf_sel(double* acc, const double* f, double w, bool full) {
+#pragma unroll
+    for (int k = 0; k < NF; ++k) {
+        if (full || stage_field(k)) {
+            if (P == 0) acc[k] = (double)(float)(acc[k] + f[k] * w);
+            else acc[k] = acc[k] + f[k] * w;
+        }
+    }
+}
+
+// exact::tsc<P, true, NF> with the cache above (simplehyperoctree.h:1554-1644)
+// full = false: only the fields stage_field names are summed (the others
+// keep what they held)
+template <int P, int NF>
+__device__ void tsc_lean(const DevTree& t, LeanCache& cache, double x, double y, double z, double* result, int* loc,
+                         bool full) {
+    const double two = 2.0;
+    const double point[3] = {x, y, z};
+    if (point[0] == 0.0 && point[1] == 0.0 && point[2] == 0.0) {
+        cic<P, NF>(t, 0., 0., 0., result, loc);
+        return;
+    }
+    uint32_t ek[3] = {axis_coord(x, 32), axis_coord(y, 32), axis_coord(z, 32)};
+    unsigned fl = 0;
+    uint32_t g0;
+    // a point inside the cached central cell, a leaf: that cell is the finest
+    // stored one containing it (what the descent from the base returns)
+    if (!cache.dflt && cache.leaf && (ek[0] >> (32 - cache.lvl)) == cache.cx[1] &&
+        (ek[1] >> (32 - cache.lvl)) == cache.cy[1] && (ek[2] >> (32 - cache.lvl)) == cache.cz[1]) {
+        g0 = cache.g[((uint64_t)cache.cur * 27 + 13) * LEAN_BLOCK];
+        fl = cache.lvl;
+    } else {
+        g0 = find_cell(t, 32, ek[0], ek[1], ek[2], &fl);
+    }
+    *loc = (g0 == NIL) ? -1 : (int)fl;
+    bool invalid = (g0 == NIL);
+    unsigned idl = invalid ? 32u : fl;
+    uint32_t idk[3] = {ek[0] >> (32 - idl), ek[1] >> (32 - idl), ek[2] >> (32 - idl)};
+    unsigned ilvl = idl, lvl = idl;
+    double half = EXTENT / 1.0 * (1.0 / (1.0 * (double)(1ull << idl))) / two;
+    double cell[NF];
+    zerof<NF>(result);
+    do {
+        ilvl = lvl;
+        const double twohalves = two * half;
+        bool hit;
+        if (cache.dflt) hit = !invalid && idl == 0;  // 27 default elements hold the root key
+        else hit = !invalid && cache.lvl == idl && cache.cx[1] == idk[0] && cache.cy[1] == idk[1] &&
+                   cache.cz[1] == idk[2];
+        if (hit) {
+            if (cache.dflt) {  // default elements carry Data(): the sum stays zero
+                zerof<NF>(result);
+                return;
+            }
+            double wx[3], wy[3], wz[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const int a = (i == 1) ? 0 : 1;
+                wx[i] = tsc_axis_w(a, fabs((cell_center(idl, cache.cx[i]) - point[0]) / (twohalves)));
+                wy[i] = tsc_axis_w(a, fabs((cell_center(idl, cache.cy[i]) - point[1]) / (twohalves)));
+                wz[i] = tsc_axis_w(a, fabs((cell_center(idl, cache.cz[i]) - point[2]) / (twohalves)));
+            }
+            const uint32_t* g = cache.g + (uint64_t)cache.cur * 27 * LEAN_BLOCK;
+            for (int ic = 0; ic < 27; ++ic) {
+                const int ix = ic / 9, iy = (ic / 3) % 3, iz = ic % 3;
+                const double w = pick3(wx, ix) * pick3(wy, iy) * pick3(wz, iz);
+                loadf_sel<P, NF>(t, g[ic * LEAN_BLOCK], cell, full);
+                macf_sel<P, NF>(result, cell, w, full);
+            }
+            return;
+        }
+        if (!invalid) {
+            uint32_t nx[3], ny[3], nz[3];
+            double wx[3], wy[3], wz[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const int a = (i == 1) ? 0 : 1;
+                nx[i] = axis_coord(point[0] + (i - 1) * twohalves, ilvl);
+                ny[i] = axis_coord(point[1] + (i - 1) * twohalves, ilvl);
+                nz[i] = axis_coord(point[2] + (i - 1) * twohalves, ilvl);
+                wx[i] = tsc_axis_w(a, fabs((cell_center(ilvl, nx[i]) - point[0]) / (twohalves)));
+                wy[i] = tsc_axis_w(a, fabs((cell_center(ilvl, ny[i]) - point[1]) / (twohalves)));
+                wz[i] = tsc_axis_w(a, fabs((cell_center(ilvl, nz[i]) - point[2]) / (twohalves)));
+            }
+            // the other set receives the new cells; it becomes the cache only when all 27 are found
+            uint32_t* gn = cache.g + (uint64_t)(cache.cur ^ 1u) * 27 * LEAN_BLOCK;
+            int ic = 0;
+            for (; ic < 27; ++ic) {
+                const int ix = ic / 9, iy = (ic / 3) % 3, iz = ic % 3;
+                unsigned vl = 0;
+                const uint32_t gv = find_cell(t, ilvl, pick3(nx, ix), pick3(ny, iy), pick3(nz, iz), &vl);
+                if (gv == NIL) {
+                    zerof<NF>(result);
+                    break;
+                }
+                lvl = lvl < vl ? lvl : vl;
+                if (lvl < ilvl) {
+                    zerof<NF>(result);
+                    break;
+                }
+                const double w = pick3(wx, ix) * pick3(wy, iy) * pick3(wz, iz);
+                loadf_sel<P, NF>(t, gv, cell, full);
+                macf_sel<P, NF>(result, cell, w, full);
+                gn[ic * LEAN_BLOCK] = gv;
+            }
+            if (ic == 27) {
+                cache.dflt = false;
+                cache.lvl = ilvl;
+                cache.cur ^= 1u;
+                cache.leaf = t.child_mask[gn[13 * LEAN_BLOCK]] == 0;
+#pragma unroll
+                for (int i = 0; i < 3; ++i) { cache.cx[i] = nx[i]; cache.cy[i] = ny[i]; cache.cz[i] = nz[i]; }
+            }
+        }
+        if (lvl < ilvl) {
+            const unsigned el = lvl + 1;
+            idl = lvl;
+            idk[0] = axis_coord(x, el) >> 1;
+            idk[1] = axis_coord(y, el) >> 1;
+            idk[2] = axis_coord(z, el) >> 1;
+            invalid = false;
+            zerof<NF>(result);
+            half = EXTENT / 1.0 * (1.0 / (1.0 * (double)(1ull << idl))) / two;
+        }
+    } while (lvl < ilvl);
+}
+
+template <int P, int NF>
+__device__ __forceinline__ void interp_lean(int order, const DevTree& t, LeanCache& cache, double x, double y,
+                                            double z, double* d, int* loc, bool full = true) {
+    if (order == 0) ngp<P, NF>(t, x, y, z, d, loc);
+    else if (order == 1) cic<P, NF>(t, x, y, z, d, loc);
+    else tsc_lean<P, NF>(t, cache, x, y, z, d, loc, full);
+}
+
+template <int NF>
+struct LeanState {
+    double c[9];   // a, t, x, y, z, kt, kx, ky, kz of the last element
+    double d[NF];  // the fields interpolated there
+    double lambda, s, isw, iswold, laplacian, redshift;
+    double dl, a0, phi0c2, v0n, stopv;
+    int level, hint;
+    uint32_t appended;
+};
+
+// the MPF_NOUT record of the last element (ray_launch and ray_step write the
+// same values)
+template <int NF>
+__device__ void lean_record(const LeanState<NF>& st, double* o, uint32_t nsteps) {
+    const double zero = 0.0, one = 1.0, two = 2.0;
+    const double c2 = C2;
+    const double* c = st.c;
+    const double* d = st.d;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) o[k] = c[k];
+    const double phi = d[F_PHI];
+    o[MPF_OUT_LEVEL] = (double)st.level;
+    o[MPF_OUT_RHO] = d[F_RHO];
+    o[MPF_OUT_PHI] = phi;
+    o[MPF_OUT_DPHIDX] = d[F_DX];
+    o[MPF_OUT_DPHIDY] = d[F_DY];
+    o[MPF_OUT_DPHIDZ] = d[F_DZ];
+    o[MPF_OUT_ISW] = st.isw;
+    o[MPF_OUT_ISWOLD] = st.iswold;
+    o[MPF_OUT_LAMBDA] = st.lambda;
+    o[MPF_OUT_S] = st.s;
+    o[MPF_OUT_REDSHIFT] = st.redshift;
+    o[MPF_OUT_LAPLACIAN] = st.laplacian;
+    o[MPF_OUT_ERROR] = one - ((one - two * (phi / c2)) * (c[CKX] * c[CKX] + c[CKY] * c[CKY] + c[CKZ] * c[CKZ])) /
+                                 (c2 * (one + two * (phi / c2)) * c[CKT] * c[CKT]);
+    if (st.appended == 0) {  // the launch element (ray_launch)
+        o[MPF_OUT_AH] = zero;
+        o[MPF_OUT_DPHIDT] = d[F_DT];
+        o[MPF_OUT_DPHIDL] = zero;
+        o[MPF_OUT_CHI] = zero;
+        o[MPF_OUT_DSDL2] = (c[CA] * c[CA]) * (-(c2 * (one + two * (phi / c2)) * c[CKT] * c[CKT]) +
+                                              ((one - two * (phi / c2)) * (c[CKX] * c[CKX] + c[CKY] * c[CKY] + c[CKZ] * c[CKZ])));
+    } else {                 // an appended element (ray_step)
+        o[MPF_OUT_AH] = d[F_A];
+        o[MPF_OUT_DPHIDT] = -d[F_DT];
+        o[MPF_OUT_DPHIDL] = d[F_DT] * c[CKT] + (d[F_DX] * c[CKX] + d[F_DY] * c[CKY] + d[F_DZ] * c[CKZ]);
+        o[MPF_OUT_CHI] = sqrt(c[CX] * c[CX] + c[CY] * c[CY] + c[CZ] * c[CZ]);
+        o[MPF_OUT_DSDL2] = (c[CA] * c[CA]) * ((-c2 * (one + two * (phi / c2)) * c[CKT] * c[CKT]) +
+                                              ((one - two * (phi / c2)) * (c[CKX] * c[CKX] + c[CKY] * c[CKY] + c[CKZ] * c[CKZ])));
+    }
+    o[MPF_OUT_NSTEPS] = (double)nsteps;
+}
+
+template <int P, int NF>
+__device__ void lean_launch(const IntegrateArgs& A, const DevTree& t, LeanCache& cache, const double* in9,
+                            LeanState<NF>& st) {
+    const double zero = 0.0, one = 1.0, two = 2.0;
+    const double c = C_LIGHT, c2 = C2;
+    double* b = st.c;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) b[k] = in9[k];
+    lean_cache_init(cache);
+    st.hint = 0;
+    int loc = -1;
+    if (A.order >= 0) interp_lean<P, NF>(A.order, t, cache, b[CX], b[CY], b[CZ], st.d, &loc);
+    else { zerof<NF>(st.d); st.d[F_A] = one; }
+    const double* d = st.d;
+    const double norm = sqrt((c2 * (one + two * (d[F_PHI] / c2)) * b[CKT] * b[CKT]) /
+                             ((one - two * (d[F_PHI] / c2)) * (b[CKX] * b[CKX] + b[CKY] * b[CKY] + b[CKZ] * b[CKZ])));
+    b[CKX] *= norm;
+    b[CKY] *= norm;
+    b[CKZ] *= norm;
+    b[CA] = cosmo_interp(b[CT], A.ct, A.ca, A.ncos, st.hint);
+    unsigned fl = 0;
+    const uint32_t gl = locate_point(t, b[CX], b[CY], b[CZ], &fl);
+    const int lvl0 = (gl == NIL) ? -1 : (int)fl;
+    st.level = lvl0 < 0 ? 0 : lvl0;
+    st.lambda = zero; st.s = zero; st.isw = zero; st.iswold = zero; st.laplacian = zero; st.redshift = zero;
+    const double ratio = b[CA] * b[CA] * (A.length / c) / A.nsteps;
+    st.dl = level_extent(lvl0) * ratio;
+    st.a0 = b[CA];
+    st.phi0c2 = d[F_PHI] / c2;
+    st.v0n = (NF == 10) ? (A.vobs[0] * b[CKX] + A.vobs[1] * b[CKY] + A.vobs[2] * b[CKZ]) / (c * b[CKT]) : 0.0;
+    st.appended = 0;
+    st.stopv = A.stop_value;
+}
+
+// one iteration of the integration loop (ray_step without surfaces or a
+// stored trajectory); true while the ray continues
+template <int P, int NF>
+__device__ bool lean_step(const IntegrateArgs& A, const DevTree& t, LeanCache& cache, LeanState<NF>& st,
+                          uint64_t ray) {
+    const double zero = 0.0, one = 1.0, two = 2.0, six = 6.0;
+    const double c = C_LIGHT, c2 = C2;
+    const double min = 0.0 - (EXTENT / two), max = 0.0 + (EXTENT / two);
+    const double scale = A.length;
+    const double* bk = st.c;
+    const double dl = st.dl;
+    double kin[9], ksum[9], kout[9], data[NF];
+    int loc = -1;
+    // k1 at the last element: its fields are the ones interpolated when it
+    // was appended (the reference recomputes them through its cache, with the
+    // same result)
+    if (A.order >= 0) {
+#pragma unroll
+        for (int k = 0; k < NF; ++k) data[k] = st.d[k];
+    } else zerof<NF>(data);
+    double dadt = cosmo_interp(bk[CT], A.ct, A.cdadt, A.ncos, st.hint);
+    rhs(bk, data, dadt, scale, kout);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) { ksum[i] = kout[i]; kin[i] = bk[i] + dl / two * kout[i]; }
+    // stages 1-3 interpolate at the k2, k4 and k4 points and advance the RK4
+    // sums; stage 4 interpolates at the new point (kin then holds it)
+    for (int stg = 1; stg <= 4; ++stg) {
+        if (A.order >= 0) interp_lean<P, NF>(A.order, t, cache, kin[CX], kin[CY], kin[CZ], data, &loc, stg == 4);
+        else {
+            zerof<NF>(data);
+            if (stg == 4) {
+                data[F_A] = one;
+                unsigned fl = 0;
+                const uint32_t g = locate_point(t, kin[CX], kin[CY], kin[CZ], &fl);
+                loc = (g == NIL) ? -1 : (int)fl;
+            }
+        }
+        if (stg == 4) break;
+        dadt = cosmo_interp(kin[CT], A.ct, A.cdadt, A.ncos, st.hint);
+        rhs(kin, data, dadt, scale, kout);
+        if (stg == 1) {
+#pragma unroll
+            for (int i = 0; i < 9; ++i) { ksum[i] = ksum[i] + two * kout[i]; kin[i] = bk[i] + dl / two * kout[i]; }
+        } else if (stg == 2) {
+#pragma unroll
+            for (int i = 0; i < 9; ++i) { ksum[i] = ksum[i] + two * kout[i]; kin[i] = bk[i] + dl * kout[i]; }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 9; ++i) kin[i] = bk[i] + (dl / six) * (ksum[i] + kout[i]);
+        }
+    }
+    const double* p = kin;
+    if (!(!(p[CA] < zero) && ((p[CX] > min) && (p[CX] < max) && (p[CY] > min) && (p[CY] < max) && (p[CZ] > min) &&
+                              (p[CZ] < max))))
+        zerof<NF>(data);
+    if (emptyf<NF>(data)) return false;   // not appended: the ray ends on the last element
+    // the new element (ray_step's expressions)
+    const double* od = st.d;
+    const double bk_dphidt = (st.appended == 0) ? od[F_DT] : -od[F_DT];
+    const double p_dphidt = -data[F_DT];
+    const double laplacian = (data[F_PHI] - od[F_PHI]) / dl;
+    const double isw = st.isw + (bk_dphidt + p_dphidt) * (p[CT] - bk[CT]) / c2;
+    const double iswold = st.iswold - (2 * (data[F_PHI] - od[F_PHI]) / c2 -
+                                       ((od[F_DX] + data[F_DX]) * (p[CX] - bk[CX]) * scale +
+                                        (od[F_DY] + data[F_DY]) * (p[CY] - bk[CY]) * scale +
+                                        (od[F_DZ] + data[F_DZ]) * (p[CZ] - bk[CZ]) * scale) /
+                                           c2);
+    const double chi = sqrt(p[CX] * p[CX] + p[CY] * p[CY] + p[CZ] * p[CZ]);
+    const double lambda = st.lambda + dl;
+    const double sv = st.s + sqrt((p[CX] - bk[CX]) * (p[CX] - bk[CX]) + (p[CY] - bk[CY]) * (p[CY] - bk[CY]) +
+                                  (p[CZ] - bk[CZ]) * (p[CZ] - bk[CZ]));
+    double redshift;
+    if (NF == 10)
+        redshift = st.a0 / p[CA] *
+                       (one + st.phi0c2 - data[F_PHI] / c2 + isw +
+                        (data[F_VX] * p[CKX] + data[F_VY] * p[CKY] + data[F_VZ] * p[CKZ]) / (c * p[CKT]) - st.v0n) -
+                   one;
+    else redshift = st.a0 / p[CA] * (one + st.phi0c2 - data[F_PHI] / c2) - one;
+    if (signbit(redshift) || signbit(p[CA])) {
+        // the final pop would erase this element: keep the current last one
+        lean_record<NF>(st, A.out + (uint64_t)MPF_NOUT * ray, st.appended);
+    }
+    const double ratio = p[CA] * p[CA] * (scale / c) / A.nsteps;
+    st.dl = level_extent(loc) * ratio;
+    const double pkx = p[CX], pky = p[CY], pkz = p[CZ], pt = p[CT], pa = p[CA];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) st.c[i] = p[i];
+#pragma unroll
+    for (int k = 0; k < NF; ++k) st.d[k] = data[k];
+    st.level = loc < 0 ? 0 : loc;
+    st.laplacian = laplacian;
+    st.isw = isw;
+    st.iswold = iswold;
+    st.lambda = lambda;
+    st.s = sv;
+    st.redshift = redshift;
+    st.appended++;
+    if (A.stop_kind >= 0) {
+        bool brk;
+        switch (A.stop_kind) {
+            case 0: brk = chi > st.stopv; break;
+            case 1: brk = lambda > st.stopv; break;
+            case 2: brk = redshift > st.stopv; break;
+            case 3: brk = pt > st.stopv; break;
+            case 4: brk = pa < st.stopv; break;
+            case 5: brk = (pkx * A.kit[0] + pky * A.kit[1] + pkz * A.kit[2]) > st.stopv; break;
+            case 6: brk = sv > st.stopv; break;
+            default: brk = true; break;
+        }
+        if (brk) return false;
+    }
+    return true;
+}
+
+template <int NF>
+__device__ void lean_finish(const IntegrateArgs& A, const LeanState<NF>& st, uint64_t ray) {
+    double* o = A.out + (uint64_t)MPF_NOUT * ray;
+    if (signbit(st.redshift) || signbit(st.c[CA])) {
+        if (st.appended == 0) {  // the trajectory becomes empty
+            for (int k = 0; k < MPF_NOUT; ++k) o[k] = 0.0;
+            o[MPF_OUT_NSTEPS] = -1.0;
+            return;
+        }
+        // the element before the last was written when the last was appended
+        atomicAdd(A.steps, (unsigned long long)(st.appended - 1));
+        return;
+    }
+    lean_record<NF>(st, o, st.appended);
+    atomicAdd(A.steps, (unsigned long long)st.appended);
+}
+
+template <int P, int NF>
+__global__ void __launch_bounds__(LEAN_BLOCK, 2) integrate_lean_kernel(IntegrateArgs A, DevTree t) {
+    __shared__ uint32_t cache_g[2 * 27 * LEAN_BLOCK];
+    LeanState<NF> st;
+    LeanCache cache;
+    cache.g = cache_g + threadIdx.x;
+    lean_cache_init(cache);
+    bool active = false, drained = false;
+    uint64_t ray = 0;
+    const unsigned lane = __lane_id();
+    while (true) {
+        const bool need = !active && !drained;
+        const unsigned long long m = __ballot(need);
+        if (m && ((unsigned)__popcll(m) >= A.refill || !__any(active))) {
+            const int leader = __ffsll((long long)m) - 1;
+            unsigned long long base = 0;
+            if ((int)lane == leader) base = atomicAdd(A.queue, (unsigned long long)__popcll(m));
+            base = __shfl(base, leader);
+            if (need) {
+                const uint64_t my = base + (uint64_t)__popcll(m & ((1ull << lane) - 1ull));
+                if (my < A.n) {
+                    ray = A.queue_order ? (uint64_t)A.queue_order[my] : my;
+                    lean_launch<P, NF>(A, t, cache, A.in9 + 9 * ray, st);
+                    if (A.ray_value) st.stopv = A.ray_value[ray];
+                    active = !emptyf<NF>(st.d);
+                    if (!active) lean_finish<NF>(A, st, ray);
+                } else {
+                    drained = true;
+                }
+            }
+        }
+        if (!__any(active)) {
+            if (__all(drained)) break;
+            continue;
+        }
+        if (active) {
+            if (!lean_step<P, NF>(A, t, cache, st, ray)) {
+                active = false;
+                lean_finish<NF>(A, st, ray);
+            }
+        }
+    }
+}
+
 template <int P>
 __global__ void locate_kernel(DevTree t, const double* xyz, uint64_t n, int64_t* elem, uint32_t* level) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -480,6 +936,17 @@ hipError_t launch_surface_final(const IntegrateArgs& A, hipStream_t st) {
 }
 
 hipError_t launch_integrate_exact(int precision, const IntegrateArgs& A, const DevTree& t, int grid, hipStream_t st) {
+    if (A.nsurf == 0 && A.traj == nullptr) {  // plain integration: the lean kernel
+        if (t.vel) {
+            if (precision == 0) hipLaunchKernelGGL((integrate_lean_kernel<0, 10>), dim3(grid), dim3(LEAN_BLOCK), 0, st, A, t);
+            else hipLaunchKernelGGL((integrate_lean_kernel<1, 10>), dim3(grid), dim3(LEAN_BLOCK), 0, st, A, t);
+        } else if (precision == 0) {
+            hipLaunchKernelGGL((integrate_lean_kernel<0, 7>), dim3(grid), dim3(LEAN_BLOCK), 0, st, A, t);
+        } else {
+            hipLaunchKernelGGL((integrate_lean_kernel<1, 7>), dim3(grid), dim3(LEAN_BLOCK), 0, st, A, t);
+        }
+        return hipGetLastError();
+    }
     // a tree with a velocity field: 10 interpolated fields and the Doppler redshift
     if (t.vel) {
         if (precision == 0) hipLaunchKernelGGL((integrate_exact_kernel<0, 10>), dim3(grid), dim3(256), 0, st, A, t);
