@@ -142,6 +142,10 @@ struct mpf_ctx {
     mpf::DevTree dt;
     mpf::SplitTree split;
     bool split_ready = false;
+    // cell-major field copies of the lean exact integrator, built at its first TSC run
+    DevBuf<float> q32;
+    DevBuf<double> q64;
+    bool quad_ready = false;
     // velocity field: FP64 sums (n x 4) between begin and finish, the field
     // (3 x n) the tree carries after it, particle staging, parent work array
     DevBuf<double> vacc;
@@ -237,6 +241,7 @@ int mpf_destroy(mpf_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->base.release(); c->child_first.release(); c->elem.release(); c->child_mask.release();
     c->f32.release(); c->f64.release(); c->cosmo.release(); c->counters.release();
+    c->q32.release(); c->q64.release();
     c->io_in.release(); c->io_out.release(); c->io_surf.release(); c->io_rv.release(); c->scratch.release();
     c->order.release(); c->order_work.release(); c->homo.release();
     c->map_acc.release(); c->io_i64.release(); c->io_u32.release();
@@ -271,6 +276,7 @@ int mpf_load_octree_records(mpf_ctx* c, const void* rec48, uint64_t n) {
     mpf::unpack_fields32(rec48, t, soa);
     c->have_tree = false;
     c->split_ready = false;
+    c->quad_ready = false;
     c->f64.release();
     if (c->base.ensure(t.base.size()) != hipSuccess || c->child_first.ensure(n) != hipSuccess ||
         c->elem.ensure(n) != hipSuccess || c->child_mask.ensure(n) != hipSuccess || c->f32.ensure(7 * n) != hipSuccess)
@@ -322,6 +328,7 @@ int mpf_load_octree_fields64(mpf_ctx* c, const double* f7, uint64_t n) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->dt.f64 = c->f64.p;
     c->split_ready = false;
+    c->quad_ready = false;
     return 0;
 }
 
@@ -416,7 +423,8 @@ int mpf_velocity_finish(mpf_ctx* c) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->dt.vel = c->vel.p;
     c->vel_open = false;
-    c->split_ready = false;  // rho changed
+    c->split_ready = false;
+    c->quad_ready = false;  // rho changed
     return 0;
 }
 
@@ -441,6 +449,7 @@ int mpf_velocity_set(mpf_ctx* c, const float* rho, const float* vel3, uint64_t n
         if (c->dt.f64)
             HIPCHK(c, hipMemcpyAsync(c->f64.p, r64.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
         c->split_ready = false;
+    c->quad_ready = false;
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->dt.vel = c->vel.p;
@@ -671,7 +680,22 @@ static int integrate_impl(mpf_ctx* c, const double* dev_in9, uint64_t n, const m
             // plain integration runs the lean kernel, two 256-lane blocks per CU (its TSC caches
             // take 55 KB of LDS per block); the kernel with surfaces or trajectories four
             const uint64_t blocks = (n + 255) / 256;
-            const unsigned dflt = (A.nsurf == 0 && !A.traj) ? 2u : 4u;
+            const bool lean = A.nsurf == 0 && !A.traj;
+            if (lean && c->cfg.order == MPF_TSC && !c->quad_ready) {
+                // the lean integrator's TSC reads each cell's fields in 16-byte loads
+                c->dt.q32 = nullptr;
+                c->dt.q64 = nullptr;
+                if (c->q32.ensure(8 * c->ncells) != hipSuccess ||
+                    (c->dt.f64 && c->q64.ensure(8 * c->ncells) != hipSuccess))
+                    return fail(c, ERR_NOMEM, "cell-major field copies");
+                HIPCHK(c, mpf::launch_field_quads(c->dt, c->q32.p, c->dt.f64 ? c->q64.p : nullptr, c->stream));
+                c->dt.q32 = reinterpret_cast<const float4*>(c->q32.p);
+                c->dt.q64 = c->dt.f64 ? reinterpret_cast<const double2*>(c->q64.p) : nullptr;
+                c->quad_ready = true;
+                HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+            }
+            if (!c->quad_ready) { c->dt.q32 = nullptr; c->dt.q64 = nullptr; }
+            const unsigned dflt = lean ? 2u : 4u;
             const int grid = (int)std::min<uint64_t>(blocks, (uint64_t)c->cus * (L.blocks_per_cu ? L.blocks_per_cu : dflt));
             HIPCHK(c, hipEventRecord(c->evk0, c->stream));
             HIPCHK(c, mpf::launch_integrate_exact(c->cfg.precision, A, c->dt, grid, c->stream));

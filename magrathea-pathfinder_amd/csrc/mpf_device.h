@@ -25,6 +25,11 @@ struct DevTree {
     const float* f32 = nullptr;                      // 7 x n, structure of arrays
     const double* f64 = nullptr;                     // 7 x n or null
     const float* vel = nullptr;                      // 3 x n cell velocity / c, or null
+    // cell-major copies of the fields for the lean integrator's TSC, or null:
+    // q32[2 g] = (dphidx, dphidy, dphidz, dphidt), q32[2 g + 1] = (rho, phi, a, 0);
+    // q64[4 g .. 4 g + 3] = (dphidx, dphidy), (dphidz, dphidt), (rho, phi), (a, 0)
+    const float4* q32 = nullptr;
+    const double2* q64 = nullptr;
     uint64_t n = 0;
 };
 

@@ -453,13 +453,26 @@ __device__ __forceinline__ T pick3(const T* a, int i) { return (i == 0) ? a[0] :
 // changes none of these
 __device__ __forceinline__ bool stage_field(int k) { return k == F_DX || k == F_DY || k == F_DZ || k == F_DT; }
 
+// The lean TSC reads the cell-major copies (integrate_impl builds them
+// before every lean TSC launch): FP64 ones when the tier is FP64 and the tree
+// has FP64 fields, else FP32 ones.
 template <int P, int NF>
 __device__ __forceinline__ void loadf_sel(const DevTree& t, uint32_t g, double* d, bool full) {
-#pragma unroll
-    for (int k = 0; k < 7; ++k) {
-        if (full || stage_field(k)) {
-            if (P == 1 && t.f64) d[k] = t.f64[(uint64_t)k * t.n + g];
-            else d[k] = (double)t.f32[(uint64_t)k * t.n + g];
+    if (P == 1 && t.q64) {
+        const double2* q = t.q64 + 4 * (uint64_t)g;
+        const double2 a = q[0], b = q[1];
+        d[F_DX] = a.x; d[F_DY] = a.y; d[F_DZ] = b.x; d[F_DT] = b.y;
+        if (full) {
+            const double2 e = q[2], f = q[3];
+            d[F_RHO] = e.x; d[F_PHI] = e.y; d[F_A] = f.x;
+        }
+    } else {
+        const float4* q = t.q32 + 2 * (uint64_t)g;
+        const float4 a = q[0];
+        d[F_DX] = (double)a.x; d[F_DY] = (double)a.y; d[F_DZ] = (double)a.z; d[F_DT] = (double)a.w;
+        if (full) {
+            const float4 b = q[1];
+            d[F_RHO] = (double)b.x; d[F_PHI] = (double)b.y; d[F_A] = (double)b.z;
         }
     }
     if (NF == 10 && full) {
@@ -532,6 +545,7 @@ __device__ void tsc_lean(const DevTree& t, LeanCache& cache, double x, double y,
                 wz[i] = tsc_axis_w(a, fabs((cell_center(idl, cache.cz[i]) - point[2]) / (twohalves)));
             }
             const uint32_t* g = cache.g + (uint64_t)cache.cur * 27 * LEAN_BLOCK;
+#pragma unroll 1  // one cell at a time: unrolled, the loads hoisted across cells spill registers
             for (int ic = 0; ic < 27; ++ic) {
                 const int ix = ic / 9, iy = (ic / 3) % 3, iz = ic % 3;
                 const double w = pick3(wx, ix) * pick3(wy, iy) * pick3(wz, iz);
@@ -556,6 +570,7 @@ __device__ void tsc_lean(const DevTree& t, LeanCache& cache, double x, double y,
             // the other set receives the new cells; it becomes the cache only when all 27 are found
             uint32_t* gn = cache.g + (uint64_t)(cache.cur ^ 1u) * 27 * LEAN_BLOCK;
             int ic = 0;
+#pragma unroll 1  // one cell at a time: unrolled, the loads hoisted across cells spill registers
             for (; ic < 27; ++ic) {
                 const int ix = ic / 9, iy = (ic / 3) % 3, iz = ic % 3;
                 unsigned vl = 0;
@@ -906,6 +921,21 @@ __global__ void interp_seq_kernel(DevTree t, const double* xyz, uint64_t n, doub
     }
 }
 
+__global__ void field_quads_kernel(DevTree t, float* __restrict__ q32, double* __restrict__ q64) {
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= t.n) return;
+    const float* f = t.f32;
+    float* o = q32 + 8 * g;
+    o[0] = f[F_DX * t.n + g]; o[1] = f[F_DY * t.n + g]; o[2] = f[F_DZ * t.n + g]; o[3] = f[F_DT * t.n + g];
+    o[4] = f[F_RHO * t.n + g]; o[5] = f[F_PHI * t.n + g]; o[6] = f[F_A * t.n + g]; o[7] = 0.0f;
+    if (q64) {
+        const double* d = t.f64;
+        double* p = q64 + 8 * g;
+        p[0] = d[F_DX * t.n + g]; p[1] = d[F_DY * t.n + g]; p[2] = d[F_DZ * t.n + g]; p[3] = d[F_DT * t.n + g];
+        p[4] = d[F_RHO * t.n + g]; p[5] = d[F_PHI * t.n + g]; p[6] = d[F_A * t.n + g]; p[7] = 0.0;
+    }
+}
+
 // a_c = S1 - S2 / D at each valid surface, then A = 1 - 2 a L / c^2 (lensing.h:470-477)
 __global__ void surface_final_kernel(IntegrateArgs A) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -927,6 +957,12 @@ __global__ void surface_final_kernel(IntegrateArgs A) {
 }
 
 }  // namespace
+
+hipError_t launch_field_quads(const DevTree& t, float* q32, double* q64, hipStream_t st) {
+    if (t.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(field_quads_kernel, dim3((unsigned)((t.n + 255) / 256)), dim3(256), 0, st, t, q32, q64);
+    return hipGetLastError();
+}
 
 hipError_t launch_surface_final(const IntegrateArgs& A, hipStream_t st) {
     const uint64_t m = A.n * (uint64_t)A.nsurf;

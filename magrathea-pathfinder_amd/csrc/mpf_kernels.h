@@ -53,6 +53,8 @@ struct IntegrateArgs {
 };
 
 hipError_t launch_integrate_exact(int precision, const IntegrateArgs& A, const DevTree& t, int grid, hipStream_t st);
+// the cell-major copies DevTree::q32 / q64 from the structure-of-arrays fields (q64 when t.f64)
+hipError_t launch_field_quads(const DevTree& t, float* q32, double* q64, hipStream_t st);
 // velocity field (mpf_velocity.hip): one deposit pass at level lvl into the
 // n x 4 FP64 sums acc; then the normalisation and the parent fill, with
 // level_first[l - min_level] the first level-sorted cell of level l
