@@ -2,13 +2,18 @@
 
 Workload "amr256_tsc": a level-8 base grid (256^3) refined
 to level 11 wherever the lognormal overdensity exceeds 8, FP64 fields; 1024 x
-1024 rays on a regular grid over a 10 x 10 degree cone, TSC interpolation,
-integrated to redshift 2.  One bench step = one pass of every ray to its stop
-(a launch of the split-precision integrator over the whole batch), inputs
-resident in HBM.  The plain FP64 tier runs the same rays for the accuracy
-check (relative end-redshift error <= 1e-7) and its own throughput; the
-reference's CPU path (or the oracle when the reference is absent) is timed on
-a bounded subset of the rays.
+1024 rays on a regular grid over a 10 x 10 degree narrow field (the
+reference's narrow-cone shape, 99.9 deg^2), TSC interpolation, integrated to
+redshift 2.  One bench step = one pass of every ray to its stop (a launch of
+the split-precision integrator over the whole batch), inputs resident in HBM.
+The plain FP64 tier runs the same rays for the accuracy check (relative
+end-redshift error <= 1e-7) and its own throughput; the reference's CPU path
+(or the oracle when the reference is absent) is timed on a bounded subset of
+the rays.  The roofline is the VALU issue rate (the kernels are VALU-bound;
+their VALU instruction counts and HBM traffic come from the rocprofv3 counter
+summaries under profiles/).  Further records on the same octree: the uneven
+Jacobian/maps batch, the catalogue root-finder and, on its own 128^3 grid,
+the plain-method workload in full with the reference's CPU path on every ray.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload amr256_tsc]
 """
@@ -24,28 +29,88 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "magrathea-pathfinder_amd"))
 
 HBM_PEAK = 8.0e12       # MI355X HBM3E, bytes/s (AMD public spec)
+# VALU issue peak: 256 CUs x 2.4 GHz x one wave64 VALU instruction per CU per
+# cycle (4 SIMDs, each retiring a full-rate wave64 FP32 or FP64 instruction
+# every 4 cycles on 16 lanes).  x 64 lanes x 2 flop (FMA) = 78.6 TFLOP/s, the
+# published FP64 vector peak.  Packed FP32 is not generated (build flags).
+VALU_PEAK = 256 * 2.4e9
 WORKLOAD_TEXT = {
     "amr256_tsc": "amr256_tsc: level-8 base (256^3, 8000 Mpc/h box) refined to level 11 where the lognormal "
-                  "overdensity > 8 (37.5 M cells), 1024x1024 rays over 10x10 deg, TSC, to z=2",
-    "uniform128_cic": "uniform128_cic: uniform 128^3 grid (3000 Mpc/h box), 64x64 rays over 5x5 deg, CIC, to z=0.5",
+                  "overdensity > 8 (37.5 M cells), 1024x1024 rays on a regular grid over a 10x10 deg narrow field "
+                  "(99.9 deg^2) around (20, 20) deg, TSC, to z=2",
+    "uniform128_cic": "uniform128_cic: uniform 128^3 grid (3000 Mpc/h box), 64x64 rays on a regular grid over a "
+                      "5x5 deg narrow field (25.0 deg^2) around (20, 20) deg, CIC, to z=0.5",
 }
-B_STEP_SPLIT_TSC = 540  # 27 cells x 5 fields x 4 bytes per RK4 step
-B_STEP_FP64_TSC = 1080  # 27 cells x 5 fields x 8 bytes
+B_STEP_SPLIT_TSC = 540  # 27 cells x 5 fields x 4 bytes per RK4 step (equivalent gather, not a roofline)
 BF16_PEAK = 2.5e15      # MI355X dense BF16 matrix-core peak, flop/s (AMD public spec)
 MLP_FLOP_PER_SAMPLE = 2 * (10 * 128 + 128 * 128 + 128 * 2)   # 35,840: the 10-128-128-2 network's forward
+# threads of the CPU baseline: one GPU's share of the host's cores (the
+# reference's Utility::parallelize, utility.h:55, starts hardware_concurrency()
+# threads; here the host is shared by the node's GPUs)
+CPU_SHARE = int(os.environ.get("MPF_CPU_THREADS", "16"))
+PROFILES = {"split": "integrate_split_tsc_amr256_1m_rays.txt", "fp64": "integrate_exact_fp64_tsc_amr256_1m_rays.txt",
+            "ref": "integrate_exact_ref_tsc_amr256_1m_rays.txt",
+            "jacobian": "integrate_split_jacobian_amr256_4m_rays.txt",
+            "uniform128_cic": "integrate_exact_ref_cic_uniform128_4k_rays.txt"}
 
 
-def profiled_traffic(name="integrate_split_tsc_amr256_1m_rays.txt"):
-    """HBM bytes per launch of the split kernel from the committed rocprofv3
-    FETCH_SIZE/WRITE_SIZE summary (calibrated with a copy kernel), or None."""
+def profiled(name, key):
+    """A 'key: value' figure of a committed rocprofv3 summary under
+    profiles/ (counter passes of the same kernel and workload), or None."""
     path = os.path.join(ROOT, "profiles", name)
     try:
         for line in open(path):
-            if line.startswith("traffic_bytes_per_launch:"):
-                return float(line.split(":")[1])
-    except OSError:
+            if line.startswith(key + ":"):
+                return float(line.split(":")[1].split()[0])
+    except (OSError, ValueError):
         pass
     return None
+
+
+def valu_roofline(profile, steps, kms, gather_bytes=None):
+    """VALU roofline of a ray kernel: VALU wave-instructions per ray-step from
+    the SQ_INSTS_VALU pass of the committed profile (same kernel, same
+    workload) x this launch's ray-steps / its HIP-event kernel time, against
+    VALU_PEAK; HBM traffic (FETCH_SIZE/WRITE_SIZE passes) beside it."""
+    insts = profiled(profile, "valu_insts_per_launch")
+    psteps = profiled(profile, "ray_steps_per_launch")
+    traffic = profiled(profile, "traffic_bytes_per_launch")
+    busy = profiled(profile, "valu_busy")
+    per_step = insts / psteps if insts and psteps else None
+    achieved = per_step * steps / (kms / 1e3) if per_step else None
+    r = {"bound": "valu", "achieved": achieved / 1e9 if achieved else None, "peak": VALU_PEAK / 1e9,
+         "unit": "G VALU wave-instr/s", "frac": achieved / VALU_PEAK if achieved else None,
+         "traffic": traffic, "kernel_ms": kms, "valu_insts_per_ray_step": per_step,
+         "valu_busy_profiled": busy,
+         "hbm_GB_per_s": traffic / (kms / 1e3) / 1e9 if traffic else None,
+         "hbm_frac": traffic / (kms / 1e3) / HBM_PEAK if traffic else None,
+         "profile": "profiles/" + profile,
+         "note": "achieved = SQ_INSTS_VALU per ray-step (profile) x ray-steps of this launch / its HIP-event time; "
+                 "peak = 256 CU x 2.4 GHz x 1 wave64 VALU instr per CU-cycle (x 64 lanes x 2 = the 78.6 TFLOP/s "
+                 "FP64 vector peak); traffic = FETCH_SIZE x 2 + WRITE_SIZE x 1 (copy-kernel calibration), HBM bytes "
+                 "per launch"}
+    if gather_bytes:
+        r["equivalent_gather_GB_per_s"] = steps * gather_bytes / (kms / 1e3) / 1e9
+        r["equivalent_gather_note"] = (f"{gather_bytes} B per step: a stencil gather the kernel does not make "
+                                       "(it evaluates precomputed blocks held in L1/L2); not a roofline")
+    return r
+
+
+def host_cpu():
+    """Host CPU description: model, logical CPUs, CPUs this process may use."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"cpu_model": model, "nproc": os.cpu_count(), "usable_cpus": usable}
 
 
 def autotune_summary(trials, seconds):
@@ -64,16 +129,7 @@ def log(*a):
 
 def build_workload(name):
     from mpf import synth
-    w = dict(synth.WORKLOADS[name])
-    t0 = time.time()
-    table = synth.flat_lcdm_table(h=0.7, omega_m=0.3, a_min=0.05, n=4096)
-    rec, f64, length = synth.lightcone_tree(w["base_input_level"], w["box_mpch"], w["sigma"], w["threshold"],
-                                            w["max_input_level"], w["seed"], noise=w.get("noise", 0.0),
-                                            decay=w.get("decay", 0.0), table=table, device="cpu")
-    ph, th = synth.cone_grid(w["phi0"], w["theta0"], w["aperture"], w["nside"])
-    in9 = synth.launch_photons(ph, th)
-    log(f"workload {name}: {rec.size} cells, {in9.shape[0]} rays, built in {time.time() - t0:.1f}s")
-    return w, rec, f64, length, table, in9
+    return synth.build_workload(name, log=log)
 
 
 def cpu_baseline(rec, table, length, in9, w, nthreads, target_s=15.0):
@@ -101,9 +157,70 @@ def cpu_baseline(rec, table, length, in9, w, nthreads, target_s=15.0):
     _, steps = lib.integrate(h, order, table, length, 10, 2, w["z_stop"], sample, **kw)
     dt = time.time() - t0
     lib.free(h)
-    return {"value": steps / dt, "unit": "ray-RK4-steps/s", "cores": nthreads, "kind": kind,
-            "sample": f"{sample.shape[0]} of the {in9.shape[0]} rays (every {stride}th), {steps} RK4 steps, "
-                      f"{dt:.1f}s, {nthreads} threads"}
+    return dict({"value": steps / dt, "unit": "ray-RK4-steps/s", "cores": nthreads, "kind": kind,
+                 "sample": f"{sample.shape[0]} of the {in9.shape[0]} rays (every {stride}th), {steps} RK4 steps, "
+                           f"{dt:.1f}s, {nthreads} threads"}, **host_cpu())
+
+
+def uniform_cic(steps, warmup, nthreads, cpu=True):
+    """The plain-method workload in full (uniform128_cic: 128^3 uniform grid,
+    4096 rays, CIC, to z = 0.5) in the REF tier (bit-identical to the
+    reference) and the FP64 tier: kernel throughput over `steps` passes, and
+    the reference's CPU path (oracle/_ref, or the oracle) on every ray, whose
+    output the REF tier must equal bit for bit."""
+    from mpf import capi
+    w, rec, f64, length, table, in9 = build_workload("uniform128_cic")
+    ctx = capi.Context(length, order=w["order"], precision=capi.PREC_REF, nsteps=10, device=0)
+    ctx.load_records(rec)
+    ctx.load_fields64(f64)
+    ctx.set_cosmology(table)
+    n = in9.shape[0]
+    d_in, d_out = ctx.dev_alloc(in9.nbytes), ctx.dev_alloc(n * capi.NOUT * 8)
+    ctx.upload(d_in, in9)
+    stop = ("redshift", w["z_stop"])
+    rec_out = {"workload": WORKLOAD_TEXT["uniform128_cic"], "rays": n}
+    outs = {}
+    for name, prec in (("ref", capi.PREC_REF), ("fp64", capi.PREC_FP64)):
+        ctx.set_precision(prec)
+        for _ in range(warmup):
+            ctx.integrate_dev(d_in, n, d_out, *stop, sync=True)
+        kms = []
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.integrate_dev(d_in, n, d_out, *stop, sync=False)
+            st, _ = ctx.last_run()
+            kms.append(ctx.last_kernel_ms())
+        ctx.synchronize()
+        wall = time.perf_counter() - t0
+        outs[name] = ctx.download(d_out, (n, capi.NOUT))
+        k = float(np.mean(kms))
+        rec_out[name] = {"value": st * steps / wall, "unit": "ray-RK4-steps/s", "ms_per_step": wall / steps * 1e3,
+                         "kernel_ms": k, "kernel_steps_per_s": st / (k / 1e3), "rk4_steps_per_pass": int(st)}
+        if name == "ref":
+            rec_out[name]["roofline"] = valu_roofline(PROFILES["uniform128_cic"], st, k)
+    ctx.dev_free(d_in)
+    ctx.dev_free(d_out)
+    ctx.close()
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+        kind = "reference" if orc.ref_available() else "port"
+        lib = orc.RefHarness() if kind == "reference" else orc.Oracle()
+        h = lib.tree(rec)
+        t0 = time.time()
+        ref, rsteps = lib.integrate(h, w["order"], table, length, 10, 2, w["z_stop"], in9, nthreads=nthreads)
+        dt = time.time() - t0
+        lib.free(h)
+        rec_out["cpu_baseline"] = dict({"value": rsteps / dt, "unit": "ray-RK4-steps/s", "cores": nthreads,
+                                        "kind": kind, "sample": f"all {n} rays, {rsteps} RK4 steps, {dt:.2f}s, "
+                                                                f"{nthreads} threads"}, **host_cpu())
+        rec_out["ref_tier_bit_identical_to_cpu"] = bool(np.array_equal(outs["ref"], ref)) if kind == "reference" \
+            else None
+        rec_out["steps_match_cpu"] = bool(rsteps == rec_out["ref"]["rk4_steps_per_pass"])
+    z, zr = outs["fp64"][:, capi.OUT["redshift"]], outs["ref"][:, capi.OUT["redshift"]]
+    rec_out["fp64_vs_ref_rel_end_redshift_max"] = float(np.max(np.abs(z - zr) / np.abs(zr)))
+    return rec_out
 
 
 def jacobian_maps(ctx, w, length, table, n=4194304, nside=2048, zmin=0.1, zmax=2.0, seed=7):
@@ -112,7 +229,7 @@ def jacobian_maps(ctx, w, length, table, n=4194304, nside=2048, zmin=0.1, zmax=2
     with the infinitesimal Jacobian there, then kappa, gamma1, gamma2 binned
     into RING maps of the given nside (split tier)."""
     from mpf import capi, synth
-    ph, th = synth.cone_uniform(w["phi0"], w["theta0"], w["aperture"], n, seed)
+    ph, th = synth.narrow_field_uniform(w["phi0"], w["theta0"], w["aperture"], n, seed)
     in9 = synth.launch_photons(ph, th)
     zs = np.random.default_rng(seed + 1).uniform(zmin, zmax, n)
     npix = 12 * nside * nside
@@ -150,8 +267,8 @@ def jacobian_maps(ctx, w, length, table, n=4194304, nside=2048, zmin=0.1, zmax=2
     for p in (d_in, d_rv, d_out, d_surf, d_maps):
         ctx.dev_free(p)
     valid = surf[:, capi.SURF["valid"]] != 0
-    b_step = B_STEP_SPLIT_TSC + 6 * 27 * 3 * 4   # + 6 offset samples of grad phi (FP32)
-    return {"workload": f"{n} rays uniform over {w['aperture']}x{w['aperture']} deg, per-ray z_s ~ U[{zmin}, {zmax}], "
+    return {"workload": f"{n} rays uniform over the {w['aperture']}x{w['aperture']} deg narrow field "
+                        f"({synth.narrow_field_area_deg2(w['aperture']):.1f} deg^2), per-ray z_s ~ U[{zmin}, {zmax}], "
                         f"infinitesimal Jacobian at z_s, kappa/gamma1/gamma2 binned into nside {nside} RING maps",
             "value": steps / (wall_int / 1e3 + wall_bin / 1e3), "unit": "ray-RK4-steps/s (integration + maps)",
             "kernel_steps_per_s": steps / (kms / 1e3), "rk4_steps": int(steps), "integrate_ms": ms,
@@ -160,10 +277,7 @@ def jacobian_maps(ctx, w, length, table, n=4194304, nside=2048, zmin=0.1, zmax=2
             "pixels_hit": int(np.count_nonzero(maps[3])), "kappa_rms": float(np.sqrt(np.mean(
                 maps[0][maps[3] > 0] ** 2))),
             "launch_config": tuned, "autotune": autotune_summary(trials, t_tune),
-            "roofline": {"bound": "hbm", "bytes_per_step": b_step,
-                         "achieved": steps / (kms / 1e3) * b_step / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                         "frac": steps / (kms / 1e3) * b_step / HBM_PEAK,
-                         "traffic": profiled_traffic("integrate_split_jacobian_amr256_4m_rays.txt")}}
+            "roofline": valu_roofline(PROFILES["jacobian"], steps, kms)}
 
 
 def velocity_field(ctx, rec, n=1 << 24, seed=13):
@@ -207,23 +321,22 @@ def velocity_field(ctx, rec, n=1 << 24, seed=13):
 
 def catalogue(ctx, w, length, table, n=1048576, zmin=0.1, zmax=1.5, acc=1e-8, ntrain=16384, seed=11):
     """Catalogue root-finding: n sources uniform in comoving volume in the
-    cone (circular, the area of the square aperture) at z in [zmin, zmax],
+    narrow field of the ray workload (same axis, same area) at z in [zmin, zmax],
     Newton to acc (rad) with the infinitesimal Jacobian, split tier: plain
     (unlensed first guess) against the first guess from the Born features
     (and the network when it beats them on a held-out cone; trained on pairs
     from 4 other cones of the same octree)."""
-    import math
     from mpf import capi, proposal, synth
     box = w["box_mpch"]
     dmin = synth.comoving_distance_of_z(zmin) * 0.7 / synth.MPC_SI / box
     dmax = synth.comoving_distance_of_z(zmax) * 0.7 / synth.MPC_SI / box
-    half = math.degrees(math.acos(1 - (math.radians(w["aperture"]) ** 2) / (2 * math.pi)))
-    src = proposal.cone_sources(n, w["phi0"], w["theta0"], half, dmin, dmax, seed)
+    ap = w["aperture"]
+    src = proposal.field_sources(n, w["phi0"], w["theta0"], ap, dmin, dmax, seed)
     # training pairs on 4 other cones, held-out cone for the network's gate
     t0 = time.perf_counter()
     pairs = []
     for k, (p, t) in enumerate(((80.0, 50.0), (150.0, 70.0), (230.0, 40.0), (300.0, 110.0))):
-        s = proposal.cone_sources(ntrain, p, t, half, dmin, dmax, seed + 100 + k)
+        s = proposal.field_sources(ntrain, p, t, ap, dmin, dmax, seed + 100 + k)
         fe, ta, _, _ = proposal.training_pairs(ctx, s, accuracy=acc)
         pairs.append((fe, ta))
     t_pairs = time.perf_counter() - t0
@@ -272,8 +385,9 @@ def catalogue(ctx, w, length, table, n=1048576, zmin=0.1, zmax=1.5, acc=1e-8, nt
                "flop_per_sample": MLP_FLOP_PER_SAMPLE, "achieved_TFLOP_s": flop / min(t_inf) / 1e12,
                "peak_TFLOP_s": BF16_PEAK / 1e12, "frac": flop / min(t_inf) / BF16_PEAK,
                "note": "wall time of one call (normalise, MFMA forward, denormalise, one scratch allocation)"}
-    return {"workload": f"{n} sources uniform in comoving volume, z in [{zmin}, {zmax}], cone of {half:.2f} deg "
-                        f"half-angle, Newton to {acc} rad, infinitesimal Jacobian, split tier",
+    return {"workload": f"{n} sources uniform in comoving volume, z in [{zmin}, {zmax}], {ap}x{ap} deg narrow field "
+                        f"({synth.narrow_field_area_deg2(ap):.1f} deg^2) of the ray workload, Newton to {acc} rad, "
+                        f"infinitesimal Jacobian, split tier",
             "metric": "catalogue sources/s", "value": runs["seeded"]["sources_per_s"], "unit": "sources/s",
             "plain": runs["plain"], "seeded": runs["seeded"],
             "speedup_seeded_over_plain": runs["seeded"]["sources_per_s"] / runs["plain"]["sources_per_s"],
@@ -300,6 +414,8 @@ def main():
     ap.add_argument("--launch-config", default=None, metavar="R,B,C,W",
                     help="use this launch configuration (refill, sort bucket, blocks per CU, waves per SIMD) for "
                          "every workload instead of the autotuner's (profiling runs)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the library's default launch configuration (profiling runs)")
     ap.add_argument("--tier", choices=("split", "fp64", "ref"), default="split",
                     help="precision tier of the timed passes (profiling runs of the exact tiers)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
@@ -366,7 +482,7 @@ def main():
         log(f"warmup {i}: {steps} steps, kernel {ms:.1f} ms, wall {time.time() - t0:.2f}s")
     # launch configuration tuned on this batch (untimed, like the warmup)
     tuned, trials, t_tune = ctx.launch_config(), np.zeros((0, 6)), 0.0
-    if FIXED_LAUNCH is None:
+    if FIXED_LAUNCH is None and not args.no_autotune:
         t0 = time.perf_counter()
         tuned, trials = ctx.autotune_dev(d_in, n, d_out, *stop)
         t_tune = time.perf_counter() - t0
@@ -385,49 +501,49 @@ def main():
     value = total_steps / wall
     kms = float(np.mean(kernel_ms))
     kernel_rate = steps / (kms / 1e3)
-    log(f"split: {steps} steps per pass, {value / 1e9:.3f} G steps/s wall, {kernel_rate / 1e9:.3f} G steps/s kernel")
+    log(f"{args.tier}: {steps} steps per pass, {value / 1e9:.3f} G steps/s wall, {kernel_rate / 1e9:.3f} G steps/s kernel")
     out_split = ctx.download(d_out, (n, capi.NOUT))
+    tier_text = {"split": ("split FP32-perturbation/FP64-background", "fp32-perturbation/fp64-background"),
+                 "fp64": ("exact FP64 tier", "f64"), "ref": ("exact REF tier, bit-identical to the reference", "f32")}
+    if tier != {"split": capi.PREC_SPLIT, "fp64": capi.PREC_FP64, "ref": capi.PREC_REF}[args.tier]:
+        args.tier = "fp64"
+    scheme = "TSC" if order == 2 else "CIC"
     result = {
-        "metric": "ray-RK4-steps/s (split FP32-perturbation/FP64-background, TSC)",
+        "metric": f"ray-RK4-steps/s ({tier_text[args.tier][0]}, {scheme})",
         "value": value, "unit": "ray-RK4-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32-perturbation/fp64-background", "data": "synthetic (seeded GRF lightcone octree)",
+        "dtype": tier_text[args.tier][1], "data": "synthetic (seeded GRF lightcone octree)",
         "config": {"workload": WORKLOAD_TEXT[args.workload],
                    "cells": int(ncells), "levels": [int(lmin), int(lmax)], "rays": int(n),
                    "rk4_steps_per_pass": int(steps)},
         "launch_config": tuned, "autotune": autotune_summary(trials, t_tune),
     }
-    bstep = B_STEP_SPLIT_TSC if order == 2 else 320 // 2
-    achieved = kernel_rate * bstep
-    traffic = profiled_traffic() if order == 2 else None
-    result["roofline"] = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                          "frac": achieved / HBM_PEAK, "traffic": traffic,
-                          "bytes_per_step": bstep, "kernel_ms": kms,
-                          "integration_ms": float(np.mean(run_ms)),
-                          "traffic_GB_per_s": traffic / (kms / 1e3) / 1e9 if traffic else None,
-                          "note": "achieved = algorithmic stencil bytes (27 cells x 5 fields x 4 B per step) over the "
-                                  "persistent kernel's HIP-event time; traffic = HBM bytes per launch from the "
-                                  "FETCH_SIZE/WRITE_SIZE counters (profiles/): the stencil blocks are served from "
-                                  "L2/MALL, so achieved can approach or pass the HBM peak"}
-    if not args.no_fp64:
+    prof = PROFILES[args.tier] if args.workload == "amr256_tsc" else PROFILES["uniform128_cic"]
+    result["roofline"] = valu_roofline(prof, steps, kms, B_STEP_SPLIT_TSC if args.tier == "split" else None)
+    result["roofline"]["integration_ms"] = float(np.mean(run_ms))
+    if not args.no_fp64 and args.tier == "split":
         ctx.set_precision(capi.PREC_FP64)
         t0 = time.time()
         ctx.integrate_dev(d_in, n, d_out, *stop, sync=True)
         s64, ms64 = ctx.last_run()
+        k64 = ctx.last_kernel_ms()
         out64 = ctx.download(d_out, (n, capi.NOUT))
-        log(f"fp64 tier: {s64} steps, kernel {ms64:.1f} ms ({s64 / (ms64 / 1e3) / 1e9:.3f} G steps/s)")
+        log(f"fp64 tier: {s64} steps, kernel {k64:.1f} ms ({s64 / (k64 / 1e3) / 1e9:.3f} G steps/s)")
         ok = (out64[:, capi.OUT["nsteps"]] >= 0) & (out_split[:, capi.OUT["nsteps"]] >= 0)
         z64 = out64[ok, capi.OUT["redshift"]]
         zs = out_split[ok, capi.OUT["redshift"]]
         rel_z = np.abs(zs - z64) / np.abs(z64)
         dpos = np.linalg.norm(out_split[ok, 2:5] - out64[ok, 2:5], axis=1) / np.linalg.norm(out64[ok, 2:5], axis=1)
-        result["fp64_tier"] = {"value": s64 / (ms64 / 1e3), "unit": "ray-RK4-steps/s", "kernel_ms": ms64,
-                               "steps": int(s64), "speedup_split_over_fp64": kernel_rate / (s64 / (ms64 / 1e3)),
+        rf = valu_roofline(PROFILES["fp64"], s64, k64) if order == 2 else None
+        result["fp64_tier"] = {"value": s64 / (k64 / 1e3), "unit": "ray-RK4-steps/s", "kernel_ms": k64,
+                               "integration_ms": ms64,
+                               "steps": int(s64), "speedup_split_over_fp64": kernel_rate / (s64 / (k64 / 1e3)),
                                "rel_end_redshift_err_max": float(rel_z.max()),
                                "rel_end_redshift_err_median": float(np.median(rel_z)),
                                "rel_end_position_err_max": float(dpos.max()),
                                "step_count_mismatch_rays": int(np.sum(out64[:, capi.OUT["nsteps"]] !=
-                                                                      out_split[:, capi.OUT["nsteps"]]))}
+                                                                      out_split[:, capi.OUT["nsteps"]])),
+                               "roofline": rf}
         ctx.set_precision(capi.PREC_SPLIT)
     if not args.no_extra and order == 2:
         t0 = time.time()
@@ -436,8 +552,12 @@ def main():
         t0 = time.time()
         result["catalogue"] = catalogue(ctx, w, length, table)
         log(f"catalogue: {result['catalogue']} ({time.time() - t0:.1f}s)")
+        t0 = time.time()
+        result["uniform128_cic"] = uniform_cic(max(args.steps, 5), max(args.warmup, 1),
+                                               min(CPU_SHARE, host_cpu()["usable_cpus"]), cpu=not args.no_cpu)
+        log(f"uniform128_cic: {result['uniform128_cic']} ({time.time() - t0:.1f}s)")
     if not args.no_cpu:
-        nth = min(16, os.cpu_count() or 1)
+        nth = min(CPU_SHARE, host_cpu()["usable_cpus"])
         t0 = time.time()
         result["cpu_baseline"] = cpu_baseline(rec, table, length, in9, w, nth)
         log(f"cpu baseline: {result['cpu_baseline']} ({time.time() - t0:.1f}s)")

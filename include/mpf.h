@@ -466,9 +466,14 @@ int mpf_mlp_train(mpf_ctx* ctx, const float* feat, const float* target, uint64_t
 /* Loss and gradient (MPF_MLP_NPARAM floats) of one batch, parameters
  * unchanged: the training step without the update. */
 int mpf_mlp_gradient(mpf_ctx* ctx, const float* feat, const float* target, uint64_t n, float* grad, float* loss);
-/* Predicted targets (n x MPF_MLP_NOUT). */
+/* Predicted targets (n x MPF_MLP_NOUT).  The _dev form allocates nothing:
+ * one persistent kernel holds the BF16 weights in LDS and normalises the
+ * features and scales the outputs itself. */
 int mpf_mlp_infer(mpf_ctx* ctx, const float* feat, uint64_t n, float* out);
 int mpf_mlp_infer_dev(mpf_ctx* ctx, const float* dev_feat, uint64_t n, float* dev_out);
+/* Duration (ms, HIP events on the context's stream) of the last inference
+ * kernel; -1 before the first. */
+int mpf_mlp_last_ms(mpf_ctx* ctx, double* ms);
 /* Features of sources (n x 6 doubles as in mpf_catalogue_solve). */
 int mpf_catalogue_features(mpf_ctx* ctx, const double* src6, uint64_t n, float* feat);
 int mpf_catalogue_features_dev(mpf_ctx* ctx, const double* dev_src6, uint64_t n, float* dev_feat);
@@ -480,6 +485,13 @@ int mpf_catalogue_seed_dev(mpf_ctx* ctx, const double* dev_src6, uint64_t n, int
  * accumulation (row-major 16 x 16 floats): checks the operand layout the
  * network's kernels assume. */
 int mpf_mfma_probe(mpf_ctx* ctx, const float* a, const float* b, float* c);
+/* One 16 x 16 x 32 product on gfx950's matrix cores, c = bf16(a) bf16(b)
+ * (a 16 x 32, b 32 x 16, c 16 x 16, row-major floats), with the operand
+ * layout the network's kernels use (layout 0: slot j of lane l holds
+ * K = 8 (l >> 4) + j in both operands), or with B's slots given the K of two
+ * halves (layout 1: 4 (l >> 4) + j and 16 + 4 (l >> 4) + j) while A keeps
+ * layout 0: a mismatched pairing, so that a check can show it is caught. */
+int mpf_mfma_probe32(mpf_ctx* ctx, const float* a, const float* b, int layout, float* c);
 
 /* Cones and photon launch (host, no device needed).  A cone is MPF_CONE_N
  * doubles: vertex x, y, z, base centre x, y, z, half-angle (rad), the three

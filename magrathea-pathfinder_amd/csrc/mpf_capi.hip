@@ -1475,6 +1475,13 @@ int mpf_mlp_infer_dev(mpf_ctx* c, const float* dev_feat, uint64_t n, float* dev_
     return 0;
 }
 
+int mpf_mlp_last_ms(mpf_ctx* c, double* ms) {
+    if (!c || !ms) return ERR_INVAL;
+    if (int rc = need_mlp(c)) return rc;
+    *ms = (double)mpf::mlp_last_ms(c->mlp);
+    return 0;
+}
+
 int mpf_mlp_infer(mpf_ctx* c, const float* feat, uint64_t n, float* out) {
     if (int rc = need_mlp(c)) return rc;
     if (n && (!feat || !out)) return ERR_INVAL;
@@ -1584,6 +1591,25 @@ int mpf_mfma_probe(mpf_ctx* c, const float* a, const float* b, float* out) {
         hipError_t e = mpf::launch_mfma_probe(d.p, d.p + 256, d.p + 512, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         if (e == hipSuccess) e = hipMemcpy(out, d.p + 512, 1024, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = hipfail(c, e, "probe");
+    }
+    d.release();
+    return rc;
+}
+
+int mpf_mfma_probe32(mpf_ctx* c, const float* a, const float* b, int layout, float* out) {
+    if (!c || !a || !b || !out || (layout != 0 && layout != 1)) return ERR_INVAL;
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    DevBuf<float> d;
+    if (d.ensure(512 + 512 + 256)) return fail(c, ERR_NOMEM, "probe buffer");
+    int rc = 0;
+    if (hipMemcpy(d.p, a, 2048, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d.p + 512, b, 2048, hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(c, ERR_HIP, "probe upload");
+    if (!rc) {
+        hipError_t e = mpf::launch_mfma_probe32(d.p, d.p + 512, d.p + 1024, layout, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess) e = hipMemcpy(out, d.p + 1024, 1024, hipMemcpyDeviceToHost);
         if (e != hipSuccess) rc = hipfail(c, e, "probe");
     }
     d.release();

@@ -1,22 +1,27 @@
 // Learned first guess for the catalogue root-finder: a 10 -> 128 -> 128 -> 2
-// GELU network with BF16 operands and FP32 accumulation on the matrix cores
-// (v_mfma_f32_16x16x16_bf16), trained on this device by its own forward and
-// backward kernels (mixed precision: FP32 master weights, biases and Adam
-// moments; BF16 copies of the weights, activations and back-propagated
-// deltas).  No counterpart in the reference: it seeds the Newton iterations
-// of catalogues.h:757-800, which still finish every source to the accuracy.
+// GELU network with BF16 operands and FP32 accumulation on the matrix cores,
+// trained on this device by its own forward and backward kernels (mixed
+// precision: FP32 master weights, biases and Adam moments; BF16 copies of the
+// weights, activations and back-propagated deltas).  No counterpart in the
+// reference: it seeds the Newton iterations of catalogues.h:757-800, which
+// still finish every source to the accuracy.
 //
-// MFMA 16x16x16 operand layout (wave64, lane l, r = l & 15, q = l >> 4):
-//   A (M x K): lane holds A[r][4q + j], j = 0..3
-//   B (K x N): lane holds B[4q + j][r]
-//   D (M x N): lane holds D[4q + j][r]
-// B operands are stored as B^T rows (N x K) so that a lane reads 8
-// contiguous bytes.  mfma_probe_kernel checks the layout on the device.
+// Products with K = 128 (hidden layers, weight gradients over samples) use
+// gfx950's v_mfma_f32_16x16x32_bf16, those with K = 16 (input layer, the
+// output delta) v_mfma_f32_16x16x16_bf16.  Operand layouts (wave64, lane l,
+// r = l & 15, q = l >> 4):
+//   16x16x16: A (M x K) lane holds A[r][4q + j], j < 4; B (K x N) B[4q + j][r]
+//   16x16x32: A lane holds A[r][8q + j], j < 8;      B B[8q + j][r]
+//   D (M x N), both: lane holds D[4q + j][r], j < 4
+// B operands are stored as B^T rows (N x K) so that a lane reads 8 or 16
+// contiguous bytes.  mfma_probe_kernel and mfma_probe32_kernel check the
+// layouts on the device.
 //
 // One wave computes 16 samples through all layers: the 16 x 128 hidden
 // activations go through LDS (BF16) between layers, the FP32
 // pre-activations needed by GELU' stay in registers in the D layout, which is
-// also the layout of the back-propagated deltas.
+// also the layout of the back-propagated deltas.  Inference runs a
+// persistent kernel that holds the BF16 weights in LDS.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -37,6 +42,7 @@ namespace mpf {
 namespace {
 
 typedef short s4 __attribute__((ext_vector_type(4)));
+typedef __bf16 b8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int KIN = MPF_MLP_KIN;     // 10 input features padded to one MFMA K step
@@ -74,6 +80,21 @@ __device__ inline float gelu_d(float x) {
 }
 
 __device__ inline f4 mfma(s4 a, s4 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
+__device__ inline f4 mfma32(b8 a, b8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+
+// 8 BF16 values from 16 contiguous, 16-byte aligned bytes
+__device__ inline b8 load8(const unsigned short* p) {
+    return __builtin_bit_cast(b8, *reinterpret_cast<const uint4*>(p));
+}
+__device__ inline b8 pack8(const unsigned short* v) {
+    uint4 w;
+    w.x = (unsigned)v[0] | ((unsigned)v[1] << 16);
+    w.y = (unsigned)v[2] | ((unsigned)v[3] << 16);
+    w.z = (unsigned)v[4] | ((unsigned)v[5] << 16);
+    w.w = (unsigned)v[6] | ((unsigned)v[7] << 16);
+    return __builtin_bit_cast(b8, w);
+}
+constexpr int KT32 = NH / 32;   // K steps of 32 over a hidden layer
 
 __device__ inline s4 load4(const unsigned short* p) {
     const uint2 w = *reinterpret_cast<const uint2*>(p);  // 8-byte aligned by construction
@@ -103,16 +124,14 @@ struct Saved {                  // per-sample BF16 tensors of a training batch
     unsigned short* d3;         // n x KOUT dL/dZ3
 };
 
-// Forward pass for 16 samples per wave; with TRAIN also the loss, the
+// Training step of 16 samples per wave: the forward pass, the loss, the
 // backward pass through the activations and the saved tensors.
 //   x: normalised features (MPF_MLP_NIN per sample), rows perm[i] (or i)
-//   y: outputs (MPF_MLP_NOUT per sample, normalised units)
-//   t: normalised targets; loss accumulates sum (y - t)^2
-template <bool TRAIN>
-__global__ void __launch_bounds__(256) mlp_tile_kernel(Bf16Weights W, const float* __restrict__ x,
-                                                       const uint32_t* __restrict__ perm, uint64_t n,
-                                                       float* __restrict__ y, const float* __restrict__ t, Saved S,
-                                                       float dscale, float* __restrict__ loss) {
+//   t: normalised targets; loss accumulates sum (y - t)^2 of the outputs y
+__global__ void __launch_bounds__(256) mlp_train_tile_kernel(Bf16Weights W, const float* __restrict__ x,
+                                                             const uint32_t* __restrict__ perm, uint64_t n,
+                                                             const float* __restrict__ t, Saved S, float dscale,
+                                                             float* __restrict__ loss) {
     __shared__ __align__(16) unsigned short lds_a[4][16][NH];   // H1
     __shared__ __align__(16) unsigned short lds_b[4][16][NH];   // H2, then dZ2
     __shared__ __align__(16) unsigned short lds_c[4][16][KOUT]; // dZ3
@@ -134,7 +153,7 @@ __global__ void __launch_bounds__(256) mlp_tile_kernel(Bf16Weights W, const floa
             const float v = (valid && f < MPF_MLP_NIN) ? x[src * MPF_MLP_NIN + f] : 0.0f;
             const unsigned short b = f2bf(v);
             xa[j] = (short)b;
-            if (TRAIN && valid) S.x[si * KIN + f] = b;
+            if (valid) S.x[si * KIN + f] = b;
         }
     }
     float z1[NT][4], z2[NT][4];
@@ -149,42 +168,36 @@ __global__ void __launch_bounds__(256) mlp_tile_kernel(Bf16Weights W, const floa
             z1[nt][j] = acc[j] + bias;
             const unsigned short b = f2bf(gelu(z1[nt][j]));
             h1[4 * q + j][nt * 16 + r] = b;
-            if (TRAIN && base + 4 * q + j < n) S.h1[(base + 4 * q + j) * NH + nt * 16 + r] = b;
+            if (base + 4 * q + j < n) S.h1[(base + 4 * q + j) * NH + nt * 16 + r] = b;
         }
     }
     __syncthreads();
     // layer 2: Z2 = H1 W2 + b2, H2 = gelu(Z2)
-    s4 ha[NT];
+    b8 ha[KT32];
 #pragma unroll
-    for (int kt = 0; kt < NT; ++kt) ha[kt] = load4(&h1[r][kt * 16 + 4 * q]);
+    for (int kt = 0; kt < KT32; ++kt) ha[kt] = load8(&h1[r][kt * 32 + 8 * q]);
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) {
         f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int kt = 0; kt < NT; ++kt) acc = mfma(ha[kt], load4(W.w2t + (nt * 16 + r) * NH + kt * 16 + 4 * q), acc);
+        for (int kt = 0; kt < KT32; ++kt)
+            acc = mfma32(ha[kt], load8(W.w2t + (nt * 16 + r) * NH + kt * 32 + 8 * q), acc);
         const float bias = W.b2[nt * 16 + r];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             z2[nt][j] = acc[j] + bias;
             const unsigned short b = f2bf(gelu(z2[nt][j]));
             h2[4 * q + j][nt * 16 + r] = b;
-            if (TRAIN && base + 4 * q + j < n) S.h2[(base + 4 * q + j) * NH + nt * 16 + r] = b;
+            if (base + 4 * q + j < n) S.h2[(base + 4 * q + j) * NH + nt * 16 + r] = b;
         }
     }
     __syncthreads();
     // layer 3: Y = H2 W3 + b3 (output r of sample 4q + j)
     f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int kt = 0; kt < NT; ++kt) acc = mfma(load4(&h2[r][kt * 16 + 4 * q]), load4(W.w3t + r * NH + kt * 16 + 4 * q), acc);
+    for (int kt = 0; kt < KT32; ++kt)
+        acc = mfma32(load8(&h2[r][kt * 32 + 8 * q]), load8(W.w3t + r * NH + kt * 32 + 8 * q), acc);
     const float b3 = W.b3[r];
-    if (!TRAIN) {
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const uint64_t si = base + 4 * q + j;
-            if (si < n && r < MPF_MLP_NOUT) y[si * MPF_MLP_NOUT + r] = acc[j] + b3;
-        }
-        return;
-    }
     // loss = sum over samples and outputs of (y - t)^2; dL/dY scaled by dscale
     float lsum = 0.f;
     unsigned short d3[4];
@@ -223,12 +236,12 @@ __global__ void __launch_bounds__(256) mlp_tile_kernel(Bf16Weights W, const floa
     __syncthreads();
     // dZ1 = (dZ2 W2^T) * gelu'(Z1)
 #pragma unroll
-    for (int kt = 0; kt < NT; ++kt) ha[kt] = load4(&h2[r][kt * 16 + 4 * q]);
+    for (int kt = 0; kt < KT32; ++kt) ha[kt] = load8(&h2[r][kt * 32 + 8 * q]);
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) {
         f4 g = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int kt = 0; kt < NT; ++kt) g = mfma(ha[kt], load4(W.w2 + (nt * 16 + r) * NH + kt * 16 + 4 * q), g);
+        for (int kt = 0; kt < KT32; ++kt) g = mfma32(ha[kt], load8(W.w2 + (nt * 16 + r) * NH + kt * 32 + 8 * q), g);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             if (base + 4 * q + j < n) S.d1[(base + 4 * q + j) * NH + nt * 16 + r] = f2bf(g[j] * gelu_d(z1[nt][j]));
@@ -263,17 +276,18 @@ __global__ void __launch_bounds__(64) mlp_wgrad_kernel(GradJob J0, GradJob J1, G
     const uint64_t k_end = std::min<uint64_t>(n, k_begin + WGRAD_CHUNK);
     f4 acc = {0.f, 0.f, 0.f, 0.f};
     float bsum = 0.f;
-    for (uint64_t k0 = k_begin; k0 < k_end; k0 += 16) {
-        s4 a, bb;
+    // K runs over the chunk's samples, 32 per product (zero rows past its end)
+    for (uint64_t k0 = k_begin; k0 < k_end; k0 += 32) {
+        unsigned short av[8], bv[8];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const uint64_t s = k0 + 4 * q + j;
+        for (int j = 0; j < 8; ++j) {
+            const uint64_t s = k0 + 8 * q + j;
             const bool ok = s < k_end;
-            a[j] = ok ? (short)J.act[s * J.ka + mi * 16 + r] : (short)0;
-            bb[j] = ok ? (short)J.dz[s * J.kz + ni * 16 + r] : (short)0;
-            bsum += bf2f((unsigned short)bb[j]);
+            av[j] = ok ? J.act[s * J.ka + mi * 16 + r] : (unsigned short)0;
+            bv[j] = ok ? J.dz[s * J.kz + ni * 16 + r] : (unsigned short)0;
+            bsum += bf2f(bv[j]);
         }
-        acc = mfma(a, bb, acc);
+        acc = mfma32(pack8(av), pack8(bv), acc);
     }
     float* out = part + (uint64_t)blockIdx.y * NPARAM;
 #pragma unroll
@@ -345,6 +359,122 @@ __global__ void mfma_probe_kernel(const float* __restrict__ a, const float* __re
     for (int j = 0; j < 4; ++j) c[(4 * q + j) * 16 + r] = acc[j];
 }
 
+// 16 x 16 x 32 product through one MFMA: c = bf16(a) bf16(b), a 16 x 32 and
+// b 32 x 16 row-major, with the operand layout the kernels use (split = 0:
+// slot j of lane (r, q) holds K = 8q + j in both operands).  A product only
+// needs both operands to give slot j the same K; split = 1 gives B's slots
+// the K of two halves (4q + j, 16 + 4q + j) while A keeps 8q + j, a
+// mismatched pairing that a check of the probe must see fail.
+__global__ void mfma_probe32_kernel(const float* __restrict__ a, const float* __restrict__ b, float* __restrict__ c,
+                                    int split) {
+    const int l = threadIdx.x;
+    const int r = l & 15, q = l >> 4;
+    unsigned short av[8], bv[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int ka = 8 * q + j;
+        const int kb = split ? ((j < 4) ? 4 * q + j : 16 + 4 * q + (j - 4)) : ka;
+        av[j] = f2bf(a[r * 32 + ka]);
+        bv[j] = f2bf(b[kb * 16 + r]);
+    }
+    f4 acc = {0.f, 0.f, 0.f, 0.f};
+    acc = mfma32(pack8(av), pack8(bv), acc);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) c[(4 * q + j) * 16 + r] = acc[j];
+}
+
+// Inference: a persistent grid whose blocks hold the BF16 weights and the
+// biases in LDS (loaded once) and walk tiles of 64 samples, 16 per wave.
+// Features are normalised on the way in (as normalise_kernel does for
+// training) and outputs scaled back by the target std and mean on the way
+// out, so that a call needs no buffer beyond its input and output.
+constexpr int LROW = NH + 8;   // LDS row stride (BF16) of 128-wide rows: spreads 16-byte reads over the banks
+
+__global__ void __launch_bounds__(256) mlp_infer_kernel(Bf16Weights W, const float* __restrict__ norm,
+                                                        const float* __restrict__ x, uint64_t n,
+                                                        float* __restrict__ y) {
+    __shared__ __align__(16) unsigned short w2t[NH][LROW];     // W2^T
+    __shared__ __align__(16) unsigned short w3t[KOUT][LROW];   // W3^T
+    __shared__ __align__(16) unsigned short w1t[NH][KIN];      // W1^T
+    __shared__ __align__(16) unsigned short hb[4][16][LROW];   // per wave: H1, then H2
+    __shared__ float bias[2 * NH + KOUT];
+    __shared__ float nrm[MPF_MLP_NNORM];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < NH * NH / 8; i += 256) {
+        const int row = i / (NH / 8), c8 = i % (NH / 8);
+        *reinterpret_cast<uint4*>(&w2t[row][8 * c8]) = *reinterpret_cast<const uint4*>(W.w2t + row * NH + 8 * c8);
+    }
+    for (int i = tid; i < KOUT * NH / 8; i += 256) {
+        const int row = i / (NH / 8), c8 = i % (NH / 8);
+        *reinterpret_cast<uint4*>(&w3t[row][8 * c8]) = *reinterpret_cast<const uint4*>(W.w3t + row * NH + 8 * c8);
+    }
+    for (int i = tid; i < NH * KIN / 8; i += 256)
+        *reinterpret_cast<uint4*>(&w1t[0][0] + 8 * i) = *reinterpret_cast<const uint4*>(W.w1t + 8 * i);
+    for (int i = tid; i < NH; i += 256) {
+        bias[i] = W.b1[i];
+        bias[NH + i] = W.b2[i];
+    }
+    if (tid < KOUT) bias[2 * NH + tid] = W.b3[tid];
+    if (tid < MPF_MLP_NNORM) nrm[tid] = norm[tid];
+    __syncthreads();
+    const int wave = tid >> 6;
+    const int l = tid & 63;
+    const int r = l & 15, q = l >> 4;
+    unsigned short(*h)[LROW] = hb[wave];
+    const uint64_t ngroups = (n + 63) / 64;
+    for (uint64_t g = blockIdx.x; g < ngroups; g += gridDim.x) {   // the same trip count for the block's waves
+        const uint64_t base = g * 64 + (uint64_t)wave * 16;
+        // layer 1 (K = 16): sample base + r, features 4q .. 4q + 3
+        s4 xa;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int f = 4 * q + j;
+            const uint64_t si = base + r;
+            const float v = (si < n && f < MPF_MLP_NIN)
+                                ? (x[si * MPF_MLP_NIN + f] - nrm[f]) / nrm[MPF_MLP_NIN + f] : 0.0f;
+            xa[j] = (short)f2bf(v);
+        }
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            f4 acc = {0.f, 0.f, 0.f, 0.f};
+            acc = mfma(xa, load4(&w1t[nt * 16 + r][4 * q]), acc);
+            const float b = bias[nt * 16 + r];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) h[4 * q + j][nt * 16 + r] = f2bf(gelu(acc[j] + b));
+        }
+        __syncthreads();
+        // layer 2 (K = 128): the whole H1 row of sample r into registers, then H2 over it
+        b8 ha[KT32];
+#pragma unroll
+        for (int kt = 0; kt < KT32; ++kt) ha[kt] = load8(&h[r][kt * 32 + 8 * q]);
+        __syncthreads();
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kt = 0; kt < KT32; ++kt) acc = mfma32(ha[kt], load8(&w2t[nt * 16 + r][kt * 32 + 8 * q]), acc);
+            const float b = bias[NH + nt * 16 + r];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) h[4 * q + j][nt * 16 + r] = f2bf(gelu(acc[j] + b));
+        }
+        __syncthreads();
+        // layer 3 (K = 128): output r of sample 4q + j
+        f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kt = 0; kt < KT32; ++kt) acc = mfma32(load8(&h[r][kt * 32 + 8 * q]), load8(&w3t[r][kt * 32 + 8 * q]), acc);
+        if (r < MPF_MLP_NOUT) {
+            const float b = bias[2 * NH + r];
+            const float sd = nrm[2 * MPF_MLP_NIN + MPF_MLP_NOUT + r], mu = nrm[2 * MPF_MLP_NIN + r];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint64_t si = base + 4 * q + j;
+                if (si < n) y[si * MPF_MLP_NOUT + r] = (acc[j] + b) * sd + mu;
+            }
+        }
+        __syncthreads();   // every read of H2 is done before the next tile writes H1
+    }
+}
+
 // Features of catalogue sources with exact FP64 TSC: one 27-cell stencil
 // cache per source, carried outward along the line of sight (the cached tsc()
 // of simplehyperoctree.h:1554-1644 that the integrator uses).
@@ -374,14 +504,6 @@ __global__ void normalise_kernel(const float* __restrict__ in, uint64_t n, int d
     if (i >= n * (uint64_t)dim) return;
     const int d = (int)(i % (uint64_t)dim);
     out[i] = (in[i] - mean[d]) / stdv[d];
-}
-
-__global__ void denormalise_kernel(float* __restrict__ y, uint64_t n, const float* __restrict__ mean,
-                                   const float* __restrict__ stdv) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n * (uint64_t)MPF_MLP_NOUT) return;
-    const int d = (int)(i % MPF_MLP_NOUT);
-    y[i] = y[i] * stdv[d] + mean[d];
 }
 
 // first guess = unlensed direction + Born deflection (features 1, 2) + the
@@ -454,6 +576,9 @@ struct MlpModel {
     bool norm_set = false;
     long step = 0;
     unsigned short *w1t = nullptr, *w2t = nullptr, *w2 = nullptr, *w3t = nullptr, *w3 = nullptr;
+    int cus = 0;                          // compute units of the device (inference grid)
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float last_ms = -1.f;                 // duration of the last inference kernel
 };
 
 static Bf16Weights weights_of(const MlpModel& M) {
@@ -472,6 +597,8 @@ void mlp_destroy(MlpModel* M) {
     if (!M) return;
     for (void* q : {(void*)M->p, (void*)M->g, (void*)M->m, (void*)M->v, (void*)M->wb, (void*)M->norm})
         if (q) (void)hipFree(q);
+    if (M->ev0) (void)hipEventDestroy(M->ev0);
+    if (M->ev1) (void)hipEventDestroy(M->ev1);
     delete M;
 }
 
@@ -479,9 +606,12 @@ bool mlp_create(MlpModel** out, hipStream_t st, uint64_t seed, std::string& err)
     MlpModel* M = new MlpModel;
     M->stream = st;
     const size_t nbw = (size_t)KIN * NH + 2 * NH * NH + 2 * NH * KOUT;
+    int dev = 0;
     if (hipMalloc((void**)&M->p, NPARAM * 4) || hipMalloc((void**)&M->g, NPARAM * 4) ||
         hipMalloc((void**)&M->m, NPARAM * 4) || hipMalloc((void**)&M->v, NPARAM * 4) ||
-        hipMalloc((void**)&M->wb, nbw * 2) || hipMalloc((void**)&M->norm, MPF_MLP_NNORM * 4)) {
+        hipMalloc((void**)&M->wb, nbw * 2) || hipMalloc((void**)&M->norm, MPF_MLP_NNORM * 4) ||
+        hipEventCreate(&M->ev0) || hipEventCreate(&M->ev1) || hipGetDevice(&dev) ||
+        hipDeviceGetAttribute(&M->cus, hipDeviceAttributeMultiprocessorCount, dev)) {
         err = "MLP allocation failed";
         mlp_destroy(M);
         return false;
@@ -578,12 +708,12 @@ static bool set_norm_from(MlpModel* M, const float* feat, const float* target, u
 }
 
 // loss sum and gradient (into M->g) of rows perm[0 .. nb) of the normalised set
+// (the loss is added to B.loss, which the caller clears)
 static void batch_gradient(MlpModel* M, Batch& B, const uint32_t* perm, uint64_t nb, float dscale) {
     hipStream_t st = M->stream;
-    hipLaunchKernelGGL(zero_f32, dim3(1), dim3(64), 0, st, B.loss.p, 1);
     const Bf16Weights W = weights_of(*M);
-    hipLaunchKernelGGL(mlp_tile_kernel<true>, dim3(nblk(nb, 64)), dim3(256), 0, st, W, B.xn.p, perm, nb, nullptr,
-                       B.tn.p, B.S, dscale, B.loss.p);
+    hipLaunchKernelGGL(mlp_train_tile_kernel, dim3(nblk(nb, 64)), dim3(256), 0, st, W, B.xn.p, perm, nb, B.tn.p, B.S,
+                       dscale, B.loss.p);
     const GradJob j1{B.S.x, B.S.d1, KIN, NH, OFF_W1, OFF_B1};
     const GradJob j2{B.S.h1, B.S.d2, NH, NH, OFF_W2, OFF_B2};
     const GradJob j3{B.S.h2, B.S.d3, NH, KOUT, OFF_W3, OFF_B3};
@@ -626,25 +756,25 @@ bool mlp_train(MlpModel* M, const float* feat, const float* target, uint64_t n, 
             err = "MLP permutation upload failed";
             return false;
         }
-        double esum = 0.0;
+        // the epoch's loss sums on the device and is read once
+        hipLaunchKernelGGL(zero_f32, dim3(1), dim3(64), 0, st, B.loss.p, 1);
         for (uint64_t b0 = 0; b0 < n; b0 += nbm) {
             const uint64_t nb = std::min<uint64_t>(nbm, n - b0);
             batch_gradient(M, B, B.perm.p + b0, nb, 1.0f / (float)(nb * MPF_MLP_NOUT));
-            if (loss_per_epoch) {
-                float l = 0.f;
-                if (hipMemcpyAsync(&l, B.loss.p, 4, hipMemcpyDeviceToHost, st) || hipStreamSynchronize(st)) {
-                    err = "MLP loss download failed";
-                    return false;
-                }
-                esum += (double)l;
-            }
             M->step++;
             const float c1 = 1.0f - std::pow(b1, (float)M->step), c2 = 1.0f - std::pow(b2, (float)M->step);
             hipLaunchKernelGGL(adam_kernel, dim3(nblk(NPARAM)), dim3(256), 0, st, M->p, M->g, M->m, M->v, lr, b1, b2,
                                eps, c1, c2);
             pack(M);
         }
-        if (loss_per_epoch) loss_per_epoch[ep] = (float)(esum / (double)(n * MPF_MLP_NOUT));
+        if (loss_per_epoch) {
+            float l = 0.f;
+            if (hipMemcpyAsync(&l, B.loss.p, 4, hipMemcpyDeviceToHost, st) || hipStreamSynchronize(st)) {
+                err = "MLP loss download failed";
+                return false;
+            }
+            loss_per_epoch[ep] = (float)((double)l / (double)(n * MPF_MLP_NOUT));
+        }
     }
     if (hipStreamSynchronize(st) != hipSuccess) { err = "MLP training failed"; return false; }
     return true;
@@ -657,6 +787,7 @@ bool mlp_gradient(MlpModel* M, const float* feat, const float* target, uint64_t 
     Batch B;
     if (!B.alloc(n, n)) { err = "MLP gradient buffers: allocation failed"; return false; }
     if (!upload_set(M, B, feat, target, n)) { err = "MLP batch upload failed"; return false; }
+    hipLaunchKernelGGL(zero_f32, dim3(1), dim3(64), 0, M->stream, B.loss.p, 1);
     batch_gradient(M, B, nullptr, n, 1.0f / (float)(n * MPF_MLP_NOUT));
     float l = 0.f;
     if (hipMemcpyAsync(grad, M->g, NPARAM * 4, hipMemcpyDeviceToHost, M->stream) ||
@@ -671,17 +802,22 @@ bool mlp_gradient(MlpModel* M, const float* feat, const float* target, uint64_t 
 bool mlp_infer_dev(MlpModel* M, const float* dfeat, uint64_t n, float* dout, std::string& err) {
     if (n == 0) return true;
     hipStream_t st = M->stream;
-    Tmp<float> xn;
-    if (!xn.alloc(n * MPF_MLP_NIN)) { err = "MLP inference buffer: allocation failed"; return false; }
-    hipLaunchKernelGGL(normalise_kernel, dim3(nblk(n * MPF_MLP_NIN)), dim3(256), 0, st, dfeat, n, MPF_MLP_NIN,
-                       M->norm, M->norm + MPF_MLP_NIN, xn.p);
-    hipLaunchKernelGGL(mlp_tile_kernel<false>, dim3(nblk(n, 64)), dim3(256), 0, st, weights_of(*M), xn.p, nullptr, n,
-                       dout, nullptr, Saved{}, 0.f, nullptr);
-    hipLaunchKernelGGL(denormalise_kernel, dim3(nblk(n * MPF_MLP_NOUT)), dim3(256), 0, st, dout, n,
-                       M->norm + 2 * MPF_MLP_NIN, M->norm + 2 * MPF_MLP_NIN + MPF_MLP_NOUT);
-    if (hipStreamSynchronize(st) != hipSuccess) { err = "MLP inference failed"; return false; }
+    // two 256-lane blocks per compute unit (60 KB of LDS each), or fewer for a small batch
+    const uint64_t groups = (n + 63) / 64;
+    const unsigned grid = (unsigned)std::min<uint64_t>(groups, 2ull * (uint64_t)std::max(M->cus, 1));
+    if (hipEventRecord(M->ev0, st) != hipSuccess) { err = "MLP inference: event"; return false; }
+    hipLaunchKernelGGL(mlp_infer_kernel, dim3(grid), dim3(256), 0, st, weights_of(*M), M->norm, dfeat, n, dout);
+    if (hipGetLastError() != hipSuccess || hipEventRecord(M->ev1, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        err = "MLP inference failed";
+        return false;
+    }
+    float ms = -1.f;
+    if (hipEventElapsedTime(&ms, M->ev0, M->ev1) == hipSuccess) M->last_ms = ms;
     return true;
 }
+
+float mlp_last_ms(const MlpModel* M) { return M ? M->last_ms : -1.f; }
 
 hipError_t launch_features(const DevTree& t, const double* src, uint64_t n, double length, float* feat,
                            hipStream_t st) {
@@ -697,6 +833,11 @@ hipError_t launch_seed(const double* src, const float* feat, const float* pred, 
 
 hipError_t launch_mfma_probe(const float* a, const float* b, float* c, hipStream_t st) {
     hipLaunchKernelGGL(mfma_probe_kernel, dim3(1), dim3(64), 0, st, a, b, c);
+    return hipGetLastError();
+}
+
+hipError_t launch_mfma_probe32(const float* a, const float* b, float* c, int split, hipStream_t st) {
+    hipLaunchKernelGGL(mfma_probe32_kernel, dim3(1), dim3(64), 0, st, a, b, c, split);
     return hipGetLastError();
 }
 

@@ -36,7 +36,8 @@ EXPORTS = ("mpf_create", "mpf_destroy", "mpf_last_error", "mpf_set_precision", "
            "mpf_catalogue_solve_dev", "mpf_catalogue_previous", "mpf_catalogue_write",
            "mpf_catalogue_write_previous", "mpf_mlp_init", "mpf_mlp_state", "mpf_mlp_train", "mpf_mlp_gradient",
            "mpf_mlp_infer", "mpf_mlp_infer_dev", "mpf_catalogue_features", "mpf_catalogue_features_dev",
-           "mpf_catalogue_seed", "mpf_catalogue_seed_dev", "mpf_mfma_probe", "mpf_integrate_trajectory",
+           "mpf_catalogue_seed", "mpf_catalogue_seed_dev", "mpf_mfma_probe", "mpf_mfma_probe32", "mpf_mlp_last_ms",
+           "mpf_integrate_trajectory",
            "mpf_cosmology_acquire", "mpf_cosmology_correct", "mpf_cosmology_prepare", "mpf_cosmology_last_error",
            "mpf_cone_last_error", "mpf_cones_fullsky", "mpf_cones_narrow", "mpf_narrow_cone_info", "mpf_cone_inside",
            "mpf_cone_select", "mpf_cone_targets", "mpf_hmaps_cone_pixels", "mpf_launch_angles", "mpf_launch_towards",
@@ -212,6 +213,8 @@ def load_library(path=LIB_PATH):
     lib.mpf_catalogue_seed.argtypes = [vp, vp, u64, i32, vp]
     lib.mpf_catalogue_seed_dev.argtypes = [vp, vp, u64, i32, vp]
     lib.mpf_mfma_probe.argtypes = [vp, vp, vp, vp]
+    lib.mpf_mfma_probe32.argtypes = [vp, vp, vp, ctypes.c_int, vp]
+    lib.mpf_mlp_last_ms.argtypes = [vp, ctypes.POINTER(dbl)]
     lib.mpf_integrate_trajectory.argtypes = [vp, vp, u64, ctypes.POINTER(Stop), u64, vp, vp]
     cp, dbl = ctypes.c_char_p, ctypes.c_double
     lib.mpf_cosmology_acquire.argtypes = [cp, cp, cp, dbl, vp, vp, vp, vp, vp, vp, vp, u64, vp]
@@ -941,6 +944,21 @@ class Context:
         c = np.zeros((16, 16), dtype=np.float32)
         self._check(self.lib.mpf_mfma_probe(self.h, _p(a), _p(b), _p(c)), "mpf_mfma_probe")
         return c
+
+    def mfma_probe32(self, a, b, layout=0):
+        """16 x 32 by 32 x 16 product through one 16x16x32 BF16 MFMA with the
+        given operand layout (0: the kernels'; 1: the K-halves alternative)."""
+        a = np.ascontiguousarray(a, dtype=np.float32).reshape(16, 32)
+        b = np.ascontiguousarray(b, dtype=np.float32).reshape(32, 16)
+        c = np.zeros((16, 16), dtype=np.float32)
+        self._check(self.lib.mpf_mfma_probe32(self.h, _p(a), _p(b), int(layout), _p(c)), "mpf_mfma_probe32")
+        return c
+
+    def mlp_last_ms(self):
+        """Duration (ms) of the last inference kernel."""
+        ms = ctypes.c_double(-1.0)
+        self._check(self.lib.mpf_mlp_last_ms(self.h, ctypes.byref(ms)), "mpf_mlp_last_ms")
+        return ms.value
 
     def hmaps_launch_dev(self, nside, pix_ptr, n, in9_ptr):
         self._check(self.lib.mpf_hmaps_launch_dev(self.h, int(nside), ctypes.c_void_p(pix_ptr), ctypes.c_uint64(n),

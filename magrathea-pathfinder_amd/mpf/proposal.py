@@ -44,6 +44,21 @@ def cone_sources(n, axis_phi_deg, axis_theta_deg, aperture_deg, dmin, dmax, seed
     return src
 
 
+def field_sources(n, axis_phi_deg, axis_theta_deg, aperture_deg, dmin, dmax, seed, vscale=1e-3):
+    """n sources uniform in comoving volume inside the narrow field
+    (aperture x aperture, the reference's narrow-cone shape) around
+    (axis_phi, axis_theta), at comoving distances [dmin, dmax] (box units):
+    the same sky area as synth.narrow_field_grid.  Returns n x 6."""
+    from . import synth
+    rng = np.random.default_rng(seed)
+    r = np.cbrt(dmin ** 3 + rng.uniform(0, 1, n) * (dmax ** 3 - dmin ** 3))
+    ph, th = synth.narrow_field_uniform(axis_phi_deg, axis_theta_deg, aperture_deg, n, seed + 1)
+    src = np.zeros((n, 6))
+    src[:, :3] = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], 1) * r[:, None]
+    src[:, 3:] = rng.normal(scale=vscale, size=(n, 3))
+    return src
+
+
 def deflections(out):
     """observed - unlensed direction (arcsec along e_phi and e_theta) of
     root-finder records (MPF_NCAT)."""

@@ -123,6 +123,52 @@ def cone_grid(phi0_deg, theta0_deg, aperture_deg, nside):
     return pp.ravel(), tt.ravel()
 
 
+def narrow_field_frame(phi0_deg, theta0_deg):
+    """Orthonormal frame (rows e_x, e_y, e_z) of a narrow field whose local
+    +x axis points at (phi0, theta0): e_y along e_phi, e_z along -e_theta."""
+    p, t = math.radians(phi0_deg), math.radians(theta0_deg)
+    return np.array([[math.sin(t) * math.cos(p), math.sin(t) * math.sin(p), math.cos(t)],
+                     [-math.sin(p), math.cos(p), 0.0],
+                     [-math.cos(t) * math.cos(p), -math.cos(t) * math.sin(p), math.sin(t)]])
+
+
+def _field_to_world(phi0_deg, theta0_deg, lphi, ltheta):
+    local = np.stack([np.sin(ltheta) * np.cos(lphi), np.sin(ltheta) * np.sin(lphi), np.cos(ltheta)], 1)
+    d = local @ narrow_field_frame(phi0_deg, theta0_deg)
+    return np.arctan2(d[:, 1], d[:, 0]), np.arccos(np.clip(d[:, 2], -1.0, 1.0))
+
+
+def narrow_field_area_deg2(aperture_deg):
+    """Solid angle (deg^2) of a narrow field with semi-angles
+    thetay = thetaz = aperture / 2: 4 thetay sin(thetaz), the sky portion of
+    Miscellaneous::GenerateNarrowCones (miscellaneous.h:332)."""
+    h = math.radians(aperture_deg) / 2
+    return 4 * h * math.sin(h) * (180 / math.pi) ** 2
+
+
+def narrow_field_grid(phi0_deg, theta0_deg, aperture_deg, nside):
+    """Regular nside x nside grid over the narrow field of the reference's
+    narrow cones (miscellaneous.h:338-343: |phi| < thetay and
+    |theta - pi/2| < thetaz in the field's own frame, around its +x axis),
+    here with thetay = thetaz = aperture / 2 and the axis at (phi0, theta0):
+    an aperture x aperture field of narrow_field_area_deg2(aperture) deg^2
+    (99.9 for 10 deg).  Cell-centred local angles; returns world (phi, theta)."""
+    half = math.radians(aperture_deg) / 2
+    u = (np.arange(nside) + 0.5) / nside * 2 * half - half
+    lp, lt = np.meshgrid(u, math.pi / 2 + u, indexing="ij")
+    return _field_to_world(phi0_deg, theta0_deg, lp.ravel(), lt.ravel())
+
+
+def narrow_field_uniform(phi0_deg, theta0_deg, aperture_deg, n, seed):
+    """n directions uniform in solid angle over the narrow field of
+    narrow_field_grid: local phi uniform, local cos(theta) uniform."""
+    rng = np.random.default_rng(seed)
+    half = math.radians(aperture_deg) / 2
+    lp = rng.uniform(-half, half, n)
+    lt = np.arccos(rng.uniform(-math.sin(half), math.sin(half), n))
+    return _field_to_world(phi0_deg, theta0_deg, lp, lt)
+
+
 def cone_uniform(phi0_deg, theta0_deg, aperture_deg, n, seed):
     """n directions uniform (in solid angle) over the square aperture."""
     rng = np.random.default_rng(seed)
@@ -518,11 +564,30 @@ def lightcone_tree(base_input_level, box_mpch, sigma, threshold, max_input_level
 
 
 WORKLOADS = {
-    # uniform level-7 grid, CIC, 64x64 rays over 5x5 deg to z = 0.5
+    # uniform level-7 grid, CIC, 64x64 rays over a 5x5 deg narrow field to z = 0.5
     "uniform128_cic": dict(base_input_level=7, box_mpch=3000.0, sigma=1.0, threshold=1e30, max_input_level=7, seed=42,
                  order=1, nside=64, aperture=5.0, z_stop=0.5, phi0=20.0, theta0=20.0),
     # level-8 base refined to level 11 where the overdensity exceeds 8, TSC,
-    # 1024x1024 rays over 10x10 deg to z = 2
+    # 1024x1024 rays over a 10x10 deg narrow field to z = 2
     "amr256_tsc": dict(base_input_level=8, box_mpch=8000.0, sigma=1.5, threshold=8.0, max_input_level=11, seed=42,
                  noise=1.0, decay=0.75, order=2, nside=1024, aperture=10.0, z_stop=2.0, phi0=20.0, theta0=20.0),
 }
+
+
+def build_workload(name, log=None):
+    """The named workload of WORKLOADS: (parameters, octree records, FP64
+    fields, box length in metres, cosmology table, launch records n x 9 of
+    the rays on the regular grid over the narrow field)."""
+    import time
+    w = dict(WORKLOADS[name])
+    t0 = time.time()
+    table = flat_lcdm_table(h=0.7, omega_m=0.3, a_min=0.05, n=4096)
+    rec, f64, length = lightcone_tree(w["base_input_level"], w["box_mpch"], w["sigma"], w["threshold"],
+                                      w["max_input_level"], w["seed"], noise=w.get("noise", 0.0),
+                                      decay=w.get("decay", 0.0), table=table, device="cpu")
+    ph, th = narrow_field_grid(w["phi0"], w["theta0"], w["aperture"], w["nside"])
+    in9 = launch_photons(ph, th)
+    if log:
+        log(f"workload {name}: {rec.size} cells, {in9.shape[0]} rays, built in {time.time() - t0:.1f}s")
+    return w, rec, f64, length, table, in9
+

@@ -59,6 +59,47 @@ def test_mfma_operand_layout():
     c.close()
 
 
+def test_mfma32_operand_layout():
+    """gfx950's 16x16x32 product with the kernels' layout gives a b; a B
+    operand whose slots pair with other K than A's does not (and a
+    transposed operand would not either)."""
+    print("[case] mfma 16x16x32 probe", flush=True)
+    c = context()
+    rng = np.random.default_rng(2)
+    a = mlp.bf16(rng.normal(size=(16, 32)).astype(np.float32))
+    b = mlp.bf16(rng.normal(size=(32, 16)).astype(np.float32))
+    bound = np.abs(a) @ np.abs(b)
+    out = c.mfma_probe32(a, b, 0)
+    assert np.all(np.abs(out - a @ b) <= 1e-6 * bound)
+    assert np.max(np.abs(out - (a @ b).T)) > 0.1
+    bad = c.mfma_probe32(a, b, 1)
+    assert np.max(np.abs(bad - a @ b)) > 0.1
+    c.close()
+
+
+def test_inference_kernel_time_and_large_batch():
+    """The persistent inference kernel on 1 M samples: equal to the oracle's
+    forward pass (1e-2 of the outputs' rms, as above) on a sample of rows,
+    the same outputs from a second call, its HIP-event duration reported."""
+    print("[case] mlp inference, 1048576 samples", flush=True)
+    c = context()
+    c.mlp_init(0)
+    state = random_state(21)
+    c.mlp_state(state)
+    x, _ = data(22, 1048576)
+    y = c.mlp_infer(x)
+    ms = c.mlp_last_ms()
+    y2 = c.mlp_infer(x)
+    np.testing.assert_array_equal(y, y2)
+    rows = np.random.default_rng(23).choice(len(x), 4096, replace=False)
+    yo = mlp.predict(state, x[rows], mixed=True)
+    assert np.max(np.abs(y[rows] - yo)) <= 1e-2 * np.sqrt(np.mean(yo ** 2))
+    flop = len(x) * 2 * (10 * 128 + 128 * 128 + 128 * 2)
+    print(f"  inference kernel {ms:.4f} ms: {flop / (ms / 1e3) / 1e12:.1f} TFLOP/s", flush=True)
+    assert 0 < ms < 50
+    c.close()
+
+
 def test_init_state():
     print("[case] mlp init", flush=True)
     c = context()
