@@ -119,6 +119,109 @@ def train(ctx, feat, target, val_feat, val_target, epochs=100, batch=256, lr=1e-
                 use_network=best[0] < rms0)
 
 
+C2 = 299792458.0 ** 2
+
+
+def point_features(ctx, src, length):
+    """The alternative feature set (n x 10): the unlensed angles phi, theta
+    (rad), the comoving distance D (box units), the transverse gradient of
+    phi (along e_phi, e_theta) at chi = D/4, D/2, 3D/4 in arcsec
+    (2 D L grad_perp(phi) / c^2, L the box length in metres) and phi / c^2 at
+    D/2 (units of 1e-5): raw samples of the octree on the straight line of
+    sight (TSC through mpf_interpolate), no Born sum.  The network then learns
+    the whole deflection (targets: deflections())."""
+    src = np.asarray(src, dtype=np.float64).reshape(-1, 6)
+    x = src[:, :3]
+    D = np.linalg.norm(x, axis=1)
+    th = np.arccos(x[:, 2] / D)
+    ph = np.arctan2(x[:, 1], x[:, 0])
+    ep = np.stack([-np.sin(ph), np.cos(ph), np.zeros_like(ph)], 1)
+    et = np.stack([np.cos(ph) * np.cos(th), np.sin(ph) * np.cos(th), -np.sin(th)], 1)
+    n = x / D[:, None]
+    f = np.zeros((len(src), capi.MLP_NIN), dtype=np.float32)
+    f[:, 0], f[:, 1], f[:, 2] = ph, th, D
+    prec = ctx.precision
+    if prec == capi.PREC_SPLIT:          # interpolation is an exact-tier call
+        ctx.set_precision(capi.PREC_FP64)
+    try:
+        g3 = [ctx.interpolate(2, n * (frac * D)[:, None]) for frac in (0.25, 0.5, 0.75)]
+    finally:
+        if prec != ctx.precision:
+            ctx.set_precision(prec)
+    for k, frac in enumerate((0.25, 0.5, 0.75)):
+        g = g3[k]
+        grad = g[:, 2:5]
+        s = 2.0 * D * length / C2 * ARCSEC
+        f[:, 3 + 2 * k] = s * np.sum(grad * ep, 1)
+        f[:, 4 + 2 * k] = s * np.sum(grad * et, 1)
+        if frac == 0.5:
+            f[:, 9] = g[:, 1] / C2 * 1.0e5
+    return f
+
+
+def unlensed(src):
+    src = np.asarray(src, dtype=np.float64).reshape(-1, 6)
+    D = np.linalg.norm(src[:, :3], axis=1)
+    return np.arctan2(src[:, 1], src[:, 0]), np.arccos(src[:, 2] / D)
+
+
+def guess_from_deflection(src, defl):
+    """First guess = unlensed direction + a deflection (arcsec along e_phi, e_theta)."""
+    ph, th = unlensed(src)
+    defl = np.asarray(defl, dtype=np.float64)
+    return np.stack([ph + defl[:, 0] / (ARCSEC * np.sin(th)), th + defl[:, 1] / ARCSEC], 1)
+
+
+def compare_feature_sets(ctx, length, train_src, val_src, test_src, accuracy=1e-8, epochs=20, batch=1024,
+                         lr=1e-3, seed=0):
+    """The two feature sets of the learned first guess on the same
+    root-finder pairs: the Born-residual set (catalogue_features; the network
+    learns what the Born estimate leaves) and the raw point-sample set
+    (point_features; the network learns the whole deflection).  Trained on
+    the cones of train_src, early-stopped on val_src, judged on test_src:
+    rms of the first guess's error there (arcsec) and the Newton iterations
+    and rays from each first guess.  Returns a dict."""
+    def pairs(src):
+        out, _ = ctx.catalogue_solve(src, accuracy=accuracy)
+        ok = out[:, capi.CAT["status"]] == capi.CAT_OK
+        fb = ctx.catalogue_features(src[ok])
+        fp = point_features(ctx, src[ok], length)
+        d = deflections(out[ok]).astype(np.float32)
+        return src[ok], fb, fp, d, out[ok]
+    tr = [pairs(s) for s in train_src]
+    va = pairs(val_src)
+    te = pairs(test_src)
+    fb = np.concatenate([t[1] for t in tr])
+    fp = np.concatenate([t[2] for t in tr])
+    d = np.concatenate([t[3] for t in tr])
+    res = {"training_pairs": int(len(d))}
+    # Born-residual network: targets = deflection - Born estimate
+    fit_b = train(ctx, fb, (d - fb[:, 1:3]).astype(np.float32), va[1], va[3] - va[1][:, 1:3], epochs=epochs,
+                  batch=batch, lr=lr, seed=seed)
+    state_b = ctx.mlp_state()
+    pred_b = ctx.mlp_infer(te[1])
+    fit_p = train(ctx, fp, d, va[2], va[3], epochs=epochs, batch=batch, lr=lr, seed=seed)
+    pred_p = ctx.mlp_infer(te[2])
+    ctx.mlp_state(state_b)
+    born = te[1][:, 1:3].astype(np.float64)
+    truth = te[3].astype(np.float64)
+    guesses = {"plain": np.zeros_like(truth), "born": born, "born+network": born + pred_b,
+               "points network": pred_p.astype(np.float64)}
+    src_t = te[0]
+    for name, defl in guesses.items():
+        rms = float(np.sqrt(np.mean((defl - truth) ** 2)))
+        out, rays = ctx.catalogue_solve(src_t, init2=guess_from_deflection(src_t, defl), accuracy=accuracy,
+                                        jacobian_all_rounds=True)
+        ok = out[:, capi.CAT["status"]] == capi.CAT_OK
+        res[name] = {"rms_arcsec": rms, "mean_iterations": float(out[ok, capi.CAT["iter"]].mean()),
+                     "rays": int(rays), "converged": int(ok.sum()), "sources": int(len(src_t))}
+    res["held_out"] = {"born_rms": fit_b["val_rms_born"], "born+network_rms": fit_b["val_rms_network"],
+                       "points_network_rms": fit_p["val_rms_network"], "deflection_rms": fit_p["val_rms_born"]}
+    res["loss_born_net"] = [float(fit_b["loss"][0]), float(fit_b["loss"][-1])]
+    res["loss_points_net"] = [float(fit_p["loss"][0]), float(fit_p["loss"][-1])]
+    return res
+
+
 def solve_seeded(ctx, src, use_network=True, **newton):
     """Root-finder started from the first guesses: (records, rays)."""
     init = ctx.catalogue_seed(src, use_network)

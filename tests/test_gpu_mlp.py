@@ -289,8 +289,48 @@ def test_proposal_reduces_newton_rounds(precision):
           np.sqrt(np.mean((resid - pred) ** 2, 0)), flush=True)
     print("[gain] mean iterations plain", plain[ok, it].mean(), "born", born[ok, it].mean(), "seeded",
           seeded[ok, it].mean(), "rays", rays_plain, rays_born, rays_seeded, flush=True)
+    # the network against the Born estimate alone, with the same Newton finish
+    net, rays_net = proposal.solve_seeded(c, src, use_network=True, accuracy=acc, jacobian_all_rounds=True)
+    okn = ok & (net[:, st] == capi.CAT_OK)
+    gain = born[okn, it].mean() - net[okn, it].mean()
+    print(f"[gain] network - Born: {net[okn, it].mean():.3f} - {born[okn, it].mean():.3f} iterations "
+          f"({gain:+.3f} saved by the network), rays {rays_net} vs {rays_born}; held-out rms network "
+          f"{fit['val_rms_network']:.4f} vs Born {fit['val_rms_born']:.4f} arcsec, use_network {fit['use_network']}",
+          flush=True)
     for f in ("phi_obs", "theta_obs"):
         assert np.max(np.abs(plain[ok, capi.CAT[f]] - seeded[ok, capi.CAT[f]])) < 20 * acc
     assert seeded[ok, it].mean() < plain[ok, it].mean() - 0.5
     assert rays_seeded < 0.6 * rays_plain
+    # the gate: the first guess in use costs at most noise against Born alone,
+    # and the network is in use only where it beat Born on the held-out cone
+    assert seeded[ok, it].mean() <= born[ok, it].mean() + 0.05
+    if fit["use_network"]:
+        assert fit["val_rms_network"] < fit["val_rms_born"]
+    c.close()
+
+
+def test_feature_sets_on_held_out_cone():
+    """The two feature sets of the learned first guess on the same pairs
+    (4 training cones of 16,384 sources, a validation cone, a test cone):
+    both networks learn (the loss falls), every first guess still converges
+    to the accuracy, and the comparison is printed: rms of each first guess's
+    error and the Newton iterations it leaves."""
+    print("[case] feature sets: Born-residual vs point samples", flush=True)
+    rec, f64, length = lightcone()
+    c = capi.Context(length, order=capi.TSC, precision=capi.PREC_SPLIT, nsteps=cases.NSTEPS)
+    c.load_records(rec)
+    c.load_fields64(f64)
+    c.set_cosmology(cases.cosmology())
+    axes = ((70.0, 40.0), (140.0, 60.0), (220.0, 35.0), (300.0, 110.0))
+    train_src = [proposal.cone_sources(16384, p, t, 6.0, 0.08, 0.45, seed=300 + k) for k, (p, t) in enumerate(axes)]
+    val = proposal.cone_sources(4096, 250.0, 80.0, 6.0, 0.08, 0.45, seed=310)
+    test = proposal.cone_sources(4096, 20.0, 20.0, 6.0, 0.08, 0.45, seed=311)
+    r = proposal.compare_feature_sets(c, length, train_src, val, test, accuracy=1e-8, epochs=20, batch=1024,
+                                      lr=1e-3, seed=5)
+    for k, v in r.items():
+        print(f"[compare] {k}: {v}", flush=True)
+    assert r["loss_born_net"][1] < r["loss_born_net"][0] and r["loss_points_net"][1] < r["loss_points_net"][0]
+    for name in ("plain", "born", "born+network", "points network"):
+        assert r[name]["converged"] >= 0.98 * r[name]["sources"], name
+    assert r["born"]["mean_iterations"] < r["plain"]["mean_iterations"]
     c.close()
