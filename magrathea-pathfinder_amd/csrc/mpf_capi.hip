@@ -13,6 +13,7 @@
 #include "mpf_tree.h"
 #include "mpf_healpix.h"
 #include "mpf_order.h"
+#include "mpf_plane.h"
 #include "mpf_propagate.h"
 #include "mpf_cone.h"
 #include "mpf_mlp.h"
@@ -1918,15 +1919,9 @@ static int jacobian_bundle_plane(mpf_ctx* c, const double* phi_theta, uint64_t n
                     }
                     break;
                 }
-                int64_t first = 0, cnt = len;  // upper_bound of v along kr
-                while (cnt > 0) {
-                    const int64_t half = cnt / 2;
-                    if (v < along(first + half)) cnt = half;
-                    else { first += half + 1; cnt -= half + 1; }
-                }
-                const int64_t id = first - (first > 0);
-                const double next = along(id + 1), prev = along(id);
-                const double f = (next - v) / (next - prev);
+                int64_t id = 0;
+                double f = 0.0;
+                mpf::plane_segment(len, v, along, &id, &f);
                 const double* a = tr + (uint64_t)MPF_NOUT * id;
                 const double* b = a + MPF_NOUT;
                 fp[r][it][0] = a[MPF_OUT_X] * f + b[MPF_OUT_X] * (1 - f);

@@ -1743,7 +1743,10 @@ void mpfo_dbetadtheta(const mpfo_octree* o, int order, int precision, const doub
                 if (before) len = half;
                 else { first = first + half + 1; len = len - half - 1; }
             }
-            const uint64_t id = first - (first > 0);
+            uint64_t id = first - (first > 0);
+            /* the last element's value equal to ref: upper_bound ends at t.n and the
+             * reference reads one element past the trajectory; take the last segment */
+            if (id + 1 >= t.n) id = t.n - 2;
             const double next = bundle_param_ki(&t.p[id + 1], kind, kr), prev = bundle_param_ki(&t.p[id], kind, kr);
             const double f = (next - ref[r]) / (next - prev);
             for (int d = 0; d < 3; ++d)

@@ -44,3 +44,17 @@ def test_compiled_caller_usage_and_missing_file(tmp_path):
     r = subprocess.run([caller(), "narrow", str(tmp_path) + "/", "3", str(tmp_path / "x.bin")], capture_output=True,
                        text=True)
     assert r.returncode == 1 and r.stderr.strip()
+
+
+def test_plane_segment_clamps_to_last_segment():
+    """The plane crossing of a bundle ray (csrc/mpf_plane.h, host-compiled):
+    a plane value equal to the last element's projection takes the last
+    segment with weight 0 (the last element), where upper_bound ends past the
+    trajectory; interior values interpolate within their segment."""
+    exe = os.path.join(HERE, "capi", "_bin", "plane_segment_test")
+    if not os.path.exists(exe):
+        subprocess.run(["make", "-C", os.path.join(HERE, "capi")], check=True, capture_output=True)
+    r = subprocess.run([exe, "3", "2", "1.5", "0", "--", "0", "1", "2", "3"], capture_output=True, text=True,
+                       check=True)
+    rows = [tuple(float(x) for x in line.split()) for line in r.stdout.split("\n") if line]
+    assert rows == [(2, 0.0), (2, 1.0), (1, 0.5), (0, 1.0)]
