@@ -53,11 +53,29 @@ constexpr uint32_t SLOT_MASK = 0x7FFFFFFFu;
 // address uniform across the wave become scalar loads (the blocks are not
 // written while the integrators run)
 typedef const __attribute__((address_space(4))) float ConstF;
+typedef const __attribute__((address_space(4))) double ConstD;
 // blocks per evaluation the integrator reads with scalar loads
 #ifndef MPF_SCALAR_BLOCKS
 #define MPF_SCALAR_BLOCKS 1
 #endif
 constexpr int SCALAR_BLOCKS = MPF_SCALAR_BLOCKS;
+// ... for the Jacobian samples: 0, their lanes spread over more blocks than
+// the ray's own cell, and the scalar walk then costs more than it saves (the
+// 4 M-ray Jacobian batch: 402 ms with vector loads only, 437 ms with one
+// scalar block, 469 ms with two; bit-identical results)
+#ifndef MPF_JAC_SCALAR_BLOCKS
+#define MPF_JAC_SCALAR_BLOCKS 0
+#endif
+constexpr int JAC_SCALAR_BLOCKS = MPF_JAC_SCALAR_BLOCKS;
+// ... and for the ray's own fields in the kernel with surfaces: 0 (with it
+// the surfaces kernel spills 76 B/lane and the same batch takes 548 ms)
+#ifndef MPF_SURF_SCALAR_BLOCKS
+#define MPF_SURF_SCALAR_BLOCKS 0
+#endif
+constexpr int SURF_SCALAR_BLOCKS = MPF_SURF_SCALAR_BLOCKS;
+// device memory left free by the optional face-neighbour table: room for
+// 16 M rays with a surface (about 11 GB) and nside-4096 maps (12.9 GB)
+constexpr uint64_t NBR_RESERVE = 24ull << 30;
 
 struct SplitView {
     const uint32_t* tslot;        // cell -> block slot (| PARTIAL_BIT) or NIL
@@ -439,7 +457,7 @@ __device__ __forceinline__ bool split_eval(const DevTree& t, const SplitView& sv
 // from the cached cell's face neighbour (lookup_near).
 constexpr uint32_t JC_REUSE = 0x100u, JC_VALID = 0x200u;
 
-template <int NG>
+template <int NG, int SB>
 __device__ __forceinline__ bool jac_eval(const DevTree& t, const SplitView& sv, uint4& cell, uint32_t& cid,
                                          uint32_t& lw, double x, double y, double z, float* out) {
     const uint32_t kx32 = axis_key32(x), ky32 = axis_key32(y), kz32 = axis_key32(z);
@@ -465,21 +483,47 @@ __device__ __forceinline__ bool jac_eval(const DevTree& t, const SplitView& sv, 
         cid = g;
         lw = fl | JC_VALID | ((word & PARTIAL_BIT) ? 0u : JC_REUSE);
     }
-    const unsigned char* blk = sv.blocks + (uint64_t)c.w * BLOCK_BYTES;
-    const double4 h = *reinterpret_cast<const double4*>(blk);
-    const float u = (float)((x - h.x) * h.w);
-    const float v = (float)((y - h.y) * h.w);
-    const float w = (float)((z - h.z) * h.w);
-    const float4* cf = reinterpret_cast<const float4*>(blk + BLOCK_DOUBLES * 8);
+    // The wave evaluates the distinct blocks of its lanes (SB at most) from
+    // scalar loads, as split_eval does; the rest read their block with vector
+    // loads.  Both paths compute the same offsets and horner27 sums.
+    unsigned long long pend = SB ? __ballot(true) : 0ull;
+    for (int it = 0; it < SB && pend; ++it) {
+        const uint32_t s0 = __builtin_amdgcn_readlane(c.w, __builtin_ctzll(pend));
+        const unsigned char* b0 = sv.blocks + (uint64_t)s0 * BLOCK_BYTES;
+        ConstD* hd = (ConstD*)b0;
+        const double hx = hd[0], hy = hd[1], hz = hd[2], hw = hd[3];
+        ConstF* cs = (ConstF*)(b0 + BLOCK_DOUBLES * 8);
+        const bool mine = c.w == s0;
+        const float u = (float)((x - hx) * hw);
+        const float v = (float)((y - hy) * hw);
+        const float w = (float)((z - hz) * hw);
 #pragma unroll
-    for (int f = 1; f <= NG; ++f) {
-        float cc[FSTRIDE];
+        for (int f = 1; f <= NG; ++f) {
+            float cc[FSTRIDE];
 #pragma unroll
-        for (int i = 0; i < FSTRIDE / 4; ++i) {
-            const float4 q = cf[f * (FSTRIDE / 4) + i];
-            cc[4 * i] = q.x; cc[4 * i + 1] = q.y; cc[4 * i + 2] = q.z; cc[4 * i + 3] = q.w;
+            for (int i = 0; i < NCOEF; ++i) cc[i] = cs[f * FSTRIDE + i];
+            const float r = horner27(cc, u, v, w);
+            out[f - 1] = mine ? r : out[f - 1];
         }
-        out[f - 1] = horner27(cc, u, v, w);
+        pend &= ~__ballot(mine);
+    }
+    if (!SB || ((pend >> __lane_id()) & 1ull)) {
+        const unsigned char* blk = sv.blocks + (uint64_t)c.w * BLOCK_BYTES;
+        const double4 h = *reinterpret_cast<const double4*>(blk);
+        const float u = (float)((x - h.x) * h.w);
+        const float v = (float)((y - h.y) * h.w);
+        const float w = (float)((z - h.z) * h.w);
+        const float4* cf = reinterpret_cast<const float4*>(blk + BLOCK_DOUBLES * 8);
+#pragma unroll
+        for (int f = 1; f <= NG; ++f) {
+            float cc[FSTRIDE];
+#pragma unroll
+            for (int i = 0; i < FSTRIDE / 4; ++i) {
+                const float4 q = cf[f * (FSTRIDE / 4) + i];
+                cc[4 * i] = q.x; cc[4 * i + 1] = q.y; cc[4 * i + 2] = q.z; cc[4 * i + 3] = q.w;
+            }
+            out[f - 1] = horner27(cc, u, v, w);
+        }
     }
     return true;
 }
@@ -722,19 +766,19 @@ integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
 #pragma unroll
         for (int i = 0; i < 9; ++i) { ksum[i] = kout[i]; kin[i] = R.core[i] + (0.5 * dl) * kout[i]; }
         float F[5];
-        bool ok = split_eval<4, RC, SURF ? 0 : SCALAR_BLOCKS>(t, sv, I, kin[CX], kin[CY], kin[CZ], F);
+        bool ok = split_eval<4, RC, SURF ? SURF_SCALAR_BLOCKS : SCALAR_BLOCKS>(t, sv, I, kin[CX], kin[CY], kin[CZ], F);
         if (!ok) { F[0] = F[1] = F[2] = F[3] = 0.f; }
         dadt = cosmo_slope(kin[CT], A.ct, A.cdadt, A.cslope, A.ncos, R.cs);
         rhs_split(kin, F, dadt, inv_scale, kout);
 #pragma unroll
         for (int i = 0; i < 9; ++i) { ksum[i] += 2.0 * kout[i]; kin[i] = R.core[i] + (0.5 * dl) * kout[i]; }
-        ok = split_eval<4, RC, SURF ? 0 : SCALAR_BLOCKS>(t, sv, I, kin[CX], kin[CY], kin[CZ], F);
+        ok = split_eval<4, RC, SURF ? SURF_SCALAR_BLOCKS : SCALAR_BLOCKS>(t, sv, I, kin[CX], kin[CY], kin[CZ], F);
         if (!ok) { F[0] = F[1] = F[2] = F[3] = 0.f; }
         dadt = cosmo_slope(kin[CT], A.ct, A.cdadt, A.cslope, A.ncos, R.cs);
         rhs_split(kin, F, dadt, inv_scale, kout);
 #pragma unroll
         for (int i = 0; i < 9; ++i) { ksum[i] += 2.0 * kout[i]; kin[i] = R.core[i] + dl * kout[i]; }
-        ok = split_eval<4, RC, SURF ? 0 : SCALAR_BLOCKS>(t, sv, I, kin[CX], kin[CY], kin[CZ], F);
+        ok = split_eval<4, RC, SURF ? SURF_SCALAR_BLOCKS : SCALAR_BLOCKS>(t, sv, I, kin[CX], kin[CY], kin[CZ], F);
         if (!ok) { F[0] = F[1] = F[2] = F[3] = 0.f; }
         dadt = cosmo_slope(kin[CT], A.ct, A.cdadt, A.cslope, A.ncos, R.cs);
         rhs_split(kin, F, dadt, inv_scale, kout);
@@ -744,7 +788,7 @@ integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
         // fields at the new point
         const bool inside = !(p[CA] < 0.0) && p[CX] > -2.0 && p[CX] < 2.0 && p[CY] > -2.0 && p[CY] < 2.0 &&
                             p[CZ] > -2.0 && p[CZ] < 2.0;
-        bool cont = inside && split_eval<5, RC, SURF ? 0 : SCALAR_BLOCKS>(t, sv, I, p[CX], p[CY], p[CZ], F);
+        bool cont = inside && split_eval<5, RC, SURF ? SURF_SCALAR_BLOCKS : SCALAR_BLOCKS>(t, sv, I, p[CX], p[CY], p[CZ], F);
         if (!cont) { F[0] = F[1] = F[2] = F[3] = F[4] = 0.f; }
         // extras (integrator.h:756-776) with FP64 accumulation
         const double nphi = (double)F[4];
@@ -794,18 +838,18 @@ integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
                     double D[6];
                     float gm[3], gp[3];
                     // x pair: d/dx of dphidx
-                    if (!jac_eval<1>(t, sv, jcell[0][tl], jcid[0][tl], jlw[0][tl], p[CX] - hj, p[CY], p[CZ], gm)) gm[0] = 0.f;
-                    if (!jac_eval<1>(t, sv, jcell[1][tl], jcid[1][tl], jlw[1][tl], p[CX] + hj, p[CY], p[CZ], gp)) gp[0] = 0.f;
+                    if (!jac_eval<1, JAC_SCALAR_BLOCKS>(t, sv, jcell[0][tl], jcid[0][tl], jlw[0][tl], p[CX] - hj, p[CY], p[CZ], gm)) gm[0] = 0.f;
+                    if (!jac_eval<1, JAC_SCALAR_BLOCKS>(t, sv, jcell[1][tl], jcid[1][tl], jlw[1][tl], p[CX] + hj, p[CY], p[CZ], gp)) gp[0] = 0.f;
                     D[0] = (double)gp[0] - (double)gm[0];
                     // y pair: d/dy of dphidx, dphidy
-                    if (!jac_eval<2>(t, sv, jcell[2][tl], jcid[2][tl], jlw[2][tl], p[CX], p[CY] - hj, p[CZ], gm)) gm[0] = gm[1] = 0.f;
-                    if (!jac_eval<2>(t, sv, jcell[3][tl], jcid[3][tl], jlw[3][tl], p[CX], p[CY] + hj, p[CZ], gp)) gp[0] = gp[1] = 0.f;
+                    if (!jac_eval<2, JAC_SCALAR_BLOCKS>(t, sv, jcell[2][tl], jcid[2][tl], jlw[2][tl], p[CX], p[CY] - hj, p[CZ], gm)) gm[0] = gm[1] = 0.f;
+                    if (!jac_eval<2, JAC_SCALAR_BLOCKS>(t, sv, jcell[3][tl], jcid[3][tl], jlw[3][tl], p[CX], p[CY] + hj, p[CZ], gp)) gp[0] = gp[1] = 0.f;
                     D[1] = (double)gp[0] - (double)gm[0];
                     D[3] = (double)gp[1] - (double)gm[1];
                     // z pair: d/dz of dphidx, dphidy, dphidz
-                    if (!jac_eval<3>(t, sv, jcell[4][tl], jcid[4][tl], jlw[4][tl], p[CX], p[CY], p[CZ] - hj, gm))
+                    if (!jac_eval<3, JAC_SCALAR_BLOCKS>(t, sv, jcell[4][tl], jcid[4][tl], jlw[4][tl], p[CX], p[CY], p[CZ] - hj, gm))
                         gm[0] = gm[1] = gm[2] = 0.f;
-                    if (!jac_eval<3>(t, sv, jcell[5][tl], jcid[5][tl], jlw[5][tl], p[CX], p[CY], p[CZ] + hj, gp))
+                    if (!jac_eval<3, JAC_SCALAR_BLOCKS>(t, sv, jcell[5][tl], jcid[5][tl], jlw[5][tl], p[CX], p[CY], p[CZ] + hj, gp))
                         gp[0] = gp[1] = gp[2] = 0.f;
                     D[2] = (double)gp[0] - (double)gm[0];
                     D[4] = (double)gp[1] - (double)gm[1];
@@ -1009,9 +1053,14 @@ bool split_build(SplitTree& s, const DevTree& t, const uint8_t* level, const uin
         ok = chk(hipGetLastError(), "partial-oct flags");
     }
     if (ok && n) {
-        // the face-neighbour table is an accelerator: without the memory for
-        // it every lookup of a new cell starts from the root
-        if (hipMalloc((void**)&s.nbr, 6 * sizeof(uint2) * n) == hipSuccess) {
+        // the face-neighbour table is an accelerator (about 2 % of a launch):
+        // it is built only when NBR_RESERVE bytes stay free after it for the
+        // rays, surfaces and maps; otherwise every lookup of a new cell starts
+        // from the root
+        size_t free_b = 0, total_b = 0;
+        const bool room = hipMemGetInfo(&free_b, &total_b) == hipSuccess &&
+                          (uint64_t)free_b >= 6 * sizeof(uint2) * n + NBR_RESERVE;
+        if (room && hipMalloc((void**)&s.nbr, 6 * sizeof(uint2) * n) == hipSuccess) {
             s.nbr_bytes = 6 * sizeof(uint2) * n;
             hipLaunchKernelGGL(neighbour_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, t, d_lev, d_kx,
                                d_ky, d_kz, s.tslot, s.nbr);
