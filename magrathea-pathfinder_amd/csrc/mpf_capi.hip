@@ -143,6 +143,7 @@ struct mpf_ctx {
     mpf::DevTree dt;
     mpf::SplitTree split;
     bool split_ready = false;
+    bool split_rho_failed = false;   // the rho blocks did not fit: split surfaces carry NaN densities
     // cell-major field copies of the lean exact integrator, built at its first TSC run
     DevBuf<float> q32;
     DevBuf<double> q64;
@@ -277,6 +278,7 @@ int mpf_load_octree_records(mpf_ctx* c, const void* rec48, uint64_t n) {
     mpf::unpack_fields32(rec48, t, soa);
     c->have_tree = false;
     c->split_ready = false;
+    c->split_rho_failed = false;
     c->quad_ready = false;
     c->f64.release();
     if (c->base.ensure(t.base.size()) != hipSuccess || c->child_first.ensure(n) != hipSuccess ||
@@ -329,6 +331,7 @@ int mpf_load_octree_fields64(mpf_ctx* c, const double* f7, uint64_t n) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->dt.f64 = c->f64.p;
     c->split_ready = false;
+    c->split_rho_failed = false;
     c->quad_ready = false;
     return 0;
 }
@@ -425,6 +428,7 @@ int mpf_velocity_finish(mpf_ctx* c) {
     c->dt.vel = c->vel.p;
     c->vel_open = false;
     c->split_ready = false;
+    c->split_rho_failed = false;
     c->quad_ready = false;  // rho changed
     return 0;
 }
@@ -450,6 +454,7 @@ int mpf_velocity_set(mpf_ctx* c, const float* rho, const float* vel3, uint64_t n
         if (c->dt.f64)
             HIPCHK(c, hipMemcpyAsync(c->f64.p, r64.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
         c->split_ready = false;
+    c->split_rho_failed = false;
     c->quad_ready = false;
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -643,6 +648,17 @@ static int integrate_impl(mpf_ctx* c, const double* dev_in9, uint64_t n, const m
                 return fail(c, ERR_INVAL, "surfaces are not available in the split tier");
             if (!c->split_ready) {
                 if (int rc = ensure_split(c)) return rc;
+                HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+            }
+            // rho blocks for the surfaces' densities, built at the first run with
+            // surfaces (without the memory for them the densities stay NaN)
+            if (A.nsurf && !c->split.rho && !c->split_rho_failed) {
+                std::string err;
+                if (!mpf::split_build_rho(c->split, c->dt, c->lev.data(), c->kx.data(), c->ky.data(), c->kz.data(),
+                                          c->stream, err)) {
+                    c->split_rho_failed = true;
+                    (void)hipGetLastError();
+                }
                 HIPCHK(c, hipEventRecord(c->ev0, c->stream));
             }
         }

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <string>
+#include <vector>
 
 #include "mpf_kernels.h"
 
@@ -15,11 +16,17 @@ struct SplitTree {
     unsigned char* blocks = nullptr;  // per slot: FP64 centre, 1/size, 5 x 27 FP32 coefficients
     uint2* nbr = nullptr;             // cell x 6 faces -> same-level neighbour (see neighbour_kernel)
     size_t nbr_bytes = 0;
+    float* rho = nullptr;             // per slot: 27 FP32 coefficients of rho (+ pad), built for surfaces
+    std::vector<uint32_t> slot_cell;  // host: slot -> its stencil-centre cell
 };
 
 // level/kx/ky/kz: host arrays of the level-sorted cells
 bool split_build(SplitTree& s, const DevTree& t, const uint8_t* level, const uint32_t* kx, const uint32_t* ky,
                  const uint32_t* kz, int order, hipStream_t st, std::string& err);
+// the rho blocks (FP32 TSC polynomial of rho per slot) of a built tree, for
+// the density entries of surfaces; level/kx/ky/kz as for split_build
+bool split_build_rho(SplitTree& s, const DevTree& t, const uint8_t* level, const uint32_t* kx, const uint32_t* ky,
+                     const uint32_t* kz, hipStream_t st, std::string& err);
 void split_release(SplitTree& s);
 bool split_supports_surfaces();
 // doubles of launch record per ray that the split integrator's caller

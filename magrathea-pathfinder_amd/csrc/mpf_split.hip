@@ -81,6 +81,7 @@ struct SplitView {
     const uint32_t* tslot;        // cell -> block slot (| PARTIAL_BIT) or NIL
     const unsigned char* blocks;  // slot -> BLOCK_BYTES
     const uint2* nbr;             // cell x 6 faces -> neighbour entry
+    const float* rho;             // slot -> FSTRIDE coefficients of rho, or null
 };
 
 // Face-neighbour entries (neighbour_kernel): for cell g at level l and face
@@ -165,6 +166,39 @@ __device__ __forceinline__ double field_value(const DevTree& t, uint32_t g, int 
     return t.f64 ? t.f64[(uint64_t)slot * t.n + g] : (double)t.f32[(uint64_t)slot * t.n + g];
 }
 
+// The 27 coefficients C_pqr of sum_ijk f_ijk W_i(u) W_j(v) W_k(w), the TSC
+// interpolant of the 27 cell values v (index 9 i + 3 j + k, offsets -1..1)
+// as a tri-quadratic polynomial in the offset from the centre (FP32, the
+// 28th entry a zero pad)
+__device__ __forceinline__ void tsc_poly(const double* v, float* co) {
+    // M[p][i]: coefficient of u^p in W_i, i = -1, 0, +1
+    const double M[3][3] = {{0.125, 0.75, 0.125}, {-0.5, 0.0, 0.5}, {0.5, -1.0, 0.5}};
+    // transform along z, then y, then x
+    double a1[27], a2[27];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            for (int r = 0; r < 3; ++r) {
+                double acc = 0.0;
+                for (int k = 0; k < 3; ++k) acc += M[r][k] * v[i * 9 + j * 3 + k];
+                a1[i * 9 + j * 3 + r] = acc;
+            }
+    for (int i = 0; i < 3; ++i)
+        for (int q = 0; q < 3; ++q)
+            for (int r = 0; r < 3; ++r) {
+                double acc = 0.0;
+                for (int j = 0; j < 3; ++j) acc += M[q][j] * a1[i * 9 + j * 3 + r];
+                a2[i * 9 + q * 3 + r] = acc;
+            }
+    for (int p = 0; p < 3; ++p)
+        for (int q = 0; q < 3; ++q)
+            for (int r = 0; r < 3; ++r) {
+                double acc = 0.0;
+                for (int i = 0; i < 3; ++i) acc += M[p][i] * a2[i * 9 + q * 3 + r];
+                co[p * 9 + q * 3 + r] = (float)acc;
+            }
+    co[27] = 0.0f;
+}
+
 __global__ void __launch_bounds__(128) block_kernel(DevTree t, const uint8_t* __restrict__ lev, const uint32_t* __restrict__ kx,
                              const uint32_t* __restrict__ ky, const uint32_t* __restrict__ kz,
                              const uint32_t* __restrict__ slot_cell, uint64_t nslots, unsigned char* blocks) {
@@ -186,36 +220,31 @@ __global__ void __launch_bounds__(128) block_kernel(DevTree t, const uint8_t* __
     hd[2] = cell_center(l, cz);
     hd[3] = (double)(1ull << l) / EXTENT;  // 1 / cell size
     float* co = reinterpret_cast<float*>(blk + BLOCK_DOUBLES * 8);
-    // M[p][i]: coefficient of u^p in W_i, i = -1, 0, +1
-    const double M[3][3] = {{0.125, 0.75, 0.125}, {-0.5, 0.0, 0.5}, {0.5, -1.0, 0.5}};
     for (int f = 0; f < NFIELD; ++f) {
         double v[27];
         for (int ic = 0; ic < 27; ++ic) v[ic] = field_value(t, nb[ic], f);
-        // transform along z, then y, then x
-        double a1[27], a2[27];
-        for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j)
-                for (int r = 0; r < 3; ++r) {
-                    double acc = 0.0;
-                    for (int k = 0; k < 3; ++k) acc += M[r][k] * v[i * 9 + j * 3 + k];
-                    a1[i * 9 + j * 3 + r] = acc;
-                }
-        for (int i = 0; i < 3; ++i)
-            for (int q = 0; q < 3; ++q)
-                for (int r = 0; r < 3; ++r) {
-                    double acc = 0.0;
-                    for (int j = 0; j < 3; ++j) acc += M[q][j] * a1[i * 9 + j * 3 + r];
-                    a2[i * 9 + q * 3 + r] = acc;
-                }
-        for (int p = 0; p < 3; ++p)
-            for (int q = 0; q < 3; ++q)
-                for (int r = 0; r < 3; ++r) {
-                    double acc = 0.0;
-                    for (int i = 0; i < 3; ++i) acc += M[p][i] * a2[i * 9 + q * 3 + r];
-                    co[f * FSTRIDE + p * 9 + q * 3 + r] = (float)acc;
-                }
-        co[f * FSTRIDE + 27] = 0.0f;
+        tsc_poly(v, co + f * FSTRIDE);
     }
+}
+
+// the rho coefficients of each slot (blocks for the surfaces' densities)
+__global__ void __launch_bounds__(128) rho_block_kernel(DevTree t, const uint8_t* __restrict__ lev,
+                                                        const uint32_t* __restrict__ kx, const uint32_t* __restrict__ ky,
+                                                        const uint32_t* __restrict__ kz,
+                                                        const uint32_t* __restrict__ slot_cell, uint64_t nslots,
+                                                        float* __restrict__ rho) {
+    const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    const uint32_t g = slot_cell[s];
+    const unsigned l = lev[g];
+    double v[27];
+    for (int ic = 0; ic < 27; ++ic) {
+        const int ox = ic / 9 - 1, oy = (ic / 3) % 3 - 1, oz = ic % 3 - 1;
+        unsigned fl = 0;
+        const uint32_t h = find_cell(t, l, kx[g] + ox, ky[g] + oy, kz[g] + oz, &fl);
+        v[ic] = t.f64 ? t.f64[(uint64_t)F_RHO * t.n + h] : (double)t.f32[(uint64_t)F_RHO * t.n + h];
+    }
+    tsc_poly(v, rho + s * (uint64_t)FSTRIDE);
 }
 
 // ------------------------------------------------------------- kernel
@@ -614,14 +643,16 @@ struct SplitRay {
     uint32_t appended;
     CosmoSeg cs;
     double stopv;     // this ray's stop value (kept with surfaces only; re-read otherwise)
+    float rho;        // rho at the current point (surfaces with rho blocks)
 };
 
 __device__ __forceinline__ double level_extent(unsigned l) { return ldexp((double)EXTENT, -(int)l); }
 
 // Launch records (integrator.h:669-705) with exact FP64 fields at the start
-// point: normalised core (9), empty flag, phi, dphidx, dphidy, dphidz,
-// dphidt, level of the located cell.
+// point: normalised core (9), level of the located cell (+ LAUNCH_EMPTY for
+// an empty launch element), phi, dphidx, dphidy, dphidz, dphidt, rho.
 constexpr int LAUNCH_DOUBLES = 16;
+constexpr double LAUNCH_EMPTY = 1024.0;
 
 // Records are written in queue order (record q is the ray the queue hands
 // out at position q), so that lanes that take rays together read adjacent
@@ -651,10 +682,11 @@ __global__ void __launch_bounds__(256) launch_kernel(IntegrateArgs A, DevTree t)
     double* L = A.scratch + (uint64_t)LAUNCH_DOUBLES * i;
 #pragma unroll
     for (int k = 0; k < 9; ++k) L[k] = core[k];
-    L[9] = exact::empty7(d) ? 1.0 : 0.0;
+    // the located cell's level, + LAUNCH_EMPTY when the reference's launch element is empty
+    L[9] = (double)(loc < 0 ? 0 : loc) + (exact::empty7(d) ? LAUNCH_EMPTY : 0.0);
     L[10] = phi;
     L[11] = d[F_DX]; L[12] = d[F_DY]; L[13] = d[F_DZ]; L[14] = d[F_DT];
-    L[15] = (double)(loc < 0 ? 0 : loc);
+    L[15] = d[F_RHO];
 }
 
 // rho and a (the `ah` output) of each final point, exact FP64 TSC
@@ -683,8 +715,8 @@ integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
     I.valid = false;
     I.slot = NIL;
     SplitRay R;
-    // per-lane Jacobian sums in LDS (SurfAccLds)
-    __shared__ double surf_sums[SURF ? 6 : 1][SURF ? 256 : 1];
+    // per-lane Jacobian sums and rho maxima in LDS (SurfAccLds)
+    __shared__ double surf_sums[SURF ? SURF_LDS_DOUBLES : 1][SURF ? 256 : 1];
     SurfAccLds SA;
     SA.S = &surf_sums[0][SURF ? threadIdx.x : 0];
     SA.stride = SURF ? 256u : 0u;
@@ -724,7 +756,7 @@ integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
                     const double* L = A.scratch + (uint64_t)LAUNCH_DOUBLES * my;
 #pragma unroll
                     for (int k = 0; k < 9; ++k) R.core[k] = L[k];
-                    const bool empty = L[9] != 0.0;
+                    const bool empty = L[9] >= LAUNCH_EMPTY;
                     const double phi = L[10];
                     cosmo_seg_init(R.cs);
                     R.lambda = R.s = R.isw = R.iswold = R.redshift = R.chi = 0.0;
@@ -736,8 +768,11 @@ integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
                     R.F[0] = R.dt; R.F[1] = R.dx; R.F[2] = R.dy; R.F[3] = R.dz;
                     R.appended = 0;
                     if (SURF) R.stopv = A.ray_value ? A.ray_value[ray] : A.stop_value;
-                    if (SURF) surf_init(SA);
-                    const unsigned l0 = (unsigned)L[15];
+                    if (SURF) {
+                        surf_init(SA, L[15]);
+                        R.rho = (float)L[15];
+                    }
+                    const unsigned l0 = (unsigned)(empty ? L[9] - LAUNCH_EMPTY : L[9]);
                     R.dl = level_extent(l0) * (R.core[CA] * R.core[CA] * ratio_k);
                     I.valid = false;
                     active = !empty;
@@ -819,13 +854,29 @@ integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
             }
             (void)provisional;
             if (SURF) {
+                // rho at the new point: the rho block of its cell's slot at the
+                // offsets split_eval used (FP32), when the rho blocks are built
+                float rho_n = 0.f;
+                if (sv.rho) {
+                    const float u = (float)((p[CX] - I.cx) * I.inv);
+                    const float v = (float)((p[CY] - I.cy) * I.inv);
+                    const float w = (float)((p[CZ] - I.cz) * I.inv);
+                    const float4* rq = reinterpret_cast<const float4*>(sv.rho + (uint64_t)I.slot * FSTRIDE);
+                    float rc[FSTRIDE];
+#pragma unroll
+                    for (int i = 0; i < FSTRIDE / 4; ++i) {
+                        const float4 q = rq[i];
+                        rc[4 * i] = q.x; rc[4 * i + 1] = q.y; rc[4 * i + 2] = q.z; rc[4 * i + 3] = q.w;
+                    }
+                    rho_n = horner27(rc, u, v, w);
+                }
                 SurfElem pv, nw;
                 pv.x = R.core[CX]; pv.y = R.core[CY]; pv.z = R.core[CZ]; pv.chi = R.chi; pv.lambda = R.lambda;
                 pv.t = R.core[CT]; pv.a = R.core[CA]; pv.redshift = R.redshift; pv.s = R.s; pv.isw = R.isw;
-                pv.rho = 0.0; pv.phi = R.phi; pv.kt = R.core[CKT]; pv.kx = R.core[CKX]; pv.ky = R.core[CKY];
+                pv.rho = (double)R.rho; pv.phi = R.phi; pv.kt = R.core[CKT]; pv.kx = R.core[CKX]; pv.ky = R.core[CKY];
                 pv.kz = R.core[CKZ];
                 nw.x = p[CX]; nw.y = p[CY]; nw.z = p[CZ]; nw.chi = chi; nw.lambda = lambda; nw.t = p[CT];
-                nw.a = p[CA]; nw.redshift = redshift; nw.s = s; nw.isw = isw; nw.rho = 0.0; nw.phi = nphi;
+                nw.a = p[CA]; nw.redshift = redshift; nw.s = s; nw.isw = isw; nw.rho = (double)rho_n; nw.phi = nphi;
                 nw.kt = p[CKT]; nw.kx = p[CKX]; nw.ky = p[CKY]; nw.kz = p[CKZ];
                 double T[3] = {0.0, 0.0, 0.0};
                 if (A.jac) {
@@ -861,6 +912,7 @@ integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
                 surf_append(SA, A.surf_kind, A.nsurf, surf_vals, A.ray_value != nullptr, R.stopv,
                             A.surf_out + (uint64_t)A.nsurf * SF_NREC * ray, R.appended + 1, pv, nw, vp, vn, T,
                             A.jac != 0);
+                R.rho = rho_n;
             }
 #pragma unroll
             for (int i = 0; i < 9; ++i) R.core[i] = p[i];
@@ -915,7 +967,7 @@ integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
             }
             if (SURF) {
                 // surfaces never reached, or whose next element was popped, are rejected (hmaps.h:716);
-                // rho is not carried by the split tier
+                // rho and rhomax come from the rho blocks (NaN without them)
                 const bool pop = signbit(R.redshift) || signbit(R.core[CA]);
                 const uint32_t last = (pop && R.appended) ? R.appended - 1 : R.appended;
                 double* rec = A.surf_out + (uint64_t)A.nsurf * SF_NREC * ray;
@@ -929,8 +981,10 @@ integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
                             so[SF_VALID] = 0.0;
                             so[SF_CROSSED] = 0.0;
                         }
-                        so[SF_RHO] = __builtin_nan("");
-                        so[SF_RHOMAX] = __builtin_nan("");
+                        if (!sv.rho) {   // no rho blocks: the densities are not carried
+                            so[SF_RHO] = __builtin_nan("");
+                            so[SF_RHOMAX] = __builtin_nan("");
+                        }
                     }
                 }
             }
@@ -1078,7 +1132,44 @@ bool split_build(SplitTree& s, const DevTree& t, const uint8_t* level, const uin
     if (ok) ok = chk(hipStreamSynchronize(st), "sync");
     cleanup();
     if (!ok) split_release(s);
+    else s.slot_cell = std::move(slot_cell);
     return ok;
+}
+
+bool split_build_rho(SplitTree& s, const DevTree& t, const uint8_t* level, const uint32_t* kx, const uint32_t* ky,
+                     const uint32_t* kz, hipStream_t st, std::string& err) {
+    if (s.rho) return true;
+    const uint64_t n = t.n, ns = s.nslots;
+    if (ns == 0 || s.slot_cell.size() != ns) {
+        err = "split tier: rho blocks need the built stencil blocks";
+        return false;
+    }
+    uint8_t* d_lev = nullptr;
+    uint32_t *d_kx = nullptr, *d_ky = nullptr, *d_kz = nullptr, *d_sc = nullptr;
+    float* rho = nullptr;
+    bool ok = hipMalloc((void**)&rho, 4 * FSTRIDE * ns) == hipSuccess && hipMalloc((void**)&d_lev, n) == hipSuccess &&
+              hipMalloc((void**)&d_kx, 4 * n) == hipSuccess && hipMalloc((void**)&d_ky, 4 * n) == hipSuccess &&
+              hipMalloc((void**)&d_kz, 4 * n) == hipSuccess && hipMalloc((void**)&d_sc, 4 * ns) == hipSuccess;
+    if (!ok) err = "split tier: rho blocks: allocation failed";
+    ok = ok && hipMemcpyAsync(d_lev, level, n, hipMemcpyHostToDevice, st) == hipSuccess &&
+         hipMemcpyAsync(d_kx, kx, 4 * n, hipMemcpyHostToDevice, st) == hipSuccess &&
+         hipMemcpyAsync(d_ky, ky, 4 * n, hipMemcpyHostToDevice, st) == hipSuccess &&
+         hipMemcpyAsync(d_kz, kz, 4 * n, hipMemcpyHostToDevice, st) == hipSuccess &&
+         hipMemcpyAsync(d_sc, s.slot_cell.data(), 4 * ns, hipMemcpyHostToDevice, st) == hipSuccess;
+    if (ok) {
+        hipLaunchKernelGGL(rho_block_kernel, dim3((unsigned)((ns + 127) / 128)), dim3(128), 0, st, t, d_lev, d_kx,
+                           d_ky, d_kz, d_sc, ns, rho);
+        ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(st) == hipSuccess;
+        if (!ok) err = "split tier: rho blocks: kernel failed";
+    }
+    for (void* q : {(void*)d_lev, (void*)d_kx, (void*)d_ky, (void*)d_kz, (void*)d_sc})
+        if (q) (void)hipFree(q);
+    if (!ok) {
+        if (rho) (void)hipFree(rho);
+        return false;
+    }
+    s.rho = rho;
+    return true;
 }
 
 bool split_supports_surfaces() { return true; }
@@ -1088,13 +1179,14 @@ int split_launch_doubles() { return LAUNCH_DOUBLES; }
 void split_release(SplitTree& s) {
     if (s.mem) (void)hipFree(s.mem);
     if (s.nbr) (void)hipFree(s.nbr);
+    if (s.rho) (void)hipFree(s.rho);
     s = SplitTree();
 }
 
 hipError_t launch_integrate_split(const IntegrateArgs& A, const DevTree& t, const SplitTree& s, int cus,
                                   unsigned blocks_per_cu, unsigned waves_per_simd, hipStream_t st,
                                   hipEvent_t ev0, hipEvent_t ev1) {
-    SplitView sv{s.tslot, s.blocks, s.nbr};
+    SplitView sv{s.tslot, s.blocks, s.nbr, s.rho};
     const uint64_t blocks = (A.n + 255) / 256;
     const int grid = (int)std::min<uint64_t>(blocks, (uint64_t)cus * blocks_per_cu);
     const unsigned pgrid = (unsigned)((A.n + 255) / 256);
@@ -1127,7 +1219,7 @@ hipError_t launch_integrate_split(const IntegrateArgs& A, const DevTree& t, cons
 namespace mpf {
 hipError_t launch_features_split(const DevTree& t, const SplitTree& s, const double* src, uint64_t n, double length,
                                  float* feat, hipStream_t st) {
-    SplitView sv{s.tslot, s.blocks, s.nbr};
+    SplitView sv{s.tslot, s.blocks, s.nbr, s.rho};
     if (n)
         hipLaunchKernelGGL(features_split_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, t, sv, src, n,
                            length, feat);

@@ -62,8 +62,8 @@ __device__ __forceinline__ void surf_init(SurfAcc& s, double rho0) {
     s.crossed = 0;
 }
 
-// The same with the sums in LDS (S[k * stride], k = S1 0..2, S2 0..2) and no
-// rho (a tier that does not carry rho writes its own values later)
+// The same with the sums and the rho maxima in LDS (S[k * stride], k = S1
+// 0..2, S2 0..2, the maximum of rho over elements 0 .. i-2, over 0 .. i-1)
 struct SurfAccLds {
     double* S;
     unsigned stride;
@@ -71,13 +71,20 @@ struct SurfAccLds {
     __device__ double s1(int c) const { return S[c * stride]; }
     __device__ double s2(int c) const { return S[(3 + c) * stride]; }
     __device__ void add(int c, double d1, double d2) { S[c * stride] += d1; S[(3 + c) * stride] += d2; }
-    __device__ double rho_max() const { return 0.0; }
-    __device__ void push_rho(double) {}
+    __device__ double rho_max() const { return S[6 * stride]; }
+    __device__ void push_rho(double rho) {
+        const double cur = S[7 * stride];
+        S[6 * stride] = cur;
+        S[7 * stride] = (rho > cur) ? rho : cur;
+    }
 };
+constexpr int SURF_LDS_DOUBLES = 8;
 
-__device__ __forceinline__ void surf_init(SurfAccLds& s) {
+__device__ __forceinline__ void surf_init(SurfAccLds& s, double rho0) {
 #pragma unroll
     for (int k = 0; k < 6; ++k) s.S[k * s.stride] = 0.0;
+    s.S[6 * s.stride] = -1.0e300;
+    s.S[7 * s.stride] = rho0;
     s.crossed = 0;
 }
 

@@ -80,28 +80,37 @@ def test_bundle_jacobian_split_vs_fp64():
     cf.close()
 
 
-def test_bundle_jacobian_plane_vs_oracle():
+@pytest.mark.parametrize("precision", (capi.PREC_REF, capi.PREC_FP64, capi.PREC_SPLIT))
+def test_bundle_jacobian_plane_vs_oracle(precision):
     """stop_bundle = plane: bit for bit the oracle's (itself the reference's,
-    tests/test_bundle_jacobian.py), failure entries included."""
-    print("[case] bundle jacobian plane", flush=True)
+    tests/test_bundle_jacobian.py), failure entries included: every third
+    direction's last plane is out of reach, so its entries are the
+    reference's 42, 42, 1, 0.  A split context runs this path in the FP64
+    tier and is compared with the oracle's FP64 mode."""
+    print(f"[case] bundle jacobian plane precision={precision}", flush=True)
     import math
-    from mpf import capi
     rec, f64 = cases.tree("wide")
-    c = capi.Context(cases.LENGTH, order=capi.TSC, precision=capi.PREC_REF, nsteps=cases.NSTEPS)
+    c = capi.Context(cases.LENGTH, order=capi.TSC, precision=precision, nsteps=cases.NSTEPS)
     c.load_records(rec)
+    c.load_fields64(f64)
     c.set_cosmology(cases.cosmology())
     O = Oracle()
-    h = O.tree(rec)
+    h = O.tree(rec, f64)
     rng = np.random.default_rng(8)
-    n, ref = 10, np.array([0.2, 0.5, 0.9])
+    n, ref = 12, np.array([0.2, 0.5, 0.9])
     pt = np.c_[math.radians(20) + rng.uniform(-0.2, 0.2, n), math.radians(20) + rng.uniform(-0.2, 0.2, n)]
     dirs = np.c_[np.cos(pt[:, 0]) * np.sin(pt[:, 1]), np.sin(pt[:, 0]) * np.sin(pt[:, 1]), np.cos(pt[:, 1])]
     ki = np.repeat(dirs[:, None, :], 3, axis=1) + rng.normal(0, 0.01, (n, 3, 3))
     refs = np.tile(ref, (n, 1))
+    refs[::3, 2] = 5.0          # beyond the data: the rays never reach it
     got = c.jacobian_bundle(pt, "plane", refs, ki, refs, 1e-4)
+    oprec = 0 if precision == capi.PREC_REF else 1
     for i in range(n):
         want = O.dbetadtheta(h, 2, cases.cosmology(), cases.LENGTH, cases.NSTEPS, 5, 1e-4, [0, 0, 0], pt[i, 0],
-                             pt[i, 1], ref, ki[i], ref)
+                             pt[i, 1], refs[i], ki[i], refs[i], precision=oprec)
         np.testing.assert_array_equal(got[i], want, err_msg=f"direction {i}")
+    failed = np.all(got[:, :, 0] == 42, axis=1)
+    assert 0 < failed.sum() < n, failed.sum()
+    assert np.all(failed[::3])
     O.free(h)
     c.close()

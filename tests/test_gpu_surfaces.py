@@ -8,7 +8,8 @@ quantity bit-identical; the Jacobian within 1e-12 absolute + 1e-9 of |A - 1|
 (the device uses the position's own sines and cosines and S1 - S2/D instead
 of libm trigonometry and a per-distance sum).  Split tier against the FP64
 tier: relative redshift <= 1e-7, positions <= 1e-9, Jacobian within
-1e-10 absolute + 1e-5 of |A - 1|."""
+1e-10 absolute + 1e-5 of |A - 1|, rho and rhomax within 1e-5 relative (FP32
+polynomial blocks of rho)."""
 import numpy as np
 import pytest
 
@@ -85,6 +86,12 @@ def test_surfaces_split_vs_fp64(name):
     dp = np.abs(a[:, capi.SURF["dist"]] - b[:, capi.SURF["dist"]]) / b[:, capi.SURF["dist"]]
     assert dp.max() <= 1e-9
     jac_close(a, b, 1e-5, 1e-10)
+    # densities from the split tier's FP32 rho blocks: the value at the
+    # crossing and the running maximum before it
+    for f in ("rho", "rhomax"):
+        x, y = a[:, capi.SURF[f]], b[:, capi.SURF[f]]
+        assert np.all(np.isfinite(x)), f
+        assert np.max(np.abs(x - y) / np.abs(y)) <= 1e-5, f
     c64.close()
     cs.close()
 
