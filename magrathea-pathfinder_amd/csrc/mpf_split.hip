@@ -76,6 +76,8 @@ constexpr int SURF_SCALAR_BLOCKS = MPF_SURF_SCALAR_BLOCKS;
 // device memory left free by the optional face-neighbour table: room for
 // 16 M rays with a surface (about 11 GB) and nside-4096 maps (12.9 GB)
 constexpr uint64_t NBR_RESERVE = 24ull << 30;
+// ... and by the rho blocks: the launch records and ray order of 16 M rays
+constexpr uint64_t RHO_RESERVE = 4ull << 30;
 
 struct SplitView {
     const uint32_t* tslot;        // cell -> block slot (| PARTIAL_BIT) or NIL
@@ -1142,6 +1144,14 @@ bool split_build_rho(SplitTree& s, const DevTree& t, const uint8_t* level, const
     const uint64_t n = t.n, ns = s.nslots;
     if (ns == 0 || s.slot_cell.size() != ns) {
         err = "split tier: rho blocks need the built stencil blocks";
+        return false;
+    }
+    // the run that needs them still allocates its launch records and ray
+    // order after this: leave RHO_RESERVE bytes free
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess ||
+        (uint64_t)free_b < 4ull * FSTRIDE * ns + 21ull * n + RHO_RESERVE) {
+        err = "split tier: no room for the rho blocks";
         return false;
     }
     uint8_t* d_lev = nullptr;
