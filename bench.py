@@ -40,6 +40,9 @@ WORKLOAD_TEXT = {
                   "(99.9 deg^2) around (20, 20) deg, TSC, to z=2",
     "uniform128_cic": "uniform128_cic: uniform 128^3 grid (3000 Mpc/h box), 64x64 rays on a regular grid over a "
                       "5x5 deg narrow field (25.0 deg^2) around (20, 20) deg, CIC, to z=0.5",
+    "cone600m_tsc": "cone600m_tsc: level-9 base (512^3, 10000 Mpc/h box) refined to level 13 where the lognormal "
+                    "(sigma 2) overdensity > 8, 4096x4096 rays on a regular grid over a 20x20 deg narrow field "
+                    "(399 deg^2) around (20, 20) deg, TSC, to z=3 with the infinitesimal Jacobian, nside-4096 maps",
 }
 B_STEP_SPLIT_TSC = 540  # 27 cells x 5 fields x 4 bytes per RK4 step (equivalent gather, not a roofline)
 BF16_PEAK = 2.5e15      # MI355X dense BF16 matrix-core peak, flop/s (AMD public spec)
@@ -220,6 +223,119 @@ def uniform_cic(steps, warmup, nthreads, cpu=True):
         rec_out["steps_match_cpu"] = bool(rsteps == rec_out["ref"]["rk4_steps_per_pass"])
     z, zr = outs["fp64"][:, capi.OUT["redshift"]], outs["ref"][:, capi.OUT["redshift"]]
     rec_out["fp64_vs_ref_rel_end_redshift_max"] = float(np.max(np.abs(z - zr) / np.abs(zr)))
+    return rec_out
+
+
+def production_cone(nside_map=4096, ncat=4194304, acc=1e-8, seed=17):
+    """The single-cone production workload (cone600m_tsc): a level-9 base
+    (512^3) refined to level 13, about 600 M cells with FP64 fields, built by
+    torch on the GPU and handed to libmpf; 4096 x 4096 rays over the 20 x 20
+    deg narrow field to z = 3 with the infinitesimal Jacobian (split tier),
+    kappa/gamma1/gamma2 binned into nside-4096 maps; ncat catalogue sources in
+    the field, Born-seeded Newton to acc.  Device memory is reported at each
+    stage (hipMemGetInfo through torch)."""
+    import threading
+
+    import torch
+    from mpf import capi, proposal, synth
+    rec_out = {"workload": WORKLOAD_TEXT["cone600m_tsc"]}
+    mem = {}
+    peak = [0]
+
+    def guard(limit=220e9):
+        # the host memory of this process (the tree is built on the GPU, but
+        # its records, fields and the library's re-layout pass through here)
+        while True:
+            try:
+                rss = next(int(l.split()[1]) * 1024 for l in open("/proc/self/status") if l.startswith("VmRSS"))
+            except (OSError, StopIteration):
+                return
+            peak[0] = max(peak[0], rss)
+            if rss > limit:
+                print(f"[bench] host memory {rss / 1e9:.0f} GB above {limit / 1e9:.0f} GB: stopping", flush=True)
+                os._exit(3)
+            time.sleep(1.0)
+    threading.Thread(target=guard, daemon=True).start()
+
+    def used(tag):
+        free, total = torch.cuda.mem_get_info()
+        mem[tag] = round((total - free) / 1e9, 2)
+        log(f"device memory in use after {tag}: {mem[tag]} GB of {total / 1e9:.1f}")
+
+    t0 = time.time()
+    w, rec, f64, length, table, in9 = synth.build_workload("cone600m_tsc", log=log, device="cuda")
+    torch.cuda.empty_cache()
+    rec_out["generate_s"] = time.time() - t0
+    used("tree generation (torch cache emptied)")
+    ctx = capi.Context(length, order=2, precision=capi.PREC_SPLIT, nsteps=10, device=0)
+    t0 = time.time()
+    ctx.load_records(rec)
+    ctx.load_fields64(f64)
+    ctx.set_cosmology(table)
+    rec_out["load_s"] = time.time() - t0
+    ncells, lmin, lmax = ctx.octree_info()
+    rec_out.update(cells=int(ncells), levels=[int(lmin), int(lmax)])
+    del rec, f64
+    used("octree load")
+    n = in9.shape[0]
+    npix = 12 * nside_map * nside_map
+    d_in, d_out = ctx.dev_alloc(in9.nbytes), ctx.dev_alloc(n * capi.NOUT * 8)
+    d_surf, d_maps = ctx.dev_alloc(n * capi.NSURF * 8), ctx.dev_alloc(4 * npix * 8)
+    ctx.upload(d_in, in9)
+    used("ray and map buffers")
+    ctx.set_launch_config(surfaces=True, refill=64, sort_bucket=65536, blocks_per_cu=8, waves_per_simd=2)
+    runs = []
+    for rep in range(2):   # the first pass also builds the stencil and rho blocks
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.integrate_surfaces_dev(d_in, n, d_out, d_surf, "redshift", [w["z_stop"]], jacobian=True, sync=True)
+        t1 = time.perf_counter()
+        steps, ms = ctx.last_run()
+        runs.append((t1 - t0, steps, ctx.last_kernel_ms()))
+        if rep == 0:
+            used("first pass (stencil blocks built)")
+    wall, steps, kms = runs[-1]
+    t0 = time.perf_counter()
+    ctx.healpix_bin_dev(nside_map, d_in, d_surf, n, 1, 0, d_maps)
+    ctx.synchronize()
+    t_bin = time.perf_counter() - t0
+    hits = ctx.download(d_maps + 3 * npix * 8, (npix,))
+    surf = ctx.download(d_surf, (n, capi.NSURF))
+    valid = surf[:, capi.SURF["valid"]] != 0
+    for p in (d_in, d_out, d_surf, d_maps):
+        ctx.dev_free(p)
+    rec_out["rays"] = {"rays": int(n), "rk4_steps": int(steps), "first_pass_s": runs[0][0], "wall_s": wall,
+                       "kernel_ms": kms, "value": steps / wall, "kernel_steps_per_s": steps / (kms / 1e3),
+                       "unit": "ray-RK4-steps/s", "rays_valid": int(valid.sum()), "bin_ms": t_bin * 1e3,
+                       "pixels_hit": int(np.count_nonzero(hits)),
+                       "kappa_rms": float(np.sqrt(np.mean((1 - (surf[valid, capi.SURF["a11"]] +
+                                                                 surf[valid, capi.SURF["a22"]]) / 2) ** 2)))}
+    del surf, hits
+    # catalogue: sources in the field, Born-seeded Newton
+    box = w["box_mpch"]
+    dmin = synth.comoving_distance_of_z(0.1) * 0.7 / synth.MPC_SI / box
+    dmax = synth.comoving_distance_of_z(w["z_stop"]) * 0.7 / synth.MPC_SI / box
+    src = proposal.field_sources(ncat, w["phi0"], w["theta0"], w["aperture"], dmin, dmax, seed)
+    d_src, d_init, d_cat = ctx.dev_alloc(src.nbytes), ctx.dev_alloc(ncat * 16), ctx.dev_alloc(ncat * capi.NCAT * 8)
+    ctx.upload(d_src, src)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.catalogue_seed_dev(d_src, ncat, d_init, use_network=False)
+    crays = ctx.catalogue_solve_dev(d_src, d_init, ncat, d_cat, accuracy=acc, jacobian_all_rounds=True)
+    ctx.synchronize()
+    cwall = time.perf_counter() - t0
+    cat = ctx.download(d_cat, (ncat, capi.NCAT))
+    for p in (d_src, d_init, d_cat):
+        ctx.dev_free(p)
+    ok = cat[:, capi.CAT["status"]] == capi.CAT_OK
+    rec_out["catalogue"] = {"sources": int(ncat), "converged": int(ok.sum()), "wall_s": cwall,
+                            "sources_per_s": float(ok.sum() / cwall), "rays": int(crays),
+                            "mean_iterations": float(cat[ok, capi.CAT["iter"]].mean()),
+                            "first_guess": "Born estimate of the features (the network's gate is off, section 3.5)"}
+    used("catalogue")
+    rec_out["device_memory_GB"] = mem
+    rec_out["host_peak_rss_GB"] = round(peak[0] / 1e9, 1)
+    ctx.close()
     return rec_out
 
 
@@ -417,7 +533,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-tier comparison")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the Jacobian/maps and catalogue workloads on the same octree")
-    ap.add_argument("--only", choices=("jacobian_maps", "catalogue", "velocity"), default=None,
+    ap.add_argument("--only", choices=("jacobian_maps", "catalogue", "velocity", "cone600m"), default=None,
                     help="run one of the extra workloads alone and print its record (profiling runs)")
     ap.add_argument("--launch-config", default=None, metavar="R,B,C,W",
                     help="use this launch configuration (refill, sort bucket, blocks per CU, waves per SIMD) for "
@@ -448,6 +564,9 @@ def main():
         print(json.dumps({"copy_bytes": nb, "kernel_ms": ms, "GB_per_s_read_plus_write": 2 * nb / (min(ms) / 1e3) / 1e9}))
         return
 
+    if args.only == "cone600m":
+        print(json.dumps({"cone600m_tsc": production_cone()}), flush=True)
+        return
     w, rec, f64, length, table, in9 = build_workload(args.workload)
     order = w["order"]
     tier = {"split": capi.PREC_SPLIT, "fp64": capi.PREC_FP64, "ref": capi.PREC_REF}[args.tier]

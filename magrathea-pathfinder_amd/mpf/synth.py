@@ -571,20 +571,28 @@ WORKLOADS = {
     # 1024x1024 rays over a 10x10 deg narrow field to z = 2
     "amr256_tsc": dict(base_input_level=8, box_mpch=8000.0, sigma=1.5, threshold=8.0, max_input_level=11, seed=42,
                  noise=1.0, decay=0.75, order=2, nside=1024, aperture=10.0, z_stop=2.0, phi0=20.0, theta0=20.0),
+    # level-9 base (512^3) refined to level 13 where the sigma-2 overdensity
+    # exceeds 8 (decay 0.67 per level below the base: about 600 M cells),
+    # 4096x4096 rays over a 20x20 deg narrow field to z = 3
+    "cone600m_tsc": dict(base_input_level=9, box_mpch=10000.0, sigma=2.0, threshold=8.0, max_input_level=13,
+                         seed=42, noise=1.0, decay=0.67, order=2, nside=4096, aperture=20.0, z_stop=3.0,
+                         phi0=20.0, theta0=20.0),
 }
 
 
-def build_workload(name, log=None):
+def build_workload(name, log=None, device="cpu"):
     """The named workload of WORKLOADS: (parameters, octree records, FP64
     fields, box length in metres, cosmology table, launch records n x 9 of
-    the rays on the regular grid over the narrow field)."""
+    the rays on the regular grid over the narrow field).  device: where
+    torch generates the tree (the records come back to the host)."""
     import time
     w = dict(WORKLOADS[name])
     t0 = time.time()
     table = flat_lcdm_table(h=0.7, omega_m=0.3, a_min=0.05, n=4096)
     rec, f64, length = lightcone_tree(w["base_input_level"], w["box_mpch"], w["sigma"], w["threshold"],
                                       w["max_input_level"], w["seed"], noise=w.get("noise", 0.0),
-                                      decay=w.get("decay", 0.0), table=table, device="cpu")
+                                      decay=w.get("decay", 0.0), table=table, device=device,
+                                      verbose=log is not None and device != "cpu")
     ph, th = narrow_field_grid(w["phi0"], w["theta0"], w["aperture"], w["nside"])
     in9 = launch_photons(ph, th)
     if log:
