@@ -234,9 +234,9 @@ def production_cone(nside_map=4096, ncat=4194304, acc=1e-8, seed=17):
     kappa/gamma1/gamma2 binned into nside-4096 maps; ncat catalogue sources in
     the field, Born-seeded Newton to acc.  Device memory is reported at each
     stage (hipMemGetInfo through torch)."""
+    import subprocess
     import threading
 
-    import torch
     from mpf import capi, proposal, synth
     rec_out = {"workload": WORKLOAD_TEXT["cone600m_tsc"]}
     mem = {}
@@ -258,15 +258,38 @@ def production_cone(nside_map=4096, ncat=4194304, acc=1e-8, seed=17):
     threading.Thread(target=guard, daemon=True).start()
 
     def used(tag):
-        free, total = torch.cuda.mem_get_info()
+        free, total = capi.device_memory()
         mem[tag] = round((total - free) / 1e9, 2)
         log(f"device memory in use after {tag}: {mem[tag]} GB of {total / 1e9:.1f}")
 
+    def tree_from_child(name):
+        # torch's HIP runtime and libmpf's cannot both drive the device in one
+        # process: a child builds the tree with torch on the GPU and streams
+        # the records and FP64 fields back through a pipe
+        p = subprocess.Popen([sys.executable, os.path.abspath(__file__), "--emit-tree", name],
+                             stdout=subprocess.PIPE, stderr=None)
+        head = json.loads(p.stdout.readline())
+        rec = np.empty(head["cells"], dtype=synth.formats.RECORD_DTYPE)
+        f64 = np.empty((head["cells"], 7))
+        for arr in (rec, f64):
+            view = memoryview(arr.reshape(-1).view(np.uint8))
+            got = 0
+            while got < view.nbytes:
+                k = p.stdout.readinto(view[got:])
+                if not k:
+                    raise RuntimeError("tree generator ended early")
+                got += k
+        if p.wait() != 0:
+            raise RuntimeError(f"tree generator failed ({p.returncode})")
+        w = dict(synth.WORKLOADS[name])
+        table = synth.flat_lcdm_table(h=0.7, omega_m=0.3, a_min=0.05, n=4096)
+        ph, th = synth.narrow_field_grid(w["phi0"], w["theta0"], w["aperture"], w["nside"])
+        return w, rec, f64, head["length"], table, synth.launch_photons(ph, th)
+
     t0 = time.time()
-    w, rec, f64, length, table, in9 = synth.build_workload("cone600m_tsc", log=log, device="cuda")
-    torch.cuda.empty_cache()
+    w, rec, f64, length, table, in9 = tree_from_child("cone600m_tsc")
     rec_out["generate_s"] = time.time() - t0
-    used("tree generation (torch cache emptied)")
+    used("tree generation (in a child process)")
     ctx = capi.Context(length, order=2, precision=capi.PREC_SPLIT, nsteps=10, device=0)
     t0 = time.time()
     ctx.load_records(rec)
@@ -523,6 +546,28 @@ def catalogue(ctx, w, length, table, n=1048576, zmin=0.1, zmax=1.5, acc=1e-8, nt
             "network": network}
 
 
+def emit_tree(name):
+    """Child of production_cone: the named workload's tree built by torch on
+    the GPU, written to stdout as one JSON line (cells, length) and the raw
+    records and FP64 fields."""
+    import torch
+    from mpf import synth
+    w = dict(synth.WORKLOADS[name])
+    table = synth.flat_lcdm_table(h=0.7, omega_m=0.3, a_min=0.05, n=4096)
+    import contextlib
+    dev = "cuda" if torch.cuda.is_available() else "cpu"
+    with contextlib.redirect_stdout(sys.stderr):   # the generator's progress lines; stdout carries the tree
+        rec, f64, length = synth.lightcone_tree(w["base_input_level"], w["box_mpch"], w["sigma"], w["threshold"],
+                                                w["max_input_level"], w["seed"], noise=w.get("noise", 0.0),
+                                                decay=w.get("decay", 0.0), table=table, device=dev, verbose=True)
+    f64 = np.ascontiguousarray(f64)
+    out = sys.stdout.buffer
+    out.write((json.dumps({"cells": int(rec.size), "length": float(length)}) + "\n").encode())
+    out.write(memoryview(rec.view(np.uint8)))
+    out.write(memoryview(f64.reshape(-1).view(np.uint8)))
+    out.flush()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -544,7 +589,11 @@ def main():
                     help="precision tier of the timed passes (profiling runs of the exact tiers)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="only run the plain copy kernel over MB megabytes (HBM counter calibration)")
+    ap.add_argument("--emit-tree", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.emit_tree:
+        emit_tree(args.emit_tree)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards by rays or "
               f"cones with no exchange until the map reduction; see DESIGN.md)", file=sys.stderr)

@@ -695,6 +695,9 @@ int mpf_healpix_bin_dev(mpf_ctx* ctx, int64_t nside, const double* dev_in9, cons
 /* Device buffers on the context's device, for callers that keep inputs and
  * outputs resident between calls (mpf_integrate_dev).  Copies are
  * synchronous. */
+/* Free and total bytes of device memory of a HIP device (hipMemGetInfo):
+ * callers size their batches with it; no counterpart in the reference. */
+int mpf_device_memory(int device, uint64_t* free_bytes, uint64_t* total_bytes);
 int mpf_dev_alloc(mpf_ctx* ctx, uint64_t bytes, void** dev_ptr);
 int mpf_dev_free(mpf_ctx* ctx, void* dev_ptr);
 int mpf_dev_upload(mpf_ctx* ctx, void* dev_dst, const void* host_src, uint64_t bytes);

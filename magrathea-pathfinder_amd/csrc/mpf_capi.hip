@@ -991,6 +991,16 @@ int mpf_cosmology_prepare(int device, int precision, const char* evolution_file,
 
 const char* mpf_cosmology_last_error(void) { return g_cosmo_err.c_str(); }
 
+int mpf_device_memory(int device, uint64_t* free_bytes, uint64_t* total_bytes) {
+    if (!free_bytes || !total_bytes) return ERR_INVAL;
+    if (hipSetDevice(device) != hipSuccess) return ERR_HIP;
+    size_t f = 0, t = 0;
+    if (hipMemGetInfo(&f, &t) != hipSuccess) return ERR_HIP;
+    *free_bytes = f;
+    *total_bytes = t;
+    return 0;
+}
+
 int mpf_dev_alloc(mpf_ctx* c, uint64_t bytes, void** dev_ptr) {
     if (!c || !dev_ptr) return ERR_INVAL;
     *dev_ptr = nullptr;

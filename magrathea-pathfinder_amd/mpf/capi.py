@@ -37,6 +37,7 @@ EXPORTS = ("mpf_create", "mpf_destroy", "mpf_last_error", "mpf_set_precision", "
            "mpf_catalogue_write_previous", "mpf_mlp_init", "mpf_mlp_state", "mpf_mlp_train", "mpf_mlp_gradient",
            "mpf_mlp_infer", "mpf_mlp_infer_dev", "mpf_catalogue_features", "mpf_catalogue_features_dev",
            "mpf_catalogue_seed", "mpf_catalogue_seed_dev", "mpf_mfma_probe", "mpf_mfma_probe32", "mpf_mlp_last_ms",
+           "mpf_device_memory",
            "mpf_integrate_trajectory",
            "mpf_cosmology_acquire", "mpf_cosmology_correct", "mpf_cosmology_prepare", "mpf_cosmology_last_error",
            "mpf_cone_last_error", "mpf_cones_fullsky", "mpf_cones_narrow", "mpf_narrow_cone_info", "mpf_cone_inside",
@@ -215,6 +216,7 @@ def load_library(path=LIB_PATH):
     lib.mpf_mfma_probe.argtypes = [vp, vp, vp, vp]
     lib.mpf_mfma_probe32.argtypes = [vp, vp, vp, ctypes.c_int, vp]
     lib.mpf_mlp_last_ms.argtypes = [vp, ctypes.POINTER(dbl)]
+    lib.mpf_device_memory.argtypes = [ctypes.c_int, ctypes.POINTER(u64), ctypes.POINTER(u64)]
     lib.mpf_integrate_trajectory.argtypes = [vp, vp, u64, ctypes.POINTER(Stop), u64, vp, vp]
     cp, dbl = ctypes.c_char_p, ctypes.c_double
     lib.mpf_cosmology_acquire.argtypes = [cp, cp, cp, dbl, vp, vp, vp, vp, vp, vp, vp, u64, vp]
@@ -456,6 +458,15 @@ def cones_fullsky(ncones, radius=2.0):
     out = np.zeros((ncones, CONE_N))
     _cone_check(load_library().mpf_cones_fullsky(ncones, radius, _p(out)), "mpf_cones_fullsky")
     return out
+
+
+def device_memory(device=0):
+    """(free, total) bytes of device memory (hipMemGetInfo)."""
+    f, t = ctypes.c_uint64(0), ctypes.c_uint64(0)
+    rc = load_library().mpf_device_memory(int(device), ctypes.byref(f), ctypes.byref(t))
+    if rc:
+        raise MpfError(f"mpf_device_memory failed ({rc})")
+    return f.value, t.value
 
 
 def cones_narrow(ncones, phi_deg, theta_deg, thetay_deg, thetaz_deg, radius=2.0):
