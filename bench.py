@@ -277,7 +277,7 @@ def production_cone(nside_map=4096, ncat=4194304, acc=1e-8, seed=17):
             while got < view.nbytes:
                 k = p.stdout.readinto(view[got:])
                 if not k:
-                    raise RuntimeError("tree generator ended early")
+                    raise RuntimeError(f"tree generator ended early (exit status {p.wait()})")
                 got += k
         if p.wait() != 0:
             raise RuntimeError(f"tree generator failed ({p.returncode})")
@@ -561,11 +561,19 @@ def emit_tree(name):
                                                 w["max_input_level"], w["seed"], noise=w.get("noise", 0.0),
                                                 decay=w.get("decay", 0.0), table=table, device=dev, verbose=True)
     f64 = np.ascontiguousarray(f64)
-    out = sys.stdout.buffer
-    out.write((json.dumps({"cells": int(rec.size), "length": float(length)}) + "\n").encode())
-    out.write(memoryview(rec.view(np.uint8)))
-    out.write(memoryview(f64.reshape(-1).view(np.uint8)))
-    out.flush()
+    status = {l.split(":")[0]: l.split(":")[1].strip() for l in open("/proc/self/status") if l.startswith("Vm")}
+    avail = next((l.split(":")[1].strip() for l in open("/proc/meminfo") if l.startswith("MemAvailable")), "?")
+    print(f"[bench] tree generator: VmRSS {status.get('VmRSS')}, VmHWM {status.get('VmHWM')}, host MemAvailable "
+          f"{avail}", file=sys.stderr, flush=True)
+    fd = sys.stdout.fileno()
+    os.write(fd, (json.dumps({"cells": int(rec.size), "length": float(length)}) + "\n").encode())
+    for arr in (rec.view(np.uint8), f64.reshape(-1).view(np.uint8)):
+        mv = memoryview(arr)
+        for i in range(0, mv.nbytes, 1 << 28):
+            chunk = mv[i:i + (1 << 28)]
+            done = 0
+            while done < chunk.nbytes:
+                done += os.write(fd, chunk[done:])
 
 
 def main():
