@@ -545,12 +545,18 @@ __device__ void tsc_lean(const DevTree& t, LeanCache& cache, double x, double y,
                 wz[i] = tsc_axis_w(a, fabs((cell_center(idl, cache.cz[i]) - point[2]) / (twohalves)));
             }
             const uint32_t* g = cache.g + (uint64_t)cache.cur * 27 * LEAN_BLOCK;
-#pragma unroll 1  // one cell at a time: unrolled, the loads hoisted across cells spill registers
-            for (int ic = 0; ic < 27; ++ic) {
-                const int ix = ic / 9, iy = (ic / 3) % 3, iz = ic % 3;
-                const double w = pick3(wx, ix) * pick3(wy, iy) * pick3(wz, iz);
-                loadf_sel<P, NF>(t, g[ic * LEAN_BLOCK], cell, full);
-                macf_sel<P, NF>(result, cell, w, full);
+            // cells in the reference's order (x, then y, then z), the weight
+            // (wx wy) wz as it associates; one (x, y) pair per iteration, its 3
+            // z cells unrolled (the whole 27 unrolled hoists loads and spills)
+#pragma unroll 1
+            for (int ixy = 0; ixy < 9; ++ixy) {
+                const double wxy = pick3(wx, ixy / 3) * pick3(wy, ixy % 3);
+#pragma unroll
+                for (int iz = 0; iz < 3; ++iz) {
+                    const int ic = 3 * ixy + iz;
+                    loadf_sel<P, NF>(t, g[ic * LEAN_BLOCK], cell, full);
+                    macf_sel<P, NF>(result, cell, wxy * wz[iz], full);
+                }
             }
             return;
         }

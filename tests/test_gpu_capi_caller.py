@@ -61,3 +61,17 @@ def test_compiled_caller_catalogue_matches_binding(tmp_path):
     c.close()
     assert np.sum(ref[:, capi.CAT["status"]] == capi.CAT_OK) > 100
     np.testing.assert_array_equal(rows, ref)
+
+
+def test_device_memory_reports_the_device():
+    """mpf_device_memory: free <= total, the total an MI355X's (about 288 GiB),
+    and an allocation shows in the free bytes."""
+    print("[case] mpf_device_memory", flush=True)
+    free0, total = capi.device_memory(0)
+    assert 0 < free0 <= total and total > 250e9
+    c = capi.Context(cases.LENGTH)
+    p = c.dev_alloc(8 << 30)
+    free1, _ = capi.device_memory(0)
+    c.dev_free(p)
+    c.close()
+    assert free0 - free1 >= (8 << 30) - (64 << 20)
