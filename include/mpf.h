@@ -293,9 +293,12 @@ int mpf_set_observer_velocity(mpf_ctx* ctx, const double* vobs3);
  * from observer3 towards the unit direction plus or minus tan(opening) along
  * its screen vectors e1, e2, integrated on the device to the last of the nd
  * values of the stop parameter `kind` (MPF_STOP_RADIUS, _LAMBDA, _REDSHIFT,
- * _T, _A, or _PLANE: the position along the last ki, whole trajectories in an
- * exact tier, FP64 for a split context), each ray's position interpolated at
- * every value (along each ki for _PLANE), and
+ * _T, _A, or _PLANE: the position along the last ki, in an exact tier, FP64
+ * for a split context), each ray's position interpolated at every value
+ * (along each ki for _PLANE: each ray records its crossings of the planes
+ * while it is integrated, all directions in one launch; a direction whose
+ * rays' projections on a normal decrease somewhere is redone from whole
+ * trajectories, as upper_bound over them places it), and
  * the finite differences projected on the screen normal to ki (n x nd x 3,
  * the reference's kiTargets) over 2 dist tan(opening) (dist: n x nd).  ref:
  * n x nd values, the same for every direction (up to 16, monotonic as the
@@ -306,6 +309,10 @@ int mpf_set_observer_velocity(mpf_ctx* ctx, const double* vobs3);
 int mpf_jacobian_bundle(mpf_ctx* ctx, const double* phi_theta, uint64_t n, const double* observer3, int kind,
                         const double* ref, uint32_t nd, const double* ki, const double* dist, double opening,
                         double* out4);
+/* The number of directions of the last mpf_jacobian_bundle call with
+ * MPF_STOP_PLANE that were redone from whole trajectories.  No counterpart
+ * in the reference (which always stores whole trajectories). */
+int mpf_bundle_plane_info(const mpf_ctx* ctx, uint64_t* whole_trajectory_directions);
 /* The Born approximation of the surfaces (lensing_born maps, hmaps.h:705-713;
  * infinitesimal_born catalogues, catalogues.h:670-674): the rays follow the
  * path of Integrator::integrate<-1> (homogeneous data: straight lines of the

@@ -169,6 +169,7 @@ struct mpf_ctx {
     float last_ms = 0.f, last_kernel_ms = 0.f;
     bool kernel_timed = false;  // the last integration recorded evk0/evk1
     bool surf_jac_chi = false;  // surfaces take the Jacobian at the interpolated chi
+    uint64_t bundle_plane_fallback = 0;  // directions of the last plane bundle from whole trajectories
     bool pending = false;
     // learned first guess of the catalogue root-finder
     mpf::MlpModel* mlp = nullptr;
@@ -558,9 +559,18 @@ int mpf_interpolate(mpf_ctx* c, int order, const double* xyz, uint64_t n, double
     return 0;
 }
 
+// planes crossed by bundle rays (IntegrateArgs::nplane .. plane_out), device pointers
+struct PlaneSet {
+    int n = 0;
+    unsigned div = 4;
+    const double* val = nullptr;
+    const double* kit = nullptr;
+    double* out = nullptr;
+};
+
 static int integrate_impl(mpf_ctx* c, const double* dev_in9, uint64_t n, const mpf_stop* stop, double* dev_out,
                           const mpf_surfaces* surf, double* dev_surf, int sync, double* dev_traj = nullptr,
-                          uint64_t traj_cap = 0, const Workspace* ws = nullptr);
+                          uint64_t traj_cap = 0, const Workspace* ws = nullptr, const PlaneSet* planes = nullptr);
 static int finish_run(mpf_ctx* c, const unsigned long long* counters);
 
 int mpf_integrate_dev(mpf_ctx* c, const double* dev_in9, uint64_t n, const mpf_stop* stop, double* dev_out, int sync) {
@@ -585,9 +595,12 @@ int mpf_integrate_surfaces_dev(mpf_ctx* c, const double* dev_in9, uint64_t n, co
 
 static int integrate_impl(mpf_ctx* c, const double* dev_in9, uint64_t n, const mpf_stop* stop, double* dev_out,
                           const mpf_surfaces* surf, double* dev_surf, int sync, double* dev_traj, uint64_t traj_cap,
-                          const Workspace* ws) {
+                          const Workspace* ws, const PlaneSet* planes) {
     if (!c || !stop || (n && !dev_in9)) return ERR_INVAL;
-    if (ws && (surf || dev_traj || (n && !dev_out))) return ERR_INVAL;
+    if (ws && (surf || dev_traj || planes || (n && !dev_out))) return ERR_INVAL;
+    if (planes && (surf || dev_traj || stop->kind != MPF_STOP_PLANE || planes->n < 1 ||
+                   planes->n > MPF_MAX_SURFACES || planes->div < 1 || c->cfg.precision == MPF_PREC_SPLIT))
+        return ERR_INVAL;
     if (ws && c->cfg.precision == MPF_PREC_SPLIT && !c->split_ready)
         return fail(c, ERR_STATE, "split-tier stencil blocks not built (mpf_integrate_dev builds them on first use)");
     unsigned long long* counters = ws ? ws->counters : c->counters.p;
@@ -630,6 +643,13 @@ static int integrate_impl(mpf_ctx* c, const double* dev_in9, uint64_t n, const m
         A.traj_cap = traj_cap;
         A.traj_over = counters + 2;
     }
+    if (planes) {
+        A.nplane = planes->n;
+        A.plane_div = planes->div;
+        A.plane_val = planes->val;
+        A.plane_kit = planes->kit;
+        A.plane_out = planes->out;
+    }
     if (c->cfg.order == MPF_HOMOGENEOUS && c->cfg.precision == MPF_PREC_SPLIT)
         return fail(c, ERR_INVAL, "the homogeneous order runs in the exact tiers (REF, FP64)");
     if (dev_traj && c->cfg.precision == MPF_PREC_SPLIT)
@@ -637,7 +657,7 @@ static int integrate_impl(mpf_ctx* c, const double* dev_in9, uint64_t n, const m
     if (c->dt.vel && c->cfg.precision == MPF_PREC_SPLIT)
         return fail(c, ERR_INVAL, "a tree with a velocity field is integrated by the exact tiers (REF, FP64)");
     for (int k = 0; k < 3; ++k) A.vobs[k] = c->vobs[k];
-    const mpf_launch_cfg& L = c->launch[surf ? 1 : 0];
+    const mpf_launch_cfg& L = c->launch[(surf || planes) ? 1 : 0];
     A.refill = L.refill;
     HIPCHK(c, mpf::launch_fill_u64(counters, 3, 0ull, c->stream));
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
@@ -697,7 +717,7 @@ static int integrate_impl(mpf_ctx* c, const double* dev_in9, uint64_t n, const m
             // plain integration runs the lean kernel, two 256-lane blocks per CU (its TSC caches
             // take 55 KB of LDS per block); the kernel with surfaces or trajectories four
             const uint64_t blocks = (n + 255) / 256;
-            const bool lean = A.nsurf == 0 && !A.traj;
+            const bool lean = A.nsurf == 0 && !A.traj && A.nplane == 0;
             if (lean && c->cfg.order == MPF_TSC && !c->quad_ready) {
                 // the lean integrator's TSC reads each cell's fields in 16-byte loads
                 c->dt.q32 = nullptr;
@@ -1891,93 +1911,181 @@ static double bundle_param(const double* o, int kind) {
     }
 }
 
-// stop_bundle = plane (lensing.h:201, 281-293): per direction, the four rays'
-// whole trajectories in an exact tier, stopped on the last value along the
-// last ki, each interpolated along every ki; then the finite differences
+// The four launch photons of a bundle around (phi, theta): +-tan(opening)
+// along e1 and e2 (lensing.h:147-195)
+static void bundle_photons(const double* phi_theta, const double* observer3, double to, double* in36) {
+    double sp, cp, st, ct;
+    ::sincos(phi_theta[0], &sp, &cp);
+    ::sincos(phi_theta[1], &st, &ct);
+    const double t[3] = {cp * st, sp * st, ct};
+    const double e1[3] = {-sp, cp, 0}, e2[3] = {-cp * ct, -sp * ct, st};
+    const double rini[4][3] = {{t[0] + to * e1[0], t[1] + to * e1[1], t[2]},
+                               {t[0] - to * e1[0], t[1] - to * e1[1], t[2]},
+                               {t[0] - to * e2[0], t[1] - to * e2[1], t[2] - to * e2[2]},
+                               {t[0] + to * e2[0], t[1] + to * e2[1], t[2] + to * e2[2]}};
+    for (int k = 0; k < 4; ++k) mpf::cone::launch_towards(observer3, rini[k], in36 + 9 * k);
+}
+
+// A direction's Jacobians from its four rays' crossings fp[r][ray][xyz] of
+// its nd planes (lensing.h:308-335); a failed direction gets 42, 42, 1, 0
+static void bundle_plane_combine(const double* ki_i, const double* dist_i, uint32_t nd, double to,
+                                 const double (*fp)[4][3], bool failed, double* J) {
+    for (uint64_t q = 0; q < 4ull * nd; ++q) J[q] = 0.0;
+    if (failed) {
+        for (uint32_t q = 0; q < nd; ++q) {
+            J[4 * q] = 42; J[4 * q + 1] = 42; J[4 * q + 2] = 1; J[4 * q + 3] = 0;
+        }
+        return;
+    }
+    for (uint32_t r = 0; r < nd; ++r) {
+        double rh[3], rvv[3];
+        for (int d = 0; d < 3; ++d) {
+            rh[d] = fp[r][0][d] - fp[r][1][d];
+            rvv[d] = fp[r][2][d] - fp[r][3][d];
+        }
+        const double* k3 = ki_i + 3 * r;
+        const double phik = std::atan2(k3[1], k3[0]);
+        const double thetak = std::acos(k3[2] / std::sqrt(k3[0] * k3[0] + k3[1] * k3[1] + k3[2] * k3[2]));
+        double spk, cpk, stk, ctk;
+        ::sincos(phik, &spk, &cpk);
+        ::sincos(thetak, &stk, &ctk);
+        const double f1[3] = {-spk, cpk, 0}, f2[3] = {-cpk * ctk, -spk * ctk, stk};
+        const double dd = dist_i[r];
+        J[4 * r + 0] = (rh[0] * f1[0] + rh[1] * f1[1]) / (2 * dd * to);
+        J[4 * r + 1] = (rvv[0] * f1[0] + rvv[1] * f1[1]) / (2 * dd * to);
+        J[4 * r + 2] = (rh[0] * f2[0] + rh[1] * f2[1] + rh[2] * f2[2]) / (-2 * dd * to);
+        J[4 * r + 3] = (rvv[0] * f2[0] + rvv[1] * f2[1] + rvv[2] * f2[2]) / (-2 * dd * to);
+    }
+}
+
+// stop_bundle = plane for one direction from whole trajectories (lensing.h:201,
+// 281-304): the four rays stopped on the last value along the last ki, each
+// crossing found by upper_bound along every ki.  The batched path below
+// leaves the directions whose crossings it cannot place to this one.
+static int bundle_plane_traj(mpf_ctx* c, const double* phi_theta, const double* observer3, const double* ref,
+                             uint32_t nd, const double* ki, const double* dist, double to, double* J) {
+    uint64_t cap = 1ull << 14;
+    std::vector<double> out((uint64_t)MPF_NOUT * 4), traj;
+    double in9[36];
+    bundle_photons(phi_theta, observer3, to, in9);
+    const double* klast = ki + 3 * (nd - 1);
+    mpf_stop stop{MPF_STOP_PLANE, ref[nd - 1], {klast[0], klast[1], klast[2]}};
+    int rc = 0;
+    for (;;) {
+        traj.resize((uint64_t)MPF_NOUT * 4 * cap);
+        rc = mpf_integrate_trajectory(c, in9, 4, &stop, cap, out.data(), traj.data());
+        if (rc != ERR_NOSPC || cap >= (1ull << 22)) break;
+        cap <<= 2;
+    }
+    if (rc) return rc;
+    double fp[MPF_MAX_SURFACES][4][3];
+    bool failed = false;
+    for (int it = 0; it < 4 && !failed; ++it) {
+        const int64_t len = (int64_t)out[(uint64_t)MPF_NOUT * it + MPF_OUT_NSTEPS] + 1;
+        const double* tr = traj.data() + (uint64_t)MPF_NOUT * cap * it;
+        for (uint32_t r = 0; r < nd && !failed; ++r) {
+            const double* kr = ki + 3 * r;
+            const double v = ref[r];
+            auto along = [&](int64_t e) {
+                const double* x = tr + (uint64_t)MPF_NOUT * e;
+                return x[MPF_OUT_X] * kr[0] + x[MPF_OUT_Y] * kr[1] + x[MPF_OUT_Z] * kr[2];
+            };
+            if (len < 2 || along(len - 1) < v) {
+                failed = true;
+                break;
+            }
+            int64_t id = 0;
+            double f = 0.0;
+            mpf::plane_segment(len, v, along, &id, &f);
+            const double* a = tr + (uint64_t)MPF_NOUT * id;
+            const double* b = a + MPF_NOUT;
+            fp[r][it][0] = a[MPF_OUT_X] * f + b[MPF_OUT_X] * (1 - f);
+            fp[r][it][1] = a[MPF_OUT_Y] * f + b[MPF_OUT_Y] * (1 - f);
+            fp[r][it][2] = a[MPF_OUT_Z] * f + b[MPF_OUT_Z] * (1 - f);
+        }
+    }
+    bundle_plane_combine(ki, dist, nd, to, fp, failed, J);
+    return 0;
+}
+
+// stop_bundle = plane, batched: the four rays of every direction in one
+// exact-tier launch per chunk of directions, each ray recording where it
+// crosses its direction's planes while it is integrated (IntegrateArgs::
+// nplane); no trajectory is stored.  A ray whose projections on a normal
+// decrease somewhere (where upper_bound over the stored trajectory and the
+// first crossing could differ) sends its direction to bundle_plane_traj.
 static int jacobian_bundle_plane(mpf_ctx* c, const double* phi_theta, uint64_t n, const double* observer3,
                                  const double* ref, uint32_t nd, const double* ki, const double* dist, double opening,
                                  double* out4) {
     const double to = std::tan(opening);
-    uint64_t cap = 1ull << 14;
-    std::vector<double> out((uint64_t)MPF_NOUT * 4), traj;
     const int prec = c->cfg.precision;
     if (prec == MPF_PREC_SPLIT) c->cfg.precision = MPF_PREC_FP64;
+    const uint64_t chunk = std::min<uint64_t>(n, (1ull << 20) / nd);
+    DevBuf<double> din, dval, dkit, dpl;
+    std::vector<double> h9(36 * chunk), hpl((uint64_t)4 * chunk * nd * mpf::PL_NREC);
+    std::vector<uint64_t> fallback;
     int rc = 0;
-    for (uint64_t i = 0; i < n && !rc; ++i) {
-        double sp, cp, st, ct;
-        ::sincos(phi_theta[2 * i], &sp, &cp);
-        ::sincos(phi_theta[2 * i + 1], &st, &ct);
-        const double t[3] = {cp * st, sp * st, ct};
-        const double e1[3] = {-sp, cp, 0}, e2[3] = {-cp * ct, -sp * ct, st};
-        const double rini[4][3] = {{t[0] + to * e1[0], t[1] + to * e1[1], t[2]},
-                                   {t[0] - to * e1[0], t[1] - to * e1[1], t[2]},
-                                   {t[0] - to * e2[0], t[1] - to * e2[1], t[2] - to * e2[2]},
-                                   {t[0] + to * e2[0], t[1] + to * e2[1], t[2] + to * e2[2]}};
-        double in9[36];
-        for (int k = 0; k < 4; ++k) mpf::cone::launch_towards(observer3, rini[k], in9 + 9 * k);
-        const double* klast = ki + 3 * (i * nd + nd - 1);
-        mpf_stop stop{MPF_STOP_PLANE, ref[i * nd + nd - 1], {klast[0], klast[1], klast[2]}};
-        for (;;) {
-            traj.resize((uint64_t)MPF_NOUT * 4 * cap);
-            rc = mpf_integrate_trajectory(c, in9, 4, &stop, cap, out.data(), traj.data());
-            if (rc != ERR_NOSPC || cap >= (1ull << 22)) break;
-            cap <<= 2;
+    if (din.ensure(36 * chunk) || dval.ensure(chunk * nd) || dkit.ensure(3 * chunk * nd) ||
+        dpl.ensure((uint64_t)4 * chunk * nd * mpf::PL_NREC))
+        rc = fail(c, ERR_NOMEM, "bundle-plane buffers");
+    for (uint64_t i0 = 0; !rc && i0 < n; i0 += chunk) {
+        const uint64_t m = std::min(chunk, n - i0);
+        for (uint64_t q = 0; q < m; ++q) bundle_photons(phi_theta + 2 * (i0 + q), observer3, to, h9.data() + 36 * q);
+        if (hipMemcpy(din.p, h9.data(), 36 * m * 8, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(dval.p, ref + i0 * nd, m * nd * 8, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(dkit.p, ki + 3 * i0 * nd, 3 * m * nd * 8, hipMemcpyHostToDevice) != hipSuccess) {
+            rc = fail(c, ERR_HIP, "bundle-plane upload");
+            break;
         }
+        PlaneSet P;
+        P.n = (int)nd;
+        P.div = 4;
+        P.val = dval.p;
+        P.kit = dkit.p;
+        P.out = dpl.p;
+        mpf_stop stop{MPF_STOP_PLANE, 0.0, {0.0, 0.0, 0.0}};
+        rc = integrate_impl(c, din.p, 4 * m, &stop, nullptr, nullptr, nullptr, 1, nullptr, 0, nullptr, &P);
         if (rc) break;
-        double* J = out4 + 4 * (uint64_t)nd * i;
-        for (uint64_t q = 0; q < 4ull * nd; ++q) J[q] = 0.0;
-        double fp[MPF_MAX_SURFACES][4][3];
-        bool failed = false;
-        for (int it = 0; it < 4 && !failed; ++it) {
-            const int64_t len = (int64_t)out[(uint64_t)MPF_NOUT * it + MPF_OUT_NSTEPS] + 1;
-            const double* tr = traj.data() + (uint64_t)MPF_NOUT * cap * it;
-            for (uint32_t r = 0; r < nd && !failed; ++r) {
-                const double* kr = ki + 3 * (i * nd + r);
-                const double v = ref[i * nd + r];
-                auto along = [&](int64_t e) {
-                    const double* x = tr + (uint64_t)MPF_NOUT * e;
-                    return x[MPF_OUT_X] * kr[0] + x[MPF_OUT_Y] * kr[1] + x[MPF_OUT_Z] * kr[2];
-                };
-                if (len < 2 || along(len - 1) < v) {
-                    failed = true;
-                    for (uint32_t q = 0; q < nd; ++q) {
-                        J[4 * q] = 42; J[4 * q + 1] = 42; J[4 * q + 2] = 1; J[4 * q + 3] = 0;
-                    }
-                    break;
+        if (hipMemcpy(hpl.data(), dpl.p, (uint64_t)4 * m * nd * mpf::PL_NREC * 8, hipMemcpyDeviceToHost) !=
+            hipSuccess) {
+            rc = fail(c, ERR_HIP, "bundle-plane download");
+            break;
+        }
+        for (uint64_t q = 0; q < m; ++q) {
+            double fp[MPF_MAX_SURFACES][4][3];
+            bool failed = false, whole = false;
+            for (int it = 0; it < 4; ++it)
+                for (uint32_t r = 0; r < nd; ++r) {
+                    const double* o = hpl.data() + ((4 * q + it) * nd + r) * mpf::PL_NREC;
+                    if (o[mpf::PL_STATE] == 0.0) failed = true;
+                    else if (o[mpf::PL_STATE] != 1.0) whole = true;
+                    else for (int d = 0; d < 3; ++d) fp[r][it][d] = o[mpf::PL_X + d];
                 }
-                int64_t id = 0;
-                double f = 0.0;
-                mpf::plane_segment(len, v, along, &id, &f);
-                const double* a = tr + (uint64_t)MPF_NOUT * id;
-                const double* b = a + MPF_NOUT;
-                fp[r][it][0] = a[MPF_OUT_X] * f + b[MPF_OUT_X] * (1 - f);
-                fp[r][it][1] = a[MPF_OUT_Y] * f + b[MPF_OUT_Y] * (1 - f);
-                fp[r][it][2] = a[MPF_OUT_Z] * f + b[MPF_OUT_Z] * (1 - f);
+            // any ray that never reaches a plane fails the direction, wherever
+            // the others cross (lensing.h:281-290 returns at the first)
+            if (whole && !failed) {
+                fallback.push_back(i0 + q);
+                continue;
             }
+            bundle_plane_combine(ki + 3 * (i0 + q) * nd, dist + (i0 + q) * nd, nd, to, fp, failed,
+                                 out4 + 4 * (uint64_t)nd * (i0 + q));
         }
-        if (failed) continue;
-        for (uint32_t r = 0; r < nd; ++r) {
-            double rh[3], rvv[3];
-            for (int d = 0; d < 3; ++d) {
-                rh[d] = fp[r][0][d] - fp[r][1][d];
-                rvv[d] = fp[r][2][d] - fp[r][3][d];
-            }
-            const double* k3 = ki + 3 * (i * nd + r);
-            const double phik = std::atan2(k3[1], k3[0]);
-            const double thetak = std::acos(k3[2] / std::sqrt(k3[0] * k3[0] + k3[1] * k3[1] + k3[2] * k3[2]));
-            double spk, cpk, stk, ctk;
-            ::sincos(phik, &spk, &cpk);
-            ::sincos(thetak, &stk, &ctk);
-            const double f1[3] = {-spk, cpk, 0}, f2[3] = {-cpk * ctk, -spk * ctk, stk};
-            const double dd = dist[i * nd + r];
-            J[4 * r + 0] = (rh[0] * f1[0] + rh[1] * f1[1]) / (2 * dd * to);
-            J[4 * r + 1] = (rvv[0] * f1[0] + rvv[1] * f1[1]) / (2 * dd * to);
-            J[4 * r + 2] = (rh[0] * f2[0] + rh[1] * f2[1] + rh[2] * f2[2]) / (-2 * dd * to);
-            J[4 * r + 3] = (rvv[0] * f2[0] + rvv[1] * f2[1] + rvv[2] * f2[2]) / (-2 * dd * to);
-        }
+    }
+    din.release(); dval.release(); dkit.release(); dpl.release();
+    c->bundle_plane_fallback = fallback.size();
+    for (uint64_t i : fallback) {
+        if (rc) break;
+        rc = bundle_plane_traj(c, phi_theta + 2 * i, observer3, ref + i * nd, nd, ki + 3 * i * nd, dist + i * nd, to,
+                               out4 + 4 * (uint64_t)nd * i);
     }
     c->cfg.precision = prec;
     return rc;
+}
+
+int mpf_bundle_plane_info(const mpf_ctx* c, uint64_t* whole_trajectory_directions) {
+    if (!c || !whole_trajectory_directions) return ERR_INVAL;
+    *whole_trajectory_directions = c->bundle_plane_fallback;
+    return 0;
 }
 
 int mpf_jacobian_bundle(mpf_ctx* c, const double* phi_theta, uint64_t n, const double* observer3, int kind,

@@ -80,7 +80,86 @@ struct RayState {
     int hint;
     uint32_t appended;
     bool has_prev;
+    uint32_t pl_crossed;  // bit j: plane j passed (IntegrateArgs::nplane)
+    bool pl_nonmono;      // a projection on some plane normal decreased
 };
+
+// projection of an element's position on a plane normal, in the reference's
+// order (lensing.h:281-293)
+__device__ __forceinline__ double along_normal(const double* r, const double* k) {
+    return r[CX] * k[0] + r[CY] * k[1] + r[CZ] * k[2];
+}
+
+// After element appended (>= 1) of a bundle ray: the first element whose
+// projection exceeds a plane's value is where std::upper_bound over the whole
+// trajectory lands when the projections do not decrease (checked here, and
+// the crossing is left to the host when they do); the crossing interpolates
+// between it and the element before (lensing.h:281-304).
+template <int NF>
+__device__ void plane_append(const IntegrateArgs& A, RayState<NF>& s, uint64_t ray) {
+    const uint64_t g = (ray / A.plane_div) * (uint64_t)A.nplane;
+    const double* pv = s.prev.v;
+    const double* nw = s.back.v;
+    for (int j = 0; j < A.nplane; ++j) {
+        const double* k = A.plane_kit + 3 * (g + j);
+        const double v = A.plane_val[g + j];
+        const double an = along_normal(nw, k), ap = along_normal(pv, k);
+        if (an < ap) s.pl_nonmono = true;
+        if ((s.pl_crossed >> j) & 1u) continue;
+        if (!(v < an)) continue;
+        s.pl_crossed |= 1u << j;
+        double* o = A.plane_out + ((uint64_t)ray * A.nplane + j) * PL_NREC;
+        const double f = (an - v) / (an - ap);
+        o[PL_FIRSTID] = (double)(s.appended - 1);
+        o[PL_F] = f;
+        o[PL_X] = pv[CX] * f + nw[CX] * (1 - f);
+        o[PL_Y] = pv[CY] * f + nw[CY] * (1 - f);
+        o[PL_Z] = pv[CZ] * f + nw[CZ] * (1 - f);
+    }
+}
+
+// The state of each plane once the trajectory is complete (after the pop of
+// integrator.h:816-821): the reference's failure test on the last element
+// (lensing.h:281-290), the crossing found while integrating, or the plane
+// met exactly by the last element (upper_bound finds nothing: the last
+// segment, f = 0, as the host's whole-trajectory path takes it).
+template <int NF>
+__device__ void plane_finish(const IntegrateArgs& A, const RayState<NF>& s, uint64_t ray) {
+    const uint64_t g = (ray / A.plane_div) * (uint64_t)A.nplane;
+    const bool pop = signbit(s.back.v[MPF_OUT_REDSHIFT]) || signbit(s.back.v[CA]);
+    const int64_t len = (int64_t)s.appended + 1 - (pop ? 1 : 0);
+    const double* last = pop ? s.prev.v : s.back.v;
+    for (int j = 0; j < A.nplane; ++j) {
+        const double* k = A.plane_kit + 3 * (g + j);
+        const double v = A.plane_val[g + j];
+        double* o = A.plane_out + ((uint64_t)ray * A.nplane + j) * PL_NREC;
+        double st;
+        if (len < 2) {
+            st = 0.0;
+        } else {
+            const double al = along_normal(last, k);
+            if (al < v) {
+                st = 0.0;
+            } else if (((s.pl_crossed >> j) & 1u) && o[PL_FIRSTID] + 1.0 <= (double)(len - 1)) {
+                st = 1.0;
+            } else if (!pop) {
+                const double* pv = s.prev.v;
+                const double* nw = s.back.v;
+                const double f = (al - v) / (al - along_normal(pv, k));
+                o[PL_FIRSTID] = (double)(len - 2);
+                o[PL_F] = f;
+                o[PL_X] = pv[CX] * f + nw[CX] * (1 - f);
+                o[PL_Y] = pv[CY] * f + nw[CY] * (1 - f);
+                o[PL_Z] = pv[CZ] * f + nw[CZ] * (1 - f);
+                st = 1.0;
+            } else {
+                st = 2.0;
+            }
+            if (st == 1.0 && s.pl_nonmono) st = 2.0;
+        }
+        o[PL_STATE] = st;
+    }
+}
 
 // element k of a ray's trajectory, the element index in the NSTEPS slot
 __device__ __forceinline__ void traj_store(const IntegrateArgs& A, uint64_t ray, uint32_t k, const double* rec) {
@@ -146,6 +225,8 @@ __device__ void ray_launch(const IntegrateArgs& A, const DevTree& t, TscCache& c
     s.has_prev = false;
     if (A.nsurf) surf_init(s.surf, d[F_RHO]);
     s.stopv = A.stop_value;
+    s.pl_crossed = 0;
+    s.pl_nonmono = false;
 }
 
 __device__ __forceinline__ SurfElem surf_elem(const double* r) {
@@ -300,6 +381,7 @@ __device__ bool ray_step(const IntegrateArgs& A, const DevTree& t, TscCache& cac
                         A.surf_out + (uint64_t)A.nsurf * SF_NREC * ray, s.appended, surf_elem(s.prev.v),
                         surf_elem(s.back.v), vp, vn, T, A.jac != 0);
         }
+        if (A.nplane) plane_append(A, s, ray);
     }
     if (A.stop_kind >= 0) {
         const double* b = s.back.v;
@@ -310,7 +392,14 @@ __device__ bool ray_step(const IntegrateArgs& A, const DevTree& t, TscCache& cac
             case 2: brk = b[MPF_OUT_REDSHIFT] > s.stopv; break;
             case 3: brk = b[CT] > s.stopv; break;
             case 4: brk = b[CA] < s.stopv; break;
-            case 5: brk = (p[CX] * A.kit[0] + p[CY] * A.kit[1] + p[CZ] * A.kit[2]) > s.stopv; break;
+            case 5:
+                if (A.nplane) {  // the group's last plane (lensing.h:201)
+                    const double* k = A.plane_kit + 3 * ((ray / A.plane_div) * (uint64_t)A.nplane + A.nplane - 1);
+                    brk = (p[CX] * k[0] + p[CY] * k[1] + p[CZ] * k[2]) > s.stopv;
+                } else {
+                    brk = (p[CX] * A.kit[0] + p[CY] * A.kit[1] + p[CZ] * A.kit[2]) > s.stopv;
+                }
+                break;
             case 6: brk = b[MPF_OUT_S] > s.stopv; break;
             default: brk = true; break;
         }
@@ -343,6 +432,7 @@ __device__ __forceinline__ void ray_finish(const IntegrateArgs& A, RayState<NF>&
     // erase the last element if its redshift or a is negative (integrator.h:816-821)
     const Rec* r = &s.back;
     uint32_t steps = s.appended;
+    if (A.nplane) plane_finish(A, s, ray);
     if (A.nsurf) {
         const bool pop = signbit(s.back.v[MPF_OUT_REDSHIFT]) || signbit(s.back.v[CA]);
         surf_finish(A, s, ray, (pop && s.appended) ? s.appended - 1 : s.appended);
@@ -390,6 +480,7 @@ __global__ void __launch_bounds__(256) integrate_exact_kernel(IntegrateArgs A, D
                     ray_launch<P, NF>(A, t, cache, A.in9 + 9 * ray, s);
                     if (A.traj) traj_store(A, ray, 0, s.back.v);
                     if (A.ray_value) s.stopv = A.ray_value[ray];
+                    if (A.nplane) s.stopv = A.plane_val[(ray / A.plane_div) * (uint64_t)A.nplane + A.nplane - 1];
                     active = !emptyf<NF>(s.data);
                     if (!active) ray_finish(A, s, ray);
                 } else {
@@ -978,7 +1069,7 @@ hipError_t launch_surface_final(const IntegrateArgs& A, hipStream_t st) {
 }
 
 hipError_t launch_integrate_exact(int precision, const IntegrateArgs& A, const DevTree& t, int grid, hipStream_t st) {
-    if (A.nsurf == 0 && A.traj == nullptr) {  // plain integration: the lean kernel
+    if (A.nsurf == 0 && A.traj == nullptr && A.nplane == 0) {  // plain integration: the lean kernel
         if (t.vel) {
             if (precision == 0) hipLaunchKernelGGL((integrate_lean_kernel<0, 10>), dim3(grid), dim3(LEAN_BLOCK), 0, st, A, t);
             else hipLaunchKernelGGL((integrate_lean_kernel<1, 10>), dim3(grid), dim3(LEAN_BLOCK), 0, st, A, t);

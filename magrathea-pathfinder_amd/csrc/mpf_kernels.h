@@ -50,7 +50,25 @@ struct IntegrateArgs {
     double* traj = nullptr;
     uint64_t traj_cap = 0;
     unsigned long long* traj_over = nullptr;
+    // planes crossed by bundle rays (Lensing::dbetadtheta with stop_bundle =
+    // plane, exact tiers): the rays ray_group * plane_div .. + plane_div - 1
+    // share nplane planes, values plane_val[group * nplane + j] and normals
+    // plane_kit[3 * (group * nplane + j)]; each ray stops on its group's last
+    // plane (stop_kind 5) and records in plane_out[(ray * nplane + j) *
+    // PL_NREC] where its trajectory crosses plane j
+    int nplane = 0;
+    unsigned plane_div = 1;
+    const double* plane_val = nullptr;
+    const double* plane_kit = nullptr;
+    double* plane_out = nullptr;
 };
+
+// per ray and plane record of IntegrateArgs::plane_out (doubles).  PL_STATE:
+// 1 the crossing is in PL_X..PL_Z, 0 the ray never reaches the plane (the
+// reference's failure entries), 2 the projections along the normal were not
+// non-decreasing along the trajectory, or the trajectory ended on the plane
+// after a popped element: the crossing needs the whole trajectory
+enum { PL_STATE = 0, PL_FIRSTID, PL_F, PL_X, PL_Y, PL_Z, PL_NREC };
 
 hipError_t launch_integrate_exact(int precision, const IntegrateArgs& A, const DevTree& t, int grid, hipStream_t st);
 // the cell-major copies DevTree::q32 / q64 from the structure-of-arrays fields (q64 when t.f64)

@@ -112,5 +112,72 @@ def test_bundle_jacobian_plane_vs_oracle(precision):
     failed = np.all(got[:, :, 0] == 42, axis=1)
     assert 0 < failed.sum() < n, failed.sum()
     assert np.all(failed[::3])
+    # every crossing was placed while integrating (no whole trajectories)
+    assert c.bundle_plane_info() == 0
     O.free(h)
     c.close()
+
+
+@pytest.mark.parametrize("precision", (capi.PREC_REF, capi.PREC_FP64))
+def test_bundle_jacobian_plane_whole_trajectory_directions(precision):
+    """Directions whose rays' projections on the normal decrease go through
+    whole trajectories (upper_bound over them, as the reference), the others
+    through the crossings recorded while integrating, in one call: every
+    other direction's normal points back to the observer (projections
+    decrease from the first step, the plane value behind the observer ends
+    the rays at once), and every result equals the oracle's bit for bit."""
+    print(f"[case] bundle jacobian plane whole-trajectory directions precision={precision}", flush=True)
+    import math
+    c, rec, f64 = ctx("wide", precision)
+    O = Oracle()
+    h = O.tree(rec, f64)
+    rng = np.random.default_rng(11)
+    n = 16
+    pt = np.c_[math.radians(20) + rng.uniform(-0.2, 0.2, n), math.radians(20) + rng.uniform(-0.2, 0.2, n)]
+    dirs = np.c_[np.cos(pt[:, 0]) * np.sin(pt[:, 1]), np.sin(pt[:, 0]) * np.sin(pt[:, 1]), np.cos(pt[:, 1])]
+    ki = (dirs + rng.normal(0, 0.01, (n, 3)))[:, None, :].copy()
+    ref = rng.uniform(0.2, 0.6, (n, 1))
+    back = np.arange(n) % 2 == 1
+    ki[back] = -ki[back]
+    ref[back] = -0.3
+    dist = rng.uniform(0.2, 0.6, (n, 1))
+    got = c.jacobian_bundle(pt, "plane", ref, ki, dist, 1e-4)
+    assert c.bundle_plane_info() == int(back.sum())
+    oprec = 0 if precision == capi.PREC_REF else 1
+    for i in range(n):
+        want = O.dbetadtheta(h, 2, cases.cosmology(), cases.LENGTH, cases.NSTEPS, 5, 1e-4, [0, 0, 0], pt[i, 0],
+                             pt[i, 1], ref[i], ki[i], dist[i], precision=oprec)
+        np.testing.assert_array_equal(got[i], want, err_msg=f"direction {i}")
+    O.free(h)
+    c.close()
+
+
+def test_bundle_jacobian_plane_batch_split_context():
+    """A catalogue-sized batch of plane bundles (4,096 directions, 16,384
+    rays in one launch) from a split context equals the FP64 context's bit for
+    bit (both run the FP64 tier), with no direction left to whole
+    trajectories; a sample is checked against the oracle."""
+    print("[case] bundle jacobian plane batch of 4096 directions", flush=True)
+    import math
+    cs, rec, f64 = ctx("wide", capi.PREC_SPLIT)
+    cf, _, _ = ctx("wide", capi.PREC_FP64)
+    rng = np.random.default_rng(12)
+    n = 4096
+    pt = np.c_[math.radians(20) + rng.uniform(-0.3, 0.3, n), math.radians(20) + rng.uniform(-0.3, 0.3, n)]
+    dirs = np.c_[np.cos(pt[:, 0]) * np.sin(pt[:, 1]), np.sin(pt[:, 0]) * np.sin(pt[:, 1]), np.cos(pt[:, 1])]
+    ki = (dirs + rng.normal(0, 0.01, (n, 3)))[:, None, :].copy()
+    ref = rng.uniform(0.1, 0.7, (n, 1))
+    a = cs.jacobian_bundle(pt, "plane", ref, ki, ref, 2e-4)
+    assert cs.bundle_plane_info() == 0
+    b = cf.jacobian_bundle(pt, "plane", ref, ki, ref, 2e-4)
+    np.testing.assert_array_equal(a, b)
+    assert np.all(np.isfinite(a)) and np.all(a[:, 0, 0] != 42)
+    O = Oracle()
+    h = O.tree(rec, f64)
+    for i in range(0, n, 512):
+        want = O.dbetadtheta(h, 2, cases.cosmology(), cases.LENGTH, cases.NSTEPS, 5, 2e-4, [0, 0, 0], pt[i, 0],
+                             pt[i, 1], ref[i], ki[i], ref[i], precision=1)
+        np.testing.assert_array_equal(a[i], want, err_msg=f"direction {i}")
+    O.free(h)
+    cs.close()
+    cf.close()
