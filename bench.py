@@ -232,8 +232,9 @@ def production_cone(nside_map=4096, ncat=4194304, acc=1e-8, seed=17):
     torch on the GPU and handed to libmpf; 4096 x 4096 rays over the 20 x 20
     deg narrow field to z = 3 with the infinitesimal Jacobian (split tier),
     kappa/gamma1/gamma2 binned into nside-4096 maps; ncat catalogue sources in
-    the field, Born-seeded Newton to acc.  Device memory is reported at each
-    stage (hipMemGetInfo through torch)."""
+    the field, Newton to acc from the network's first guess (trained on 4 other
+    cones of the same octree), beside the Born estimate alone.  Device memory
+    is reported at each stage (hipMemGetInfo through torch)."""
     import subprocess
     import threading
 
@@ -339,22 +340,27 @@ def production_cone(nside_map=4096, ncat=4194304, acc=1e-8, seed=17):
     dmin = synth.comoving_distance_of_z(0.1) * 0.7 / synth.MPC_SI / box
     dmax = synth.comoving_distance_of_z(w["z_stop"]) * 0.7 / synth.MPC_SI / box
     src = proposal.field_sources(ncat, w["phi0"], w["theta0"], w["aperture"], dmin, dmax, seed)
+    # the first guess's network, trained on 4 other cones of this octree
+    fit, first_guess = train_first_guess(ctx, w, dmin, dmax, acc, 1048576, 262144, 12, seed)
     d_src, d_init, d_cat = ctx.dev_alloc(src.nbytes), ctx.dev_alloc(ncat * 16), ctx.dev_alloc(ncat * capi.NCAT * 8)
     ctx.upload(d_src, src)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    ctx.catalogue_seed_dev(d_src, ncat, d_init, use_network=False)
-    crays = ctx.catalogue_solve_dev(d_src, d_init, ncat, d_cat, accuracy=acc, jacobian_all_rounds=True)
-    ctx.synchronize()
-    cwall = time.perf_counter() - t0
-    cat = ctx.download(d_cat, (ncat, capi.NCAT))
+    cruns = {}
+    for name in ("born", "seeded"):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.catalogue_seed_dev(d_src, ncat, d_init, use_network=fit["use_network"] and name == "seeded")
+        crays = ctx.catalogue_solve_dev(d_src, d_init, ncat, d_cat, accuracy=acc, jacobian_all_rounds=True)
+        ctx.synchronize()
+        cwall = time.perf_counter() - t0
+        cat = ctx.download(d_cat, (ncat, capi.NCAT))
+        ok = cat[:, capi.CAT["status"]] == capi.CAT_OK
+        cruns[name] = {"converged": int(ok.sum()), "wall_s": cwall, "sources_per_s": float(ok.sum() / cwall),
+                       "rays": int(crays), "mean_iterations": float(cat[ok, capi.CAT["iter"]].mean())}
     for p in (d_src, d_init, d_cat):
         ctx.dev_free(p)
-    ok = cat[:, capi.CAT["status"]] == capi.CAT_OK
-    rec_out["catalogue"] = {"sources": int(ncat), "converged": int(ok.sum()), "wall_s": cwall,
-                            "sources_per_s": float(ok.sum() / cwall), "rays": int(crays),
-                            "mean_iterations": float(cat[ok, capi.CAT["iter"]].mean()),
-                            "first_guess": "Born estimate of the features (the network's gate is off, section 3.5)"}
+    rec_out["catalogue"] = dict(cruns["seeded"], sources=int(ncat), born=cruns["born"], first_guess=first_guess,
+                                note="seeded: the Born estimate plus the network's correction (when it beat Born on "
+                                     "the held-out cone); born: the Born estimate alone")
     used("catalogue")
     rec_out["device_memory_GB"] = mem
     rec_out["host_peak_rss_GB"] = round(peak[0] / 1e9, 1)
@@ -458,8 +464,33 @@ def velocity_field(ctx, rec, n=1 << 24, seed=13):
     return out
 
 
+TRAIN_AXES = ((80.0, 50.0), (150.0, 70.0), (230.0, 40.0), (300.0, 110.0), (120.0, 130.0))
+
+
+def train_first_guess(ctx, w, dmin, dmax, acc, ntrain, nval, epochs, seed):
+    """The network of the first guess trained on the root-finder's own pairs
+    from 4 cones of the same octree (ntrain sources each, same field shape as
+    the workload, other axes), early-stopped and gated on a 5th (nval)."""
+    from mpf import proposal
+    t0 = time.perf_counter()
+    pairs = []
+    for k, (p, t) in enumerate(TRAIN_AXES):
+        s = proposal.field_sources(ntrain if k < 4 else nval, p, t, w["aperture"], dmin, dmax, seed + 100 + k)
+        fe, ta, _, _ = proposal.training_pairs(ctx, s, accuracy=acc)
+        pairs.append((fe, ta))
+    t_pairs = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    fit = proposal.train(ctx, np.concatenate([p[0] for p in pairs[:4]]), np.concatenate([p[1] for p in pairs[:4]]),
+                         pairs[4][0], pairs[4][1], epochs=epochs, batch=1024, lr=1e-3, seed=seed, check_every=1)
+    t_train = time.perf_counter() - t0
+    return fit, {"training_pairs": int(sum(len(p[0]) for p in pairs[:4])), "cones": 4,
+                 "epochs_kept": fit["epoch"], "held_out_rms_arcsec_born": fit["val_rms_born"],
+                 "held_out_rms_arcsec_network": fit["val_rms_network"], "use_network": bool(fit["use_network"]),
+                 "pairs_s": t_pairs, "train_s": t_train}
+
+
 def catalogue(ctx, w, length, table, n=1048576, zmin=0.1, zmax=1.5, acc=1e-8, ntrain=1048576, nval=262144,
-              epochs=10, seed=11):
+              epochs=12, seed=11):
     """Catalogue root-finding: n sources uniform in comoving volume in the
     narrow field of the ray workload (same axis, same area) at z in [zmin, zmax],
     Newton to acc (rad) with the infinitesimal Jacobian, split tier: plain
@@ -474,31 +505,19 @@ def catalogue(ctx, w, length, table, n=1048576, zmin=0.1, zmax=1.5, acc=1e-8, nt
     dmax = synth.comoving_distance_of_z(zmax) * 0.7 / synth.MPC_SI / box
     ap = w["aperture"]
     src = proposal.field_sources(n, w["phi0"], w["theta0"], ap, dmin, dmax, seed)
-    # training pairs on 4 other cones, a 5th held out for early stopping and the gate
-    t0 = time.perf_counter()
-    pairs = []
-    axes = ((80.0, 50.0), (150.0, 70.0), (230.0, 40.0), (300.0, 110.0), (120.0, 130.0))
-    for k, (p, t) in enumerate(axes):
-        s = proposal.field_sources(ntrain if k < 4 else nval, p, t, ap, dmin, dmax, seed + 100 + k)
-        fe, ta, _, _ = proposal.training_pairs(ctx, s, accuracy=acc)
-        pairs.append((fe, ta))
-    t_pairs = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    fit = proposal.train(ctx, np.concatenate([p[0] for p in pairs[:4]]), np.concatenate([p[1] for p in pairs[:4]]),
-                         pairs[4][0], pairs[4][1], epochs=epochs, batch=1024, lr=1e-3, seed=seed, check_every=2)
-    t_train = time.perf_counter() - t0
+    fit, first_guess = train_first_guess(ctx, w, dmin, dmax, acc, ntrain, nval, epochs, seed)
     d_src, d_init = ctx.dev_alloc(src.nbytes), ctx.dev_alloc(n * 16)
     d_out = ctx.dev_alloc(n * capi.NCAT * 8)
     ctx.upload(d_src, src)
     st, it = capi.CAT["status"], capi.CAT["iter"]
     runs = {}
-    for name in ("plain", "seeded"):
+    for name in ("plain", "born", "seeded"):
         ctx.synchronize()
         t0 = time.perf_counter()
         if name == "plain":
             rays = ctx.catalogue_solve_dev(d_src, 0, n, d_out, accuracy=acc)
         else:
-            ctx.catalogue_seed_dev(d_src, n, d_init, use_network=fit["use_network"])
+            ctx.catalogue_seed_dev(d_src, n, d_init, use_network=fit["use_network"] and name == "seeded")
             rays = ctx.catalogue_solve_dev(d_src, d_init, n, d_out, accuracy=acc, jacobian_all_rounds=True)
         ctx.synchronize()
         wall = time.perf_counter() - t0
@@ -536,14 +555,13 @@ def catalogue(ctx, w, length, table, n=1048576, zmin=0.1, zmax=1.5, acc=1e-8, nt
                         f"({synth.narrow_field_area_deg2(ap):.1f} deg^2) of the ray workload, Newton to {acc} rad, "
                         f"infinitesimal Jacobian, split tier",
             "metric": "catalogue sources/s", "value": runs["seeded"]["sources_per_s"], "unit": "sources/s",
-            "plain": runs["plain"], "seeded": runs["seeded"],
+            "plain": runs["plain"], "born": runs["born"], "seeded": runs["seeded"],
             "speedup_seeded_over_plain": runs["seeded"]["sources_per_s"] / runs["plain"]["sources_per_s"],
+            "iterations_saved_by_network": runs["born"]["mean_iterations"] - runs["seeded"]["mean_iterations"],
             "max_angle_difference_rad": dmax_ang,
-            "first_guess": {"training_pairs": int(sum(len(p[0]) for p in pairs[:4])), "epochs_kept": fit["epoch"],
-                            "held_out_rms_arcsec_born": fit["val_rms_born"],
-                            "held_out_rms_arcsec_network": fit["val_rms_network"],
-                            "use_network": bool(fit["use_network"]), "pairs_s": t_pairs, "train_s": t_train},
-            "network": network}
+            "first_guess": first_guess, "network": network,
+            "runs_note": "plain: unlensed start; born: the Born estimate of the features alone; seeded: the Born "
+                         "estimate plus the network's correction (when it beat Born on the held-out cone)"}
 
 
 def emit_tree(name):

@@ -429,16 +429,22 @@ int mpf_catalogue_write_previous(const char* prefix, const double* rows18, uint6
  * accumulation on the matrix cores, FP32 master weights, Adam), one per
  * context, trained on pairs from the root-finder itself.
  *
- * Features per source (MPF_MLP_NIN floats), from the octree along the
- * straight line of sight (TSC at chi_k = (k + 1/2) D / 128, k < 128: exact
+ * Features per source (MPF_MLP_NIN floats), from the octree along straight
+ * lines from the observer (TSC at chi_k = (k + 1/2) D / 256, k < 256: exact
  * FP64 with one stencil cache per line, or the split tier's FP32 stencil
  * blocks when the context precision is MPF_PREC_SPLIT), invariant under
  * rotations of the sky:
  *   0 comoving distance D (box units); 1, 2 Born deflection
  *   2/c^2 sum (D - chi)/D grad_perp(phi) dchi along e_phi and e_theta
- *   (arcsec); 3, 4 the same over the far half (k >= 64); 5, 6 over the first
- *   quarter (k < 32); 7, 8 2 D grad_perp(phi) / c^2 at k = 127 (arcsec);
- *   9 phi / c^2 at k = 63 (units of 1e-5).
+ *   (arcsec) on the line to the source; 3, 4 the same sum on the line of the
+ *   same length towards the source's angles plus that deflection (theta + f2,
+ *   phi + f1 / sin(theta); projected on that line's e_phi, e_theta) less
+ *   f1, f2; 5, 6 the same difference over the far halves (k >= 128); 7, 8
+ *   the first line's sum over its far half; 9 phi / c^2 at k = 127 on the
+ *   first line (units of 1e-5).  The pairs (1, 2) .. (7, 8) and the targets
+ *   are vectors in the screen basis: training turns them together by random
+ *   angles and reflections (isotropy), so that the network cannot learn
+ *   where in the training cones a source lies from their directions.
  * Targets (MPF_MLP_NOUT floats): the deflection left after the Born
  * estimate, (phi_obs - phi) sin(theta) - f1 and theta_obs - theta - f2
  * (arcsec).  The first guess is phi + (f1 + y0) / sin(theta),

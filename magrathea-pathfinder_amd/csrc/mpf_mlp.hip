@@ -482,6 +482,7 @@ struct ExactSampler {
     const DevTree& t;
     exact::TscCache cache;
     __device__ explicit ExactSampler(const DevTree& tree) : t(tree) { exact::cache_init(cache); }
+    __device__ void restart() { exact::cache_init(cache); }
     __device__ void operator()(double x, double y, double z, double* g) {
         double d[7];
         int loc;

@@ -1001,7 +1001,8 @@ struct SplitSampler {
     const DevTree& t;
     const SplitView& sv;
     Interp<false> I;
-    __device__ SplitSampler(const DevTree& tree, const SplitView& view) : t(tree), sv(view) {
+    __device__ SplitSampler(const DevTree& tree, const SplitView& view) : t(tree), sv(view) { restart(); }
+    __device__ void restart() {
         I.valid = false;
         I.slot = NIL;
     }

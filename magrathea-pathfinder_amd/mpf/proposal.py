@@ -93,22 +93,57 @@ def training_pairs(ctx, src, **newton):
     return feat, targets(out[ok], feat), out, rays
 
 
-def train(ctx, feat, target, val_feat, val_target, epochs=100, batch=256, lr=1e-3, seed=0, check_every=5):
+# feature pairs that are vectors in the screen basis (e_phi, e_theta)
+VECTOR_PAIRS = ((1, 2), (3, 4), (5, 6), (7, 8))
+
+
+def turned(feat, target, rng):
+    """Every sample's screen vectors (the feature pairs and the target) turned
+    by one random angle, after a reflection e_theta -> -e_theta for half of
+    the samples.  The lensing field has no preferred direction on the screen,
+    so the turned pairs are as likely as the originals; a network trained on
+    them cannot tell where in the training cones a source lies from the
+    directions of its deflection, only how the target follows the features."""
+    n = len(feat)
+    psi = rng.uniform(0.0, 2.0 * math.pi, n).astype(np.float32)
+    c, s = np.cos(psi), np.sin(psi)
+    flip = np.where(rng.uniform(size=n) < 0.5, -1.0, 1.0).astype(np.float32)
+    f = np.array(feat, dtype=np.float32)
+    t = np.array(target, dtype=np.float32)
+
+    def turn(x, y):
+        y = y * flip
+        return c * x - s * y, s * x + c * y
+    for a, b in VECTOR_PAIRS:
+        f[:, a], f[:, b] = turn(f[:, a], f[:, b])
+    t[:, 0], t[:, 1] = turn(t[:, 0], t[:, 1])
+    return f, t
+
+
+def train(ctx, feat, target, val_feat, val_target, epochs=100, batch=256, lr=1e-3, seed=0, check_every=5,
+          turn=True):
     """Fresh network trained on (feat, target), early-stopped on a held-out
     set (another cone): the state with the lowest held-out residual is kept.
-    The network is worth using only if that residual is below the one of the
-    Born estimate alone (the held-out targets' rms).  Returns a dict: loss per
-    epoch, held-out rms without and with the network, the epoch kept and
-    use_network."""
+    With turn, every epoch sees the pairs turned and reflected at random
+    (turned()).  The network is worth using only if that residual is below
+    the one of the Born estimate alone (the held-out targets' rms).  Returns
+    a dict: loss per epoch, held-out rms without and with the network, the
+    epoch kept and use_network."""
     ctx.mlp_init(seed)
     val_target = np.asarray(val_target, dtype=np.float64)
     rms0 = float(np.sqrt(np.mean(val_target ** 2)))
+    rng = np.random.default_rng(seed)
     losses = []
     done = 0
     best = (math.inf, 0, None)
     while done < epochs:
         k = min(check_every, epochs - done)
-        losses += list(ctx.mlp_train(feat, target, epochs=k, batch=batch, lr=lr, seed=seed + done))
+        if turn:
+            for e in range(k):
+                f, t = turned(feat, target, rng)
+                losses += list(ctx.mlp_train(f, t, epochs=1, batch=batch, lr=lr, seed=seed + done + e))
+        else:
+            losses += list(ctx.mlp_train(feat, target, epochs=k, batch=batch, lr=lr, seed=seed + done))
         done += k
         r = float(np.sqrt(np.mean((ctx.mlp_infer(val_feat) - val_target) ** 2)))
         if r < best[0]:
@@ -200,7 +235,8 @@ def compare_feature_sets(ctx, length, train_src, val_src, test_src, accuracy=1e-
                   batch=batch, lr=lr, seed=seed)
     state_b = ctx.mlp_state()
     pred_b = ctx.mlp_infer(te[1])
-    fit_p = train(ctx, fp, d, va[2], va[3], epochs=epochs, batch=batch, lr=lr, seed=seed)
+    # absolute angles among the point features: nothing to turn
+    fit_p = train(ctx, fp, d, va[2], va[3], epochs=epochs, batch=batch, lr=lr, seed=seed, turn=False)
     pred_p = ctx.mlp_infer(te[2])
     ctx.mlp_state(state_b)
     born = te[1][:, 1:3].astype(np.float64)

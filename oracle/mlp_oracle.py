@@ -31,7 +31,7 @@ OFF_B3 = OFF_W3 + NH * KOUT
 NPARAM = OFF_B3 + KOUT
 NNORM = 24
 ARCSEC = 206264.80624709636
-FEAT_SAMPLES = 128
+FEAT_SAMPLES = 256
 
 
 def bf16(a):
@@ -127,34 +127,48 @@ def predict(state, x, mixed=True):
     return y * s[NPARAM + 2 * NIN + NOUT:] + s[NPARAM + 2 * NIN:NPARAM + 2 * NIN + NOUT]
 
 
+def _line_basis(phi, theta):
+    cp, sp, ct, st = np.cos(phi), np.sin(phi), np.cos(theta), np.sin(theta)
+    nv = np.stack([st * cp, st * sp, ct], -1)
+    ep = np.stack([-sp, cp, np.zeros_like(sp)], -1)
+    et = np.stack([cp * ct, sp * ct, -st], -1)
+    return nv, ep, et
+
+
 def features(sample_line, src6, length, c2):
     """Source features (include/mpf.h).  sample_line(points (FEAT_SAMPLES x 3),
     ordered outward along one line of sight) -> (FEAT_SAMPLES x 7) fields
     (rho, phi, dx, dy, dz, a, dt): the device samples each line with one
-    stencil cache carried outward."""
+    stencil cache carried outward.  Two lines per source: to the source, and
+    towards the source's angles plus the first line's Born deflection."""
     src = np.asarray(src6, dtype=np.float64).reshape(-1, 6)
     n = src.shape[0]
     D = np.linalg.norm(src[:, :3], axis=1)
     theta = np.arccos(src[:, 2] / D)
     phi = np.arctan2(src[:, 1], src[:, 0])
-    cp, sp, ct, st = np.cos(phi), np.sin(phi), np.cos(theta), np.sin(theta)
-    ep = np.stack([-sp, cp, np.zeros_like(sp)], 1)
-    et = np.stack([cp * ct, sp * ct, -st], 1)
-    nv = src[:, :3] / D[:, None]
     g2 = 2.0 * length / c2 * ARCSEC
     k = np.arange(FEAT_SAMPLES)
     far = k >= FEAT_SAMPLES // 2
-    near = k < FEAT_SAMPLES // 4
+
+    def line(ph, th, Di):
+        nv, ep, et = _line_basis(ph, th)
+        chi = Di * (k + 0.5) / FEAT_SAMPLES
+        d = sample_line(nv[None, :] * chi[:, None])
+        wgt = (Di - chi) / Di * (Di / FEAT_SAMPLES) * g2
+        wp = wgt * (d[:, 2:5] @ ep)
+        wt = wgt * (d[:, 2:5] @ et)
+        return wp, wt, d
+
     f = np.zeros((n, NIN))
     for i in range(n):
-        chi = D[i] * (k + 0.5) / FEAT_SAMPLES
-        d = sample_line(nv[i][None, :] * chi[:, None])
-        wp = (D[i] - chi) / D[i] * (D[i] / FEAT_SAMPLES) * g2 * (d[:, 2:5] @ ep[i])
-        wt = (D[i] - chi) / D[i] * (D[i] / FEAT_SAMPLES) * g2 * (d[:, 2:5] @ et[i])
-        f[i, 1:3] = wp.sum(), wt.sum()
-        f[i, 3:5] = wp[far].sum(), wt[far].sum()
-        f[i, 5:7] = wp[near].sum(), wt[near].sum()
-        f[i, 7:9] = (d[-1, 2:5] @ ep[i]) * g2 * D[i], (d[-1, 2:5] @ et[i]) * g2 * D[i]
+        wp, wt, d = line(phi[i], theta[i], D[i])
+        b = np.array([wp.sum(), wt.sum()])
+        bf = np.array([wp[far].sum(), wt[far].sum()])
+        f[i, 1:3] = b
+        f[i, 7:9] = bf
         f[i, 9] = d[FEAT_SAMPLES // 2 - 1, 1] / c2 * 1.0e5
+        wp1, wt1, _ = line(phi[i] + b[0] / (ARCSEC * np.sin(theta[i])), theta[i] + b[1] / ARCSEC, D[i])
+        f[i, 3:5] = wp1.sum() - b[0], wt1.sum() - b[1]
+        f[i, 5:7] = wp1[far].sum() - bf[0], wt1[far].sum() - bf[1]
     f[:, 0] = D
     return f

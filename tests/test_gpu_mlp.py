@@ -189,7 +189,9 @@ def test_training_fits_a_smooth_function():
 @pytest.mark.parametrize("precision", (capi.PREC_FP64, capi.PREC_SPLIT))
 def test_features_vs_oracle(precision):
     """Exact tier: FP64 TSC with a cache per line, as the oracle's cached
-    tsc(); split tier: FP32 stencil blocks (FP32 coefficients: 1e-5)."""
+    tsc(); split tier: FP32 stencil blocks (FP32 coefficients: 1e-5).
+    Features 3, 4 (5, 6) are differences of two Born sums, each as accurate
+    as features 1, 2 (7, 8): their error is measured against that scale."""
     print(f"[case] catalogue features precision={precision}", flush=True)
     rec, f64 = cases.tree("amr")
     c = context(precision)
@@ -202,6 +204,8 @@ def test_features_vs_oracle(precision):
     fo = mlp.features(lambda pts: O.interp(h, 3, pts, precision=1), src, cases.LENGTH, synth.C_LIGHT ** 2)
     O.free(h)
     scale = np.max(np.abs(fo), 0) + 1e-30
+    scale[3:5] = scale[1:3]
+    scale[5:7] = scale[7:9]
     tol = 1e-6 if precision == capi.PREC_FP64 else 1e-5
     err = np.max(np.abs(feat - fo) / scale, 0)
     assert np.all(err <= tol), err

@@ -123,7 +123,10 @@ def test_predict_uses_the_normalisation():
 def test_features_closed_forms():
     """phi == 0 gives zero deflection features; a uniform gradient g gives the
     Born sum sum_k (1 - (k + 1/2)/N) / N = 1/2 of 2 g_perp L D / c^2 (N
-    samples), 1/8 of it over the far half and 7/32 over the first quarter."""
+    samples) and 1/8 of it over the far half; the second line, towards
+    theta + f2 (a gradient along z deflects along e_theta only), sees
+    g_perp = -sin(theta + f2) g, so f4 = 1/2 2 g L D / c^2 (sin(theta) -
+    sin(theta + f2)) and f6 = 1/8 of the same."""
     src = np.array([[0.1, 0.2, 0.3, 0, 0, 0], [0.3, -0.1, 0.2, 0, 0, 0]])
     L, c2 = 1.0e25, 8.98755178736818e16
     f0 = mlp.features(lambda pts: np.zeros((len(pts), 7)), src, L, c2)
@@ -142,6 +145,10 @@ def test_features_closed_forms():
     full = 2 * (-np.sin(theta) * g) * L * D / c2 * mlp.ARCSEC
     np.testing.assert_allclose(f[:, [1, 3, 5, 7]], 0, atol=1e-20)
     np.testing.assert_allclose(f[:, 2], 0.5 * full, rtol=1e-12)
-    np.testing.assert_allclose(f[:, 4], 0.125 * full, rtol=1e-12)
-    np.testing.assert_allclose(f[:, 6], 7 / 32 * full, rtol=1e-12)
-    np.testing.assert_allclose(f[:, 8], full, rtol=1e-12)
+    theta1 = theta + f[:, 2] / mlp.ARCSEC
+    second = 0.5 * 2 * g * L * D / c2 * mlp.ARCSEC * (np.sin(theta) - np.sin(theta1))
+    assert np.all(np.abs(second) > 0)
+    np.testing.assert_allclose(f[:, 4], second, rtol=1e-6)
+    np.testing.assert_allclose(f[:, 6], 0.25 * second, rtol=1e-6)
+    np.testing.assert_allclose(f[:, 8], 0.125 * full, rtol=1e-12)
+    assert np.all(f[:, 9] == 0)
