@@ -1552,11 +1552,21 @@ int mpf_catalogue_features_dev(mpf_ctx* c, const double* dev_src6, uint64_t n, f
     if (!c || (n && (!dev_src6 || !dev_feat))) return ERR_INVAL;
     if (!c->have_tree) return fail(c, ERR_STATE, "no octree loaded");
     HIPCHK(c, hipSetDevice(c->cfg.device));
+    // sources by direction: a wavefront's lines of sight then share cells
+    // (each source's features are its own, whatever the order)
+    const uint32_t* order = nullptr;
+    if (n > 1 && n < (1ull << 31)) {
+        const size_t wb = mpf::ray_order_work_bytes(n);
+        if (c->order.ensure(n) != hipSuccess || c->order_work.ensure(wb) != hipSuccess)
+            return fail(c, ERR_NOMEM, "source-order buffers");
+        HIPCHK(c, mpf::launch_source_order(dev_src6, n, c->order.p, c->order_work.p, wb, c->stream));
+        order = c->order.p;
+    }
     if (c->cfg.precision == MPF_PREC_SPLIT && c->cfg.order == MPF_TSC) {
         if (int rc = ensure_split(c)) return rc;
-        HIPCHK(c, mpf::launch_features_split(c->dt, c->split, dev_src6, n, c->cfg.length, dev_feat, c->stream));
+        HIPCHK(c, mpf::launch_features_split(c->dt, c->split, dev_src6, n, order, c->cfg.length, dev_feat, c->stream));
     } else {
-        HIPCHK(c, mpf::launch_features(c->dt, dev_src6, n, c->cfg.length, dev_feat, c->stream));
+        HIPCHK(c, mpf::launch_features(c->dt, dev_src6, n, order, c->cfg.length, dev_feat, c->stream));
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;

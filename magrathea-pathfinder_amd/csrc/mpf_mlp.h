@@ -23,8 +23,9 @@ bool mlp_gradient(MlpModel* M, const float* feat, const float* target, uint64_t 
 bool mlp_infer_dev(MlpModel* M, const float* dfeat, uint64_t n, float* dout, std::string& err);
 float mlp_last_ms(const MlpModel* M);   // duration of the last inference kernel (ms), -1 if none
 
-hipError_t launch_features(const DevTree& t, const double* src, uint64_t n, double length, float* feat,
-                           hipStream_t st);
+// order (may be null): the source each thread takes (launch_source_order)
+hipError_t launch_features(const DevTree& t, const double* src, uint64_t n, const uint32_t* order, double length,
+                           float* feat, hipStream_t st);
 hipError_t launch_seed(const double* src, const float* feat, const float* pred, uint64_t n, double* init,
                        hipStream_t st);
 hipError_t launch_mfma_probe(const float* a, const float* b, float* c, hipStream_t st);

@@ -491,10 +491,11 @@ struct ExactSampler {
     }
 };
 
-__global__ void features_kernel(DevTree t, const double* __restrict__ src, uint64_t n, double length,
-                                float* __restrict__ feat) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__global__ void features_kernel(DevTree t, const double* __restrict__ src, uint64_t n,
+                                const uint32_t* __restrict__ order, double length, float* __restrict__ feat) {
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const uint64_t i = order ? (uint64_t)order[j] : j;
     ExactSampler sample(t);
     source_features(src + 6 * i, length, sample, feat + (uint64_t)MPF_MLP_NIN * i);
 }
@@ -820,9 +821,9 @@ bool mlp_infer_dev(MlpModel* M, const float* dfeat, uint64_t n, float* dout, std
 
 float mlp_last_ms(const MlpModel* M) { return M ? M->last_ms : -1.f; }
 
-hipError_t launch_features(const DevTree& t, const double* src, uint64_t n, double length, float* feat,
-                           hipStream_t st) {
-    if (n) hipLaunchKernelGGL(features_kernel, dim3(nblk(n)), dim3(256), 0, st, t, src, n, length, feat);
+hipError_t launch_features(const DevTree& t, const double* src, uint64_t n, const uint32_t* order, double length,
+                           float* feat, hipStream_t st) {
+    if (n) hipLaunchKernelGGL(features_kernel, dim3(nblk(n)), dim3(256), 0, st, t, src, n, order, length, feat);
     return hipGetLastError();
 }
 

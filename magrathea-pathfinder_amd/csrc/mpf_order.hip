@@ -74,6 +74,16 @@ __global__ void direction_keys_kernel(const double* __restrict__ in9, const uint
     key[j] = ((j / bucket) << 32) | (uint64_t)direction_code(r[6], r[7], r[8]);
 }
 
+// key of the source sort: the direction of the position
+__global__ void source_keys_kernel(const double* __restrict__ src6, uint64_t n, uint64_t* __restrict__ key,
+                                   uint32_t* __restrict__ idx) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double* s = src6 + 6 * i;
+    key[i] = (uint64_t)direction_code(s[0], s[1], s[2]);
+    idx[i] = (uint32_t)i;
+}
+
 unsigned grid_for(uint64_t n) { return (unsigned)((n + 255) / 256); }
 
 int bits_for(uint64_t v) {
@@ -122,6 +132,22 @@ hipError_t launch_ray_order(const double* in9, const double* ray_value, int smal
     if ((e = hipGetLastError()) != hipSuccess) return e;
     const int end_bit = 32 + bits_for((n - 1) / bucket);
     return hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, key_a, key_b, idx, order, (int)n, 0, end_bit, st);
+}
+
+hipError_t launch_source_order(const double* src6, uint64_t n, uint32_t* order, void* work, size_t work_bytes,
+                               hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    unsigned char* w = static_cast<unsigned char*>(work);
+    uint64_t* key_a = reinterpret_cast<uint64_t*>(w);
+    uint64_t* key_b = key_a + n;
+    uint32_t* idx = reinterpret_cast<uint32_t*>(key_b + n);
+    unsigned char* temp = reinterpret_cast<unsigned char*>(idx + n);
+    temp += (256 - (reinterpret_cast<uintptr_t>(temp) & 255)) & 255;
+    size_t temp_bytes = work_bytes - (size_t)(temp - w);
+    hipLaunchKernelGGL(source_keys_kernel, dim3(grid_for(n)), dim3(256), 0, st, src6, n, key_a, idx);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, key_a, key_b, idx, order, (int)n, 0, 32, st);
 }
 
 }  // namespace mpf

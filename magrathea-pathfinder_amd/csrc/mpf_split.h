@@ -33,8 +33,9 @@ bool split_supports_surfaces();
 // provides in IntegrateArgs::scratch
 int split_launch_doubles();
 // features of catalogue sources (learned first guess) from the stencil blocks
-hipError_t launch_features_split(const DevTree& t, const SplitTree& s, const double* src, uint64_t n, double length,
-                                 float* feat, hipStream_t st);
+// order (may be null): the source each thread takes (launch_source_order)
+hipError_t launch_features_split(const DevTree& t, const SplitTree& s, const double* src, uint64_t n,
+                                 const uint32_t* order, double length, float* feat, hipStream_t st);
 // grid = cus x blocks_per_cu workgroups of 256 lanes; waves_per_simd picks
 // the kernel variant: without surfaces 1 = coefficient block kept in
 // registers (1 wave per SIMD), 2..4 = coefficients re-read, at least that

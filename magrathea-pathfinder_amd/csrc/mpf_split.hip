@@ -386,7 +386,8 @@ __device__ __forceinline__ uint32_t lookup_near(const DevTree& t, const SplitVie
 // the answer; else the descent continues from it) or from the neighbour
 // entry, and ends in the cell the root descent finds.
 // SB: distinct blocks of the wave read with scalar loads (0: vector loads only)
-template <int NF, bool RC, int SB = 0>
+// F0: the first field evaluated (out[0 .. F0 - 1] are left as they are)
+template <int NF, bool RC, int SB = 0, int F0 = 0>
 __device__ __forceinline__ bool split_eval(const DevTree& t, const SplitView& sv, Interp<RC>& I, double x, double y,
                                           double z, float* out) {
     const uint32_t kx32 = axis_key32(x), ky32 = axis_key32(y), kz32 = axis_key32(z);
@@ -427,7 +428,7 @@ __device__ __forceinline__ bool split_eval(const DevTree& t, const SplitView& sv
         float m[9];
         monomials9(v, w, m);
 #pragma unroll
-        for (int f = 0; f < NF; ++f) out[f] = mono27(I.c + f * NCOEF, m, u);
+        for (int f = F0; f < NF; ++f) out[f] = mono27(I.c + f * NCOEF, m, u);
     } else {
         // The wave walks the distinct blocks of its lanes (SB at most) with
         // scalar loads of the coefficients: the vector memory path returns
@@ -442,7 +443,7 @@ __device__ __forceinline__ bool split_eval(const DevTree& t, const SplitView& sv
             float m[9];
             monomials9(v, w, m);
 #pragma unroll
-            for (int f = 0; f < NF; ++f) {
+            for (int f = F0; f < NF; ++f) {
                 float c[NCOEF];
 #pragma unroll
                 for (int i = 0; i < NCOEF; ++i) c[i] = cs[f * FSTRIDE + i];
@@ -457,11 +458,11 @@ __device__ __forceinline__ bool split_eval(const DevTree& t, const SplitView& sv
             const float4* cf = reinterpret_cast<const float4*>(sv.blocks + (uint64_t)I.slot * BLOCK_BYTES + BLOCK_DOUBLES * 8);
             float4 q[2][FSTRIDE / 4];
 #pragma unroll
-            for (int i = 0; i < FSTRIDE / 4; ++i) q[0][i] = cf[i];
+            for (int i = 0; i < FSTRIDE / 4; ++i) q[F0 & 1][i] = cf[F0 * (FSTRIDE / 4) + i];
             float m[9];
             monomials9(v, w, m);
 #pragma unroll
-            for (int f = 0; f < NF; ++f) {
+            for (int f = F0; f < NF; ++f) {
                 if (f + 1 < NF) {
 #pragma unroll
                     for (int i = 0; i < FSTRIDE / 4; ++i) q[(f + 1) & 1][i] = cf[(f + 1) * (FSTRIDE / 4) + i];
@@ -1007,8 +1008,8 @@ struct SplitSampler {
         I.slot = NIL;
     }
     __device__ void operator()(double x, double y, double z, double* g) {
-        float o[NFIELD];
-        if (!split_eval<NFIELD, false>(t, sv, I, x, y, z, o)) {
+        float o[NFIELD];   // o[0], dphi/dt, is not evaluated
+        if (!split_eval<NFIELD, false, 0, 1>(t, sv, I, x, y, z, o)) {
             g[0] = g[1] = g[2] = g[3] = 0.0;
             return;
         }
@@ -1016,10 +1017,12 @@ struct SplitSampler {
     }
 };
 
+// order (may be null): the source each thread takes (by direction)
 __global__ void __launch_bounds__(256) features_split_kernel(DevTree t, SplitView sv, const double* __restrict__ src, uint64_t n,
-                                      double length, float* __restrict__ feat) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+                                      const uint32_t* __restrict__ order, double length, float* __restrict__ feat) {
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const uint64_t i = order ? (uint64_t)order[j] : j;
     SplitSampler sample(t, sv);
     source_features(src + 6 * i, length, sample, feat + (uint64_t)MPF_MLP_NIN * i);
 }
@@ -1228,12 +1231,13 @@ hipError_t launch_integrate_split(const IntegrateArgs& A, const DevTree& t, cons
 }  // namespace mpf
 
 namespace mpf {
-hipError_t launch_features_split(const DevTree& t, const SplitTree& s, const double* src, uint64_t n, double length,
+hipError_t launch_features_split(const DevTree& t, const SplitTree& s, const double* src, uint64_t n,
+                                 const uint32_t* order, double length,
                                  float* feat, hipStream_t st) {
     SplitView sv{s.tslot, s.blocks, s.nbr, s.rho};
     if (n)
         hipLaunchKernelGGL(features_split_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, t, sv, src, n,
-                           length, feat);
+                           order, length, feat);
     return hipGetLastError();
 }
 }  // namespace mpf
