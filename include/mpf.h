@@ -335,9 +335,12 @@ typedef struct {
     /* 0: the Jacobian is integrated with the rays of rounds >= 3 (where the
      * Newton update needs it, catalogues.h:509-517) and, for sources that
      * converged earlier, by one more ray at the converged direction;
-     * 1: with every round's rays, no extra ray.  Same results; 1 integrates
-     * fewer rays when most sources converge within three rounds (a good
-     * first guess), more when they need many rounds. */
+     * 1: with every round's rays, no extra ray; 2: with the rays of rounds
+     * >= 1, and one more ray for the sources that converge at their first
+     * ray.  Same results; 1 integrates fewer rays when most sources converge
+     * within three rounds, 2 spares the first round's rays the Jacobian when
+     * few sources converge at their first ray (a good but not exact first
+     * guess). */
     uint32_t jacobian_all_rounds;
     /* 1: jacobiantype = infinitesimal_born (catalogues.h:670-674): the
      * Jacobian columns of converged sources come from the homogeneous path

@@ -1097,6 +1097,7 @@ int mpf_catalogue_solve_dev(mpf_ctx* c, const double* dev_src6, const double* de
     }
     if (cfg->use_previous != 0 && cfg->use_previous != 2)
         return fail(c, ERR_INVAL, "use_previous must be 0 or 2 (mode 1 is mpf_catalogue_previous)");
+    if (cfg->jacobian_all_rounds > 2) return fail(c, ERR_INVAL, "jacobian_all_rounds must be 0, 1 or 2");
     const bool rerun = cfg->use_previous == 2;
     HIPCHK(c, hipSetDevice(c->cfg.device));
     if (total_rays) *total_rays = 0;
@@ -1191,13 +1192,14 @@ int mpf_catalogue_solve_dev(mpf_ctx* c, const double* dev_src6, const double* de
         rays += 4 * m;
         return 0;
     };
+    // the first round whose rays carry the Jacobian
+    const int jac_from = cfg->jacobian_all_rounds == 1 ? 0 : cfg->jacobian_all_rounds == 2 ? 1 : 3;
     unsigned int zero2[2] = {0u, 0u};
     CATCHK(hipMemcpyAsync(cnt.p + 1, zero2, 4, hipMemcpyHostToDevice, c->stream));
     for (int round = 0; nact > 0 && round <= (int)cfg->max_iterations; ++round) {
         // the bundle replaces the ray's own Jacobian from iteration 5 (catalogues.h:509-523); the
         // re-run mode updates without one
-        const int jac =
-            ((round >= 3 || cfg->jacobian_all_rounds) && !(bundle && round >= 5) && !rerun) ? 1 : 0;
+        const int jac = (round >= jac_from && !(bundle && round >= 5) && !rerun) ? 1 : 0;
         CATCHK(mpf::launch_cat_launch(ang.p, act.p, nact, in9.p, dact.p, dist.p, c->stream));
         mpf_surfaces sf{};
         sf.kind = MPF_STOP_RADIUS;

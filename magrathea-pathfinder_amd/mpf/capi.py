@@ -118,7 +118,8 @@ def make_newton(accuracy=1e-8, max_iterations=10, vobs=(0.0, 0.0, 0.0), h=0.7, m
     rejected sources."""
     c = NewtonCfg()
     c.use_previous = int(use_previous)
-    c.jacobian_all_rounds = int(bool(jacobian_all_rounds))
+    # False/True or the schedule 0, 1, 2 of include/mpf.h
+    c.jacobian_all_rounds = int(jacobian_all_rounds)
     c.jacobian_born = int(bool(jacobian_born))
     if bundle is not None:
         kind, plane, opening = bundle

@@ -241,9 +241,14 @@ def test_jacobian_all_rounds_same_results():
     src = proposal.cone_sources(200, 20.0, 20.0, 8.0, 0.05, 0.45, seed=13)
     a, ra = c.catalogue_solve(src, accuracy=1e-12)
     b, rb = c.catalogue_solve(src, accuracy=1e-12, jacobian_all_rounds=True)
+    d, rd = c.catalogue_solve(src, accuracy=1e-12, jacobian_all_rounds=2)
     np.testing.assert_array_equal(a, b)
-    early = np.sum((a[:, capi.CAT["status"]] == capi.CAT_OK) & (a[:, capi.CAT["iter"]] <= 3))
+    np.testing.assert_array_equal(a, d)
+    ok = a[:, capi.CAT["status"]] == capi.CAT_OK
+    early = np.sum(ok & (a[:, capi.CAT["iter"]] <= 3))
     assert ra - rb == early
+    # schedule 2: one more ray for the sources that converged at their first ray
+    assert rd - rb == np.sum(ok & (a[:, capi.CAT["iter"]] <= 1))
     c.close()
 
 
