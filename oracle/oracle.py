@@ -493,6 +493,10 @@ class RefVelocity:
 
 
 JACTYPES = {"infinitesimal": 0, "infinitesimal_born": 1, "bundle": 2}
+# one element of the reference's octree container: the 128-bit key, then
+# Gravity<float> in reverse tuple order (gravity.h:45-87), 48 bytes
+OCTREE_RECORD = np.dtype([("lo", "<u8"), ("hi", "<u8"), ("dphidt", "<f4"), ("a", "<f4"), ("dphidx", "<f4"),
+                          ("dphidy", "<f4"), ("dphidz", "<f4"), ("phi", "<f4"), ("rho", "<f4"), ("pad", "<u4")])
 PLANES = {"sachs": 0, "normal": 1}
 
 
@@ -613,6 +617,39 @@ class RefCatalogue:
         tyz = np.zeros(2)
         self.lib.refc_narrow_cones(celldir.encode(), ctypes.c_uint32(ncones), _p(c), _p(r), _p(m), _p(tyz))
         return c, r, m, tyz
+
+    def prep_import(self, path, cone7, micro_radius, isfullsky=1, rotm1=None, coarseonly=0, ncoarse=0):
+        """Input::importascii of one ASCII cell file with PreparationASCII's
+        filter: the selected 48-byte records."""
+        c7 = np.ascontiguousarray(cone7, dtype=np.float64).reshape(7)
+        rot = None if rotm1 is None else np.ascontiguousarray(rotm1, dtype=np.float64).reshape(9)
+        self.lib.refc_prep_import.restype = _u64
+        n = self.lib.refc_prep_import(str(path).encode(), _p(c7), ctypes.c_double(micro_radius), int(isfullsky),
+                                      _p(rot) if rot is not None else None, int(coarseonly), ctypes.c_uint(ncoarse),
+                                      None, _u64(0))
+        out = np.zeros(n, dtype=OCTREE_RECORD)
+        self.lib.refc_prep_import(str(path).encode(), _p(c7), ctypes.c_double(micro_radius), int(isfullsky),
+                                  _p(rot) if rot is not None else None, int(coarseonly), ctypes.c_uint(ncoarse),
+                                  _p(out), _u64(n))
+        return out
+
+    def prep_correct(self, rec, filler, reserve, table4, correction=1, coarsecorrection=1, h=0.7, omegam=0.3,
+                     lboxmpch=1000.0, mpc=3.08568e22, capacity=None):
+        """Miscellaneous::correctOctree on the records (in their order), the
+        container's spare storage (reserve records) filled with copies of
+        filler: the corrected records."""
+        rec = np.ascontiguousarray(rec)
+        filler = np.ascontiguousarray(np.asarray(filler).reshape(1))
+        tab = np.ascontiguousarray(table4, dtype=np.float64)
+        cap = int(capacity or 16 * rec.size + 64)
+        out = np.zeros(cap, dtype=OCTREE_RECORD)
+        self.lib.refc_prep_correct.restype = _u64
+        m = self.lib.refc_prep_correct(_p(rec), _u64(rec.size), _p(filler), _u64(reserve), int(correction),
+                                       int(coarsecorrection), ctypes.c_double(h), ctypes.c_double(omegam),
+                                       ctypes.c_double(lboxmpch), ctypes.c_double(mpc), _p(tab),
+                                       _u64(tab.shape[1]), _p(out), _u64(cap))
+        assert m <= cap
+        return out[:m]
 
     def get_targets(self, src8, cones7, icone):
         s = np.ascontiguousarray(src8, dtype=np.float64).reshape(-1, 8)

@@ -361,4 +361,70 @@ uint64_t refc_get_targets(const double* src8, uint64_t n, const double* cones7, 
     return sel.size();
 }
 
+/* ---- octree preparation (create_octree's ASCII branch) ---- */
+
+/* The parameters Input::importascii, Input::sistemize and Input::correct read
+ * (Create_octree::ReadParamFile, create_octree.h:120-146). */
+struct PrepParams {
+    unsigned int isfullsky = 1, coarseonly = 0, ncoarse = 0, correction = 1, coarsecorrection = 1, acorrection = 0;
+    double mpc = 3.08568e22;
+};
+
+/* Input::importascii<position, extent> (input.h:997-1069) of one ASCII cell
+ * file into an empty octree, with the filter of
+ * Create_octree::PreparationASCII (create_octree.h:560: Input::collide with
+ * the central sphere of radius micro_radius at the origin and the cone
+ * cone7); rotm1: 3 x 3 row-major, used when isfullsky == 0.  The selected
+ * records (at most capacity) go to out48; returns their count. */
+uint64_t refc_prep_import(const char* path, const double* cone7, double micro_radius, int isfullsky,
+                          const double* rotm1, int coarseonly, unsigned ncoarse, void* out48, uint64_t capacity) {
+    Octree oc;
+    PrepParams P;
+    P.isfullsky = (unsigned)isfullsky;
+    P.coarseonly = (unsigned)coarseonly;
+    P.ncoarse = ncoarse;
+    std::array<std::array<double, 3>, 3> R{};
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) R[i][j] = rotm1 ? rotm1[3 * i + j] : (i == j ? 1.0 : 0.0);
+    const std::vector<Cone<point>> cones = make_cones(cone7, 1);
+    const Sphere micro(point{{0.0, 0.0, 0.0}}, micro_radius);
+    Input::importascii<position, extent>(P, R, oc, std::string(path), [&](const Element& e) {
+        return Input::collide(oc, std::get<0>(e), micro, cones[0]);
+    });
+    const uint64_t n = oc.size();
+    if (n) std::memcpy(out48, (const void*)oc.data(), std::min<uint64_t>(n, capacity) * sizeof(Element));
+    return n;
+}
+
+/* Miscellaneous::correctOctree (miscellaneous.h:467-479: Input::sistemize,
+ * dphidt = 0, Input::correct(parameters, octree, amin), dphi/da to dphi/dt
+ * through Utility::interpolate2) on n records in the given order, with the
+ * cosmology table cosmo_tab (4 x ncos: t, a, da/dt, d2a/dt2).  The container
+ * holds `reserve` records of storage, the spare ones copies of filler48
+ * (SimpleHyperOctree::update reads the element past the end of its
+ * container, simplehyperoctree.h:1285-1300: a copy of the last cell there
+ * leaves that cell as it is).  The corrected records (at most capacity) go
+ * to out48; returns their count. */
+uint64_t refc_prep_correct(const void* rec48, uint64_t n, const void* filler48, uint64_t reserve, int correction,
+                           int coarsecorrection, double h, double omegam, double lboxmpch, double mpc,
+                           const double* cosmo_tab, uint64_t ncos, void* out48, uint64_t capacity) {
+    Octree oc;
+    if (reserve < n + 1) reserve = n + 1;
+    oc.reserve(reserve);
+    oc.resize(reserve);
+    for (uint64_t i = n; i < reserve; ++i) std::memcpy((void*)(oc.data() + i), filler48, sizeof(Element));
+    if (n) std::memcpy((void*)oc.data(), rec48, n * sizeof(Element));
+    oc.resize(n);
+    PrepParams P;
+    P.correction = (unsigned)correction;
+    P.coarsecorrection = (unsigned)coarsecorrection;
+    P.mpc = mpc;
+    const Cosmo cosmo = make_cosmo(cosmo_tab, ncos);
+    double amin = 0.0;
+    Miscellaneous::correctOctree(oc, cosmo, P, h, omegam, lboxmpch, amin);
+    const uint64_t m = oc.size();
+    if (m) std::memcpy(out48, (const void*)oc.data(), std::min<uint64_t>(m, capacity) * sizeof(Element));
+    return m;
+}
+
 }  // extern "C"
