@@ -309,6 +309,15 @@ int mpf_set_observer_velocity(mpf_ctx* ctx, const double* vobs3);
 int mpf_jacobian_bundle(mpf_ctx* ctx, const double* phi_theta, uint64_t n, const double* observer3, int kind,
                         const double* ref, uint32_t nd, const double* ki, const double* dist, double opening,
                         double* out4);
+/* Tidal blocks of the split tier (built at the first split run with the
+ * infinitesimal Jacobian, when the memory allows): per qualifying cell, the
+ * six differences of grad phi one cell away on either side (the samples of
+ * lensing.h:421-448) as polynomials in the offset, so that a step reads one
+ * block instead of looking up six neighbour cells.  enable 1: allowed (the
+ * default), 0: freed and not built again (every sample looked up), -1: only
+ * query.  nblocks (may be null) receives the number built.  No counterpart in
+ * the reference. */
+int mpf_split_tidal_blocks(mpf_ctx* ctx, int enable, uint64_t* nblocks);
 /* The number of directions of the last mpf_jacobian_bundle call with
  * MPF_STOP_PLANE that were redone from whole trajectories.  No counterpart
  * in the reference (which always stores whole trajectories). */

@@ -144,6 +144,8 @@ struct mpf_ctx {
     mpf::SplitTree split;
     bool split_ready = false;
     bool split_rho_failed = false;   // the rho blocks did not fit: split surfaces carry NaN densities
+    bool split_tidal_failed = false; // the tidal blocks did not fit: the Jacobian samples every neighbour
+    bool split_tidal = true;         // tidal blocks allowed (mpf_split_tidal_blocks)
     // cell-major field copies of the lean exact integrator, built at its first TSC run
     DevBuf<float> q32;
     DevBuf<double> q64;
@@ -280,6 +282,7 @@ int mpf_load_octree_records(mpf_ctx* c, const void* rec48, uint64_t n) {
     c->have_tree = false;
     c->split_ready = false;
     c->split_rho_failed = false;
+    c->split_tidal_failed = false;
     c->quad_ready = false;
     c->f64.release();
     if (c->base.ensure(t.base.size()) != hipSuccess || c->child_first.ensure(n) != hipSuccess ||
@@ -333,6 +336,7 @@ int mpf_load_octree_fields64(mpf_ctx* c, const double* f7, uint64_t n) {
     c->dt.f64 = c->f64.p;
     c->split_ready = false;
     c->split_rho_failed = false;
+    c->split_tidal_failed = false;
     c->quad_ready = false;
     return 0;
 }
@@ -430,6 +434,7 @@ int mpf_velocity_finish(mpf_ctx* c) {
     c->vel_open = false;
     c->split_ready = false;
     c->split_rho_failed = false;
+    c->split_tidal_failed = false;
     c->quad_ready = false;  // rho changed
     return 0;
 }
@@ -456,6 +461,7 @@ int mpf_velocity_set(mpf_ctx* c, const float* rho, const float* vel3, uint64_t n
             HIPCHK(c, hipMemcpyAsync(c->f64.p, r64.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
         c->split_ready = false;
     c->split_rho_failed = false;
+    c->split_tidal_failed = false;
     c->quad_ready = false;
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -677,6 +683,17 @@ static int integrate_impl(mpf_ctx* c, const double* dev_in9, uint64_t n, const m
                 if (!mpf::split_build_rho(c->split, c->dt, c->lev.data(), c->kx.data(), c->ky.data(), c->kz.data(),
                                           c->stream, err)) {
                     c->split_rho_failed = true;
+                    (void)hipGetLastError();
+                }
+                HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+            }
+            // tidal blocks for the Jacobian, built at the first run with it
+            // (without the memory for them every sample is looked up)
+            if (A.nsurf && A.jac && c->split_tidal && !c->split.tidal && !c->split_tidal_failed) {
+                std::string err;
+                if (!mpf::split_build_tidal(c->split, c->dt, c->lev.data(), c->kx.data(), c->ky.data(), c->kz.data(),
+                                            c->stream, err)) {
+                    c->split_tidal_failed = true;
                     (void)hipGetLastError();
                 }
                 HIPCHK(c, hipEventRecord(c->ev0, c->stream));
@@ -2092,6 +2109,22 @@ static int jacobian_bundle_plane(mpf_ctx* c, const double* phi_theta, uint64_t n
     }
     c->cfg.precision = prec;
     return rc;
+}
+
+int mpf_split_tidal_blocks(mpf_ctx* c, int enable, uint64_t* nblocks) {
+    if (!c || (enable != 0 && enable != 1 && enable != -1)) return ERR_INVAL;
+    if (enable == 0 && c->split.tidal) {
+        HIPCHK(c, hipSetDevice(c->cfg.device));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        (void)hipFree(c->split.tidal);
+        (void)hipFree(c->split.tidx);
+        c->split.tidal = nullptr;
+        c->split.tidx = nullptr;
+        c->split.ntidal = 0;
+    }
+    if (enable >= 0) c->split_tidal = enable != 0;
+    if (nblocks) *nblocks = c->split.tidal ? c->split.ntidal : 0;
+    return 0;
 }
 
 int mpf_bundle_plane_info(const mpf_ctx* c, uint64_t* whole_trajectory_directions) {

@@ -17,6 +17,11 @@ struct SplitTree {
     uint2* nbr = nullptr;             // cell x 6 faces -> same-level neighbour (see neighbour_kernel)
     size_t nbr_bytes = 0;
     float* rho = nullptr;             // per slot: 27 FP32 coefficients of rho (+ pad), built for surfaces
+    // tidal blocks (built for the Jacobian): slot -> index into `tidal`, or
+    // NIL when the slot's cell does not qualify (split_build_tidal)
+    uint32_t* tidx = nullptr;
+    float* tidal = nullptr;           // per index: 6 x 28 FP32 coefficients
+    uint64_t ntidal = 0;
     std::vector<uint32_t> slot_cell;  // host: slot -> its stencil-centre cell
 };
 
@@ -27,6 +32,15 @@ bool split_build(SplitTree& s, const DevTree& t, const uint8_t* level, const uin
 // the density entries of surfaces; level/kx/ky/kz as for split_build
 bool split_build_rho(SplitTree& s, const DevTree& t, const uint8_t* level, const uint32_t* kx, const uint32_t* ky,
                      const uint32_t* kz, hipStream_t st, std::string& err);
+// the tidal blocks of a built tree, for the infinitesimal Jacobian: for a
+// slot whose cell is a leaf, its own stencil centre, with the six same-level
+// face neighbours leaves and their own stencil centres, the differences of
+// grad phi between the samples one cell away on either side (the six
+// components the tidal tensor uses) as tri-quadratic polynomials in the
+// offset from the cell's centre; level/kx/ky/kz as for split_build.  Built
+// only with TIDAL_RESERVE bytes left free after them.
+bool split_build_tidal(SplitTree& s, const DevTree& t, const uint8_t* level, const uint32_t* kx, const uint32_t* ky,
+                       const uint32_t* kz, hipStream_t st, std::string& err);
 void split_release(SplitTree& s);
 bool split_supports_surfaces();
 // doubles of launch record per ray that the split integrator's caller

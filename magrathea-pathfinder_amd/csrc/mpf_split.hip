@@ -78,12 +78,21 @@ constexpr int SURF_SCALAR_BLOCKS = MPF_SURF_SCALAR_BLOCKS;
 constexpr uint64_t NBR_RESERVE = 24ull << 30;
 // ... and by the rho blocks: the launch records and ray order of 16 M rays
 constexpr uint64_t RHO_RESERVE = 4ull << 30;
+// ... and by the tidal blocks
+constexpr uint64_t TIDAL_RESERVE = 4ull << 30;
+// tidal blocks: the differences (+ sample minus - sample, one cell away) of
+// (dphidx along x, dphidx along y, dphidx along z, dphidy along y, dphidy
+// along z, dphidz along z), FSTRIDE coefficients each
+constexpr int TIDAL_COMPONENTS = 6;
+constexpr int TIDAL_FLOATS = TIDAL_COMPONENTS * FSTRIDE;   // 168: 672 bytes
 
 struct SplitView {
     const uint32_t* tslot;        // cell -> block slot (| PARTIAL_BIT) or NIL
     const unsigned char* blocks;  // slot -> BLOCK_BYTES
     const uint2* nbr;             // cell x 6 faces -> neighbour entry
     const float* rho;             // slot -> FSTRIDE coefficients of rho, or null
+    const uint32_t* tidx;         // slot -> tidal block index or NIL, or null
+    const float* tidal;           // tidal block -> TIDAL_FLOATS coefficients
 };
 
 // Face-neighbour entries (neighbour_kernel): for cell g at level l and face
@@ -247,6 +256,72 @@ __global__ void __launch_bounds__(128) rho_block_kernel(DevTree t, const uint8_t
         v[ic] = t.f64 ? t.f64[(uint64_t)F_RHO * t.n + h] : (double)t.f32[(uint64_t)F_RHO * t.n + h];
     }
     tsc_poly(v, rho + s * (uint64_t)FSTRIDE);
+}
+
+// A slot qualifies for a tidal block when its cell is a leaf that is its
+// own stencil centre and its six same-level face neighbours are leaves and
+// their own stencil centres: then the Jacobian samples one cell away from a
+// point of the cell lie in those neighbours at the point's own offset, and
+// each sample is the neighbour's TSC polynomial at that offset.
+__device__ __forceinline__ bool own_centre(const DevTree& t, const uint32_t* __restrict__ tslot,
+                                           const uint32_t* __restrict__ slot_cell, uint32_t g) {
+    if (g == NIL || t.child_mask[g] != 0u) return false;
+    const uint32_t w = tslot[g];
+    return w != NIL && (w & PARTIAL_BIT) == 0u && slot_cell[w] == g;
+}
+
+__global__ void tidal_flag_kernel(DevTree t, const uint8_t* __restrict__ lev, const uint32_t* __restrict__ kx,
+                                  const uint32_t* __restrict__ ky, const uint32_t* __restrict__ kz,
+                                  const uint32_t* __restrict__ tslot, const uint32_t* __restrict__ slot_cell,
+                                  uint64_t nslots, uint32_t* __restrict__ flag) {
+    const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    const uint32_t g = slot_cell[s];
+    bool ok = own_centre(t, tslot, slot_cell, g);
+    const unsigned l = lev[g];
+    const long long lim = (long long)((1ull << l) - 1ull);
+    for (unsigned d = 0; d < 6 && ok; ++d) {
+        long long c[3] = {(long long)kx[g], (long long)ky[g], (long long)kz[g]};
+        c[d >> 1] += (d & 1u) ? 1 : -1;
+        if (c[d >> 1] < 0 || c[d >> 1] > lim) { ok = false; break; }
+        unsigned fl = 0;
+        const uint32_t h = find_cell(t, l, (uint32_t)c[0], (uint32_t)c[1], (uint32_t)c[2], &fl);
+        ok = h != NIL && fl == l && own_centre(t, tslot, slot_cell, h);
+    }
+    flag[s] = ok ? 1u : 0u;
+}
+
+__global__ void __launch_bounds__(128) tidal_block_kernel(DevTree t, const uint8_t* __restrict__ lev,
+                                                          const uint32_t* __restrict__ kx,
+                                                          const uint32_t* __restrict__ ky,
+                                                          const uint32_t* __restrict__ kz,
+                                                          const uint32_t* __restrict__ slot_cell,
+                                                          const uint32_t* __restrict__ tidx, uint64_t nslots,
+                                                          float* __restrict__ tidal) {
+    const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslots || tidx[s] == NIL) return;
+    const uint32_t g = slot_cell[s];
+    const unsigned l = lev[g];
+    // (field of grad phi: 1 dphidx, 2 dphidy, 3 dphidz; axis of the shift)
+    const int comp[TIDAL_COMPONENTS][2] = {{1, 0}, {1, 1}, {1, 2}, {2, 1}, {2, 2}, {3, 2}};
+    float* out = tidal + (uint64_t)tidx[s] * TIDAL_FLOATS;
+    for (int k = 0; k < TIDAL_COMPONENTS; ++k) {
+        const int f = comp[k][0], axis = comp[k][1];
+        double v[27];
+        for (int ic = 0; ic < 27; ++ic) {
+            const int o[3] = {ic / 9 - 1, (ic / 3) % 3 - 1, ic % 3 - 1};
+            uint32_t cp[3], cm[3];
+            cp[0] = kx[g] + o[0]; cp[1] = ky[g] + o[1]; cp[2] = kz[g] + o[2];
+            for (int a = 0; a < 3; ++a) cm[a] = cp[a];
+            cp[axis] += 1u;
+            cm[axis] -= 1u;
+            unsigned fl = 0;
+            const uint32_t hp = find_cell(t, l, cp[0], cp[1], cp[2], &fl);
+            const uint32_t hm = find_cell(t, l, cm[0], cm[1], cm[2], &fl);
+            v[ic] = field_value(t, hp, f) - field_value(t, hm, f);
+        }
+        tsc_poly(v, out + k * FSTRIDE);
+    }
 }
 
 // ------------------------------------------------------------- kernel
@@ -891,23 +966,45 @@ integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
                     const unsigned tl = threadIdx.x;
                     double D[6];
                     float gm[3], gp[3];
-                    // x pair: d/dx of dphidx
-                    if (!jac_eval<1, JAC_SCALAR_BLOCKS>(t, sv, jcell[0][tl], jcid[0][tl], jlw[0][tl], p[CX] - hj, p[CY], p[CZ], gm)) gm[0] = 0.f;
-                    if (!jac_eval<1, JAC_SCALAR_BLOCKS>(t, sv, jcell[1][tl], jcid[1][tl], jlw[1][tl], p[CX] + hj, p[CY], p[CZ], gp)) gp[0] = 0.f;
-                    D[0] = (double)gp[0] - (double)gm[0];
-                    // y pair: d/dy of dphidx, dphidy
-                    if (!jac_eval<2, JAC_SCALAR_BLOCKS>(t, sv, jcell[2][tl], jcid[2][tl], jlw[2][tl], p[CX], p[CY] - hj, p[CZ], gm)) gm[0] = gm[1] = 0.f;
-                    if (!jac_eval<2, JAC_SCALAR_BLOCKS>(t, sv, jcell[3][tl], jcid[3][tl], jlw[3][tl], p[CX], p[CY] + hj, p[CZ], gp)) gp[0] = gp[1] = 0.f;
-                    D[1] = (double)gp[0] - (double)gm[0];
-                    D[3] = (double)gp[1] - (double)gm[1];
-                    // z pair: d/dz of dphidx, dphidy, dphidz
-                    if (!jac_eval<3, JAC_SCALAR_BLOCKS>(t, sv, jcell[4][tl], jcid[4][tl], jlw[4][tl], p[CX], p[CY], p[CZ] - hj, gm))
-                        gm[0] = gm[1] = gm[2] = 0.f;
-                    if (!jac_eval<3, JAC_SCALAR_BLOCKS>(t, sv, jcell[5][tl], jcid[5][tl], jlw[5][tl], p[CX], p[CY], p[CZ] + hj, gp))
-                        gp[0] = gp[1] = gp[2] = 0.f;
-                    D[2] = (double)gp[0] - (double)gm[0];
-                    D[4] = (double)gp[1] - (double)gm[1];
-                    D[5] = (double)gp[2] - (double)gm[2];
+                    // the cell's tidal block, when it has one: the six
+                    // differences at the point's own offset, one block read
+                    const uint32_t ti = sv.tidx ? sv.tidx[I.slot] : NIL;
+                    if (ti != NIL) {
+                        const float u = (float)((p[CX] - I.cx) * I.inv);
+                        const float v = (float)((p[CY] - I.cy) * I.inv);
+                        const float w = (float)((p[CZ] - I.cz) * I.inv);
+                        float mv[9];
+                        monomials9(v, w, mv);
+                        const float4* tq = reinterpret_cast<const float4*>(sv.tidal + (uint64_t)ti * TIDAL_FLOATS);
+#pragma unroll
+                        for (int k = 0; k < TIDAL_COMPONENTS; ++k) {
+                            float cf[FSTRIDE];
+#pragma unroll
+                            for (int i = 0; i < FSTRIDE / 4; ++i) {
+                                const float4 q = tq[k * (FSTRIDE / 4) + i];
+                                cf[4 * i] = q.x; cf[4 * i + 1] = q.y; cf[4 * i + 2] = q.z; cf[4 * i + 3] = q.w;
+                            }
+                            D[k] = (double)mono27(cf, mv, u);
+                        }
+                    } else {
+                        // x pair: d/dx of dphidx
+                        if (!jac_eval<1, JAC_SCALAR_BLOCKS>(t, sv, jcell[0][tl], jcid[0][tl], jlw[0][tl], p[CX] - hj, p[CY], p[CZ], gm)) gm[0] = 0.f;
+                        if (!jac_eval<1, JAC_SCALAR_BLOCKS>(t, sv, jcell[1][tl], jcid[1][tl], jlw[1][tl], p[CX] + hj, p[CY], p[CZ], gp)) gp[0] = 0.f;
+                        D[0] = (double)gp[0] - (double)gm[0];
+                        // y pair: d/dy of dphidx, dphidy
+                        if (!jac_eval<2, JAC_SCALAR_BLOCKS>(t, sv, jcell[2][tl], jcid[2][tl], jlw[2][tl], p[CX], p[CY] - hj, p[CZ], gm)) gm[0] = gm[1] = 0.f;
+                        if (!jac_eval<2, JAC_SCALAR_BLOCKS>(t, sv, jcell[3][tl], jcid[3][tl], jlw[3][tl], p[CX], p[CY] + hj, p[CZ], gp)) gp[0] = gp[1] = 0.f;
+                        D[1] = (double)gp[0] - (double)gm[0];
+                        D[3] = (double)gp[1] - (double)gm[1];
+                        // z pair: d/dz of dphidx, dphidy, dphidz
+                        if (!jac_eval<3, JAC_SCALAR_BLOCKS>(t, sv, jcell[4][tl], jcid[4][tl], jlw[4][tl], p[CX], p[CY], p[CZ] - hj, gm))
+                            gm[0] = gm[1] = gm[2] = 0.f;
+                        if (!jac_eval<3, JAC_SCALAR_BLOCKS>(t, sv, jcell[5][tl], jcid[5][tl], jlw[5][tl], p[CX], p[CY], p[CZ] + hj, gp))
+                            gp[0] = gp[1] = gp[2] = 0.f;
+                        D[2] = (double)gp[0] - (double)gm[0];
+                        D[4] = (double)gp[1] - (double)gm[1];
+                        D[5] = (double)gp[2] - (double)gm[2];
+                    }
                     tidal_from_differences(p[CX], p[CY], p[CZ], chi, jdl, D, T);
                 }
                 const double vp = surf_param(A.surf_kind, R.chi, R.lambda, R.redshift, R.core[CT], R.core[CA], R.s);
@@ -1186,6 +1283,73 @@ bool split_build_rho(SplitTree& s, const DevTree& t, const uint8_t* level, const
     return true;
 }
 
+bool split_build_tidal(SplitTree& s, const DevTree& t, const uint8_t* level, const uint32_t* kx, const uint32_t* ky,
+                       const uint32_t* kz, hipStream_t st, std::string& err) {
+    if (s.tidal) return true;
+    const uint64_t n = t.n, ns = s.nslots;
+    if (ns == 0 || s.slot_cell.size() != ns) {
+        err = "split tier: tidal blocks need the built stencil blocks";
+        return false;
+    }
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || (uint64_t)free_b < 13ull * n + 8ull * ns + TIDAL_RESERVE) {
+        err = "split tier: no room for the tidal blocks";
+        return false;
+    }
+    uint8_t* d_lev = nullptr;
+    uint32_t *d_kx = nullptr, *d_ky = nullptr, *d_kz = nullptr, *d_sc = nullptr, *d_flag = nullptr, *tidx = nullptr;
+    float* tidal = nullptr;
+    auto cleanup = [&]() {
+        for (void* q : {(void*)d_lev, (void*)d_kx, (void*)d_ky, (void*)d_kz, (void*)d_sc, (void*)d_flag})
+            if (q) (void)hipFree(q);
+    };
+    bool ok = hipMalloc((void**)&d_lev, n) == hipSuccess && hipMalloc((void**)&d_kx, 4 * n) == hipSuccess &&
+              hipMalloc((void**)&d_ky, 4 * n) == hipSuccess && hipMalloc((void**)&d_kz, 4 * n) == hipSuccess &&
+              hipMalloc((void**)&d_sc, 4 * ns) == hipSuccess && hipMalloc((void**)&d_flag, 4 * ns) == hipSuccess;
+    if (!ok) err = "split tier: tidal blocks: allocation failed";
+    ok = ok && hipMemcpyAsync(d_lev, level, n, hipMemcpyHostToDevice, st) == hipSuccess &&
+         hipMemcpyAsync(d_kx, kx, 4 * n, hipMemcpyHostToDevice, st) == hipSuccess &&
+         hipMemcpyAsync(d_ky, ky, 4 * n, hipMemcpyHostToDevice, st) == hipSuccess &&
+         hipMemcpyAsync(d_kz, kz, 4 * n, hipMemcpyHostToDevice, st) == hipSuccess &&
+         hipMemcpyAsync(d_sc, s.slot_cell.data(), 4 * ns, hipMemcpyHostToDevice, st) == hipSuccess;
+    std::vector<uint32_t> flag(ns);
+    if (ok) {
+        hipLaunchKernelGGL(tidal_flag_kernel, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, st, t, d_lev, d_kx,
+                           d_ky, d_kz, s.tslot, d_sc, ns, d_flag);
+        ok = hipGetLastError() == hipSuccess &&
+             hipMemcpyAsync(flag.data(), d_flag, 4 * ns, hipMemcpyDeviceToHost, st) == hipSuccess &&
+             hipStreamSynchronize(st) == hipSuccess;
+        if (!ok) err = "split tier: tidal blocks: flag kernel failed";
+    }
+    // compact indices of the qualifying slots
+    uint64_t m = 0;
+    for (uint64_t q = 0; ok && q < ns; ++q) flag[q] = flag[q] ? (uint32_t)m++ : NIL;
+    if (ok && (hipMemGetInfo(&free_b, &total_b) != hipSuccess ||
+               (uint64_t)free_b < 4ull * TIDAL_FLOATS * m + 4ull * ns + TIDAL_RESERVE)) {
+        err = "split tier: no room for the tidal blocks";
+        ok = false;
+    }
+    ok = ok && hipMalloc((void**)&tidx, 4 * ns) == hipSuccess &&
+         hipMalloc((void**)&tidal, 4ull * TIDAL_FLOATS * std::max<uint64_t>(m, 1)) == hipSuccess;
+    if (ok) {
+        ok = hipMemcpyAsync(tidx, flag.data(), 4 * ns, hipMemcpyHostToDevice, st) == hipSuccess;
+        hipLaunchKernelGGL(tidal_block_kernel, dim3((unsigned)((ns + 127) / 128)), dim3(128), 0, st, t, d_lev, d_kx,
+                           d_ky, d_kz, d_sc, tidx, ns, tidal);
+        ok = ok && hipGetLastError() == hipSuccess && hipStreamSynchronize(st) == hipSuccess;
+        if (!ok) err = "split tier: tidal blocks: kernel failed";
+    }
+    cleanup();
+    if (!ok) {
+        if (tidx) (void)hipFree(tidx);
+        if (tidal) (void)hipFree(tidal);
+        return false;
+    }
+    s.tidx = tidx;
+    s.tidal = tidal;
+    s.ntidal = m;
+    return true;
+}
+
 bool split_supports_surfaces() { return true; }
 
 int split_launch_doubles() { return LAUNCH_DOUBLES; }
@@ -1194,13 +1358,15 @@ void split_release(SplitTree& s) {
     if (s.mem) (void)hipFree(s.mem);
     if (s.nbr) (void)hipFree(s.nbr);
     if (s.rho) (void)hipFree(s.rho);
+    if (s.tidx) (void)hipFree(s.tidx);
+    if (s.tidal) (void)hipFree(s.tidal);
     s = SplitTree();
 }
 
 hipError_t launch_integrate_split(const IntegrateArgs& A, const DevTree& t, const SplitTree& s, int cus,
                                   unsigned blocks_per_cu, unsigned waves_per_simd, hipStream_t st,
                                   hipEvent_t ev0, hipEvent_t ev1) {
-    SplitView sv{s.tslot, s.blocks, s.nbr, s.rho};
+    SplitView sv{s.tslot, s.blocks, s.nbr, s.rho, s.tidx, s.tidal};
     const uint64_t blocks = (A.n + 255) / 256;
     const int grid = (int)std::min<uint64_t>(blocks, (uint64_t)cus * blocks_per_cu);
     const unsigned pgrid = (unsigned)((A.n + 255) / 256);
@@ -1234,7 +1400,7 @@ namespace mpf {
 hipError_t launch_features_split(const DevTree& t, const SplitTree& s, const double* src, uint64_t n,
                                  const uint32_t* order, double length,
                                  float* feat, hipStream_t st) {
-    SplitView sv{s.tslot, s.blocks, s.nbr, s.rho};
+    SplitView sv{s.tslot, s.blocks, s.nbr, s.rho, s.tidx, s.tidal};
     if (n)
         hipLaunchKernelGGL(features_split_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, t, sv, src, n,
                            order, length, feat);

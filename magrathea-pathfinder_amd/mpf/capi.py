@@ -44,6 +44,7 @@ EXPORTS = ("mpf_create", "mpf_destroy", "mpf_last_error", "mpf_set_precision", "
            "mpf_cone_select", "mpf_cone_targets", "mpf_hmaps_cone_pixels", "mpf_launch_angles", "mpf_launch_towards",
            "mpf_launch_cone", "mpf_launch_bundle", "mpf_set_launch_config", "mpf_get_launch_config", "mpf_autotune",
            "mpf_last_kernel_ms", "mpf_propagate", "mpf_jacobian_bundle", "mpf_bundle_plane_info",
+           "mpf_split_tidal_blocks",
            "mpf_ray_statistics", "mpf_hmaps_homogeneous", "mpf_hmaps_components_dev", "mpf_healpix_write_fits",
            "mpf_integrate_surfaces_born", "mpf_integrate_surfaces_born_dev",
            "mpf_prep_last_error", "mpf_prep_create", "mpf_prep_destroy", "mpf_prep_clear", "mpf_prep_import_ascii",
@@ -233,6 +234,7 @@ def load_library(path=LIB_PATH):
     lib.mpf_ray_statistics.argtypes = [vp, vp, vp, u64, u64, vp, vp, u64, i32, i32, u32, dbl, dbl, vp, u64, vp, vp]
     lib.mpf_jacobian_bundle.argtypes = [vp, vp, u64, vp, i32, vp, u32, vp, vp, dbl, vp]
     lib.mpf_bundle_plane_info.argtypes = [vp, ctypes.POINTER(u64)]
+    lib.mpf_split_tidal_blocks.argtypes = [vp, i32, ctypes.POINTER(u64)]
     lib.mpf_propagate.argtypes = [vp, vp, u64, u32, dbl, vp, i32, dbl, u64, vp, vp, vp]
     lib.mpf_last_kernel_ms.argtypes = [vp, ctypes.POINTER(dbl)]
     lib.mpf_set_launch_config.argtypes = [vp, i32, ctypes.POINTER(LaunchCfg)]
@@ -1102,6 +1104,13 @@ class Context:
                                                  _p(ref), ctypes.c_uint32(nd), _p(ki), _p(dist),
                                                  ctypes.c_double(opening), _p(out)), "mpf_jacobian_bundle")
         return out
+
+    def split_tidal_blocks(self, enable=-1):
+        """Allow (1) or free and stop (0) the split tier's tidal blocks; -1
+        only queries.  Returns the number built."""
+        v = ctypes.c_uint64(0)
+        self._check(self.lib.mpf_split_tidal_blocks(self.h, int(enable), ctypes.byref(v)), "mpf_split_tidal_blocks")
+        return int(v.value)
 
     def bundle_plane_info(self):
         """Directions of the last plane bundle redone from whole trajectories."""

@@ -145,6 +145,7 @@ def test_jacobian_batch_split_vs_fp64_and_repeatable():
     c = context("amr256_tsc", capi.PREC_SPLIT)
     out1, surf1 = c.integrate_surfaces(in9, "redshift", [0.0], jacobian=True, ray_value=zs)
     out2, surf2 = c.integrate_surfaces(in9, "redshift", [0.0], jacobian=True, ray_value=zs)
+    print(f"  tidal blocks: {c.split_tidal_blocks()}", flush=True)
     # bit patterns (a NaN entry would compare unequal as a float)
     dout = np.flatnonzero(np.any(out1.view(np.uint64) != out2.view(np.uint64), axis=0))
     dsurf = np.flatnonzero(np.any(surf1[:, 0, :].view(np.uint64) != surf2[:, 0, :].view(np.uint64), axis=0))

@@ -67,15 +67,22 @@ def test_surfaces_exact_vs_oracle(name, precision):
     c.close()
 
 
+@pytest.mark.parametrize("tidal", (1, 0))
 @pytest.mark.parametrize("name", ("amr", "uniform"))
-def test_surfaces_split_vs_fp64(name):
-    print(f"[case] surfaces split-vs-fp64 tree={name}", flush=True)
+def test_surfaces_split_vs_fp64(name, tidal):
+    """Split against FP64, with the tidal blocks (the Jacobian's samples
+    from one block per qualifying cell) and with every sample looked up."""
+    print(f"[case] surfaces split-vs-fp64 tree={name} tidal={tidal}", flush=True)
     c64, _, _ = ctx(name, capi.PREC_FP64)
     cs, _, _ = ctx(name, capi.PREC_SPLIT)
+    cs.split_tidal_blocks(tidal)
     in9 = cases.rays(n_side=24, aperture=40.0)
     vals = SURFS["redshift"]
     o64, s64 = c64.integrate_surfaces(in9, "redshift", vals)
     os_, ss = cs.integrate_surfaces(in9, "redshift", vals)
+    nt = cs.split_tidal_blocks()
+    print(f"  tidal blocks: {nt}", flush=True)
+    assert (nt > 0) == bool(tidal)
     same = (s64[..., 0] == ss[..., 0]) & (s64[..., 1] == ss[..., 1])
     assert np.mean(~same) <= 0.02
     ok = same & (s64[..., 0] == 1.0)
