@@ -331,7 +331,7 @@ def production_cone(nside_map=4096, ncat=4194304, acc=1e-8, seed=17):
     rec_out["rays"] = {"rays": int(n), "rk4_steps": int(steps), "first_pass_s": runs[0][0], "wall_s": wall,
                        "kernel_ms": kms, "value": steps / wall, "kernel_steps_per_s": steps / (kms / 1e3),
                        "unit": "ray-RK4-steps/s", "rays_valid": int(valid.sum()), "bin_ms": t_bin * 1e3,
-                       "pixels_hit": int(np.count_nonzero(hits)),
+                       "pixels_hit": int(np.count_nonzero(hits)), "tidal_blocks": ctx.split_tidal_blocks(),
                        "kappa_rms": float(np.sqrt(np.mean((1 - (surf[valid, capi.SURF["a11"]] +
                                                                  surf[valid, capi.SURF["a22"]]) / 2) ** 2)))}
     del surf, hits
@@ -422,6 +422,7 @@ def jacobian_maps(ctx, w, length, table, n=4194304, nside=2048, zmin=0.1, zmax=2
             "pixels_hit": int(np.count_nonzero(maps[3])), "kappa_rms": float(np.sqrt(np.mean(
                 maps[0][maps[3] > 0] ** 2))),
             "launch_config": tuned, "autotune": autotune_summary(trials, t_tune),
+            "tidal_blocks": ctx.split_tidal_blocks(),
             "roofline": valu_roofline(PROFILES["jacobian"], steps, kms)}
 
 

@@ -37,8 +37,8 @@ bool split_build_rho(SplitTree& s, const DevTree& t, const uint8_t* level, const
 // face neighbours leaves and their own stencil centres, the differences of
 // grad phi between the samples one cell away on either side (the six
 // components the tidal tensor uses) as tri-quadratic polynomials in the
-// offset from the cell's centre; level/kx/ky/kz as for split_build.  Built
-// only with TIDAL_RESERVE bytes left free after them.
+// offset from the cell's centre; level/kx/ky/kz as for split_build.  As
+// many as leave TIDAL_RESERVE bytes free, the first qualifying slots first.
 bool split_build_tidal(SplitTree& s, const DevTree& t, const uint8_t* level, const uint32_t* kx, const uint32_t* ky,
                        const uint32_t* kz, hipStream_t st, std::string& err);
 void split_release(SplitTree& s);

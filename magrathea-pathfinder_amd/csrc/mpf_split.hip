@@ -1321,11 +1321,17 @@ bool split_build_tidal(SplitTree& s, const DevTree& t, const uint8_t* level, con
              hipStreamSynchronize(st) == hipSuccess;
         if (!ok) err = "split tier: tidal blocks: flag kernel failed";
     }
-    // compact indices of the qualifying slots
+    // compact indices of the qualifying slots; when the memory does not hold
+    // all their blocks, the first ones in slot order (coarse levels first:
+    // a coarse cell is crossed by more rays, and a crossing takes nsteps
+    // steps whatever the cell's size)
+    uint64_t cap = 0;
+    if (ok && hipMemGetInfo(&free_b, &total_b) == hipSuccess &&
+        (uint64_t)free_b > 4ull * ns + TIDAL_RESERVE)
+        cap = ((uint64_t)free_b - 4ull * ns - TIDAL_RESERVE) / (4ull * TIDAL_FLOATS);
     uint64_t m = 0;
-    for (uint64_t q = 0; ok && q < ns; ++q) flag[q] = flag[q] ? (uint32_t)m++ : NIL;
-    if (ok && (hipMemGetInfo(&free_b, &total_b) != hipSuccess ||
-               (uint64_t)free_b < 4ull * TIDAL_FLOATS * m + 4ull * ns + TIDAL_RESERVE)) {
+    for (uint64_t q = 0; ok && q < ns; ++q) flag[q] = (flag[q] && m < cap) ? (uint32_t)m++ : NIL;
+    if (ok && m == 0) {
         err = "split tier: no room for the tidal blocks";
         ok = false;
     }
