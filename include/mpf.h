@@ -209,6 +209,13 @@ int mpf_ray_statistics(const double* rows, const double* dist, const uint64_t* n
                        const double* ref_rows, const double* ref_dist, uint64_t m, int interp, int stat,
                        uint32_t nstat, double amin, double length, double* out4, uint64_t cap_out, uint64_t* nout,
                        uint64_t* nused);
+/* The statistics file of rays.cpp:441-452 (Output::save, output.h:228-241)
+ * from mpf_ray_statistics' out4 (nout rows): "# count", then one line per
+ * abscissa with the reference element's redshift, its statistic, the mean
+ * and the standard deviation at 17 significant digits; count is the number
+ * of rays used (nused).  Host only; -EINVAL when the file cannot be
+ * written. */
+int mpf_ray_statistics_write(const char* path, const double* out4, uint64_t nout, uint64_t count);
 /* Whole trajectories (the reference's Evolution<Photon>, integrator.h:605,
  * 776, after the final pop), exact tiers: traj receives up to `capacity`
  * elements per ray, element k of ray i at traj[(i*capacity + k)*MPF_NOUT]

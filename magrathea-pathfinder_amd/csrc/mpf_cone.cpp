@@ -24,8 +24,9 @@ const double kGolden = (double)1.61803398874989484820458683436563811772030917980
 // sin and cos of one variable through one sincos call: the reference build
 // (g++ -O3) merges such pairs into sincos, whose last bit can differ from
 // separate sin and cos calls (it keeps separate calls where the argument is
-// an expression written twice, as cos(atan2(..)) and sin(atan2(..)) in the
-// bundle launch)
+// a library call written twice, as cos(atan2(..)) and sin(atan2(..)),
+// cos(acos(..)) and sin(acos(..)) in the bundle launch: the two calls may set
+// errno, so they are not merged into one value)
 inline void sin_cos(double x, double& s, double& c) { ::sincos(x, &s, &c); }
 
 double axis_length2(const Cone& c) {
@@ -362,7 +363,8 @@ void launch_bundle(const double in9[9], uint32_t count, double angle, double rot
     const double r = std::sqrt((kx * kx) + (ky * ky) + (kz * kz));
     double sa, ca, sintheta, costheta, sinphi, cosphi, sinpsi, cospsi;
     sin_cos(angle, sa, ca);
-    sin_cos(std::acos(kz / r), sintheta, costheta);
+    costheta = std::cos(std::acos(kz / r));
+    sintheta = std::sin(std::acos(kz / r));
     cosphi = std::cos(std::atan2(ky, kx));
     sinphi = std::sin(std::atan2(ky, kx));
     sin_cos(rotation, sinpsi, cospsi);

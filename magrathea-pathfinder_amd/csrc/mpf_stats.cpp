@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <vector>
 
 #include "../../include/mpf.h"
@@ -109,4 +110,20 @@ extern "C" int mpf_ray_statistics(const double* rows, const double* dist, const 
             out4[4 * k + 3] = sd[k];
         }
     return 0;
+}
+
+// The statistics file of rays.cpp:441-452 (Output::save, output.h:228-241):
+// "# count", then per abscissa the reference element's redshift, its
+// statistic, the mean and the standard deviation, max_digits10 (17)
+// significant digits in the default float format.
+extern "C" int mpf_ray_statistics_write(const char* path, const double* out4, uint64_t nout, uint64_t count) {
+    if (!path || (nout && !out4)) return ERR_INVAL;
+    FILE* f = std::fopen(path, "w");
+    if (!f) return ERR_INVAL;
+    bool ok = std::fprintf(f, "# %llu\n", (unsigned long long)count) > 0;
+    for (uint64_t i = 0; ok && i < nout; ++i)
+        ok = std::fprintf(f, "%.17g %.17g %.17g %.17g\n", out4[4 * i], out4[4 * i + 1], out4[4 * i + 2],
+                          out4[4 * i + 3]) > 0;
+    ok = (std::fclose(f) == 0) && ok;
+    return ok ? 0 : ERR_INVAL;
 }

@@ -45,7 +45,7 @@ EXPORTS = ("mpf_create", "mpf_destroy", "mpf_last_error", "mpf_set_precision", "
            "mpf_launch_cone", "mpf_launch_bundle", "mpf_set_launch_config", "mpf_get_launch_config", "mpf_autotune",
            "mpf_last_kernel_ms", "mpf_propagate", "mpf_jacobian_bundle", "mpf_bundle_plane_info",
            "mpf_split_tidal_blocks",
-           "mpf_ray_statistics", "mpf_hmaps_homogeneous", "mpf_hmaps_components_dev", "mpf_healpix_write_fits",
+           "mpf_ray_statistics", "mpf_ray_statistics_write", "mpf_hmaps_homogeneous", "mpf_hmaps_components_dev", "mpf_healpix_write_fits",
            "mpf_integrate_surfaces_born", "mpf_integrate_surfaces_born_dev",
            "mpf_prep_last_error", "mpf_prep_create", "mpf_prep_destroy", "mpf_prep_clear", "mpf_prep_import_ascii",
            "mpf_prep_load_records", "mpf_prep_update", "mpf_prep_interpolate", "mpf_prep_correct", "mpf_prep_size",
@@ -232,6 +232,7 @@ def load_library(path=LIB_PATH):
     lib.mpf_integrate_surfaces_born_dev.argtypes = [vp, vp, u64, ctypes.POINTER(Surfaces), vp, vp, i32]
     lib.mpf_healpix_write_fits.argtypes = [cp, vp, i64, i32, ctypes.c_char, i32]
     lib.mpf_ray_statistics.argtypes = [vp, vp, vp, u64, u64, vp, vp, u64, i32, i32, u32, dbl, dbl, vp, u64, vp, vp]
+    lib.mpf_ray_statistics_write.argtypes = [cp, vp, u64, u64]
     lib.mpf_jacobian_bundle.argtypes = [vp, vp, u64, vp, i32, vp, u32, vp, vp, dbl, vp]
     lib.mpf_bundle_plane_info.argtypes = [vp, ctypes.POINTER(u64)]
     lib.mpf_split_tidal_blocks.argtypes = [vp, i32, ctypes.POINTER(u64)]
@@ -586,6 +587,14 @@ def ray_statistics(trajs, dists, ref_rows, ref_dist, interp="redshift", stat="di
     if lib.mpf_ray_statistics(*args, _p(out), nout.value, ctypes.byref(nout), ctypes.byref(nused)):
         raise MpfError("mpf_ray_statistics: invalid arguments")
     return out[:nout.value], nused.value
+
+
+def ray_statistics_write(path, out4, count):
+    """The statistics file of rays.cpp (Output::save) from ray_statistics' rows."""
+    out4 = np.ascontiguousarray(out4, dtype=np.float64).reshape(-1, 4)
+    if load_library().mpf_ray_statistics_write(str(path).encode(), _p(out4), ctypes.c_uint64(out4.shape[0]),
+                                               ctypes.c_uint64(int(count))):
+        raise MpfError(f"mpf_ray_statistics_write: cannot write {path}")
 
 
 def launch_angles(phi_theta, observer=(0.0, 0.0, 0.0)):

@@ -1535,7 +1535,9 @@ void mpfo_interpolate(const double* x, const double* y, uint64_t n, const double
  * on a circle of half-angle `angle` turned by `rotation`, after the photon
  * itself.  sin and cos of one variable come from one sincos call, as the
  * reference build computes them; the two trigonometric calls of atan2(...)
- * stay separate. */
+ * and those of acos(...) stay separate (the two library calls of each are
+ * not merged into one value, and glibc's sincos and cos can differ in the
+ * last bit, as at 1.8608994686324885). */
 void mpfo_launch_bundle(const double* in9, unsigned count, double angle, double rotation, double* out9) {
     const double pi = 4.0 * atan(1.0);
     const double step = (pi + pi) / (double)(count + (!count));
@@ -1543,7 +1545,8 @@ void mpfo_launch_bundle(const double* in9, unsigned count, double angle, double 
     const double r = sqrt((kx * kx) + (ky * ky) + (kz * kz));
     double sa, ca, sth, cth, sps, cps;
     sincos(angle, &sa, &ca);
-    sincos(acos(kz / r), &sth, &cth);
+    cth = cos(acos(kz / r));
+    sth = sin(acos(kz / r));
     const double cph = cos(atan2(ky, kx));
     const double sph = sin(atan2(ky, kx));
     sincos(rotation, &sps, &cps);

@@ -664,6 +664,33 @@ def ref_available():
     return os.path.exists(REF_LIB)
 
 
+REF_RAYS = os.path.join(HERE, "_ref", "rays")
+CONDA_LIB = os.path.dirname(HDF5_LIB)
+
+
+def rays_available():
+    """The reference's own rays program (oracle/_ref/rays) and the MPI and
+    HDF5 libraries it was linked against."""
+    return os.path.exists(REF_RAYS) and os.path.exists(os.path.join(CONDA_LIB, "libmpi.so"))
+
+
+def run_rays(params, workdir, timeout=600):
+    """Runs the reference's rays program as one MPI task on a parameter file
+    written from params ("key = value" lines) in workdir; returns its
+    standard output.  The system C++ runtime comes first on the library
+    path, the MPI and HDF5 libraries after it."""
+    import subprocess
+    path = os.path.join(workdir, "raytracer.txt")
+    with open(path, "w") as fh:
+        for k, v in params.items():
+            fh.write(f"{k} = {v}\n")
+    env = dict(os.environ, LD_LIBRARY_PATH="/usr/lib/x86_64-linux-gnu:" + CONDA_LIB)
+    r = subprocess.run([REF_RAYS, path], cwd=workdir, env=env, capture_output=True, text=True, timeout=timeout)
+    if r.returncode != 0:
+        raise RuntimeError(f"rays exited with {r.returncode}: {r.stdout[-2000:]} {r.stderr[-2000:]}")
+    return r.stdout
+
+
 def refc_available():
     return os.path.exists(REFC_LIB) and os.path.exists(HDF5_LIB)
 

@@ -123,6 +123,46 @@ def test_launch_bundle_closed_form():
     np.testing.assert_allclose(gaps, gaps[0], rtol=1e-10)
 
 
+def test_launch_bundle_matches_reference_on_random_directions():
+    """Integrator::launch (integrator.h:345-382) compiled from the reference,
+    against the library's and the oracle's bundle launch on random photons,
+    half-angles and rotations.  The reference build takes sin and cos of one
+    variable from one sincos call but keeps the two calls of cos(acos(..)) and
+    sin(acos(..)) (and of atan2) separate; glibc's sincos and cos differ in
+    the last bit at some arguments (as at 1.8608994686324885), which a few
+    thousand directions meet."""
+    from oracle import Oracle, RefHarness, ref_available
+    if not ref_available():
+        pytest.skip("reference harness not built")
+    R, O = RefHarness(), Oracle()
+    rng = np.random.default_rng(5)
+    for _ in range(3000):
+        k = rng.normal(size=3)
+        k /= np.linalg.norm(k)
+        in9 = np.array([1.0, 0.0, 0.0, 0.0, 0.0, 1.0, *k])
+        count, angle, rot = int(rng.integers(3, 12)), float(rng.uniform(1e-4, 0.2)), float(rng.uniform(0, 2 * np.pi))
+        want = R.launch_bundle(in9, count, angle, rot, True)
+        np.testing.assert_array_equal(capi.launch_bundle(in9, count, angle, rot, center=True), want)
+        np.testing.assert_array_equal(O.launch_bundle(in9, count, angle, rot), want)
+
+
+def test_bundle_theta_from_separate_sin_and_cos():
+    """The direction that separates sincos from cos in the last bit: the
+    bundle's z components come from cos(acos(kz)), not from sincos."""
+    kz = -0.28605105861782765
+    in9 = np.array([1.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.7896345801127843, -0.5428185900959241, kz])
+    r = math.sqrt(in9[6] ** 2 + in9[7] ** 2 + in9[8] ** 2)
+    th = math.acos(kz / r)
+    rot, angle = 3.9788187068425933, 0.01
+    b = capi.launch_bundle(in9, 6, angle, rot, center=True)
+    sps, cps = math.sin(rot), math.cos(rot)
+    rcos, rsin = r * math.cos(angle), r * math.sin(angle)
+    step = (math.pi + math.pi) / 6
+    for i in range(6):
+        x, y = rsin * math.cos(i * step), rsin * math.sin(i * step)
+        assert b[i + 1, 8] == math.sin(th) * sps * x + math.sin(th) * cps * y + math.cos(th) * rcos
+
+
 # ------------------------------------------------------------- narrow cones
 
 NARROW = (16, 20.0, 70.0, 5.0, 5.0)
