@@ -9,7 +9,10 @@ and the device's propagate to it in tests/test_gpu_propagate.py):
     mpf_cosmology_correct;
   * the launch of ntrajectories rays into cone 0 with the seeded draws and
     their bundle rotations (mpf_launch_cone), and each ray's bundle
-    (mpf_launch_bundle);
+    (mpf_launch_bundle), for full-sky cones (mpf_cones_fullsky) and for
+    narrow cones read from an ASCII info_narrow_cone.txt (typefile 2,
+    mpf_cones_narrow: the rays go into the cones turned into the octree's
+    frame, rays.cpp:191-197 and 233-236);
   * every ray's central trajectory with its angular diameter distance, and
     the reference ray's, as the program writes them with savemode = 0
     (integrator.h:998-1010): every field of every element, exactly;
@@ -25,6 +28,7 @@ sum allows (the reference adds the rays in the order its threads finish
 them): n ulp of the mean, and n ulp of the deviation plus 2 ulp of the mean
 (the deviation is the spread of values of the mean's size).  CPU only."""
 import math
+import os
 
 import numpy as np
 import pytest
@@ -36,6 +40,8 @@ from oracle import Oracle, rays_available, run_rays
 pytestmark = pytest.mark.skipif(not rays_available(), reason="the reference's rays program (oracle/_ref) not built")
 
 NCONES, SEED, MICRO, NTRAJ, NBUNDLE, OPENING, NSTEPS, NCOARSE, NSTAT = 8, 11, 16, 12, 6, 0.01, 10, 6, 40
+# the narrow field: phi, theta (degrees; phi is read as an integer) and the half-openings
+NARROW = (20, 70.0, 5.0, 5.0)
 STATS = ("distance", "distance2", "homogeneous", "inhomogeneous")
 INTERPS = (("redshift", capi.STOP_REDSHIFT), ("r", capi.STOP_RADIUS))
 EXTENT = 4
@@ -89,10 +95,16 @@ def read_stat(path):
     return int(lines[0][2:]), [ln.split(" ") for ln in lines[1:]]
 
 
-@pytest.fixture(scope="module")
-def run(tmp_path_factory):
-    d = tmp_path_factory.mktemp("rays")
+@pytest.fixture(scope="module", params=("fullsky", "narrow"))
+def run(request, tmp_path_factory):
+    sky = request.param
+    d = tmp_path_factory.mktemp("rays_" + sky)
     evol, par, cell = synth.ramses_evolution_files(str(d), h=0.7, omega_m=0.3, n=1200)
+    if sky == "narrow":
+        os.makedirs(cell, exist_ok=True)
+        phi, theta, ty, tz = NARROW
+        with open(os.path.join(cell, "info_narrow_cone.txt"), "w") as fh:
+            fh.write(f"phi = {phi}\ntheta = {theta}\nthetay = {ty}\nthetaz = {tz}\n")
     cones = d / "cones"
     out = d / "out"
     cones.mkdir()
@@ -100,14 +112,16 @@ def run(tmp_path_factory):
     rec, _ = cases.tree("wide")
     rec.tofile(cones / "cone_00000")
     base = dict(conefmt="cone_%05d", ncones=NCONES, conedir=str(cones) + "/", seed=SEED, microcoeff=MICRO,
-                ntrajectories=NTRAJ, isfullsky=1, paramfile=par, evolfile=evol, rhoch2=1.87847e-26, mpc=MPC,
+                ntrajectories=NTRAJ, isfullsky=int(sky == "fullsky"), paramfile=par, evolfile=evol,
+                rhoch2=1.87847e-26, mpc=MPC,
                 ncoarse=NCOARSE, ray_targets="random", massmin=0, massmax=0, nbundlemin=NBUNDLE, nbundlecnt=1,
                 openingmin=OPENING, openingcnt=1, outputdir=str(out) + "/", nsteps=NSTEPS, savemode=0,
-                makestat=1, nstat=NSTAT, celldir=cell + "/", typefile=0, base="none", halos=0)
+                makestat=1, nstat=NSTAT, celldir=cell + "/", typefile=0 if sky == "fullsky" else 2, base="none",
+                halos=0)
     for interp, _ in INTERPS:
         for stat in STATS:
             run_rays(dict(base, stop_ray=interp, statistic=stat, outputprefix=f"rays_{interp}"), str(d))
-    return d, rec, evol, par, cell
+    return d, rec, evol, par, cell, sky
 
 
 @pytest.fixture(scope="module")
@@ -115,7 +129,7 @@ def mine(run):
     """The same computation through the library's host calls and the oracle:
     per interpolation parameter the rays' (central rows, distances) and the
     reference ray's."""
-    d, rec, evol, par, cell = run
+    d, rec, evol, par, cell, sky = run
     # the cosmology of rays.cpp:201-213
     tab, h, om, lb = capi.cosmology_acquire(evol, par, cell + "/", mpc=MPC)
     length = lb * MPC / h
@@ -126,7 +140,10 @@ def mine(run):
     tab = np.array(tab)
     tab[1] = capi.cosmology_correct(tab[0], tab[1], traj[:, capi.OUT["t"]], traj[:, capi.OUT["a"]])
     # the rays of cone 0 and their bundles
-    cones = capi.cones_fullsky(NCONES, 2.0)
+    if sky == "fullsky":
+        cones = capi.cones_fullsky(NCONES, 2.0)
+    else:
+        cones = capi.cones_narrow(NCONES, *NARROW)[1]
     in9, rot = capi.launch_cone(cones, 0, NTRAJ, SEED, micro_radius=1.0 / MICRO)
     h = O.tree(rec)
     hh = O.tree(uniform_records(NCOARSE // 2, 0.0))
