@@ -743,7 +743,11 @@ int mpf_dev_copy(mpf_ctx* ctx, void* dev_dst, const void* dev_src, uint64_t byte
  * mpf_catalogue_solve*).  No counterpart in the reference, which splits rays
  * over threads in contiguous chunks (Utility::parallelize,
  * utility.h:127-143).  It changes no result, only the time.  kind 0: runs
- * without surfaces; kind 1: runs with surfaces (maps, catalogue rounds). */
+ * without surfaces; kind 1: runs with surfaces (maps, catalogue rounds).  A
+ * new context starts with refill 64 and sort bucket 65,536 for both kinds:
+ * the lanes of a wavefront then hold rays of neighbouring directions, which
+ * the kernels' block reuse needs (in input order a Jacobian batch of 4 M
+ * rays ran 12 times slower). */
 typedef struct {
     /* idle lanes a wavefront gathers before it takes rays from the queue
      * (1..64; 64: whole wavefronts of rays at a time) */
@@ -765,8 +769,9 @@ int mpf_get_launch_config(mpf_ctx* ctx, int kind, mpf_launch_cfg* cfg);
 /* Autotuner of the launch configuration: runs the given device-resident
  * batch (stop as for mpf_integrate_dev, or surfaces as for
  * mpf_integrate_surfaces_dev when surf is not NULL) under candidate
- * configurations, one knob at a time (sort bucket, refill, waves per SIMD,
- * blocks per CU), keeps a change that saves at least 1 % of the kernel time
+ * configurations, one knob at a time (refill, sort bucket, waves per SIMD,
+ * blocks per CU, then the sort bucket again), keeps a change that saves at
+ * least 1 % of the kernel time
  * (HIP events, ray ordering included), stores the result in the context for
  * that kind and returns it in best.  trials (may be NULL): up to max_trials
  * rows of 6 doubles (refill, sort_bucket, blocks_per_cu, waves_per_simd,

@@ -177,7 +177,7 @@ struct mpf_ctx {
     mpf::MlpModel* mlp = nullptr;
     // launch configuration of the persistent integrators: [0] without
     // surfaces, [1] with surfaces; the ray order they read
-    mpf_launch_cfg launch[2] = {{1, 0, 0, 0}, {1, 0, 0, 0}};
+    mpf_launch_cfg launch[2] = {{64, 65536, 0, 0}, {64, 65536, 0, 0}};
     DevBuf<uint32_t> order;
     DevBuf<double> homo;   // homogeneous values of the map surfaces
     DevBuf<unsigned char> order_work;
@@ -1855,8 +1855,12 @@ int mpf_autotune(mpf_ctx* c, const double* dev_in9, uint64_t n, const mpf_stop* 
     if (int rc = run(cur, ms)) return rc;
     if (int rc = run(cur, best_ms)) return rc;
     // coordinate descent, one knob at a time; a change is kept when it saves
-    // at least 1 % (the run-to-run spread of the kernel time)
-    const uint64_t buckets[] = {0, 4096, 65536, n};
+    // at least 1 % (the run-to-run spread of the kernel time).  The refill
+    // first: with refill 1 a wave takes rays as lanes free up, which undoes
+    // most of any order the sort buckets give, so buckets compared there
+    // are not the ones that win at refill 64; the buckets again last, once
+    // the occupancy is set.
+    const uint64_t buckets[] = {0, 4096, 16384, 65536, 262144, 1048576, n};
     const uint32_t refills[] = {1, 16, 32, 64};
     const uint32_t waves_plain[] = {1, 2, 3}, waves_surf[] = {1, 2};
     const uint32_t blocks_split[] = {4, 8, 16}, blocks_exact[] = {2, 4, 8};
@@ -1875,12 +1879,14 @@ int mpf_autotune(mpf_ctx* c, const double* dev_in9, uint64_t n, const mpf_stop* 
         cur = keep;
         return 0;
     };
-    int rc = try_knob([](mpf_launch_cfg& L, uint64_t v) { L.sort_bucket = v; }, buckets);
-    if (!rc) rc = try_knob([](mpf_launch_cfg& L, uint32_t v) { L.refill = v; }, refills);
+    auto bucket = [](mpf_launch_cfg& L, uint64_t v) { L.sort_bucket = v; };
+    int rc = try_knob([](mpf_launch_cfg& L, uint32_t v) { L.refill = v; }, refills);
+    if (!rc) rc = try_knob(bucket, buckets);
     if (!rc && split && !surf) rc = try_knob([](mpf_launch_cfg& L, uint32_t v) { L.waves_per_simd = v; }, waves_plain);
     if (!rc && split && surf) rc = try_knob([](mpf_launch_cfg& L, uint32_t v) { L.waves_per_simd = v; }, waves_surf);
     if (!rc && split) rc = try_knob([](mpf_launch_cfg& L, uint32_t v) { L.blocks_per_cu = v; }, blocks_split);
     if (!rc && !split) rc = try_knob([](mpf_launch_cfg& L, uint32_t v) { L.blocks_per_cu = v; }, blocks_exact);
+    if (!rc) rc = try_knob(bucket, buckets);
     if (rc) return rc;
     c->launch[kind] = cur;
     if (best) *best = cur;

@@ -73,6 +73,13 @@ constexpr int JAC_SCALAR_BLOCKS = MPF_JAC_SCALAR_BLOCKS;
 #define MPF_SURF_SCALAR_BLOCKS 0
 #endif
 constexpr int SURF_SCALAR_BLOCKS = MPF_SURF_SCALAR_BLOCKS;
+// ... and instead, in that kernel, a cache of blocks per wave in LDS
+// (WaveCache): entries per wave, 0 for none
+#ifndef MPF_WAVE_CACHE
+#define MPF_WAVE_CACHE 8
+#endif
+constexpr int WAVE_CACHE = MPF_WAVE_CACHE;
+constexpr unsigned WAVE_CACHE_MOD = WAVE_CACHE > 0 ? (unsigned)WAVE_CACHE : 1u;
 // device memory left free by the optional face-neighbour table: room for
 // 16 M rays with a surface (about 11 GB) and nside-4096 maps (12.9 GB)
 constexpr uint64_t NBR_RESERVE = 24ull << 30;
@@ -380,9 +387,90 @@ struct Interp {
     bool reuse;          // ... and its cell stores no children: every point inside has the same finest cell
     uint32_t cell;       // ... that cell
     uint32_t slot;
+    int ent;             // WaveCache entry last found for the slot (-1: none)
     double cx, cy, cz, inv;
     float c[RC ? NFIELD * NCOEF : 1];
 };
+
+// A cache in LDS of the coefficient blocks of one wave's lanes: WAVE_CACHE
+// entries of a block's coefficients (ENTRY_F4 16-byte chunks) and their
+// slots.  A lane keeps the entry of its block (Interp::ent); when that
+// entry's tag no longer names the lane's slot, the wave finds the slot among
+// the tags, or loads the block into an entry that no lane of the current
+// evaluation uses (the active lanes load it together, one 16-byte chunk
+// each).  A block then crosses the vector memory path once per wave instead
+// of once per lane; the lanes read their coefficients from LDS.  With more
+// distinct blocks in one evaluation than entries the wave evaluates in
+// rounds: the lanes left without an entry wait for the next round, in which
+// the entries of the lanes done are free.  Only the wave's own lanes touch
+// its entries, and a wave's LDS operations complete in program order.
+constexpr int ENTRY_F4 = NFIELD * FSTRIDE / 4;  // 35
+typedef float F4v __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) F4v LdsF4;
+typedef const __attribute__((address_space(1))) F4v GlobalF4;
+typedef __attribute__((address_space(3))) uint32_t LdsU32;
+struct WaveCache {
+    LdsF4* data;    // WAVE_CACHE x ENTRY_F4 chunks
+    LdsU32* tags;   // WAVE_CACHE slots (NIL: empty), then the next entry to replace
+};
+
+// The entry that holds slot's coefficients for the lanes that need one and
+// whose entry `ent` does not hold it (need && !hit), or -1 when every entry
+// is read by other lanes of this round; every active lane of the wave calls
+// it.  Out of line: it runs when some lane of the wave moved to another block.
+__device__ __noinline__ int wave_cache_miss(const unsigned char* blocks, LdsF4* data, LdsU32* tags, int ent,
+                                            uint32_t slot, bool hit, bool need) {
+    unsigned long long miss = __ballot(need && !hit);
+    // entries that lanes of this evaluation read
+    unsigned used = 0u;
+#pragma unroll
+    for (int j = 0; j < WAVE_CACHE; ++j)
+        if (__ballot(hit && ent == j)) used |= 1u << j;
+    const unsigned long long act = __ballot(true);
+    const unsigned n = (unsigned)__popcll(act);
+    const unsigned r = (unsigned)__popcll(act & ((1ull << __lane_id()) - 1ull));
+    while (miss) {
+        const uint32_t s0 = __builtin_amdgcn_readlane(slot, __builtin_ctzll(miss));
+        int e = -1;
+#pragma unroll
+        for (int j = 0; j < WAVE_CACHE; ++j)
+            if (tags[j] == s0) e = j;
+        e = __builtin_amdgcn_readfirstlane(e);
+        if (e < 0) {
+            const unsigned start = __builtin_amdgcn_readfirstlane(tags[WAVE_CACHE]);
+            for (int k = 0; k < WAVE_CACHE; ++k) {
+                const unsigned j = (start + (unsigned)k) % WAVE_CACHE_MOD;
+                if (!((used >> j) & 1u)) {
+                    e = (int)j;
+                    break;
+                }
+            }
+            if (e < 0) break;  // every entry is read in this evaluation
+            GlobalF4* src = (GlobalF4*)(blocks + (uint64_t)s0 * BLOCK_BYTES + BLOCK_DOUBLES * 8);
+            LdsF4* dst = data + e * ENTRY_F4;
+            for (unsigned c = r; c < (unsigned)ENTRY_F4; c += n) dst[c] = src[c];
+            if (r == 0) {
+                tags[e] = s0;
+                tags[WAVE_CACHE] = (uint32_t)(e + 1) % WAVE_CACHE_MOD;
+            }
+        }
+        used |= 1u << e;
+        const bool mine = need && !hit && slot == s0;
+        if (mine) ent = e;
+        miss &= ~__ballot(mine);
+    }
+    if ((miss >> __lane_id()) & 1ull) ent = -1;
+    return ent;
+}
+
+// For the lanes that need one, sets ent to the entry holding slot's
+// coefficients, or -1 (none free this round); every active lane of the wave
+// calls it.
+__device__ __forceinline__ void wave_cache_find(const SplitView& sv, const WaveCache& W, int& ent, uint32_t slot,
+                                                bool need) {
+    const bool hit = need && ent >= 0 && W.tags[ent] == slot;
+    if (__ballot(need && !hit)) ent = wave_cache_miss(sv.blocks, W.data, W.tags, ent, slot, hit, need);
+}
 
 template <bool RC>
 __device__ __forceinline__ void load_block(const SplitView& sv, uint32_t slot, Interp<RC>& I) {
@@ -462,9 +550,10 @@ __device__ __forceinline__ uint32_t lookup_near(const DevTree& t, const SplitVie
 // entry, and ends in the cell the root descent finds.
 // SB: distinct blocks of the wave read with scalar loads (0: vector loads only)
 // F0: the first field evaluated (out[0 .. F0 - 1] are left as they are)
-template <int NF, bool RC, int SB = 0, int F0 = 0>
+// WC: the vector loads go through the wave's block cache W (SB = 0)
+template <int NF, bool RC, int SB = 0, int F0 = 0, bool WC = false>
 __device__ __forceinline__ bool split_eval(const DevTree& t, const SplitView& sv, Interp<RC>& I, double x, double y,
-                                          double z, float* out) {
+                                          double z, float* out, const WaveCache* W = nullptr) {
     const uint32_t kx32 = axis_key32(x), ky32 = axis_key32(y), kz32 = axis_key32(z);
     uint32_t ax = 0u, ay = 0u, az = 0u;
     if (I.valid) {
@@ -532,9 +621,33 @@ __device__ __forceinline__ bool split_eval(const DevTree& t, const SplitView& sv
             // evaluated: two round trips to L2 per evaluation instead of NF
             const float4* cf = reinterpret_cast<const float4*>(sv.blocks + (uint64_t)I.slot * BLOCK_BYTES + BLOCK_DOUBLES * 8);
             float4 q[2][FSTRIDE / 4];
+            float m[9];
+            if (WC) {
+                // from the wave's block cache (LDS, short latency: one field
+                // at a time), in rounds while some lane has no entry
+                monomials9(v, w, m);
+                bool done = false;
+                do {
+                    wave_cache_find(sv, *W, I.ent, I.slot, !done);
+                    if (!done && I.ent >= 0) {
+                        const LdsF4* cl = W->data + I.ent * ENTRY_F4;
+#pragma unroll
+                        for (int f = F0; f < NF; ++f) {
+                            float c[FSTRIDE];
+#pragma unroll
+                            for (int i = 0; i < FSTRIDE / 4; ++i) {
+                                const F4v e = cl[f * (FSTRIDE / 4) + i];
+                                c[4 * i] = e.x; c[4 * i + 1] = e.y; c[4 * i + 2] = e.z; c[4 * i + 3] = e.w;
+                            }
+                            out[f] = mono27(c, m, u);
+                        }
+                        done = true;
+                    }
+                } while (__ballot(!done));
+                return true;
+            }
 #pragma unroll
             for (int i = 0; i < FSTRIDE / 4; ++i) q[F0 & 1][i] = cf[F0 * (FSTRIDE / 4) + i];
-            float m[9];
             monomials9(v, w, m);
 #pragma unroll
             for (int f = F0; f < NF; ++f) {
@@ -806,6 +919,18 @@ integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
     __shared__ uint4 jcell[SURF ? 6 : 1][SURF ? 256 : 1];
     __shared__ uint32_t jlw[SURF ? 6 : 1][SURF ? 256 : 1];
     __shared__ uint32_t jcid[SURF ? 6 : 1][SURF ? 256 : 1];
+    // the block cache of each of the 4 waves (WaveCache)
+    constexpr bool WCE = SURF && !RC && WAVE_CACHE > 0;
+    __shared__ F4v wc_data[WCE ? 4 * WAVE_CACHE * ENTRY_F4 : 1];
+    __shared__ uint32_t wc_tags[WCE ? 4 * (WAVE_CACHE + 1) : 1];
+    WaveCache WCa;
+    I.ent = -1;
+    if (WCE) {
+        const unsigned wv = threadIdx.x / 64u;
+        WCa.data = (LdsF4*)(wc_data + wv * WAVE_CACHE * ENTRY_F4);
+        WCa.tags = (LdsU32*)(wc_tags + wv * (WAVE_CACHE + 1));
+        if (__lane_id() <= (unsigned)WAVE_CACHE) WCa.tags[__lane_id()] = __lane_id() < (unsigned)WAVE_CACHE ? NIL : 0u;
+    }
     if (SURF) {
         if (threadIdx.x == 0) {
 #pragma unroll
@@ -879,19 +1004,20 @@ integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
 #pragma unroll
         for (int i = 0; i < 9; ++i) { ksum[i] = kout[i]; kin[i] = R.core[i] + (0.5 * dl) * kout[i]; }
         float F[5];
-        bool ok = split_eval<4, RC, SURF ? SURF_SCALAR_BLOCKS : SCALAR_BLOCKS>(t, sv, I, kin[CX], kin[CY], kin[CZ], F);
+        constexpr int EB = SURF ? SURF_SCALAR_BLOCKS : SCALAR_BLOCKS;
+        bool ok = split_eval<4, RC, EB, 0, WCE>(t, sv, I, kin[CX], kin[CY], kin[CZ], F, &WCa);
         if (!ok) { F[0] = F[1] = F[2] = F[3] = 0.f; }
         dadt = cosmo_slope(kin[CT], A.ct, A.cdadt, A.cslope, A.ncos, R.cs);
         rhs_split(kin, F, dadt, inv_scale, kout);
 #pragma unroll
         for (int i = 0; i < 9; ++i) { ksum[i] += 2.0 * kout[i]; kin[i] = R.core[i] + (0.5 * dl) * kout[i]; }
-        ok = split_eval<4, RC, SURF ? SURF_SCALAR_BLOCKS : SCALAR_BLOCKS>(t, sv, I, kin[CX], kin[CY], kin[CZ], F);
+        ok = split_eval<4, RC, EB, 0, WCE>(t, sv, I, kin[CX], kin[CY], kin[CZ], F, &WCa);
         if (!ok) { F[0] = F[1] = F[2] = F[3] = 0.f; }
         dadt = cosmo_slope(kin[CT], A.ct, A.cdadt, A.cslope, A.ncos, R.cs);
         rhs_split(kin, F, dadt, inv_scale, kout);
 #pragma unroll
         for (int i = 0; i < 9; ++i) { ksum[i] += 2.0 * kout[i]; kin[i] = R.core[i] + dl * kout[i]; }
-        ok = split_eval<4, RC, SURF ? SURF_SCALAR_BLOCKS : SCALAR_BLOCKS>(t, sv, I, kin[CX], kin[CY], kin[CZ], F);
+        ok = split_eval<4, RC, EB, 0, WCE>(t, sv, I, kin[CX], kin[CY], kin[CZ], F, &WCa);
         if (!ok) { F[0] = F[1] = F[2] = F[3] = 0.f; }
         dadt = cosmo_slope(kin[CT], A.ct, A.cdadt, A.cslope, A.ncos, R.cs);
         rhs_split(kin, F, dadt, inv_scale, kout);
@@ -901,7 +1027,7 @@ integrate_split_kernel(IntegrateArgs A, DevTree t, SplitView sv) {
         // fields at the new point
         const bool inside = !(p[CA] < 0.0) && p[CX] > -2.0 && p[CX] < 2.0 && p[CY] > -2.0 && p[CY] < 2.0 &&
                             p[CZ] > -2.0 && p[CZ] < 2.0;
-        bool cont = inside && split_eval<5, RC, SURF ? SURF_SCALAR_BLOCKS : SCALAR_BLOCKS>(t, sv, I, p[CX], p[CY], p[CZ], F);
+        bool cont = inside && split_eval<5, RC, EB, 0, WCE>(t, sv, I, p[CX], p[CY], p[CZ], F, &WCa);
         if (!cont) { F[0] = F[1] = F[2] = F[3] = F[4] = 0.f; }
         // extras (integrator.h:756-776) with FP64 accumulation
         const double nphi = (double)F[4];
