@@ -747,7 +747,7 @@ int mpf_dev_copy(mpf_ctx* ctx, void* dev_dst, const void* dev_src, uint64_t byte
  * new context starts with refill 64 and sort bucket 65,536 for both kinds:
  * the lanes of a wavefront then hold rays of neighbouring directions, which
  * the kernels' block reuse needs (in input order a Jacobian batch of 4 M
- * rays ran 12 times slower). */
+ * rays ran 15 times slower). */
 typedef struct {
     /* idle lanes a wavefront gathers before it takes rays from the queue
      * (1..64; 64: whole wavefronts of rays at a time) */
