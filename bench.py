@@ -340,14 +340,14 @@ def production_cone(nside_map=4096, ncat=4194304, acc=1e-8, seed=17):
     dmin = synth.comoving_distance_of_z(0.1) * 0.7 / synth.MPC_SI / box
     dmax = synth.comoving_distance_of_z(w["z_stop"]) * 0.7 / synth.MPC_SI / box
     src = proposal.field_sources(ncat, w["phi0"], w["theta0"], w["aperture"], dmin, dmax, seed)
-    # the first guess's network, trained on 4 other cones of this octree
-    fit, first_guess = train_first_guess(ctx, w, dmin, dmax, acc, 1048576, 262144, 12, seed)
-    d_src, d_init, d_cat = ctx.dev_alloc(src.nbytes), ctx.dev_alloc(ncat * 16), ctx.dev_alloc(ncat * capi.NCAT * 8)
-    ctx.upload(d_src, src)
     # the sources are 4x sparser in angle than the ray grid: larger buckets keep
     # a wavefront's rays close in direction (seeded solve: 1.63 s with buckets
     # of 65,536, 1.25 s with 262,144, 1.22 s with 1,048,576, 1.36 s unbucketed)
     ctx.set_launch_config(surfaces=True, refill=64, sort_bucket=1048576, blocks_per_cu=8, waves_per_simd=2)
+    # the first guess's network, trained on 4 other cones of this octree
+    fit, first_guess = train_first_guess(ctx, w, dmin, dmax, acc, 1048576, 262144, 12, seed)
+    d_src, d_init, d_cat = ctx.dev_alloc(src.nbytes), ctx.dev_alloc(ncat * 16), ctx.dev_alloc(ncat * capi.NCAT * 8)
+    ctx.upload(d_src, src)
     cruns = {}
     for name in ("born", "seeded"):
         ctx.synchronize()
