@@ -80,6 +80,7 @@ constexpr int SURF_SCALAR_BLOCKS = MPF_SURF_SCALAR_BLOCKS;
 #endif
 constexpr int WAVE_CACHE = MPF_WAVE_CACHE;
 constexpr unsigned WAVE_CACHE_MOD = WAVE_CACHE > 0 ? (unsigned)WAVE_CACHE : 1u;
+static_assert(WAVE_CACHE >= 0 && WAVE_CACHE <= 32, "the entries in use are a 32-bit mask");
 // device memory left free by the optional face-neighbour table: room for
 // 16 M rays with a surface (about 11 GB) and nside-4096 maps (12.9 GB)
 constexpr uint64_t NBR_RESERVE = 24ull << 30;
@@ -1243,6 +1244,10 @@ struct SplitSampler {
 // order (may be null): the source each thread takes (by direction)
 __global__ void __launch_bounds__(256) features_split_kernel(DevTree t, SplitView sv, const double* __restrict__ src, uint64_t n,
                                       const uint32_t* __restrict__ order, double length, float* __restrict__ feat) {
+    // (vector loads: the samples of a wave's lanes sit at the same fraction
+    // of different source distances, so they spread over more blocks than a
+    // per-wave block cache holds; with it the features took 41 ms instead of
+    // 16 ms for 1 M sources)
     const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     const uint64_t i = order ? (uint64_t)order[j] : j;
