@@ -469,8 +469,23 @@ __device__ __noinline__ int wave_cache_miss(const unsigned char* blocks, LdsF4* 
 // calls it.
 __device__ __forceinline__ void wave_cache_find(const SplitView& sv, const WaveCache& W, int& ent, uint32_t slot,
                                                 bool need) {
-    const bool hit = need && ent >= 0 && W.tags[ent] == slot;
-    if (__ballot(need && !hit)) ent = wave_cache_miss(sv.blocks, W.data, W.tags, ent, slot, hit, need);
+    bool hit = need && ent >= 0 && W.tags[ent] == slot;
+    if (__ballot(need && !hit)) {
+        // a lane that moved to another block looks for it among the tags
+        // first (some lane of 64 moves at most evaluations, and its new block
+        // is mostly cached already); only a block that is not calls the wave
+        // out of line to load it
+        if (need && !hit) {
+#pragma unroll
+            for (int j = 0; j < WAVE_CACHE; ++j) {
+                if (W.tags[j] == slot) {
+                    ent = j;
+                    hit = true;
+                }
+            }
+        }
+        if (__ballot(need && !hit)) ent = wave_cache_miss(sv.blocks, W.data, W.tags, ent, slot, hit, need);
+    }
 }
 
 template <bool RC>
